@@ -1,0 +1,383 @@
+"""bosss_amd -- MI355X backend for the BoSSS DG-operator / block-solver path.
+
+Python mirror of the reference's host interfaces over the C ABI of
+``libbosss_hip.so`` (declared in ``include/bosss_hip.h``).  Names follow the
+reference so parity tests read like its own tests:
+
+* ``Grid2D.Cartesian2DGrid`` / ``Grid3D.Cartesian3DGrid``
+  (BoSSS.Foundation.Grid/Grid2D.cs:396, Grid3D.cs:88)
+* ``SipOperator.Evaluate(alpha, beta, output)``  (IEvaluatorNonLin.Evaluate,
+  F/DifferentialOperator.cs:1601) and ``ComputeMatrix``
+  (F/DifferentialOperator.cs:1787)
+* ``BlockMsrMatrix.SpMV(alpha, x, beta, y)``  (P/BlockMsrMatrix.cs:2970)
+* ``BlockJacobi``, ``SoftGMRES``, ``FlexGMRES`` with ``Solve(X, B)``
+  (AS/BlockJacobi.cs, AS/SoftGMRES.cs, AS/FlexGMRES.cs)
+
+There is no CPU fallback: every call goes to the HIP library, and importing
+this package raises if the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbosss_hip.so")
+
+BH_DEVICE_PTR = 1
+BH_FP64, BH_FP32, BH_FP16_MFMA = 0, 1, 2
+
+_c_d = ctypes.c_double
+_c_i = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_c_p = ctypes.c_void_p
+_c_f = ctypes.c_float
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+# exported symbol -> (restype, argtypes); every declaration of include/bosss_hip.h
+SIGNATURES = {
+    "bh_init": (_c_i, [_c_i, _PP]),
+    "bh_finalize": (_c_i, [_c_p]),
+    "bh_last_error": (_c_i, [ctypes.c_char_p, _c_i]),
+    "bh_sync": (_c_i, [_c_p]),
+    "bh_malloc": (_c_i, [_c_p, ctypes.c_size_t, _PP]),
+    "bh_free": (_c_i, [_c_p, _c_p]),
+    "bh_memcpy_h2d": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
+    "bh_memcpy_d2h": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
+    "bh_event_record": (_c_i, [_c_p, _c_i]),
+    "bh_event_elapsed": (_c_i, [_c_p, _c_i, _c_i, ctypes.POINTER(_c_f)]),
+    "bh_grid_cartesian": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_p, _c_i, _c_p, _c_i, _PP]),
+    "bh_grid_destroy": (_c_i, [_c_p]),
+    "bh_grid_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
+    "bh_grid_edges": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p]),
+    "bh_grid_cj": (_c_i, [_c_p, _c_p]),
+    "bh_sip_create": (_c_i, [_c_p, _c_i, _c_d, _c_p, _c_p, _PP]),
+    "bh_op_destroy": (_c_i, [_c_p]),
+    "bh_op_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i64)]),
+    "bh_sip_num_bnd_nodes": (_c_i, [_c_p, ctypes.POINTER(_c_i)]),
+    "bh_sip_bnd_nodes": (_c_i, [_c_p, _c_p]),
+    "bh_sip_set_bnd_values": (_c_i, [_c_p, _c_p]),
+    "bh_sip_set_rhs": (_c_i, [_c_p, _c_p, _c_i]),
+    "bh_op_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i, _c_i]),
+    "bh_op_assemble": (_c_i, [_c_p, _PP, _c_p, _c_i]),
+    "bh_bsr_create": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p, _PP]),
+    "bh_bsr_destroy": (_c_i, [_c_p]),
+    "bh_bsr_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i64)]),
+    "bh_bsr_export": (_c_i, [_c_p, _c_p, _c_p, _c_p]),
+    "bh_bsr_spmv": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i]),
+    "bh_bjacobi_create": (_c_i, [_c_p, _c_d, _c_i, _c_i, _PP]),
+    "bh_gmres_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
+    "bh_fgmres_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
+    "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
+    "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
+                               ctypes.POINTER(_c_d)]),
+    "bh_solver_destroy": (_c_i, [_c_p]),
+}
+
+_LIB = None
+
+
+class BoSSSHipError(RuntimeError):
+    """A non-zero status of the native library (message from bh_last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libbosss_hip.so (raises if it is missing: there is no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run __graft_entry__.build() or make -C bosss_amd/csrc")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(2048)
+        lib().bh_last_error(buf, 2048)
+        msg = buf.value.decode(errors="replace")
+        if rc == -5:
+            raise ArithmeticError(msg)
+        raise BoSSSHipError(rc, msg)
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class DeviceArray:
+    """A device (HBM) buffer owned by a Context: the device-pointer path of the C ABI."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = ctypes.c_void_p()
+        _check(lib().bh_malloc(ctx.h, self.n * self.dtype.itemsize, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype or a.dtype)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        _check(lib().bh_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib().bh_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(lib().bh_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _vec_args(x):
+    """(pointer, flags) of a host numpy array or a DeviceArray."""
+    if isinstance(x, DeviceArray):
+        return x.ptr, BH_DEVICE_PTR
+    return x.ctypes.data, 0
+
+
+class Context:
+    """One GPU, one stream (bh_init)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().bh_init(device, ctypes.byref(h)))
+        self.h = h.value
+
+    def sync(self):
+        _check(lib().bh_sync(self.h))
+
+    def record(self, slot):
+        _check(lib().bh_event_record(self.h, slot))
+
+    def elapsed_ms(self, s0, s1):
+        ms = ctypes.c_float()
+        _check(lib().bh_event_elapsed(self.h, s0, s1, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            _check(lib().bh_finalize(self.h))
+            self.h = None
+
+
+class Grid:
+    """Cartesian grid (reference numbering; see bh_grid_cartesian)."""
+
+    def __init__(self, ctx, *nodes):
+        self.ctx = ctx
+        self.D = len(nodes)
+        ns = [np.ascontiguousarray(a, np.float64) for a in nodes] + [None] * (3 - len(nodes))
+        h = ctypes.c_void_p()
+        _check(lib().bh_grid_cartesian(ctx.h, self.D, _ptr(ns[0]), len(ns[0]), _ptr(ns[1]), len(ns[1]),
+                                       _ptr(ns[2]), len(ns[2]) if ns[2] is not None else 0, ctypes.byref(h)))
+        self.h = h.value
+        D, J, E = _c_i(), _c_i(), _c_i()
+        _check(lib().bh_grid_info(self.h, ctypes.byref(D), ctypes.byref(J), ctypes.byref(E)))
+        self.J, self.E = J.value, E.value
+
+    def edges(self):
+        a = [np.zeros(self.E, np.int32) for _ in range(4)]
+        _check(lib().bh_grid_edges(self.h, *[x.ctypes.data for x in a]))
+        return a
+
+    def cj(self):
+        c = np.zeros(self.J)
+        _check(lib().bh_grid_cj(self.h, c.ctypes.data))
+        return c
+
+    def __del__(self):
+        try:
+            lib().bh_grid_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Grid2D:
+    @staticmethod
+    def Cartesian2DGrid(ctx, xNodes, yNodes):
+        return Grid(ctx, xNodes, yNodes)
+
+
+class Grid3D:
+    @staticmethod
+    def Cartesian3DGrid(ctx, xNodes, yNodes, zNodes):
+        return Grid(ctx, xNodes, yNodes, zNodes)
+
+
+class BlockMsrMatrix:
+    """Device block-sparse matrix (bh_bsr)."""
+
+    def __init__(self, ctx, handle=None, nrows=None, bs=None, rowptr=None, col=None, vals=None):
+        self.ctx = ctx
+        if handle is None:
+            h = ctypes.c_void_p()
+            rp = np.ascontiguousarray(rowptr, np.int64)
+            cl = np.ascontiguousarray(col, np.int32)
+            vv = np.ascontiguousarray(vals, np.float64)
+            _check(lib().bh_bsr_create(ctx.h, nrows, bs, rp.ctypes.data, cl.ctypes.data, vv.ctypes.data,
+                                       ctypes.byref(h)))
+            handle = h.value
+        self.h = handle
+        n, bs_, nnzb = _c_i(), _c_i(), _c_i64()
+        _check(lib().bh_bsr_info(self.h, ctypes.byref(n), ctypes.byref(bs_), ctypes.byref(nnzb)))
+        self.nrows, self.bs, self.nnzb = n.value, bs_.value, nnzb.value
+        self.N = self.nrows * self.bs
+
+    def export(self):
+        rp = np.zeros(self.nrows + 1, np.int64)
+        col = np.zeros(self.nnzb, np.int32)
+        val = np.zeros(self.nnzb * self.bs * self.bs)
+        _check(lib().bh_bsr_export(self.h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        return rp, col, val.reshape(self.nnzb, self.bs, self.bs)
+
+    def SpMV(self, alpha, x, beta, y, prec=BH_FP64):
+        xp, fx = _vec_args(x)
+        yp, fy = _vec_args(y)
+        assert fx == fy, "x and y must both be host or both be device arrays"
+        _check(lib().bh_bsr_spmv(self.h, alpha, xp, beta, yp, prec, fx))
+        return y
+
+    def __del__(self):
+        try:
+            lib().bh_bsr_destroy(self.h)
+        except Exception:
+            pass
+
+
+class SipOperator:
+    """DifferentialOperator with one SIPLaplace component + RHS source."""
+
+    def __init__(self, grid, k, nu, penalty_base=1.3, bc=None):
+        self.grid = grid
+        self.ctx = grid.ctx
+        nu = np.ascontiguousarray(np.broadcast_to(nu, (grid.J,)), np.float64)
+        bc = np.ascontiguousarray(bc if bc is not None else np.zeros(2 * grid.D), np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().bh_sip_create(grid.h, k, penalty_base, nu.ctypes.data, bc.ctypes.data, ctypes.byref(h)))
+        self.h = h.value
+        Np, N = _c_i(), _c_i64()
+        _check(lib().bh_op_info(self.h, ctypes.byref(Np), ctypes.byref(N)))
+        self.Np, self.N = Np.value, N.value
+
+    def bnd_nodes(self):
+        n = _c_i()
+        _check(lib().bh_sip_num_bnd_nodes(self.h, ctypes.byref(n)))
+        X = np.zeros(n.value * self.grid.D)
+        _check(lib().bh_sip_bnd_nodes(self.h, X.ctypes.data))
+        return X.reshape(n.value, self.grid.D)
+
+    def set_bnd_values(self, g):
+        g = np.ascontiguousarray(g, np.float64)
+        _check(lib().bh_sip_set_bnd_values(self.h, g.ctypes.data))
+
+    def set_rhs(self, f):
+        if f is None:
+            _check(lib().bh_sip_set_rhs(self.h, None, 0))
+        else:
+            p, fl = _vec_args(f if isinstance(f, DeviceArray) else np.ascontiguousarray(f, np.float64))
+            _check(lib().bh_sip_set_rhs(self.h, p, fl))
+
+    def Evaluate(self, alpha, beta, output, u, affine=True, prec=BH_FP64):
+        """output <- beta*output + alpha*Op(u)  (IEvaluatorNonLin.Evaluate)."""
+        up, fu = _vec_args(u)
+        yp, fy = _vec_args(output)
+        assert fu == fy
+        _check(lib().bh_op_evaluate(self.h, alpha, up, beta, yp, 1 if affine else 0, prec, fu))
+        return output
+
+    def ComputeMatrix(self):
+        """Assemble on the device -> (BlockMsrMatrix, affine vector)."""
+        h = ctypes.c_void_p()
+        aff = np.zeros(self.N)
+        _check(lib().bh_op_assemble(self.h, ctypes.byref(h), aff.ctypes.data, 0))
+        return BlockMsrMatrix(self.ctx, handle=h.value), aff
+
+    def __del__(self):
+        try:
+            lib().bh_op_destroy(self.h)
+        except Exception:
+            pass
+
+
+class _Solver:
+    def __init__(self, handle, N, keep=()):
+        self.h = handle
+        self.N = N
+        self._keep = keep
+
+    def Solve(self, X, B):
+        xp, fx = _vec_args(X)
+        bp, fb = _vec_args(B)
+        assert fx == fb
+        _check(lib().bh_solver_solve(self.h, xp, bp, fx))
+        return X
+
+    def stats(self):
+        it, cv, r0, r = _c_i(), _c_i(), _c_d(), _c_d()
+        _check(lib().bh_solver_stats(self.h, ctypes.byref(it), ctypes.byref(cv), ctypes.byref(r0), ctypes.byref(r)))
+        return dict(iterations=it.value, converged=bool(cv.value), res0=r0.value, res=r.value)
+
+    @property
+    def ThisLevelIterations(self):
+        return self.stats()["iterations"]
+
+    @property
+    def Converged(self):
+        return self.stats()["converged"]
+
+    def __del__(self):
+        try:
+            lib().bh_solver_destroy(self.h)
+        except Exception:
+            pass
+
+
+class BlockJacobi(_Solver):
+    def __init__(self, M, omega=1.0, NoOfIterations=1, prec=BH_FP64):
+        h = ctypes.c_void_p()
+        _check(lib().bh_bjacobi_create(M.h, omega, NoOfIterations, prec, ctypes.byref(h)))
+        super().__init__(h.value, M.N, (M,))
+
+
+class SoftGMRES(_Solver):
+    def __init__(self, M=None, op=None, Precond=None, MaxKrylovDim=80, ConvergenceCriterion=1e-10,
+                 MaxSolverIterations=10000):
+        h = ctypes.c_void_p()
+        _check(lib().bh_gmres_create(M.h if M else None, op.h if op else None, Precond.h if Precond else None,
+                                     MaxKrylovDim, ConvergenceCriterion, MaxSolverIterations, ctypes.byref(h)))
+        super().__init__(h.value, (M or op).N, (M, op, Precond))
+
+
+class FlexGMRES(_Solver):
+    def __init__(self, M=None, op=None, Precond=None, MaxKrylovDim=30, ConvergenceCriterion=1e-8,
+                 MaxSolverIterations=10000):
+        h = ctypes.c_void_p()
+        _check(lib().bh_fgmres_create(M.h if M else None, op.h if op else None, Precond.h if Precond else None,
+                                      MaxKrylovDim, ConvergenceCriterion, MaxSolverIterations, ctypes.byref(h)))
+        super().__init__(h.value, (M or op).N, (M, op, Precond))

@@ -1,0 +1,598 @@
+// extern "C" entry points of libbosss_hip.so (declared in include/bosss_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "bh_internal.hpp"
+#include "bh_solvers.hpp"
+
+namespace bh {
+const std::string& last_error();
+}
+using namespace bh;
+
+#define BH_API_BEGIN try {
+#define BH_API_END                                   \
+    return 0;                                        \
+    }                                                \
+    catch (const bh::Error& e) {                     \
+        bh::set_last_error(e.what());                \
+        return e.code;                               \
+    }                                                \
+    catch (const std::exception& e) {                \
+        bh::set_last_error(e.what());                \
+        return -99;                                  \
+    }
+
+// ------------------------------------------------------------------ helpers
+namespace {
+
+// Stage a host or device vector on the device for the duration of a call.
+struct Staged {
+    bh_ctx* ctx;
+    DevBuf buf;
+    void* dptr;
+    Staged(bh_ctx* c, const void* p, size_t bytes, bool device, bool upload) : ctx(c) {
+        if (device || p == nullptr) { dptr = (void*)p; return; }
+        buf.alloc(bytes);
+        dptr = buf.p;
+        if (upload) ctx->upload(dptr, p, bytes);
+    }
+    void download(void* host, size_t bytes, bool device) {
+        if (!device && host) ctx->download(host, dptr, bytes);
+    }
+};
+
+void build_grid_edges(bh_grid* g) {
+    const int D = g->D;
+    for (int c = 0; c < g->J; ++c) {
+        int idx[3];
+        g->cell_ijk(c, idx);
+        for (int d = 0; d < D; ++d) {
+            if (idx[d] == 0) { g->e_in.push_back(c); g->e_out.push_back(-1); g->e_dir.push_back(d); g->e_side.push_back(2 * d); }
+            if (idx[d] == g->n[d] - 1) {
+                g->e_in.push_back(c); g->e_out.push_back(-1); g->e_dir.push_back(d); g->e_side.push_back(2 * d + 1);
+            } else {
+                int nb[3] = {idx[0], idx[1], idx[2]};
+                nb[d] += 1;
+                g->e_in.push_back(c); g->e_out.push_back(g->cell_index(nb[0], nb[1], nb[2]));
+                g->e_dir.push_back(d); g->e_side.push_back(-1);
+            }
+        }
+    }
+    // cj (F/GridData_CellData.cs:666-700): boundary faces count twice
+    g->cj.assign(g->J, 0.0);
+    for (int c = 0; c < g->J; ++c) {
+        int idx[3];
+        g->cell_ijk(c, idx);
+        double h[3], vol = 1, surf = 0, bnd = 0;
+        for (int d = 0; d < D; ++d) { h[d] = g->width[d][idx[d]]; vol *= h[d]; }
+        for (int d = 0; d < D; ++d) {
+            double ar = vol / h[d];
+            surf += 2 * ar;
+            if (idx[d] == 0) bnd += ar;
+            if (idx[d] == g->n[d] - 1) bnd += ar;
+        }
+        g->cj[c] = 0.5 * (surf + bnd) / vol;
+    }
+}
+
+}  // namespace
+
+void bh_op::rebuild_affine() {
+    const int D = this->D;
+    const int Np = this->Np;
+    std::vector<double> aff;
+    bool any_bnd = false;
+    for (double v : gbnd) if (v != 0.0) { any_bnd = true; break; }
+    affine.alloc(N * sizeof(double));
+    fill(ctx, affine.as<double>(), 0.0, N);
+    if (any_bnd) {
+        aff.assign(N, 0.0);
+        bh_grid* g = grid;
+        const int nqf = tab.nqf;
+        for (size_t e = 0; e < g->e_in.size(); ++e) {
+            if (bnd_slot[e] < 0) continue;
+            const int c = g->e_in[e], d = g->e_dir[e], side = g->e_side[e], s = side & 1;
+            int idx[3];
+            g->cell_ijk(c, idx);
+            double h[3], detJ = 1;
+            for (int dd = 0; dd < D; ++dd) { h[dd] = g->width[dd][idx[dd]]; detJ *= 0.5 * h[dd]; }
+            const double dA = detJ / (0.5 * h[d]);
+            const double sdinv = 1.0 / std::sqrt(detJ);
+            const double sgn = s ? 1.0 : -1.0;
+            const double nuP = h_nu[c];
+            const double pn = 2.0 * penalty_base * penalty_deg * g->cj[c];
+            for (int q = 0; q < nqf; ++q) {
+                const double gv = gbnd[bnd_slot[e] + q];
+                const double wq = tab.fw[q] * dA * sdinv;
+                for (int m = 0; m < Np; ++m) {
+                    const double phi = tab.T[((size_t)side * nqf + q) * Np + m];
+                    const double dn = sgn * (2.0 / h[d]) * tab.Nn[((size_t)side * nqf + q) * Np + m];
+                    double v;
+                    if (bc[side] == 0) v = nuP * gv * pn * phi - nuP * gv * dn;   // ipLaplace.cs:224-236 at u = 0
+                    else v = nuP * gv * phi;                                     // ipLaplace.cs:238-243
+                    aff[(size_t)c * Np + m] += wq * v;
+                }
+            }
+        }
+        ctx->upload(affine.p, aff.data(), N * sizeof(double));
+    }
+    if (rhs.p) axpby(ctx, -1.0, rhs.as<double>(), 1.0, affine.as<double>(), N);  // -rhs*v (SipPoissonMain.cs:431-434)
+    affine_valid = true;
+}
+
+extern "C" {
+
+int bh_last_error(char* buf, int len) {
+    const std::string& s = bh::last_error();
+    if (buf && len > 0) {
+        std::strncpy(buf, s.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)s.size();
+}
+
+int bh_init(int device, bh_ctx** out) {
+    BH_API_BEGIN
+    int n = 0;
+    BH_HIP(hipGetDeviceCount(&n));
+    BH_CHECK(device >= 0 && device < n, "bh_init: no such HIP device");
+    BH_HIP(hipSetDevice(device));
+    bh_ctx* c = new bh_ctx();
+    c->device = device;
+    BH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) BH_HIP(hipEventCreate(&e));
+    *out = c;
+    BH_API_END
+}
+
+int bh_finalize(bh_ctx* c) {
+    BH_API_BEGIN
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->stream);
+    c->red.release();
+    c->red_out.release();
+    if (c->host_red) (void)hipHostFree(c->host_red);
+    for (auto& e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    BH_API_END
+}
+
+int bh_sync(bh_ctx* c) {
+    BH_API_BEGIN
+    c->sync();
+    BH_API_END
+}
+
+int bh_malloc(bh_ctx* c, size_t bytes, void** p) {
+    BH_API_BEGIN
+    (void)c;
+    BH_HIP(hipMalloc(p, bytes));
+    BH_API_END
+}
+int bh_free(bh_ctx* c, void* p) {
+    BH_API_BEGIN
+    BH_HIP(hipStreamSynchronize(c->stream));
+    BH_HIP(hipFree(p));
+    BH_API_END
+}
+int bh_memcpy_h2d(bh_ctx* c, void* dst, const void* src, size_t bytes) {
+    BH_API_BEGIN
+    c->upload(dst, src, bytes);
+    BH_API_END
+}
+int bh_memcpy_d2h(bh_ctx* c, void* dst, const void* src, size_t bytes) {
+    BH_API_BEGIN
+    c->download(dst, src, bytes);
+    BH_API_END
+}
+int bh_event_record(bh_ctx* c, int slot) {
+    BH_API_BEGIN
+    BH_CHECK(slot >= 0 && slot < 8, "event slot out of range");
+    BH_HIP(hipEventRecord(c->ev[slot], c->stream));
+    BH_API_END
+}
+int bh_event_elapsed(bh_ctx* c, int s0, int s1, float* ms) {
+    BH_API_BEGIN
+    BH_CHECK(s0 >= 0 && s0 < 8 && s1 >= 0 && s1 < 8, "event slot out of range");
+    BH_HIP(hipEventSynchronize(c->ev[s1]));
+    BH_HIP(hipEventElapsedTime(ms, c->ev[s0], c->ev[s1]));
+    BH_API_END
+}
+
+// ------------------------------------------------------------------ grid
+int bh_grid_cartesian(bh_ctx* ctx, int D, const double* x, int nx1, const double* y, int ny1, const double* z,
+                      int nz1, bh_grid** out) {
+    BH_API_BEGIN
+    BH_CHECK(D == 2 || D == 3, "bh_grid_cartesian: D must be 2 or 3");
+    const double* src[3] = {x, y, z};
+    int len[3] = {nx1, ny1, nz1};
+    auto g = std::make_unique<bh_grid>();
+    g->ctx = ctx;
+    g->D = D;
+    g->nmax = 1;
+    for (int d = 0; d < D; ++d) {
+        BH_CHECK(src[d] && len[d] >= 2, "bh_grid_cartesian: need at least 2 nodes per direction");
+        g->nodes[d].assign(src[d], src[d] + len[d]);
+        g->n[d] = len[d] - 1;
+        g->width[d].resize(g->n[d]);
+        for (int i = 0; i < g->n[d]; ++i) {
+            g->width[d][i] = src[d][i + 1] - src[d][i];
+            BH_CHECK(g->width[d][i] > 0, "bh_grid_cartesian: nodes must be strictly increasing");  // CheckMonotonicity
+        }
+        g->nmax = std::max(g->nmax, g->n[d]);
+    }
+    int64_t J = (int64_t)g->n[0] * g->n[1] * (D == 3 ? g->n[2] : 1);
+    BH_CHECK(J < (1LL << 31), "grid too large");
+    g->J = (int)J;
+    build_grid_edges(g.get());
+    std::vector<double> w(3 * (size_t)g->nmax, 0.0);
+    for (int d = 0; d < D; ++d) std::copy(g->width[d].begin(), g->width[d].end(), w.begin() + (size_t)d * g->nmax);
+    g->d_width.alloc(w.size() * sizeof(double));
+    ctx->upload(g->d_width.p, w.data(), w.size() * sizeof(double));
+    *out = g.release();
+    BH_API_END
+}
+
+int bh_grid_destroy(bh_grid* g) {
+    BH_API_BEGIN
+    delete g;
+    BH_API_END
+}
+
+int bh_grid_info(bh_grid* g, int* D, int* J, int* E) {
+    BH_API_BEGIN
+    if (D) *D = g->D;
+    if (J) *J = g->J;
+    if (E) *E = (int)g->e_in.size();
+    BH_API_END
+}
+
+int bh_grid_edges(bh_grid* g, int* in, int* out, int* dir, int* side) {
+    BH_API_BEGIN
+    std::copy(g->e_in.begin(), g->e_in.end(), in);
+    std::copy(g->e_out.begin(), g->e_out.end(), out);
+    std::copy(g->e_dir.begin(), g->e_dir.end(), dir);
+    std::copy(g->e_side.begin(), g->e_side.end(), side);
+    BH_API_END
+}
+
+int bh_grid_cj(bh_grid* g, double* cj) {
+    BH_API_BEGIN
+    std::copy(g->cj.begin(), g->cj.end(), cj);
+    BH_API_END
+}
+
+// ------------------------------------------------------------------ SIP
+int bh_sip_create(bh_grid* g, int k, double penalty_base, const double* nu, const int* bc, bh_op** out) {
+    BH_API_BEGIN
+    BH_CHECK(k >= 1 && k <= (g->D == 2 ? 4 : 3), "bh_sip_create: supported degrees are 1..4 (2D), 1..3 (3D)");
+    auto op = std::make_unique<bh_op>();
+    op->ctx = g->ctx;
+    op->grid = g;
+    op->D = g->D;
+    op->k = k;
+    op->tab.build(g->D, k, k + 1);   // Gauss rule of order 2k (precision 2k+1), F/RefElement.cs:855-884
+    op->Np = op->tab.Np;
+    op->N = (int64_t)g->J * op->Np;
+    op->penalty_base = penalty_base;
+    const double D = g->D, p = k;
+    op->penalty_deg = std::max((p + 1) * (p + D) / D, (p + 1.0) * (p + 1.0));  // ipLaplace.cs:100-110
+    for (int f = 0; f < 2 * g->D; ++f) op->bc[f] = bc ? bc[f] : 0;
+    op->h_nu.assign(nu, nu + g->J);
+    for (double v : op->h_nu) BH_CHECK(std::isfinite(v), "Inf/NaN in diffusion coefficient");
+    bh_ctx* ctx = g->ctx;
+    op->nu.alloc(g->J * sizeof(double));
+    op->nu32.alloc(g->J * sizeof(float));
+    ctx->upload(op->nu.p, nu, g->J * sizeof(double));
+    convert_f64_f32(ctx, op->nu.as<double>(), op->nu32.as<float>(), g->J);
+    // tables
+    std::vector<double> t;
+    op->off_G = t.size(); t.insert(t.end(), op->tab.G.begin(), op->tab.G.end());
+    op->off_T = t.size(); t.insert(t.end(), op->tab.T.begin(), op->tab.T.end());
+    op->off_N = t.size(); t.insert(t.end(), op->tab.Nn.begin(), op->tab.Nn.end());
+    op->off_vw = t.size(); t.insert(t.end(), op->tab.vw.begin(), op->tab.vw.end());
+    op->off_fw = t.size(); t.insert(t.end(), op->tab.fw.begin(), op->tab.fw.end());
+    op->tab64.alloc(t.size() * sizeof(double));
+    ctx->upload(op->tab64.p, t.data(), t.size() * sizeof(double));
+    std::vector<float> t32(t.begin(), t.end());
+    op->tab32.alloc(t32.size() * sizeof(float));
+    ctx->upload(op->tab32.p, t32.data(), t32.size() * sizeof(float));
+    // boundary slots
+    op->bnd_slot.assign(g->e_in.size(), -1);
+    int slot = 0;
+    for (size_t e = 0; e < g->e_in.size(); ++e)
+        if (g->e_out[e] < 0) { op->bnd_slot[e] = slot; slot += op->tab.nqf; }
+    op->gbnd.assign(slot, 0.0);
+    *out = op.release();
+    BH_API_END
+}
+
+int bh_op_destroy(bh_op* op) {
+    BH_API_BEGIN
+    if (op) (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+    BH_API_END
+}
+
+int bh_op_info(bh_op* op, int* Np, int64_t* N) {
+    BH_API_BEGIN
+    if (Np) *Np = op->Np;
+    if (N) *N = op->N;
+    BH_API_END
+}
+
+int bh_sip_num_bnd_nodes(bh_op* op, int* count) {
+    BH_API_BEGIN
+    *count = (int)op->gbnd.size();
+    BH_API_END
+}
+
+int bh_sip_bnd_nodes(bh_op* op, double* X) {
+    BH_API_BEGIN
+    bh_grid* g = op->grid;
+    const int D = op->D, nqf = op->tab.nqf;
+    for (size_t e = 0; e < g->e_in.size(); ++e) {
+        const int slot = op->bnd_slot[e];
+        if (slot < 0) continue;
+        const int c = g->e_in[e], d = g->e_dir[e], s = g->e_side[e] & 1;
+        int idx[3];
+        g->cell_ijk(c, idx);
+        for (int q = 0; q < nqf; ++q) {
+            int m = 0;
+            for (int dd = 0; dd < D; ++dd) {
+                double xi = (dd == d) ? (s ? 1.0 : -1.0) : op->tab.fxi[q * (D - 1) + m++];
+                X[(size_t)(slot + q) * D + dd] = g->nodes[dd][idx[dd]] + 0.5 * (xi + 1.0) * g->width[dd][idx[dd]];
+            }
+        }
+    }
+    BH_API_END
+}
+
+int bh_sip_set_bnd_values(bh_op* op, const double* gv) {
+    BH_API_BEGIN
+    std::copy(gv, gv + op->gbnd.size(), op->gbnd.begin());
+    bool any = false;
+    for (double v : op->gbnd) any |= (v != 0.0);
+    op->has_affine = any || op->rhs.p;
+    op->affine_valid = false;
+    BH_API_END
+}
+
+int bh_sip_set_rhs(bh_op* op, const double* f, int flags) {
+    BH_API_BEGIN
+    if (!f) {
+        op->rhs.release();
+    } else {
+        op->rhs.alloc(op->N * sizeof(double));
+        if (flags & BH_DEVICE_PTR) copy(op->ctx, op->rhs.as<double>(), f, op->N);
+        else op->ctx->upload(op->rhs.p, f, op->N * sizeof(double));
+    }
+    bool any = false;
+    for (double v : op->gbnd) any |= (v != 0.0);
+    op->has_affine = any || op->rhs.p;
+    op->affine_valid = false;
+    BH_API_END
+}
+
+int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double* y, int affine, int prec, int flags) {
+    BH_API_BEGIN
+    const bool dev = flags & BH_DEVICE_PTR;
+    bh_ctx* ctx = op->ctx;
+    const int64_t N = op->N;
+    if (prec == BH_FP64) {
+        Staged su(ctx, u, N * sizeof(double), dev, true);
+        Staged sy(ctx, y, N * sizeof(double), dev, beta != 0.0);
+        sip_apply(op, alpha, su.dptr, beta, sy.dptr, affine != 0, BH_FP64);
+        sy.download(y, N * sizeof(double), dev);
+    } else if (prec == BH_FP32) {
+        BH_CHECK(!affine || !op->has_affine, "FP32 evaluate: linear part only");
+        if (dev) {
+            sip_apply(op, alpha, u, beta, y, false, BH_FP32);   // float* device vectors
+        } else {
+            DevBuf d64(N * sizeof(double)), u32(N * sizeof(float)), y32(N * sizeof(float));
+            ctx->upload(d64.p, u, N * sizeof(double));
+            convert_f64_f32(ctx, d64.as<double>(), u32.as<float>(), N);
+            if (beta != 0.0) {
+                ctx->upload(d64.p, y, N * sizeof(double));
+                convert_f64_f32(ctx, d64.as<double>(), y32.as<float>(), N);
+            }
+            sip_apply(op, alpha, u32.p, beta, y32.p, false, BH_FP32);
+            convert_f32_f64(ctx, y32.as<float>(), d64.as<double>(), N);
+            ctx->download(y, d64.p, N * sizeof(double));
+        }
+    } else {
+        throw Error(-3, "bh_op_evaluate: unsupported precision");
+    }
+    if (!dev) ctx->sync();
+    BH_API_END
+}
+
+int bh_op_assemble(bh_op* op, bh_bsr** out, double* affine, int flags) {
+    BH_API_BEGIN
+    bh_grid* g = op->grid;
+    bh_ctx* ctx = op->ctx;
+    const int Np = op->Np, J = g->J;
+    auto M = std::make_unique<bh_bsr>();
+    M->ctx = ctx;
+    M->nrows = J;
+    M->bs = Np;
+    // sparsity: self + face neighbours, ascending columns (SortedDictionary rows of P/BlockMsrMatrix.cs:191-340)
+    M->h_rowptr.assign(J + 1, 0);
+    std::vector<int> cols;
+    cols.reserve((size_t)J * (2 * g->D + 1));
+    for (int c = 0; c < J; ++c) {
+        int idx[3];
+        g->cell_ijk(c, idx);
+        int tmp[7], nt = 0;
+        tmp[nt++] = c;
+        for (int d = 0; d < g->D; ++d) {
+            if (idx[d] > 0) { int nb[3] = {idx[0], idx[1], idx[2]}; nb[d]--; tmp[nt++] = g->cell_index(nb[0], nb[1], nb[2]); }
+            if (idx[d] < g->n[d] - 1) { int nb[3] = {idx[0], idx[1], idx[2]}; nb[d]++; tmp[nt++] = g->cell_index(nb[0], nb[1], nb[2]); }
+        }
+        std::sort(tmp, tmp + nt);
+        cols.insert(cols.end(), tmp, tmp + nt);
+        M->h_rowptr[c + 1] = M->h_rowptr[c] + nt;
+    }
+    M->h_col = std::move(cols);
+    M->nnzb = M->h_rowptr[J];
+    M->rowptr.alloc((J + 1) * sizeof(int64_t));
+    M->col.alloc(M->nnzb * sizeof(int));
+    ctx->upload(M->rowptr.p, M->h_rowptr.data(), (J + 1) * sizeof(int64_t));
+    ctx->upload(M->col.p, M->h_col.data(), M->nnzb * sizeof(int));
+    M->vals.alloc((size_t)M->nnzb * Np * Np * sizeof(double));
+    BH_HIP(hipMemsetAsync(M->vals.p, 0, M->vals.bytes, ctx->stream));
+    sip_assemble(op, M.get());
+    if (affine) {
+        if (op->has_affine && !op->affine_valid) op->rebuild_affine();
+        if (op->has_affine) {
+            if (flags & BH_DEVICE_PTR) copy(ctx, affine, op->affine.as<double>(), op->N);
+            else ctx->download(affine, op->affine.p, op->N * sizeof(double));
+        } else {
+            if (flags & BH_DEVICE_PTR) fill(ctx, affine, 0.0, op->N);
+            else std::fill(affine, affine + op->N, 0.0);
+        }
+    }
+    ctx->sync();
+    *out = M.release();
+    BH_API_END
+}
+
+// ------------------------------------------------------------------ BSR
+int bh_bsr_create(bh_ctx* ctx, int nrows, int bs, const int64_t* rowptr, const int* col, const double* vals,
+                  bh_bsr** out) {
+    BH_API_BEGIN
+    BH_CHECK(nrows >= 0 && bs >= 1 && bs <= 64, "bh_bsr_create: invalid sizes");
+    auto M = std::make_unique<bh_bsr>();
+    M->ctx = ctx;
+    M->nrows = nrows;
+    M->bs = bs;
+    M->h_rowptr.assign(rowptr, rowptr + nrows + 1);
+    M->nnzb = M->h_rowptr[nrows];
+    BH_CHECK(M->h_rowptr[0] == 0, "bh_bsr_create: rowptr[0] must be 0");
+    for (int r = 0; r < nrows; ++r) BH_CHECK(M->h_rowptr[r + 1] >= M->h_rowptr[r], "bh_bsr_create: rowptr not monotone");
+    M->h_col.assign(col, col + M->nnzb);
+    for (int64_t p = 0; p < M->nnzb; ++p) BH_CHECK(M->h_col[p] >= 0 && M->h_col[p] < nrows, "bh_bsr_create: column out of range");
+    M->rowptr.alloc((nrows + 1) * sizeof(int64_t));
+    M->col.alloc(std::max<int64_t>(M->nnzb, 1) * sizeof(int));
+    ctx->upload(M->rowptr.p, M->h_rowptr.data(), (nrows + 1) * sizeof(int64_t));
+    ctx->upload(M->col.p, M->h_col.data(), M->nnzb * sizeof(int));
+    // blocks: row-major in, column-major on the device
+    std::vector<double> v((size_t)M->nnzb * bs * bs);
+    for (int64_t p = 0; p < M->nnzb; ++p)
+        for (int i = 0; i < bs; ++i)
+            for (int j = 0; j < bs; ++j) v[(size_t)p * bs * bs + (size_t)j * bs + i] = vals[(size_t)p * bs * bs + (size_t)i * bs + j];
+    M->vals.alloc(std::max<size_t>(v.size(), 1) * sizeof(double));
+    ctx->upload(M->vals.p, v.data(), v.size() * sizeof(double));
+    *out = M.release();
+    BH_API_END
+}
+
+int bh_bsr_destroy(bh_bsr* M) {
+    BH_API_BEGIN
+    if (M) (void)hipStreamSynchronize(M->ctx->stream);
+    delete M;
+    BH_API_END
+}
+
+int bh_bsr_info(bh_bsr* M, int* nrows, int* bs, int64_t* nnzb) {
+    BH_API_BEGIN
+    if (nrows) *nrows = M->nrows;
+    if (bs) *bs = M->bs;
+    if (nnzb) *nnzb = M->nnzb;
+    BH_API_END
+}
+
+int bh_bsr_export(bh_bsr* M, int64_t* rowptr, int* col, double* vals) {
+    BH_API_BEGIN
+    const int bs = M->bs;
+    if (rowptr) std::copy(M->h_rowptr.begin(), M->h_rowptr.end(), rowptr);
+    if (col) std::copy(M->h_col.begin(), M->h_col.end(), col);
+    if (vals) {
+        std::vector<double> v((size_t)M->nnzb * bs * bs);
+        M->ctx->download(v.data(), M->vals.p, v.size() * sizeof(double));
+        for (int64_t p = 0; p < M->nnzb; ++p)
+            for (int i = 0; i < bs; ++i)
+                for (int j = 0; j < bs; ++j) vals[(size_t)p * bs * bs + (size_t)i * bs + j] = v[(size_t)p * bs * bs + (size_t)j * bs + i];
+    }
+    BH_API_END
+}
+
+int bh_bsr_spmv(bh_bsr* M, double alpha, const void* x, double beta, void* y, int prec, int flags) {
+    BH_API_BEGIN
+    const bool dev = flags & BH_DEVICE_PTR;
+    bh_ctx* ctx = M->ctx;
+    const int64_t N = (int64_t)M->nrows * M->bs;
+    if (prec == BH_FP64) {
+        Staged sx(ctx, x, N * sizeof(double), dev, true);
+        Staged sy(ctx, y, N * sizeof(double), dev, beta != 0.0);
+        bsr_spmv(M, alpha, sx.dptr, beta, sy.dptr, BH_FP64);
+        sy.download(y, N * sizeof(double), dev);
+    } else if (prec == BH_FP32) {
+        BH_CHECK(dev, "FP32 SpMV takes device float vectors");
+        bsr_spmv(M, alpha, x, beta, y, BH_FP32);
+    } else throw Error(-3, "bh_bsr_spmv: unsupported precision");
+    if (!dev) ctx->sync();
+    BH_API_END
+}
+
+// ------------------------------------------------------------------ solvers
+int bh_bjacobi_create(bh_bsr* M, double omega, int iterations, int prec, bh_solver** out) {
+    BH_API_BEGIN
+    BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_bjacobi_create: precision must be FP64 or FP32");
+    *out = new BlockJacobiSolver(M, omega, iterations, prec);
+    BH_API_END
+}
+
+static int make_gmres(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double tol, int maxit, bool flex, bh_solver** out) {
+    BH_API_BEGIN
+    BH_CHECK(M || op, "GMRES needs a matrix or an operator");
+    BH_CHECK(restart >= 1 && restart <= 256, "GMRES restart length must be in 1..256");
+    LinOp A;
+    A.M = M;
+    A.op = M ? nullptr : op;
+    A.N = M ? (int64_t)M->nrows * M->bs : op->N;
+    if (P) BH_CHECK(P->N == A.N, "preconditioner size mismatch");
+    *out = new GmresSolver(A, P, restart, tol, maxit, flex);
+    BH_API_END
+}
+
+int bh_gmres_create(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double tol, int maxit, bh_solver** out) {
+    return make_gmres(M, op, P, restart, tol, maxit, false, out);
+}
+int bh_fgmres_create(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double tol, int maxit, bh_solver** out) {
+    return make_gmres(M, op, P, restart, tol, maxit, true, out);
+}
+
+int bh_solver_solve(bh_solver* s, double* x, const double* b, int flags) {
+    BH_API_BEGIN
+    const bool dev = flags & BH_DEVICE_PTR;
+    const int64_t N = s->N;
+    Staged sx(s->ctx, x, N * sizeof(double), dev, true);
+    Staged sb(s->ctx, b, N * sizeof(double), dev, true);
+    s->solve_dev((double*)sx.dptr, (const double*)sb.dptr);
+    sx.download(x, N * sizeof(double), dev);
+    s->ctx->sync();
+    if (!std::isfinite(s->res)) throw Error(-5, "solver: NaN/Inf residual");  // CheckAndGetNorm, AS/SoftGMRES.cs:462-470
+    BH_API_END
+}
+
+int bh_solver_stats(bh_solver* s, int* it, int* conv, double* r0, double* r) {
+    BH_API_BEGIN
+    if (it) *it = s->iterations;
+    if (conv) *conv = s->converged;
+    if (r0) *r0 = s->res0;
+    if (r) *r = s->res;
+    BH_API_END
+}
+
+int bh_solver_destroy(bh_solver* s) {
+    BH_API_BEGIN
+    if (s) (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    BH_API_END
+}
+
+}  // extern "C"

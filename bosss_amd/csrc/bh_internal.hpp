@@ -1,0 +1,194 @@
+// Internal declarations of libbosss_hip.so (host C++ + HIP for gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bosss_hip.h"
+
+namespace bh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define BH_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::bh::Error(-2, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                                      __FILE__ + ":" + std::to_string(__LINE__));             \
+    } while (0)
+
+#define BH_CHECK(cond, msg)                                      \
+    do {                                                         \
+        if (!(cond)) throw ::bh::Error(-1, std::string(msg));    \
+    } while (0)
+
+// Device buffer owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n) BH_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+}  // namespace bh
+
+// ------------------------------------------------------------------ context
+struct bh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    // small pinned host staging for reductions
+    double* host_red = nullptr;
+    bh::DevBuf red;        // reduction partials
+    bh::DevBuf red_out;    // final reduction results
+    void upload(void* dst, const void* src, size_t bytes);
+    void download(void* dst, const void* src, size_t bytes);
+    void sync();
+};
+
+// ------------------------------------------------------------------ grid
+// Tensor-product Cartesian grid: cell widths per index and direction, the
+// reference's cell numbering, boundary sides.
+struct bh_grid {
+    bh_ctx* ctx = nullptr;
+    int D = 2;
+    int n[3] = {1, 1, 1};
+    std::vector<double> nodes[3];
+    std::vector<double> width[3];
+    int J = 0;
+    // edges (own enumeration: per cell, per direction: '-' boundary face, then '+' face)
+    std::vector<int> e_in, e_out, e_dir, e_side;
+    std::vector<double> cj;
+    bh::DevBuf d_width;     // 3 * nmax doubles: widths per direction (padded)
+    int nmax = 1;
+    int cell_index(int i, int j, int k) const {
+        return D == 2 ? i * n[1] + j : i + j * n[0] + k * n[0] * n[1];
+    }
+    void cell_ijk(int c, int* idx) const {
+        if (D == 2) { idx[0] = c / n[1]; idx[1] = c % n[1]; idx[2] = 0; }
+        else { idx[0] = c % n[0]; idx[1] = (c / n[0]) % n[1]; idx[2] = c / (n[0] * n[1]); }
+    }
+};
+
+// ------------------------------------------------------------------ tables
+// Reference-element tables of the orthonormal basis at the Gauss nodes.
+namespace bh {
+int basis_count(int D, int k);
+void basis_exponents(int D, int k, std::vector<int>& ex);
+void basis_eval(int D, int k, const double* xi, double* val, double* grad);
+void gauss_rule(int n, double* x, double* w);
+
+struct SipTables {
+    int D, k, Np, nq1, nqv, nqf;
+    std::vector<double> vw;     // nqv weights
+    std::vector<double> G;      // [nqv][D][Np]  d psi_n / d xi_d at volume nodes
+    std::vector<double> fw;     // nqf face weights
+    std::vector<double> T;      // [2D][nqf][Np] psi_n at face nodes of face f = 2d+s
+    std::vector<double> Nn;     // [2D][nqf][Np] d psi_n / d xi_d at face nodes
+    std::vector<double> vxi;    // nqv*D reference volume nodes
+    std::vector<double> fxi;    // nqf*(D-1) reference face-parameter nodes
+    void build(int D, int k, int nq1);
+};
+}  // namespace bh
+
+// ------------------------------------------------------------------ BSR
+struct bh_bsr {
+    bh_ctx* ctx = nullptr;
+    int nrows = 0, bs = 0;
+    int64_t nnzb = 0;
+    std::vector<int64_t> h_rowptr;
+    std::vector<int> h_col;
+    bh::DevBuf rowptr, col;     // int64 / int32
+    bh::DevBuf vals;            // FP64, nnzb blocks, each column-major
+    bh::DevBuf vals32;          // FP32 copy (lazy)
+    bh::DevBuf diag_idx;        // int64 per row: index of the diagonal block or -1
+    void ensure_fp32();
+    void ensure_diag();
+};
+
+// ------------------------------------------------------------------ SIP operator
+struct bh_op {
+    bh_ctx* ctx = nullptr;
+    bh_grid* grid = nullptr;
+    int D = 2, k = 1, Np = 1;
+    int64_t N = 0;
+    double penalty_base = 1.3, penalty_deg = 1.0;
+    int bc[6] = {0, 0, 0, 0, 0, 0};
+    bh::SipTables tab;
+    std::vector<double> h_nu;
+    bh::DevBuf nu, nu32;
+    bh::DevBuf tab64, tab32;    // device copies of the tables: G | T | Nn | vw | fw
+    size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0;
+    // boundary values and RHS -> affine vector
+    std::vector<double> gbnd;
+    std::vector<int> bnd_slot;  // per edge: first slot or -1
+    bh::DevBuf rhs;             // N doubles (or empty)
+    bh::DevBuf affine;          // N doubles (valid if affine_valid)
+    bool affine_valid = false;
+    bool has_affine = false;
+    void rebuild_affine();
+};
+
+// ------------------------------------------------------------------ solvers
+struct bh_solver {
+    virtual ~bh_solver() = default;
+    bh_ctx* ctx = nullptr;
+    int64_t N = 0;
+    int iterations = 0;
+    int converged = 0;
+    double res0 = 0, res = 0;
+    // x, b device pointers (FP64)
+    virtual void solve_dev(double* x, const double* b) = 0;
+};
+
+namespace bh {
+// kernels / launchers (defined in the .hip files)
+void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec);
+void sip_assemble(bh_op* op, bh_bsr* M);
+void bsr_spmv(bh_bsr* A, double alpha, const void* x, double beta, void* y, int prec);
+void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32);
+void bjacobi_sweep(bh_bsr* A, const void* dinv, double omega, const void* b, const void* xin, void* xout, int prec);
+void convert_f64_f32(bh_ctx* ctx, const double* src, float* dst, int64_t n);
+void convert_f32_f64(bh_ctx* ctx, const float* src, double* dst, int64_t n);
+void fill(bh_ctx* ctx, double* x, double v, int64_t n);
+void fill32(bh_ctx* ctx, float* x, float v, int64_t n);
+void copy(bh_ctx* ctx, double* dst, const double* src, int64_t n);
+void axpby(bh_ctx* ctx, double a, const double* x, double b, double* y, int64_t n);  // y = a x + b y
+void axpby32(bh_ctx* ctx, float a, const float* x, float b, float* y, int64_t n);
+// h[k] = <w, V_k>, k < m; V_k = V + k*ldv ; result in host array h
+void multi_dot(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* h);
+double dot(bh_ctx* ctx, const double* a, const double* b, int64_t n);
+double nrm2(bh_ctx* ctx, const double* a, int64_t n);
+// w -= sum_k h[k] V_k  (h host array, m <= 128)
+void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
+}  // namespace bh

@@ -1,0 +1,425 @@
+// Block-sparse (BlockMsrMatrix) SpMV, block-Jacobi, batched block inversion
+// and the vector kernels of the Krylov solvers, for gfx950.
+//
+// BSR SpMV design: one wave64 per block row.  Blocks are stored column-major
+// and the blocks of one row are contiguous, so the row's values form a stream
+// of "columns" of bs entries.  The 64 lanes are split into G = 64/bs groups of
+// bs lanes (lane = g*bs + r); at step t the groups read columns t*G .. t*G+G-1,
+// i.e. G*bs consecutive values: every value load is fully coalesced and each
+// value is read exactly once (the HBM-bound part).  x entries are
+// group-uniform gathers served by L1/L2.  Group partial sums are combined in
+// LDS.  This is the successor of the reference's cell-blocked CSR kernel
+// blockMultiply2 (ilPSP.LinSolvers.monkey/cuda-src/CudaMatrixKernelDP.cu:57).
+#include "bh_internal.hpp"
+
+namespace bh {
+
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void bsr_spmv_kernel(int nrows, const int64_t* __restrict__ rowptr,
+                                                       const int* __restrict__ col, const T* __restrict__ vals,
+                                                       const T* __restrict__ x, T* __restrict__ y, T alpha, T beta,
+                                                       const T* __restrict__ b_sub) {
+    constexpr int G = 64 / BS;
+    __shared__ T red[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wave;
+    const bool active = r < nrows;
+    const int g = lane / BS, ri = lane - g * BS;
+    T acc = 0;
+    if (active && g < G) {
+        const int64_t p0 = rowptr[r], p1 = rowptr[r + 1];
+        const int ncol = (int)(p1 - p0) * BS;
+        const T* __restrict__ base = vals + (size_t)p0 * BS * BS;
+        int q = g;
+#pragma unroll 4
+        for (; q < ncol; q += G) {
+            const int b = q / BS, cc = q - b * BS;
+            const T xv = x[(size_t)col[p0 + b] * BS + cc];
+            acc += base[(size_t)q * BS + ri] * xv;
+        }
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (active && g == 0) {
+        T s = acc;
+#pragma unroll
+        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * BS + ri];
+        const size_t o = (size_t)r * BS + ri;
+        T out = alpha * s;
+        if (b_sub) out = b_sub[o] - s;          // residual form: y = b - A x
+        else if (beta != T(0)) out += beta * y[o];
+        y[o] = out;
+    }
+}
+
+template <typename T>
+static void launch_spmv(bh_bsr* A, T alpha, const T* x, T beta, T* y, const T* bsub, const T* vals) {
+    dim3 blk(256), grd((A->nrows + 3) / 4);
+    hipStream_t st = A->ctx->stream;
+    const int64_t* rp = A->rowptr.as<int64_t>();
+    const int* cl = A->col.as<int>();
+#define BH_CASE(B) case B: hipLaunchKernelGGL((bsr_spmv_kernel<T, B>), grd, blk, 0, st, A->nrows, rp, cl, vals, x, y, alpha, beta, bsub); break;
+    switch (A->bs) {
+        BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
+        BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)
+        default: throw Error(-3, "bsr_spmv: unsupported block size " + std::to_string(A->bs));
+    }
+#undef BH_CASE
+    BH_HIP(hipGetLastError());
+}
+
+void bsr_spmv(bh_bsr* A, double alpha, const void* x, double beta, void* y, int prec) {
+    if (prec == BH_FP64) {
+        launch_spmv<double>(A, alpha, (const double*)x, beta, (double*)y, nullptr, A->vals.as<double>());
+    } else if (prec == BH_FP32) {
+        A->ensure_fp32();
+        launch_spmv<float>(A, (float)alpha, (const float*)x, (float)beta, (float*)y, nullptr, A->vals32.as<float>());
+    } else throw Error(-3, "bsr_spmv: unsupported precision");
+}
+
+// residual r = b - A x
+void bsr_residual(bh_bsr* A, const void* b, const void* x, void* r, int prec) {
+    if (prec == BH_FP64) {
+        launch_spmv<double>(A, 1.0, (const double*)x, 0.0, (double*)r, (const double*)b, A->vals.as<double>());
+    } else {
+        A->ensure_fp32();
+        launch_spmv<float>(A, 1.0f, (const float*)x, 0.0f, (float*)r, (const float*)b, A->vals32.as<float>());
+    }
+}
+
+// ------------------------------------------------------------ block inverse
+// Batched Gauss-Jordan with partial pivoting, one wave per block (bs <= 64),
+// lane = column of the augmented system [A | I].  Replaces the per-cell
+// InvertInPlace (LAPACK DGETRF/DGETRI, IL/IMatrixExtensions.cs:2221-2335)
+// of BlockJacobi.Init (AS/BlockJacobi.cs:100-146).
+__global__ __launch_bounds__(64) void invert_blocks_kernel(int nrows, int bs, const int64_t* __restrict__ diag_idx,
+                                                           const double* __restrict__ vals, double* __restrict__ inv,
+                                                           float* __restrict__ inv32, int* __restrict__ status) {
+    extern __shared__ double M[];   // bs x (2 bs), row-major
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= nrows) return;
+    const int W = 2 * bs;
+    const int64_t bi = diag_idx[r];
+    for (int i = 0; i < bs; ++i)
+        for (int j = lane; j < W; j += 64) {
+            double v;
+            if (j < bs) v = (bi >= 0) ? vals[(size_t)bi * bs * bs + (size_t)j * bs + i] : 0.0;  // column-major block
+            else v = (j - bs == i) ? 1.0 : 0.0;
+            M[i * W + j] = v;
+        }
+    __syncthreads();
+    bool ok = true;
+    for (int c = 0; c < bs; ++c) {
+        // pivot search (every lane scans; bs <= 64 so this is cheap)
+        int p = c;
+        double best = fabs(M[c * W + c]);
+        for (int i = c + 1; i < bs; ++i) {
+            double v = fabs(M[i * W + c]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (best == 0.0) { ok = false; break; }
+        __syncthreads();
+        if (p != c)
+            for (int j = lane; j < W; j += 64) { double t = M[p * W + j]; M[p * W + j] = M[c * W + j]; M[c * W + j] = t; }
+        __syncthreads();
+        const double piv = 1.0 / M[c * W + c];
+        __syncthreads();
+        for (int j = lane; j < W; j += 64) M[c * W + j] *= piv;
+        __syncthreads();
+        for (int i = 0; i < bs; ++i) {
+            if (i == c) continue;
+            const double f = M[i * W + c];
+            __syncthreads();
+            for (int j = lane; j < W; j += 64) M[i * W + j] -= f * M[c * W + j];
+            __syncthreads();
+        }
+    }
+    if (!ok) { if (lane == 0) status[0] = r + 1; return; }
+    // store the inverse column-major (as the matrix blocks)
+    for (int i = 0; i < bs; ++i)
+        for (int j = lane; j < bs; j += 64) {
+            const double v = M[i * W + bs + j];
+            inv[(size_t)r * bs * bs + (size_t)j * bs + i] = v;
+            if (inv32) inv32[(size_t)r * bs * bs + (size_t)j * bs + i] = (float)v;
+        }
+}
+
+void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32) {
+    A->ensure_diag();
+    const int bs = A->bs;
+    BH_CHECK(bs <= 64, "block inversion supports block sizes <= 64");
+    dinv64.alloc((size_t)A->nrows * bs * bs * sizeof(double));
+    if (dinv32) dinv32->alloc((size_t)A->nrows * bs * bs * sizeof(float));
+    DevBuf st(sizeof(int));
+    BH_HIP(hipMemsetAsync(st.p, 0, sizeof(int), A->ctx->stream));
+    size_t shm = (size_t)bs * 2 * bs * sizeof(double);
+    hipLaunchKernelGGL(invert_blocks_kernel, dim3(A->nrows), dim3(64), shm, A->ctx->stream, A->nrows, bs,
+                       A->diag_idx.as<int64_t>(), A->vals.as<double>(), dinv64.as<double>(),
+                       dinv32 ? dinv32->as<float>() : nullptr, st.as<int>());
+    BH_HIP(hipGetLastError());
+    int h = 0;
+    A->ctx->download(&h, st.p, sizeof(int));
+    BH_CHECK(h == 0, "block inversion: singular diagonal block in row " + std::to_string(h - 1));
+}
+
+// ------------------------------------------------------------ block Jacobi
+// One fused sweep: xout = xin + omega * Dinv (b - A xin).  Algebraically the
+// reference's update x = omega*Dinv*(b - A x + D x) + (1-omega) x
+// (AS/BlockJacobi.cs:196-255) with one block GEMV instead of two.
+// Phase 1 (same wave layout as the SpMV) forms the block-row residual in LDS,
+// phase 2 multiplies by the row's Dinv block (column-major, coalesced).
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void bjacobi_sweep_kernel(int nrows, const int64_t* __restrict__ rowptr,
+                                                            const int* __restrict__ col, const T* __restrict__ vals,
+                                                            const T* __restrict__ dinv, const T* __restrict__ b,
+                                                            const T* __restrict__ xin, T* __restrict__ xout, T omega) {
+    constexpr int G = 64 / BS;
+    __shared__ T red[4][64];
+    __shared__ T res[4][BS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wave;
+    const bool active = r < nrows;
+    const int g = lane / BS, ri = lane - g * BS;
+    T acc = 0;
+    if (active && g < G) {
+        const int64_t p0 = rowptr[r], p1 = rowptr[r + 1];
+        const int ncol = (int)(p1 - p0) * BS;
+        const T* __restrict__ base = vals + (size_t)p0 * BS * BS;
+#pragma unroll 4
+        for (int q = g; q < ncol; q += G) {
+            const int bb = q / BS, cc = q - bb * BS;
+            acc += base[(size_t)q * BS + ri] * xin[(size_t)col[p0 + bb] * BS + cc];
+        }
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (active && g == 0) {
+        T s = acc;
+#pragma unroll
+        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * BS + ri];
+        res[wave][ri] = b[(size_t)r * BS + ri] - s;
+    }
+    __syncthreads();
+    // phase 2: (Dinv q)_ri = sum_c Dinv[c*BS + ri] q_c, columns split over groups
+    T acc2 = 0;
+    if (active && g < G) {
+        const T* __restrict__ Di = dinv + (size_t)r * BS * BS;
+#pragma unroll 4
+        for (int cc = g; cc < BS; cc += G) acc2 += Di[cc * BS + ri] * res[wave][cc];
+    }
+    __syncthreads();
+    red[wave][lane] = acc2;
+    __syncthreads();
+    if (active && g == 0) {
+        T s = acc2;
+#pragma unroll
+        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * BS + ri];
+        const size_t o = (size_t)r * BS + ri;
+        xout[o] = xin[o] + omega * s;
+    }
+}
+
+void bjacobi_sweep(bh_bsr* A, const void* dinv, double omega, const void* b, const void* xin, void* xout, int prec) {
+    dim3 blk(256), grd((A->nrows + 3) / 4);
+    hipStream_t st = A->ctx->stream;
+    const int64_t* rp = A->rowptr.as<int64_t>();
+    const int* cl = A->col.as<int>();
+    if (prec == BH_FP64) {
+#define BH_CASE(B) case B: hipLaunchKernelGGL((bjacobi_sweep_kernel<double, B>), grd, blk, 0, st, A->nrows, rp, cl, A->vals.as<double>(), (const double*)dinv, (const double*)b, (const double*)xin, (double*)xout, omega); break;
+        switch (A->bs) {
+            BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
+            BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)
+            default: throw Error(-3, "bjacobi: unsupported block size");
+        }
+#undef BH_CASE
+    } else {
+        A->ensure_fp32();
+#define BH_CASE(B) case B: hipLaunchKernelGGL((bjacobi_sweep_kernel<float, B>), grd, blk, 0, st, A->nrows, rp, cl, A->vals32.as<float>(), (const float*)dinv, (const float*)b, (const float*)xin, (float*)xout, (float)omega); break;
+        switch (A->bs) {
+            BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
+            BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)
+            default: throw Error(-3, "bjacobi: unsupported block size");
+        }
+#undef BH_CASE
+    }
+    BH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ vectors
+__global__ void cvt_d2f(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = (float)s[i];
+}
+__global__ void cvt_f2d(const float* __restrict__ s, double* __restrict__ d, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = (double)s[i];
+}
+template <typename T>
+__global__ void fill_k(T* __restrict__ x, T v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
+}
+template <typename T>
+__global__ void axpby_k(T a, const T* __restrict__ x, T b, T* __restrict__ y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (b == T(0) ? T(0) : b * y[i]) + a * x[i];
+}
+
+static dim3 grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+void convert_f64_f32(bh_ctx* ctx, const double* s, float* d, int64_t n) {
+    hipLaunchKernelGGL(cvt_d2f, grid_for(n), dim3(256), 0, ctx->stream, s, d, n);
+}
+void convert_f32_f64(bh_ctx* ctx, const float* s, double* d, int64_t n) {
+    hipLaunchKernelGGL(cvt_f2d, grid_for(n), dim3(256), 0, ctx->stream, s, d, n);
+}
+void fill(bh_ctx* ctx, double* x, double v, int64_t n) {
+    hipLaunchKernelGGL((fill_k<double>), grid_for(n), dim3(256), 0, ctx->stream, x, v, n);
+}
+void fill32(bh_ctx* ctx, float* x, float v, int64_t n) {
+    hipLaunchKernelGGL((fill_k<float>), grid_for(n), dim3(256), 0, ctx->stream, x, v, n);
+}
+void copy(bh_ctx* ctx, double* dst, const double* src, int64_t n) {
+    BH_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+}
+void axpby(bh_ctx* ctx, double a, const double* x, double b, double* y, int64_t n) {
+    hipLaunchKernelGGL((axpby_k<double>), grid_for(n), dim3(256), 0, ctx->stream, a, x, b, y, n);
+}
+void axpby32(bh_ctx* ctx, float a, const float* x, float b, float* y, int64_t n) {
+    hipLaunchKernelGGL((axpby_k<float>), grid_for(n), dim3(256), 0, ctx->stream, a, x, b, y, n);
+}
+
+// ------------------------------------------------------------ reductions
+// Multi-dot h[k] = <w, V_k>, k < m (m <= 16 per launch): each thread keeps m
+// partial sums, a wave64 butterfly (__shfl_xor) reduces them, the 4 waves of
+// a block combine in LDS, and a second kernel sums the per-block partials in
+// a fixed order (deterministic, no atomics).
+constexpr int MD_BLOCKS = 1024;
+constexpr int MD_M = 16;
+
+template <int M>
+__global__ __launch_bounds__(256) void multidot_k(const double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                                                  int m, int64_t n, double* __restrict__ part) {
+    double acc[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) acc[k] = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double wi = w ? w[i] : 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if (k < m) {
+                const double v = V[(size_t)k * ldv + i];
+                acc[k] += (w ? wi : v) * v;   // w == nullptr: squared norms of V_k
+            }
+    }
+    __shared__ double sm[4][M];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) sm[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        const int k = threadIdx.x;
+        part[(size_t)blockIdx.x * M + k] = sm[0][k] + sm[1][k] + sm[2][k] + sm[3][k];
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_parts_k(const double* __restrict__ part, int nb, int M, double* __restrict__ out) {
+    const int k = blockIdx.x;
+    double s = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[(size_t)i * M + k];
+    __shared__ double sm[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) sm[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+
+static void ensure_red(bh_ctx* ctx) {
+    if (!ctx->red.p) ctx->red.alloc((size_t)MD_BLOCKS * MD_M * sizeof(double));
+    if (!ctx->red_out.p) ctx->red_out.alloc(256 * sizeof(double));
+    if (!ctx->host_red) BH_HIP(hipHostMalloc((void**)&ctx->host_red, 256 * sizeof(double), hipHostMallocDefault));
+}
+
+static void multi_dot_dev(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* dout) {
+    ensure_red(ctx);
+    int64_t nb = (n + 255) / 256;
+    if (nb > MD_BLOCKS) nb = MD_BLOCKS;
+    hipLaunchKernelGGL((multidot_k<MD_M>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, w, V, ldv, m, n, ctx->red.as<double>());
+    hipLaunchKernelGGL(reduce_parts_k, dim3(m), dim3(256), 0, ctx->stream, ctx->red.as<double>(), (int)nb, MD_M, dout);
+    BH_HIP(hipGetLastError());
+}
+
+void multi_dot(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* h) {
+    ensure_red(ctx);
+    BH_CHECK(m <= 256, "multi_dot: too many vectors");
+    for (int k0 = 0; k0 < m; k0 += MD_M) {
+        int mm = std::min(MD_M, m - k0);
+        multi_dot_dev(ctx, w, V + (size_t)k0 * ldv, ldv, mm, n, ctx->red_out.as<double>() + k0);
+    }
+    BH_HIP(hipMemcpyAsync(ctx->host_red, ctx->red_out.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < m; ++k) h[k] = ctx->host_red[k];
+}
+
+double dot(bh_ctx* ctx, const double* a, const double* b, int64_t n) {
+    double h;
+    multi_dot(ctx, a, b, 0, 1, n, &h);
+    return h;
+}
+double nrm2(bh_ctx* ctx, const double* a, int64_t n) {
+    double h;
+    ensure_red(ctx);
+    multi_dot_dev(ctx, nullptr, a, 0, 1, n, ctx->red_out.as<double>());
+    BH_HIP(hipMemcpyAsync(ctx->host_red, ctx->red_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    h = ctx->host_red[0];
+    return std::sqrt(h);
+}
+
+// w <- w + sign * sum_k h[k] V_k   (coefficients passed by value, m <= 32 per launch)
+struct Coefs { double c[32]; };
+__global__ __launch_bounds__(256) void multi_axpy_k(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                                                    int m, int64_t n, Coefs h) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = w[i];
+        for (int k = 0; k < m; ++k) s += h.c[k] * V[(size_t)k * ldv + i];
+        w[i] = s;
+    }
+}
+
+void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign) {
+    for (int k0 = 0; k0 < m; k0 += 32) {
+        Coefs c{};
+        int mm = std::min(32, m - k0);
+        for (int k = 0; k < mm; ++k) c.c[k] = sign * h[k0 + k];
+        hipLaunchKernelGGL(multi_axpy_k, grid_for(n), dim3(256), 0, ctx->stream, w, V + (size_t)k0 * ldv, ldv, mm, n, c);
+    }
+    BH_HIP(hipGetLastError());
+}
+
+}  // namespace bh
+
+// ------------------------------------------------------------ bh_bsr members
+void bh_bsr::ensure_fp32() {
+    if (vals32.p) return;
+    int64_t n = nnzb * bs * bs;
+    vals32.alloc(n * sizeof(float));
+    bh::convert_f64_f32(ctx, vals.as<double>(), vals32.as<float>(), n);
+}
+
+void bh_bsr::ensure_diag() {
+    if (diag_idx.p) return;
+    std::vector<int64_t> di(nrows, -1);
+    for (int r = 0; r < nrows; ++r)
+        for (int64_t p = h_rowptr[r]; p < h_rowptr[r + 1]; ++p)
+            if (h_col[p] == r) di[r] = p;
+    diag_idx.alloc(nrows * sizeof(int64_t));
+    ctx->upload(diag_idx.p, di.data(), nrows * sizeof(int64_t));
+}

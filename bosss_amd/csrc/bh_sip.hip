@@ -1,0 +1,374 @@
+// Matrix-free SIP operator evaluation and on-device assembly for gfx950.
+//
+// What it computes is the reference's SIP evaluate (SIPLaplace,
+// L3-solution/BoSSS.Solution.NSECommon/ipLaplace.cs:100-250, driven by
+// NECQuadratureVolume / NECQuadratureEdge, F/NECQuadratureVolume.cs:377-870,
+// F/NECQuadratureEdge.cs:561-1400): volume integral of -nu grad u . grad v,
+// interior-face consistency / symmetry / penalty terms and Dirichlet/Neumann
+// boundary terms, all by Gauss quadrature of the orthonormal modal basis.
+//
+// How (MI355X design): one lane per cell.  The reference-element tables
+// (basis gradients at volume nodes, traces and normal derivatives at face
+// nodes) are identical for every cell, so each table entry is a wave-uniform
+// operand fetched with scalar loads; the per-cell work is register-resident
+// FMA chains (basis-at-node, flux, flux-to-modes) with no LDS and no atomics.
+// Each cell evaluates its own side of all 2D faces (the neighbour's trace is
+// recomputed from the neighbour's modes, read through L2), so y is written
+// exactly once per apply: HBM traffic = u + y + nu.
+#include <cmath>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+
+template <typename R>
+struct SipArgs {
+    int J;
+    int n0, n1, n2;
+    int nmax;
+    const double* __restrict__ width;  // [3][nmax]
+    const R* __restrict__ nu;
+    const R* __restrict__ G;    // [nqv][D][Np]
+    const R* __restrict__ T;    // [2D][nqf][Np]
+    const R* __restrict__ Nn;   // [2D][nqf][Np]
+    const R* __restrict__ vw;   // [nqv]
+    const R* __restrict__ fw;   // [nqf]
+    double pen;                 // penalty_base * penalty_deg
+    int bc_neumann;             // bit per boundary side
+    const R* __restrict__ u;
+    R* __restrict__ y;
+    const double* __restrict__ affine;
+    double alpha, beta;
+};
+
+constexpr int binom(int n, int k) {
+    int r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+template <int D>
+__device__ __forceinline__ void decode(int c, int n0, int n1, int* idx) {
+    if (D == 2) { idx[0] = c / n1; idx[1] = c - idx[0] * n1; idx[2] = 0; }
+    else { int s = n0 * n1; idx[2] = c / s; int r = c - idx[2] * s; idx[1] = r / n0; idx[0] = r - idx[1] * n0; }
+}
+template <int D>
+__device__ __forceinline__ int encode(const int* idx, int n0, int n1) {
+    return D == 2 ? idx[0] * n1 + idx[1] : idx[0] + idx[1] * n0 + idx[2] * n0 * n1;
+}
+
+// cj = (surface area + boundary area) * 0.5 / volume (F/GridData_CellData.cs:666-700)
+template <int D, typename R>
+__device__ __forceinline__ double cell_cj(const SipArgs<R>& a, const int* idx, const double* h) {
+    int nn[3] = {a.n0, a.n1, a.n2};
+    double vol = 1, surf = 0, bnd = 0;
+    for (int d = 0; d < D; ++d) vol *= h[d];
+    for (int d = 0; d < D; ++d) {
+        double ar = vol / h[d];
+        surf += 2 * ar;
+        if (idx[d] == 0) bnd += ar;
+        if (idx[d] == nn[d] - 1) bnd += ar;
+    }
+    return 0.5 * (surf + bnd) / vol;
+}
+
+// Contribution of face f (= 2d + s, outward normal sgn*e_d) of cell P to
+// y_P (register array).  uP: modes of P; uQ: modes of the neighbour Q (may be
+// all zero), isBnd: boundary face.  Quantities follow ipLaplace.cs:160-250
+// written from P's side.
+template <int D, int K, typename R, bool SELF, bool NB>
+__device__ __forceinline__ void face_term(const SipArgs<R>& a, int f, const R* uP, const R* uQ,
+                                          R sdPinv, R sdQinv, R gP, R gQ, R dA, R nuP, R nuQ,
+                                          R mu, bool isBnd, R* y) {
+    constexpr int Np = binom(K + D, D);
+    constexpr int NQF = ipow(K + 1, D - 1);
+    const int d = f >> 1, s = f & 1;
+    const R sgn = s ? R(1) : R(-1);
+    const int fQ = 2 * d + (1 - s);
+    const R* __restrict__ TP = a.T + (size_t)f * NQF * Np;
+    const R* __restrict__ NP = a.Nn + (size_t)f * NQF * Np;
+    const R* __restrict__ TQ = a.T + (size_t)fQ * NQF * Np;
+    const R* __restrict__ NQ = a.Nn + (size_t)fQ * NQF * Np;
+    const R nuMax = (fabs((double)nuP) > fabs((double)nuQ)) ? nuP : nuQ;
+#pragma unroll 1
+    for (int q = 0; q < NQF; ++q) {
+        R vP = 0, dP = 0, vQ = 0, dQ = 0;
+        if (SELF) {
+#pragma unroll
+            for (int n = 0; n < Np; ++n) { vP += TP[q * Np + n] * uP[n]; dP += NP[q * Np + n] * uP[n]; }
+        }
+        if (NB && !isBnd) {
+#pragma unroll
+            for (int n = 0; n < Np; ++n) { vQ += TQ[q * Np + n] * uQ[n]; dQ += NQ[q * Np + n] * uQ[n]; }
+        }
+        vP *= sdPinv;
+        dP *= sgn * gP * sdPinv;   // outward normal derivative of u_P
+        vQ *= sdQinv;
+        dQ *= sgn * gQ * sdQinv;   // derivative of u_Q along P's outward normal
+        R cv, cd;                  // coefficients of v_P and of (dv_P/dn)
+        if (isBnd) {
+            // Dirichlet with g = 0 (the affine part is added separately); Neumann: none
+            cv = nuP * dP - R(2) * mu * nuP * vP;
+            cd = nuP * vP;
+        } else {
+            R jump = vP - vQ;
+            cv = R(0.5) * (nuP * dP + nuQ * dQ) - mu * nuMax * jump;
+            cd = R(0.5) * nuP * jump;
+        }
+        R wq = a.fw[q] * dA * sdPinv;
+        cv *= wq;
+        cd *= wq * sgn * gP;
+#pragma unroll
+        for (int m = 0; m < Np; ++m) y[m] += cv * TP[q * Np + m] + cd * NP[q * Np + m];
+    }
+}
+
+template <int D, int K, typename R>
+__device__ __forceinline__ void volume_term(const SipArgs<R>& a, const R* uP, const R* s2, R nuP, R* y) {
+    constexpr int Np = binom(K + D, D);
+    constexpr int NQV = ipow(K + 1, D);
+#pragma unroll 1
+    for (int q = 0; q < NQV; ++q) {
+        const R* __restrict__ Gq = a.G + (size_t)q * D * Np;
+        R g[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            R acc = 0;
+#pragma unroll
+            for (int n = 0; n < Np; ++n) acc += Gq[d * Np + n] * uP[n];
+            g[d] = -nuP * s2[d] * a.vw[q] * acc;
+        }
+#pragma unroll
+        for (int m = 0; m < Np; ++m) {
+            R acc = y[m];
+#pragma unroll
+            for (int d = 0; d < D; ++d) acc += Gq[d * Np + m] * g[d];
+            y[m] = acc;
+        }
+    }
+}
+
+// geometry of cell with logical index idx
+template <int D, typename R>
+__device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, double* h) {
+    for (int d = 0; d < D; ++d) h[d] = a.width[d * a.nmax + idx[d]];
+}
+
+template <int D, int K, typename R>
+__global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
+    constexpr int Np = binom(K + D, D);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.J) return;
+    int idx[3];
+    decode<D>(c, a.n0, a.n1, idx);
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    double h[3];
+    cell_geom<D>(a, idx, h);
+    R uP[Np], y[Np];
+    const R* up = a.u + (size_t)c * Np;
+#pragma unroll
+    for (int n = 0; n < Np; ++n) { uP[n] = up[n]; y[n] = 0; }
+    double detJ = 1;
+    for (int d = 0; d < D; ++d) detJ *= 0.5 * h[d];
+    const R nuP = a.nu[c];
+    R s2[D];
+    for (int d = 0; d < D; ++d) s2[d] = R(4.0 / (h[d] * h[d]));
+    volume_term<D, K, R>(a, uP, s2, nuP, y);
+    const double cjP = cell_cj<D>(a, idx, h);
+    const R sdPinv = R(1.0 / sqrt(detJ));
+#pragma unroll 1
+    for (int f = 0; f < 2 * D; ++f) {
+        const int d = f >> 1, s = f & 1;
+        const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+        const double dA = detJ / (0.5 * h[d]);
+        const R gP = R(2.0 / h[d]);
+        if (isBnd) {
+            if ((a.bc_neumann >> f) & 1) continue;
+            R uQ[1] = {0};
+            face_term<D, K, R, true, false>(a, f, uP, uQ, sdPinv, R(0), gP, R(0), R(dA), nuP, nuP,
+                                            R(a.pen * cjP), true, y);
+        } else {
+            int nidx[3] = {idx[0], idx[1], idx[2]};
+            nidx[d] += s ? 1 : -1;
+            const int cq = encode<D>(nidx, a.n0, a.n1);
+            double hq[3] = {h[0], h[1], h[2]};
+            hq[d] = a.width[d * a.nmax + nidx[d]];
+            const double detQ = detJ / h[d] * hq[d];
+            const double cjQ = cell_cj<D>(a, nidx, hq);
+            R uQ[Np];
+            const R* uq = a.u + (size_t)cq * Np;
+#pragma unroll
+            for (int n = 0; n < Np; ++n) uQ[n] = uq[n];
+            face_term<D, K, R, true, true>(a, f, uP, uQ, sdPinv, R(1.0 / sqrt(detQ)), gP, R(2.0 / hq[d]), R(dA),
+                                           nuP, a.nu[cq], R(a.pen * fmax(cjP, cjQ)), false, y);
+        }
+    }
+    R* yp = a.y + (size_t)c * Np;
+    const R al = R(a.alpha), be = R(a.beta);
+    if (a.affine) {
+        const double* af = a.affine + (size_t)c * Np;
+#pragma unroll
+        for (int n = 0; n < Np; ++n) y[n] += R(af[n]);
+    }
+    if (a.beta == 0.0) {
+#pragma unroll
+        for (int n = 0; n < Np; ++n) yp[n] = al * y[n];
+    } else {
+#pragma unroll
+        for (int n = 0; n < Np; ++n) yp[n] = be * yp[n] + al * y[n];
+    }
+}
+
+// Assembly: thread = (cell r, trial mode n).  Column n of the diagonal block
+// is the apply with u = e_n on r only; column n of the block coupling r to
+// its neighbour across face f is the face-f term with u_P = 0, u_Q = e_n.
+// Blocks are stored column-major, so each thread writes contiguous columns.
+// Follows the split of LECVolumeQuadrature2 / LECEdgeQuadrature2
+// (F/LECVolumeQuadrature2.cs:643-1140, F/LECEdgeQuadrature2.cs:1370-1560).
+template <int D, int K>
+__global__ __launch_bounds__(256) void sip_assemble_kernel(SipArgs<double> a, const int64_t* __restrict__ rowptr,
+                                                           const int* __restrict__ col, double* __restrict__ vals) {
+    constexpr int Np = binom(K + D, D);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.J * Np) return;
+    const int c = (int)(t / Np), n = (int)(t % Np);
+    int idx[3];
+    decode<D>(c, a.n0, a.n1, idx);
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    double h[3];
+    cell_geom<D>(a, idx, h);
+    double e[Np], z[Np], y[Np];
+#pragma unroll
+    for (int m = 0; m < Np; ++m) { e[m] = (m == n) ? 1.0 : 0.0; z[m] = 0; y[m] = 0; }
+    double detJ = 1;
+    for (int d = 0; d < D; ++d) detJ *= 0.5 * h[d];
+    const double nuP = a.nu[c];
+    double s2[D];
+    for (int d = 0; d < D; ++d) s2[d] = 4.0 / (h[d] * h[d]);
+    volume_term<D, K, double>(a, e, s2, nuP, y);
+    const double cjP = cell_cj<D>(a, idx, h);
+    const double sdPinv = 1.0 / sqrt(detJ);
+    const int64_t r0 = rowptr[c], r1 = rowptr[c + 1];
+    for (int f = 0; f < 2 * D; ++f) {
+        const int d = f >> 1, s = f & 1;
+        const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+        const double dA = detJ / (0.5 * h[d]);
+        const double gP = 2.0 / h[d];
+        if (isBnd) {
+            if ((a.bc_neumann >> f) & 1) continue;
+            face_term<D, K, double, true, false>(a, f, e, z, sdPinv, 0.0, gP, 0.0, dA, nuP, nuP, a.pen * cjP, true, y);
+        } else {
+            int nidx[3] = {idx[0], idx[1], idx[2]};
+            nidx[d] += s ? 1 : -1;
+            const int cq = encode<D>(nidx, a.n0, a.n1);
+            double hq[3] = {h[0], h[1], h[2]};
+            hq[d] = a.width[d * a.nmax + nidx[d]];
+            const double detQ = detJ / h[d] * hq[d];
+            const double cjQ = cell_cj<D>(a, nidx, hq);
+            const double mu = a.pen * fmax(cjP, cjQ);
+            const double nuQ = a.nu[cq];
+            // self part into the diagonal block column
+            face_term<D, K, double, true, false>(a, f, e, z, sdPinv, 1.0 / sqrt(detQ), gP, 2.0 / hq[d], dA, nuP, nuQ, mu,
+                                                 false, y);
+            // coupling block (c, cq), column n
+            double yo[Np];
+#pragma unroll
+            for (int m = 0; m < Np; ++m) yo[m] = 0;
+            face_term<D, K, double, false, true>(a, f, z, e, sdPinv, 1.0 / sqrt(detQ), gP, 2.0 / hq[d], dA, nuP, nuQ, mu,
+                                                 false, yo);
+            int64_t b = -1;
+            for (int64_t p = r0; p < r1; ++p) if (col[p] == cq) b = p;
+            if (b >= 0) {
+                double* dst = vals + (size_t)b * Np * Np + (size_t)n * Np;
+#pragma unroll
+                for (int m = 0; m < Np; ++m) dst[m] = yo[m];
+            }
+        }
+    }
+    int64_t b = -1;
+    for (int64_t p = r0; p < r1; ++p) if (col[p] == c) b = p;
+    if (b >= 0) {
+        double* dst = vals + (size_t)b * Np * Np + (size_t)n * Np;
+#pragma unroll
+        for (int m = 0; m < Np; ++m) dst[m] = y[m];
+    }
+}
+
+template <typename R>
+static SipArgs<R> make_args(bh_op* op, const void* u, void* y, double alpha, double beta, bool add_affine) {
+    SipArgs<R> a{};
+    bh_grid* g = op->grid;
+    a.J = g->J;
+    a.n0 = g->n[0]; a.n1 = g->n[1]; a.n2 = g->n[2];
+    a.nmax = g->nmax;
+    a.width = g->d_width.as<double>();
+    const bool f32 = sizeof(R) == 4;
+    a.nu = (const R*)(f32 ? op->nu32.p : op->nu.p);
+    const char* tb = (const char*)(f32 ? op->tab32.p : op->tab64.p);
+    const size_t es = sizeof(R);
+    a.G = (const R*)(tb + op->off_G * es);
+    a.T = (const R*)(tb + op->off_T * es);
+    a.Nn = (const R*)(tb + op->off_N * es);
+    a.vw = (const R*)(tb + op->off_vw * es);
+    a.fw = (const R*)(tb + op->off_fw * es);
+    a.pen = op->penalty_base * op->penalty_deg;
+    a.bc_neumann = 0;
+    for (int f = 0; f < 2 * op->D; ++f) if (op->bc[f] == 1) a.bc_neumann |= 1 << f;
+    a.u = (const R*)u;
+    a.y = (R*)y;
+    a.affine = (add_affine && op->has_affine) ? op->affine.as<double>() : nullptr;
+    a.alpha = alpha; a.beta = beta;
+    return a;
+}
+
+template <typename R, int D, int K>
+static void launch_apply(bh_op* op, const SipArgs<R>& a) {
+    dim3 blk(256), grd((a.J + 255) / 256);
+    hipLaunchKernelGGL((sip_apply_kernel<D, K, R>), grd, blk, 0, op->ctx->stream, a);
+    BH_HIP(hipGetLastError());
+}
+
+template <typename R>
+static void dispatch_apply(bh_op* op, const SipArgs<R>& a) {
+    const int D = op->D, K = op->k;
+#define BH_CASE(DD, KK) if (D == DD && K == KK) return launch_apply<R, DD, KK>(op, a);
+    BH_CASE(2, 1) BH_CASE(2, 2) BH_CASE(2, 3) BH_CASE(2, 4)
+    BH_CASE(3, 1) BH_CASE(3, 2) BH_CASE(3, 3)
+#undef BH_CASE
+    throw Error(-3, "sip_apply: unsupported (D, k) = (" + std::to_string(D) + ", " + std::to_string(K) + ")");
+}
+
+void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec) {
+    if (add_affine && op->has_affine && !op->affine_valid) op->rebuild_affine();
+    if (prec == BH_FP64) {
+        dispatch_apply<double>(op, make_args<double>(op, u, y, alpha, beta, add_affine));
+    } else if (prec == BH_FP32) {
+        if (add_affine && op->has_affine) throw Error(-3, "FP32 evaluate supports the linear part only");
+        dispatch_apply<float>(op, make_args<float>(op, u, y, alpha, beta, false));
+    } else {
+        throw Error(-3, "sip_apply: precision not supported by this entry point");
+    }
+}
+
+template <int D, int K>
+static void launch_asm(bh_op* op, bh_bsr* M) {
+    SipArgs<double> a = make_args<double>(op, nullptr, nullptr, 1.0, 0.0, false);
+    constexpr int Np = binom(K + D, D);
+    int64_t nt = (int64_t)a.J * Np;
+    dim3 blk(256), grd((unsigned)((nt + 255) / 256));
+    hipLaunchKernelGGL((sip_assemble_kernel<D, K>), grd, blk, 0, op->ctx->stream, a, M->rowptr.as<int64_t>(),
+                       M->col.as<int>(), M->vals.as<double>());
+    BH_HIP(hipGetLastError());
+}
+
+void sip_assemble(bh_op* op, bh_bsr* M) {
+    const int D = op->D, K = op->k;
+#define BH_CASE(DD, KK) if (D == DD && K == KK) return launch_asm<DD, KK>(op, M);
+    BH_CASE(2, 1) BH_CASE(2, 2) BH_CASE(2, 3) BH_CASE(2, 4)
+    BH_CASE(3, 1) BH_CASE(3, 2) BH_CASE(3, 3)
+#undef BH_CASE
+    throw Error(-3, "sip_assemble: unsupported (D, k)");
+}
+
+}  // namespace bh

@@ -1,0 +1,144 @@
+/* bosss_hip.h -- C ABI of libbosss_hip.so, the MI355X (gfx950) backend for the
+ * BoSSS DG-operator / block-solver hot path.
+ *
+ * Conventions (mirroring the native bindings BoSSS already uses through
+ * DynLibLoader, e.g. ilPSP.LinSolvers.PARDISO/Wrapper_MKL.cs:33-115):
+ *   - every entry point returns an int status: 0 = OK, < 0 = error; the text of
+ *     the last error of the calling thread is read with bh_last_error();
+ *   - handles are opaque; the caller owns every array it passes in; the library
+ *     copies what it keeps into device memory it owns and frees in *_destroy;
+ *   - vector arguments are host pointers unless BH_DEVICE_PTR is set in the
+ *     `flags` argument, in which case they are device pointers (HBM-resident,
+ *     e.g. from bh_malloc) and no host<->device copy happens;
+ *   - all work of one context is ordered on one HIP stream; calls return when
+ *     the result is complete (host pointers) or enqueued (device pointers;
+ *     bh_sync waits);
+ *   - DOF vectors are cell-major: index = cell * Np + mode, cells in the
+ *     numbering of the reference's Cartesian grid generators, modes in the
+ *     order of the reference's orthonormal polynomial tables
+ *     (F/UnsetteledCoordinateMapping.cs:696-711, F/Square.cs:192-260,
+ *     F/Cube_polynomials.cs:13-60).
+ * Reference paths are relative to the BoSSS source tree `src/`:
+ *   F/ = L2-foundation/BoSSS.Foundation, P/ = L1-platform/BoSSS.Platform,
+ *   AS/ = L3-solution/BoSSS.Solution.AdvancedSolvers, IL/ = ilPSP/layer_1.2-ilPSP/ilPSP.
+ */
+#ifndef BOSSS_HIP_H
+#define BOSSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bh_ctx bh_ctx;
+typedef struct bh_grid bh_grid;
+typedef struct bh_op bh_op;
+typedef struct bh_bsr bh_bsr;
+typedef struct bh_solver bh_solver;
+
+#define BH_DEVICE_PTR 1 /* vector arguments are device pointers */
+
+/* precisions */
+#define BH_FP64 0
+#define BH_FP32 1
+#define BH_FP16_MFMA 2 /* FP16 inputs, FP32 accumulation on MFMA */
+
+/* ---------------------------------------------------------------- context
+ * Replaces the library load + init of DynLibLoader (MW/DynLibLoader.cs:30). */
+int bh_init(int device, bh_ctx** ctx);
+int bh_finalize(bh_ctx* ctx);
+int bh_last_error(char* buf, int len);
+int bh_sync(bh_ctx* ctx);
+int bh_malloc(bh_ctx* ctx, size_t bytes, void** dptr);
+int bh_free(bh_ctx* ctx, void* dptr);
+int bh_memcpy_h2d(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
+int bh_memcpy_d2h(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* elapsed device time (ms) of the last `bh_timed_*` region on the context stream */
+int bh_event_record(bh_ctx* ctx, int slot);
+int bh_event_elapsed(bh_ctx* ctx, int slot0, int slot1, float* ms);
+
+/* ---------------------------------------------------------------- grid
+ * Cartesian grid from node arrays, as Grid2D.Cartesian2DGrid
+ * (BoSSS.Foundation.Grid/Grid2D.cs:396-600, GlobalID = i*nY + j at :586) and
+ * Grid3D.Cartesian3DGrid (Grid3D.cs:88-300, GlobalID = i + j*nX + k*nX*nY at
+ * :261).  z may be NULL for D = 2. */
+int bh_grid_cartesian(bh_ctx* ctx, int D, const double* x, int nx1, const double* y, int ny1,
+                      const double* z, int nz1, bh_grid** grid);
+int bh_grid_destroy(bh_grid* grid);
+int bh_grid_info(bh_grid* grid, int* D, int* J, int* E);
+/* edges (IN cell, OUT cell or -1, normal direction, boundary side or -1) and
+ * cj (F/GridData_EdgeData.cs:2337-2604 CellIndices/EdgeTags;
+ * F/GridData_CellData.cs:666-700 cj) -- exported for bit-exact index checks */
+int bh_grid_edges(bh_grid* grid, int* in, int* out, int* dir, int* side);
+int bh_grid_cj(bh_grid* grid, double* cj);
+
+/* ---------------------------------------------------------------- SIP operator
+ * DifferentialOperator with one SIPLaplace component (ipLaplace.cs:30-250;
+ * ipPoisson/SipPoissonMain.cs:188-205) plus an RHS source (-rhs*v,
+ * SipPoissonMain.cs:405-436).  nu: per-cell diffusion (J values);
+ * bc[2*D]: 0 = Dirichlet, 1 = Neumann per boundary side (-x,+x,-y,+y,-z,+z);
+ * penalty_base as the ipFlux ctor argument (SipControl.cs:76 default 1.3). */
+int bh_sip_create(bh_grid* grid, int k, double penalty_base, const double* nu, const int* bc, bh_op** op);
+int bh_op_destroy(bh_op* op);
+int bh_op_info(bh_op* op, int* Np, int64_t* N);
+/* boundary values g at the boundary-edge quadrature nodes (layout: boundary
+ * edges in edge order x face nodes); bh_sip_bnd_nodes returns their physical
+ * coordinates (count * D) so the host can evaluate g there (ipFlux.g_Diri /
+ * g_Neum, SipPoissonMain.cs:370-385). */
+int bh_sip_num_bnd_nodes(bh_op* op, int* count);
+int bh_sip_bnd_nodes(bh_op* op, double* X);
+int bh_sip_set_bnd_values(bh_op* op, const double* g);
+/* RHS source field as DG coefficients (N values; NULL clears). */
+int bh_sip_set_rhs(bh_op* op, const double* f, int flags);
+/* IEvaluatorNonLin.Evaluate(alpha, beta, output) (F/IEvaluator.cs;
+ * F/DifferentialOperator.cs:1601-1685):  y <- beta*y + alpha*Op(u), with the
+ * affine (boundary-value and source) part included when `affine` != 0.
+ * prec: BH_FP64 (drop-in), BH_FP32 or BH_FP16_MFMA (preconditioner use). */
+int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double* y, int affine,
+                   int prec, int flags);
+/* IEvaluatorLinear.ComputeMatrix (F/DifferentialOperator.cs:1787-1860):
+ * assemble the operator matrix on the device and return the affine vector
+ * (N values, host or device per flags; may be NULL). */
+int bh_op_assemble(bh_op* op, bh_bsr** mat, double* affine, int flags);
+
+/* ---------------------------------------------------------------- BlockMsrMatrix
+ * Block-sparse matrix over cell blocks (P/BlockMsrMatrix.cs:191).  rowptr has
+ * nrows+1 entries, col[nnzb] block-column indices (ascending per row), vals
+ * nnzb blocks of bs*bs doubles, each block row-major. */
+int bh_bsr_create(bh_ctx* ctx, int nrows, int bs, const int64_t* rowptr, const int* col,
+                  const double* vals, bh_bsr** mat);
+int bh_bsr_destroy(bh_bsr* mat);
+int bh_bsr_info(bh_bsr* mat, int* nrows, int* bs, int64_t* nnzb);
+int bh_bsr_export(bh_bsr* mat, int64_t* rowptr, int* col, double* vals);
+/* BlockMsrMatrix.SpMV(alpha, x, beta, y) (P/BlockMsrMatrix.cs:2970-3420):
+ * y <- beta*y + alpha*A*x; prec BH_FP64 or BH_FP32 (FP32 copy of the values
+ * and FP32 vectors, made on first use). */
+int bh_bsr_spmv(bh_bsr* mat, double alpha, const void* x, double beta, void* y, int prec, int flags);
+
+/* ---------------------------------------------------------------- solvers
+ * ISolverSmootherTemplate (AS/ISmootherTemplate.cs): Init(op) at create time,
+ * Solve(X, B) via bh_solver_solve, statistics via bh_solver_stats. */
+/* BlockJacobi (AS/BlockJacobi.cs:35-260): omega, NoOfIterations (sweeps =
+ * NoOfIterations + 1, as the reference's `iter <= NoOfIterations`). */
+int bh_bjacobi_create(bh_bsr* mat, double omega, int iterations, int prec, bh_solver** solver);
+/* SoftGMRES (AS/SoftGMRES.cs:75-470), left preconditioned, restart m; tol as
+ * ConvergenceCriterion of IterativeSolverConfig.DefaultTermination
+ * (AS/IterativeSolverConfig.cs:48-58).  precond may be NULL.  The operator is
+ * the matrix `mat`, or the matrix-free SIP operator `op` when mat is NULL. */
+int bh_gmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, double tol, int maxit,
+                    bh_solver** solver);
+/* FlexGMRES (AS/FlexGMRES.cs): right preconditioned, flexible (the
+ * preconditioner may change per iteration, e.g. FP32 / learned), convergence
+ * judged on the true FP64 residual ||b - A x|| / ||b|| <= tol. */
+int bh_fgmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, double tol, int maxit,
+                     bh_solver** solver);
+int bh_solver_solve(bh_solver* solver, double* x, const double* b, int flags);
+int bh_solver_stats(bh_solver* solver, int* iterations, int* converged, double* res0, double* res);
+int bh_solver_destroy(bh_solver* solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BOSSS_HIP_H */
