@@ -35,6 +35,82 @@ def sip_flops(n, D, k):
     return J * 4 * D * Np * Nq + F * 16 * Np * Nqf
 
 
+def legendre_1d(k, xi):
+    """orthonormal 1D Legendre sqrt(n+1/2) P_n at xi, shape (len(xi), k+1)."""
+    P = np.zeros((len(xi), k + 1))
+    P[:, 0] = 1.0
+    if k > 0:
+        P[:, 1] = xi
+    for n in range(2, k + 1):
+        P[:, n] = ((2 * n - 1) * xi * P[:, n - 1] - (n - 1) * P[:, n - 2]) / n
+    return P * np.sqrt(np.arange(k + 1) + 0.5)
+
+
+def sine_rhs_coeffs(n, k, seed, nmodes=16, kmax=6):
+    """DG coefficients (cell-major, reference mode order) of
+    f = sum_m a_m prod_d sin(pi w_md x_d), a_m ~ N(0,1), w_md in 1..kmax, on the
+    uniform n^3 grid of [0,1]^3: the projection is separable per direction."""
+    from math import comb
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(nmodes)
+    w = rng.integers(1, kmax + 1, (nmodes, 3))
+    xg, wg = np.polynomial.legendre.leggauss(8)
+    L = legendre_1d(k, xg)                              # (8, k+1)
+    h = 1.0 / n
+    lo = np.arange(n) * h
+    X = lo[:, None] + 0.5 * (xg[None, :] + 1.0) * h     # (n, 8)
+    # 1D integrals S[m, d, i, e] = int_cell sin(pi w x) psi_e / sqrt(h/2) dx
+    S = np.einsum("mdiq,q,qe->mdie", np.sin(np.pi * w[:, :, None, None] * X[None, None]), wg, L) * np.sqrt(h / 2)
+    ex = [(a_, b_, d - a_ - b_) for d in range(k + 1) for a_ in range(d + 1) for b_ in range(d - a_ + 1)]
+    ex = np.array(ex)                                   # (Np, 3) exponents (x, y, z)
+    Np = comb(k + 3, 3)
+    assert len(ex) == Np
+    # cell c = i + j*n + kk*n*n
+    C = np.zeros((n, n, n, Np))                         # [kk, j, i, mode]
+    for m in range(nmodes):
+        Sx = S[m, 0][:, ex[:, 0]]                       # (n_i, Np)
+        Sy = S[m, 1][:, ex[:, 1]]
+        Sz = S[m, 2][:, ex[:, 2]]
+        C += a[m] * Sz[:, None, None, :] * Sy[None, :, None, :] * Sx[None, None, :, :]
+    return C.reshape(-1, Np).ravel()
+
+
+def solve_leg(B, ctx, grid, op, n, k, out):
+    """time-to-solve: FP64 FGMRES to 1e-8 true relative residual with the
+    p/h-multigrid K-cycle preconditioner in FP64 (plain) vs FP32 (mixed)."""
+    N = op.N
+    f = sine_rhs_coeffs(n, k, seed=16)
+    b = B.DeviceArray.from_host(ctx, f)
+    t0 = time.perf_counter()
+    M, _ = op.ComputeMatrix()
+    ctx.sync()
+    t_asm = time.perf_counter() - t0
+    res = {"matrix_assembly_s": t_asm}
+    for name, prec in (("plain_fp64", B.BH_FP64), ("mixed_fp32", B.BH_FP32)):
+        t0 = time.perf_counter()
+        mg = B.PMultigrid(op, M=M, degrees=[k, 1], min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2, prec=prec)
+        ctx.sync()
+        t_init = time.perf_counter() - t0
+        fg = B.FlexGMRES(op=op, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=500)
+        x = B.DeviceArray.from_host(ctx, np.zeros(N))
+        ctx.sync()
+        t0 = time.perf_counter()
+        fg.Solve(x, b)
+        ctx.sync()
+        t_run = time.perf_counter() - t0
+        st = fg.stats()
+        # independent FP64 check of the true residual
+        r = B.DeviceArray(ctx, N)
+        op.Evaluate(1.0, 0.0, r, x, affine=False)
+        rr = f - r.download()
+        relres = float(np.linalg.norm(rr) / np.linalg.norm(f))
+        res[name] = {"solver_init_s": t_init, "solver_run_s": t_run, "iterations": st["iterations"],
+                     "converged": st["converged"], "true_rel_residual": relres}
+        del fg, mg, x, r
+    res["speedup_mixed_vs_plain"] = res["plain_fp64"]["solver_run_s"] / res["mixed_fp32"]["solver_run_s"]
+    out["time_to_solve"] = res
+
+
 def cpu_baseline(k, target_s=12.0):
     """Oracle (CPU restatement of the reference's quadrature evaluate) on a
     bounded sample: 3D 24^3 cells, same k and coefficient law."""
@@ -51,7 +127,7 @@ def cpu_baseline(k, target_s=12.0):
         op.evaluate(u)
         reps += 1
         dt = time.perf_counter() - t0
-        if dt > target_s or reps >= 200:
+        if dt > target_s or reps >= 2000:
             break
     return dict(value=op.N * reps / dt, unit="DOF/s", cores=int(O.lib().orc_num_threads()), kind="port",
                 sample=f"oracle FP64 SIP evaluate, 3D {n}^3 cells k={k} ({op.N} DOF), {reps} applies in {dt:.1f} s")
@@ -126,6 +202,8 @@ def main():
                      "algorithmic_flop": flops, "algorithmic_bytes": hbm_bytes,
                      "hbm_frac_at_algorithmic_bytes": hbm_bytes / (kern_ms * 1e-3) / PEAK_HBM},
     }
+    if not args.no_solve:
+        solve_leg(B, ctx, grid, op, n, k, out)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(k)
     print(json.dumps(out), flush=True)

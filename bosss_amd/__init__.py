@@ -18,6 +18,7 @@ this package raises if the library has not been built.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -68,6 +69,7 @@ SIGNATURES = {
     "bh_bjacobi_create": (_c_i, [_c_p, _c_d, _c_i, _c_i, _PP]),
     "bh_gmres_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_fgmres_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
+    "bh_pmg_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _c_i, _PP]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
     "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
                                ctypes.POINTER(_c_d)]),
@@ -124,6 +126,7 @@ class DeviceArray:
         p = ctypes.c_void_p()
         _check(lib().bh_malloc(ctx.h, self.n * self.dtype.itemsize, ctypes.byref(p)))
         self.ptr = p.value
+        ctx._children.append(weakref.ref(self))
 
     @classmethod
     def from_host(cls, ctx, a, dtype=None):
@@ -143,9 +146,11 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             _check(lib().bh_free(self.ctx.h, self.ptr))
-            self.ptr = None
+        self.ptr = None
+
+    destroy = free
 
     def __del__(self):
         try:
@@ -161,6 +166,31 @@ def _vec_args(x):
     return x.ctypes.data, 0
 
 
+class _Owned:
+    """A native handle owned by a Context.  The context destroys every live
+    child before it finalises, so no destructor can run against a freed
+    context (native objects synchronise on the context's stream)."""
+
+    _destroy_fn = None
+
+    def _register(self, ctx, handle):
+        self.h = handle
+        self._ctx_ref = ctx
+        ctx._children.append(weakref.ref(self))
+
+    def destroy(self):
+        h = getattr(self, "h", None)
+        if h and self._ctx_ref.h:
+            getattr(lib(), self._destroy_fn)(h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU, one stream (bh_init)."""
 
@@ -168,6 +198,7 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().bh_init(device, ctypes.byref(h)))
         self.h = h.value
+        self._children = []
 
     def sync(self):
         _check(lib().bh_sync(self.h))
@@ -182,12 +213,25 @@ class Context:
 
     def close(self):
         if self.h:
+            for r in reversed(self._children):
+                o = r()
+                if o is not None:
+                    o.destroy()
+            self._children = []
             _check(lib().bh_finalize(self.h))
             self.h = None
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
-class Grid:
+
+class Grid(_Owned):
     """Cartesian grid (reference numbering; see bh_grid_cartesian)."""
+
+    _destroy_fn = "bh_grid_destroy"
 
     def __init__(self, ctx, *nodes):
         self.ctx = ctx
@@ -196,7 +240,7 @@ class Grid:
         h = ctypes.c_void_p()
         _check(lib().bh_grid_cartesian(ctx.h, self.D, _ptr(ns[0]), len(ns[0]), _ptr(ns[1]), len(ns[1]),
                                        _ptr(ns[2]), len(ns[2]) if ns[2] is not None else 0, ctypes.byref(h)))
-        self.h = h.value
+        self._register(ctx, h.value)
         D, J, E = _c_i(), _c_i(), _c_i()
         _check(lib().bh_grid_info(self.h, ctypes.byref(D), ctypes.byref(J), ctypes.byref(E)))
         self.J, self.E = J.value, E.value
@@ -211,12 +255,6 @@ class Grid:
         _check(lib().bh_grid_cj(self.h, c.ctypes.data))
         return c
 
-    def __del__(self):
-        try:
-            lib().bh_grid_destroy(self.h)
-        except Exception:
-            pass
-
 
 class Grid2D:
     @staticmethod
@@ -230,8 +268,10 @@ class Grid3D:
         return Grid(ctx, xNodes, yNodes, zNodes)
 
 
-class BlockMsrMatrix:
+class BlockMsrMatrix(_Owned):
     """Device block-sparse matrix (bh_bsr)."""
+
+    _destroy_fn = "bh_bsr_destroy"
 
     def __init__(self, ctx, handle=None, nrows=None, bs=None, rowptr=None, col=None, vals=None):
         self.ctx = ctx
@@ -243,7 +283,7 @@ class BlockMsrMatrix:
             _check(lib().bh_bsr_create(ctx.h, nrows, bs, rp.ctypes.data, cl.ctypes.data, vv.ctypes.data,
                                        ctypes.byref(h)))
             handle = h.value
-        self.h = handle
+        self._register(ctx, handle)
         n, bs_, nnzb = _c_i(), _c_i(), _c_i64()
         _check(lib().bh_bsr_info(self.h, ctypes.byref(n), ctypes.byref(bs_), ctypes.byref(nnzb)))
         self.nrows, self.bs, self.nnzb = n.value, bs_.value, nnzb.value
@@ -263,15 +303,11 @@ class BlockMsrMatrix:
         _check(lib().bh_bsr_spmv(self.h, alpha, xp, beta, yp, prec, fx))
         return y
 
-    def __del__(self):
-        try:
-            lib().bh_bsr_destroy(self.h)
-        except Exception:
-            pass
 
-
-class SipOperator:
+class SipOperator(_Owned):
     """DifferentialOperator with one SIPLaplace component + RHS source."""
+
+    _destroy_fn = "bh_op_destroy"
 
     def __init__(self, grid, k, nu, penalty_base=1.3, bc=None):
         self.grid = grid
@@ -280,7 +316,8 @@ class SipOperator:
         bc = np.ascontiguousarray(bc if bc is not None else np.zeros(2 * grid.D), np.int32)
         h = ctypes.c_void_p()
         _check(lib().bh_sip_create(grid.h, k, penalty_base, nu.ctypes.data, bc.ctypes.data, ctypes.byref(h)))
-        self.h = h.value
+        self._register(self.ctx, h.value)
+        self.k = k
         Np, N = _c_i(), _c_i64()
         _check(lib().bh_op_info(self.h, ctypes.byref(Np), ctypes.byref(N)))
         self.Np, self.N = Np.value, N.value
@@ -318,16 +355,12 @@ class SipOperator:
         _check(lib().bh_op_assemble(self.h, ctypes.byref(h), aff.ctypes.data, 0))
         return BlockMsrMatrix(self.ctx, handle=h.value), aff
 
-    def __del__(self):
-        try:
-            lib().bh_op_destroy(self.h)
-        except Exception:
-            pass
 
+class _Solver(_Owned):
+    _destroy_fn = "bh_solver_destroy"
 
-class _Solver:
-    def __init__(self, handle, N, keep=()):
-        self.h = handle
+    def __init__(self, ctx, handle, N, keep=()):
+        self._register(ctx, handle)
         self.N = N
         self._keep = keep
 
@@ -351,18 +384,12 @@ class _Solver:
     def Converged(self):
         return self.stats()["converged"]
 
-    def __del__(self):
-        try:
-            lib().bh_solver_destroy(self.h)
-        except Exception:
-            pass
-
 
 class BlockJacobi(_Solver):
     def __init__(self, M, omega=1.0, NoOfIterations=1, prec=BH_FP64):
         h = ctypes.c_void_p()
         _check(lib().bh_bjacobi_create(M.h, omega, NoOfIterations, prec, ctypes.byref(h)))
-        super().__init__(h.value, M.N, (M,))
+        super().__init__(M.ctx, h.value, M.N, (M,))
 
 
 class SoftGMRES(_Solver):
@@ -371,7 +398,7 @@ class SoftGMRES(_Solver):
         h = ctypes.c_void_p()
         _check(lib().bh_gmres_create(M.h if M else None, op.h if op else None, Precond.h if Precond else None,
                                      MaxKrylovDim, ConvergenceCriterion, MaxSolverIterations, ctypes.byref(h)))
-        super().__init__(h.value, (M or op).N, (M, op, Precond))
+        super().__init__((M or op).ctx, h.value, (M or op).N, (M, op, Precond))
 
 
 class FlexGMRES(_Solver):
@@ -380,4 +407,18 @@ class FlexGMRES(_Solver):
         h = ctypes.c_void_p()
         _check(lib().bh_fgmres_create(M.h if M else None, op.h if op else None, Precond.h if Precond else None,
                                       MaxKrylovDim, ConvergenceCriterion, MaxSolverIterations, ctypes.byref(h)))
-        super().__init__(h.value, (M or op).N, (M, op, Precond))
+        super().__init__((M or op).ctx, h.value, (M or op).N, (M, op, Precond))
+
+
+class PMultigrid(_Solver):
+    """p/h multigrid V-cycle preconditioner (bh_pmg_create); the p-multigrid of
+    AS/PmgConfig.cs on the MultigridOperator hierarchy."""
+
+    def __init__(self, op, M=None, degrees=None, min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2,
+                 prec=BH_FP64, matrix_free=True):
+        degrees = np.ascontiguousarray(degrees if degrees is not None else [op.k, 1], np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().bh_pmg_create(op.h, M.h if M else None, degrees.ctypes.data, len(degrees), min_coarse, omega,
+                                   pre, post, kcycle_inner, prec, 1 if matrix_free else 0, ctypes.byref(h)))
+        super().__init__(op.ctx, h.value, op.N, (op, M))
+

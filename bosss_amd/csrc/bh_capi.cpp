@@ -5,6 +5,7 @@
 
 #include "bh_internal.hpp"
 #include "bh_solvers.hpp"
+#include "bh_mg.hpp"
 
 namespace bh {
 const std::string& last_error();
@@ -564,6 +565,29 @@ int bh_gmres_create(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double tol,
 }
 int bh_fgmres_create(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double tol, int maxit, bh_solver** out) {
     return make_gmres(M, op, P, restart, tol, maxit, true, out);
+}
+
+int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int min_coarse, double omega,
+                  int pre, int post, int kcycle_inner, int prec, int matrix_free, bh_solver** out) {
+    BH_API_BEGIN
+    BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_pmg_create: precision must be FP64 or FP32");
+    BH_CHECK(pre >= 1 && post >= 0, "bh_pmg_create: need pre >= 1, post >= 0");
+    auto mg = std::make_unique<PMultigrid>();
+    mg->prec = prec;
+    mg->omega = omega;
+    mg->pre = pre;
+    mg->post = post;
+    mg->kin = kcycle_inner;
+    mg->kcycle_level = kcycle_inner > 0 ? ndeg - 1 : -1;
+    bh_bsr* Af = fine_mat;
+    if (!Af) {
+        BH_CHECK(bh_op_assemble(op, &Af, nullptr, 0) == 0, "bh_pmg_create: assembly failed: " + bh::last_error());
+        mg->ownedAf = Af;
+    }
+    std::vector<int> deg(degrees, degrees + ndeg);
+    mg->build(op, Af, deg, min_coarse, matrix_free != 0);
+    *out = mg.release();
+    BH_API_END
 }
 
 int bh_solver_solve(bh_solver* s, double* x, const double* b, int flags) {

@@ -169,6 +169,9 @@ struct bh_solver {
     double res0 = 0, res = 0;
     // x, b device pointers (FP64)
     virtual void solve_dev(double* x, const double* b) = 0;
+    // preconditioner use: x <- M^-1 b with a zero initial guess (the
+    // reference clears the output before Precond.Solve, AS/SoftGMRES.cs:456-462)
+    virtual void precondition(double* x, const double* b);
 };
 
 namespace bh {
@@ -189,6 +192,9 @@ void axpby32(bh_ctx* ctx, float a, const float* x, float b, float* y, int64_t n)
 void multi_dot(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* h);
 double dot(bh_ctx* ctx, const double* a, const double* b, int64_t n);
 double nrm2(bh_ctx* ctx, const double* a, int64_t n);
+template <typename T> void multi_dot_t(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* h);
+template <typename T> double nrm2_t(bh_ctx* ctx, const T* a, int64_t n);
+template <typename T> void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
 // w -= sum_k h[k] V_k  (h host array, m <= 128)
 void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
 }  // namespace bh

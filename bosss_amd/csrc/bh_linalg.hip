@@ -299,18 +299,19 @@ void axpby32(bh_ctx* ctx, float a, const float* x, float b, float* y, int64_t n)
 constexpr int MD_BLOCKS = 1024;
 constexpr int MD_M = 16;
 
-template <int M>
-__global__ __launch_bounds__(256) void multidot_k(const double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+// T: vector precision; partial sums are accumulated in double.
+template <typename T, int M>
+__global__ __launch_bounds__(256) void multidot_k(const T* __restrict__ w, const T* __restrict__ V, int64_t ldv,
                                                   int m, int64_t n, double* __restrict__ part) {
     double acc[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) acc[k] = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double wi = w ? w[i] : 0.0;
+        const double wi = w ? (double)w[i] : 0.0;
 #pragma unroll
         for (int k = 0; k < M; ++k)
             if (k < m) {
-                const double v = V[(size_t)k * ldv + i];
+                const double v = (double)V[(size_t)k * ldv + i];
                 acc[k] += (w ? wi : v) * v;   // w == nullptr: squared norms of V_k
             }
     }
@@ -347,62 +348,81 @@ static void ensure_red(bh_ctx* ctx) {
     if (!ctx->host_red) BH_HIP(hipHostMalloc((void**)&ctx->host_red, 256 * sizeof(double), hipHostMallocDefault));
 }
 
-static void multi_dot_dev(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* dout) {
+template <typename T>
+static void multi_dot_dev(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* dout) {
     ensure_red(ctx);
     int64_t nb = (n + 255) / 256;
     if (nb > MD_BLOCKS) nb = MD_BLOCKS;
-    hipLaunchKernelGGL((multidot_k<MD_M>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, w, V, ldv, m, n, ctx->red.as<double>());
+    hipLaunchKernelGGL((multidot_k<T, MD_M>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, w, V, ldv, m, n, ctx->red.as<double>());
     hipLaunchKernelGGL(reduce_parts_k, dim3(m), dim3(256), 0, ctx->stream, ctx->red.as<double>(), (int)nb, MD_M, dout);
     BH_HIP(hipGetLastError());
 }
 
-void multi_dot(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* h) {
+template <typename T>
+void multi_dot_t(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* h) {
     ensure_red(ctx);
     BH_CHECK(m <= 256, "multi_dot: too many vectors");
     for (int k0 = 0; k0 < m; k0 += MD_M) {
         int mm = std::min(MD_M, m - k0);
-        multi_dot_dev(ctx, w, V + (size_t)k0 * ldv, ldv, mm, n, ctx->red_out.as<double>() + k0);
+        multi_dot_dev<T>(ctx, w, V + (size_t)k0 * ldv, ldv, mm, n, ctx->red_out.as<double>() + k0);
     }
     BH_HIP(hipMemcpyAsync(ctx->host_red, ctx->red_out.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < m; ++k) h[k] = ctx->host_red[k];
 }
 
-double dot(bh_ctx* ctx, const double* a, const double* b, int64_t n) {
-    double h;
-    multi_dot(ctx, a, b, 0, 1, n, &h);
-    return h;
-}
-double nrm2(bh_ctx* ctx, const double* a, int64_t n) {
-    double h;
+template <typename T>
+double nrm2_t(bh_ctx* ctx, const T* a, int64_t n) {
     ensure_red(ctx);
-    multi_dot_dev(ctx, nullptr, a, 0, 1, n, ctx->red_out.as<double>());
+    multi_dot_dev<T>(ctx, nullptr, a, 0, 1, n, ctx->red_out.as<double>());
     BH_HIP(hipMemcpyAsync(ctx->host_red, ctx->red_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
-    h = ctx->host_red[0];
-    return std::sqrt(h);
+    return std::sqrt(ctx->host_red[0]);
 }
+
+void multi_dot(bh_ctx* ctx, const double* w, const double* V, int64_t ldv, int m, int64_t n, double* h) {
+    multi_dot_t<double>(ctx, w, V, ldv, m, n, h);
+}
+double dot(bh_ctx* ctx, const double* a, const double* b, int64_t n) {
+    double h;
+    multi_dot_t<double>(ctx, a, b, 0, 1, n, &h);
+    return h;
+}
+double nrm2(bh_ctx* ctx, const double* a, int64_t n) { return nrm2_t<double>(ctx, a, n); }
 
 // w <- w + sign * sum_k h[k] V_k   (coefficients passed by value, m <= 32 per launch)
 struct Coefs { double c[32]; };
-__global__ __launch_bounds__(256) void multi_axpy_k(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+template <typename T>
+__global__ __launch_bounds__(256) void multi_axpy_k(T* __restrict__ w, const T* __restrict__ V, int64_t ldv,
                                                     int m, int64_t n, Coefs h) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double s = w[i];
-        for (int k = 0; k < m; ++k) s += h.c[k] * V[(size_t)k * ldv + i];
-        w[i] = s;
+        double s = (double)w[i];
+        for (int k = 0; k < m; ++k) s += h.c[k] * (double)V[(size_t)k * ldv + i];
+        w[i] = (T)s;
     }
 }
 
-void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign) {
+template <typename T>
+void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* h, double sign) {
     for (int k0 = 0; k0 < m; k0 += 32) {
         Coefs c{};
         int mm = std::min(32, m - k0);
         for (int k = 0; k < mm; ++k) c.c[k] = sign * h[k0 + k];
-        hipLaunchKernelGGL(multi_axpy_k, grid_for(n), dim3(256), 0, ctx->stream, w, V + (size_t)k0 * ldv, ldv, mm, n, c);
+        hipLaunchKernelGGL((multi_axpy_k<T>), grid_for(n), dim3(256), 0, ctx->stream, w, V + (size_t)k0 * ldv, ldv, mm, n, c);
     }
     BH_HIP(hipGetLastError());
 }
+
+void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign) {
+    multi_axpy_t<double>(ctx, w, V, ldv, m, n, h, sign);
+}
+
+template void multi_dot_t<float>(bh_ctx*, const float*, const float*, int64_t, int, int64_t, double*);
+template void multi_dot_t<double>(bh_ctx*, const double*, const double*, int64_t, int, int64_t, double*);
+template double nrm2_t<float>(bh_ctx*, const float*, int64_t);
+template double nrm2_t<double>(bh_ctx*, const double*, int64_t);
+template void multi_axpy_t<float>(bh_ctx*, float*, const float*, int64_t, int, int64_t, const double*, double);
+template void multi_axpy_t<double>(bh_ctx*, double*, const double*, int64_t, int, int64_t, const double*, double);
 
 }  // namespace bh
 

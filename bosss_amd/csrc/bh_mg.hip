@@ -1,0 +1,199 @@
+// Multigrid transfer / smoothing kernels for gfx950 (FP64 and FP32).
+//
+// The hierarchy mirrors the reference's p-multigrid over the orthonormal
+// (hierarchical) modal basis plus aggregation h-coarsening
+// (AS/PmgConfig.cs:80-200, AS/MultigridOperator.cs:573-640,774-840,
+// AS/AggregationGridBasis.cs): restricting to a lower degree keeps the
+// leading modes of every cell (the modes are degree-graded), so R is mode
+// truncation, P = R^T zero-extends, and the Galerkin operator R A P is the
+// leading sub-block of every matrix block.  Aggregation of the last p-level
+// merges 2^D cells; the coarse space holds the same-degree polynomials on the
+// aggregate box (AggregationGridBasis), injected into each fine cell by the
+// L2-projection block P_i (exact: the polynomial spaces nest).
+#include "bh_internal.hpp"
+#include "bh_mg.hpp"
+
+namespace bh {
+
+static dim3 grid1d(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+// valsc[p][j][i] = valsf[p][j][i] for i, j < bsc (column-major blocks)
+__global__ void extract_subblocks_k(int64_t nnzb, int bsf, int bsc, const double* __restrict__ vf, double* __restrict__ vc) {
+    const int64_t tot = nnzb * bsc * bsc;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = t / (bsc * bsc);
+        const int r = (int)(t - p * bsc * bsc);
+        const int j = r / bsc, i = r - j * bsc;
+        vc[t] = vf[p * bsf * bsf + (int64_t)j * bsf + i];
+    }
+}
+
+template <typename T>
+__global__ void p_restrict_k(int64_t J, int bsf, int bsc, const T* __restrict__ rf, T* __restrict__ rc) {
+    const int64_t tot = J * bsc;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / bsc;
+        const int m = (int)(t - j * bsc);
+        rc[t] = rf[j * bsf + m];
+    }
+}
+
+template <typename T>
+__global__ void p_prolong_add_k(int64_t J, int bsf, int bsc, const T* __restrict__ xc, T* __restrict__ xf) {
+    const int64_t tot = J * bsc;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / bsc;
+        const int m = (int)(t - j * bsc);
+        xf[j * bsf + m] += xc[t];
+    }
+}
+
+// Aggregation transfer with per-fine-cell prolongation blocks P_i (bs x bs,
+// row-major [fine mode][coarse mode]):  restriction rc_I = sum_{i in I} P_i^T r_i,
+// prolongation x_i += P_i xc_parent(i).  One thread per (coarse cell, mode) /
+// (fine cell, mode).
+template <typename T>
+__global__ void agg_restrict_k(int Jc, int bs, const int* __restrict__ cptr, const int* __restrict__ cidx,
+                               const T* __restrict__ P, const T* __restrict__ rf, T* __restrict__ rc) {
+    const int64_t tot = (int64_t)Jc * bs;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(t / bs), n = (int)(t - (int64_t)I * bs);
+        T s = 0;
+        for (int p = cptr[I]; p < cptr[I + 1]; ++p) {
+            const int i = cidx[p];
+            const T* Pi = P + (size_t)i * bs * bs;
+            const T* ri = rf + (size_t)i * bs;
+            for (int m = 0; m < bs; ++m) s += Pi[m * bs + n] * ri[m];
+        }
+        rc[t] = s;
+    }
+}
+
+template <typename T>
+__global__ void agg_prolong_add_k(int Jf, int bs, const int* __restrict__ parent, const T* __restrict__ P,
+                                  const T* __restrict__ xc, T* __restrict__ xf) {
+    const int64_t tot = (int64_t)Jf * bs;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / bs), m = (int)(t - (int64_t)i * bs);
+        const T* Pi = P + (size_t)i * bs * bs + (size_t)m * bs;
+        const T* xcI = xc + (size_t)parent[i] * bs;
+        T s = 0;
+        for (int n = 0; n < bs; ++n) s += Pi[n] * xcI[n];
+        xf[t] += s;
+    }
+}
+
+// x (+)= omega * Dinv r, one wave per block row, groups of BS lanes over the
+// columns of the (column-major) inverse block.
+template <typename T, int BS, bool ADD>
+__global__ __launch_bounds__(256) void block_gemv_k(int nrows, const T* __restrict__ Dinv, const T* __restrict__ r,
+                                                    T* __restrict__ x, T omega) {
+    constexpr int G = 64 / BS;
+    __shared__ T red[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    const bool active = row < nrows;
+    const int g = lane / BS, ri = lane - g * BS;
+    T acc = 0;
+    if (active && g < G) {
+        const T* __restrict__ Di = Dinv + (size_t)row * BS * BS;
+        const T* __restrict__ rr = r + (size_t)row * BS;
+#pragma unroll 4
+        for (int c = g; c < BS; c += G) acc += Di[c * BS + ri] * rr[c];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (active && g == 0) {
+        T s = acc;
+#pragma unroll
+        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * BS + ri];
+        const size_t o = (size_t)row * BS + ri;
+        x[o] = ADD ? x[o] + omega * s : omega * s;
+    }
+}
+
+// x = Ainv b (dense, row-major n x n), one wave per row
+template <typename T>
+__global__ __launch_bounds__(256) void dense_gemv_k(int n, const T* __restrict__ A, const T* __restrict__ b, T* __restrict__ x) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    if (row >= n) return;
+    T s = 0;
+    for (int c = lane; c < n; c += 64) s += A[(size_t)row * n + c] * b[c];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) x[row] = s;
+}
+
+template <typename T>
+__global__ void copy_k(const T* __restrict__ s, T* __restrict__ d, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+// ------------------------------------------------------------------ launchers
+void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc) {
+    hipLaunchKernelGGL(extract_subblocks_k, grid1d(nnzb * bsc * bsc), dim3(256), 0, ctx->stream, nnzb, bsf, bsc, vf, vc);
+    BH_HIP(hipGetLastError());
+}
+
+template <typename T>
+void p_restrict(bh_ctx* ctx, int64_t J, int bsf, int bsc, const T* rf, T* rc) {
+    hipLaunchKernelGGL((p_restrict_k<T>), grid1d(J * bsc), dim3(256), 0, ctx->stream, J, bsf, bsc, rf, rc);
+}
+template <typename T>
+void p_prolong_add(bh_ctx* ctx, int64_t J, int bsf, int bsc, const T* xc, T* xf) {
+    hipLaunchKernelGGL((p_prolong_add_k<T>), grid1d(J * bsc), dim3(256), 0, ctx->stream, J, bsf, bsc, xc, xf);
+}
+template <typename T>
+void agg_restrict(bh_ctx* ctx, int Jc, int bs, const int* cptr, const int* cidx, const T* P, const T* rf, T* rc) {
+    hipLaunchKernelGGL((agg_restrict_k<T>), grid1d((int64_t)Jc * bs), dim3(256), 0, ctx->stream, Jc, bs, cptr, cidx, P, rf, rc);
+}
+template <typename T>
+void agg_prolong_add(bh_ctx* ctx, int Jf, int bs, const int* parent, const T* P, const T* xc, T* xf) {
+    hipLaunchKernelGGL((agg_prolong_add_k<T>), grid1d((int64_t)Jf * bs), dim3(256), 0, ctx->stream, Jf, bs, parent, P, xc, xf);
+}
+
+template <typename T>
+void block_gemv(bh_ctx* ctx, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add) {
+    dim3 blk(256), grd((nrows + 3) / 4);
+#define BH_CASE(B)                                                                                                \
+    case B:                                                                                                       \
+        if (add) hipLaunchKernelGGL((block_gemv_k<T, B, true>), grd, blk, 0, ctx->stream, nrows, Dinv, r, x, (T)omega); \
+        else hipLaunchKernelGGL((block_gemv_k<T, B, false>), grd, blk, 0, ctx->stream, nrows, Dinv, r, x, (T)omega);    \
+        break;
+    switch (bs) {
+        BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
+        BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)
+        default: throw Error(-3, "block_gemv: unsupported block size");
+    }
+#undef BH_CASE
+    BH_HIP(hipGetLastError());
+}
+
+template <typename T>
+void dense_gemv(bh_ctx* ctx, int n, const T* A, const T* b, T* x) {
+    hipLaunchKernelGGL((dense_gemv_k<T>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, A, b, x);
+}
+
+template <typename T>
+void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
+    hipLaunchKernelGGL((copy_k<T>), grid1d(n), dim3(256), 0, ctx->stream, s, d, n);
+}
+
+#define BH_INST(T)                                                                                   \
+    template void p_restrict<T>(bh_ctx*, int64_t, int, int, const T*, T*);                           \
+    template void p_prolong_add<T>(bh_ctx*, int64_t, int, int, const T*, T*);                        \
+    template void agg_restrict<T>(bh_ctx*, int, int, const int*, const int*, const T*, const T*, T*); \
+    template void agg_prolong_add<T>(bh_ctx*, int, int, const int*, const T*, const T*, T*);         \
+    template void block_gemv<T>(bh_ctx*, int, int, const T*, const T*, T*, double, bool);             \
+    template void dense_gemv<T>(bh_ctx*, int, const T*, const T*, T*);                               \
+    template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);
+BH_INST(float)
+BH_INST(double)
+#undef BH_INST
+
+}  // namespace bh

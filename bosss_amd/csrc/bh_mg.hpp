@@ -1,0 +1,61 @@
+// p/h multigrid preconditioner of libbosss_hip.so.
+#pragma once
+#include <vector>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+
+void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc);
+template <typename T> void p_restrict(bh_ctx*, int64_t J, int bsf, int bsc, const T* rf, T* rc);
+template <typename T> void p_prolong_add(bh_ctx*, int64_t J, int bsf, int bsc, const T* xc, T* xf);
+template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr, const int* cidx, const T* P, const T* rf, T* rc);
+template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const T* P, const T* xc, T* xf);
+template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
+template <typename T> void dense_gemv(bh_ctx*, int n, const T* A, const T* b, T* x);
+template <typename T> void vcopy(bh_ctx*, const T* s, T* d, int64_t n);
+void bsr_residual(bh_bsr* A, const void* b, const void* x, void* r, int prec);
+
+struct MgLevel {
+    int J = 0, bs = 1;
+    int64_t N = 0;
+    int n[3] = {1, 1, 1};          // logical grid dims
+    int deg = 0;                   // DG degree of the level
+    bh_bsr* A = nullptr;           // level operator (Galerkin)
+    bool ownA = false;
+    bh_op* op = nullptr;           // matrix-free operator (finest level only)
+    DevBuf dinv64, dinv32;
+    // transfer to the next coarser level: 0 = degree truncation, 1 = aggregation
+    int kind = 0;
+    DevBuf cptr, cidx, parent, P64, P32;   // aggregation: children CSR, parents, P_i blocks
+    std::vector<double> lo, h;     // cell boxes (J x D), for the aggregation prolongation
+    // work vectors (precision of the cycle)
+    DevBuf b, x, r;
+    // K-cycle storage (inner flexible Krylov on this level): rhs, V, Z, w
+    DevBuf kb, kv, kz, kw;
+};
+
+struct PMultigrid : bh_solver {
+    int prec = BH_FP64;
+    double omega = 1.0;
+    int pre = 1, post = 1;
+    bool matrix_free = true;
+    int kcycle_level = -1;         // level solved by `kin` inner FGMRES steps (K-cycle), -1: none
+    int kin = 0;
+    std::vector<MgLevel> lv;
+    int ncoarse = 0;
+    DevBuf Ainv64, Ainv32;
+    DevBuf in32, out32;
+    bh_bsr* ownedAf = nullptr;
+    ~PMultigrid() override;
+    void build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bool matrix_free);
+    void solve_dev(double* x, const double* b) override;        // x <- x + V(b - A x)
+    void precondition(double* x, const double* b) override;     // x <- V(b)
+    template <typename T> void vcycle(int l);
+    template <typename T> void kcycle(int l);
+    template <typename T> void solve_level(int l) { if (l == kcycle_level && kin > 0) kcycle<T>(l); else vcycle<T>(l); }
+    template <typename T> void residual(int l, const T* b, const T* x, T* r);
+    template <typename T> T* vec(DevBuf& d) { return (T*)d.p; }
+};
+
+}  // namespace bh

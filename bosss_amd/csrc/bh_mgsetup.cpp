@@ -1,0 +1,465 @@
+// p/h multigrid preconditioner (host orchestration of the bh_mg.hip kernels).
+//
+// Hierarchy (AS/PmgConfig.cs:80-200 p-levels, AS/MultigridOperator.cs:573-640
+// Galerkin setup, AS/AggregationGridBasis.cs + F/CoarseningAlgorithms.cs:555-611
+// aggregation sequence):
+//   level 0         : the fine DG operator, degree k (matrix-free or BSR)
+//   p-levels        : degrees given by the caller (e.g. 3 -> 1); the operator
+//                     is the Galerkin sub-block (leading modes of each cell)
+//   h-levels        : cells of the last p-level are aggregated 2^D at a time
+//                     (Cartesian index halving); the coarse space holds the
+//                     polynomials of the same degree on the aggregate box;
+//                     A_c = P^T A P, until at most `min_coarse` cells remain
+//   coarsest        : dense inverse (applied as a GEMV on the device)
+// Cycle: V(pre, post) with block-Jacobi smoothing x <- x + omega Dinv (b - A x)
+// on every level (AS/BlockJacobi.cs:196-255), in FP64 or FP32.
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "bh_mg.hpp"
+#include "bh_solvers.hpp"
+
+namespace bh {
+
+PMultigrid::~PMultigrid() {
+    for (auto& L : lv)
+        if (L.ownA) delete L.A;
+    delete ownedAf;
+}
+
+static bh_bsr* make_bsr(bh_ctx* ctx, int nrows, int bs, const std::vector<int64_t>& rp, const std::vector<int>& col) {
+    bh_bsr* M = new bh_bsr();
+    M->ctx = ctx;
+    M->nrows = nrows;
+    M->bs = bs;
+    M->h_rowptr = rp;
+    M->h_col = col;
+    M->nnzb = rp[nrows];
+    M->rowptr.alloc((nrows + 1) * sizeof(int64_t));
+    M->col.alloc(std::max<int64_t>(M->nnzb, 1) * sizeof(int));
+    ctx->upload(M->rowptr.p, rp.data(), (nrows + 1) * sizeof(int64_t));
+    ctx->upload(M->col.p, col.data(), M->nnzb * sizeof(int));
+    M->vals.alloc(std::max<int64_t>(M->nnzb, 1) * bs * bs * sizeof(double));
+    return M;
+}
+
+// Gauss-Jordan inverse of a dense row-major matrix (coarsest level)
+static void dense_inverse(std::vector<double>& A, int n) {
+    std::vector<double> I(n * (size_t)n, 0.0);
+    for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r) if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)p * n + c])) p = r;
+        BH_CHECK(A[(size_t)p * n + c] != 0.0, "multigrid: singular coarsest-level operator");
+        if (p != c)
+            for (int k = 0; k < n; ++k) { std::swap(A[(size_t)p * n + k], A[(size_t)c * n + k]); std::swap(I[(size_t)p * n + k], I[(size_t)c * n + k]); }
+        const double piv = 1.0 / A[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) { A[(size_t)c * n + k] *= piv; I[(size_t)c * n + k] *= piv; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) { A[(size_t)r * n + k] -= f * A[(size_t)c * n + k]; I[(size_t)r * n + k] -= f * I[(size_t)c * n + k]; }
+        }
+    }
+    A.swap(I);
+}
+
+static int cell_of(int D, const int* n, int i, int j, int k) {
+    return D == 2 ? i * n[1] + j : i + j * n[0] + k * n[0] * n[1];
+}
+static void ijk_of(int D, const int* n, int c, int* idx) {
+    if (D == 2) { idx[0] = c / n[1]; idx[1] = c % n[1]; idx[2] = 0; }
+    else { idx[0] = c % n[0]; idx[1] = (c / n[0]) % n[1]; idx[2] = c / (n[0] * n[1]); }
+}
+
+// L2-projection block of the coarse-box basis onto a fine cell:
+// P[m][n] = int_f phi^f_m phi^c_n = sqrt(detJ_f/detJ_c) sum_q w_q psi_m(xi_q) psi_n(xi_c(x_q))
+static void projection_block(int D, int deg, const double* lof, const double* hf, const double* loc, const double* hc,
+                             double* P) {
+    const int Np = basis_count(D, deg), n1 = deg + 1;
+    std::vector<double> x(n1), w(n1), pf(Np), pc(Np);
+    gauss_rule(n1, x.data(), w.data());
+    double dJf = 1, dJc = 1;
+    for (int d = 0; d < D; ++d) { dJf *= 0.5 * hf[d]; dJc *= 0.5 * hc[d]; }
+    const double sc = std::sqrt(dJf / dJc);
+    std::fill(P, P + Np * Np, 0.0);
+    int nq = 1;
+    for (int d = 0; d < D; ++d) nq *= n1;
+    for (int q = 0; q < nq; ++q) {
+        double xi[3], xc[3], wq = 1;
+        int r = q;
+        for (int d = 0; d < D; ++d) {
+            const int id = r % n1;
+            r /= n1;
+            xi[d] = x[id];
+            wq *= w[id];
+            const double X = lof[d] + 0.5 * (xi[d] + 1.0) * hf[d];
+            xc[d] = 2.0 * (X - loc[d]) / hc[d] - 1.0;
+        }
+        basis_eval(D, deg, xi, pf.data(), nullptr);
+        basis_eval(D, deg, xc, pc.data(), nullptr);
+        for (int m = 0; m < Np; ++m)
+            for (int n = 0; n < Np; ++n) P[m * Np + n] += sc * wq * pf[m] * pc[n];
+    }
+}
+
+void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bool mf) {
+    ctx = op->ctx;
+    N = op->N;
+    matrix_free = mf;
+    const int D = op->D;
+    BH_CHECK(!degrees.empty() && degrees[0] == op->k, "multigrid: first degree must be the operator's degree");
+    for (size_t i = 1; i < degrees.size(); ++i)
+        BH_CHECK(degrees[i] < degrees[i - 1] && degrees[i] >= 0, "multigrid: degrees must decrease");
+    BH_CHECK(Af->bs == op->Np && Af->nrows == op->grid->J, "multigrid: matrix does not match the operator");
+    const int J = op->grid->J;
+    // p-levels
+    for (size_t i = 0; i < degrees.size(); ++i) {
+        MgLevel L;
+        L.J = J;
+        L.deg = degrees[i];
+        L.bs = basis_count(D, degrees[i]);
+        L.N = (int64_t)J * L.bs;
+        for (int d = 0; d < D; ++d) L.n[d] = op->grid->n[d];
+        if (i == 0) {
+            L.A = Af;
+            L.ownA = false;
+            L.op = mf ? op : nullptr;
+        } else {
+            L.A = make_bsr(ctx, J, L.bs, Af->h_rowptr, Af->h_col);
+            L.ownA = true;
+            extract_subblocks(ctx, Af->nnzb, Af->bs, L.bs, Af->vals.as<double>(), L.A->vals.as<double>());
+        }
+        L.kind = 0;
+        lv.push_back(std::move(L));
+    }
+    // h-levels: aggregation of the last p-level
+    {
+        MgLevel& L0 = lv.back();
+        L0.lo.resize((size_t)J * D);
+        L0.h.resize((size_t)J * D);
+        for (int c = 0; c < J; ++c) {
+            int idx[3];
+            op->grid->cell_ijk(c, idx);
+            for (int d = 0; d < D; ++d) {
+                L0.lo[(size_t)c * D + d] = op->grid->nodes[d][idx[d]];
+                L0.h[(size_t)c * D + d] = op->grid->width[d][idx[d]];
+            }
+        }
+    }
+    while (lv.back().J > min_coarse) {
+        MgLevel& F = lv.back();
+        int nc[3] = {1, 1, 1};
+        bool can = false;
+        for (int d = 0; d < D; ++d) { nc[d] = (F.n[d] + 1) / 2; can |= F.n[d] > 1; }
+        if (!can) break;
+        const int bs = F.bs, deg = F.deg;
+        const int Jc = nc[0] * nc[1] * (D == 3 ? nc[2] : 1);
+        std::vector<int> parent(F.J);
+        std::vector<double> loc((size_t)Jc * D, 1e300), hic((size_t)Jc * D, -1e300);
+        for (int c = 0; c < F.J; ++c) {
+            int idx[3];
+            ijk_of(D, F.n, c, idx);
+            const int I = cell_of(D, nc, idx[0] / 2, idx[1] / 2, idx[2] / 2);
+            parent[c] = I;
+            for (int d = 0; d < D; ++d) {
+                loc[(size_t)I * D + d] = std::min(loc[(size_t)I * D + d], F.lo[(size_t)c * D + d]);
+                hic[(size_t)I * D + d] = std::max(hic[(size_t)I * D + d], F.lo[(size_t)c * D + d] + F.h[(size_t)c * D + d]);
+            }
+        }
+        std::vector<double> hc((size_t)Jc * D);
+        for (size_t t = 0; t < hc.size(); ++t) hc[t] = hic[t] - loc[t];
+        // prolongation blocks
+        // (the block depends only on the fine box relative to the coarse box:
+        // cache by that relative geometry, quantised far below round-off of the setup)
+        std::vector<double> P((size_t)F.J * bs * bs);
+        std::map<std::vector<long long>, int> seen;
+        for (int c = 0; c < F.J; ++c) {
+            const double* lf = &F.lo[(size_t)c * D];
+            const double* hf = &F.h[(size_t)c * D];
+            const double* lc = &loc[(size_t)parent[c] * D];
+            const double* hcc = &hc[(size_t)parent[c] * D];
+            std::vector<long long> key(2 * D);
+            for (int d = 0; d < D; ++d) {
+                key[2 * d] = std::llround((lf[d] - lc[d]) / hcc[d] * 1e13);
+                key[2 * d + 1] = std::llround(hf[d] / hcc[d] * 1e13);
+            }
+            auto it = seen.find(key);
+            if (it != seen.end()) {
+                std::copy(&P[(size_t)it->second * bs * bs], &P[(size_t)(it->second + 1) * bs * bs], &P[(size_t)c * bs * bs]);
+            } else {
+                projection_block(D, deg, lf, hf, lc, hcc, &P[(size_t)c * bs * bs]);
+                seen.emplace(std::move(key), c);
+            }
+        }
+        std::vector<int> cptr(Jc + 1, 0), cidx(F.J);
+        for (int c = 0; c < F.J; ++c) cptr[parent[c] + 1]++;
+        for (int I = 0; I < Jc; ++I) cptr[I + 1] += cptr[I];
+        {
+            std::vector<int> fillp(cptr.begin(), cptr.end() - 1);
+            for (int c = 0; c < F.J; ++c) cidx[fillp[parent[c]]++] = c;
+        }
+        // Galerkin A_c = P^T A P (blocks: Ac[I][J] += P_i^T A_ij P_j), row-major blocks on the host
+        std::vector<double> fv((size_t)F.A->nnzb * bs * bs);
+        ctx->download(fv.data(), F.A->vals.p, fv.size() * sizeof(double));   // column-major blocks
+        std::vector<std::map<int, std::vector<double>>> rows(Jc);
+        std::vector<double> t1(bs * bs), t2(bs * bs);
+        for (int i = 0; i < F.J; ++i) {
+            const double* Pi = &P[(size_t)i * bs * bs];
+            for (int64_t p = F.A->h_rowptr[i]; p < F.A->h_rowptr[i + 1]; ++p) {
+                const int j = F.A->h_col[p];
+                const double* Aij = &fv[(size_t)p * bs * bs];   // [col][row]
+                const double* Pj = &P[(size_t)j * bs * bs];
+                // t1 = A_ij P_j  (row-major)
+                for (int r = 0; r < bs; ++r)
+                    for (int n = 0; n < bs; ++n) {
+                        double s = 0;
+                        for (int c = 0; c < bs; ++c) s += Aij[(size_t)c * bs + r] * Pj[c * bs + n];
+                        t1[r * bs + n] = s;
+                    }
+                // t2 = P_i^T t1
+                for (int m = 0; m < bs; ++m)
+                    for (int n = 0; n < bs; ++n) {
+                        double s = 0;
+                        for (int r = 0; r < bs; ++r) s += Pi[r * bs + m] * t1[r * bs + n];
+                        t2[m * bs + n] = s;
+                    }
+                auto& blk = rows[parent[i]][parent[j]];
+                if (blk.empty()) blk.assign(bs * bs, 0.0);
+                for (int t = 0; t < bs * bs; ++t) blk[t] += t2[t];
+            }
+        }
+        std::vector<int64_t> rp(Jc + 1, 0);
+        std::vector<int> col;
+        std::vector<double> val;
+        for (int I = 0; I < Jc; ++I) {
+            for (auto& kv : rows[I]) {
+                col.push_back(kv.first);
+                for (int c = 0; c < bs; ++c)          // to column-major
+                    for (int r = 0; r < bs; ++r) val.push_back(kv.second[r * bs + c]);
+            }
+            rp[I + 1] = (int64_t)col.size();
+        }
+        F.kind = 1;
+        F.cptr.alloc(cptr.size() * sizeof(int));
+        F.cidx.alloc(cidx.size() * sizeof(int));
+        F.parent.alloc(parent.size() * sizeof(int));
+        F.P64.alloc(P.size() * sizeof(double));
+        F.P32.alloc(P.size() * sizeof(float));
+        ctx->upload(F.cptr.p, cptr.data(), cptr.size() * sizeof(int));
+        ctx->upload(F.cidx.p, cidx.data(), cidx.size() * sizeof(int));
+        ctx->upload(F.parent.p, parent.data(), parent.size() * sizeof(int));
+        ctx->upload(F.P64.p, P.data(), P.size() * sizeof(double));
+        convert_f64_f32(ctx, F.P64.as<double>(), F.P32.as<float>(), (int64_t)P.size());
+        MgLevel C;
+        C.J = Jc;
+        C.deg = deg;
+        C.bs = bs;
+        C.N = (int64_t)Jc * bs;
+        for (int d = 0; d < 3; ++d) C.n[d] = nc[d];
+        C.lo = loc;
+        C.h = hc;
+        C.A = make_bsr(ctx, Jc, bs, rp, col);
+        C.ownA = true;
+        ctx->upload(C.A->vals.p, val.data(), val.size() * sizeof(double));
+        lv.push_back(std::move(C));
+    }
+    // smoother inverses and work vectors
+    const bool f32 = prec == BH_FP32;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    for (size_t l = 0; l + 1 < lv.size(); ++l) {
+        MgLevel& L = lv[l];
+        bsr_invert_diag(L.A, L.dinv64, f32 ? &L.dinv32 : nullptr);
+        if (f32) { L.dinv64.release(); if (!(l == 0 && L.op)) L.A->ensure_fp32(); }
+    }
+    for (auto& L : lv) {
+        L.b.alloc(L.N * es);
+        L.x.alloc(L.N * es);
+        L.r.alloc(L.N * es);
+    }
+    if (kcycle_level >= (int)lv.size() - 1) kcycle_level = -1;   // the coarsest level is solved directly
+    if (kcycle_level > 0 && kin > 0) {
+        MgLevel& K = lv[kcycle_level];
+        K.kb.alloc(K.N * es);
+        K.kv.alloc((size_t)(kin + 1) * K.N * es);
+        K.kz.alloc((size_t)kin * K.N * es);
+        K.kw.alloc(K.N * es);
+    }
+    // coarsest: dense inverse
+    MgLevel& Cc = lv.back();
+    ncoarse = (int)Cc.N;
+    BH_CHECK(ncoarse <= 4096, "multigrid: coarsest level too large for the dense solve");
+    std::vector<double> dense((size_t)ncoarse * ncoarse, 0.0);
+    {
+        const int bs = Cc.bs;
+        std::vector<double> v((size_t)Cc.A->nnzb * bs * bs);
+        ctx->download(v.data(), Cc.A->vals.p, v.size() * sizeof(double));
+        for (int r = 0; r < Cc.J; ++r)
+            for (int64_t p = Cc.A->h_rowptr[r]; p < Cc.A->h_rowptr[r + 1]; ++p) {
+                const int c = Cc.A->h_col[p];
+                for (int i = 0; i < bs; ++i)
+                    for (int j = 0; j < bs; ++j)
+                        dense[(size_t)(r * bs + i) * ncoarse + c * bs + j] = v[(size_t)p * bs * bs + (size_t)j * bs + i];
+            }
+    }
+    dense_inverse(dense, ncoarse);
+    Ainv64.alloc(dense.size() * sizeof(double));
+    ctx->upload(Ainv64.p, dense.data(), dense.size() * sizeof(double));
+    if (f32) {
+        Ainv32.alloc(dense.size() * sizeof(float));
+        convert_f64_f32(ctx, Ainv64.as<double>(), Ainv32.as<float>(), (int64_t)dense.size());
+    }
+    ctx->sync();
+}
+
+template <typename T>
+void PMultigrid::residual(int l, const T* b, const T* x, T* r) {
+    MgLevel& L = lv[l];
+    const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
+    if (L.op) {
+        vcopy<T>(ctx, b, r, L.N);
+        sip_apply(L.op, -1.0, x, 1.0, r, false, p);
+    } else {
+        bsr_residual(L.A, b, x, r, p);
+    }
+}
+
+template <typename T>
+void PMultigrid::vcycle(int l) {
+    MgLevel& L = lv[l];
+    T* b = vec<T>(L.b);
+    T* x = vec<T>(L.x);
+    T* r = vec<T>(L.r);
+    const bool f32 = sizeof(T) == 4;
+    if (l + 1 == (int)lv.size()) {
+        dense_gemv<T>(ctx, ncoarse, (const T*)(f32 ? Ainv32.p : Ainv64.p), b, x);
+        return;
+    }
+    const T* Dinv = (const T*)(f32 ? L.dinv32.p : L.dinv64.p);
+    // pre-smoothing from a zero guess: the first sweep is x = omega Dinv b
+    block_gemv<T>(ctx, L.J, L.bs, Dinv, b, x, omega, false);
+    for (int s = 1; s < pre; ++s) {
+        residual<T>(l, b, x, r);
+        block_gemv<T>(ctx, L.J, L.bs, Dinv, r, x, omega, true);
+    }
+    residual<T>(l, b, x, r);
+    MgLevel& C = lv[l + 1];
+    const T* P = (const T*)(f32 ? L.P32.p : L.P64.p);
+    if (L.kind == 0) p_restrict<T>(ctx, L.J, L.bs, C.bs, r, vec<T>(C.b));
+    else agg_restrict<T>(ctx, C.J, L.bs, L.cptr.as<int>(), L.cidx.as<int>(), P, r, vec<T>(C.b));
+    solve_level<T>(l + 1);
+    if (L.kind == 0) p_prolong_add<T>(ctx, L.J, L.bs, C.bs, vec<T>(C.x), x);
+    else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), P, vec<T>(C.x), x);
+    for (int s = 0; s < post; ++s) {
+        residual<T>(l, b, x, r);
+        block_gemv<T>(ctx, L.J, L.bs, Dinv, r, x, omega, true);
+    }
+}
+
+template <typename T>
+static void axpby_t(bh_ctx* ctx, double a, const T* x, double b, T* y, int64_t n) {
+    if (sizeof(T) == 4) axpby32(ctx, (float)a, (const float*)x, (float)b, (float*)y, n);
+    else axpby(ctx, a, (const double*)x, b, (double*)y, n);
+}
+
+// K-cycle on level l: `kin` steps of flexible GMRES on A_l x = b_l from a zero
+// guess, each preconditioned by the V-cycle from level l -- the Krylov-
+// accelerated coarse solve of the reference's K-cycle / OrthonormalizationMultigrid
+// configurations (AS/OrthonormalizationMultigrid.cs:797-1100, AS/PmgConfig.cs:80-200),
+// standing in for the direct low-order solve of the p-two-grid (AS/LevelPmg.cs:411).
+template <typename T>
+void PMultigrid::kcycle(int l) {
+    MgLevel& L = lv[l];
+    const int64_t n = L.N;
+    const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
+    T* kb = vec<T>(L.kb);
+    T* V = vec<T>(L.kv);
+    T* Z = vec<T>(L.kz);
+    T* w = vec<T>(L.kw);
+    vcopy<T>(ctx, vec<T>(L.b), kb, n);
+    const double beta = nrm2_t<T>(ctx, kb, n);
+    if (!(beta > 0) || !std::isfinite(beta)) {
+        axpby_t<T>(ctx, 0.0, kb, 0.0, vec<T>(L.x), n);
+        return;
+    }
+    axpby_t<T>(ctx, 1.0 / beta, kb, 0.0, V, n);
+    std::vector<double> H((size_t)(kin + 1) * kin, 0.0), h1(kin + 1), h2(kin + 1);
+    int used = 0;
+    for (int j = 0; j < kin; ++j) {
+        vcopy<T>(ctx, V + (size_t)j * n, vec<T>(L.b), n);
+        vcycle<T>(l);
+        vcopy<T>(ctx, vec<T>(L.x), Z + (size_t)j * n, n);
+        bsr_spmv(L.A, 1.0, Z + (size_t)j * n, 0.0, w, p);
+        multi_dot_t<T>(ctx, w, V, n, j + 1, n, h1.data());
+        multi_axpy_t<T>(ctx, w, V, n, j + 1, n, h1.data(), -1.0);
+        multi_dot_t<T>(ctx, w, V, n, j + 1, n, h2.data());
+        multi_axpy_t<T>(ctx, w, V, n, j + 1, n, h2.data(), -1.0);
+        for (int i = 0; i <= j; ++i) H[(size_t)i * kin + j] = h1[i] + h2[i];
+        const double nw = nrm2_t<T>(ctx, w, n);
+        H[(size_t)(j + 1) * kin + j] = nw;
+        used = j + 1;
+        if (!(nw > 1e-300)) break;
+        if (j + 1 < kin) axpby_t<T>(ctx, 1.0 / nw, w, 0.0, V + (size_t)(j + 1) * n, n);
+    }
+    // least squares min || beta e1 - H y || by Givens rotations (host, tiny)
+    std::vector<double> g(used + 1, 0.0), y(used, 0.0);
+    g[0] = beta;
+    std::vector<double> R(H);
+    for (int j = 0; j < used; ++j) {
+        const double a = R[(size_t)j * kin + j], b = R[(size_t)(j + 1) * kin + j];
+        const double rr = std::hypot(a, b);
+        const double c = rr > 0 ? a / rr : 1.0, s = rr > 0 ? b / rr : 0.0;
+        for (int k = j; k < used; ++k) {
+            const double t1 = R[(size_t)j * kin + k], t2 = R[(size_t)(j + 1) * kin + k];
+            R[(size_t)j * kin + k] = c * t1 + s * t2;
+            R[(size_t)(j + 1) * kin + k] = -s * t1 + c * t2;
+        }
+        const double t1 = g[j], t2 = g[j + 1];
+        g[j] = c * t1 + s * t2;
+        g[j + 1] = -s * t1 + c * t2;
+    }
+    for (int r = used - 1; r >= 0; --r) {
+        double a = g[r];
+        for (int c = r + 1; c < used; ++c) a -= R[(size_t)r * kin + c] * y[c];
+        y[r] = a / R[(size_t)r * kin + r];
+    }
+    axpby_t<T>(ctx, 0.0, kb, 0.0, vec<T>(L.x), n);
+    multi_axpy_t<T>(ctx, vec<T>(L.x), Z, n, used, n, y.data(), 1.0);
+}
+
+void PMultigrid::precondition(double* x, const double* b) {
+    MgLevel& L = lv[0];
+    if (prec == BH_FP32) {
+        convert_f64_f32(ctx, b, vec<float>(L.b), N);
+        vcycle<float>(0);
+        convert_f32_f64(ctx, vec<float>(L.x), x, N);
+    } else {
+        copy(ctx, vec<double>(L.b), b, N);
+        vcycle<double>(0);
+        copy(ctx, x, vec<double>(L.x), N);
+    }
+    iterations++;
+}
+
+void PMultigrid::solve_dev(double* x, const double* b) {
+    // x <- x + V(b - A x) in FP64
+    DevBuf r(N * sizeof(double)), c(N * sizeof(double));
+    if (lv[0].op) {
+        copy(ctx, r.as<double>(), b, N);
+        sip_apply(lv[0].op, -1.0, x, 1.0, r.p, false, BH_FP64);
+    } else {
+        bsr_residual(lv[0].A, b, x, r.p, BH_FP64);
+    }
+    precondition(c.as<double>(), r.as<double>());
+    axpby(ctx, 1.0, c.as<double>(), 1.0, x, N);
+    converged = 1;
+}
+
+template void PMultigrid::vcycle<float>(int);
+template void PMultigrid::vcycle<double>(int);
+template void PMultigrid::kcycle<float>(int);
+template void PMultigrid::kcycle<double>(int);
+
+}  // namespace bh

@@ -12,6 +12,11 @@ namespace bh {
 
 void bsr_residual(bh_bsr* A, const void* b, const void* x, void* r, int prec);
 
+void bh_solver_precondition_default(bh_solver* s, double* x, const double* b) {
+    fill(s->ctx, x, 0.0, s->N);
+    s->solve_dev(x, b);
+}
+
 // ------------------------------------------------------------ LinOp
 void LinOp::apply(const double* x, double* y) const {
     if (M) bsr_spmv(M, 1.0, x, 0.0, y, BH_FP64);
@@ -132,8 +137,7 @@ GmresSolver::GmresSolver(LinOp A_, bh_solver* P_, int m_, double tol, int maxit,
 
 void GmresSolver::precond(const double* in, double* out) {
     if (P) {
-        fill(ctx, out, 0.0, N);
-        P->solve_dev(out, in);
+        P->precondition(out, in);
     } else {
         copy(ctx, out, in, N);
     }
@@ -269,3 +273,5 @@ void GmresSolver::solve_flexible(double* X, const double* B) {
 }
 
 }  // namespace bh
+
+void bh_solver::precondition(double* x, const double* b) { bh::bh_solver_precondition_default(this, x, b); }

@@ -134,6 +134,25 @@ int bh_gmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, dou
  * judged on the true FP64 residual ||b - A x|| / ||b|| <= tol. */
 int bh_fgmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, double tol, int maxit,
                      bh_solver** solver);
+/* p/h-multigrid preconditioner (one V-cycle per Solve), the MI355X form of
+ * the reference's p-multigrid (AS/PmgConfig.cs:80-200, AS/LevelPmg.cs) on the
+ * MultigridOperator hierarchy (AS/MultigridOperator.cs:473-840) with
+ * aggregation h-levels (AS/AggregationGridBasis.cs, F/CoarseningAlgorithms.cs:555-611):
+ *   degrees[0..ndeg-1]: DG degree per p-level, degrees[0] = op's degree, strictly
+ *     decreasing; the level operators are the Galerkin sub-blocks (leading modes);
+ *   if the last degree is 0, the cells are then aggregated 2^D at a time
+ *     (Galerkin P^T A P) until at most min_coarse cells remain; the coarsest
+ *     level is solved with a dense inverse;
+ *   smoothing: block-Jacobi (AS/BlockJacobi.cs), `pre`/`post` sweeps, omega;
+ *   kcycle_inner > 0: the last p-level is solved by that many flexible GMRES
+ *     steps preconditioned by its own V-cycle (K-cycle, the Krylov-accelerated
+ *     coarse solve of AS/OrthonormalizationMultigrid.cs / LevelPmg's low-order
+ *     solve, AS/LevelPmg.cs:411); 0: plain V-cycle;
+ *   prec BH_FP64 or BH_FP32 (whole cycle in that precision);
+ *   matrix_free != 0: the finest level's residuals use the matrix-free operator.
+ * fine_mat: the assembled operator of `op` (NULL: assembled internally). */
+int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int min_coarse, double omega,
+                  int pre, int post, int kcycle_inner, int prec, int matrix_free, bh_solver** solver);
 int bh_solver_solve(bh_solver* solver, double* x, const double* b, int flags);
 int bh_solver_stats(bh_solver* solver, int* iterations, int* converged, double* res0, double* res);
 int bh_solver_destroy(bh_solver* solver);

@@ -65,6 +65,13 @@ def lib():
             "orc_bjacobi_solve": (_i, [_p, _d, _i, _dp, _dp]),
             "orc_soft_gmres": (_i, [_p, _i, _d, _i, _i, _d, _i, _dp, _dp,
                                     ctypes.POINTER(_i), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+            "orc_pmg_vcycle": (_i, [_p, _p, _ip, _i, _i, _d, _i, _i, _dp, _dp, ctypes.POINTER(_i)]),
+            "orc_pmg_create": (_p, [_p, _p, _ip, _i, _i, _d, _i, _i]),
+            "orc_pmg_free": (None, [_p]),
+            "orc_pmg_levels": (_i, [_p, _p]),
+            "orc_pmg_vcycle_from": (None, [_p, _i, _dp, _dp]),
+            "orc_pmg_level_spmv": (None, [_p, _i, _dp, _dp]),
+            "orc_pmg_set": (None, [_p, _d, _i, _i]),
             "orc_num_threads": (_i, []),
             "orc_set_threads": (None, [_i]),
         }
@@ -180,6 +187,16 @@ class Bsr:
             raise ArithmeticException_("block-Jacobi init failed")
         return x
 
+    def pmg_vcycle(self, grid, b, degrees, min_coarse=64, omega=1.0, pre=1, post=1):
+        """one p/h multigrid V-cycle from a zero guess (see oracle.cpp pmg_build)."""
+        x = np.zeros(self.nrows * self.bs)
+        nl = _i()
+        rc = lib().orc_pmg_vcycle(self.h, grid.h, np.ascontiguousarray(degrees, np.int32), len(degrees), min_coarse,
+                                  omega, pre, post, np.ascontiguousarray(b, np.float64), x, ctypes.byref(nl))
+        if rc != 0:
+            raise ArithmeticException_("multigrid setup failed")
+        return x, nl.value
+
     def gmres(self, b, x=None, precond=1, omega=1.0, bj_iters=1, restart=30, tol=1e-10, maxit=10000):
         if x is None:
             x = np.zeros(self.nrows * self.bs)
@@ -260,5 +277,38 @@ class Sip:
     def __del__(self):
         try:
             lib().orc_sip_free(self.h)
+        except Exception:
+            pass
+
+
+class Pmg:
+    """Persistent oracle p/h multigrid hierarchy (see oracle.cpp pmg_build)."""
+
+    def __init__(self, A, grid, degrees, min_coarse=64, omega=1.0, pre=1, post=1):
+        self.h = lib().orc_pmg_create(A.h, grid.h, np.ascontiguousarray(degrees, np.int32), len(degrees), min_coarse,
+                                      omega, pre, post)
+        if not self.h:
+            raise ArithmeticException_("multigrid setup failed")
+        n = lib().orc_pmg_levels(self.h, None)
+        sz = np.zeros(n, np.int32)
+        lib().orc_pmg_levels(self.h, sz.ctypes.data)
+        self.sizes = sz
+
+    def vcycle(self, b, level=0):
+        x = np.zeros(self.sizes[level])
+        lib().orc_pmg_vcycle_from(self.h, level, np.ascontiguousarray(b, np.float64), x)
+        return x
+
+    def spmv(self, level, x):
+        y = np.zeros(self.sizes[level])
+        lib().orc_pmg_level_spmv(self.h, level, np.ascontiguousarray(x, np.float64), y)
+        return y
+
+    def set(self, omega, pre, post):
+        lib().orc_pmg_set(self.h, omega, pre, post)
+
+    def __del__(self):
+        try:
+            lib().orc_pmg_free(self.h)
         except Exception:
             pass

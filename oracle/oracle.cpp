@@ -787,6 +787,221 @@ GmresStats soft_gmres(const Bsr& A, const Precond& P, double* X, const double* B
     return st;
 }
 
+
+// ============================================================ multigrid ==
+// p/h multigrid restated from the reference's p-multigrid over the
+// hierarchical orthonormal basis (AS/PmgConfig.cs:80-200, AS/LevelPmg.cs) and
+// aggregation coarsening (AS/AggregationGridBasis.cs,
+// F/CoarseningAlgorithms.cs:555-611) with Galerkin operators
+// (AS/MultigridOperator.cs:573-640): lower degree = leading modes of each
+// cell (R = truncation, P = R^T); the last p-level is aggregated 2^D cells at
+// a time, the coarse space being the same-degree polynomials on the aggregate
+// box, injected by the L2 projection P_i; A_c = P^T A P; block-Jacobi
+// smoothing (AS/BlockJacobi.cs:196-255); dense solve on the coarsest level.
+struct MgLev {
+    Bsr A;
+    std::vector<double> Dinv;
+    int n[3] = {1, 1, 1};
+    int deg = 0;
+    std::vector<double> lo, h;  // cell boxes J x D
+    int kind = 0;               // transfer to next: 0 truncation, 1 aggregation
+    std::vector<int> parent;
+    std::vector<double> P;      // J x bs x bs, [fine mode][coarse mode]
+};
+
+struct Pmg {
+    std::vector<MgLev> lv;
+    std::vector<double> Ainv;   // coarsest dense inverse (row-major)
+    int nc = 0;
+    double omega = 1.0;
+    int pre = 1, post = 1;
+};
+
+// P[m][n] = int_f phi^f_m phi^c_n dx, evaluated with the fine cell's Gauss rule
+static void agg_projection(int D, int deg, const double* lof, const double* hf, const double* loc, const double* hc, double* P) {
+    int Np = basis_count(D, deg), n1 = deg + 1;
+    std::vector<double> X, W;
+    tensor_rule(D, n1, X, W);
+    double dJf = 1, dJc = 1;
+    for (int d = 0; d < D; ++d) { dJf *= 0.5 * hf[d]; dJc *= 0.5 * hc[d]; }
+    std::vector<double> pf(Np), pc(Np);
+    std::fill(P, P + Np * Np, 0.0);
+    for (size_t q = 0; q < W.size(); ++q) {
+        double xc[3];
+        for (int d = 0; d < D; ++d) {
+            double x = lof[d] + 0.5 * (X[q * D + d] + 1.0) * hf[d];
+            xc[d] = 2.0 * (x - loc[d]) / hc[d] - 1.0;
+        }
+        basis_eval(D, deg, &X[q * D], pf.data(), nullptr);
+        basis_eval(D, deg, xc, pc.data(), nullptr);
+        for (int m = 0; m < Np; ++m)
+            for (int n = 0; n < Np; ++n) P[m * Np + n] += W[q] * pf[m] * pc[n] * std::sqrt(dJf / dJc);
+    }
+}
+
+Pmg* pmg_build(const Bsr& Af, const Grid& g, const int* deg, int ndeg, int min_coarse, double omega, int pre, int post) {
+    Pmg* P = new Pmg();
+    P->omega = omega; P->pre = pre; P->post = post;
+    int D = g.D;
+    for (int l = 0; l < ndeg; ++l) {
+        MgLev L;
+        int bs = basis_count(D, deg[l]);
+        L.deg = deg[l];
+        L.A.nrows = Af.nrows; L.A.bs = bs; L.A.rowptr = Af.rowptr; L.A.col = Af.col;
+        L.A.val.resize((size_t)Af.rowptr[Af.nrows] * bs * bs);
+        for (int64_t p = 0; p < Af.rowptr[Af.nrows]; ++p)
+            for (int i = 0; i < bs; ++i)
+                for (int j = 0; j < bs; ++j)
+                    L.A.val[(size_t)p * bs * bs + i * bs + j] = Af.val[(size_t)p * Af.bs * Af.bs + i * Af.bs + j];
+        for (int d = 0; d < D; ++d) L.n[d] = g.n[d];
+        P->lv.push_back(std::move(L));
+    }
+    {
+        MgLev& L0 = P->lv.back();
+        L0.lo = g.lo;
+        L0.h = g.h;
+    }
+    auto idx_of = [&](const int* n, int c, int* idx) {
+        if (D == 2) { idx[0] = c / n[1]; idx[1] = c % n[1]; idx[2] = 0; }
+        else { idx[0] = c % n[0]; idx[1] = (c / n[0]) % n[1]; idx[2] = c / (n[0] * n[1]); }
+    };
+    auto num = [&](const int* n, int i, int j, int k) { return D == 2 ? i * n[1] + j : i + j * n[0] + k * n[0] * n[1]; };
+    while (P->lv.back().A.nrows > min_coarse) {
+        MgLev& F = P->lv.back();
+        int nc[3] = {1, 1, 1};
+        bool can = false;
+        for (int d = 0; d < D; ++d) { nc[d] = (F.n[d] + 1) / 2; can |= F.n[d] > 1; }
+        if (!can) break;
+        int Jf = F.A.nrows, Jc = nc[0] * nc[1] * (D == 3 ? nc[2] : 1), bs = F.A.bs;
+        F.parent.resize(Jf);
+        std::vector<double> loc((size_t)Jc * D, 1e300), hic((size_t)Jc * D, -1e300), hc((size_t)Jc * D);
+        for (int c = 0; c < Jf; ++c) {
+            int idx[3];
+            idx_of(F.n, c, idx);
+            int I = num(nc, idx[0] / 2, idx[1] / 2, idx[2] / 2);
+            F.parent[c] = I;
+            for (int d = 0; d < D; ++d) {
+                loc[(size_t)I * D + d] = std::min(loc[(size_t)I * D + d], F.lo[(size_t)c * D + d]);
+                hic[(size_t)I * D + d] = std::max(hic[(size_t)I * D + d], F.lo[(size_t)c * D + d] + F.h[(size_t)c * D + d]);
+            }
+        }
+        for (size_t t = 0; t < hc.size(); ++t) hc[t] = hic[t] - loc[t];
+        F.P.resize((size_t)Jf * bs * bs);
+        for (int c = 0; c < Jf; ++c)
+            agg_projection(D, F.deg, &F.lo[(size_t)c * D], &F.h[(size_t)c * D], &loc[(size_t)F.parent[c] * D],
+                           &hc[(size_t)F.parent[c] * D], &F.P[(size_t)c * bs * bs]);
+        F.kind = 1;
+        std::vector<std::vector<std::pair<int, std::vector<double>>>> rows(Jc);
+        std::vector<double> t1(bs * bs), t2(bs * bs);
+        for (int i = 0; i < Jf; ++i) {
+            const double* Pi = &F.P[(size_t)i * bs * bs];
+            for (int64_t p = F.A.rowptr[i]; p < F.A.rowptr[i + 1]; ++p) {
+                int j = F.A.col[p];
+                const double* Aij = &F.A.val[(size_t)p * bs * bs];
+                const double* Pj = &F.P[(size_t)j * bs * bs];
+                for (int r = 0; r < bs; ++r)
+                    for (int n = 0; n < bs; ++n) {
+                        double s = 0;
+                        for (int c = 0; c < bs; ++c) s += Aij[r * bs + c] * Pj[c * bs + n];
+                        t1[r * bs + n] = s;
+                    }
+                for (int m = 0; m < bs; ++m)
+                    for (int n = 0; n < bs; ++n) {
+                        double s = 0;
+                        for (int r = 0; r < bs; ++r) s += Pi[r * bs + m] * t1[r * bs + n];
+                        t2[m * bs + n] = s;
+                    }
+                auto& row = rows[F.parent[i]];
+                bool found = false;
+                for (auto& e : row)
+                    if (e.first == F.parent[j]) { for (int t = 0; t < bs * bs; ++t) e.second[t] += t2[t]; found = true; break; }
+                if (!found) row.push_back({F.parent[j], t2});
+            }
+        }
+        MgLev C;
+        C.A.nrows = Jc; C.A.bs = bs;
+        C.A.rowptr.assign(Jc + 1, 0);
+        for (int I = 0; I < Jc; ++I) {
+            std::sort(rows[I].begin(), rows[I].end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (auto& e : rows[I]) { C.A.col.push_back(e.first); C.A.val.insert(C.A.val.end(), e.second.begin(), e.second.end()); }
+            C.A.rowptr[I + 1] = (int64_t)C.A.col.size();
+        }
+        for (int d = 0; d < 3; ++d) C.n[d] = nc[d];
+        C.deg = F.deg;
+        C.lo = loc;
+        C.h = hc;
+        P->lv.push_back(std::move(C));
+    }
+    for (size_t l = 0; l + 1 < P->lv.size(); ++l) {
+        BlockJacobi bj;
+        bj.init(&P->lv[l].A);
+        P->lv[l].Dinv = bj.Dinv;
+    }
+    MgLev& Cc = P->lv.back();
+    int bs = Cc.A.bs;
+    P->nc = Cc.A.nrows * bs;
+    P->Ainv.assign((size_t)P->nc * P->nc, 0.0);
+    for (int r = 0; r < Cc.A.nrows; ++r)
+        for (int64_t p = Cc.A.rowptr[r]; p < Cc.A.rowptr[r + 1]; ++p)
+            for (int i = 0; i < bs; ++i)
+                for (int j = 0; j < bs; ++j)
+                    P->Ainv[(size_t)(r * bs + i) * P->nc + Cc.A.col[p] * bs + j] = Cc.A.val[(size_t)p * bs * bs + i * bs + j];
+    if (!invert_block(P->Ainv.data(), P->nc)) throw std::runtime_error("singular coarse matrix");
+    return P;
+}
+
+static void bgemv(const MgLev& L, const double* r, double* x, double omega, bool add) {
+    int bs = L.A.bs;
+    for (int row = 0; row < L.A.nrows; ++row) {
+        const double* Di = &L.Dinv[(size_t)row * bs * bs];
+        for (int i = 0; i < bs; ++i) {
+            double s = 0;
+            for (int j = 0; j < bs; ++j) s += Di[i * bs + j] * r[(size_t)row * bs + j];
+            x[(size_t)row * bs + i] = (add ? x[(size_t)row * bs + i] : 0.0) + omega * s;
+        }
+    }
+}
+
+void pmg_vcycle(const Pmg& P, int l, const double* b, double* x) {
+    const MgLev& L = P.lv[l];
+    size_t N = (size_t)L.A.nrows * L.A.bs;
+    if (l + 1 == (int)P.lv.size()) {
+        for (int i = 0; i < P.nc; ++i) {
+            double s = 0;
+            for (int j = 0; j < P.nc; ++j) s += P.Ainv[(size_t)i * P.nc + j] * b[j];
+            x[i] = s;
+        }
+        return;
+    }
+    std::vector<double> r(N);
+    auto resid = [&]() { std::copy(b, b + N, r.begin()); bsr_spmv(L.A, -1.0, x, 1.0, r.data()); };
+    bgemv(L, b, x, P.omega, false);
+    for (int s = 1; s < P.pre; ++s) { resid(); bgemv(L, r.data(), x, P.omega, true); }
+    resid();
+    const MgLev& C = P.lv[l + 1];
+    size_t Nc = (size_t)C.A.nrows * C.A.bs;
+    std::vector<double> bc(Nc, 0.0), xc(Nc, 0.0);
+    int bs = L.A.bs;
+    if (L.kind == 0) {
+        for (int j = 0; j < L.A.nrows; ++j)
+            for (int m = 0; m < C.A.bs; ++m) bc[(size_t)j * C.A.bs + m] = r[(size_t)j * L.A.bs + m];
+    } else {
+        for (int i = 0; i < L.A.nrows; ++i)
+            for (int n = 0; n < bs; ++n)
+                for (int m = 0; m < bs; ++m) bc[(size_t)L.parent[i] * bs + n] += L.P[(size_t)i * bs * bs + m * bs + n] * r[(size_t)i * bs + m];
+    }
+    pmg_vcycle(P, l + 1, bc.data(), xc.data());
+    if (L.kind == 0) {
+        for (int j = 0; j < L.A.nrows; ++j)
+            for (int m = 0; m < C.A.bs; ++m) x[(size_t)j * L.A.bs + m] += xc[(size_t)j * C.A.bs + m];
+    } else {
+        for (int i = 0; i < L.A.nrows; ++i)
+            for (int m = 0; m < bs; ++m)
+                for (int n = 0; n < bs; ++n) x[(size_t)i * bs + m] += L.P[(size_t)i * bs * bs + m * bs + n] * xc[(size_t)L.parent[i] * bs + n];
+    }
+    for (int s = 0; s < P.post; ++s) { resid(); bgemv(L, r.data(), x, P.omega, true); }
+}
+
 }  // namespace orc
 
 // ================================================================== C API ==
@@ -946,6 +1161,43 @@ int orc_soft_gmres(void* mp, int precond, double omega, int bj_iters, int m, dou
     GmresStats st = soft_gmres(*A, P, x, b, m, T);
     *iters = st.iters; *res0 = st.res0; *res = st.res;
     return st.converged ? 0 : 1;
+}
+// persistent hierarchy: create / V-cycle from a level / level SpMV / transfers
+void* orc_pmg_create(void* mp, void* gp, const int* deg, int ndeg, int min_coarse, double omega, int pre, int post) {
+    try { return pmg_build(*(Bsr*)mp, *(Grid*)gp, deg, ndeg, min_coarse, omega, pre, post); } catch (...) { return nullptr; }
+}
+void orc_pmg_free(void* p) { delete (Pmg*)p; }
+int orc_pmg_levels(void* p, int* sizes) {
+    Pmg* P = (Pmg*)p;
+    for (size_t l = 0; l < P->lv.size(); ++l) if (sizes) sizes[l] = P->lv[l].A.nrows * P->lv[l].A.bs;
+    return (int)P->lv.size();
+}
+void orc_pmg_vcycle_from(void* p, int l, const double* b, double* x) {
+    Pmg* P = (Pmg*)p;
+    size_t N = (size_t)P->lv[l].A.nrows * P->lv[l].A.bs;
+    std::fill(x, x + N, 0.0);
+    pmg_vcycle(*P, l, b, x);
+}
+void orc_pmg_level_spmv(void* p, int l, const double* x, double* y) {
+    Pmg* P = (Pmg*)p;
+    bsr_spmv(P->lv[l].A, 1.0, x, 0.0, y);
+}
+void orc_pmg_set(void* p, double omega, int pre, int post) {
+    Pmg* P = (Pmg*)p;
+    P->omega = omega; P->pre = pre; P->post = post;
+}
+// one V-cycle from a zero guess: x = V(b)
+int orc_pmg_vcycle(void* mp, void* gp, const int* deg, int ndeg, int min_coarse, double omega, int pre, int post,
+                   const double* b, double* x, int* nlevels) {
+    try {
+        Pmg* P = pmg_build(*(Bsr*)mp, *(Grid*)gp, deg, ndeg, min_coarse, omega, pre, post);
+        size_t N = (size_t)P->lv[0].A.nrows * P->lv[0].A.bs;
+        std::fill(x, x + N, 0.0);
+        pmg_vcycle(*P, 0, b, x);
+        if (nlevels) *nlevels = (int)P->lv.size();
+        delete P;
+        return 0;
+    } catch (...) { return -1; }
 }
 int orc_num_threads() {
 #ifdef _OPENMP

@@ -1,0 +1,134 @@
+"""GPU parity of the p/h multigrid preconditioner against the oracle hierarchy.
+
+V-cycle: the device cycle (FP64) equals the oracle's restated cycle to 1e-11
+(same hierarchy: Galerkin p-sub-blocks, degree-p aggregation, block-Jacobi,
+dense coarsest solve).  K-cycle: equals a numpy restatement of
+`kin` flexible-GMRES steps on the oracle's level operator.  FP32 cycle within
+1e-4 of FP64.  End-to-end: FGMRES + multigrid reaches 1e-8 true residual.
+"""
+import numpy as np
+import pytest
+
+import bosss_amd as B
+import oracle as O
+from tests.helpers import log_uniform_nu, nodes_3d
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _problem(ctx, n=(6, 5, 4), k=3, seed=3):
+    nodes = nodes_3d(*n, jitter=0.3, seed=seed)
+    og = O.Grid(*nodes)
+    nu = log_uniform_nu(og.J, seed)
+    gg = B.Grid(ctx, *nodes)
+    oo = O.Sip(og, k, nu)
+    go = B.SipOperator(gg, k, nu)
+    Ao, _ = oo.assemble()
+    return og, gg, oo, go, Ao
+
+
+@pytest.mark.parametrize("matrix_free", [False, True])
+@pytest.mark.parametrize("degrees", [[3, 1], [3, 2, 1, 0]])
+def test_vcycle_matches_oracle(ctx, matrix_free, degrees):
+    _say(f"case vcycle degrees={degrees} matrix_free={matrix_free}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(1).standard_normal(oo.N)
+    ref, nl = Ao.pmg_vcycle(og, b, degrees, min_coarse=8, omega=0.7, pre=2, post=1)
+    mg = B.PMultigrid(go, degrees=degrees, min_coarse=8, omega=0.7, pre=2, post=1, kcycle_inner=0,
+                      matrix_free=matrix_free)
+    out = mg.Solve(np.zeros(oo.N), b)   # x = 0 + V(b - 0)
+    assert nl >= 3
+    assert rel(out, ref) < 1e-11
+
+
+def _kcycle_numpy(P, level, b, kin):
+    """kin flexible-GMRES steps on level `level` (CGS2), V-cycle preconditioned."""
+    n = b.size
+    beta = np.linalg.norm(b)
+    V = [b / beta]
+    Z = []
+    H = np.zeros((kin + 1, kin))
+    for j in range(kin):
+        z = P.vcycle(V[j], level)
+        Z.append(z)
+        w = P.spmv(level, z)
+        for _ in range(2):
+            h = np.array([w @ v for v in V])
+            w = w - sum(hh * v for hh, v in zip(h, V))
+            H[:j + 1, j] += h
+        H[j + 1, j] = np.linalg.norm(w)
+        V.append(w / H[j + 1, j])
+    e = np.zeros(kin + 1)
+    e[0] = beta
+    y = np.linalg.lstsq(H, e, rcond=None)[0]
+    return np.array(Z).T @ y
+
+
+def test_kcycle_matches_numpy_restatement(ctx):
+    _say("case kcycle")
+    og, gg, oo, go, Ao = _problem(ctx)
+    degrees = [3, 1]
+    kin = 3
+    P = O.Pmg(Ao, og, degrees, min_coarse=8, omega=0.7, pre=2, post=2)
+    b = np.random.default_rng(2).standard_normal(oo.N)
+    # fine level of the cycle in numpy: BJ pre-smoothing, truncation, K-cycle, post-smoothing
+    rp, col, val = Ao.export()
+    Np, J = oo.Np, og.J
+    Dinv = np.array([np.linalg.inv(val[[p for p in range(rp[j], rp[j + 1]) if col[p] == j][0]]) for j in range(J)])
+    A = lambda v: Ao.spmv(1.0, v, 0.0, np.zeros(oo.N))
+    bj = lambda r: np.einsum("jab,jb->ja", Dinv, r.reshape(J, Np)).ravel()
+    x = 0.7 * bj(b)
+    x += 0.7 * bj(b - A(x))
+    r = b - A(x)
+    rc = r.reshape(J, Np)[:, :4].ravel()
+    xc = _kcycle_numpy(P, 1, rc, kin)
+    xx = x.reshape(J, Np).copy()
+    xx[:, :4] += xc.reshape(J, 4)
+    x = xx.ravel()
+    for _ in range(2):
+        x += 0.7 * bj(b - A(x))
+    mg = B.PMultigrid(go, degrees=degrees, min_coarse=8, omega=0.7, pre=2, post=2, kcycle_inner=kin)
+    out = mg.Solve(np.zeros(oo.N), b)
+    assert rel(out, x) < 1e-9
+
+
+def test_fp32_cycle_close_to_fp64(ctx):
+    _say("case fp32 cycle")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(4).standard_normal(oo.N)
+    m64 = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=2, prec=B.BH_FP64)
+    m32 = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=2, prec=B.BH_FP32)
+    x64 = m64.Solve(np.zeros(oo.N), b)
+    x32 = m32.Solve(np.zeros(oo.N), b)
+    assert rel(x32, x64) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [B.BH_FP64, B.BH_FP32])
+def test_fgmres_multigrid_solve(ctx, prec):
+    """FGMRES(30) + p/h multigrid (K-cycle) to 1e-8 true FP64 residual, 16^3 k=3, log-uniform nu."""
+    _say(f"case fgmres+mg prec={prec}")
+    n = 16
+    x = np.linspace(0, 1, n + 1)
+    og = O.Grid(x, x, x)
+    nu = log_uniform_nu(og.J, 11)
+    gg = B.Grid(ctx, x, x, x)
+    go = B.SipOperator(gg, 3, nu)
+    oo = O.Sip(og, 3, nu)
+    b = np.random.default_rng(5).standard_normal(go.N)
+    mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=64, kcycle_inner=2, prec=prec)
+    fg = B.FlexGMRES(op=go, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=300)
+    xs = fg.Solve(np.zeros(go.N), b)
+    st = fg.stats()
+    _say("  stats", st)
+    assert st["converged"]
+    assert st["iterations"] < 150
+    r = b - oo.evaluate(xs)
+    assert np.linalg.norm(r) <= 1.0001e-8 * np.linalg.norm(b)
