@@ -70,6 +70,18 @@ SIGNATURES = {
     "bh_gmres_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_fgmres_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_pmg_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _c_i, _PP]),
+    "bh_mlp_create": (_c_i, [_c_p, _c_i, _c_i, _c_i, ctypes.c_uint64, _PP]),
+    "bh_mlp_destroy": (_c_i, [_c_p]),
+    "bh_mlp_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
+    "bh_mlp_get_params": (_c_i, [_c_p, _c_p]),
+    "bh_mlp_set_params": (_c_i, [_c_p, _c_p]),
+    "bh_mlp_forward": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_i]),
+    "bh_mlp_gradient": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, ctypes.POINTER(_c_d), _c_i]),
+    "bh_mlp_adam_step": (_c_i, [_c_p, _c_f]),
+    "bh_mlp_features": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
+    "bh_mlp_targets": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
+    "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
+    "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
     "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
                                ctypes.POINTER(_c_d)]),
@@ -145,6 +157,11 @@ class DeviceArray:
         _check(lib().bh_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
         return out
 
+    def view(self, offset, n):
+        """non-owning view of elements [offset, offset+n) (device-pointer argument)."""
+        assert 0 <= offset and offset + n <= self.n
+        return _DeviceView(self, offset, n)
+
     def free(self):
         if self.ptr and self.ctx.h:
             _check(lib().bh_free(self.ctx.h, self.ptr))
@@ -159,9 +176,23 @@ class DeviceArray:
             pass
 
 
+class _DeviceView:
+    def __init__(self, base, offset, n):
+        self.base = base
+        self.ctx = base.ctx
+        self.dtype = base.dtype
+        self.n = n
+        self.ptr = base.ptr + offset * base.dtype.itemsize
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib().bh_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+
 def _vec_args(x):
     """(pointer, flags) of a host numpy array or a DeviceArray."""
-    if isinstance(x, DeviceArray):
+    if isinstance(x, (DeviceArray, _DeviceView)):
         return x.ptr, BH_DEVICE_PTR
     return x.ctypes.data, 0
 
@@ -422,3 +453,76 @@ class PMultigrid(_Solver):
                                    pre, post, kcycle_inner, prec, 1 if matrix_free else 0, ctypes.byref(h)))
         super().__init__(op.ctx, h.value, op.N, (op, M))
 
+
+
+class CellMLP(_Owned):
+    """Per-cell correction network (bh_mlp_*): in -> 128 -> 128 -> out, ReLU,
+    BF16 weights / FP32 accumulation (MFMA) for inference, FP32 training."""
+
+    _destroy_fn = "bh_mlp_destroy"
+
+    def __init__(self, ctx, n_in, n_out, seed=0, n_hidden=128):
+        h = ctypes.c_void_p()
+        _check(lib().bh_mlp_create(ctx.h, n_in, n_hidden, n_out, seed, ctypes.byref(h)))
+        self._register(ctx, h.value)
+        self.ctx = ctx
+        self.n_in, self.n_out, self.n_hidden = n_in, n_out, n_hidden
+        n, ip, op_ = _c_i(), _c_i(), _c_i()
+        _check(lib().bh_mlp_info(self.h, ctypes.byref(n), ctypes.byref(ip), ctypes.byref(op_)))
+        self.n_params, self.in_p, self.out_p = n.value, ip.value, op_.value
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().bh_mlp_get_params(self.h, p.ctypes.data))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _check(lib().bh_mlp_set_params(self.h, p.ctypes.data))
+
+    def forward(self, X, Y=None):
+        if isinstance(X, (DeviceArray, _DeviceView)):
+            n = X.n // self.in_p
+            _check(lib().bh_mlp_forward(self.h, n, X.ptr, Y.ptr, BH_DEVICE_PTR))
+            return Y
+        X = np.ascontiguousarray(X, np.float32)
+        n = X.shape[0]
+        Y = np.zeros((n, self.out_p), np.float32)
+        _check(lib().bh_mlp_forward(self.h, n, X.ctypes.data, Y.ctypes.data, 0))
+        return Y
+
+    def gradient(self, X, T, want_grad=True):
+        loss = _c_d()
+        g = np.zeros(self.n_params, np.float32) if want_grad else None
+        if isinstance(X, (DeviceArray, _DeviceView)):
+            n = X.n // self.in_p
+            _check(lib().bh_mlp_gradient(self.h, n, X.ptr, T.ptr, _ptr(g), ctypes.byref(loss), BH_DEVICE_PTR))
+        else:
+            X = np.ascontiguousarray(X, np.float32)
+            T = np.ascontiguousarray(T, np.float32)
+            _check(lib().bh_mlp_gradient(self.h, X.shape[0], X.ctypes.data, T.ctypes.data, _ptr(g), ctypes.byref(loss), 0))
+        return loss.value, g
+
+    def adam_step(self, lr):
+        _check(lib().bh_mlp_adam_step(self.h, lr))
+
+
+def mlp_features(op, r, X, s, in_p):
+    """device feature map of a residual (all DeviceArrays)."""
+    _check(lib().bh_mlp_features(op.h, r.ptr, X.ptr, s.ptr, in_p, BH_DEVICE_PTR))
+
+
+def mlp_targets(op, s, e, T, out_p):
+    _check(lib().bh_mlp_targets(op.h, s.ptr, e.ptr, T.ptr, out_p, BH_DEVICE_PTR))
+
+
+def pmg_set_learned(pmg, mlp):
+    _check(lib().bh_pmg_set_learned(pmg.h, mlp.h if mlp is not None else None))
+    pmg._mlp = mlp
+
+
+def pmg_learned_stats(pmg):
+    a, r = _c_i(), _c_i()
+    _check(lib().bh_pmg_learned_stats(pmg.h, ctypes.byref(a), ctypes.byref(r)))
+    return a.value, r.value

@@ -6,6 +6,7 @@
 #include "bh_internal.hpp"
 #include "bh_solvers.hpp"
 #include "bh_mg.hpp"
+#include "bh_mlp.hpp"
 
 namespace bh {
 const std::string& last_error();
@@ -587,6 +588,113 @@ int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int
     std::vector<int> deg(degrees, degrees + ndeg);
     mg->build(op, Af, deg, min_coarse, matrix_free != 0);
     *out = mg.release();
+    BH_API_END
+}
+
+// ------------------------------------------------------------------ learned correction
+int bh_mlp_create(bh_ctx* ctx, int n_in, int n_hidden, int n_out, uint64_t seed, bh_mlp** out) {
+    BH_API_BEGIN
+    BH_CHECK(n_in >= 1 && n_in <= 32 && n_out >= 1 && n_out <= 32, "bh_mlp_create: widths must be in 1..32");
+    auto w = std::make_unique<bh_mlp>();
+    w->m = new Mlp(ctx, n_in, n_hidden, n_out, seed);
+    *out = w.release();
+    BH_API_END
+}
+int bh_mlp_destroy(bh_mlp* w) {
+    BH_API_BEGIN
+    if (w) {
+        (void)hipStreamSynchronize(w->m->ctx->stream);
+        delete w->m;
+        delete w;
+    }
+    BH_API_END
+}
+int bh_mlp_info(bh_mlp* w, int* n, int* in_p, int* out_p) {
+    BH_API_BEGIN
+    if (n) *n = w->m->nw;
+    if (in_p) *in_p = w->m->in_p;
+    if (out_p) *out_p = w->m->out_p;
+    BH_API_END
+}
+int bh_mlp_get_params(bh_mlp* w, float* p) {
+    BH_API_BEGIN
+    w->m->ctx->download(p, w->m->params.p, w->m->nw * sizeof(float));
+    BH_API_END
+}
+int bh_mlp_set_params(bh_mlp* w, const float* p) {
+    BH_API_BEGIN
+    w->m->set_params(p);
+    w->m->ctx->sync();
+    BH_API_END
+}
+int bh_mlp_forward(bh_mlp* w, int n, const float* X, float* Y, int flags) {
+    BH_API_BEGIN
+    Mlp* m = w->m;
+    const bool dev = flags & BH_DEVICE_PTR;
+    Staged sx(m->ctx, X, (size_t)n * m->in_p * sizeof(float), dev, true);
+    Staged sy(m->ctx, Y, (size_t)n * m->out_p * sizeof(float), dev, false);
+    m->infer(n, (const float*)sx.dptr, (float*)sy.dptr);
+    sy.download(Y, (size_t)n * m->out_p * sizeof(float), dev);
+    if (!dev) m->ctx->sync();
+    BH_API_END
+}
+int bh_mlp_gradient(bh_mlp* w, int n, const float* X, const float* T, float* grad, double* loss, int flags) {
+    BH_API_BEGIN
+    Mlp* m = w->m;
+    BH_CHECK(n >= 1, "bh_mlp_gradient: empty batch");
+    const bool dev = flags & BH_DEVICE_PTR;
+    Staged sx(m->ctx, X, (size_t)n * m->in_p * sizeof(float), dev, true);
+    Staged st(m->ctx, T, (size_t)n * m->out_p * sizeof(float), dev, true);
+    const double l = m->gradient(n, nullptr, (const float*)sx.dptr, (const float*)st.dptr);
+    if (loss) *loss = l;
+    if (grad) m->ctx->download(grad, m->grad.p, m->nw * sizeof(float));
+    BH_API_END
+}
+int bh_mlp_adam_step(bh_mlp* w, float lr) {
+    BH_API_BEGIN
+    w->m->adam(lr);
+    BH_API_END
+}
+int bh_mlp_features(bh_op* op, const double* r, float* X, float* s, int in_p, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(in_p >= op->Np + 2 * op->D, "bh_mlp_features: in_p too small");
+    const bool dev = flags & BH_DEVICE_PTR;
+    const int J = op->grid->J;
+    Staged sr(op->ctx, r, op->N * sizeof(double), dev, true);
+    Staged sx(op->ctx, X, (size_t)J * in_p * sizeof(float), dev, false);
+    Staged ss(op->ctx, s, (size_t)J * sizeof(float), dev, false);
+    mlp_features(op, sr.dptr, BH_FP64, (float*)sx.dptr, (float*)ss.dptr, in_p);
+    sx.download(X, (size_t)J * in_p * sizeof(float), dev);
+    ss.download(s, (size_t)J * sizeof(float), dev);
+    if (!dev) op->ctx->sync();
+    BH_API_END
+}
+int bh_mlp_targets(bh_op* op, const float* s, const double* e, float* T, int out_p, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(out_p >= op->Np, "bh_mlp_targets: out_p too small");
+    const bool dev = flags & BH_DEVICE_PTR;
+    const int J = op->grid->J;
+    Staged ss(op->ctx, s, (size_t)J * sizeof(float), dev, true);
+    Staged se(op->ctx, e, op->N * sizeof(double), dev, true);
+    Staged st(op->ctx, T, (size_t)J * out_p * sizeof(float), dev, false);
+    mlp_targets(op, (const float*)ss.dptr, (const double*)se.dptr, out_p, (float*)st.dptr);
+    st.download(T, (size_t)J * out_p * sizeof(float), dev);
+    if (!dev) op->ctx->sync();
+    BH_API_END
+}
+int bh_pmg_set_learned(bh_solver* s, bh_mlp* w) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_set_learned: solver is not a multigrid preconditioner");
+    mg->attach_learned(w ? w->m : nullptr);
+    BH_API_END
+}
+int bh_pmg_learned_stats(bh_solver* s, int* acc, int* rej) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_learned_stats: solver is not a multigrid preconditioner");
+    if (acc) *acc = mg->learned_accepted;
+    if (rej) *rej = mg->learned_rejected;
     BH_API_END
 }
 

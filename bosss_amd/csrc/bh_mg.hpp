@@ -6,6 +6,8 @@
 
 namespace bh {
 
+struct Mlp;
+
 void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc);
 template <typename T> void p_restrict(bh_ctx*, int64_t J, int bsf, int bsc, const T* rf, T* rc);
 template <typename T> void p_prolong_add(bh_ctx*, int64_t J, int bsf, int bsc, const T* xc, T* xf);
@@ -47,6 +49,13 @@ struct PMultigrid : bh_solver {
     DevBuf Ainv64, Ainv32;
     DevBuf in32, out32;
     bh_bsr* ownedAf = nullptr;
+    // learned correction after the cycle (guarded by residual minimisation)
+    struct Mlp* learned = nullptr;
+    bh_op* fine_op = nullptr;
+    DevBuf lX, ls, lY, le, lrq;     // features, scales, outputs, correction, [r' | A e]
+    int learned_accepted = 0, learned_rejected = 0;
+    template <typename T> void learned_correction();
+    void attach_learned(struct Mlp* m);
     ~PMultigrid() override;
     void build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bool matrix_free);
     void solve_dev(double* x, const double* b) override;        // x <- x + V(b - A x)

@@ -18,6 +18,7 @@
 #include <map>
 
 #include "bh_mg.hpp"
+#include "bh_mlp.hpp"
 #include "bh_solvers.hpp"
 
 namespace bh {
@@ -109,6 +110,7 @@ void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, i
     ctx = op->ctx;
     N = op->N;
     matrix_free = mf;
+    fine_op = op;
     const int D = op->D;
     BH_CHECK(!degrees.empty() && degrees[0] == op->k, "multigrid: first degree must be the operator's degree");
     for (size_t i = 1; i < degrees.size(); ++i)
@@ -429,15 +431,60 @@ void PMultigrid::kcycle(int l) {
     multi_axpy_t<T>(ctx, vec<T>(L.x), Z, n, used, n, y.data(), 1.0);
 }
 
+void PMultigrid::attach_learned(Mlp* m) {
+    learned = m;
+    learned_accepted = learned_rejected = 0;
+    if (!m) return;
+    BH_CHECK(fine_op, "learned correction needs the fine-level operator");
+    BH_CHECK(m->out == fine_op->Np && m->in == fine_op->Np + 2 * fine_op->D, "learned correction: network widths do not match the operator");
+    const int J = fine_op->grid->J;
+    const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
+    lX.alloc((size_t)J * m->in_p * sizeof(float));
+    ls.alloc((size_t)J * sizeof(float));
+    lY.alloc((size_t)J * m->out_p * sizeof(float));
+    le.alloc(N * es);
+    lrq.alloc(2 * N * es);
+}
+
+// After the cycle (x = x_mg in lv[0].x): r' = b - A x_mg, e = network(r'),
+// q = A e, alpha = <q, r'> / <q, q>.  The step x += alpha e minimises the
+// residual along e; it is taken only when alpha > 0 (the network's direction
+// agrees with the residual), otherwise the plain multigrid result stands.
+template <typename T>
+void PMultigrid::learned_correction() {
+    MgLevel& L = lv[0];
+    const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
+    T* rq = vec<T>(lrq);
+    T* rp = rq;
+    T* q = rq + N;
+    T* e = vec<T>(le);
+    residual<T>(0, vec<T>(L.b), vec<T>(L.x), rp);
+    mlp_features(fine_op, rp, p, lX.as<float>(), ls.as<float>(), learned->in_p);
+    learned->infer(fine_op->grid->J, lX.as<float>(), lY.as<float>());
+    mlp_correction(fine_op, ls.as<float>(), lY.as<float>(), learned->out_p, e, p);
+    sip_apply(fine_op, 1.0, e, 0.0, q, false, p);
+    double h[2];
+    multi_dot_t<T>(ctx, q, rq, N, 2, N, h);   // <q, r'>, <q, q>
+    const double alpha = h[1] > 0 ? h[0] / h[1] : 0.0;
+    if (alpha > 0 && std::isfinite(alpha)) {
+        axpby_t<T>(ctx, alpha, e, 1.0, vec<T>(L.x), N);
+        learned_accepted++;
+    } else {
+        learned_rejected++;
+    }
+}
+
 void PMultigrid::precondition(double* x, const double* b) {
     MgLevel& L = lv[0];
     if (prec == BH_FP32) {
         convert_f64_f32(ctx, b, vec<float>(L.b), N);
         vcycle<float>(0);
+        if (learned) learned_correction<float>();
         convert_f32_f64(ctx, vec<float>(L.x), x, N);
     } else {
         copy(ctx, vec<double>(L.b), b, N);
         vcycle<double>(0);
+        if (learned) learned_correction<double>();
         copy(ctx, x, vec<double>(L.x), N);
     }
     iterations++;

@@ -153,6 +153,36 @@ int bh_fgmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, do
  * fine_mat: the assembled operator of `op` (NULL: assembled internally). */
 int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int min_coarse, double omega,
                   int pre, int post, int kcycle_inner, int prec, int matrix_free, bh_solver** solver);
+/* ---------------------------------------------------------------- learned correction
+ * No counterpart in the reference: a per-cell MLP (in -> 128 -> 128 -> out,
+ * ReLU; BF16 weights, FP32 accumulation on MFMA for inference; FP32 backward
+ * and Adam on the device) used as an extra correction after the multigrid
+ * cycle, accepted only when it reduces the residual (the acceptance rule of
+ * OrthonormalizationMultigrid.MinimizeResidual, AS/OrthonormalizationMultigrid.cs:149-215).
+ * Parameter layout (FP32, flat): W1[128][in_p], b1[128], W2[128][128], b2[128],
+ * W3[out_p][128], b3[out_p], in_p/out_p = in/out rounded up to 16 (padding zero).
+ * Feature / target arrays are row-major [cell][in_p] / [cell][out_p] FP32. */
+typedef struct bh_mlp bh_mlp;
+int bh_mlp_create(bh_ctx* ctx, int n_in, int n_hidden, int n_out, uint64_t seed, bh_mlp** mlp);
+int bh_mlp_destroy(bh_mlp* mlp);
+int bh_mlp_info(bh_mlp* mlp, int* n_params, int* in_p, int* out_p);
+int bh_mlp_get_params(bh_mlp* mlp, float* params);
+int bh_mlp_set_params(bh_mlp* mlp, const float* params);
+/* Y = network(X) for ncells rows (inference path: BF16 weights, MFMA) */
+int bh_mlp_forward(bh_mlp* mlp, int ncells, const float* X, float* Y, int flags);
+/* mean-squared-error loss over the first n_out outputs and its gradient
+ * (FP32, weights rounded to BF16 as in inference); grad may be NULL */
+int bh_mlp_gradient(bh_mlp* mlp, int ncells, const float* X, const float* T, float* grad, double* loss, int flags);
+/* one Adam step with the gradient of the last bh_mlp_gradient call */
+int bh_mlp_adam_step(bh_mlp* mlp, float lr);
+/* feature map of a residual r (N values, FP64) on op's grid -> X[J][in_p], scales s[J];
+ * target map of an error e -> T[J][out_p] with the same scales */
+int bh_mlp_features(bh_op* op, const double* r, float* X, float* s, int in_p, int flags);
+int bh_mlp_targets(bh_op* op, const float* s, const double* e, float* T, int out_p, int flags);
+/* attach (mlp != NULL) or detach the learned correction of a bh_pmg_create solver */
+int bh_pmg_set_learned(bh_solver* pmg, bh_mlp* mlp);
+int bh_pmg_learned_stats(bh_solver* pmg, int* accepted, int* rejected);
+
 int bh_solver_solve(bh_solver* solver, double* x, const double* b, int flags);
 int bh_solver_stats(bh_solver* solver, int* iterations, int* converged, double* res0, double* res);
 int bh_solver_destroy(bh_solver* solver);
