@@ -82,6 +82,11 @@ SIGNATURES = {
     "bh_mlp_targets": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
     "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
+    "bh_euler_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _PP]),
+    "bh_euler_destroy": (_c_i, [_c_p]),
+    "bh_euler_info": (_c_i, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i)]),
+    "bh_euler_rhs": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i]),
+    "bh_euler_tvd3": (_c_i, [_c_p, _c_p, _c_d, _c_i, _c_i, _c_i]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
     "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
                                ctypes.POINTER(_c_d)]),
@@ -526,3 +531,32 @@ def pmg_learned_stats(pmg):
     a, r = _c_i(), _c_i()
     _check(lib().bh_pmg_learned_stats(pmg.h, ctypes.byref(a), ctypes.byref(r)))
     return a.value, r.value
+
+
+class EulerOperator(_Owned):
+    """Explicit DG Euler/HLLC operator on a periodic Cartesian grid (bh_euler_*)."""
+
+    _destroy_fn = "bh_euler_destroy"
+
+    def __init__(self, grid, k=2, nq1=4, gamma=1.4, mach=None):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self.mach = mach if mach is not None else 1.0 / np.sqrt(gamma)
+        h = ctypes.c_void_p()
+        _check(lib().bh_euler_create(grid.h, k, nq1, gamma, self.mach, ctypes.byref(h)))
+        self._register(self.ctx, h.value)
+        N, Np = _c_i64(), _c_i()
+        _check(lib().bh_euler_info(self.h, ctypes.byref(N), ctypes.byref(Np)))
+        self.N, self.Np = N.value, Np.value
+
+    def rhs(self, U, R, prec=BH_FP64):
+        up, fu = _vec_args(U)
+        rp, fr = _vec_args(R)
+        assert fu == fr
+        _check(lib().bh_euler_rhs(self.h, up, rp, prec, fu))
+        return R
+
+    def tvd3(self, U, dt, steps, prec=BH_FP64):
+        up, fu = _vec_args(U)
+        _check(lib().bh_euler_tvd3(self.h, up, dt, steps, prec, fu))
+        return U

@@ -7,6 +7,7 @@
 #include "bh_solvers.hpp"
 #include "bh_mg.hpp"
 #include "bh_mlp.hpp"
+#include "bh_euler.hpp"
 
 namespace bh {
 const std::string& last_error();
@@ -695,6 +696,54 @@ int bh_pmg_learned_stats(bh_solver* s, int* acc, int* rej) {
     BH_CHECK(mg, "bh_pmg_learned_stats: solver is not a multigrid preconditioner");
     if (acc) *acc = mg->learned_accepted;
     if (rej) *rej = mg->learned_rejected;
+    BH_API_END
+}
+
+// ------------------------------------------------------------------ Euler
+int bh_euler_create(bh_grid* g, int k, int nq1, double gamma, double mach, bh_euler** out) {
+    BH_API_BEGIN
+    BH_CHECK(gamma > 1.0 && mach > 0.0, "bh_euler_create: need gamma > 1, mach > 0");
+    auto w = std::make_unique<bh_euler>();
+    w->e = new EulerOp(g, k, nq1, gamma, mach);
+    *out = w.release();
+    BH_API_END
+}
+int bh_euler_destroy(bh_euler* w) {
+    BH_API_BEGIN
+    if (w) {
+        (void)hipStreamSynchronize(w->e->ctx->stream);
+        delete w->e;
+        delete w;
+    }
+    BH_API_END
+}
+int bh_euler_info(bh_euler* w, int64_t* N, int* Np) {
+    BH_API_BEGIN
+    if (N) *N = w->e->N;
+    if (Np) *Np = w->e->Np;
+    BH_API_END
+}
+int bh_euler_rhs(bh_euler* w, const void* U, void* Rv, int prec, int flags) {
+    BH_API_BEGIN
+    EulerOp* e = w->e;
+    const bool dev = flags & BH_DEVICE_PTR;
+    BH_CHECK(prec == BH_FP64 || dev, "bh_euler_rhs: FP32 takes device float vectors");
+    Staged su(e->ctx, U, e->N * sizeof(double), dev, true);
+    Staged sr(e->ctx, Rv, e->N * sizeof(double), dev, false);
+    e->rhs(su.dptr, sr.dptr, prec);
+    sr.download(Rv, e->N * sizeof(double), dev);
+    if (!dev) e->ctx->sync();
+    BH_API_END
+}
+int bh_euler_tvd3(bh_euler* w, void* U, double dt, int steps, int prec, int flags) {
+    BH_API_BEGIN
+    EulerOp* e = w->e;
+    const bool dev = flags & BH_DEVICE_PTR;
+    BH_CHECK(prec == BH_FP64 || dev, "bh_euler_tvd3: FP32 takes device float vectors");
+    Staged su(e->ctx, U, e->N * sizeof(double), dev, true);
+    e->tvd3(su.dptr, dt, steps, prec);
+    su.download(U, e->N * sizeof(double), dev);
+    if (!dev) e->ctx->sync();
     BH_API_END
 }
 

@@ -183,6 +183,24 @@ int bh_mlp_targets(bh_op* op, const float* s, const double* e, float* T, int out
 int bh_pmg_set_learned(bh_solver* pmg, bh_mlp* mlp);
 int bh_pmg_learned_stats(bh_solver* pmg, int* accepted, int* rejected);
 
+/* ---------------------------------------------------------------- compressible Euler
+ * Explicit DG for the Euler equations on a periodic Cartesian grid: the CNS
+ * operator with the HLLC fluxes OptimizedHLLC{Density,Momentum,Energy}Flux
+ * (L3-solution/BoSSS.Solution.CompressibleFlowCommon/Convection/OptimizedHLLC*.cs,
+ * MachScaling = gamma*mach^2) and RungeKutta with RungeKuttaScheme.TVD3
+ * (L3-solution/BoSSS.Solution/TimeStepping/RungeKuttaScheme.cs:145-167).
+ * State layout [cell][variable][mode], variables rho, m_1..m_D, E.
+ * nq1 Gauss points per direction (CNS uses order 3k: nq1 = 4 for k = 2).
+ * Supported (D, k, nq1): (3,1,2), (3,1,3), (3,2,4), (2,1,3), (2,2,4), (2,3,5). */
+typedef struct bh_euler bh_euler;
+int bh_euler_create(bh_grid* grid, int k, int nq1, double gamma, double mach, bh_euler** e);
+int bh_euler_destroy(bh_euler* e);
+int bh_euler_info(bh_euler* e, int64_t* N, int* Np);
+/* R = dU/dt (prec BH_FP64: double vectors; BH_FP32: float device vectors) */
+int bh_euler_rhs(bh_euler* e, const void* U, void* R, int prec, int flags);
+/* `steps` TVD3 steps of size dt, in place */
+int bh_euler_tvd3(bh_euler* e, void* U, double dt, int steps, int prec, int flags);
+
 int bh_solver_solve(bh_solver* solver, double* x, const double* b, int flags);
 int bh_solver_stats(bh_solver* solver, int* iterations, int* converged, double* res0, double* res);
 int bh_solver_destroy(bh_solver* solver);

@@ -72,6 +72,12 @@ def lib():
             "orc_pmg_vcycle_from": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_level_spmv": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_set": (None, [_p, _d, _i, _i]),
+            "orc_euler_create": (_p, [_p, _i, _i, _d, _d]),
+            "orc_euler_free": (None, [_p]),
+            "orc_euler_rhs": (None, [_p, _dp, _dp]),
+            "orc_euler_tvd3": (None, [_p, _dp, _d, _i]),
+            "orc_hllc": (None, [_i, _d, _d, _dp, _dp, _dp, _dp]),
+            "orc_euler_flux": (None, [_i, _d, _d, _dp, _dp]),
             "orc_num_threads": (_i, []),
             "orc_set_threads": (None, [_i]),
         }
@@ -312,3 +318,48 @@ class Pmg:
             lib().orc_pmg_free(self.h)
         except Exception:
             pass
+
+
+class Euler:
+    """Explicit DG for the compressible Euler equations on a periodic Cartesian
+    grid with the HLLC flux (see oracle.cpp, Euler section)."""
+
+    def __init__(self, grid, k=2, nq1=4, gamma=1.4, mach=None):
+        self.grid = grid
+        self.k = k
+        self.D = grid.D
+        self.Np = basis_count(grid.D, k)
+        self.nv = grid.D + 2
+        self.N = grid.J * self.nv * self.Np
+        self.gamma = gamma
+        self.mach = mach if mach is not None else 1.0 / np.sqrt(gamma)
+        self.h = lib().orc_euler_create(grid.h, k, nq1, gamma, self.mach)
+
+    def rhs(self, U):
+        R = np.zeros(self.N)
+        lib().orc_euler_rhs(self.h, np.ascontiguousarray(U, np.float64), R)
+        return R
+
+    def tvd3(self, U, dt, steps):
+        U = np.ascontiguousarray(U, np.float64).copy()
+        lib().orc_euler_tvd3(self.h, U, dt, steps)
+        return U
+
+    def __del__(self):
+        try:
+            lib().orc_euler_free(self.h)
+        except Exception:
+            pass
+
+
+def hllc(D, gamma, mach, Ui, Uo, n):
+    F = np.zeros(D + 2)
+    lib().orc_hllc(D, gamma, mach, np.ascontiguousarray(Ui, np.float64), np.ascontiguousarray(Uo, np.float64),
+                   np.ascontiguousarray(n, np.float64), F)
+    return F
+
+
+def euler_flux(D, gamma, mach, U):
+    F = np.zeros((D + 2) * D)
+    lib().orc_euler_flux(D, gamma, mach, np.ascontiguousarray(U, np.float64), F)
+    return F.reshape(D + 2, D)

@@ -1002,6 +1002,184 @@ void pmg_vcycle(const Pmg& P, int l, const double* b, double* x) {
     for (int s = 0; s < P.post; ++s) { resid(); bgemv(L, r.data(), x, P.omega, true); }
 }
 
+
+// ================================================================ Euler ==
+// Compressible Euler, explicit DG on a periodic Cartesian grid.  State layout
+// [cell][variable][mode] with variables (rho, m_1..m_D, E) (the cell-major,
+// field-major coordinate map of F/UnsetteledCoordinateMapping.cs:696-711).
+// dU/dt = R(U) = int F(U) . grad(phi) - oint Fhat(U-, U+, n) phi, with the
+// HLLC flux of OptimizedHLLC{Density,Momentum,Energy}Flux.InnerEdgeFlux
+// (L3-solution/BoSSS.Solution.CompressibleFlowCommon/Convection/
+// OptimizedHLLCDensityFlux.cs:45-167 and the Momentum/Energy variants) and the
+// volume fluxes of the same classes' Flux(); non-dimensionalisation with
+// MachScaling = gamma * Mach^2 as there.
+struct Euler {
+    const Grid* g = nullptr;
+    int D = 3, k = 2, Np = 10, nq1 = 4, nqv = 64, nqf = 16, nv = 5;
+    double gamma = 1.4, MS = 1.0;
+    std::vector<double> VX, VW, FX, FW;
+    std::vector<double> vpsi, vgpsi, fpsi;   // reference basis at volume nodes / face f nodes
+};
+
+static void hllc_flux(int D, double gamma, double MS, const double* Ui, const double* Uo, const double* n, double* F) {
+    double rIn = Ui[0], rOut = Uo[0];
+    double m2In = 0, m2Out = 0, vnIn = 0, vnOut = 0;
+    for (int d = 0; d < D; ++d) {
+        m2In += Ui[d + 1] * Ui[d + 1];
+        m2Out += Uo[d + 1] * Uo[d + 1];
+        vnIn += Ui[d + 1] * n[d];
+        vnOut += Uo[d + 1] * n[d];
+    }
+    vnIn /= rIn;
+    vnOut /= rOut;
+    double EIn = Ui[D + 1], EOut = Uo[D + 1];
+    double pIn = (gamma - 1.0) * (EIn - MS * 0.5 * m2In / rIn);
+    double pOut = (gamma - 1.0) * (EOut - MS * 0.5 * m2Out / rOut);
+    double Mach = std::sqrt(MS / gamma);
+    double aIn = std::sqrt(pIn / rIn) / Mach, aOut = std::sqrt(pOut / rOut) / Mach;
+    double rMean = 0.5 * (rIn + rOut), pMean = 0.5 * (pIn + pOut), aMean = 0.5 * (aIn + aOut);
+    double jump = vnOut - vnIn;
+    double pStar = std::max(0.0, pMean - 0.5 * MS * jump * aMean * rMean);
+    double qIn = pStar > pIn ? std::sqrt(1.0 + 0.5 * (gamma + 1.0) * (pStar / pIn - 1.0) / gamma) : 1.0;
+    double qOut = pStar > pOut ? std::sqrt(1.0 + 0.5 * (gamma + 1.0) * (pStar / pOut - 1.0) / gamma) : 1.0;
+    double sIn = vnIn - aIn * qIn, sOut = vnOut + aOut * qOut;
+    double cIn = rIn * (sIn - vnIn), cOut = rOut * (sOut - vnOut);
+    double sStar = (cOut * vnOut - cIn * vnIn + (pIn - pOut) / MS) / (cOut - cIn);
+    if (sStar > 0.0) {
+        F[0] = vnIn * rIn;
+        for (int c = 0; c < D; ++c) F[c + 1] = vnIn * Ui[c + 1] + pIn / MS * n[c];
+        F[D + 1] = vnIn * (EIn + pIn);
+        if (sIn <= 0.0) {
+            double f = (sIn - vnIn) / (sIn - sStar);
+            F[0] += sIn * (rIn * f - rIn);
+            for (int c = 0; c < D; ++c)
+                F[c + 1] += sIn * (rIn * f * (Ui[c + 1] / rIn + (sStar - vnIn) * n[c]) - Ui[c + 1]);
+            double factor = rIn * sStar * MS + pIn / (sIn - vnIn);
+            F[D + 1] += sIn * (f * (EIn + factor * (sStar - vnIn)) - EIn);
+        }
+    } else {
+        F[0] = vnOut * rOut;
+        for (int c = 0; c < D; ++c) F[c + 1] = vnOut * Uo[c + 1] + pOut / MS * n[c];
+        F[D + 1] = vnOut * (EOut + pOut);
+        if (sOut >= 0.0) {
+            double f = (sOut - vnOut) / (sOut - sStar);
+            F[0] += sOut * (rOut * f - rOut);
+            for (int c = 0; c < D; ++c)
+                F[c + 1] += sOut * (rOut * f * (Uo[c + 1] / rOut + (sStar - vnOut) * n[c]) - Uo[c + 1]);
+            double factor = rOut * sStar * MS + pOut / (sOut - vnOut);
+            F[D + 1] += sOut * (f * (EOut + factor * (sStar - vnOut)) - EOut);
+        }
+    }
+}
+
+// physical flux F[var][d] (OptimizedHLLC*Flux.Flux)
+static void euler_flux(int D, double gamma, double MS, const double* U, double* F) {
+    double r = U[0], E = U[D + 1], m2 = 0;
+    for (int d = 0; d < D; ++d) m2 += U[d + 1] * U[d + 1];
+    double p = (gamma - 1.0) * (E - MS * 0.5 * m2 / r);
+    for (int d = 0; d < D; ++d) F[0 * D + d] = U[d + 1];
+    for (int c = 0; c < D; ++c)
+        for (int d = 0; d < D; ++d) F[(c + 1) * D + d] = U[c + 1] * U[d + 1] / r + (c == d ? p / MS : 0.0);
+    for (int d = 0; d < D; ++d) F[(D + 1) * D + d] = U[d + 1] / r * (E + p);
+}
+
+Euler* euler_create(const Grid* g, int k, int nq1, double gamma, double mach) {
+    Euler* e = new Euler();
+    e->g = g; e->D = g->D; e->k = k; e->Np = basis_count(g->D, k); e->nq1 = nq1; e->nv = g->D + 2;
+    e->gamma = gamma; e->MS = gamma * mach * mach;
+    tensor_rule(e->D, nq1, e->VX, e->VW);
+    tensor_rule(e->D - 1, nq1, e->FX, e->FW);
+    e->nqv = (int)e->VW.size(); e->nqf = (int)e->FW.size();
+    int D = e->D, Np = e->Np;
+    e->vpsi.resize((size_t)e->nqv * Np); e->vgpsi.resize((size_t)e->nqv * Np * D);
+    for (int q = 0; q < e->nqv; ++q) basis_eval(D, k, &e->VX[q * D], &e->vpsi[(size_t)q * Np], &e->vgpsi[(size_t)q * Np * D]);
+    e->fpsi.resize((size_t)2 * D * e->nqf * Np);
+    std::vector<double> gtmp((size_t)Np * D);
+    for (int f = 0; f < 2 * D; ++f)
+        for (int q = 0; q < e->nqf; ++q) {
+            double xi[3];
+            face_node_xi(D, f / 2, f % 2, &e->FX[q * (D - 1 > 0 ? D - 1 : 1)], xi);
+            basis_eval(D, k, xi, &e->fpsi[((size_t)f * e->nqf + q) * Np], gtmp.data());
+        }
+    return e;
+}
+
+static int periodic_nb(const Grid& g, int c, int d, int s) {
+    int idx[3] = {g.ijk[3 * c], g.ijk[3 * c + 1], g.ijk[3 * c + 2]};
+    idx[d] = (idx[d] + (s ? 1 : -1) + g.n[d]) % g.n[d];
+    return g.cell_index(idx[0], idx[1], idx[2]);
+}
+
+void euler_rhs(const Euler& e, const double* U, double* R) {
+    const Grid& g = *e.g;
+    int D = e.D, Np = e.Np, nv = e.nv, blk = nv * Np;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int j = 0; j < g.J; ++j) {
+        double res[5 * 64];
+        std::fill(res, res + blk, 0.0);
+        double detJ = 1;
+        for (int d = 0; d < D; ++d) detJ *= 0.5 * g.h[D * j + d];
+        double sc = 1.0 / std::sqrt(detJ);
+        const double* Uj = U + (size_t)j * blk;
+        for (int q = 0; q < e.nqv; ++q) {
+            const double* psi = &e.vpsi[(size_t)q * Np];
+            const double* gpsi = &e.vgpsi[(size_t)q * Np * D];
+            double Uq[5] = {0, 0, 0, 0, 0};
+            for (int c = 0; c < nv; ++c)
+                for (int n = 0; n < Np; ++n) Uq[c] += Uj[c * Np + n] * psi[n] * sc;
+            double F[5 * 3];
+            euler_flux(D, e.gamma, e.MS, Uq, F);
+            double w = e.VW[q] * detJ;
+            for (int c = 0; c < nv; ++c)
+                for (int m = 0; m < Np; ++m) {
+                    double a = 0;
+                    for (int d = 0; d < D; ++d) a += F[c * D + d] * gpsi[m * D + d] * sc * 2.0 / g.h[D * j + d];
+                    res[c * Np + m] += w * a;
+                }
+        }
+        for (int f = 0; f < 2 * D; ++f) {
+            int d = f / 2, s = f % 2;
+            int nb = periodic_nb(g, j, d, s);
+            double detN = 1;
+            for (int dd = 0; dd < D; ++dd) detN *= 0.5 * g.h[D * nb + dd];
+            double scN = 1.0 / std::sqrt(detN);
+            double dA = detJ / (0.5 * g.h[D * j + d]);
+            double nrm[3] = {0, 0, 0};
+            nrm[d] = s ? 1.0 : -1.0;
+            const double* Un = U + (size_t)nb * blk;
+            int fo = 2 * d + (1 - s);
+            for (int q = 0; q < e.nqf; ++q) {
+                const double* pi = &e.fpsi[((size_t)f * e.nqf + q) * Np];
+                const double* po = &e.fpsi[((size_t)fo * e.nqf + q) * Np];
+                double Ui[5] = {0, 0, 0, 0, 0}, Uo[5] = {0, 0, 0, 0, 0};
+                for (int c = 0; c < nv; ++c)
+                    for (int n = 0; n < Np; ++n) { Ui[c] += Uj[c * Np + n] * pi[n] * sc; Uo[c] += Un[c * Np + n] * po[n] * scN; }
+                double Fh[5];
+                hllc_flux(D, e.gamma, e.MS, Ui, Uo, nrm, Fh);
+                double w = e.FW[q] * dA;
+                for (int c = 0; c < nv; ++c)
+                    for (int m = 0; m < Np; ++m) res[c * Np + m] -= w * Fh[c] * pi[m] * sc;
+            }
+        }
+        std::copy(res, res + blk, R + (size_t)j * blk);
+    }
+}
+
+// Butcher form of RungeKuttaScheme.TVD3 (L3-solution/BoSSS.Solution/TimeStepping/
+// RungeKuttaScheme.cs:145-167): a21 = 1, a31 = a32 = 1/4, b = (1/6, 1/6, 2/3).
+void euler_tvd3(const Euler& e, double* U, double dt, int steps) {
+    size_t n = (size_t)e.g->J * e.nv * e.Np;
+    std::vector<double> k1(n), k2(n), k3(n), t(n);
+    for (int s = 0; s < steps; ++s) {
+        euler_rhs(e, U, k1.data());
+        for (size_t i = 0; i < n; ++i) t[i] = U[i] + dt * k1[i];
+        euler_rhs(e, t.data(), k2.data());
+        for (size_t i = 0; i < n; ++i) t[i] = U[i] + dt * 0.25 * (k1[i] + k2[i]);
+        euler_rhs(e, t.data(), k3.data());
+        for (size_t i = 0; i < n; ++i) U[i] += dt * (k1[i] / 6.0 + k2[i] / 6.0 + 2.0 * k3[i] / 3.0);
+    }
+}
+
 }  // namespace orc
 
 // ================================================================== C API ==
@@ -1199,6 +1377,14 @@ int orc_pmg_vcycle(void* mp, void* gp, const int* deg, int ndeg, int min_coarse,
         return 0;
     } catch (...) { return -1; }
 }
+void* orc_euler_create(void* g, int k, int nq1, double gamma, double mach) { return euler_create((Grid*)g, k, nq1, gamma, mach); }
+void orc_euler_free(void* e) { delete (Euler*)e; }
+void orc_euler_rhs(void* e, const double* U, double* R) { euler_rhs(*(Euler*)e, U, R); }
+void orc_euler_tvd3(void* e, double* U, double dt, int steps) { euler_tvd3(*(Euler*)e, U, dt, steps); }
+void orc_hllc(int D, double gamma, double mach, const double* Ui, const double* Uo, const double* n, double* F) {
+    hllc_flux(D, gamma, gamma * mach * mach, Ui, Uo, n, F);
+}
+void orc_euler_flux(int D, double gamma, double mach, const double* U, double* F) { euler_flux(D, gamma, gamma * mach * mach, U, F); }
 int orc_num_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();
