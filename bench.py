@@ -111,6 +111,142 @@ def solve_leg(B, ctx, grid, op, n, k, out):
     out["time_to_solve"] = res
 
 
+def xdg_leg(B, ctx, out, n=48, seed=3):
+    """Two-phase XDG Poisson, sphere in a 48^3 grid (k = 2, diffusion 1 inside /
+    1000 outside, agglomeration threshold 0.1): set-up (cut-cell rules,
+    device assembly, agglomeration), variable-block SpMV and the solve to 1e-8
+    with FGMRES + FP32 two-level preconditioner."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.4, 0.6, 3)
+    r = float(rng.uniform(0.2, 0.3))
+    x = np.linspace(0.0, 1.0, n + 1)
+    g = B.Grid(ctx, x, x, x)
+    t0 = time.perf_counter()
+    X = B.XdgPoisson(g, 2, c, r, 1.0, 1000.0, agg_threshold=0.1)
+    ctx.sync()
+    t_setup = time.perf_counter() - t0
+    X.set_rhs([1.0, 1.0], None)
+    res = {"grid": f"{n}^3", "center": c.tolist(), "radius": r, "cut_cells": X.ncut, "agglomeration_pairs": X.npairs,
+           "dof_full": X.N_full, "dof_agglomerated": X.N, "cut_rule_nodes": X.nnodes, "setup_s": t_setup}
+    # variable-block SpMV on the agglomerated matrix (10 / 20 DOF blocks)
+    for name, prec, dt in (("spmv_fp64", B.BH_FP64, np.float64), ("spmv_fp32", B.BH_FP32, np.float32)):
+        xv = B.DeviceArray.from_host(ctx, rng.standard_normal(X.N).astype(dt))
+        yv = B.DeviceArray(ctx, X.N, dt)
+        for _ in range(3):
+            X.SpMV(1.0, xv, 0.0, yv, which=1, prec=prec)
+        ctx.sync()
+        reps = 50
+        ctx.record(2)
+        for _ in range(reps):
+            X.SpMV(1.0, xv, 0.0, yv, which=1, prec=prec)
+        ctx.record(3)
+        ctx.sync()
+        ms = ctx.elapsed_ms(2, 3) / reps
+        es = np.dtype(dt).itemsize
+        nbytes = X.nvals * es + 2 * X.N * es
+        res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "hbm_frac": nbytes / (ms * 1e-3) / PEAK_HBM}
+        del xv, yv
+    b = X.rhs(1)
+    for name, prec in (("solve_fp32_precond", B.BH_FP32), ("solve_fp64_precond", B.BH_FP64)):
+        t0 = time.perf_counter()
+        P = B.XdgPrecond(X, coarse_degree=1, prec=prec)
+        ctx.sync()
+        t_init = time.perf_counter() - t0
+        S = B.XdgFlexGMRES(X, P, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=1000)
+        u = np.zeros(X.N)
+        t0 = time.perf_counter()
+        S.Solve(u, b)
+        t_run = time.perf_counter() - t0
+        st = S.stats()
+        yv = X.SpMV(1.0, u, 0.0, np.zeros(X.N), which=1)
+        res[name] = {"precond_init_s": t_init, "solver_run_s": t_run, "iterations": st["iterations"],
+                     "converged": st["converged"],
+                     "true_rel_residual": float(np.linalg.norm(b - yv) / np.linalg.norm(b))}
+        del S, P
+    out["xdg"] = res
+
+
+def euler_initial_state(n, k, seed, gamma=1.4, mach=0.2, nmodes=8, kmax=3):
+    """Conserved-variable DG coefficients [cell][rho, m_x, m_y, m_z, E][mode] on the
+    periodic n^3 grid of [0,1]^3: density 1 + 0.1 * (8 seeded Fourier modes,
+    normalised), a seeded divergence-free Fourier velocity field scaled to
+    max |u| = mach (sound speed 1), pressure 1/gamma; L2 projection with 4 Gauss
+    points per direction, one z-layer of cells at a time."""
+    from math import comb
+    rng = np.random.default_rng(seed)
+    kv = rng.integers(-kmax, kmax + 1, (nmodes, 3))
+    kv[np.all(kv == 0, axis=1)] = [1, 0, 0]
+    A = rng.standard_normal(nmodes)
+    ph = rng.uniform(0, 2 * np.pi, nmodes)
+    av = rng.standard_normal((nmodes, 3))
+    dirs = np.cross(av, kv)                         # (a x k) . k = 0: divergence-free
+    psi = rng.uniform(0, 2 * np.pi, nmodes)
+    xg, wg = np.polynomial.legendre.leggauss(4)
+    L = legendre_1d(k, xg)
+    ex = np.array([(a_, b_, d - a_ - b_) for d in range(k + 1) for a_ in range(d + 1) for b_ in range(d - a_ + 1)])
+    Np = comb(k + 3, 3)
+    # reference basis at the 64 tensor nodes, node order (qx fastest)
+    qi = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(4), indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    Psi = L[qi[:, 0]][:, ex[:, 0]] * L[qi[:, 1]][:, ex[:, 1]] * L[qi[:, 2]][:, ex[:, 2]]   # (64, Np)
+    W = wg[qi[:, 0]] * wg[qi[:, 1]] * wg[qi[:, 2]]
+    h = 1.0 / n
+    sc = np.sqrt((h / 2) ** 3)
+    PW = Psi * W[:, None] * sc                     # (64, Np)
+    c1 = np.arange(n) * h
+    # scale factors from a coarse sampling of the fields
+    xs = np.random.default_rng(seed + 1).random((20000, 3))
+    srho = np.abs(np.sin(2 * np.pi * xs @ kv.T + ph) @ A).max()
+    sv = np.sqrt(((np.sin(2 * np.pi * xs @ kv.T + psi) @ dirs) ** 2).sum(1)).max()
+    U = np.zeros((n, n, n, 5, Np))
+    for kz in range(n):
+        X = np.zeros((n, n, 64, 3))
+        X[..., 0] = c1[None, :, None] + 0.5 * (xg[qi[:, 0]] + 1) * h      # [j, i, q]
+        X[..., 1] = c1[:, None, None] + 0.5 * (xg[qi[:, 1]] + 1) * h
+        X[..., 2] = kz * h + 0.5 * (xg[qi[:, 2]] + 1) * h
+        rho = 1.0 + 0.1 * (np.sin(2 * np.pi * X @ kv.T + ph) @ A) / srho
+        u = (np.sin(2 * np.pi * X @ kv.T + psi) @ dirs) * (mach / sv)
+        p = 1.0 / gamma
+        E = p / (gamma - 1) + 0.5 * rho * (u ** 2).sum(-1)
+        F = np.stack([rho, rho * u[..., 0], rho * u[..., 1], rho * u[..., 2], E], -2)   # [j, i, 5, q]
+        U[kz] = F @ PW
+    return U.ravel()
+
+
+def euler_leg(B, ctx, out, n=96, k=2, steps=20, seed=4):
+    """Explicit DG Euler / HLLC, periodic 96^3, k = 2 (44.2M DOF): 20 TVD3 steps in
+    FP64 and FP32 from the same state; throughput and FP32-vs-FP64 difference."""
+    x = np.linspace(0.0, 1.0, n + 1)
+    g = B.Grid(ctx, x, x, x)
+    e = B.EulerOperator(g, k=k, nq1=4, gamma=1.4)
+    t0 = time.perf_counter()
+    U0 = euler_initial_state(n, k, seed)
+    t_init = time.perf_counter() - t0
+    dt = 0.3 * (1.0 / n) / ((2 * k + 1) * 1.2)
+    res = {"grid": f"{n}^3 periodic", "k": k, "dof": int(e.N), "steps": steps, "dt": dt, "initial_projection_s": t_init}
+    finals = {}
+    for name, prec, dtp in (("fp64", B.BH_FP64, np.float64), ("fp32", B.BH_FP32, np.float32)):
+        dU = B.DeviceArray.from_host(ctx, U0.astype(dtp))
+        e.tvd3(dU, dt, 1, prec=prec)      # warm-up step (then restart from U0)
+        dU.upload(U0.astype(dtp))
+        ctx.sync()
+        ctx.record(4)
+        e.tvd3(dU, dt, steps, prec=prec)
+        ctx.record(5)
+        ctx.sync()
+        ms = ctx.elapsed_ms(4, 5)
+        finals[name] = dU.download().astype(np.float64)
+        res[name] = {"ms_total": ms, "ms_per_step": ms / steps, "dof_steps_per_s": e.N * steps / (ms * 1e-3),
+                     "finite": bool(np.isfinite(finals[name]).all())}
+        del dU
+    res["fp32_vs_fp64_rel_l2"] = float(np.linalg.norm(finals["fp32"] - finals["fp64"]) / np.linalg.norm(finals["fp64"]))
+    # conservation of the cell-mean totals over the run (FP64)
+    t0_ = U0.reshape(-1, 5, e.Np)[:, :, 0].sum(0)
+    t1_ = finals["fp64"].reshape(-1, 5, e.Np)[:, :, 0].sum(0)
+    res["fp64_conservation_rel"] = float(np.abs(t1_ - t0_).max() / np.abs(t0_).max())
+    out["euler"] = res
+
+
 def cpu_baseline(k, target_s=12.0):
     """Oracle (CPU restatement of the reference's quadrature evaluate) on a
     bounded sample: 3D 24^3 cells, same k and coefficient law."""
@@ -142,6 +278,8 @@ def main():
     ap.add_argument("--k", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-xdg", action="store_true")
+    ap.add_argument("--no-euler", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU kernels; the multi-GPU "
@@ -204,6 +342,10 @@ def main():
     }
     if not args.no_solve:
         solve_leg(B, ctx, grid, op, n, k, out)
+    if not args.no_xdg:
+        xdg_leg(B, ctx, out)
+    if not args.no_euler:
+        euler_leg(B, ctx, out)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(k)
     print(json.dumps(out), flush=True)

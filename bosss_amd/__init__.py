@@ -87,6 +87,20 @@ SIGNATURES = {
     "bh_euler_info": (_c_i, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i)]),
     "bh_euler_rhs": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i]),
     "bh_euler_tvd3": (_c_i, [_c_p, _c_p, _c_d, _c_i, _c_i, _c_i]),
+    "bh_xdg_create": (_c_i, [_c_p, _c_i, _c_p, _c_d, _c_d, _c_d, _c_d, _c_d, _c_i, _PP]),
+    "bh_xdg_destroy": (_c_i, [_c_p]),
+    "bh_xdg_info": (_c_i, [_c_p, _c_p]),
+    "bh_xdg_cells": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p]),
+    "bh_xdg_agglomeration": (_c_i, [_c_p, _c_p]),
+    "bh_xdg_elements": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "bh_xdg_matrix": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "bh_xdg_bnd_nodes": (_c_i, [_c_p, _c_p]),
+    "bh_xdg_set_rhs": (_c_i, [_c_p, _c_p, _c_p]),
+    "bh_xdg_rhs": (_c_i, [_c_p, _c_i, _c_p, _c_i]),
+    "bh_xdg_spmv": (_c_i, [_c_p, _c_i, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i]),
+    "bh_xdg_extend": (_c_i, [_c_p, _c_p, _c_p]),
+    "bh_xdg_precond_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
+    "bh_xdg_fgmres_create": (_c_i, [_c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
     "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
                                ctypes.POINTER(_c_d)]),
@@ -216,7 +230,7 @@ class _Owned:
 
     def destroy(self):
         h = getattr(self, "h", None)
-        if h and self._ctx_ref.h:
+        if h and (self._ctx_ref.h or self._ctx_ref is HOST):
             getattr(lib(), self._destroy_fn)(h)
         self.h = None
 
@@ -264,12 +278,27 @@ class Context:
             pass
 
 
+class _HostOnly:
+    """Stand-in context of host-only objects (grid + XDG set-up without a GPU)."""
+
+    h = None
+
+    def __init__(self):
+        self._children = []
+
+
+HOST = _HostOnly()
+
+
 class Grid(_Owned):
-    """Cartesian grid (reference numbering; see bh_grid_cartesian)."""
+    """Cartesian grid (reference numbering; see bh_grid_cartesian).  With
+    ``ctx=None`` the grid is host-only: it serves the XDG set-up (cut-cell
+    rules, agglomeration) and nothing that runs on the GPU."""
 
     _destroy_fn = "bh_grid_destroy"
 
     def __init__(self, ctx, *nodes):
+        ctx = HOST if ctx is None else ctx
         self.ctx = ctx
         self.D = len(nodes)
         ns = [np.ascontiguousarray(a, np.float64) for a in nodes] + [None] * (3 - len(nodes))
@@ -560,3 +589,135 @@ class EulerOperator(_Owned):
         up, fu = _vec_args(U)
         _check(lib().bh_euler_tvd3(self.h, up, dt, steps, prec, fu))
         return U
+
+
+class XdgPoisson(_Owned):
+    """Two-phase SIP Poisson with a spherical interface (bh_xdg_*): the
+    XLaplace_Bulk / XLaplace_Interface operator of XdgPoisson3
+    (L3-solution/BoSSS.Solution.XNSECommon/XLaplace.cs:50-346) on cut cells,
+    with cell agglomeration (X/AgglomerationAlgorithm.cs)."""
+
+    _destroy_fn = "bh_xdg_destroy"
+
+    def __init__(self, grid, k, center, radius, nu_in, nu_out, penalty_base=1.3, agg_threshold=0.1, quad_order=0):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self.k = k
+        c = np.zeros(3)
+        c[:grid.D] = center
+        h = ctypes.c_void_p()
+        _check(lib().bh_xdg_create(grid.h, k, c.ctypes.data, radius, nu_in, nu_out, penalty_base, agg_threshold,
+                                   quad_order, ctypes.byref(h)))
+        self._register(self.ctx, h.value)
+        info = np.zeros(12, np.int64)
+        _check(lib().bh_xdg_info(self.h, info.ctypes.data))
+        (self.N_full, self.N, self.ncut, self.npairs, self.nfailed, self.nelem, self.nnodes, self.nvals_full,
+         self.nvals, self.nbnd, self.Np, self.J) = [int(v) for v in info]
+
+    def cells(self):
+        J = self.J
+        m = np.zeros(J, np.int32)
+        vol = np.zeros((2, J))
+        ia = np.zeros(J)
+        ln = np.zeros((2, J))
+        _check(lib().bh_xdg_cells(self.h, m.ctypes.data, vol.ctypes.data, ia.ctypes.data, ln.ctypes.data))
+        return dict(spmask=m, vol=vol, iface_area=ia, len=ln)
+
+    def agglomeration(self):
+        p = np.zeros((self.npairs, 3), np.int32)
+        if self.npairs:
+            _check(lib().bh_xdg_agglomeration(self.h, p.ctypes.data))
+        return p
+
+    def elements(self):
+        ints = np.zeros((self.nelem, 9), np.int32)
+        dbls = np.zeros((self.nelem, 3))
+        nx = np.zeros((self.nnodes, 3))
+        nw = np.zeros(self.nnodes)
+        nn = np.zeros((self.nnodes, 3))
+        _check(lib().bh_xdg_elements(self.h, ints.ctypes.data, dbls.ctypes.data, nx.ctypes.data, nw.ctypes.data,
+                                     nn.ctypes.data))
+        return dict(ints=ints, dbls=dbls, nx=nx, nw=nw, nn=nn)
+
+    def matrix(self, which=1):
+        """(dof_off, rowptr, col, voff, vals) of the full (0) or agglomerated (1) matrix."""
+        J = self.J
+        nv = self.nvals if which else self.nvals_full
+        off = np.zeros(J + 1, np.int64)
+        rp = np.zeros(J + 1, np.int64)
+        _check(lib().bh_xdg_matrix(self.h, which, off.ctypes.data, rp.ctypes.data, None, None, None))
+        nb = int(rp[-1])
+        col = np.zeros(nb, np.int32)
+        voff = np.zeros(nb + 1, np.int64)
+        vals = np.zeros(nv)
+        _check(lib().bh_xdg_matrix(self.h, which, None, None, col.ctypes.data, voff.ctypes.data, vals.ctypes.data))
+        return off, rp, col, voff, vals
+
+    def scipy_matrix(self, which=1):
+        import scipy.sparse as sp
+        off, rp, col, voff, vals = self.matrix(which)
+        rows, cols, data = [], [], []
+        for r in range(self.J):
+            R = int(off[r + 1] - off[r])
+            for p in range(rp[r], rp[r + 1]):
+                c = col[p]
+                C = int(off[c + 1] - off[c])
+                blk = vals[voff[p]:voff[p] + R * C].reshape(C, R).T
+                ii, jj = np.meshgrid(np.arange(R) + off[r], np.arange(C) + off[c], indexing="ij")
+                rows.append(ii.ravel()); cols.append(jj.ravel()); data.append(blk.ravel())
+        n = int(off[-1])
+        return sp.csr_matrix((np.concatenate(data), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+    def bnd_nodes(self):
+        x = np.zeros((self.nbnd, 3))
+        if self.nbnd:
+            _check(lib().bh_xdg_bnd_nodes(self.h, x.ctypes.data))
+        return x
+
+    def set_rhs(self, f, g=None):
+        f = np.ascontiguousarray(f, np.float64)
+        gp = None
+        if g is not None:
+            g = np.ascontiguousarray(g, np.float64)
+            assert g.shape == (self.nbnd,)
+            gp = g.ctypes.data
+        _check(lib().bh_xdg_set_rhs(self.h, f.ctypes.data, gp))
+
+    def rhs(self, which=1):
+        b = np.zeros(self.N if which else self.N_full)
+        _check(lib().bh_xdg_rhs(self.h, which, b.ctypes.data, 0))
+        return b
+
+    def SpMV(self, alpha, x, beta, y, which=1, prec=BH_FP64):
+        xp, fx = _vec_args(x)
+        yp, fy = _vec_args(y)
+        assert fx == fy
+        _check(lib().bh_xdg_spmv(self.h, which, alpha, xp, beta, yp, prec, fx))
+        return y
+
+    def extend(self, u):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros(self.N_full)
+        _check(lib().bh_xdg_extend(self.h, u.ctypes.data, out.ctypes.data))
+        return out
+
+
+class XdgPrecond(_Solver):
+    """Two-level preconditioner of the agglomerated XDG system (bh_xdg_precond_create)."""
+
+    def __init__(self, xdg, coarse_degree=1, min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2, prec=BH_FP32):
+        h = ctypes.c_void_p()
+        _check(lib().bh_xdg_precond_create(xdg.h, coarse_degree, min_coarse, omega, pre, post, kcycle_inner, prec,
+                                           ctypes.byref(h)))
+        super().__init__(xdg.ctx, h.value, xdg.N, (xdg,))
+
+
+class XdgFlexGMRES(_Solver):
+    """FGMRES on the agglomerated XDG system (bh_xdg_fgmres_create)."""
+
+    def __init__(self, xdg, Precond=None, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=10000):
+        h = ctypes.c_void_p()
+        _check(lib().bh_xdg_fgmres_create(xdg.h, Precond.h if Precond else None, MaxKrylovDim, ConvergenceCriterion,
+                                          MaxSolverIterations, ctypes.byref(h)))
+        super().__init__(xdg.ctx, h.value, xdg.N, (xdg, Precond))
+

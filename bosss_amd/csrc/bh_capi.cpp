@@ -8,24 +8,13 @@
 #include "bh_mg.hpp"
 #include "bh_mlp.hpp"
 #include "bh_euler.hpp"
+#include "bh_xdgdev.hpp"
 
 namespace bh {
 const std::string& last_error();
 }
 using namespace bh;
 
-#define BH_API_BEGIN try {
-#define BH_API_END                                   \
-    return 0;                                        \
-    }                                                \
-    catch (const bh::Error& e) {                     \
-        bh::set_last_error(e.what());                \
-        return e.code;                               \
-    }                                                \
-    catch (const std::exception& e) {                \
-        bh::set_last_error(e.what());                \
-        return -99;                                  \
-    }
 
 // ------------------------------------------------------------------ helpers
 namespace {
@@ -233,8 +222,10 @@ int bh_grid_cartesian(bh_ctx* ctx, int D, const double* x, int nx1, const double
     build_grid_edges(g.get());
     std::vector<double> w(3 * (size_t)g->nmax, 0.0);
     for (int d = 0; d < D; ++d) std::copy(g->width[d].begin(), g->width[d].end(), w.begin() + (size_t)d * g->nmax);
-    g->d_width.alloc(w.size() * sizeof(double));
-    ctx->upload(g->d_width.p, w.data(), w.size() * sizeof(double));
+    if (ctx) {   // a grid without context serves host-side set-up only (cut-cell rules, agglomeration)
+        g->d_width.alloc(w.size() * sizeof(double));
+        ctx->upload(g->d_width.p, w.data(), w.size() * sizeof(double));
+    }
     *out = g.release();
     BH_API_END
 }
@@ -777,3 +768,201 @@ int bh_solver_destroy(bh_solver* s) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ XDG two-phase Poisson
+namespace bh {
+void xdg_build(bh_xdg* X);
+void xdg_set_rhs(bh_xdg* X, const double* f, const double* g);
+void xdg_extend(const bh_xdg* X, const double* ured, double* ufull);
+}
+
+int bh_xdg_create(bh_grid* g, int k, const double* center, double radius, double nu_in, double nu_out,
+                  double penalty_base, double agg_threshold, int quad_order, bh_xdg** out) {
+    BH_API_BEGIN
+    BH_CHECK(g && center && out, "bh_xdg_create: null argument");
+    BH_CHECK(k >= 1 && k <= 3, "bh_xdg_create: degree must be 1..3");
+    BH_CHECK(radius > 0, "bh_xdg_create: radius must be positive");
+    BH_CHECK(agg_threshold >= 0 && agg_threshold < 0.5, "bh_xdg_create: agglomeration threshold must be in [0, 0.5)");
+    auto X = std::make_unique<bh_xdg>();
+    X->ctx = g->ctx;
+    X->grid = g;
+    X->P = std::make_unique<XdgProblem>();
+    XdgProblem& p = *X->P;
+    p.ctx = g->ctx;
+    p.g = g;
+    p.k = k;
+    p.q = quad_order > 0 ? quad_order : k + 1;
+    for (int d = 0; d < 3; ++d) p.S.c[d] = d < g->D ? center[d] : 0.0;
+    p.S.r = radius;
+    p.nu[0] = nu_in;
+    p.nu[1] = nu_out;
+    p.penalty_base = penalty_base;
+    p.agg_threshold = agg_threshold;
+    p.setup();
+    BH_CHECK(p.n_failed == 0, "bh_xdg_create: agglomeration failed for " + std::to_string(p.n_failed) + " cells");
+    if (X->ctx) xdg_build(X.get());
+    *out = X.release();
+    BH_API_END
+}
+
+int bh_xdg_destroy(bh_xdg* X) {
+    BH_API_BEGIN
+    if (X && X->ctx) (void)hipStreamSynchronize(X->ctx->stream);
+    delete X;
+    BH_API_END
+}
+
+int bh_xdg_info(bh_xdg* X, int64_t* info) {
+    BH_API_BEGIN
+    const XdgProblem& p = *X->P;
+    info[0] = p.N_full;
+    info[1] = p.N_red;
+    info[2] = p.ncut;
+    info[3] = (int64_t)p.agg_pairs.size() / 3;
+    info[4] = p.n_failed;
+    info[5] = (int64_t)p.elems.size();
+    info[6] = (int64_t)p.nw.size();
+    info[7] = X->Afull.nvals;
+    info[8] = X->Ared.nvals;
+    info[9] = (int64_t)X->bnd_w.size();
+    info[10] = p.Np;
+    info[11] = p.g->J;
+    BH_API_END
+}
+
+int bh_xdg_cells(bh_xdg* X, int* spmask, double* vol, double* iface_area, double* len) {
+    BH_API_BEGIN
+    const XdgProblem& p = *X->P;
+    const int J = p.g->J;
+    if (spmask) std::copy(p.spmask.begin(), p.spmask.end(), spmask);
+    for (int s = 0; s < 2; ++s) {
+        if (vol) std::copy(p.vol[s].begin(), p.vol[s].end(), vol + (size_t)s * J);
+        if (len) std::copy(p.len[s].begin(), p.len[s].end(), len + (size_t)s * J);
+    }
+    if (iface_area) std::copy(p.iface_area.begin(), p.iface_area.end(), iface_area);
+    BH_API_END
+}
+
+int bh_xdg_agglomeration(bh_xdg* X, int* pairs) {
+    BH_API_BEGIN
+    std::copy(X->P->agg_pairs.begin(), X->P->agg_pairs.end(), pairs);
+    BH_API_END
+}
+
+int bh_xdg_elements(bh_xdg* X, int* ints, double* dbls, double* nx, double* nw, double* nn) {
+    BH_API_BEGIN
+    const XdgProblem& p = *X->P;
+    for (size_t e = 0; e < p.elems.size(); ++e) {
+        const XElem& E = p.elems[e];
+        if (ints) {
+            int* o = ints + 9 * e;
+            o[0] = E.type; o[1] = E.cellP; o[2] = E.cellQ; o[3] = E.spP; o[4] = E.spQ;
+            o[5] = E.dir; o[6] = E.side; o[7] = E.r0; o[8] = E.rn;
+        }
+        if (dbls) { dbls[3 * e] = E.nuP; dbls[3 * e + 1] = E.nuQ; dbls[3 * e + 2] = E.mu; }
+    }
+    if (nx) std::copy(p.nx.begin(), p.nx.end(), nx);
+    if (nw) std::copy(p.nw.begin(), p.nw.end(), nw);
+    if (nn) std::copy(p.nn.begin(), p.nn.end(), nn);
+    BH_API_END
+}
+
+int bh_xdg_matrix(bh_xdg* X, int which, int64_t* dof_off, int64_t* rowptr, int* col, int64_t* voff, double* vals) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    BH_CHECK(which == 0 || which == 1, "bh_xdg_matrix: which must be 0 (full) or 1 (agglomerated)");
+    const Vbsr& M = which ? X->Ared : X->Afull;
+    if (dof_off) std::copy(M.roff.begin(), M.roff.end(), dof_off);
+    if (rowptr) std::copy(M.rowptr.begin(), M.rowptr.end(), rowptr);
+    if (col) std::copy(M.col.begin(), M.col.end(), col);
+    if (voff) std::copy(M.voff.begin(), M.voff.end(), voff);
+    if (vals) X->ctx->download(vals, M.vals.p, M.nvals * sizeof(double));
+    BH_API_END
+}
+
+int bh_xdg_bnd_nodes(bh_xdg* X, double* xyz) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    std::copy(X->bnd_x.begin(), X->bnd_x.end(), xyz);
+    BH_API_END
+}
+
+int bh_xdg_set_rhs(bh_xdg* X, const double* f, const double* g) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    BH_CHECK(f, "bh_xdg_set_rhs: source values needed");
+    xdg_set_rhs(X, f, g);
+    BH_API_END
+}
+
+int bh_xdg_rhs(bh_xdg* X, int which, double* b, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    const XdgProblem& p = *X->P;
+    if (which == 0) {
+        BH_CHECK(!(flags & BH_DEVICE_PTR), "bh_xdg_rhs: the full right-hand side is a host vector");
+        BH_CHECK((int64_t)X->rhs_full.size() == p.N_full, "bh_xdg_rhs: no right-hand side set");
+        std::copy(X->rhs_full.begin(), X->rhs_full.end(), b);
+    } else {
+        BH_CHECK((int64_t)X->rhs_full.size() == p.N_full, "bh_xdg_rhs: no right-hand side set");
+        if (flags & BH_DEVICE_PTR) BH_HIP(hipMemcpyAsync(b, X->rhs_red.p, p.N_red * sizeof(double), hipMemcpyDeviceToDevice, X->ctx->stream));
+        else X->ctx->download(b, X->rhs_red.p, p.N_red * sizeof(double));
+        X->ctx->sync();
+    }
+    BH_API_END
+}
+
+int bh_xdg_spmv(bh_xdg* X, int which, double alpha, const void* x, double beta, void* y, int prec, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    BH_CHECK(which == 0 || which == 1, "bh_xdg_spmv: which must be 0 (full) or 1 (agglomerated)");
+    BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_xdg_spmv: precision must be FP64 or FP32");
+    Vbsr& M = which ? X->Ared : X->Afull;
+    const bool dev = flags & BH_DEVICE_PTR;
+    const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
+    Staged sx(X->ctx, x, M.N * es, dev, true);
+    Staged sy(X->ctx, y, M.N * es, dev, beta != 0.0);
+    M.spmv(alpha, sx.dptr, beta, sy.dptr, prec);
+    sy.download(y, M.N * es, dev);
+    X->ctx->sync();
+    BH_API_END
+}
+
+int bh_xdg_extend(bh_xdg* X, const double* ured, double* ufull) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    xdg_extend(X, ured, ufull);
+    BH_API_END
+}
+
+int bh_xdg_precond_create(bh_xdg* X, int coarse_degree, int min_coarse, double omega, int pre, int post,
+                          int kcycle_inner, int prec, bh_solver** out) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_xdg_precond_create: precision must be FP64 or FP32");
+    BH_CHECK(pre >= 1 && post >= 0, "bh_xdg_precond_create: need pre >= 1, post >= 0");
+    BH_CHECK(coarse_degree >= 0 && coarse_degree <= X->P->k, "bh_xdg_precond_create: coarse degree out of range");
+    auto P = std::make_unique<XdgPrecond>();
+    P->A = &X->Ared;
+    P->prec = prec;
+    P->Np = X->P->Np;
+    P->omega = omega;
+    P->pre = pre;
+    P->post = post;
+    P->build(X->grid, coarse_degree, min_coarse, kcycle_inner);
+    *out = P.release();
+    BH_API_END
+}
+
+int bh_xdg_fgmres_create(bh_xdg* X, bh_solver* P, int restart, double tol, int maxit, bh_solver** out) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    BH_CHECK(restart >= 1 && restart <= 256, "GMRES restart length must be in 1..256");
+    LinOp A;
+    A.V = &X->Ared;
+    A.N = X->Ared.N;
+    if (P) BH_CHECK(P->N == A.N, "preconditioner size mismatch");
+    *out = new GmresSolver(A, P, restart, tol, maxit, true);
+    BH_API_END
+}
+

@@ -198,3 +198,17 @@ template <typename T> void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t l
 // w -= sum_k h[k] V_k  (h host array, m <= 128)
 void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
 }  // namespace bh
+
+// extern "C" wrappers: translate exceptions into error codes + last-error text
+#define BH_API_BEGIN try {
+#define BH_API_END                                   \
+    return 0;                                        \
+    }                                                \
+    catch (const bh::Error& e) {                     \
+        bh::set_last_error(e.what());                \
+        return e.code;                               \
+    }                                                \
+    catch (const std::exception& e) {                \
+        bh::set_last_error(e.what());                \
+        return -99;                                  \
+    }

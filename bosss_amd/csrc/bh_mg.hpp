@@ -58,6 +58,8 @@ struct PMultigrid : bh_solver {
     void attach_learned(struct Mlp* m);
     ~PMultigrid() override;
     void build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bool matrix_free);
+    // hierarchy over a Cartesian grid from an assembled matrix of degree degrees[0] (mfop: optional matrix-free level 0)
+    void build_grid(bh_ctx* ctx, bh_grid* grid, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bh_op* mfop);
     void solve_dev(double* x, const double* b) override;        // x <- x + V(b - A x)
     void precondition(double* x, const double* b) override;     // x <- V(b)
     template <typename T> void vcycle(int l);

@@ -107,16 +107,21 @@ static void projection_block(int D, int deg, const double* lof, const double* hf
 }
 
 void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bool mf) {
-    ctx = op->ctx;
-    N = op->N;
-    matrix_free = mf;
-    fine_op = op;
-    const int D = op->D;
     BH_CHECK(!degrees.empty() && degrees[0] == op->k, "multigrid: first degree must be the operator's degree");
+    BH_CHECK(Af->bs == op->Np && Af->nrows == op->grid->J, "multigrid: matrix does not match the operator");
+    fine_op = op;
+    build_grid(op->ctx, op->grid, Af, degrees, min_coarse, mf ? op : nullptr);
+}
+
+void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bh_op* mfop) {
+    ctx = c;
+    const int D = grid->D;
+    N = (int64_t)grid->J * basis_count(D, degrees[0]);
+    matrix_free = mfop != nullptr;
     for (size_t i = 1; i < degrees.size(); ++i)
         BH_CHECK(degrees[i] < degrees[i - 1] && degrees[i] >= 0, "multigrid: degrees must decrease");
-    BH_CHECK(Af->bs == op->Np && Af->nrows == op->grid->J, "multigrid: matrix does not match the operator");
-    const int J = op->grid->J;
+    BH_CHECK(Af->bs == basis_count(D, degrees[0]) && Af->nrows == grid->J, "multigrid: matrix does not match the grid");
+    const int J = grid->J;
     // p-levels
     for (size_t i = 0; i < degrees.size(); ++i) {
         MgLevel L;
@@ -124,11 +129,11 @@ void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, i
         L.deg = degrees[i];
         L.bs = basis_count(D, degrees[i]);
         L.N = (int64_t)J * L.bs;
-        for (int d = 0; d < D; ++d) L.n[d] = op->grid->n[d];
+        for (int d = 0; d < D; ++d) L.n[d] = grid->n[d];
         if (i == 0) {
             L.A = Af;
             L.ownA = false;
-            L.op = mf ? op : nullptr;
+            L.op = mfop;
         } else {
             L.A = make_bsr(ctx, J, L.bs, Af->h_rowptr, Af->h_col);
             L.ownA = true;
@@ -144,10 +149,10 @@ void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, i
         L0.h.resize((size_t)J * D);
         for (int c = 0; c < J; ++c) {
             int idx[3];
-            op->grid->cell_ijk(c, idx);
+            grid->cell_ijk(c, idx);
             for (int d = 0; d < D; ++d) {
-                L0.lo[(size_t)c * D + d] = op->grid->nodes[d][idx[d]];
-                L0.h[(size_t)c * D + d] = op->grid->width[d][idx[d]];
+                L0.lo[(size_t)c * D + d] = grid->nodes[d][idx[d]];
+                L0.h[(size_t)c * D + d] = grid->width[d][idx[d]];
             }
         }
     }
@@ -282,7 +287,7 @@ void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, i
         L.r.alloc(L.N * es);
     }
     if (kcycle_level >= (int)lv.size() - 1) kcycle_level = -1;   // the coarsest level is solved directly
-    if (kcycle_level > 0 && kin > 0) {
+    if (kcycle_level >= 0 && kin > 0) {
         MgLevel& K = lv[kcycle_level];
         K.kb.alloc(K.N * es);
         K.kv.alloc((size_t)(kin + 1) * K.N * es);

@@ -6,11 +6,14 @@
 
 namespace bh {
 
-// The operator a Krylov solver works on: an assembled BlockMsrMatrix or the
-// matrix-free SIP operator (linear part).
+struct Vbsr;
+
+// The operator a Krylov solver works on: an assembled BlockMsrMatrix, the
+// matrix-free SIP operator (linear part) or a variable-block (XDG) matrix.
 struct LinOp {
     bh_bsr* M = nullptr;
     bh_op* op = nullptr;
+    Vbsr* V = nullptr;
     int64_t N = 0;
     void apply(const double* x, double* y) const;                  // y = A x
     void residual(const double* b, const double* x, double* r) const;  // r = b - A x

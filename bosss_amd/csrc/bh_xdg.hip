@@ -1,0 +1,492 @@
+// XDG two-phase SIP Poisson on the device (gfx950): element-driven assembly
+// with per-cell quadrature rules, the variable-block matrix (10 / 20 DOF
+// blocks), its SpMV and block-Jacobi smoothing.
+//
+// Forms: XLaplace_Bulk (per species, SIPLaplace with species diffusion and
+// species length scales; L3-solution/BoSSS.Solution.XNSECommon/XLaplace.cs:50-200)
+// and XLaplace_Interface in SIP mode (XLaplace.cs:206-346), assembled as
+// XEvaluatorLinear does (X/XSpatialOperatorMk2_XEvaluators.cs:163) into a
+// block matrix over (cell, species) blocks.
+//
+// One wave per element.  Uncut cells and uncut faces use the tensor Gauss
+// rule (27 / 9 nodes for k = 2), cut cells and cut faces their own rules
+// (a few hundred nodes) -- both kinds are items of the same launch.  At each
+// node the lanes evaluate the orthonormal basis (and gradients) of both sides
+// into LDS, then every lane accumulates its (test, trial) entries of the
+// element matrix; the element matrix is added to the global blocks with FP64
+// atomics (elements sharing a block are few: the order changes round-off only).
+#include <cmath>
+
+#include "bh_internal.hpp"
+#include "bh_xdg.hpp"
+#include "bh_xdgdev.hpp"
+
+namespace bh {
+
+struct XArgs {
+    int D, k, Np, nq1;
+    int n0, n1, n2;
+    const double* __restrict__ nodes1;  // [3][nmax+1] grid node coordinates
+    int nstride;
+    const XElem* __restrict__ elems;
+    int nelem;
+    const double* __restrict__ nx;
+    const double* __restrict__ nw;
+    const double* __restrict__ nn;
+    const int* __restrict__ ex;         // [Np][D] exponents
+    const double* __restrict__ gx;      // 1D Gauss nodes / weights (nq1)
+    const double* __restrict__ gw;
+    const int* __restrict__ spmask;
+    const int64_t* __restrict__ rowptr;
+    const int* __restrict__ col;
+    const int64_t* __restrict__ voff;
+    const int* __restrict__ bsz;
+    double* __restrict__ vals;
+};
+
+__device__ __forceinline__ void cell_box(const XArgs& a, int c, double* lo, double* h) {
+    int idx[3];
+    if (a.D == 2) { idx[0] = c / a.n1; idx[1] = c - idx[0] * a.n1; idx[2] = 0; }
+    else { int sl = a.n0 * a.n1; idx[2] = c / sl; int r = c - idx[2] * sl; idx[1] = r / a.n0; idx[0] = r - idx[1] * a.n0; }
+    for (int d = 0; d < a.D; ++d) {
+        lo[d] = a.nodes1[d * a.nstride + idx[d]];
+        h[d] = a.nodes1[d * a.nstride + idx[d] + 1] - lo[d];
+    }
+}
+
+// basis value / gradient of all modes at physical x in cell (lo, h); lane-parallel over modes
+__device__ void basis_at(const XArgs& a, const double* lo, const double* h, const double* x, int m, double* val, double* grad) {
+    double P[3][8], dP[3][8];
+    double detJ = 1;
+    for (int d = 0; d < a.D; ++d) {
+        const double xi = 2.0 * (x[d] - lo[d]) / h[d] - 1.0;
+        detJ *= 0.5 * h[d];
+        P[d][0] = 1; dP[d][0] = 0;
+        if (a.k > 0) { P[d][1] = xi; dP[d][1] = 1; }
+        for (int n = 2; n <= a.k; ++n) {
+            P[d][n] = ((2.0 * n - 1.0) * xi * P[d][n - 1] - (n - 1.0) * P[d][n - 2]) / n;
+            dP[d][n] = n * P[d][n - 1] + xi * dP[d][n - 1];
+        }
+        for (int n = 0; n <= a.k; ++n) { const double s = sqrt(n + 0.5); P[d][n] *= s; dP[d][n] *= s; }
+    }
+    const double sc = 1.0 / sqrt(detJ);
+    const int* e = a.ex + m * a.D;
+    double v = sc;
+    for (int d = 0; d < a.D; ++d) v *= P[d][e[d]];
+    *val = v;
+    for (int d = 0; d < a.D; ++d) {
+        double g = sc * 2.0 / h[d];
+        for (int dd = 0; dd < a.D; ++dd) g *= (dd == d) ? dP[dd][e[dd]] : P[dd][e[dd]];
+        grad[d] = g;
+    }
+}
+
+__device__ __forceinline__ int64_t find_block(const XArgs& a, int r, int c) {
+    for (int64_t p = a.rowptr[r]; p < a.rowptr[r + 1]; ++p)
+        if (a.col[p] == c) return p;
+    return -1;
+}
+
+// position (0 / 1) of species sp inside cell c's block
+__device__ __forceinline__ int spos(const XArgs& a, int c, int sp) { return (sp == 1 && (a.spmask[c] & 1)) ? 1 : 0; }
+
+constexpr int XMAXNP = 20;
+
+__global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
+    __shared__ double sv[2][XMAXNP], sd[2][XMAXNP], sg[XMAXNP][3];
+    __shared__ double node[4];
+    const int lane = threadIdx.x;
+    const int Np = a.Np, D = a.D;
+    for (int e = blockIdx.x; e < a.nelem; e += gridDim.x) {
+        const XElem E = a.elems[e];
+        double loP[3], hP[3], loQ[3], hQ[3];
+        cell_box(a, E.cellP, loP, hP);
+        cell_box(a, E.cellQ, loQ, hQ);
+        const int nblk = E.type == 0 ? 1 : (E.type == 2 ? 1 : 4);
+        const int nent = nblk * Np * Np;
+        // entries owned by this lane: t = lane + 64 j
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        // normal (faces: +-e_dir; interface: per node)
+        int nnodes;
+        if (E.rn > 0) nnodes = E.rn;
+        else if (E.type == 0) { nnodes = 1; for (int d = 0; d < D; ++d) nnodes *= a.nq1; }
+        else { nnodes = 1; for (int d = 0; d < D - 1; ++d) nnodes *= a.nq1; }
+        for (int qn = 0; qn < nnodes; ++qn) {
+            // node coordinates and weight
+            if (lane == 0) {
+                double x[3] = {0, 0, 0}, w;
+                if (E.rn > 0) {
+                    for (int d = 0; d < 3; ++d) x[d] = a.nx[(size_t)(E.r0 + qn) * 3 + d];
+                    w = a.nw[E.r0 + qn];
+                } else if (E.type == 0) {
+                    int r = qn;
+                    w = 1;
+                    for (int d = 0; d < D; ++d) {
+                        const int id = r % a.nq1;
+                        r /= a.nq1;
+                        x[d] = loP[d] + 0.5 * (a.gx[id] + 1.0) * hP[d];
+                        w *= a.gw[id] * 0.5 * hP[d];
+                    }
+                } else {
+                    int r = qn;
+                    w = 1;
+                    for (int d = 0; d < D; ++d) {
+                        if (d == E.dir) {
+                            x[d] = (E.type == 2 && E.side == 0) ? loP[d] : loP[d] + hP[d];
+                            continue;
+                        }
+                        const int id = r % a.nq1;
+                        r /= a.nq1;
+                        x[d] = loP[d] + 0.5 * (a.gx[id] + 1.0) * hP[d];
+                        w *= a.gw[id] * 0.5 * hP[d];
+                    }
+                }
+                node[0] = x[0]; node[1] = x[1]; node[2] = x[2]; node[3] = w;
+            }
+            __syncthreads();
+            const double x[3] = {node[0], node[1], node[2]};
+            const double w = node[3];
+            double nrm[3] = {0, 0, 0};
+            if (E.type == 3) for (int d = 0; d < 3; ++d) nrm[d] = a.nn[(size_t)(E.r0 + qn) * 3 + d];
+            else if (E.type == 1) nrm[E.dir] = 1.0;
+            else if (E.type == 2) nrm[E.dir] = E.side ? 1.0 : -1.0;
+            // basis of both sides at the node
+            if (lane < 2 * Np) {
+                const int side = lane / Np, m = lane - side * Np;
+                double v, g[3];
+                basis_at(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
+                double dn = 0;
+                for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
+                sv[side][m] = v;
+                sd[side][m] = dn;
+                if (side == 0) for (int d = 0; d < 3; ++d) sg[m][d] = d < D ? g[d] : 0.0;
+            }
+            __syncthreads();
+            for (int j = 0; j < 7; ++j) {
+                const int t = lane + 64 * j;
+                if (t >= nent) break;
+                const int blk = t / (Np * Np), rem = t - blk * Np * Np;
+                const int m = rem / Np, n = rem - m * Np;   // test m, trial n
+                double val;
+                if (E.type == 0) {
+                    double s = 0;
+                    for (int d = 0; d < D; ++d) s += sg[m][d] * sg[n][d];
+                    val = -E.nuP * s;
+                } else if (E.type == 2) {
+                    // SIPLaplace.BoundaryEdgeForm, Dirichlet, linear part
+                    val = E.nuP * sd[0][n] * sv[0][m] + E.nuP * sd[0][m] * sv[0][n] - E.nuP * E.mu * sv[0][n] * sv[0][m];
+                } else {
+                    const double vP_m = sv[0][m], dP_m = sd[0][m], vQ_m = sv[1][m], dQ_m = sd[1][m];
+                    const double vP_n = sv[0][n], dP_n = sd[0][n], vQ_n = sv[1][n], dQ_n = sd[1][n];
+                    const double nP = E.nuP, nQ = E.nuQ, mu = E.mu;
+                    if (blk == 0) val = 0.5 * nP * dP_n * vP_m + 0.5 * nP * dP_m * vP_n - mu * vP_n * vP_m;        // PP
+                    else if (blk == 1) val = 0.5 * nQ * dQ_n * vP_m - 0.5 * nP * dP_m * vQ_n + mu * vQ_n * vP_m;  // test P, trial Q
+                    else if (blk == 2) val = -0.5 * nP * dP_n * vQ_m + 0.5 * nQ * dQ_m * vP_n + mu * vP_n * vQ_m; // test Q, trial P
+                    else val = -0.5 * nQ * dQ_n * vQ_m - 0.5 * nQ * dQ_m * vQ_n - mu * vQ_n * vQ_m;              // QQ
+                }
+                acc[j] += w * val;
+            }
+            __syncthreads();
+        }
+        // scatter into the global blocks
+        for (int j = 0; j < 7; ++j) {
+            const int t = lane + 64 * j;
+            if (t >= nent) break;
+            const int blk = t / (Np * Np), rem = t - blk * Np * Np;
+            const int m = rem / Np, n = rem - m * Np;
+            int rc, cc, rsp, csp;
+            if (E.type == 0 || E.type == 2) { rc = cc = E.cellP; rsp = csp = E.spP; }
+            else {
+                const bool testQ = blk >= 2, trialQ = (blk == 1 || blk == 3);
+                rc = testQ ? E.cellQ : E.cellP;
+                rsp = testQ ? E.spQ : E.spP;
+                cc = trialQ ? E.cellQ : E.cellP;
+                csp = trialQ ? E.spQ : E.spP;
+            }
+            const int64_t p = find_block(a, rc, cc);
+            if (p < 0) continue;
+            const int R = a.bsz[rc];
+            const int row = spos(a, rc, rsp) * Np + m, colj = spos(a, cc, csp) * Np + n;
+            atomicAdd(&a.vals[a.voff[p] + (int64_t)colj * R + row], acc[j]);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ variable-block SpMV
+// One wave per block row; row size R (<= 64) is wave-uniform: lanes = G groups
+// x R rows; each group walks the row's blocks column by column (blocks are
+// column-major, so the G groups read G*R consecutive values).
+template <typename T>
+__global__ __launch_bounds__(256) void vbsr_spmv_kernel(int nrows, const int* __restrict__ rsz, const int64_t* __restrict__ roff,
+                                                        const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+                                                        const int64_t* __restrict__ voff, const T* __restrict__ vals,
+                                                        const T* __restrict__ x, T* __restrict__ y, T alpha, T beta,
+                                                        const T* __restrict__ bsub) {
+    __shared__ T red[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wave;
+    const bool active = r < nrows;
+    const int R = active ? rsz[r] : 1;
+    const int G = R > 0 ? 64 / R : 1;
+    const int g = R > 0 ? lane / R : 64, ri = R > 0 ? lane - g * R : 0;
+    T acc = 0;
+    if (active && R > 0 && g < G) {
+        for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+            const int c = col[p];
+            const int C = rsz[c];
+            const T* __restrict__ blk = vals + voff[p];
+            const T* __restrict__ xc = x + roff[c];
+            for (int cc = g; cc < C; cc += G) acc += blk[(int64_t)cc * R + ri] * xc[cc];
+        }
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (active && R > 0 && g == 0) {
+        T s = acc;
+        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * R + ri];
+        const int64_t o = roff[r] + ri;
+        T out = alpha * s;
+        if (bsub) out = bsub[o] - s;
+        else if (beta != T(0)) out += beta * y[o];
+        y[o] = out;
+    }
+}
+
+// batched Gauss-Jordan of the (variable-size) diagonal blocks
+__global__ __launch_bounds__(64) void vbsr_invert_kernel(int nrows, const int* __restrict__ rsz, const int64_t* __restrict__ diag,
+                                                         const int64_t* __restrict__ voff, const int64_t* __restrict__ doff,
+                                                         const double* __restrict__ vals, double* __restrict__ inv,
+                                                         float* __restrict__ inv32, int* __restrict__ status) {
+    extern __shared__ double M[];
+    const int r = blockIdx.x, lane = threadIdx.x;
+    const int bs = rsz[r];
+    if (bs == 0) return;
+    const int W = 2 * bs;
+    const int64_t bi = diag[r];
+    for (int i = 0; i < bs; ++i)
+        for (int j = lane; j < W; j += 64)
+            M[i * W + j] = j < bs ? (bi >= 0 ? vals[voff[bi] + (int64_t)j * bs + i] : 0.0) : ((j - bs == i) ? 1.0 : 0.0);
+    __syncthreads();
+    for (int c = 0; c < bs; ++c) {
+        int p = c;
+        double best = fabs(M[c * W + c]);
+        for (int i = c + 1; i < bs; ++i) { const double v = fabs(M[i * W + c]); if (v > best) { best = v; p = i; } }
+        if (best == 0.0) { if (lane == 0) status[0] = r + 1; return; }
+        __syncthreads();
+        if (p != c) for (int j = lane; j < W; j += 64) { const double t = M[p * W + j]; M[p * W + j] = M[c * W + j]; M[c * W + j] = t; }
+        __syncthreads();
+        const double piv = 1.0 / M[c * W + c];
+        __syncthreads();
+        for (int j = lane; j < W; j += 64) M[c * W + j] *= piv;
+        __syncthreads();
+        for (int i = 0; i < bs; ++i) {
+            if (i == c) continue;
+            const double f = M[i * W + c];
+            __syncthreads();
+            for (int j = lane; j < W; j += 64) M[i * W + j] -= f * M[c * W + j];
+            __syncthreads();
+        }
+    }
+    for (int i = 0; i < bs; ++i)
+        for (int j = lane; j < bs; j += 64) {
+            const double v = M[i * W + bs + j];
+            inv[doff[r] + (int64_t)j * bs + i] = v;
+            if (inv32) inv32[doff[r] + (int64_t)j * bs + i] = (float)v;
+        }
+}
+
+// x (+)= omega * Dinv r per variable block
+template <typename T>
+__global__ __launch_bounds__(256) void vblock_gemv_kernel(int nrows, const int* __restrict__ rsz, const int64_t* __restrict__ roff,
+                                                          const int64_t* __restrict__ doff, const T* __restrict__ Dinv,
+                                                          const T* __restrict__ r, T* __restrict__ x, T omega, int add) {
+    __shared__ T red[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    const bool active = row < nrows;
+    const int R = active ? rsz[row] : 1;
+    const int G = R > 0 ? 64 / R : 1;
+    const int g = R > 0 ? lane / R : 64, ri = R > 0 ? lane - g * R : 0;
+    T acc = 0;
+    if (active && R > 0 && g < G) {
+        const T* Di = Dinv + doff[row];
+        const T* rr = r + roff[row];
+        for (int c = g; c < R; c += G) acc += Di[(int64_t)c * R + ri] * rr[c];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (active && R > 0 && g == 0) {
+        T s = acc;
+        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * R + ri];
+        const int64_t o = roff[row] + ri;
+        x[o] = add ? x[o] + omega * s : omega * s;
+    }
+}
+
+// species-merged coarse space: rc(c, m) = sum over the species blocks of cell c of r(c, s, m), m < Npc
+template <typename T>
+__global__ void xdg_restrict_k(int nrows, int Np, int Npc, const int* __restrict__ rsz, const int64_t* __restrict__ roff,
+                               const T* __restrict__ r, T* __restrict__ rc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrows * Npc) return;
+    const int c = (int)(t / Npc), m = (int)(t - (int64_t)c * Npc);
+    const int ns = rsz[c] / Np;
+    T s = 0;
+    for (int q = 0; q < ns; ++q) s += r[roff[c] + q * Np + m];
+    rc[t] = s;
+}
+
+template <typename T>
+__global__ void xdg_prolong_k(int nrows, int Np, int Npc, const int* __restrict__ rsz, const int64_t* __restrict__ roff,
+                              const T* __restrict__ xc, T* __restrict__ x) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrows * Npc) return;
+    const int c = (int)(t / Npc), m = (int)(t - (int64_t)c * Npc);
+    const int ns = rsz[c] / Np;
+    const T v = xc[t];
+    for (int q = 0; q < ns; ++q) x[roff[c] + q * Np + m] += v;
+}
+
+template <typename T>
+void xdg_restrict(bh_ctx* ctx, const Vbsr& A, int Np, int Npc, const T* r, T* rc) {
+    const int64_t n = (int64_t)A.nrows * Npc;
+    hipLaunchKernelGGL((xdg_restrict_k<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A.nrows, Np, Npc,
+                       A.rsz_d.as<int>(), A.roff_d.as<int64_t>(), r, rc);
+    BH_HIP(hipGetLastError());
+}
+
+template <typename T>
+void xdg_prolong_add(bh_ctx* ctx, const Vbsr& A, int Np, int Npc, const T* xc, T* x) {
+    const int64_t n = (int64_t)A.nrows * Npc;
+    hipLaunchKernelGGL((xdg_prolong_k<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A.nrows, Np, Npc,
+                       A.rsz_d.as<int>(), A.roff_d.as<int64_t>(), xc, x);
+    BH_HIP(hipGetLastError());
+}
+template void xdg_restrict<float>(bh_ctx*, const Vbsr&, int, int, const float*, float*);
+template void xdg_restrict<double>(bh_ctx*, const Vbsr&, int, int, const double*, double*);
+template void xdg_prolong_add<float>(bh_ctx*, const Vbsr&, int, int, const float*, float*);
+template void xdg_prolong_add<double>(bh_ctx*, const Vbsr&, int, int, const double*, double*);
+
+// ------------------------------------------------------------------ host
+void Vbsr::upload_structure() {
+    rowptr_d.alloc((nrows + 1) * sizeof(int64_t));
+    col_d.alloc(std::max<size_t>(col.size(), 1) * sizeof(int));
+    voff_d.alloc(std::max<size_t>(voff.size(), 1) * sizeof(int64_t));
+    rsz_d.alloc(std::max(nrows, 1) * sizeof(int));
+    roff_d.alloc((nrows + 1) * sizeof(int64_t));
+    ctx->upload(rowptr_d.p, rowptr.data(), (nrows + 1) * sizeof(int64_t));
+    ctx->upload(col_d.p, col.data(), col.size() * sizeof(int));
+    ctx->upload(voff_d.p, voff.data(), voff.size() * sizeof(int64_t));
+    ctx->upload(rsz_d.p, rsz.data(), nrows * sizeof(int));
+    ctx->upload(roff_d.p, roff.data(), (nrows + 1) * sizeof(int64_t));
+}
+
+void Vbsr::build(bh_ctx* c, int n, const std::vector<int>& sizes, const std::vector<std::vector<int>>& cols) {
+    ctx = c;
+    nrows = n;
+    rsz = sizes;
+    roff.assign(n + 1, 0);
+    for (int r = 0; r < n; ++r) roff[r + 1] = roff[r] + rsz[r];
+    N = roff[n];
+    rowptr.assign(n + 1, 0);
+    col.clear();
+    voff.clear();
+    int64_t off = 0;
+    for (int r = 0; r < n; ++r) {
+        for (int cc : cols[r]) {
+            if (rsz[cc] == 0 || rsz[r] == 0) continue;
+            col.push_back(cc);
+            voff.push_back(off);
+            off += (int64_t)rsz[r] * rsz[cc];
+        }
+        rowptr[r + 1] = (int64_t)col.size();
+    }
+    voff.push_back(off);
+    nvals = off;
+    vals.alloc(std::max<int64_t>(nvals, 1) * sizeof(double));
+    BH_HIP(hipMemsetAsync(vals.p, 0, vals.bytes, ctx->stream));
+    upload_structure();
+}
+
+void Vbsr::ensure_fp32() {
+    if (vals32.p) return;
+    vals32.alloc(std::max<int64_t>(nvals, 1) * sizeof(float));
+    convert_f64_f32(ctx, vals.as<double>(), vals32.as<float>(), nvals);
+}
+
+void Vbsr::spmv(double alpha, const void* x, double beta, void* y, int prec, const void* bsub) {
+    dim3 blk(256), grd((nrows + 3) / 4);
+    if (prec == BH_FP32) {
+        ensure_fp32();
+        hipLaunchKernelGGL((vbsr_spmv_kernel<float>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
+                           rowptr_d.as<int64_t>(), col_d.as<int>(), voff_d.as<int64_t>(), vals32.as<float>(), (const float*)x,
+                           (float*)y, (float)alpha, (float)beta, (const float*)bsub);
+    } else {
+        hipLaunchKernelGGL((vbsr_spmv_kernel<double>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
+                           rowptr_d.as<int64_t>(), col_d.as<int>(), voff_d.as<int64_t>(), vals.as<double>(), (const double*)x,
+                           (double*)y, alpha, beta, (const double*)bsub);
+    }
+    BH_HIP(hipGetLastError());
+}
+
+void Vbsr::invert_diag(bool want32) {
+    std::vector<int64_t> di(nrows, -1), doff(nrows + 1, 0);
+    for (int r = 0; r < nrows; ++r) {
+        for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) if (col[p] == r) di[r] = p;
+        doff[r + 1] = doff[r] + (int64_t)rsz[r] * rsz[r];
+    }
+    diag_d.alloc(std::max(nrows, 1) * sizeof(int64_t));
+    doff_d.alloc((nrows + 1) * sizeof(int64_t));
+    ctx->upload(diag_d.p, di.data(), nrows * sizeof(int64_t));
+    ctx->upload(doff_d.p, doff.data(), (nrows + 1) * sizeof(int64_t));
+    dinv64.alloc(std::max<int64_t>(doff[nrows], 1) * sizeof(double));
+    if (want32) dinv32.alloc(std::max<int64_t>(doff[nrows], 1) * sizeof(float));
+    DevBuf st(sizeof(int));
+    BH_HIP(hipMemsetAsync(st.p, 0, sizeof(int), ctx->stream));
+    int maxbs = 1;
+    for (int v : rsz) maxbs = std::max(maxbs, v);
+    BH_CHECK(maxbs <= 64, "variable-block inverse: block size > 64");
+    hipLaunchKernelGGL(vbsr_invert_kernel, dim3(nrows), dim3(64), (size_t)maxbs * 2 * maxbs * sizeof(double), ctx->stream, nrows,
+                       rsz_d.as<int>(), diag_d.as<int64_t>(), voff_d.as<int64_t>(), doff_d.as<int64_t>(), vals.as<double>(),
+                       dinv64.as<double>(), want32 ? dinv32.as<float>() : nullptr, st.as<int>());
+    BH_HIP(hipGetLastError());
+    int h = 0;
+    ctx->download(&h, st.p, sizeof(int));
+    BH_CHECK(h == 0, "variable-block inverse: singular diagonal block in row " + std::to_string(h - 1));
+}
+
+void Vbsr::block_gemv(const void* r, void* x, double omega, bool add, int prec) {
+    dim3 blk(256), grd((nrows + 3) / 4);
+    if (prec == BH_FP32)
+        hipLaunchKernelGGL((vblock_gemv_kernel<float>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
+                           doff_d.as<int64_t>(), dinv32.as<float>(), (const float*)r, (float*)x, (float)omega, add ? 1 : 0);
+    else
+        hipLaunchKernelGGL((vblock_gemv_kernel<double>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
+                           doff_d.as<int64_t>(), dinv64.as<double>(), (const double*)r, (double*)x, omega, add ? 1 : 0);
+    BH_HIP(hipGetLastError());
+}
+
+void xdg_assemble(XdgDevice& X, Vbsr& M) {
+    XdgProblem& P = *X.P;
+    XArgs a{};
+    a.D = P.D; a.k = P.k; a.Np = P.Np; a.nq1 = P.k + 1;
+    a.n0 = P.g->n[0]; a.n1 = P.g->n[1]; a.n2 = P.g->n[2];
+    a.nodes1 = X.nodes1.as<double>();
+    a.nstride = X.nstride;
+    a.elems = X.elems.as<XElem>();
+    a.nelem = (int)P.elems.size();
+    a.nx = X.nx.as<double>(); a.nw = X.nw.as<double>(); a.nn = X.nn.as<double>();
+    a.ex = X.ex.as<int>();
+    a.gx = X.gx.as<double>(); a.gw = X.gw.as<double>();
+    a.spmask = X.spmask.as<int>();
+    a.rowptr = M.rowptr_d.as<int64_t>(); a.col = M.col_d.as<int>(); a.voff = M.voff_d.as<int64_t>(); a.bsz = M.rsz_d.as<int>();
+    a.vals = M.vals.as<double>();
+    BH_CHECK(P.Np <= XMAXNP && 4 * P.Np * P.Np <= 7 * 64, "XDG assembly: degree too high");
+    int blocks = a.nelem < 65536 ? a.nelem : 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(xdg_assemble_kernel, dim3(blocks), dim3(64), 0, X.ctx->stream, a);
+    BH_HIP(hipGetLastError());
+}
+
+}  // namespace bh

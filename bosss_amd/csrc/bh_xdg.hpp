@@ -1,0 +1,73 @@
+// XDG (extended DG) two-phase Poisson with a spherical level set: cut-cell
+// quadrature, agglomeration, element lists and the variable-block matrix.
+#pragma once
+#include <vector>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+
+// Quadrature rule with physical nodes (n x 3), weights and (interface only) normals
+struct QRule {
+    std::vector<double> x, w, nrm;
+    int n() const { return (int)w.size(); }
+};
+
+struct Sphere {
+    double c[3];
+    double r;
+    double phi(const double* x, int D) const {
+        double s = 0;
+        for (int d = 0; d < D; ++d) s += (x[d] - c[d]) * (x[d] - c[d]);
+        return std::sqrt(s) - r;
+    }
+};
+
+// Cut-cell rules for a sphere and an axis-aligned box (bh_xdgsetup.cpp).
+// species: -1 = inside (phi < 0, "A"), +1 = outside ("B").
+void sphere_volume_rule(int D, const double* lo, const double* hi, const Sphere& S, int species, int q, QRule& out);
+void sphere_interface_rule(int D, const double* lo, const double* hi, const Sphere& S, int q, QRule& out);
+// face in the plane x_dir = lo[dir] (lo[dir] == hi[dir])
+void sphere_face_rule(int D, const double* lo, const double* hi, int dir, const Sphere& S, int species, int q, QRule& out);
+
+// element of the XDG bilinear form (one wave of the assembly kernel each)
+struct XElem {
+    int type;        // 0 volume, 1 interior face, 2 boundary face, 3 interface
+    int cellP, cellQ;
+    int spP, spQ;    // species index (0 = A, 1 = B) of each side
+    int dir;         // face normal direction (faces), -1 otherwise
+    int side;        // boundary side / +1 if normal = +e_dir from P to Q
+    int r0, rn;      // rule slice [r0, r0+rn) in the node arrays, rn = 0: tensor rule
+    double nuP, nuQ, mu;
+};
+
+struct XdgProblem {
+    bh_ctx* ctx;
+    bh_grid* g;
+    int D, k, Np, q;
+    Sphere S;
+    double nu[2];
+    double penalty_base, penalty_deg, agg_threshold;
+    // per cell
+    std::vector<int> nspec;          // 1 or 2
+    std::vector<int> spmask;         // bit 0: A present, bit 1: B present
+    std::vector<double> vol[2];      // species volumes
+    std::vector<double> iface_area;  // interface area per cell
+    std::vector<double> len[2];      // agglomerated length scales per species
+    std::vector<double> frac[2];     // volume fractions
+    // elements and node storage
+    std::vector<XElem> elems;
+    std::vector<double> nx, nw, nn;  // nodes (3 per), weights, normals (3 per)
+    // agglomeration: per species, source cell -> target cell (-1 none)
+    std::vector<int> agg_target[2];
+    std::vector<int> agg_pairs;      // flattened (source, target, species) triples, in discovery order
+    int n_failed = 0;
+    // DOF maps: full (every present species block) and reduced (agglomeration sources removed)
+    std::vector<int64_t> off_full, off_red;
+    std::vector<int> bsz_full, bsz_red;   // row block sizes
+    int64_t N_full = 0, N_red = 0;
+    int ncut = 0;
+    void setup();
+};
+
+}  // namespace bh

@@ -1,0 +1,483 @@
+// XDG system on the device: upload of the element lists, assembly of the
+// full (cell, species) block matrix, agglomeration into the reduced system,
+// right-hand side, and the two-level preconditioner.
+//
+// Agglomeration (X/CellAgglomerator.cs:1157 GetRowManipulationMatrix,
+// :1755 ManipulateMatrixAndRHS): the DG polynomial of species s in a source
+// cell is the extension of the target cell's polynomial, u_src = E u_tgt with
+// E_mn = int_{src} phi^src_m phi^tgt_n dx (exact: a polynomial of degree k
+// restricted to a box).  With the injection Q (identity on kept blocks, E on
+// source blocks) the agglomerated system is Q^T A Q u~ = Q^T b, and the
+// full-space solution is Q u~.  Source blocks are removed from the reduced
+// DOF map (a cell keeps the species blocks that are not sources).
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "bh_solvers.hpp"
+#include "bh_xdgdev.hpp"
+
+namespace bh {
+
+void XdgDevice::upload() {
+    XdgProblem& p = *P;
+    bh_grid* g = p.g;
+    nstride = g->nmax + 1;
+    std::vector<double> n1((size_t)3 * nstride, 0.0);
+    for (int d = 0; d < p.D; ++d) std::copy(g->nodes[d].begin(), g->nodes[d].end(), n1.begin() + (size_t)d * nstride);
+    nodes1.alloc(n1.size() * sizeof(double));
+    ctx->upload(nodes1.p, n1.data(), n1.size() * sizeof(double));
+    elems.alloc(std::max<size_t>(p.elems.size(), 1) * sizeof(XElem));
+    ctx->upload(elems.p, p.elems.data(), p.elems.size() * sizeof(XElem));
+    auto up = [&](DevBuf& b, const std::vector<double>& v) {
+        b.alloc(std::max<size_t>(v.size(), 1) * sizeof(double));
+        ctx->upload(b.p, v.data(), v.size() * sizeof(double));
+    };
+    up(nx, p.nx);
+    up(nw, p.nw);
+    up(nn, p.nn);
+    std::vector<int> e;
+    basis_exponents(p.D, p.k, e);
+    ex.alloc(e.size() * sizeof(int));
+    ctx->upload(ex.p, e.data(), e.size() * sizeof(int));
+    std::vector<double> x(p.k + 1), w(p.k + 1);
+    gauss_rule(p.k + 1, x.data(), w.data());
+    up(gx, x);
+    up(gw, w);
+    spmask.alloc(p.spmask.size() * sizeof(int));
+    ctx->upload(spmask.p, p.spmask.data(), p.spmask.size() * sizeof(int));
+}
+
+// ------------------------------------------------------------ host helpers
+namespace {
+
+// physical basis values / gradients of cell c at physical x
+void phys_basis(const XdgProblem& p, int c, const double* x, double* val, double* grad) {
+    int idx[3];
+    p.g->cell_ijk(c, idx);
+    double xi[3], h[3], detJ = 1;
+    for (int d = 0; d < p.D; ++d) {
+        const double lo = p.g->nodes[d][idx[d]];
+        h[d] = p.g->nodes[d][idx[d] + 1] - lo;
+        xi[d] = 2.0 * (x[d] - lo) / h[d] - 1.0;
+        detJ *= 0.5 * h[d];
+    }
+    std::vector<double> g((size_t)p.Np * p.D);
+    basis_eval(p.D, p.k, xi, val, grad ? g.data() : nullptr);
+    const double sc = 1.0 / std::sqrt(detJ);
+    for (int m = 0; m < p.Np; ++m) {
+        val[m] *= sc;
+        if (grad)
+            for (int d = 0; d < p.D; ++d) grad[m * p.D + d] = g[(size_t)m * p.D + d] * sc * 2.0 / h[d];
+    }
+}
+
+// nodes of an element (tensor rule generated like the assembly kernel does)
+void elem_nodes(const XdgProblem& p, const XElem& E, std::vector<double>& X, std::vector<double>& W) {
+    X.clear();
+    W.clear();
+    if (E.rn > 0) {
+        X.assign(p.nx.begin() + (size_t)E.r0 * 3, p.nx.begin() + (size_t)(E.r0 + E.rn) * 3);
+        W.assign(p.nw.begin() + E.r0, p.nw.begin() + E.r0 + E.rn);
+        return;
+    }
+    const int q = p.k + 1;
+    std::vector<double> gx(q), gw(q);
+    gauss_rule(q, gx.data(), gw.data());
+    int idx[3];
+    p.g->cell_ijk(E.cellP, idx);
+    double lo[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    for (int d = 0; d < p.D; ++d) { lo[d] = p.g->nodes[d][idx[d]]; h[d] = p.g->nodes[d][idx[d] + 1] - lo[d]; }
+    const int nd = E.type == 0 ? p.D : p.D - 1;
+    int n = 1;
+    for (int i = 0; i < nd; ++i) n *= q;
+    for (int t = 0; t < n; ++t) {
+        double x[3] = {0, 0, 0}, w = 1;
+        int r = t;
+        for (int d = 0; d < p.D; ++d) {
+            if (E.type != 0 && d == E.dir) {
+                x[d] = (E.type == 2 && E.side == 0) ? lo[d] : lo[d] + h[d];
+                continue;
+            }
+            const int id = r % q;
+            r /= q;
+            x[d] = lo[d] + 0.5 * (gx[id] + 1.0) * h[d];
+            w *= gw[id] * 0.5 * h[d];
+        }
+        X.insert(X.end(), x, x + 3);
+        W.push_back(w);
+    }
+}
+
+inline int spos_of(const XdgProblem& p, int c, int sp) { return (sp == 1 && (p.spmask[c] & 1)) ? 1 : 0; }
+inline int spos_red(const XdgProblem& p, int c, int sp) {
+    // position of species sp among the kept (non-source) species blocks of cell c
+    int pos = 0;
+    for (int s = 0; s < sp; ++s)
+        if (((p.spmask[c] >> s) & 1) && p.agg_target[s][c] < 0) ++pos;
+    return pos;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ build
+void xdg_build(bh_xdg* X) {
+    XdgProblem& p = *X->P;
+    bh_ctx* ctx = X->ctx;
+    const int J = p.g->J, D = p.D, Np = p.Np;
+    X->X.ctx = ctx;
+    X->X.P = &p;
+    X->X.upload();
+    // full matrix structure: every cell couples to itself and its face neighbours
+    std::vector<std::vector<int>> cols(J);
+    for (int c = 0; c < J; ++c) {
+        int idx[3];
+        p.g->cell_ijk(c, idx);
+        cols[c].push_back(c);
+        for (int d = 0; d < D; ++d)
+            for (int s = -1; s <= 1; s += 2) {
+                int nb[3] = {idx[0], idx[1], idx[2]};
+                nb[d] += s;
+                if (nb[d] < 0 || nb[d] >= p.g->n[d]) continue;
+                cols[c].push_back(p.g->cell_index(nb[0], nb[1], nb[2]));
+            }
+        std::sort(cols[c].begin(), cols[c].end());
+    }
+    X->Afull.build(ctx, J, p.bsz_full, cols);
+    xdg_assemble(X->X, X->Afull);
+    // extension blocks of the agglomeration pairs
+    const int npair = (int)p.agg_pairs.size() / 3;
+    X->ext.assign((size_t)npair * Np * Np, 0.0);
+    {
+        const int q = p.k + 1;
+        std::vector<double> gx(q), gw(q), vs(Np), vt(Np);
+        gauss_rule(q, gx.data(), gw.data());
+        int nq = 1;
+        for (int d = 0; d < D; ++d) nq *= q;
+        for (int i = 0; i < npair; ++i) {
+            const int src = p.agg_pairs[3 * i], tgt = p.agg_pairs[3 * i + 1];
+            int idx[3];
+            p.g->cell_ijk(src, idx);
+            double lo[3], h[3];
+            for (int d = 0; d < D; ++d) { lo[d] = p.g->nodes[d][idx[d]]; h[d] = p.g->nodes[d][idx[d] + 1] - lo[d]; }
+            double* Eb = &X->ext[(size_t)i * Np * Np];
+            for (int t = 0; t < nq; ++t) {
+                double x[3] = {0, 0, 0}, w = 1;
+                int r = t;
+                for (int d = 0; d < D; ++d) {
+                    const int id = r % q;
+                    r /= q;
+                    x[d] = lo[d] + 0.5 * (gx[id] + 1.0) * h[d];
+                    w *= gw[id] * 0.5 * h[d];
+                }
+                phys_basis(p, src, x, vs.data(), nullptr);
+                phys_basis(p, tgt, x, vt.data(), nullptr);
+                for (int m = 0; m < Np; ++m)
+                    for (int n = 0; n < Np; ++n) Eb[m * Np + n] += w * vs[m] * vt[n];
+            }
+        }
+    }
+    // reduced (agglomerated) matrix Q^T A Q on the host (FP64)
+    std::vector<double> fv((size_t)X->Afull.nvals);
+    ctx->download(fv.data(), X->Afull.vals.p, fv.size() * sizeof(double));
+    // map (cell, species) -> (reduced cell, position, extension block or -1)
+    std::vector<int> pair_of[2];
+    for (int s = 0; s < 2; ++s) pair_of[s].assign(J, -1);
+    for (int i = 0; i < npair; ++i) pair_of[p.agg_pairs[3 * i + 2]][p.agg_pairs[3 * i]] = i;
+    auto target = [&](int c, int s, int& rc, int& rpos, int& pe) {
+        const int t = p.agg_target[s][c];
+        if (t >= 0) { rc = t; pe = pair_of[s][c]; }
+        else { rc = c; pe = -1; }
+        rpos = spos_red(p, rc, s);
+    };
+    std::vector<std::map<int, std::vector<double>>> rows(J);   // row-major dense blocks
+    std::vector<double> tmp((size_t)Np * Np), tmp2((size_t)Np * Np);
+    for (int r = 0; r < J; ++r) {
+        const int Rf = p.bsz_full[r];
+        for (int64_t bp = X->Afull.rowptr[r]; bp < X->Afull.rowptr[r + 1]; ++bp) {
+            const int c = X->Afull.col[bp];
+            const double* B = &fv[X->Afull.voff[bp]];   // column-major Rf x Cf
+            for (int sr = 0; sr < 2; ++sr) {
+                if (!((p.spmask[r] >> sr) & 1)) continue;
+                for (int sc = 0; sc < 2; ++sc) {
+                    if (!((p.spmask[c] >> sc) & 1)) continue;
+                    const int r0 = spos_of(p, r, sr) * Np, c0 = spos_of(p, c, sc) * Np;
+                    // sub-block S[m][n] = B[r0+m, c0+n]
+                    for (int m = 0; m < Np; ++m)
+                        for (int n = 0; n < Np; ++n) tmp[m * Np + n] = B[(size_t)(c0 + n) * Rf + r0 + m];
+                    int rr, rpos, pr, cc, cpos, pc;
+                    target(r, sr, rr, rpos, pr);
+                    target(c, sc, cc, cpos, pc);
+                    bool zero = true;
+                    for (double v : tmp) if (v != 0.0) { zero = false; break; }
+                    if (zero) continue;
+                    if (pr >= 0) {   // E^T S
+                        const double* E = &X->ext[(size_t)pr * Np * Np];
+                        for (int m = 0; m < Np; ++m)
+                            for (int n = 0; n < Np; ++n) {
+                                double s = 0;
+                                for (int t = 0; t < Np; ++t) s += E[t * Np + m] * tmp[t * Np + n];
+                                tmp2[m * Np + n] = s;
+                            }
+                        tmp.swap(tmp2);
+                    }
+                    if (pc >= 0) {   // S E
+                        const double* E = &X->ext[(size_t)pc * Np * Np];
+                        for (int m = 0; m < Np; ++m)
+                            for (int n = 0; n < Np; ++n) {
+                                double s = 0;
+                                for (int t = 0; t < Np; ++t) s += tmp[m * Np + t] * E[t * Np + n];
+                                tmp2[m * Np + n] = s;
+                            }
+                        tmp.swap(tmp2);
+                    }
+                    const int Rr = p.bsz_red[rr], Cr = p.bsz_red[cc];
+                    auto& blk = rows[rr][cc];
+                    if (blk.empty()) blk.assign((size_t)Rr * Cr, 0.0);
+                    for (int m = 0; m < Np; ++m)
+                        for (int n = 0; n < Np; ++n) blk[(size_t)(rpos * Np + m) * Cr + cpos * Np + n] += tmp[m * Np + n];
+                }
+            }
+        }
+    }
+    fv.clear();
+    fv.shrink_to_fit();
+    {
+        std::vector<std::vector<int>> rcols(J);
+        for (int r = 0; r < J; ++r) {
+            if (p.bsz_red[r] == 0) continue;
+            // the diagonal block is always present
+            if (!rows[r].count(r)) rows[r][r].assign((size_t)p.bsz_red[r] * p.bsz_red[r], 0.0);
+            for (auto& kv : rows[r]) rcols[r].push_back(kv.first);
+        }
+        X->Ared.build(ctx, J, p.bsz_red, rcols);
+        std::vector<double> rv((size_t)X->Ared.nvals);
+        for (int r = 0; r < J; ++r) {
+            const int R = p.bsz_red[r];
+            for (int64_t bp = X->Ared.rowptr[r]; bp < X->Ared.rowptr[r + 1]; ++bp) {
+                const int c = X->Ared.col[bp];
+                const int C = p.bsz_red[c];
+                const auto& blk = rows[r][c];
+                double* o = &rv[X->Ared.voff[bp]];
+                for (int j = 0; j < C; ++j)
+                    for (int i = 0; i < R; ++i) o[(size_t)j * R + i] = blk[(size_t)i * C + j];
+            }
+        }
+        ctx->upload(X->Ared.vals.p, rv.data(), rv.size() * sizeof(double));
+    }
+    // boundary nodes (Dirichlet data)
+    X->bnd_x.clear(); X->bnd_w.clear(); X->bnd_nu.clear(); X->bnd_mu.clear();
+    X->bnd_cell.clear(); X->bnd_sp.clear(); X->bnd_dir.clear(); X->bnd_side.clear();
+    std::vector<double> NX, NW;
+    for (const XElem& E : p.elems) {
+        if (E.type != 2) continue;
+        elem_nodes(p, E, NX, NW);
+        for (size_t t = 0; t < NW.size(); ++t) {
+            X->bnd_x.insert(X->bnd_x.end(), &NX[3 * t], &NX[3 * t] + 3);
+            X->bnd_w.push_back(NW[t]);
+            X->bnd_nu.push_back(E.nuP);
+            X->bnd_mu.push_back(E.mu);
+            X->bnd_cell.push_back(E.cellP);
+            X->bnd_sp.push_back(E.spP);
+            X->bnd_dir.push_back(E.dir);
+            X->bnd_side.push_back(E.side);
+        }
+    }
+    X->rhs_red.alloc(std::max<int64_t>(p.N_red, 1) * sizeof(double));
+    ctx->sync();
+}
+
+// right-hand side: F - affine, F_m = int f_s phi_m (species-constant source f_s),
+// affine = boundary terms of the Dirichlet data g (SIPLaplace.BoundaryEdgeForm):
+// -nu g dphi_m/dn + nu mu g phi_m
+void xdg_set_rhs(bh_xdg* X, const double* f, const double* g) {
+    XdgProblem& p = *X->P;
+    const int Np = p.Np, D = p.D;
+    X->fsrc = {f[0], f[1]};
+    X->rhs_full.assign((size_t)p.N_full, 0.0);
+    std::vector<double> NX, NW, v(Np), gr((size_t)Np * D);
+    for (const XElem& E : p.elems) {
+        if (E.type != 0) continue;
+        const double fs = f[E.spP];
+        if (fs == 0.0) continue;
+        elem_nodes(p, E, NX, NW);
+        double* o = &X->rhs_full[p.off_full[E.cellP] + spos_of(p, E.cellP, E.spP) * Np];
+        for (size_t t = 0; t < NW.size(); ++t) {
+            phys_basis(p, E.cellP, &NX[3 * t], v.data(), nullptr);
+            for (int m = 0; m < Np; ++m) o[m] += NW[t] * fs * v[m];
+        }
+    }
+    if (g)
+        for (size_t t = 0; t < X->bnd_w.size(); ++t) {
+            const int c = X->bnd_cell[t];
+            phys_basis(p, c, &X->bnd_x[3 * t], v.data(), gr.data());
+            const double nsign = X->bnd_side[t] ? 1.0 : -1.0;
+            const double nu = X->bnd_nu[t], mu = X->bnd_mu[t], w = X->bnd_w[t];
+            double* o = &X->rhs_full[p.off_full[c] + spos_of(p, c, X->bnd_sp[t]) * Np];
+            for (int m = 0; m < Np; ++m) {
+                const double dn = nsign * gr[(size_t)m * D + X->bnd_dir[t]];
+                o[m] -= w * (-nu * g[t] * dn + nu * mu * g[t] * v[m]);
+            }
+        }
+    // reduced: Q^T b
+    std::vector<double> red((size_t)p.N_red, 0.0);
+    const int J = p.g->J;
+    std::vector<int> pair_of[2];
+    for (int s = 0; s < 2; ++s) pair_of[s].assign(J, -1);
+    const int npair = (int)p.agg_pairs.size() / 3;
+    for (int i = 0; i < npair; ++i) pair_of[p.agg_pairs[3 * i + 2]][p.agg_pairs[3 * i]] = i;
+    for (int c = 0; c < J; ++c)
+        for (int s = 0; s < 2; ++s) {
+            if (!((p.spmask[c] >> s) & 1)) continue;
+            const double* b = &X->rhs_full[p.off_full[c] + spos_of(p, c, s) * Np];
+            const int t = p.agg_target[s][c];
+            if (t < 0) {
+                double* o = &red[p.off_red[c] + spos_red(p, c, s) * Np];
+                for (int m = 0; m < Np; ++m) o[m] += b[m];
+            } else {
+                const double* E = &X->ext[(size_t)pair_of[s][c] * Np * Np];
+                double* o = &red[p.off_red[t] + spos_red(p, t, s) * Np];
+                for (int n = 0; n < Np; ++n) {
+                    double a = 0;
+                    for (int m = 0; m < Np; ++m) a += E[m * Np + n] * b[m];
+                    o[n] += a;
+                }
+            }
+        }
+    X->ctx->upload(X->rhs_red.p, red.data(), red.size() * sizeof(double));
+}
+
+// u_full = Q u_red (host)
+void xdg_extend(const bh_xdg* X, const double* ured, double* ufull) {
+    const XdgProblem& p = *X->P;
+    const int J = p.g->J, Np = p.Np;
+    std::vector<int> pair_of[2];
+    for (int s = 0; s < 2; ++s) pair_of[s].assign(J, -1);
+    const int npair = (int)p.agg_pairs.size() / 3;
+    for (int i = 0; i < npair; ++i) pair_of[p.agg_pairs[3 * i + 2]][p.agg_pairs[3 * i]] = i;
+    for (int c = 0; c < J; ++c)
+        for (int s = 0; s < 2; ++s) {
+            if (!((p.spmask[c] >> s) & 1)) continue;
+            double* o = &ufull[p.off_full[c] + spos_of(p, c, s) * Np];
+            const int t = p.agg_target[s][c];
+            if (t < 0) {
+                const double* u = &ured[p.off_red[c] + spos_red(p, c, s) * Np];
+                for (int m = 0; m < Np; ++m) o[m] = u[m];
+            } else {
+                const double* E = &X->ext[(size_t)pair_of[s][c] * Np * Np];
+                const double* u = &ured[p.off_red[t] + spos_red(p, t, s) * Np];
+                for (int m = 0; m < Np; ++m) {
+                    double a = 0;
+                    for (int n = 0; n < Np; ++n) a += E[m * Np + n] * u[n];
+                    o[m] = a;
+                }
+            }
+        }
+}
+
+// ------------------------------------------------------------ preconditioner
+XdgPrecond::~XdgPrecond() { delete Ac; }
+
+void XdgPrecond::build(bh_grid* g, int degc, int min_coarse, int kcycle_inner) {
+    ctx = A->ctx;
+    N = A->N;
+    const int D = g->D, J = g->J;
+    Npc = basis_count(D, degc);
+    BH_CHECK(Npc <= Np, "XDG preconditioner: coarse degree above the DG degree");
+    // block inverses of the smoother
+    A->invert_diag(prec == BH_FP32);
+    if (prec == BH_FP32) A->ensure_fp32();
+    // coarse matrix Ac = R^T A R (species merged, first Npc modes)
+    std::vector<double> v((size_t)A->nvals);
+    ctx->download(v.data(), A->vals.p, v.size() * sizeof(double));
+    std::vector<int64_t> rp(J + 1, 0);
+    std::vector<int> col;
+    std::vector<double> cv;
+    for (int r = 0; r < J; ++r) {
+        const int R = A->rsz[r];
+        for (int64_t bp = A->rowptr[r]; bp < A->rowptr[r + 1]; ++bp) {
+            const int c = A->col[bp];
+            const int C = A->rsz[c];
+            const double* B = &v[A->voff[bp]];
+            col.push_back(c);
+            std::vector<double> blk((size_t)Npc * Npc, 0.0);   // column-major
+            for (int sr = 0; sr < R / Np; ++sr)
+                for (int sc = 0; sc < C / Np; ++sc)
+                    for (int n = 0; n < Npc; ++n)
+                        for (int m = 0; m < Npc; ++m) blk[(size_t)n * Npc + m] += B[(size_t)(sc * Np + n) * R + sr * Np + m];
+            cv.insert(cv.end(), blk.begin(), blk.end());
+        }
+        rp[r + 1] = (int64_t)col.size();
+    }
+    Ac = new bh_bsr();
+    Ac->ctx = ctx;
+    Ac->nrows = J;
+    Ac->bs = Npc;
+    Ac->h_rowptr = rp;
+    Ac->h_col = col;
+    Ac->nnzb = rp[J];
+    Ac->rowptr.alloc((J + 1) * sizeof(int64_t));
+    Ac->col.alloc(std::max<int64_t>(Ac->nnzb, 1) * sizeof(int));
+    ctx->upload(Ac->rowptr.p, rp.data(), (J + 1) * sizeof(int64_t));
+    ctx->upload(Ac->col.p, col.data(), col.size() * sizeof(int));
+    Ac->vals.alloc(std::max<size_t>(cv.size(), 1) * sizeof(double));
+    ctx->upload(Ac->vals.p, cv.data(), cv.size() * sizeof(double));
+    cmg.reset(new PMultigrid());
+    cmg->prec = prec;
+    cmg->omega = omega;
+    cmg->pre = pre;
+    cmg->post = post;
+    cmg->kcycle_level = kcycle_inner > 0 ? 0 : -1;
+    cmg->kin = kcycle_inner;
+    cmg->build_grid(ctx, g, Ac, {degc}, min_coarse, nullptr);
+    const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
+    b.alloc(N * es);
+    x.alloc(N * es);
+    r.alloc(N * es);
+    ctx->sync();
+}
+
+template <typename T>
+void XdgPrecond::cycle() {
+    const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
+    T* bb = (T*)b.p;
+    T* xx = (T*)x.p;
+    T* rr = (T*)r.p;
+    A->block_gemv(bb, xx, omega, false, p);
+    for (int s = 1; s < pre; ++s) {
+        A->spmv(-1.0, xx, 0.0, rr, p, bb);
+        A->block_gemv(rr, xx, omega, true, p);
+    }
+    A->spmv(-1.0, xx, 0.0, rr, p, bb);
+    MgLevel& C = cmg->lv[0];
+    xdg_restrict<T>(ctx, *A, Np, Npc, rr, (T*)C.b.p);
+    cmg->solve_level<T>(0);
+    xdg_prolong_add<T>(ctx, *A, Np, Npc, (const T*)C.x.p, xx);
+    for (int s = 0; s < post; ++s) {
+        A->spmv(-1.0, xx, 0.0, rr, p, bb);
+        A->block_gemv(rr, xx, omega, true, p);
+    }
+}
+
+void XdgPrecond::precondition(double* xo, const double* bi) {
+    if (prec == BH_FP32) {
+        convert_f64_f32(ctx, bi, (float*)b.p, N);
+        cycle<float>();
+        convert_f32_f64(ctx, (const float*)x.p, xo, N);
+    } else {
+        copy(ctx, (double*)b.p, bi, N);
+        cycle<double>();
+        copy(ctx, xo, (const double*)x.p, N);
+    }
+    iterations++;
+}
+
+void XdgPrecond::solve_dev(double* xo, const double* bi) {
+    DevBuf rr(N * sizeof(double)), c(N * sizeof(double));
+    A->spmv(-1.0, xo, 0.0, rr.p, BH_FP64, bi);
+    precondition(c.as<double>(), rr.as<double>());
+    axpby(ctx, 1.0, c.as<double>(), 1.0, xo, N);
+    converged = 1;
+}
+
+}  // namespace bh

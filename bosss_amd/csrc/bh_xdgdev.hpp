@@ -1,0 +1,89 @@
+// XDG device objects: variable-block matrix, device copies of the element
+// lists and rules, the agglomerated system and its two-level preconditioner.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "bh_internal.hpp"
+#include "bh_mg.hpp"
+#include "bh_xdg.hpp"
+
+namespace bh {
+
+// Block-sparse matrix with per-row block sizes (cell blocks of 10 or 20 DOF
+// for k = 2 in 3D).  Blocks are stored column-major, one after another in
+// row order; voff[p] is the first value of block p.
+struct Vbsr {
+    bh_ctx* ctx = nullptr;
+    int nrows = 0;
+    int64_t N = 0, nvals = 0;
+    std::vector<int> rsz, col;
+    std::vector<int64_t> roff, rowptr, voff;
+    DevBuf vals, vals32;
+    DevBuf rowptr_d, col_d, voff_d, rsz_d, roff_d;
+    DevBuf diag_d, doff_d, dinv64, dinv32;
+    void build(bh_ctx* c, int n, const std::vector<int>& sizes, const std::vector<std::vector<int>>& cols);
+    void upload_structure();
+    void ensure_fp32();
+    // y <- alpha A x + beta y; with bsub: y <- bsub - A x
+    void spmv(double alpha, const void* x, double beta, void* y, int prec, const void* bsub = nullptr);
+    void invert_diag(bool want32);
+    // x (+)= omega Dinv r
+    void block_gemv(const void* r, void* x, double omega, bool add, int prec);
+    int64_t find(int r, int c) const {
+        for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) if (col[p] == c) return p;
+        return -1;
+    }
+};
+
+struct XdgDevice {
+    bh_ctx* ctx = nullptr;
+    XdgProblem* P = nullptr;
+    int nstride = 0;
+    DevBuf nodes1, elems, nx, nw, nn, ex, gx, gw, spmask;
+    void upload();
+};
+
+void xdg_assemble(XdgDevice& X, Vbsr& M);
+// coarse-space transfer: per cell, the first Npc modes of every species block are
+// summed (restriction) / receive the coarse value (prolongation)
+template <typename T> void xdg_restrict(bh_ctx*, const Vbsr& A, int Np, int Npc, const T* r, T* rc);
+template <typename T> void xdg_prolong_add(bh_ctx*, const Vbsr& A, int Np, int Npc, const T* xc, T* x);
+
+// Two-level preconditioner of the agglomerated XDG system: FP32 (or FP64)
+// block-Jacobi smoothing on the variable-block matrix and a coarse
+// correction in the species-merged degree-1 space, solved by the p/h
+// multigrid of the SIP path (aggregation levels + K-cycle + dense coarsest).
+struct XdgPrecond : bh_solver {
+    Vbsr* A = nullptr;
+    int prec = BH_FP32;
+    int Np = 0, Npc = 0;
+    int pre = 2, post = 2;
+    double omega = 0.8;
+    bh_bsr* Ac = nullptr;                  // coarse matrix (uniform blocks of Npc)
+    std::unique_ptr<PMultigrid> cmg;
+    DevBuf b, x, r, xc;                    // work vectors in the cycle precision
+    ~XdgPrecond() override;
+    void build(bh_grid* g, int degc, int min_coarse, int kcycle_inner);
+    template <typename T> void cycle();
+    void precondition(double* x, const double* b) override;
+    void solve_dev(double* x, const double* b) override;   // x <- x + M(b - A x)
+};
+
+}  // namespace bh
+
+// C-ABI object
+struct bh_xdg {
+    bh_ctx* ctx = nullptr;
+    bh_grid* grid = nullptr;
+    std::unique_ptr<bh::XdgProblem> P;
+    bh::XdgDevice X;
+    bh::Vbsr Afull, Ared;
+    std::vector<double> ext;                // per agglomeration pair: Np x Np extension block, row-major [src mode][tgt mode]
+    std::vector<double> bnd_x;              // boundary nodes (3 per) of the Dirichlet data
+    std::vector<double> bnd_w, bnd_nu, bnd_mu;
+    std::vector<int> bnd_cell, bnd_sp, bnd_dir, bnd_side;
+    std::vector<double> rhs_full;           // host copy of the full right-hand side
+    bh::DevBuf rhs_red;
+    std::vector<double> fsrc{0.0, 0.0};
+};

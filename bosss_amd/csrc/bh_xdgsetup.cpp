@@ -1,0 +1,622 @@
+// XDG setup on the host (FP64): cut-cell quadrature for a spherical level
+// set, species metrics and length scales, agglomeration, element lists.
+//
+// Cut-cell rules: the dimension-reduction construction of Saye
+// ("High-order quadrature methods for implicitly defined surfaces and volumes
+// in hyperrectangles", SIAM J. Sci. Comput. 37 (2015)), which the reference
+// offers as its default cut-cell rule (X/Quadrature/SayeGaussRule_*.cs,
+// AppControl.cs:1323), specialised to phi = |x - c| - r: every root needed by
+// the construction is a closed-form square root, so no Newton iteration is
+// required.  In a box, the height direction k is the largest component of
+// the radial direction at the box centre; the remaining coordinates are
+// integrated with Gauss pieces split at the curves where the interface meets
+// the faces x_k = lo_k / hi_k (kinks of the height integrand), and along k
+// the Gauss pieces are split at the interface.  Boxes where a height
+// direction is not uniformly valid (silhouette or tangency inside) are
+// subdivided.
+#include <algorithm>
+#include <cmath>
+#include <functional>
+
+#include "bh_xdg.hpp"
+
+namespace bh {
+
+namespace {
+
+struct Gauss1 {
+    std::vector<double> x, w;
+    explicit Gauss1(int q) : x(q), w(q) { gauss_rule(q, x.data(), w.data()); }
+};
+
+// split [a,b] at the breakpoints in (a,b)
+std::vector<double> pieces(double a, double b, std::vector<double> bp) {
+    std::vector<double> p{a};
+    std::sort(bp.begin(), bp.end());
+    const double eps = 1e-14 * std::max(1.0, std::fabs(b - a));
+    for (double v : bp)
+        if (v > a + eps && v < b - eps && v > p.back() + eps) p.push_back(v);
+    p.push_back(b);
+    return p;
+}
+
+// roots c0 +- sqrt(R2) of (t - c0)^2 = R2
+void add_roots(std::vector<double>& v, double c0, double R2) {
+    if (R2 > 0) { const double s = std::sqrt(R2); v.push_back(c0 - s); v.push_back(c0 + s); }
+}
+
+// Outer-direction nodes of a piece [a, b] whose integrand may carry a
+// square-root singularity at an anchor point e outside the piece or at its end
+// (the extreme x_o = co +- R of a circle, where two breakpoints of the inner
+// direction meet).  If the nearest anchor lies within one piece length of the
+// piece, the piece is integrated in t with x_o = e + (f - e) t^2 (f the far end),
+// which turns sqrt(x_o - e) into a linear function of t; otherwise plain Gauss.
+void outer_nodes(double a, double b, const std::vector<double>& anchors, const Gauss1& G, std::vector<double>& X,
+                 std::vector<double>& W) {
+    const int q = (int)G.x.size();
+    const double L = b - a;
+    const double tol = 1e-12 * std::max(std::fabs(L), 1e-300);
+    double el = -1e300, er = 1e300;
+    for (double e : anchors) {
+        if (e <= a + tol) el = std::max(el, e);
+        if (e >= b - tol) er = std::min(er, e);
+    }
+    const bool use_l = a - el <= L, use_r = er - b <= L;
+    auto plain = [&](double x0, double x1) {
+        for (int i = 0; i < q; ++i) { X.push_back(x0 + 0.5 * (G.x[i] + 1) * (x1 - x0)); W.push_back(0.5 * (x1 - x0) * G.w[i]); }
+    };
+    // x = e + (f - e) t^2 for t in [t0, 1], covering [x_near, f]
+    auto graded = [&](double e, double xn, double f) {
+        const double t0 = std::sqrt(std::max(0.0, (xn - e) / (f - e)));
+        for (int i = 0; i < q; ++i) {
+            const double t = t0 + 0.5 * (G.x[i] + 1) * (1.0 - t0);
+            X.push_back(e + (f - e) * t * t);
+            W.push_back(std::fabs(f - e) * 2.0 * t * 0.5 * (1.0 - t0) * G.w[i]);
+        }
+    };
+    if (use_l && use_r) {
+        const double m = 0.5 * (a + b);
+        graded(el, a, m);
+        graded(er, b, m);
+    } else if (use_l) {
+        graded(el, a, b);
+    } else if (use_r) {
+        graded(er, b, a);
+    } else {
+        plain(a, b);
+    }
+}
+
+// Generic 2D integration over [lo_o,hi_o] x [lo_k2,hi_k2] of a function whose
+// smoothness breaks along circles |x' - c'|^2 = R2_b (b over `radii`): outer
+// breakpoints where a circle crosses x_k2 = lo/hi and at the circle's extremes
+// x_o = co +- R (also the anchors of outer_nodes), inner breakpoints at the
+// circle crossings.
+void circle_2d(double lo_o, double hi_o, double lo_k, double hi_k, double co, double ck, const std::vector<double>& radii,
+               const Gauss1& G, const std::function<void(double, double, double)>& visit) {
+    std::vector<double> bo, anchors;
+    for (double R2 : radii) {
+        if (R2 <= 0) continue;
+        add_roots(bo, co, R2 - (lo_k - ck) * (lo_k - ck));
+        add_roots(bo, co, R2 - (hi_k - ck) * (hi_k - ck));
+        add_roots(bo, co, R2);
+        add_roots(anchors, co, R2);
+    }
+    const auto po = pieces(lo_o, hi_o, bo);
+    const int q = (int)G.x.size();
+    std::vector<double> X, W;
+    for (size_t a = 0; a + 1 < po.size(); ++a) {
+        X.clear();
+        W.clear();
+        outer_nodes(po[a], po[a + 1], anchors, G, X, W);
+        for (size_t i = 0; i < X.size(); ++i) {
+            const double xo = X[i], wo = W[i];
+            std::vector<double> bk;
+            for (double R2 : radii) add_roots(bk, ck, R2 - (xo - co) * (xo - co));
+            const auto pk = pieces(lo_k, hi_k, bk);
+            for (size_t b = 0; b + 1 < pk.size(); ++b) {
+                const double ka = pk[b], kb = pk[b + 1];
+                for (int j = 0; j < q; ++j) visit(xo, ka + 0.5 * (G.x[j] + 1) * (kb - ka), wo * 0.5 * (kb - ka) * G.w[j]);
+            }
+        }
+    }
+}
+
+int argmax_abs(const double* v, const int* dims, int n) {
+    int best = dims[0];
+    for (int i = 1; i < n; ++i)
+        if (std::fabs(v[dims[i]]) > std::fabs(v[best])) best = dims[i];
+    return best;
+}
+
+// Volume (mode 0) or interface (mode 1) rule in a 3D/2D box, recursive.
+void sphere_rule_rec(int D, const double* lo, const double* hi, const Sphere& S, int species, int mode, const Gauss1& G,
+                     int depth, QRule& out) {
+    const double r2 = S.r * S.r;
+    {
+        // a box (or sub-box) the sphere does not cut: tensor Gauss rule or nothing
+        double dmin2 = 0, dmax2 = 0;
+        for (int d = 0; d < D; ++d) {
+            const double a = lo[d] - S.c[d], b = hi[d] - S.c[d];
+            dmin2 += (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
+            dmax2 += std::max(a * a, b * b);
+        }
+        const bool inside = dmax2 <= r2, outside = dmin2 >= r2;
+        if (inside || outside) {
+            if (mode == 1 || inside != (species < 0)) return;
+            const int q = (int)G.x.size();
+            int n = 1;
+            for (int d = 0; d < D; ++d) n *= q;
+            for (int t = 0; t < n; ++t) {
+                double x[3] = {0, 0, 0}, w = 1;
+                int r = t;
+                for (int d = 0; d < D; ++d) {
+                    const int id = r % q;
+                    r /= q;
+                    x[d] = lo[d] + 0.5 * (G.x[id] + 1) * (hi[d] - lo[d]);
+                    w *= 0.5 * (hi[d] - lo[d]) * G.w[id];
+                }
+                out.x.insert(out.x.end(), x, x + 3);
+                out.w.push_back(w);
+            }
+            return;
+        }
+    }
+    double xc[3], rad[3];
+    for (int d = 0; d < D; ++d) { xc[d] = 0.5 * (lo[d] + hi[d]); rad[d] = xc[d] - S.c[d]; }
+    int all[3] = {0, 1, 2};
+    const int k = argmax_abs(rad, all, D);
+    int od[2], no = 0;
+    for (int d = 0; d < D; ++d) if (d != k) od[no++] = d;
+    // silhouette check: r^2 - |x' - c'|^2 must not change sign over the face box
+    double mn = 0, mx = 0;
+    for (int i = 0; i < no; ++i) {
+        const int d = od[i];
+        const double a = lo[d] - S.c[d], b = hi[d] - S.c[d];
+        const double m1 = (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
+        mn += m1;
+        mx += std::max(a * a, b * b);
+    }
+    const bool ok_sil = (r2 - mx > 0) || (r2 - mn <= 0);
+    auto subdivide = [&]() {
+        const int nc = 1 << D;
+        for (int cidx = 0; cidx < nc; ++cidx) {
+            double l2[3], h2[3];
+            for (int d = 0; d < D; ++d) {
+                const bool up = (cidx >> d) & 1;
+                l2[d] = up ? xc[d] : lo[d];
+                h2[d] = up ? hi[d] : xc[d];
+            }
+            sphere_rule_rec(D, l2, h2, S, species, mode, G, depth + 1, out);
+        }
+    };
+    if (!ok_sil) {
+        if (depth < 6) { subdivide(); return; }
+    }
+    // emit one node at x (full coordinates)
+    auto emit_line = [&](const double* xp) {
+        // along k: roots t = c_k +- sqrt(r^2 - |x'-c'|^2)
+        double s2 = r2;
+        for (int i = 0; i < no; ++i) s2 -= (xp[od[i]] - S.c[od[i]]) * (xp[od[i]] - S.c[od[i]]);
+        return s2;
+    };
+    if (D == 3) {
+        const int d0 = od[0], d1 = od[1];
+        // inner 2D direction: larger radial component
+        const int k2 = std::fabs(rad[d0]) >= std::fabs(rad[d1]) ? d0 : d1;
+        const int o = k2 == d0 ? d1 : d0;
+        std::vector<double> radii;
+        radii.push_back(r2 - (lo[k] - S.c[k]) * (lo[k] - S.c[k]));
+        radii.push_back(r2 - (hi[k] - S.c[k]) * (hi[k] - S.c[k]));
+        circle_2d(lo[o], hi[o], lo[k2], hi[k2], S.c[o], S.c[k2], radii, G, [&](double xo, double xk2, double w2) {
+            double xp[3];
+            xp[o] = xo;
+            xp[k2] = xk2;
+            xp[k] = 0;
+            const double s2 = emit_line(xp);
+            if (mode == 0) {
+                std::vector<double> bt;
+                add_roots(bt, S.c[k], s2);
+                const auto pt = pieces(lo[k], hi[k], bt);
+                for (size_t b = 0; b + 1 < pt.size(); ++b) {
+                    const double ta = pt[b], tb = pt[b + 1];
+                    double xm[3] = {xp[0], xp[1], xp[2]};
+                    xm[k] = 0.5 * (ta + tb);
+                    if ((S.phi(xm, 3) < 0) != (species < 0)) continue;
+                    const int q = (int)G.x.size();
+                    for (int j = 0; j < q; ++j) {
+                        double x[3] = {xp[0], xp[1], xp[2]};
+                        x[k] = ta + 0.5 * (G.x[j] + 1) * (tb - ta);
+                        out.x.insert(out.x.end(), x, x + 3);
+                        out.w.push_back(w2 * 0.5 * (tb - ta) * G.w[j]);
+                    }
+                }
+            } else if (s2 > 0) {
+                const double s = std::sqrt(s2);
+                for (double t : {S.c[k] - s, S.c[k] + s}) {
+                    if (t <= lo[k] || t >= hi[k]) continue;
+                    double x[3] = {xp[0], xp[1], xp[2]};
+                    x[k] = t;
+                    out.x.insert(out.x.end(), x, x + 3);
+                    out.w.push_back(w2 * S.r / std::fabs(t - S.c[k]));   // dS = |grad phi| / |d phi/dx_k| dx'
+                    for (int d = 0; d < 3; ++d) out.nrm.push_back((x[d] - S.c[d]) / S.r);
+                }
+            }
+        });
+    } else {
+        // D == 2: one outer direction o, height direction k
+        const int o = od[0];
+        const int q = (int)G.x.size();
+        std::vector<double> bo;
+        add_roots(bo, S.c[o], r2 - (lo[k] - S.c[k]) * (lo[k] - S.c[k]));
+        add_roots(bo, S.c[o], r2 - (hi[k] - S.c[k]) * (hi[k] - S.c[k]));
+        const auto po = pieces(lo[o], hi[o], bo);
+        std::vector<double> anchors, XO, WO;
+        add_roots(anchors, S.c[o], r2);   // silhouette: the height roots meet there
+        for (size_t a = 0; a + 1 < po.size(); ++a) {
+            XO.clear();
+            WO.clear();
+            outer_nodes(po[a], po[a + 1], anchors, G, XO, WO);
+            for (size_t i = 0; i < XO.size(); ++i) {
+                const double xo = XO[i], wo = WO[i];
+                double xp[3] = {0, 0, 0};
+                xp[o] = xo;
+                const double s2 = r2 - (xo - S.c[o]) * (xo - S.c[o]);
+                if (mode == 0) {
+                    std::vector<double> bt;
+                    add_roots(bt, S.c[k], s2);
+                    const auto pt = pieces(lo[k], hi[k], bt);
+                    for (size_t b = 0; b + 1 < pt.size(); ++b) {
+                        double xm[3] = {xp[0], xp[1], 0};
+                        xm[k] = 0.5 * (pt[b] + pt[b + 1]);
+                        if ((S.phi(xm, 2) < 0) != (species < 0)) continue;
+                        for (int j = 0; j < q; ++j) {
+                            double x[3] = {xp[0], xp[1], 0};
+                            x[k] = pt[b] + 0.5 * (G.x[j] + 1) * (pt[b + 1] - pt[b]);
+                            out.x.insert(out.x.end(), x, x + 3);
+                            out.w.push_back(wo * 0.5 * (pt[b + 1] - pt[b]) * G.w[j]);
+                        }
+                    }
+                } else if (s2 > 0) {
+                    const double s = std::sqrt(s2);
+                    for (double t : {S.c[k] - s, S.c[k] + s}) {
+                        if (t <= lo[k] || t >= hi[k]) continue;
+                        double x[3] = {xp[0], xp[1], 0};
+                        x[k] = t;
+                        out.x.insert(out.x.end(), x, x + 3);
+                        out.w.push_back(wo * S.r / std::fabs(t - S.c[k]));
+                        for (int d = 0; d < 3; ++d) out.nrm.push_back(d < 2 ? (x[d] - S.c[d]) / S.r : 0.0);
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void sphere_volume_rule(int D, const double* lo, const double* hi, const Sphere& S, int species, int q, QRule& out) {
+    Gauss1 G(q);
+    sphere_rule_rec(D, lo, hi, S, species, 0, G, 0, out);
+}
+
+void sphere_interface_rule(int D, const double* lo, const double* hi, const Sphere& S, int q, QRule& out) {
+    Gauss1 G(q);
+    sphere_rule_rec(D, lo, hi, S, 0, 1, G, 0, out);
+}
+
+// Face rule: the face is a (D-1)-box in the plane x_dir = lo[dir]; the level
+// set restricted to it is a circle (3D) / two points (2D).
+void sphere_face_rule(int D, const double* lo, const double* hi, int dir, const Sphere& S, int species, int q, QRule& out) {
+    Gauss1 G(q);
+    const double xd = lo[dir];
+    const double R2 = S.r * S.r - (xd - S.c[dir]) * (xd - S.c[dir]);
+    if (D == 2) {
+        const int o = 1 - dir;
+        std::vector<double> bp;
+        add_roots(bp, S.c[o], R2);
+        const auto p = pieces(lo[o], hi[o], bp);
+        for (size_t a = 0; a + 1 < p.size(); ++a) {
+            double xm[3] = {0, 0, 0};
+            xm[dir] = xd;
+            xm[o] = 0.5 * (p[a] + p[a + 1]);
+            if ((S.phi(xm, 2) < 0) != (species < 0)) continue;
+            for (int i = 0; i < q; ++i) {
+                double x[3] = {0, 0, 0};
+                x[dir] = xd;
+                x[o] = p[a] + 0.5 * (G.x[i] + 1) * (p[a + 1] - p[a]);
+                out.x.insert(out.x.end(), x, x + 3);
+                out.w.push_back(0.5 * (p[a + 1] - p[a]) * G.w[i]);
+            }
+        }
+        return;
+    }
+    int od[2], no = 0;
+    for (int d = 0; d < 3; ++d) if (d != dir) od[no++] = d;
+    const double rad0 = 0.5 * (lo[od[0]] + hi[od[0]]) - S.c[od[0]];
+    const double rad1 = 0.5 * (lo[od[1]] + hi[od[1]]) - S.c[od[1]];
+    const int k2 = std::fabs(rad0) >= std::fabs(rad1) ? od[0] : od[1];
+    const int o = k2 == od[0] ? od[1] : od[0];
+    const std::vector<double> radii{R2};
+    // pieces never straddle the circle: keep the nodes of the requested species
+    circle_2d(lo[o], hi[o], lo[k2], hi[k2], S.c[o], S.c[k2], radii, G, [&](double xo, double xk, double w) {
+        double x[3];
+        x[dir] = xd;
+        x[o] = xo;
+        x[k2] = xk;
+        if ((S.phi(x, 3) < 0) != (species < 0)) return;
+        out.x.insert(out.x.end(), x, x + 3);
+        out.w.push_back(w);
+    });
+}
+
+// ------------------------------------------------------------------ setup
+void XdgProblem::setup() {
+    D = g->D;
+    Np = basis_count(D, k);
+    const double dd = D, p = k;
+    penalty_deg = std::max((p + 1) * (p + dd) / dd, (p + 1.0) * (p + 1.0));   // XLaplace.cs CoefficientUpdate
+    const int J = g->J;
+    nspec.assign(J, 1);
+    spmask.assign(J, 0);
+    for (int s = 0; s < 2; ++s) { vol[s].assign(J, 0.0); len[s].assign(J, 0.0); frac[s].assign(J, 0.0); agg_target[s].assign(J, -1); }
+    iface_area.assign(J, 0.0);
+    elems.clear();
+    nx.clear(); nw.clear(); nn.clear();
+    auto box = [&](int c, double* lo, double* hi) {
+        int idx[3];
+        g->cell_ijk(c, idx);
+        for (int d = 0; d < 3; ++d) { lo[d] = d < D ? g->nodes[d][idx[d]] : 0; hi[d] = d < D ? g->nodes[d][idx[d] + 1] : 0; }
+    };
+    auto push_rule = [&](const QRule& R, int& r0, int& rn) {
+        r0 = (int)nw.size();
+        rn = R.n();
+        nx.insert(nx.end(), R.x.begin(), R.x.end());
+        nw.insert(nw.end(), R.w.begin(), R.w.end());
+        if (!R.nrm.empty()) nn.insert(nn.end(), R.nrm.begin(), R.nrm.end());
+        else nn.insert(nn.end(), R.x.size(), 0.0);
+    };
+    // classify cells: distance range of the box to the centre
+    std::vector<int> cut(J, 0);
+    std::vector<int> vol_rule0[2], vol_rulen[2], if_r0(J, 0), if_rn(J, 0);
+    for (int s = 0; s < 2; ++s) { vol_rule0[s].assign(J, 0); vol_rulen[s].assign(J, 0); }
+    ncut = 0;
+    for (int c = 0; c < J; ++c) {
+        double lo[3], hi[3];
+        box(c, lo, hi);
+        double dmin2 = 0, dmax2 = 0;
+        for (int d = 0; d < D; ++d) {
+            const double a = lo[d] - S.c[d], b = hi[d] - S.c[d];
+            dmin2 += (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
+            dmax2 += std::max(a * a, b * b);
+        }
+        double V = 1;
+        for (int d = 0; d < D; ++d) V *= hi[d] - lo[d];
+        const double r2 = S.r * S.r;
+        if (dmax2 <= r2) { spmask[c] = 1; vol[0][c] = V; }
+        else if (dmin2 >= r2) { spmask[c] = 2; vol[1][c] = V; }
+        else {
+            // candidate cut cell: compute the species rules
+            QRule RA, RB, RI;
+            sphere_volume_rule(D, lo, hi, S, -1, q, RA);
+            sphere_volume_rule(D, lo, hi, S, +1, q, RB);
+            sphere_interface_rule(D, lo, hi, S, q, RI);
+            double va = 0, vb = 0, ai = 0;
+            for (double w : RA.w) va += w;
+            for (double w : RB.w) vb += w;
+            for (double w : RI.w) ai += w;
+            vol[0][c] = va;
+            vol[1][c] = vb;
+            iface_area[c] = ai;
+            int m = (va > 0 ? 1 : 0) | (vb > 0 ? 2 : 0);
+            spmask[c] = m;
+            if (m == 3) {
+                cut[c] = 1;
+                ++ncut;
+                push_rule(RA, vol_rule0[0][c], vol_rulen[0][c]);
+                push_rule(RB, vol_rule0[1][c], vol_rulen[1][c]);
+                push_rule(RI, if_r0[c], if_rn[c]);
+            }
+        }
+        nspec[c] = (spmask[c] == 3) ? 2 : 1;
+        for (int s = 0; s < 2; ++s) frac[s][c] = vol[s][c] / V;
+    }
+    // faces: per species, area and rule; (cell, dir) '+' faces and boundary faces
+    struct FaceInfo { int a, b, dir, side; double area[2]; int r0[2], rn[2]; bool tensor[2]; };
+    std::vector<FaceInfo> faces;
+    std::vector<double> edge_sum[2];
+    for (int s = 0; s < 2; ++s) edge_sum[s].assign(J, 0.0);
+    for (int c = 0; c < J; ++c) {
+        int idx[3];
+        g->cell_ijk(c, idx);
+        double lo[3], hi[3];
+        box(c, lo, hi);
+        for (int d = 0; d < D; ++d)
+            for (int s = 0; s < 2; ++s) {
+                const bool bnd = s == 0 ? idx[d] == 0 : idx[d] == g->n[d] - 1;
+                if (!bnd && s == 0) continue;   // interior faces once, from the '-' cell
+                FaceInfo F{};
+                F.a = c;
+                F.dir = d;
+                F.side = s;
+                if (bnd) F.b = -1;
+                else { int nb[3] = {idx[0], idx[1], idx[2]}; nb[d]++; F.b = g->cell_index(nb[0], nb[1], nb[2]); }
+                double fl[3], fh[3];
+                for (int e = 0; e < 3; ++e) { fl[e] = lo[e]; fh[e] = hi[e]; }
+                fl[d] = fh[d] = s ? hi[d] : lo[d];
+                double A = 1;
+                for (int e = 0; e < D; ++e) if (e != d) A *= fh[e] - fl[e];
+                // face cut?
+                double dmin2 = 0, dmax2 = 0;
+                for (int e = 0; e < D; ++e) {
+                    const double a = fl[e] - S.c[e], b = fh[e] - S.c[e];
+                    dmin2 += (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
+                    dmax2 += std::max(a * a, b * b);
+                }
+                const double r2 = S.r * S.r;
+                for (int sp = 0; sp < 2; ++sp) { F.area[sp] = 0; F.rn[sp] = 0; F.tensor[sp] = false; }
+                if (dmax2 <= r2) { F.area[0] = A; F.tensor[0] = true; }
+                else if (dmin2 >= r2) { F.area[1] = A; F.tensor[1] = true; }
+                else {
+                    for (int sp = 0; sp < 2; ++sp) {
+                        QRule R;
+                        sphere_face_rule(D, fl, fh, d, S, sp == 0 ? -1 : +1, q, R);
+                        double a = 0;
+                        for (double w : R.w) a += w;
+                        F.area[sp] = a;
+                        if (a > 0) push_rule(R, F.r0[sp], F.rn[sp]);
+                    }
+                }
+                for (int sp = 0; sp < 2; ++sp) {
+                    edge_sum[sp][c] += F.area[sp];
+                    if (F.b >= 0) edge_sum[sp][F.b] += F.area[sp];
+                }
+                faces.push_back(F);
+            }
+    }
+    // face index of every (cell, local face 2d+s)
+    std::vector<int> cell_face((size_t)J * 6, -1);
+    for (size_t fi = 0; fi < faces.size(); ++fi) {
+        const FaceInfo& F = faces[fi];
+        cell_face[(size_t)F.a * 6 + 2 * F.dir + F.side] = (int)fi;
+        if (F.b >= 0) cell_face[(size_t)F.b * 6 + 2 * F.dir + 0] = (int)fi;
+    }
+    // agglomeration (X/AgglomerationAlgorithm.cs:393-460 sources, :2064-2140 direct targets)
+    agg_pairs.clear();
+    n_failed = 0;
+    const double EmptyEdge = 1e-10;
+    for (int sp = 0; sp < 2; ++sp) {
+        std::vector<char> source(J, 0);
+        for (int c = 0; c < J; ++c)
+            if (cut[c]) {
+                double f = std::min(1.0, std::max(0.0, frac[sp][c]));
+                if (f <= agg_threshold) source[c] = 1;
+            }
+        for (int c = 0; c < J; ++c) {
+            if (!source[c]) continue;
+            int idx[3];
+            g->cell_ijk(c, idx);
+            // neighbours in the reference's face order (per direction: '-' then '+')
+            int nbs[6], fidx[6], nnb = 0;
+            double areas[6];
+            for (int d = 0; d < D; ++d)
+                for (int s = 0; s < 2; ++s) {
+                    int nb[3] = {idx[0], idx[1], idx[2]};
+                    nb[d] += s ? 1 : -1;
+                    const bool inside = nb[d] >= 0 && nb[d] < g->n[d];
+                    nbs[nnb] = inside ? g->cell_index(nb[0], nb[1], nb[2]) : -1;
+                    fidx[nnb] = 2 * d + s;
+                    areas[nnb] = 0;
+                    ++nnb;
+                }
+            // cut-edge areas of species sp on the faces of c
+            for (int i = 0; i < nnb; ++i) {
+                const int fi = cell_face[(size_t)c * 6 + fidx[i]];
+                areas[i] = fi >= 0 ? faces[fi].area[sp] : 0.0;
+            }
+            bool nonEmpty = false;
+            for (int i = 0; i < nnb; ++i) if (nbs[i] >= 0 && areas[i] > EmptyEdge) nonEmpty = true;
+            if (vol[sp][c] <= 0) nonEmpty = false;
+            double fmax = -1.0;
+            int tgt = -1;
+            for (int i = 0; i < nnb; ++i) {
+                const int nb = nbs[i];
+                if (nb < 0) continue;
+                if (areas[i] <= EmptyEdge && nonEmpty) continue;
+                if (!((spmask[nb] >> sp) & 1) || source[nb]) continue;   // AggCandidates: species present, not a source
+                if (frac[sp][nb] > fmax) { fmax = frac[sp][nb]; tgt = nb; }
+            }
+            if (tgt < 0) { ++n_failed; continue; }
+            agg_target[sp][c] = tgt;
+            agg_pairs.push_back(c);
+            agg_pairs.push_back(tgt);
+            agg_pairs.push_back(sp);
+        }
+    }
+    // length scales (MultiphaseAgglomerator.cs:870-1030): Volume / (interface area +
+    // cut-edge areas excluding agglomeration edges), accumulated over agglomeration pairs
+    std::vector<double> surf[2], volA[2];
+    for (int sp = 0; sp < 2; ++sp) {
+        surf[sp].assign(J, 0.0);
+        volA[sp] = vol[sp];
+        for (int c = 0; c < J; ++c) surf[sp][c] = ((spmask[c] >> sp) & 1) ? iface_area[c] : 0.0;
+        for (auto& F : faces) {
+            const bool aggEdge = F.b >= 0 && (agg_target[sp][F.a] == F.b || agg_target[sp][F.b] == F.a);
+            if (aggEdge) continue;
+            surf[sp][F.a] += F.area[sp];
+            if (F.b >= 0) surf[sp][F.b] += F.area[sp];
+        }
+        for (int c = 0; c < J; ++c) {
+            const int t = agg_target[sp][c];
+            if (t >= 0) { surf[sp][t] += surf[sp][c]; volA[sp][t] += volA[sp][c]; }
+        }
+        for (int c = 0; c < J; ++c) {
+            const int t = agg_target[sp][c];
+            if (t >= 0) { surf[sp][c] = surf[sp][t]; volA[sp][c] = volA[sp][t]; }
+        }
+        for (int c = 0; c < J; ++c) len[sp][c] = surf[sp][c] > 1e-25 ? volA[sp][c] / surf[sp][c] : 0.0;
+    }
+    // elements
+    const double pb = penalty_base * penalty_deg;
+    for (int c = 0; c < J; ++c)
+        for (int sp = 0; sp < 2; ++sp) {
+            if (!((spmask[c] >> sp) & 1)) continue;
+            XElem E{};
+            E.type = 0; E.cellP = E.cellQ = c; E.spP = E.spQ = sp; E.dir = -1; E.side = 0;
+            E.r0 = cut[c] ? vol_rule0[sp][c] : 0;
+            E.rn = cut[c] ? vol_rulen[sp][c] : 0;
+            if (cut[c] && E.rn == 0) continue;
+            E.nuP = E.nuQ = nu[sp];
+            E.mu = 0;
+            elems.push_back(E);
+        }
+    for (int c = 0; c < J; ++c)
+        if (cut[c] && if_rn[c] > 0) {
+            XElem E{};
+            E.type = 3; E.cellP = E.cellQ = c; E.spP = 0; E.spQ = 1; E.dir = -1; E.side = 0;
+            E.r0 = if_r0[c]; E.rn = if_rn[c];
+            E.nuP = nu[0]; E.nuQ = nu[1];
+            // XLaplace_Interface.GetPenalty (SIP mode): max(1/l_A, 1/l_B) * max(|muA|,|muB|)*sign(muA)
+            E.mu = penalty_base * penalty_deg * std::max(1.0 / len[0][c], 1.0 / len[1][c]) *
+                   std::max(std::fabs(nu[0]), std::fabs(nu[1])) * (nu[0] >= 0 ? 1.0 : -1.0);
+            elems.push_back(E);
+        }
+    for (auto& F : faces)
+        for (int sp = 0; sp < 2; ++sp) {
+            if (F.area[sp] <= 0) continue;
+            XElem E{};
+            E.dir = F.dir;
+            E.spP = E.spQ = sp;
+            E.cellP = F.a;
+            E.nuP = E.nuQ = nu[sp];
+            E.r0 = F.tensor[sp] ? 0 : F.r0[sp];
+            E.rn = F.tensor[sp] ? 0 : F.rn[sp];
+            if (F.b < 0) {
+                E.type = 2; E.cellQ = F.a; E.side = F.side;
+                E.mu = 2.0 * pb / len[sp][F.a];                            // SIPLaplace boundary: 2 * penalty(jIn)
+            } else {
+                if (!((spmask[F.b] >> sp) & 1) || !((spmask[F.a] >> sp) & 1)) continue;
+                E.type = 1; E.cellQ = F.b; E.side = 1;
+                E.mu = pb * std::max(1.0 / len[sp][F.a], 1.0 / len[sp][F.b]) * nu[sp];   // XLaplace_Bulk.GetPenalty
+            }
+            elems.push_back(E);
+        }
+    // DOF maps
+    off_full.assign(J + 1, 0);
+    bsz_full.assign(J, 0);
+    off_red.assign(J + 1, 0);
+    bsz_red.assign(J, 0);
+    for (int c = 0; c < J; ++c) {
+        int nf = 0, nr = 0;
+        for (int sp = 0; sp < 2; ++sp)
+            if ((spmask[c] >> sp) & 1) { ++nf; if (agg_target[sp][c] < 0) ++nr; }
+        bsz_full[c] = nf * Np;
+        bsz_red[c] = nr * Np;
+        off_full[c + 1] = off_full[c] + bsz_full[c];
+        off_red[c + 1] = off_red[c] + bsz_red[c];
+    }
+    N_full = off_full[J];
+    N_red = off_red[J];
+}
+
+}  // namespace bh

@@ -63,7 +63,8 @@ int bh_event_elapsed(bh_ctx* ctx, int slot0, int slot1, float* ms);
  * Cartesian grid from node arrays, as Grid2D.Cartesian2DGrid
  * (BoSSS.Foundation.Grid/Grid2D.cs:396-600, GlobalID = i*nY + j at :586) and
  * Grid3D.Cartesian3DGrid (Grid3D.cs:88-300, GlobalID = i + j*nX + k*nX*nY at
- * :261).  z may be NULL for D = 2. */
+ * :261).  z may be NULL for D = 2.  ctx may be NULL: such a grid serves the
+ * host-side set-up only (XDG cut-cell rules and agglomeration, bh_xdg_create). */
 int bh_grid_cartesian(bh_ctx* ctx, int D, const double* x, int nx1, const double* y, int ny1,
                       const double* z, int nz1, bh_grid** grid);
 int bh_grid_destroy(bh_grid* grid);
@@ -200,6 +201,57 @@ int bh_euler_info(bh_euler* e, int64_t* N, int* Np);
 int bh_euler_rhs(bh_euler* e, const void* U, void* R, int prec, int flags);
 /* `steps` TVD3 steps of size dt, in place */
 int bh_euler_tvd3(bh_euler* e, void* U, double dt, int steps, int prec, int flags);
+
+/* ---------------------------------------------------------------- XDG two-phase Poisson
+ * Cut-cell SIP Poisson with a spherical level set phi = |x - c| - r, species
+ * A (phi < 0, diffusion nu_in) and B (phi > 0, nu_out): XLaplace_Bulk and
+ * XLaplace_Interface in SIP mode (L3-solution/BoSSS.Solution.XNSECommon/
+ * XLaplace.cs:50-346), assembled as XEvaluatorLinear.ComputeMatrix does
+ * (X/XSpatialOperatorMk2_XEvaluators.cs:163) into (cell, species) blocks;
+ * cut-cell rules by dimension reduction (X/Quadrature/SayeGaussRule_*.cs);
+ * agglomeration of cut cells with volume fraction <= agg_threshold
+ * (X/AgglomerationAlgorithm.cs:393-460, 2064-2140) applied as
+ * CellAgglomerator.ManipulateMatrixAndRHS (X/CellAgglomerator.cs:1755).
+ * X/ = L2-foundation/BoSSS.Foundation.XDG.  Dirichlet boundary everywhere.
+ * DOF layout: cell-major, per cell the present species blocks (A then B),
+ * Np modes each; the agglomerated ("reduced") system drops source blocks. */
+typedef struct bh_xdg bh_xdg;
+/* quad_order: Gauss points per 1D piece of the cut-cell rules (<= 0: k + 1) */
+int bh_xdg_create(bh_grid* grid, int k, const double* center, double radius, double nu_in, double nu_out,
+                  double penalty_base, double agg_threshold, int quad_order, bh_xdg** x);
+int bh_xdg_destroy(bh_xdg* x);
+/* info[12]: N_full, N_red, cut cells, agglomeration pairs, failed sources, elements,
+ * cut-rule nodes, full matrix values, reduced matrix values, boundary nodes, Np, J */
+int bh_xdg_info(bh_xdg* x, int64_t* info);
+/* per cell: species mask (bit 0 A, bit 1 B); volumes [2][J]; interface area [J];
+ * agglomerated length scales [2][J] (X/MultiphaseAgglomerator.cs:870-1030) */
+int bh_xdg_cells(bh_xdg* x, int* spmask, double* vol, double* iface_area, double* len);
+/* agglomeration pairs (source cell, target cell, species) in discovery order */
+int bh_xdg_agglomeration(bh_xdg* x, int* pairs);
+/* element list (ints[9]: type 0 volume / 1 interior face / 2 boundary face / 3 interface,
+ * cellP, cellQ, speciesP, speciesQ, dir, side, rule start, rule length (0: tensor Gauss);
+ * dbls[3]: nuP, nuQ, penalty) and the cut-rule nodes (xyz), weights and interface normals */
+int bh_xdg_elements(bh_xdg* x, int* ints, double* dbls, double* nx, double* nw, double* nn);
+/* which = 0 full, 1 agglomerated: DOF offsets [J+1], block rows [J+1], block columns,
+ * value offsets [nblocks+1] and column-major block values (FP64) */
+int bh_xdg_matrix(bh_xdg* x, int which, int64_t* dof_off, int64_t* rowptr, int* col, int64_t* voff, double* vals);
+/* Dirichlet data: physical boundary quadrature nodes (3 per node) */
+int bh_xdg_bnd_nodes(bh_xdg* x, double* xyz);
+/* right-hand side from species-constant sources f[2] and Dirichlet values g at the boundary nodes (or NULL: 0) */
+int bh_xdg_set_rhs(bh_xdg* x, const double* f, const double* g);
+/* which = 0: full right-hand side (host); 1: agglomerated right-hand side */
+int bh_xdg_rhs(bh_xdg* x, int which, double* b, int flags);
+/* y <- alpha A x + beta y with the full (0) or agglomerated (1) matrix (BlockMsrMatrix.SpMV) */
+int bh_xdg_spmv(bh_xdg* x, int which, double alpha, const void* xv, double beta, void* y, int prec, int flags);
+/* full-space DG coefficients from agglomerated ones (host vectors) */
+int bh_xdg_extend(bh_xdg* x, const double* ured, double* ufull);
+/* two-level preconditioner of the agglomerated system: block-Jacobi (FP32/FP64) on
+ * the cell blocks and a species-merged degree-`coarse_degree` coarse space solved
+ * by the aggregation multigrid (K-cycle with kcycle_inner inner steps) */
+int bh_xdg_precond_create(bh_xdg* x, int coarse_degree, int min_coarse, double omega, int pre, int post,
+                          int kcycle_inner, int prec, bh_solver** solver);
+/* FGMRES on the agglomerated system (AS/FlexGMRES.cs), FP64 */
+int bh_xdg_fgmres_create(bh_xdg* x, bh_solver* precond, int restart, double tol, int maxit, bh_solver** solver);
 
 int bh_solver_solve(bh_solver* solver, double* x, const double* b, int flags);
 int bh_solver_stats(bh_solver* solver, int* iterations, int* converged, double* res0, double* res);

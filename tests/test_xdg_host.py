@@ -1,0 +1,138 @@
+"""XDG set-up on the host (no GPU): the oracle's closed-form geometry, the
+product's cut-cell rules against it, and the agglomeration map bit-exact.
+
+The product's host set-up runs through the C ABI with a grid created without a
+device context (bh_grid_cartesian with ctx = NULL)."""
+import math
+
+import numpy as np
+import pytest
+
+import bosss_amd as B
+from oracle import xdg as OX
+
+
+# ---------------------------------------------------------------- oracle pins
+def test_disk_rect_closed_forms():
+    R = 0.3
+    assert OX.disk_rect_area(R, 0, 0, -1, 1, -1, 1) == pytest.approx(math.pi * R * R, rel=1e-15)
+    assert OX.disk_rect_area(R, 0, 0, 0, 1, 0, 1) == pytest.approx(math.pi * R * R / 4, rel=1e-15)
+    assert OX.disk_rect_area(R, 0, 0, -0.1, 0.1, -0.05, 0.05) == pytest.approx(0.02, rel=1e-15)
+    assert OX.disk_rect_area(R, 0, 0, 0.4, 1, -1, 1) == 0.0
+    # half-disk strip: x in [0, R] -> half the disk
+    assert OX.disk_rect_area(R, 0, 0, 0, R, -1, 1) == pytest.approx(math.pi * R * R / 2, rel=1e-15)
+    # circular segment beyond x = a: R^2 acos(a/R) - a sqrt(R^2 - a^2)
+    a = 0.1
+    seg = R * R * math.acos(a / R) - a * math.sqrt(R * R - a * a)
+    assert OX.disk_rect_area(R, 0, 0, a, 1, -1, 1) == pytest.approx(seg, rel=1e-14)
+    assert OX.circle_rect_angle(R, 0, 0, -1, 1, -1, 1) == pytest.approx(2 * math.pi, rel=1e-15)
+    assert OX.circle_rect_angle(R, 0, 0, 0, 1, 0, 1) == pytest.approx(math.pi / 2, rel=1e-15)
+    assert OX.circle_rect_angle(R, 0, 0, a, 1, -1, 1) == pytest.approx(2 * math.acos(a / R), rel=1e-14)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_oracle_geometry_sums(D):
+    """Per-cell volumes / interface areas of the oracle sum to the ball's."""
+    r = 0.27
+    c = [0.5, 0.47, 0.52][:D]
+    x = np.linspace(0, 1, 9)
+    G = OX.Geometry([x] * D, c, r)
+    if D == 3:
+        V, A = 4 / 3 * math.pi * r ** 3, 4 * math.pi * r * r
+    else:
+        V, A = math.pi * r * r, 2 * math.pi * r
+    assert G.vol[0].sum() == pytest.approx(V, rel=1e-12)
+    assert G.iface.sum() == pytest.approx(A, rel=1e-11)
+    assert np.allclose(G.vol.sum(0), G.cellvol, rtol=1e-13)
+
+
+def _product(D, n, c, r, q, ctx=None, k=2, nu=(1.0, 1000.0), agg=0.1):
+    x = np.linspace(0, 1, n + 1)
+    g = B.Grid(ctx, *([x] * D))
+    X = B.XdgPoisson(g, k, c, r, nu[0], nu[1], agg_threshold=agg, quad_order=q)
+    return x, g, X
+
+
+# ---------------------------------------------------------------- cut-cell rules
+@pytest.mark.parametrize("D,n,q,tol_v,tol_a", [(2, 10, 6, 1e-10, 1e-9), (2, 10, 10, 1e-13, 1e-13),
+                                               (3, 12, 6, 1e-8, 1e-7), (3, 12, 10, 1e-12, 1e-11)])
+def test_cut_rules_vs_oracle(D, n, q, tol_v, tol_a):
+    """Species volumes and interface areas per cell, product rules vs closed forms."""
+    r = 0.27 if D == 3 else 0.31
+    c = [0.5, 0.47, 0.52][:D]
+    x, g, X = _product(D, n, c, r, q)
+    G = OX.Geometry([x] * D, c, r)
+    cp = X.cells()
+    h = 1.0 / n
+    assert np.array_equal(cp["spmask"], G.spmask)
+    assert np.abs(cp["vol"] - G.vol).max() <= tol_v * h ** D
+    assert np.abs(cp["iface_area"] - G.iface).max() <= tol_a * h ** (D - 1)
+
+
+def test_cut_rule_moments():
+    """Second moments of the ball and its surface from the product's rules
+    (int (x-c)^2 dV = 4 pi r^5 / 15, int (x-c)^2 dS = 4 pi r^4 / 3, odd moments 0)."""
+    r, c = 0.27, np.array([0.5, 0.47, 0.52])
+    x, g, X = _product(3, 12, c, r, 8)
+    e = X.elements()
+    ints, nx, nw, nn = e["ints"], e["nx"], e["nw"], e["nn"]
+    G = OX.Geometry([x] * 3, c, r)
+    m2v = np.zeros(3)
+    m1v = np.zeros(3)
+    m2s = np.zeros(3)
+    ns = np.zeros(3)
+    for (t, P, Q, sP, sQ, d, side, r0, rn) in ints:
+        if t == 0 and sP == 0:
+            if rn > 0:
+                X_, W_ = nx[r0:r0 + rn], nw[r0:r0 + rn]
+            else:
+                lo, hi = G.box(P)
+                X_, W_ = OX._tensor_nodes(3, 7, lo, hi - lo, 0)   # exact for degree 2
+            m2v += (W_[:, None] * (X_ - c) ** 2).sum(0)
+            m1v += (W_[:, None] * (X_ - c)).sum(0)
+        if t == 3:
+            X_, W_ = nx[r0:r0 + rn], nw[r0:r0 + rn]
+            m2s += (W_[:, None] * (X_ - c) ** 2).sum(0)
+            ns += (W_[:, None] * nn[r0:r0 + rn]).sum(0)      # closed surface: int n dS = 0
+    assert np.allclose(m2v, 4 * math.pi * r ** 5 / 15, rtol=1e-10)
+    assert np.abs(m1v).max() <= 1e-12
+    assert np.allclose(m2s, 4 * math.pi * r ** 4 / 3, rtol=1e-10)
+    assert np.abs(ns).max() <= 1e-11
+
+
+# ---------------------------------------------------------------- agglomeration
+@pytest.mark.parametrize("D,n,c,r,thr", [(3, 12, [0.5, 0.47, 0.52], 0.27, 0.1), (3, 16, [0.43, 0.55, 0.51], 0.22, 0.1),
+                                          (2, 24, [0.52, 0.47], 0.31, 0.1), (2, 24, [0.52, 0.47], 0.31, 0.3)])
+def test_agglomeration_bit_exact(D, n, c, r, thr):
+    """Agglomeration pairs (source, target, species) equal the oracle's, which
+    derives them from closed-form volumes and face areas (integer work: exact)."""
+    x, g, X = _product(D, n, c, r, 6, agg=thr)
+    G = OX.Geometry([x] * D, c, r)
+    pairs, target, failed = G.agglomerate(thr)
+    assert not failed
+    got = X.agglomeration()
+    assert np.array_equal(got, pairs)
+    # invariants: sources are cut with fraction <= thr, targets are face neighbours with fraction > thr
+    frac = G.vol / G.cellvol
+    for (s_, t_, sp) in got:
+        assert G.cut[s_] and frac[sp, s_] <= thr
+        assert frac[sp, t_] > thr
+        d = np.abs(np.array(G.ijk(s_)) - np.array(G.ijk(t_)))
+        assert d.sum() == 1
+    # length scales from the oracle's own areas
+    cp = X.cells()
+    L = G.length_scales(target)
+    ok = L > 0
+    assert np.allclose(cp["len"][ok], L[ok], rtol=1e-7)
+    assert np.all(cp["len"][~ok] == 0)
+
+
+def test_dof_maps():
+    """Full DOF map: one Np block per present species; the agglomerated map drops
+    the source blocks (closed-form counts)."""
+    x, g, X = _product(3, 12, [0.5, 0.47, 0.52], 0.27, 3)
+    cp = X.cells()
+    nsp = np.array([bin(int(m)).count("1") for m in cp["spmask"]])
+    assert X.N_full == nsp.sum() * X.Np
+    assert X.N == X.N_full - X.npairs * X.Np
+    assert X.ncut == int((cp["spmask"] == 3).sum())
