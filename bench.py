@@ -86,7 +86,7 @@ def solve_leg(B, ctx, grid, op, n, k, out):
     ctx.sync()
     t_asm = time.perf_counter() - t0
     res = {"matrix_assembly_s": t_asm}
-    for name, prec in (("plain_fp64", B.BH_FP64), ("mixed_fp32", B.BH_FP32)):
+    for name, prec in (("plain_fp64", B.BH_FP64), ("mixed_fp32", B.BH_FP32), ("mixed_fp16_mfma", B.BH_FP16_MFMA)):
         t0 = time.perf_counter()
         mg = B.PMultigrid(op, M=M, degrees=[k, 1], min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2, prec=prec)
         ctx.sync()
@@ -108,6 +108,7 @@ def solve_leg(B, ctx, grid, op, n, k, out):
                      "converged": st["converged"], "true_rel_residual": relres}
         del fg, mg, x, r
     res["speedup_mixed_vs_plain"] = res["plain_fp64"]["solver_run_s"] / res["mixed_fp32"]["solver_run_s"]
+    res["speedup_mfma_vs_plain"] = res["plain_fp64"]["solver_run_s"] / res["mixed_fp16_mfma"]["solver_run_s"]
     out["time_to_solve"] = res
 
 
@@ -340,6 +341,29 @@ def main():
                      "algorithmic_flop": flops, "algorithmic_bytes": hbm_bytes,
                      "hbm_frac_at_algorithmic_bytes": hbm_bytes / (kern_ms * 1e-3) / PEAK_HBM},
     }
+    # reduced-precision applies of the same operator (preconditioner use): FP32
+    # vectors; vector FP32 FMA vs FP16 MFMA contractions with FP32 accumulation
+    u64 = u.download()
+    ref = y.download()
+    u32 = B.DeviceArray.from_host(ctx, u64.astype(np.float32))
+    y32 = B.DeviceArray(ctx, N, np.float32)
+    bytes32 = 2 * N * 4 + grid.J * 4
+    for name, prec, mfma_peak in (("apply_fp32", B.BH_FP32, 157.3e12), ("apply_fp16_mfma", B.BH_FP16_MFMA, 2.5e15)):
+        for _ in range(3):
+            op.Evaluate(1.0, 0.0, y32, u32, affine=False, prec=prec)
+        ctx.sync()
+        ctx.record(0)
+        for _ in range(args.steps):
+            op.Evaluate(1.0, 0.0, y32, u32, affine=False, prec=prec)
+        ctx.record(1)
+        ctx.sync()
+        ms = ctx.elapsed_ms(0, 1) / args.steps
+        err = float(np.linalg.norm(y32.download().astype(np.float64) - ref) / np.linalg.norm(ref))
+        out[name] = {"kernel_ms": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": bytes32,
+                     "hbm_GBps": bytes32 / (ms * 1e-3) / 1e9, "hbm_frac": bytes32 / (ms * 1e-3) / PEAK_HBM,
+                     "algorithmic_flop": flops, "TFLOPs": flops / (ms * 1e-3) / 1e12,
+                     "compute_frac": flops / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}
+    del u32, y32
     if not args.no_solve:
         solve_leg(B, ctx, grid, op, n, k, out)
     if not args.no_xdg:

@@ -381,10 +381,11 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
         Staged sy(ctx, y, N * sizeof(double), dev, beta != 0.0);
         sip_apply(op, alpha, su.dptr, beta, sy.dptr, affine != 0, BH_FP64);
         sy.download(y, N * sizeof(double), dev);
-    } else if (prec == BH_FP32) {
-        BH_CHECK(!affine || !op->has_affine, "FP32 evaluate: linear part only");
+    } else if (prec == BH_FP32 || prec == BH_FP16_MFMA) {
+        const bool mfma = prec == BH_FP16_MFMA;
+        BH_CHECK(mfma || !affine || !op->has_affine, "FP32 evaluate: linear part only");
         if (dev) {
-            sip_apply(op, alpha, u, beta, y, false, BH_FP32);   // float* device vectors
+            sip_apply(op, alpha, u, beta, y, mfma && affine, prec);   // float* device vectors
         } else {
             DevBuf d64(N * sizeof(double)), u32(N * sizeof(float)), y32(N * sizeof(float));
             ctx->upload(d64.p, u, N * sizeof(double));
@@ -393,7 +394,7 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
                 ctx->upload(d64.p, y, N * sizeof(double));
                 convert_f64_f32(ctx, d64.as<double>(), y32.as<float>(), N);
             }
-            sip_apply(op, alpha, u32.p, beta, y32.p, false, BH_FP32);
+            sip_apply(op, alpha, u32.p, beta, y32.p, mfma && affine, prec);
             convert_f32_f64(ctx, y32.as<float>(), d64.as<double>(), N);
             ctx->download(y, d64.p, N * sizeof(double));
         }
@@ -563,10 +564,13 @@ int bh_fgmres_create(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double tol
 int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int min_coarse, double omega,
                   int pre, int post, int kcycle_inner, int prec, int matrix_free, bh_solver** out) {
     BH_API_BEGIN
-    BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_pmg_create: precision must be FP64 or FP32");
+    BH_CHECK(prec == BH_FP64 || prec == BH_FP32 || prec == BH_FP16_MFMA,
+             "bh_pmg_create: precision must be FP64, FP32 or FP16_MFMA");
     BH_CHECK(pre >= 1 && post >= 0, "bh_pmg_create: need pre >= 1, post >= 0");
+    BH_CHECK(prec != BH_FP16_MFMA || matrix_free, "bh_pmg_create: FP16_MFMA needs the matrix-free fine level");
     auto mg = std::make_unique<PMultigrid>();
-    mg->prec = prec;
+    mg->prec = prec == BH_FP16_MFMA ? BH_FP32 : prec;
+    mg->mfma = prec == BH_FP16_MFMA;
     mg->omega = omega;
     mg->pre = pre;
     mg->post = post;

@@ -148,6 +148,7 @@ struct bh_op {
     std::vector<double> h_nu;
     bh::DevBuf nu, nu32;
     bh::DevBuf tab64, tab32;    // device copies of the tables: G | T | Nn | vw | fw
+    bh::DevBuf tab16;           // FP16 MFMA tables (built on first FP16-MFMA apply)
     size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0;
     // boundary values and RHS -> affine vector
     std::vector<double> gbnd;
@@ -177,6 +178,8 @@ struct bh_solver {
 namespace bh {
 // kernels / launchers (defined in the .hip files)
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec);
+// FP32 vectors, FP16 MFMA contractions with FP32 accumulation
+void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine);
 void sip_assemble(bh_op* op, bh_bsr* M);
 void bsr_spmv(bh_bsr* A, double alpha, const void* x, double beta, void* y, int prec);
 void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32);

@@ -39,6 +39,7 @@ struct MgLevel {
 
 struct PMultigrid : bh_solver {
     int prec = BH_FP64;
+    bool mfma = false;             // FP32 cycle: matrix-free level 0 on the FP16 matrix cores
     double omega = 1.0;
     int pre = 1, post = 1;
     bool matrix_free = true;

@@ -327,7 +327,7 @@ void PMultigrid::residual(int l, const T* b, const T* x, T* r) {
     const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
     if (L.op) {
         vcopy<T>(ctx, b, r, L.N);
-        sip_apply(L.op, -1.0, x, 1.0, r, false, p);
+        sip_apply(L.op, -1.0, x, 1.0, r, false, (p == BH_FP32 && mfma) ? BH_FP16_MFMA : p);
     } else {
         bsr_residual(L.A, b, x, r, p);
     }

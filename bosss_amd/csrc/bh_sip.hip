@@ -346,6 +346,8 @@ void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, boo
     } else if (prec == BH_FP32) {
         if (add_affine && op->has_affine) throw Error(-3, "FP32 evaluate supports the linear part only");
         dispatch_apply<float>(op, make_args<float>(op, u, y, alpha, beta, false));
+    } else if (prec == BH_FP16_MFMA) {
+        sip_apply_mfma(op, alpha, (const float*)u, beta, (float*)y, add_affine);
     } else {
         throw Error(-3, "sip_apply: precision not supported by this entry point");
     }

@@ -149,7 +149,8 @@ int bh_fgmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, do
  *     steps preconditioned by its own V-cycle (K-cycle, the Krylov-accelerated
  *     coarse solve of AS/OrthonormalizationMultigrid.cs / LevelPmg's low-order
  *     solve, AS/LevelPmg.cs:411); 0: plain V-cycle;
- *   prec BH_FP64 or BH_FP32 (whole cycle in that precision);
+ *   prec BH_FP64 or BH_FP32 (whole cycle in that precision); BH_FP16_MFMA: FP32
+ *     cycle whose matrix-free finest level runs on the FP16 matrix cores;
  *   matrix_free != 0: the finest level's residuals use the matrix-free operator.
  * fine_mat: the assembled operator of `op` (NULL: assembled internally). */
 int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int min_coarse, double omega,

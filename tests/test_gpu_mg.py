@@ -111,7 +111,7 @@ def test_fp32_cycle_close_to_fp64(ctx):
     assert rel(x32, x64) < 1e-4
 
 
-@pytest.mark.parametrize("prec", [B.BH_FP64, B.BH_FP32])
+@pytest.mark.parametrize("prec", [B.BH_FP64, B.BH_FP32, B.BH_FP16_MFMA])
 def test_fgmres_multigrid_solve(ctx, prec):
     """FGMRES(30) + p/h multigrid (K-cycle) to 1e-8 true FP64 residual, 16^3 k=3, log-uniform nu."""
     _say(f"case fgmres+mg prec={prec}")
