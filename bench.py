@@ -86,6 +86,27 @@ def solve_leg(B, ctx, grid, op, n, k, out):
     ctx.sync()
     t_asm = time.perf_counter() - t0
     res = {"matrix_assembly_s": t_asm}
+    # BlockMsrMatrix SpMV of the assembled operator (20x20 blocks, HBM-bound)
+    nblk = M.nnzb
+    rng = np.random.default_rng(7)
+    for name, prec, dt in (("bsr_spmv_fp64", B.BH_FP64, np.float64), ("bsr_spmv_fp32", B.BH_FP32, np.float32)):
+        xv = B.DeviceArray.from_host(ctx, rng.standard_normal(N).astype(dt))
+        yv = B.DeviceArray(ctx, N, dt)
+        for _ in range(3):
+            M.SpMV(1.0, xv, 0.0, yv, prec=prec)
+        ctx.sync()
+        reps = 20
+        ctx.record(2)
+        for _ in range(reps):
+            M.SpMV(1.0, xv, 0.0, yv, prec=prec)
+        ctx.record(3)
+        ctx.sync()
+        ms = ctx.elapsed_ms(2, 3) / reps
+        es = np.dtype(dt).itemsize
+        nbytes = nblk * M.bs * M.bs * es + nblk * 4 + (M.nrows + 1) * 8 + 2 * N * es
+        res[name] = {"ms": ms, "algorithmic_bytes": int(nbytes), "GBps": nbytes / (ms * 1e-3) / 1e9,
+                     "hbm_frac": nbytes / (ms * 1e-3) / PEAK_HBM}
+        del xv, yv
     for name, prec in (("plain_fp64", B.BH_FP64), ("mixed_fp32", B.BH_FP32), ("mixed_fp16_mfma", B.BH_FP16_MFMA)):
         t0 = time.perf_counter()
         mg = B.PMultigrid(op, M=M, degrees=[k, 1], min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2, prec=prec)
@@ -281,6 +302,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-xdg", action="store_true")
     ap.add_argument("--no-euler", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy (torch) as the reference kernel for HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU kernels; the multi-GPU "
@@ -341,6 +364,16 @@ def main():
                      "algorithmic_flop": flops, "algorithmic_bytes": hbm_bytes,
                      "hbm_frac_at_algorithmic_bytes": hbm_bytes / (kern_ms * 1e-3) / PEAK_HBM},
     }
+    if args.copy_probe:
+        # known-size copy kernel (copy16_k): FETCH_SIZE / WRITE_SIZE of it calibrate the counters
+        nbytes = 256 * 1024 * 1024
+        src = B.DeviceArray.from_host(ctx, np.ones(nbytes // 4, np.float32))
+        dst = B.DeviceArray(ctx, nbytes // 4, np.float32)
+        for _ in range(3):
+            dst.copy_from(src)
+        ctx.sync()
+        out["copy_probe"] = {"kernel": "copy16_k", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}
+        del src, dst
     # reduced-precision applies of the same operator (preconditioner use): FP32
     # vectors; vector FP32 FMA vs FP16 MFMA contractions with FP32 accumulation
     u64 = u.download()

@@ -45,6 +45,7 @@ SIGNATURES = {
     "bh_free": (_c_i, [_c_p, _c_p]),
     "bh_memcpy_h2d": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
     "bh_memcpy_d2h": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
+    "bh_memcpy_d2d": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
     "bh_event_record": (_c_i, [_c_p, _c_i]),
     "bh_event_elapsed": (_c_i, [_c_p, _c_i, _c_i, ctypes.POINTER(_c_f)]),
     "bh_grid_cartesian": (_c_i, [_c_p, _c_i, _c_p, _c_i, _c_p, _c_i, _c_p, _c_i, _PP]),
@@ -175,6 +176,12 @@ class DeviceArray:
         out = np.empty(self.n, self.dtype)
         _check(lib().bh_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes))
         return out
+
+    def copy_from(self, other):
+        """device-to-device copy of a same-sized array (bh_memcpy_d2d)."""
+        assert other.n == self.n and other.dtype == self.dtype
+        _check(lib().bh_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.n * self.dtype.itemsize))
+        return self
 
     def view(self, offset, n):
         """non-owning view of elements [offset, offset+n) (device-pointer argument)."""
@@ -609,10 +616,10 @@ class XdgPoisson(_Owned):
         _check(lib().bh_xdg_create(grid.h, k, c.ctypes.data, radius, nu_in, nu_out, penalty_base, agg_threshold,
                                    quad_order, ctypes.byref(h)))
         self._register(self.ctx, h.value)
-        info = np.zeros(12, np.int64)
+        info = np.zeros(13, np.int64)
         _check(lib().bh_xdg_info(self.h, info.ctypes.data))
         (self.N_full, self.N, self.ncut, self.npairs, self.nfailed, self.nelem, self.nnodes, self.nvals_full,
-         self.nvals, self.nbnd, self.Np, self.J) = [int(v) for v in info]
+         self.nvals, self.nbnd, self.Np, self.J, self.nunits) = [int(v) for v in info]
 
     def cells(self):
         J = self.J
@@ -640,11 +647,12 @@ class XdgPoisson(_Owned):
         return dict(ints=ints, dbls=dbls, nx=nx, nw=nw, nn=nn)
 
     def matrix(self, which=1):
-        """(dof_off, rowptr, col, voff, vals) of the full (0) or agglomerated (1) matrix."""
-        J = self.J
+        """(dof_off, rowptr, col, voff, vals) of the full (0, cell rows) or agglomerated
+        (1, species-unit rows) matrix."""
+        nr = self.nunits if which else self.J
         nv = self.nvals if which else self.nvals_full
-        off = np.zeros(J + 1, np.int64)
-        rp = np.zeros(J + 1, np.int64)
+        off = np.zeros(nr + 1, np.int64)
+        rp = np.zeros(nr + 1, np.int64)
         _check(lib().bh_xdg_matrix(self.h, which, off.ctypes.data, rp.ctypes.data, None, None, None))
         nb = int(rp[-1])
         col = np.zeros(nb, np.int32)
@@ -657,7 +665,7 @@ class XdgPoisson(_Owned):
         import scipy.sparse as sp
         off, rp, col, voff, vals = self.matrix(which)
         rows, cols, data = [], [], []
-        for r in range(self.J):
+        for r in range(len(off) - 1):
             R = int(off[r + 1] - off[r])
             for p in range(rp[r], rp[r + 1]):
                 c = col[p]

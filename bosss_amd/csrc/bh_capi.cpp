@@ -180,6 +180,11 @@ int bh_memcpy_d2h(bh_ctx* c, void* dst, const void* src, size_t bytes) {
     c->download(dst, src, bytes);
     BH_API_END
 }
+int bh_memcpy_d2d(bh_ctx* c, void* dst, const void* src, size_t bytes) {
+    BH_API_BEGIN
+    copy_bytes(c, dst, src, bytes);
+    BH_API_END
+}
 int bh_event_record(bh_ctx* c, int slot) {
     BH_API_BEGIN
     BH_CHECK(slot >= 0 && slot < 8, "event slot out of range");
@@ -827,10 +832,11 @@ int bh_xdg_info(bh_xdg* X, int64_t* info) {
     info[5] = (int64_t)p.elems.size();
     info[6] = (int64_t)p.nw.size();
     info[7] = X->Afull.nvals;
-    info[8] = X->Ared.nvals;
+    info[8] = X->Ared ? X->Ared->nnzb * p.Np * p.Np : 0;
     info[9] = (int64_t)X->bnd_w.size();
     info[10] = p.Np;
     info[11] = p.g->J;
+    info[12] = X->Ared ? X->Ared->nrows : 0;
     BH_API_END
 }
 
@@ -875,12 +881,23 @@ int bh_xdg_matrix(bh_xdg* X, int which, int64_t* dof_off, int64_t* rowptr, int* 
     BH_API_BEGIN
     BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
     BH_CHECK(which == 0 || which == 1, "bh_xdg_matrix: which must be 0 (full) or 1 (agglomerated)");
-    const Vbsr& M = which ? X->Ared : X->Afull;
-    if (dof_off) std::copy(M.roff.begin(), M.roff.end(), dof_off);
-    if (rowptr) std::copy(M.rowptr.begin(), M.rowptr.end(), rowptr);
-    if (col) std::copy(M.col.begin(), M.col.end(), col);
-    if (voff) std::copy(M.voff.begin(), M.voff.end(), voff);
-    if (vals) X->ctx->download(vals, M.vals.p, M.nvals * sizeof(double));
+    if (which == 0) {
+        const Vbsr& M = X->Afull;
+        if (dof_off) std::copy(M.roff.begin(), M.roff.end(), dof_off);
+        if (rowptr) std::copy(M.rowptr.begin(), M.rowptr.end(), rowptr);
+        if (col) std::copy(M.col.begin(), M.col.end(), col);
+        if (voff) std::copy(M.voff.begin(), M.voff.end(), voff);
+        if (vals) X->ctx->download(vals, M.vals.p, M.nvals * sizeof(double));
+    } else {
+        // species units of Np rows: uniform blocks
+        const bh_bsr& M = *X->Ared;
+        const int64_t b2 = (int64_t)M.bs * M.bs;
+        for (int u = 0; dof_off && u <= M.nrows; ++u) dof_off[u] = (int64_t)u * M.bs;
+        if (rowptr) std::copy(M.h_rowptr.begin(), M.h_rowptr.end(), rowptr);
+        if (col) std::copy(M.h_col.begin(), M.h_col.end(), col);
+        for (int64_t p = 0; voff && p <= M.nnzb; ++p) voff[p] = p * b2;
+        if (vals) X->ctx->download(vals, M.vals.p, M.nnzb * b2 * sizeof(double));
+    }
     BH_API_END
 }
 
@@ -921,13 +938,14 @@ int bh_xdg_spmv(bh_xdg* X, int which, double alpha, const void* x, double beta, 
     BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
     BH_CHECK(which == 0 || which == 1, "bh_xdg_spmv: which must be 0 (full) or 1 (agglomerated)");
     BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_xdg_spmv: precision must be FP64 or FP32");
-    Vbsr& M = which ? X->Ared : X->Afull;
     const bool dev = flags & BH_DEVICE_PTR;
     const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
-    Staged sx(X->ctx, x, M.N * es, dev, true);
-    Staged sy(X->ctx, y, M.N * es, dev, beta != 0.0);
-    M.spmv(alpha, sx.dptr, beta, sy.dptr, prec);
-    sy.download(y, M.N * es, dev);
+    const int64_t N = which ? X->P->N_red : X->P->N_full;
+    Staged sx(X->ctx, x, N * es, dev, true);
+    Staged sy(X->ctx, y, N * es, dev, beta != 0.0);
+    if (which) bsr_spmv(X->Ared.get(), alpha, sx.dptr, beta, sy.dptr, prec);
+    else X->Afull.spmv(alpha, sx.dptr, beta, sy.dptr, prec);
+    sy.download(y, N * es, dev);
     X->ctx->sync();
     BH_API_END
 }
@@ -947,7 +965,8 @@ int bh_xdg_precond_create(bh_xdg* X, int coarse_degree, int min_coarse, double o
     BH_CHECK(pre >= 1 && post >= 0, "bh_xdg_precond_create: need pre >= 1, post >= 0");
     BH_CHECK(coarse_degree >= 0 && coarse_degree <= X->P->k, "bh_xdg_precond_create: coarse degree out of range");
     auto P = std::make_unique<XdgPrecond>();
-    P->A = &X->Ared;
+    P->A = X->Ared.get();
+    P->Dc = &X->Dcell;
     P->prec = prec;
     P->Np = X->P->Np;
     P->omega = omega;
@@ -963,8 +982,8 @@ int bh_xdg_fgmres_create(bh_xdg* X, bh_solver* P, int restart, double tol, int m
     BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
     BH_CHECK(restart >= 1 && restart <= 256, "GMRES restart length must be in 1..256");
     LinOp A;
-    A.V = &X->Ared;
-    A.N = X->Ared.N;
+    A.M = X->Ared.get();
+    A.N = X->P->N_red;
     if (P) BH_CHECK(P->N == A.N, "preconditioner size mismatch");
     *out = new GmresSolver(A, P, restart, tol, maxit, true);
     BH_API_END

@@ -189,6 +189,7 @@ void convert_f32_f64(bh_ctx* ctx, const float* src, double* dst, int64_t n);
 void fill(bh_ctx* ctx, double* x, double v, int64_t n);
 void fill32(bh_ctx* ctx, float* x, float v, int64_t n);
 void copy(bh_ctx* ctx, double* dst, const double* src, int64_t n);
+void copy_bytes(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
 void axpby(bh_ctx* ctx, double a, const double* x, double b, double* y, int64_t n);  // y = a x + b y
 void axpby32(bh_ctx* ctx, float a, const float* x, float b, float* y, int64_t n);
 // h[k] = <w, V_k>, k < m; V_k = V + k*ldv ; result in host array h

@@ -284,6 +284,27 @@ void fill32(bh_ctx* ctx, float* x, float v, int64_t n) {
 void copy(bh_ctx* ctx, double* dst, const double* src, int64_t n) {
     BH_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
 }
+
+// plain device copy in 16-byte units (+ byte tail): a kernel of known traffic
+__global__ __launch_bounds__(256) void copy16_k(const uint4* __restrict__ s, uint4* __restrict__ d, int64_t n16,
+                                                const unsigned char* __restrict__ st, unsigned char* __restrict__ dt,
+                                                int tail) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) d[i] = s[i];
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail) dt[threadIdx.x] = st[threadIdx.x];
+}
+
+void copy_bytes(bh_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    BH_CHECK(((uintptr_t)dst & 15) == 0 && ((uintptr_t)src & 15) == 0, "device copy: pointers must be 16-byte aligned");
+    const int64_t n16 = (int64_t)(bytes / 16);
+    const int tail = (int)(bytes % 16);
+    int64_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy16_k, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (const uint4*)src, (uint4*)dst, n16,
+                       (const unsigned char*)src + n16 * 16, (unsigned char*)dst + n16 * 16, tail);
+    BH_HIP(hipGetLastError());
+}
 void axpby(bh_ctx* ctx, double a, const double* x, double b, double* y, int64_t n) {
     hipLaunchKernelGGL((axpby_k<double>), grid_for(n), dim3(256), 0, ctx->stream, a, x, b, y, n);
 }

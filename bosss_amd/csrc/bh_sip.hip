@@ -59,18 +59,19 @@ __device__ __forceinline__ int encode(const int* idx, int n0, int n1) {
 }
 
 // cj = (surface area + boundary area) * 0.5 / volume (F/GridData_CellData.cs:666-700)
-template <int D, typename R>
-__device__ __forceinline__ double cell_cj(const SipArgs<R>& a, const int* idx, const double* h) {
+// (geometry scalar S: double for the FP64 path, float for the FP32 path)
+template <int D, typename R, typename S>
+__device__ __forceinline__ S cell_cj(const SipArgs<R>& a, const int* idx, const S* h) {
     int nn[3] = {a.n0, a.n1, a.n2};
-    double vol = 1, surf = 0, bnd = 0;
+    S vol = 1, surf = 0, bnd = 0;
     for (int d = 0; d < D; ++d) vol *= h[d];
     for (int d = 0; d < D; ++d) {
-        double ar = vol / h[d];
+        S ar = vol / h[d];
         surf += 2 * ar;
         if (idx[d] == 0) bnd += ar;
         if (idx[d] == nn[d] - 1) bnd += ar;
     }
-    return 0.5 * (surf + bnd) / vol;
+    return S(0.5) * (surf + bnd) / vol;
 }
 
 // Contribution of face f (= 2d + s, outward normal sgn*e_d) of cell P to
@@ -150,9 +151,9 @@ __device__ __forceinline__ void volume_term(const SipArgs<R>& a, const R* uP, co
 }
 
 // geometry of cell with logical index idx
-template <int D, typename R>
-__device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, double* h) {
-    for (int d = 0; d < D; ++d) h[d] = a.width[d * a.nmax + idx[d]];
+template <int D, typename R, typename S>
+__device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S* h) {
+    for (int d = 0; d < D; ++d) h[d] = S(a.width[d * a.nmax + idx[d]]);
 }
 
 template <int D, int K, typename R>
@@ -163,45 +164,45 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     int idx[3];
     decode<D>(c, a.n0, a.n1, idx);
     const int nn[3] = {a.n0, a.n1, a.n2};
-    double h[3];
+    R h[3];                      // geometry in the arithmetic precision of the apply
     cell_geom<D>(a, idx, h);
     R uP[Np], y[Np];
     const R* up = a.u + (size_t)c * Np;
 #pragma unroll
     for (int n = 0; n < Np; ++n) { uP[n] = up[n]; y[n] = 0; }
-    double detJ = 1;
-    for (int d = 0; d < D; ++d) detJ *= 0.5 * h[d];
+    R detJ = 1;
+    for (int d = 0; d < D; ++d) detJ *= R(0.5) * h[d];
     const R nuP = a.nu[c];
     R s2[D];
-    for (int d = 0; d < D; ++d) s2[d] = R(4.0 / (h[d] * h[d]));
+    for (int d = 0; d < D; ++d) s2[d] = R(4) / (h[d] * h[d]);
     volume_term<D, K, R>(a, uP, s2, nuP, y);
-    const double cjP = cell_cj<D>(a, idx, h);
-    const R sdPinv = R(1.0 / sqrt(detJ));
+    const R cjP = cell_cj<D>(a, idx, h);
+    const R sdPinv = R(1) / sqrt(detJ);
+    const R pen = R(a.pen);
 #pragma unroll 1
     for (int f = 0; f < 2 * D; ++f) {
         const int d = f >> 1, s = f & 1;
         const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
-        const double dA = detJ / (0.5 * h[d]);
-        const R gP = R(2.0 / h[d]);
+        const R dA = detJ / (R(0.5) * h[d]);
+        const R gP = R(2) / h[d];
         if (isBnd) {
             if ((a.bc_neumann >> f) & 1) continue;
             R uQ[1] = {0};
-            face_term<D, K, R, true, false>(a, f, uP, uQ, sdPinv, R(0), gP, R(0), R(dA), nuP, nuP,
-                                            R(a.pen * cjP), true, y);
+            face_term<D, K, R, true, false>(a, f, uP, uQ, sdPinv, R(0), gP, R(0), dA, nuP, nuP, pen * cjP, true, y);
         } else {
             int nidx[3] = {idx[0], idx[1], idx[2]};
             nidx[d] += s ? 1 : -1;
             const int cq = encode<D>(nidx, a.n0, a.n1);
-            double hq[3] = {h[0], h[1], h[2]};
-            hq[d] = a.width[d * a.nmax + nidx[d]];
-            const double detQ = detJ / h[d] * hq[d];
-            const double cjQ = cell_cj<D>(a, nidx, hq);
+            R hq[3] = {h[0], h[1], h[2]};
+            hq[d] = R(a.width[d * a.nmax + nidx[d]]);
+            const R detQ = detJ / h[d] * hq[d];
+            const R cjQ = cell_cj<D>(a, nidx, hq);
             R uQ[Np];
             const R* uq = a.u + (size_t)cq * Np;
 #pragma unroll
             for (int n = 0; n < Np; ++n) uQ[n] = uq[n];
-            face_term<D, K, R, true, true>(a, f, uP, uQ, sdPinv, R(1.0 / sqrt(detQ)), gP, R(2.0 / hq[d]), R(dA),
-                                           nuP, a.nu[cq], R(a.pen * fmax(cjP, cjQ)), false, y);
+            face_term<D, K, R, true, true>(a, f, uP, uQ, sdPinv, R(1) / sqrt(detQ), gP, R(2) / hq[d], dA,
+                                           nuP, a.nu[cq], pen * (cjP > cjQ ? cjP : cjQ), false, y);
         }
     }
     R* yp = a.y + (size_t)c * Np;

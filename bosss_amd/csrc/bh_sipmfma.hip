@@ -39,7 +39,9 @@ constexpr int binom_(int n, int k) {
 constexpr int ipow_(int b, int e) { return e == 0 ? 1 : b * ipow_(b, e - 1); }
 }  // namespace
 
-// Table layout (FP16, offsets in halves) and the FP32 side tables.
+// Table layout (FP16, offsets in halves).  Volume rows are direction-major,
+// j = d*NQV + q, and carry the quadrature weight (GA = w_q * dpsi_n/dxi_d);
+// the face back-contraction rows carry the face weight (FB = w_q * trace).
 template <int D, int K>
 struct MfmaLayout {
     static constexpr int Np = binom_(K + D, D);
@@ -49,12 +51,11 @@ struct MfmaLayout {
     static constexpr int NVT = ((D * NQV + 31) / 32) * 2;   // 16-row volume tiles, even count
     static constexpr int NPAIR = NVT / 2;
     static constexpr int MT = (Np + 15) / 16;               // 16-column output tiles
-    static constexpr int GA = 0;                            // [NVT*16][32]   volume rows j = q*D + d, cols n
+    static constexpr int GA = 0;                            // [NVT*16][32]   rows j, cols n
     static constexpr int GB = GA + NVT * 16 * 32;           // [NPAIR][MT*16][32] (m, k-slot)
     static constexpr int FA = GB + NPAIR * MT * 16 * 32;    // [2D][2][16][32] face trace / normal-derivative rows
     static constexpr int FB = FA + 2 * D * 2 * 16 * 32;     // [2D][MT*16][32]
     static constexpr int TOTAL = FB + 2 * D * MT * 16 * 32;
-    static constexpr int NW = NVT * 16 + 16;                // FP32: volume row weights, face node weights
     static_assert(Np <= 32, "MFMA apply: Np <= 32");
     static_assert(NQF <= 16, "MFMA apply: face nodes <= 16");
     static_assert(TOTAL % 8 == 0, "16-byte staging");
@@ -66,17 +67,18 @@ __host__ __device__ constexpr int slot_row(int tile_pair, int g, int s) {
 }
 
 template <int D, int K>
-void build_mfma_tables(const SipTables& t, std::vector<uint16_t>& h16, std::vector<float>& w32, std::vector<int>& d32) {
+void build_mfma_tables(const SipTables& t, std::vector<uint16_t>& h16) {
     using L = MfmaLayout<D, K>;
     std::vector<float> f(L::TOTAL, 0.0f);
     const int Np = L::Np, NQV = L::NQV, NQF = L::NQF;
-    auto G = [&](int j, int n) -> double {   // volume row j = q*D + d
+    auto G = [&](int j, int n) -> double {   // volume row j = d*NQV + q
         if (j >= D * NQV || n >= Np) return 0.0;
-        const int q = j / D, d = j % D;
+        const int d = j / NQV, q = j % NQV;
         return t.G[((size_t)q * D + d) * Np + n];
     };
+    auto W = [&](int j) -> double { return j < D * NQV ? t.vw[j % NQV] : 0.0; };
     for (int j = 0; j < L::NVT * 16; ++j)
-        for (int n = 0; n < 32; ++n) f[L::GA + j * 32 + n] = (float)G(j, n);
+        for (int n = 0; n < 32; ++n) f[L::GA + j * 32 + n] = (float)(W(j) * G(j, n));
     for (int p = 0; p < L::NPAIR; ++p)
         for (int m = 0; m < L::MT * 16; ++m)
             for (int g = 0; g < 4; ++g)
@@ -84,6 +86,7 @@ void build_mfma_tables(const SipTables& t, std::vector<uint16_t>& h16, std::vect
                     f[L::GB + ((size_t)p * L::MT * 16 + m) * 32 + 8 * g + s] = (float)G(slot_row(p, g, s), m);
     auto T = [&](int fc, int q, int n) -> double { return (q < NQF && n < Np) ? t.T[((size_t)fc * NQF + q) * Np + n] : 0.0; };
     auto N = [&](int fc, int q, int n) -> double { return (q < NQF && n < Np) ? t.Nn[((size_t)fc * NQF + q) * Np + n] : 0.0; };
+    auto FW = [&](int q) -> double { return q < NQF ? t.fw[q] : 0.0; };
     for (int fc = 0; fc < 2 * D; ++fc) {
         for (int q = 0; q < 16; ++q)
             for (int n = 0; n < 32; ++n) {
@@ -94,7 +97,8 @@ void build_mfma_tables(const SipTables& t, std::vector<uint16_t>& h16, std::vect
             for (int g = 0; g < 4; ++g)
                 for (int s = 0; s < 8; ++s) {
                     const int q = 4 * g + (s & 3);
-                    f[L::FB + ((size_t)fc * L::MT * 16 + m) * 32 + 8 * g + s] = (float)(s < 4 ? T(fc, q, m) : N(fc, q, m));
+                    f[L::FB + ((size_t)fc * L::MT * 16 + m) * 32 + 8 * g + s] =
+                        (float)(FW(q) * (s < 4 ? T(fc, q, m) : N(fc, q, m)));
                 }
     }
     h16.resize(L::TOTAL);
@@ -104,10 +108,6 @@ void build_mfma_tables(const SipTables& t, std::vector<uint16_t>& h16, std::vect
         std::memcpy(&b, &hv, 2);
         h16[i] = b;
     }
-    w32.assign(L::NW, 0.0f);
-    d32.assign(L::NVT * 16, 0);
-    for (int j = 0; j < D * NQV; ++j) { w32[j] = (float)t.vw[j / D]; d32[j] = j % D; }
-    for (int q = 0; q < NQF; ++q) w32[L::NVT * 16 + q] = (float)t.fw[q];
 }
 
 struct MfmaArgs {
@@ -115,14 +115,13 @@ struct MfmaArgs {
     const double* __restrict__ width;
     const float* __restrict__ nu;
     const _Float16* __restrict__ tab;
-    const float* __restrict__ w;     // volume row weights | face node weights
-    const int* __restrict__ dr;      // volume row direction
     float pen;
     int bc_neumann;
     const float* __restrict__ u;
     float* __restrict__ y;
     const double* __restrict__ affine;
     float alpha, beta;
+    int tiles_per_xcd;   // tiles are split into 8 contiguous ranges, one per XCD
 };
 
 __device__ __forceinline__ h8_t lds_h8(const _Float16* p) { return *(const h8_t*)p; }
@@ -143,19 +142,6 @@ __device__ __forceinline__ int encode_cell(const int* idx, int n0, int n1) {
     return D == 2 ? idx[0] * n1 + idx[1] : idx[0] + idx[1] * n0 + idx[2] * n0 * n1;
 }
 
-template <int D>
-__device__ __forceinline__ float cj_of(const int* idx, const double* h, const int* nn) {
-    double vol = 1, surf = 0, bnd = 0;
-    for (int d = 0; d < D; ++d) vol *= h[d];
-    for (int d = 0; d < D; ++d) {
-        const double ar = vol / h[d];
-        surf += 2 * ar;
-        if (idx[d] == 0) bnd += ar;
-        if (idx[d] == nn[d] - 1) bnd += ar;
-    }
-    return (float)(0.5 * (surf + bnd) / vol);
-}
-
 constexpr int MFMA_WAVES = 8;
 
 template <int D, int K>
@@ -163,25 +149,31 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
     using L = MfmaLayout<D, K>;
     constexpr int Np = L::Np;
     __shared__ __attribute__((aligned(16))) _Float16 tab[L::TOTAL];
-    __shared__ float wsh[L::NW];
-    __shared__ int dsh[L::NVT * 16];
     for (int i = threadIdx.x; i < L::TOTAL / 8; i += blockDim.x) ((uint4*)tab)[i] = ((const uint4*)a.tab)[i];
-    for (int i = threadIdx.x; i < L::NW; i += blockDim.x) wsh[i] = a.w[i];
-    for (int i = threadIdx.x; i < L::NVT * 16; i += blockDim.x) dsh[i] = a.dr[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, grp = lane >> 4;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int ntiles = (a.J + 15) / 16;
-    for (int tile = blockIdx.x * MFMA_WAVES + wave; tile < ntiles; tile += gridDim.x * MFMA_WAVES) {
+    // XCD-aware order: workgroup b runs on XCD b % 8; XCD x walks its own contiguous
+    // range of tiles, so the y/z neighbours a wave reads sit in that XCD's L2
+    const int xcd = blockIdx.x & 7, nb_x = gridDim.x >> 3;
+    const int t0 = xcd * a.tiles_per_xcd, t1 = min(ntiles, t0 + a.tiles_per_xcd);
+    for (int tile = t0 + (blockIdx.x >> 3) * MFMA_WAVES + wave; tile < t1; tile += nb_x * MFMA_WAVES) {
         // ---------------- this lane's cell (B-operand / flux phases)
         const int c = tile * 16 + col;
         const bool valid = c < a.J;
         const int cc = valid ? c : a.J - 1;
         int idx[3];
         decode_cell<D>(cc, a.n0, a.n1, idx);
-        double h[3] = {1, 1, 1}, detJ = 1;
-        for (int d = 0; d < D; ++d) { h[d] = a.width[d * a.nmax + idx[d]]; detJ *= 0.5 * h[d]; }
+        float hP[3] = {1.f, 1.f, 1.f}, ih[3] = {1.f, 1.f, 1.f};
+        float sdPinv = 1.f, cjP = 0.f;
+        for (int d = 0; d < D; ++d) {
+            hP[d] = (float)a.width[d * a.nmax + idx[d]];
+            ih[d] = 1.f / hP[d];
+            sdPinv *= sqrtf(2.f * ih[d]);
+            cjP += (1.f + 0.5f * (float)((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
+        }
         const float nuP = a.nu[cc];
         // own modes 8*grp .. 8*grp+7, scaled to FP16
         float uf[8];
@@ -193,18 +185,19 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
             mx = fmaxf(mx, fabsf(uf[s]));
         }
         mx = lane_max4(mx);
-        const float usP = mx > 0.f ? mx : 1.f;
+        const float usP = mx > 0.f ? mx : 1.f, rusP = 1.f / usP;
         h8_t bU;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) bU[s] = (_Float16)(uf[s] / usP);
+        for (int s = 0; s < 8; ++s) bU[s] = (_Float16)(uf[s] * rusP);
         f4_t yacc[L::MT];
 #pragma unroll
         for (int mt = 0; mt < L::MT; ++mt) yacc[mt] = f4_t{0.f, 0.f, 0.f, 0.f};
         // ---------------- volume: -nu * sum_q w_q (4/h_d^2) dpsi_m/dxi_d dpsi_n/dxi_d u_n
         {
             float s2[3] = {0.f, 0.f, 0.f}, s2max = 0.f;
-            for (int d = 0; d < D; ++d) { s2[d] = (float)(4.0 / (h[d] * h[d])); s2max = fmaxf(s2max, s2[d]); }
-            const float rs0 = -s2[0] / s2max, rs1 = -s2[1] / s2max, rs2 = D == 3 ? -s2[2] / s2max : 0.f;
+            for (int d = 0; d < D; ++d) { s2[d] = 4.f * ih[d] * ih[d]; s2max = fmaxf(s2max, s2[d]); }
+            const float r2 = 1.f / s2max;
+            const float rs0 = -s2[0] * r2, rs1 = -s2[1] * r2, rs2 = D == 3 ? -s2[2] * r2 : 0.f;
             const float kap = usP * nuP * s2max;
             f4_t yv[L::MT];
 #pragma unroll
@@ -220,11 +213,11 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int j0 = (2 * p) * 16 + 4 * grp + i, j1 = j0 + 16;
-                    const int e0 = dsh[j0], e1 = dsh[j1];
-                    const float r0 = e0 == 0 ? rs0 : (e0 == 1 ? rs1 : rs2);
-                    const float r1 = e1 == 0 ? rs0 : (e1 == 1 ? rs1 : rs2);
-                    F[i] = (_Float16)(r0 * wsh[j0] * d0[i]);
-                    F[4 + i] = (_Float16)(r1 * wsh[j1] * d1[i]);
+                    const int e0 = j0 / L::NQV, e1 = j1 / L::NQV;      // row direction (rows >= D*NQV are zero)
+                    const float f0 = e0 == 0 ? rs0 : (e0 == 1 ? rs1 : rs2);
+                    const float f1 = e1 == 0 ? rs0 : (e1 == 1 ? rs1 : rs2);
+                    F[i] = (_Float16)(f0 * d0[i]);
+                    F[4 + i] = (_Float16)(f1 * d1[i]);
                 }
 #pragma unroll
                 for (int mt = 0; mt < L::MT; ++mt) {
@@ -241,8 +234,6 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
                 for (int i = 0; i < 4; ++i) yacc[mt][i] += kr[i] * yv[mt][i];
         }
         // ---------------- faces
-        const float cjP = cj_of<D>(idx, h, nn);
-        const float sdPinv = (float)(1.0 / sqrt(detJ));
 #pragma unroll 1
         for (int f = 0; f < 2 * D; ++f) {
             const int d = f >> 1, s = f & 1;
@@ -251,12 +242,12 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
             const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
             const bool skip = isBnd && ((a.bc_neumann >> f) & 1);
             int cq = cc;
-            double hq[3] = {h[0], h[1], h[2]};
+            float ihq = ih[d];
             int nidx[3] = {idx[0], idx[1], idx[2]};
             if (!isBnd) {
                 nidx[d] += s ? 1 : -1;
                 cq = encode_cell<D>(nidx, a.n0, a.n1);
-                hq[d] = a.width[d * a.nmax + nidx[d]];
+                ihq = 1.f / (float)a.width[d * a.nmax + nidx[d]];
             }
             float uq[8];
             float mq = 0.f;
@@ -267,29 +258,28 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
                 mq = fmaxf(mq, fabsf(uq[t]));
             }
             mq = lane_max4(mq);
-            const float usQ = mq > 0.f ? mq : 1.f;
+            const float usQ = mq > 0.f ? mq : 1.f, rusQ = 1.f / usQ;
             h8_t bQ;
 #pragma unroll
-            for (int t = 0; t < 8; ++t) bQ[t] = (_Float16)(uq[t] / usQ);
+            for (int t = 0; t < 8; ++t) bQ[t] = (_Float16)(uq[t] * rusQ);
             const f4_t z = {0.f, 0.f, 0.f, 0.f};
             const f4_t dT = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((f * 2 + 0) * 16 + col) * 32 + 8 * grp]), bU, z, 0, 0, 0);
             const f4_t dN = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((f * 2 + 1) * 16 + col) * 32 + 8 * grp]), bU, z, 0, 0, 0);
             const f4_t eT = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((fQ * 2 + 0) * 16 + col) * 32 + 8 * grp]), bQ, z, 0, 0, 0);
             const f4_t eN = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((fQ * 2 + 1) * 16 + col) * 32 + 8 * grp]), bQ, z, 0, 0, 0);
-            // face scalars of this lane's cell
-            double detQ = 1;
-            for (int e = 0; e < D; ++e) detQ *= 0.5 * hq[e];
-            const float sdQinv = (float)(1.0 / sqrt(detQ));
-            const float gP = (float)(2.0 / h[d]), gQ = (float)(2.0 / hq[d]);
-            const float dA = (float)(detJ / (0.5 * h[d]));
+            // face scalars of this lane's cell (float; only direction d differs for Q)
+            const float gP = 2.f * ih[d], gQ = 2.f * ihq;
+            const float sdQinv = sdPinv * sqrtf(hP[d] * ihq);
+            const float wf = gP / sdPinv;              // dA * sdPinv = (2/h_d) * detJ * sdPinv, detJ = 1/sdPinv^2
             const float nuQ = isBnd ? nuP : a.nu[cq];
             const float nuMax = fabsf(nuP) > fabsf(nuQ) ? nuP : nuQ;
-            const float mu = isBnd ? a.pen * cjP : a.pen * fmaxf(cjP, cj_of<D>(nidx, hq, nn));
+            const float bP = 1.f + 0.5f * (float)((idx[d] == 0) + (idx[d] == nn[d] - 1));
+            const float bQf = 1.f + 0.5f * (float)((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+            const float cjQ = cjP - bP * ih[d] + bQf * ihq;
+            const float mu = isBnd ? a.pen * cjP : a.pen * fmaxf(cjP, cjQ);
             float cvw[4], cdw[4], sm = 0.f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int q = 4 * grp + i;
-                const float wq = wsh[L::NVT * 16 + q] * dA * sdPinv;   // zero for padded nodes
                 const float vP = sdPinv * usP * dT[i];
                 const float dP = sgn * gP * sdPinv * usP * dN[i];
                 const float vQ = sdQinv * usQ * eT[i];
@@ -303,15 +293,15 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
                     cv = 0.5f * (nuP * dP + nuQ * dQ) - mu * nuMax * jump;
                     cd = 0.5f * nuP * jump;
                 }
-                cvw[i] = (skip || !valid) ? 0.f : wq * cv;
-                cdw[i] = (skip || !valid) ? 0.f : wq * sgn * gP * cd;
+                cvw[i] = (skip || !valid) ? 0.f : wf * cv;
+                cdw[i] = (skip || !valid) ? 0.f : wf * sgn * gP * cd;
                 sm = fmaxf(sm, fmaxf(fabsf(cvw[i]), fabsf(cdw[i])));
             }
             sm = lane_max4(sm);
-            const float sig = sm > 0.f ? sm : 1.f;
+            const float sig = sm > 0.f ? sm : 1.f, rsig = 1.f / sig;
             h8_t F;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) { F[i] = (_Float16)(cvw[i] / sig); F[4 + i] = (_Float16)(cdw[i] / sig); }
+            for (int i = 0; i < 4; ++i) { F[i] = (_Float16)(cvw[i] * rsig); F[4 + i] = (_Float16)(cdw[i] * rsig); }
             float sr[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) sr[i] = __shfl(sig, 4 * grp + i);
@@ -346,24 +336,18 @@ static void mfma_launch(bh_op* op, const MfmaArgs& a0) {
     using L = MfmaLayout<D, K>;
     if (!op->tab16.p) {
         std::vector<uint16_t> h16;
-        std::vector<float> w;
-        std::vector<int> dr;
-        build_mfma_tables<D, K>(op->tab, h16, w, dr);
-        op->tab16.alloc(h16.size() * 2 + w.size() * 4 + dr.size() * 4 + 64);
-        char* base = (char*)op->tab16.p;
-        op->ctx->upload(base, h16.data(), h16.size() * 2);
-        op->ctx->upload(base + h16.size() * 2, w.data(), w.size() * 4);
-        op->ctx->upload(base + h16.size() * 2 + w.size() * 4, dr.data(), dr.size() * 4);
+        build_mfma_tables<D, K>(op->tab, h16);
+        op->tab16.alloc(h16.size() * 2);
+        op->ctx->upload(op->tab16.p, h16.data(), h16.size() * 2);
     }
     MfmaArgs a = a0;
-    const char* base = (const char*)op->tab16.p;
-    a.tab = (const _Float16*)base;
-    a.w = (const float*)(base + (size_t)L::TOTAL * 2);
-    a.dr = (const int*)(base + (size_t)L::TOTAL * 2 + (size_t)L::NW * 4);
+    a.tab = (const _Float16*)op->tab16.p;
     const int ntiles = (a.J + 15) / 16;
-    int blocks = (ntiles + MFMA_WAVES - 1) / MFMA_WAVES;
-    if (blocks > 1024) blocks = 1024;   // workgroups loop over tiles; tables staged once per workgroup
-    hipLaunchKernelGGL((sip_apply_mfma_kernel<D, K>), dim3(blocks), dim3(64 * MFMA_WAVES), 0, op->ctx->stream, a);
+    // 8 XCDs x up to 64 workgroups each (2 per CU); each wave loops over tiles
+    int per_xcd = (ntiles + 8 * MFMA_WAVES - 1) / (8 * MFMA_WAVES);
+    if (per_xcd > 64) per_xcd = 64;
+    a.tiles_per_xcd = (ntiles + 7) / 8;
+    hipLaunchKernelGGL((sip_apply_mfma_kernel<D, K>), dim3(8 * per_xcd), dim3(64 * MFMA_WAVES), 0, op->ctx->stream, a);
     BH_HIP(hipGetLastError());
 }
 

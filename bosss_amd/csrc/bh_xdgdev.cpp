@@ -243,27 +243,63 @@ void xdg_build(bh_xdg* X) {
     fv.clear();
     fv.shrink_to_fit();
     {
-        std::vector<std::vector<int>> rcols(J);
-        for (int r = 0; r < J; ++r) {
-            if (p.bsz_red[r] == 0) continue;
-            // the diagonal block is always present
-            if (!rows[r].count(r)) rows[r][r].assign((size_t)p.bsz_red[r] * p.bsz_red[r], 0.0);
-            for (auto& kv : rows[r]) rcols[r].push_back(kv.first);
-        }
-        X->Ared.build(ctx, J, p.bsz_red, rcols);
-        std::vector<double> rv((size_t)X->Ared.nvals);
+        // The agglomerated matrix as a uniform BSR over species units (a unit = one
+        // species block of one cell, Np DOFs; cut cells hold two): the DOF layout is
+        // unchanged, the (10/20)-sized cell blocks become Np x Np unit blocks and the
+        // zero species cross blocks between neighbours drop out.  The cell-level
+        // diagonal blocks (species coupled) are kept separately for the smoother.
+        std::vector<int> unit0(J + 1, 0);
+        for (int r = 0; r < J; ++r) unit0[r + 1] = unit0[r] + p.bsz_red[r] / Np;
+        const int U = unit0[J];
+        std::vector<int64_t> urp(U + 1, 0);
+        std::vector<int> ucol;
+        std::vector<double> uval;
+        std::vector<std::vector<int>> dcols(J);
+        std::vector<double> dval;
         for (int r = 0; r < J; ++r) {
             const int R = p.bsz_red[r];
-            for (int64_t bp = X->Ared.rowptr[r]; bp < X->Ared.rowptr[r + 1]; ++bp) {
-                const int c = X->Ared.col[bp];
-                const int C = p.bsz_red[c];
-                const auto& blk = rows[r][c];
-                double* o = &rv[X->Ared.voff[bp]];
-                for (int j = 0; j < C; ++j)
-                    for (int i = 0; i < R; ++i) o[(size_t)j * R + i] = blk[(size_t)i * C + j];
+            if (R == 0) continue;
+            if (!rows[r].count(r)) rows[r][r].assign((size_t)R * R, 0.0);
+            dcols[r].push_back(r);
+            const auto& dg = rows[r][r];
+            for (int j = 0; j < R; ++j)                  // column-major cell diagonal block
+                for (int i = 0; i < R; ++i) dval.push_back(dg[(size_t)i * R + j]);
+            for (int a = 0; a < R / Np; ++a) {
+                const int u = unit0[r] + a;
+                for (auto& kv : rows[r]) {
+                    const int c = kv.first, C = p.bsz_red[c];
+                    const auto& blk = kv.second;
+                    for (int b = 0; b < C / Np; ++b) {
+                        const int v = unit0[c] + b;
+                        bool nz = (u == v);
+                        for (int m = 0; m < Np && !nz; ++m)
+                            for (int n = 0; n < Np; ++n)
+                                if (blk[(size_t)(a * Np + m) * C + b * Np + n] != 0.0) { nz = true; break; }
+                        if (!nz) continue;
+                        ucol.push_back(v);
+                        for (int n = 0; n < Np; ++n)          // column-major unit block
+                            for (int m = 0; m < Np; ++m) uval.push_back(blk[(size_t)(a * Np + m) * C + b * Np + n]);
+                    }
+                }
+                urp[u + 1] = (int64_t)ucol.size();
             }
         }
-        ctx->upload(X->Ared.vals.p, rv.data(), rv.size() * sizeof(double));
+        bh_bsr* M = new bh_bsr();
+        X->Ared.reset(M);
+        M->ctx = ctx;
+        M->nrows = U;
+        M->bs = Np;
+        M->h_rowptr = urp;
+        M->h_col = ucol;
+        M->nnzb = urp[U];
+        M->rowptr.alloc((U + 1) * sizeof(int64_t));
+        M->col.alloc(std::max<size_t>(ucol.size(), 1) * sizeof(int));
+        ctx->upload(M->rowptr.p, urp.data(), (U + 1) * sizeof(int64_t));
+        ctx->upload(M->col.p, ucol.data(), ucol.size() * sizeof(int));
+        M->vals.alloc(std::max<size_t>(uval.size(), 1) * sizeof(double));
+        ctx->upload(M->vals.p, uval.data(), uval.size() * sizeof(double));
+        X->Dcell.build(ctx, J, p.bsz_red, dcols);
+        ctx->upload(X->Dcell.vals.p, dval.data(), dval.size() * sizeof(double));
     }
     // boundary nodes (Dirichlet data)
     X->bnd_x.clear(); X->bnd_w.clear(); X->bnd_nu.clear(); X->bnd_mu.clear();
@@ -380,32 +416,36 @@ XdgPrecond::~XdgPrecond() { delete Ac; }
 
 void XdgPrecond::build(bh_grid* g, int degc, int min_coarse, int kcycle_inner) {
     ctx = A->ctx;
-    N = A->N;
+    N = (int64_t)A->nrows * A->bs;
     const int D = g->D, J = g->J;
     Npc = basis_count(D, degc);
-    BH_CHECK(Npc <= Np, "XDG preconditioner: coarse degree above the DG degree");
-    // block inverses of the smoother
-    A->invert_diag(prec == BH_FP32);
+    BH_CHECK(Npc <= Np && A->bs == Np, "XDG preconditioner: coarse degree above the DG degree");
+    BH_CHECK(Dc->N == N, "XDG preconditioner: cell blocks do not match the matrix");
+    // block inverses of the smoother (cell blocks, species coupled)
+    Dc->invert_diag(prec == BH_FP32);
     if (prec == BH_FP32) A->ensure_fp32();
-    // coarse matrix Ac = R^T A R (species merged, first Npc modes)
-    std::vector<double> v((size_t)A->nvals);
+    // coarse matrix Ac = R^T A R (species merged, first Npc modes of every unit)
+    std::vector<int> cell_of(A->nrows);
+    for (int c = 0, u = 0; c < J; ++c)
+        for (int a = 0; a < Dc->rsz[c] / Np; ++a) cell_of[u++] = c;
+    std::vector<double> v((size_t)A->nnzb * Np * Np);
     ctx->download(v.data(), A->vals.p, v.size() * sizeof(double));
+    std::vector<std::map<int, std::vector<double>>> crow(J);
+    for (int u = 0; u < A->nrows; ++u)
+        for (int64_t bp = A->h_rowptr[u]; bp < A->h_rowptr[u + 1]; ++bp) {
+            const double* B = &v[(size_t)bp * Np * Np];   // column-major Np x Np
+            auto& blk = crow[cell_of[u]][cell_of[A->h_col[bp]]];
+            if (blk.empty()) blk.assign((size_t)Npc * Npc, 0.0);
+            for (int n = 0; n < Npc; ++n)
+                for (int m = 0; m < Npc; ++m) blk[(size_t)n * Npc + m] += B[(size_t)n * Np + m];
+        }
     std::vector<int64_t> rp(J + 1, 0);
     std::vector<int> col;
     std::vector<double> cv;
     for (int r = 0; r < J; ++r) {
-        const int R = A->rsz[r];
-        for (int64_t bp = A->rowptr[r]; bp < A->rowptr[r + 1]; ++bp) {
-            const int c = A->col[bp];
-            const int C = A->rsz[c];
-            const double* B = &v[A->voff[bp]];
-            col.push_back(c);
-            std::vector<double> blk((size_t)Npc * Npc, 0.0);   // column-major
-            for (int sr = 0; sr < R / Np; ++sr)
-                for (int sc = 0; sc < C / Np; ++sc)
-                    for (int n = 0; n < Npc; ++n)
-                        for (int m = 0; m < Npc; ++m) blk[(size_t)n * Npc + m] += B[(size_t)(sc * Np + n) * R + sr * Np + m];
-            cv.insert(cv.end(), blk.begin(), blk.end());
+        for (auto& kv : crow[r]) {
+            col.push_back(kv.first);
+            cv.insert(cv.end(), kv.second.begin(), kv.second.end());
         }
         rp[r + 1] = (int64_t)col.size();
     }
@@ -443,19 +483,19 @@ void XdgPrecond::cycle() {
     T* bb = (T*)b.p;
     T* xx = (T*)x.p;
     T* rr = (T*)r.p;
-    A->block_gemv(bb, xx, omega, false, p);
+    Dc->block_gemv(bb, xx, omega, false, p);
     for (int s = 1; s < pre; ++s) {
-        A->spmv(-1.0, xx, 0.0, rr, p, bb);
-        A->block_gemv(rr, xx, omega, true, p);
+        bsr_residual(A, bb, xx, rr, p);
+        Dc->block_gemv(rr, xx, omega, true, p);
     }
-    A->spmv(-1.0, xx, 0.0, rr, p, bb);
+    bsr_residual(A, bb, xx, rr, p);
     MgLevel& C = cmg->lv[0];
-    xdg_restrict<T>(ctx, *A, Np, Npc, rr, (T*)C.b.p);
+    xdg_restrict<T>(ctx, *Dc, Np, Npc, rr, (T*)C.b.p);
     cmg->solve_level<T>(0);
-    xdg_prolong_add<T>(ctx, *A, Np, Npc, (const T*)C.x.p, xx);
+    xdg_prolong_add<T>(ctx, *Dc, Np, Npc, (const T*)C.x.p, xx);
     for (int s = 0; s < post; ++s) {
-        A->spmv(-1.0, xx, 0.0, rr, p, bb);
-        A->block_gemv(rr, xx, omega, true, p);
+        bsr_residual(A, bb, xx, rr, p);
+        Dc->block_gemv(rr, xx, omega, true, p);
     }
 }
 
@@ -474,7 +514,7 @@ void XdgPrecond::precondition(double* xo, const double* bi) {
 
 void XdgPrecond::solve_dev(double* xo, const double* bi) {
     DevBuf rr(N * sizeof(double)), c(N * sizeof(double));
-    A->spmv(-1.0, xo, 0.0, rr.p, BH_FP64, bi);
+    bsr_residual(A, bi, xo, rr.p, BH_FP64);
     precondition(c.as<double>(), rr.as<double>());
     axpby(ctx, 1.0, c.as<double>(), 1.0, xo, N);
     converged = 1;

@@ -55,7 +55,8 @@ template <typename T> void xdg_prolong_add(bh_ctx*, const Vbsr& A, int Np, int N
 // correction in the species-merged degree-1 space, solved by the p/h
 // multigrid of the SIP path (aggregation levels + K-cycle + dense coarsest).
 struct XdgPrecond : bh_solver {
-    Vbsr* A = nullptr;
+    bh_bsr* A = nullptr;                   // agglomerated matrix (species units)
+    Vbsr* Dc = nullptr;                    // its cell diagonal blocks (smoother; cell layout of the transfers)
     int prec = BH_FP32;
     int Np = 0, Npc = 0;
     int pre = 2, post = 2;
@@ -78,7 +79,9 @@ struct bh_xdg {
     bh_grid* grid = nullptr;
     std::unique_ptr<bh::XdgProblem> P;
     bh::XdgDevice X;
-    bh::Vbsr Afull, Ared;
+    bh::Vbsr Afull;                         // full (cell, species) matrix, assembly target
+    std::unique_ptr<bh_bsr> Ared;           // agglomerated matrix, uniform BSR over species units
+    bh::Vbsr Dcell;                         // cell diagonal blocks of Ared (species coupled)
     std::vector<double> ext;                // per agglomeration pair: Np x Np extension block, row-major [src mode][tgt mode]
     std::vector<double> bnd_x;              // boundary nodes (3 per) of the Dirichlet data
     std::vector<double> bnd_w, bnd_nu, bnd_mu;

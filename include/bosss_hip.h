@@ -55,6 +55,9 @@ int bh_malloc(bh_ctx* ctx, size_t bytes, void** dptr);
 int bh_free(bh_ctx* ctx, void* dptr);
 int bh_memcpy_h2d(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
 int bh_memcpy_d2h(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* device-to-device copy on the context stream by a plain copy kernel (16-byte
+ * aligned pointers); also the known-traffic kernel that calibrates HBM counters */
+int bh_memcpy_d2d(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* elapsed device time (ms) of the last `bh_timed_*` region on the context stream */
 int bh_event_record(bh_ctx* ctx, int slot);
 int bh_event_elapsed(bh_ctx* ctx, int slot0, int slot1, float* ms);
@@ -221,8 +224,9 @@ typedef struct bh_xdg bh_xdg;
 int bh_xdg_create(bh_grid* grid, int k, const double* center, double radius, double nu_in, double nu_out,
                   double penalty_base, double agg_threshold, int quad_order, bh_xdg** x);
 int bh_xdg_destroy(bh_xdg* x);
-/* info[12]: N_full, N_red, cut cells, agglomeration pairs, failed sources, elements,
- * cut-rule nodes, full matrix values, reduced matrix values, boundary nodes, Np, J */
+/* info[13]: N_full, N_red, cut cells, agglomeration pairs, failed sources, elements,
+ * cut-rule nodes, full matrix values, reduced matrix values, boundary nodes, Np, J,
+ * block rows of the agglomerated matrix (species units) */
 int bh_xdg_info(bh_xdg* x, int64_t* info);
 /* per cell: species mask (bit 0 A, bit 1 B); volumes [2][J]; interface area [J];
  * agglomerated length scales [2][J] (X/MultiphaseAgglomerator.cs:870-1030) */
@@ -233,8 +237,11 @@ int bh_xdg_agglomeration(bh_xdg* x, int* pairs);
  * cellP, cellQ, speciesP, speciesQ, dir, side, rule start, rule length (0: tensor Gauss);
  * dbls[3]: nuP, nuQ, penalty) and the cut-rule nodes (xyz), weights and interface normals */
 int bh_xdg_elements(bh_xdg* x, int* ints, double* dbls, double* nx, double* nw, double* nn);
-/* which = 0 full, 1 agglomerated: DOF offsets [J+1], block rows [J+1], block columns,
- * value offsets [nblocks+1] and column-major block values (FP64) */
+/* which = 0: the full matrix over cells (blocks of the cell's present species);
+ * which = 1: the agglomerated matrix over species units (uniform Np x Np blocks,
+ * the DOF layout of the agglomerated vectors).  Exported as DOF offsets [rows+1],
+ * block rows [rows+1], block columns, value offsets [nblocks+1] and column-major
+ * block values (FP64). */
 int bh_xdg_matrix(bh_xdg* x, int which, int64_t* dof_off, int64_t* rowptr, int* col, int64_t* voff, double* vals);
 /* Dirichlet data: physical boundary quadrature nodes (3 per node) */
 int bh_xdg_bnd_nodes(bh_xdg* x, double* xyz);
