@@ -100,6 +100,7 @@ SIGNATURES = {
     "bh_xdg_rhs": (_c_i, [_c_p, _c_i, _c_p, _c_i]),
     "bh_xdg_spmv": (_c_i, [_c_p, _c_i, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i]),
     "bh_xdg_extend": (_c_i, [_c_p, _c_p, _c_p]),
+    "bh_xdg_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i]),
     "bh_xdg_precond_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
     "bh_xdg_fgmres_create": (_c_i, [_c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
@@ -702,6 +703,14 @@ class XdgPoisson(_Owned):
         assert fx == fy
         _check(lib().bh_xdg_spmv(self.h, which, alpha, xp, beta, yp, prec, fx))
         return y
+
+    def Evaluate(self, alpha, beta, output, u):
+        """y <- beta y + alpha A u on the full DOF layout (matrix-free, bh_xdg_evaluate)."""
+        up, fu = _vec_args(u)
+        yp, fy = _vec_args(output)
+        assert fu == fy
+        _check(lib().bh_xdg_evaluate(self.h, alpha, up, beta, yp, fu))
+        return output
 
     def extend(self, u):
         u = np.ascontiguousarray(u, np.float64)

@@ -950,6 +950,19 @@ int bh_xdg_spmv(bh_xdg* X, int which, double alpha, const void* x, double beta, 
     BH_API_END
 }
 
+int bh_xdg_evaluate(bh_xdg* X, double alpha, const double* u, double beta, double* y, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");
+    const bool dev = flags & BH_DEVICE_PTR;
+    const int64_t N = X->P->N_full;
+    Staged su(X->ctx, u, N * sizeof(double), dev, true);
+    Staged sy(X->ctx, y, N * sizeof(double), dev, beta != 0.0);
+    xdg_evaluate(X->X, alpha, (const double*)su.dptr, beta, (double*)sy.dptr);
+    sy.download(y, N * sizeof(double), dev);
+    X->ctx->sync();
+    BH_API_END
+}
+
 int bh_xdg_extend(bh_xdg* X, const double* ured, double* ufull) {
     BH_API_BEGIN
     BH_CHECK(X->ctx, "XDG: this call needs a grid created with a device context");

@@ -213,6 +213,131 @@ __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ matrix-free evaluate
+// Node qn of element E: physical point and weight (tensor Gauss rule for uncut
+// items, the item's own node list for cut items).
+__device__ void element_node(const XArgs& a, const XElem& E, int qn, const double* loP, const double* hP, double* x,
+                             double* w) {
+    x[0] = x[1] = x[2] = 0;
+    if (E.rn > 0) {
+        for (int d = 0; d < 3; ++d) x[d] = a.nx[(size_t)(E.r0 + qn) * 3 + d];
+        *w = a.nw[E.r0 + qn];
+        return;
+    }
+    int r = qn;
+    double ww = 1;
+    for (int d = 0; d < a.D; ++d) {
+        if (E.type != 0 && d == E.dir) {
+            x[d] = (E.type == 2 && E.side == 0) ? loP[d] : loP[d] + hP[d];
+            continue;
+        }
+        const int id = r % a.nq1;
+        r /= a.nq1;
+        x[d] = loP[d] + 0.5 * (a.gx[id] + 1.0) * hP[d];
+        ww *= a.gw[id] * 0.5 * hP[d];
+    }
+    *w = ww;
+}
+
+struct XEvalArgs {
+    XArgs g;
+    const int64_t* __restrict__ off;   // full DOF offset per cell
+    const double* __restrict__ u;
+    double* __restrict__ y;
+    double alpha;
+};
+
+// y += alpha * A u element by element (XEvaluatorNonlin, X/XSpatialOperatorMk2_XEvaluators.cs:467-646, for
+// the linear XLaplace forms): one wave per element, the same element list and rules as the assembly.
+// Per node, the lanes evaluate both sides' basis into LDS, every lane forms the traces of u (sums over the
+// modes in LDS), and lane m (P side) / Np + m (Q side) accumulates the test-function m contribution; one
+// FP64 atomic add per (element, test function) at the end.
+__global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
+    const XArgs& a = e.g;
+    __shared__ double sv[2][XMAXNP], sd[2][XMAXNP], sg[XMAXNP][3], su[2][XMAXNP];
+    __shared__ double node[4];
+    const int lane = threadIdx.x;
+    const int Np = a.Np, D = a.D;
+    for (int ei = blockIdx.x; ei < a.nelem; ei += gridDim.x) {
+        const XElem E = a.elems[ei];
+        double loP[3], hP[3], loQ[3], hQ[3];
+        cell_box(a, E.cellP, loP, hP);
+        cell_box(a, E.cellQ, loQ, hQ);
+        const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
+        const int64_t oQ = e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np;
+        const bool twoSided = E.type == 1 || E.type == 3;
+        if (lane < Np) su[0][lane] = e.u[oP + lane];
+        else if (lane < 2 * Np) su[1][lane - Np] = twoSided ? e.u[oQ + lane - Np] : 0.0;
+        int nnodes;
+        if (E.rn > 0) nnodes = E.rn;
+        else if (E.type == 0) { nnodes = 1; for (int d = 0; d < D; ++d) nnodes *= a.nq1; }
+        else { nnodes = 1; for (int d = 0; d < D - 1; ++d) nnodes *= a.nq1; }
+        double acc = 0;
+        for (int qn = 0; qn < nnodes; ++qn) {
+            if (lane == 0) {
+                double x[3], w;
+                element_node(a, E, qn, loP, hP, x, &w);
+                node[0] = x[0]; node[1] = x[1]; node[2] = x[2]; node[3] = w;
+            }
+            __syncthreads();
+            const double x[3] = {node[0], node[1], node[2]};
+            const double w = node[3];
+            double nrm[3] = {0, 0, 0};
+            if (E.type == 3) for (int d = 0; d < 3; ++d) nrm[d] = a.nn[(size_t)(E.r0 + qn) * 3 + d];
+            else if (E.type == 1) nrm[E.dir] = 1.0;
+            else if (E.type == 2) nrm[E.dir] = E.side ? 1.0 : -1.0;
+            if (lane < 2 * Np) {
+                const int side = lane / Np, m = lane - side * Np;
+                double v, g[3];
+                basis_at(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
+                double dn = 0;
+                for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
+                sv[side][m] = v;
+                sd[side][m] = dn;
+                if (side == 0) for (int d = 0; d < 3; ++d) sg[m][d] = d < D ? g[d] : 0.0;
+            }
+            __syncthreads();
+            // traces of u (every lane, LDS broadcasts)
+            double uP = 0, dP = 0, uQ = 0, dQ = 0, gu[3] = {0, 0, 0};
+            for (int n = 0; n < Np; ++n) {
+                const double cP = su[0][n], cQ = su[1][n];
+                uP += sv[0][n] * cP;
+                dP += sd[0][n] * cP;
+                uQ += sv[1][n] * cQ;
+                dQ += sd[1][n] * cQ;
+                if (E.type == 0) for (int d = 0; d < 3; ++d) gu[d] += sg[n][d] * cP;
+            }
+            if (lane < 2 * Np) {
+                const int side = lane / Np, m = lane - side * Np;
+                double val = 0;
+                if (E.type == 0) {
+                    if (side == 0) {
+                        double s = 0;
+                        for (int d = 0; d < D; ++d) s += sg[m][d] * gu[d];
+                        val = -E.nuP * s;
+                    }
+                } else if (E.type == 2) {
+                    if (side == 0) val = E.nuP * dP * sv[0][m] + E.nuP * sd[0][m] * uP - E.nuP * E.mu * uP * sv[0][m];
+                } else {
+                    const double jump = uP - uQ, flux = 0.5 * (E.nuP * dP + E.nuQ * dQ);
+                    if (side == 0) val = flux * sv[0][m] + 0.5 * E.nuP * sd[0][m] * jump - E.mu * jump * sv[0][m];
+                    else val = -flux * sv[1][m] + 0.5 * E.nuQ * sd[1][m] * jump + E.mu * jump * sv[1][m];
+                }
+                acc += w * val;
+            }
+            __syncthreads();
+        }
+        if (lane < Np) atomicAdd(&e.y[oP + lane], e.alpha * acc);
+        else if (lane < 2 * Np && twoSided) atomicAdd(&e.y[oQ + lane - Np], e.alpha * acc);
+        __syncthreads();
+    }
+}
+
+__global__ void xdg_scale_k(double* __restrict__ y, double beta, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = beta == 0.0 ? 0.0 : beta * y[i];
+}
+
 // ------------------------------------------------------------------ variable-block SpMV
 // One wave per block row; row size R (<= 64) is wave-uniform: lanes = G groups
 // x R rows; each group walks the row's blocks column by column (blocks are
@@ -486,6 +611,33 @@ void xdg_assemble(XdgDevice& X, Vbsr& M) {
     int blocks = a.nelem < 65536 ? a.nelem : 65536;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(xdg_assemble_kernel, dim3(blocks), dim3(64), 0, X.ctx->stream, a);
+    BH_HIP(hipGetLastError());
+}
+
+void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y) {
+    XdgProblem& P = *X.P;
+    const int64_t N = P.N_full;
+    hipLaunchKernelGGL(xdg_scale_k, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, X.ctx->stream, y, beta, N);
+    XEvalArgs e{};
+    XArgs& a = e.g;
+    a.D = P.D; a.k = P.k; a.Np = P.Np; a.nq1 = P.k + 1;
+    a.n0 = P.g->n[0]; a.n1 = P.g->n[1]; a.n2 = P.g->n[2];
+    a.nodes1 = X.nodes1.as<double>();
+    a.nstride = X.nstride;
+    a.elems = X.elems.as<XElem>();
+    a.nelem = (int)P.elems.size();
+    a.nx = X.nx.as<double>(); a.nw = X.nw.as<double>(); a.nn = X.nn.as<double>();
+    a.ex = X.ex.as<int>();
+    a.gx = X.gx.as<double>(); a.gw = X.gw.as<double>();
+    a.spmask = X.spmask.as<int>();
+    e.off = X.offf.as<int64_t>();
+    e.u = u;
+    e.y = y;
+    e.alpha = alpha;
+    BH_CHECK(2 * P.Np <= 64 && P.Np <= XMAXNP, "XDG evaluate: degree too high");
+    int blocks = a.nelem < 65536 ? a.nelem : 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(xdg_evaluate_kernel, dim3(blocks), dim3(64), 0, X.ctx->stream, e);
     BH_HIP(hipGetLastError());
 }
 

@@ -46,6 +46,8 @@ void XdgDevice::upload() {
     up(gw, w);
     spmask.alloc(p.spmask.size() * sizeof(int));
     ctx->upload(spmask.p, p.spmask.data(), p.spmask.size() * sizeof(int));
+    offf.alloc(p.off_full.size() * sizeof(int64_t));
+    ctx->upload(offf.p, p.off_full.data(), p.off_full.size() * sizeof(int64_t));
 }
 
 // ------------------------------------------------------------ host helpers

@@ -40,11 +40,13 @@ struct XdgDevice {
     bh_ctx* ctx = nullptr;
     XdgProblem* P = nullptr;
     int nstride = 0;
-    DevBuf nodes1, elems, nx, nw, nn, ex, gx, gw, spmask;
+    DevBuf nodes1, elems, nx, nw, nn, ex, gx, gw, spmask, offf;
     void upload();
 };
 
 void xdg_assemble(XdgDevice& X, Vbsr& M);
+// y <- beta y + alpha A u on the full (cell, species) DOF layout, matrix-free
+void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y);
 // coarse-space transfer: per cell, the first Npc modes of every species block are
 // summed (restriction) / receive the coarse value (prolongation)
 template <typename T> void xdg_restrict(bh_ctx*, const Vbsr& A, int Np, int Npc, const T* r, T* rc);

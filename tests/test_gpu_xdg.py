@@ -70,6 +70,22 @@ def test_full_matrix_matches_oracle(ctx, D, n):
     assert spla.norm(A - A.T) <= 1e-12 * spla.norm(A)
 
 
+@pytest.mark.parametrize("D,n", [(3, 8), (2, 16)])
+def test_matrix_free_evaluate(ctx, D, n):
+    """Matrix-free quadrature evaluate over the element list (one launch, tensor
+    and cut items) equals the oracle's assembled operator applied to u, and the
+    Evaluate(alpha, beta) semantics hold."""
+    _say(f"case xdg evaluate D={D} n={n}")
+    X, G, asm, tgt, e, L = _setup(ctx, D=D, n=n, r=0.27 if D == 3 else 0.31)
+    rng = np.random.default_rng(21)
+    u = rng.standard_normal(X.N_full)
+    y0 = rng.standard_normal(X.N_full)
+    out = X.Evaluate(1.0, 0.0, np.zeros(X.N_full), u)
+    assert rel(out, asm.A @ u) <= 1e-12
+    out2 = X.Evaluate(-0.7, 1.3, y0.copy(), u)
+    assert rel(out2, 1.3 * y0 - 0.7 * (asm.A @ u)) <= 1e-12
+
+
 def test_agglomerated_matrix_and_rhs(ctx):
     _say("case xdg agglomerated matrix + rhs")
     X, G, asm, tgt, e, L = _setup(ctx, D=3, n=10)
