@@ -83,6 +83,7 @@ SIGNATURES = {
     "bh_mlp_targets": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
     "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
+    "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
     "bh_euler_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _PP]),
     "bh_euler_destroy": (_c_i, [_c_p]),
     "bh_euler_info": (_c_i, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i)]),
@@ -568,6 +569,11 @@ def pmg_learned_stats(pmg):
     a, r = _c_i(), _c_i()
     _check(lib().bh_pmg_learned_stats(pmg.h, ctypes.byref(a), ctypes.byref(r)))
     return a.value, r.value
+
+
+def pmg_set_graphs(pmg, enable):
+    """HIP-graph replay of the synchronisation-free V-cycles on (default) / off."""
+    _check(lib().bh_pmg_set_graphs(pmg.h, 1 if enable else 0))
 
 
 class EulerOperator(_Owned):

@@ -699,6 +699,18 @@ int bh_pmg_learned_stats(bh_solver* s, int* acc, int* rej) {
     BH_API_END
 }
 
+int bh_pmg_set_graphs(bh_solver* s, int enable) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(s);
+    BH_CHECK(mg || xp, "bh_pmg_set_graphs: solver is not a multigrid preconditioner");
+    if (xp) mg = xp->cmg.get();
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    mg->drop_graphs();
+    mg->use_graphs = enable != 0;
+    BH_API_END
+}
+
 // ------------------------------------------------------------------ Euler
 int bh_euler_create(bh_grid* g, int k, int nq1, double gamma, double mach, bh_euler** out) {
     BH_API_BEGIN

@@ -64,6 +64,14 @@ struct PMultigrid : bh_solver {
     void solve_dev(double* x, const double* b) override;        // x <- x + V(b - A x)
     void precondition(double* x, const double* b) override;     // x <- V(b)
     template <typename T> void vcycle(int l);
+    // V-cycles that contain no host synchronisation (the whole cycle without a
+    // K-cycle, or the inner cycle of the K-cycle) are captured once into a HIP
+    // graph and replayed: one launch instead of one per kernel of every level
+    bool use_graphs = true;
+    hipGraphExec_t gexec[2][16] = {};
+    int gcalls[2][16] = {};
+    template <typename T> void vcycle_g(int l);
+    void drop_graphs();
     template <typename T> void kcycle(int l);
     template <typename T> void solve_level(int l) { if (l == kcycle_level && kin > 0) kcycle<T>(l); else vcycle<T>(l); }
     template <typename T> void residual(int l, const T* b, const T* x, T* r);

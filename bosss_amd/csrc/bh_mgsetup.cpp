@@ -24,9 +24,47 @@
 namespace bh {
 
 PMultigrid::~PMultigrid() {
+    drop_graphs();
     for (auto& L : lv)
         if (L.ownA) delete L.A;
     delete ownedAf;
+}
+
+void PMultigrid::drop_graphs() {
+    for (int t = 0; t < 2; ++t)
+        for (int l = 0; l < 16; ++l) {
+            if (gexec[t][l]) (void)hipGraphExecDestroy(gexec[t][l]);
+            gexec[t][l] = nullptr;
+            gcalls[t][l] = 0;
+        }
+}
+
+// The first call runs eagerly (it also builds lazily created tables), the second
+// captures the cycle into a graph; later calls replay it.  The cycle's buffers
+// are fixed per level, so the captured kernel arguments stay valid.
+template <typename T>
+void PMultigrid::vcycle_g(int l) {
+    const int ti = sizeof(T) == 4 ? 1 : 0;
+    if (!use_graphs || l >= 16 || gcalls[ti][l]++ == 0) {
+        vcycle<T>(l);
+        return;
+    }
+    if (!gexec[ti][l]) {
+        hipGraph_t g = nullptr;
+        BH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            vcycle<T>(l);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        BH_HIP(hipStreamEndCapture(ctx->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec[ti][l], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BH_HIP(e);
+    }
+    BH_HIP(hipGraphLaunch(gexec[ti][l], ctx->stream));
 }
 
 static bh_bsr* make_bsr(bh_ctx* ctx, int nrows, int bs, const std::vector<int64_t>& rp, const std::vector<int>& col) {
@@ -396,7 +434,7 @@ void PMultigrid::kcycle(int l) {
     int used = 0;
     for (int j = 0; j < kin; ++j) {
         vcopy<T>(ctx, V + (size_t)j * n, vec<T>(L.b), n);
-        vcycle<T>(l);
+        vcycle_g<T>(l);
         vcopy<T>(ctx, vec<T>(L.x), Z + (size_t)j * n, n);
         bsr_spmv(L.A, 1.0, Z + (size_t)j * n, 0.0, w, p);
         multi_dot_t<T>(ctx, w, V, n, j + 1, n, h1.data());
@@ -481,14 +519,17 @@ void PMultigrid::learned_correction() {
 
 void PMultigrid::precondition(double* x, const double* b) {
     MgLevel& L = lv[0];
+    const bool sync_free = kcycle_level < 0 || kin == 0;   // no K-cycle: the whole cycle is one graph
     if (prec == BH_FP32) {
         convert_f64_f32(ctx, b, vec<float>(L.b), N);
-        vcycle<float>(0);
+        if (sync_free) vcycle_g<float>(0);
+        else vcycle<float>(0);
         if (learned) learned_correction<float>();
         convert_f32_f64(ctx, vec<float>(L.x), x, N);
     } else {
         copy(ctx, vec<double>(L.b), b, N);
-        vcycle<double>(0);
+        if (sync_free) vcycle_g<double>(0);
+        else vcycle<double>(0);
         if (learned) learned_correction<double>();
         copy(ctx, x, vec<double>(L.x), N);
     }
@@ -511,6 +552,8 @@ void PMultigrid::solve_dev(double* x, const double* b) {
 
 template void PMultigrid::vcycle<float>(int);
 template void PMultigrid::vcycle<double>(int);
+template void PMultigrid::vcycle_g<float>(int);
+template void PMultigrid::vcycle_g<double>(int);
 template void PMultigrid::kcycle<float>(int);
 template void PMultigrid::kcycle<double>(int);
 

@@ -158,6 +158,11 @@ int bh_fgmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, do
  * fine_mat: the assembled operator of `op` (NULL: assembled internally). */
 int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int min_coarse, double omega,
                   int pre, int post, int kcycle_inner, int prec, int matrix_free, bh_solver** solver);
+/* No counterpart in the reference (launch strategy): enable (default) or disable
+ * the HIP-graph replay of the host-synchronisation-free V-cycles of a
+ * bh_pmg_create or bh_xdg_precond_create preconditioner (the whole cycle without
+ * a K-cycle, else the K-cycle's inner cycle).  Results are identical either way. */
+int bh_pmg_set_graphs(bh_solver* pmg, int enable);
 /* ---------------------------------------------------------------- learned correction
  * No counterpart in the reference: a per-cell MLP (in -> 128 -> 128 -> out,
  * ReLU; BF16 weights, FP32 accumulation on MFMA for inference; FP32 backward

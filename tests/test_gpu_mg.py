@@ -111,6 +111,26 @@ def test_fp32_cycle_close_to_fp64(ctx):
     assert rel(x32, x64) < 1e-4
 
 
+@pytest.mark.parametrize("kin", [0, 2])
+def test_graph_replay_matches_eager(ctx, kin):
+    """The HIP-graph replay of the synchronisation-free V-cycles gives the same
+    preconditioner output as eager launches, on every replay, with eager work
+    and synchronisations of the program in between."""
+    _say(f"case graph replay kin={kin}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    mg_g = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=kin, prec=B.BH_FP32)
+    mg_e = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=kin, prec=B.BH_FP32)
+    B.pmg_set_graphs(mg_e, False)
+    rng = np.random.default_rng(9)
+    for rep in range(5):
+        b = rng.standard_normal(oo.N)
+        xg = mg_g.Solve(np.zeros(oo.N), b)
+        go.Evaluate(1.0, 0.0, np.zeros(go.N), b, affine=False)   # eager work between replays
+        ctx.sync()
+        xe = mg_e.Solve(np.zeros(oo.N), b)
+        assert np.array_equal(xg, xe), f"replay {rep}"
+
+
 @pytest.mark.parametrize("prec", [B.BH_FP64, B.BH_FP32, B.BH_FP16_MFMA])
 def test_fgmres_multigrid_solve(ctx, prec):
     """FGMRES(30) + p/h multigrid (K-cycle) to 1e-8 true FP64 residual, 16^3 k=3, log-uniform nu."""

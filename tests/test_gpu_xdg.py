@@ -170,6 +170,24 @@ def test_parabola_exact(ctx):
     assert err <= 1e-7 * scale
 
 
+def test_precond_graph_replay(ctx):
+    """XDG two-level preconditioner: graph replay of its coarse V-cycles equals eager launches."""
+    _say("case xdg precond graph replay")
+    x = np.linspace(0, 1, 13)
+    g = B.Grid(ctx, x, x, x)
+    X = B.XdgPoisson(g, 2, [0.5, 0.47, 0.52], 0.27, NU[0], NU[1])
+    Pg = B.XdgPrecond(X, coarse_degree=1, prec=B.BH_FP32, min_coarse=8)
+    Pe = B.XdgPrecond(X, coarse_degree=1, prec=B.BH_FP32, min_coarse=8)
+    B.pmg_set_graphs(Pe, False)
+    rng = np.random.default_rng(3)
+    for rep in range(4):
+        b = rng.standard_normal(X.N)
+        xg = Pg.Solve(np.zeros(X.N), b)
+        ctx.sync()
+        xe = Pe.Solve(np.zeros(X.N), b)
+        assert np.array_equal(xg, xe), f"replay {rep}"
+
+
 def test_solver_contrast_1000(ctx):
     """FGMRES(30) + two-level FP32 preconditioner reaches 1e-8 on a 20^3 grid at
     1:1000 contrast; the FP64 preconditioner gives the same solution."""
