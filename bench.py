@@ -20,6 +20,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (vector and matrix), FLOP/s
 PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
+USE_GRAPHS = True        # HIP-graph replay of the multigrid V-cycles (--no-graphs: eager launches)
 
 
 def sip_flops(n, D, k):
@@ -110,6 +111,7 @@ def solve_leg(B, ctx, grid, op, n, k, out):
     for name, prec in (("plain_fp64", B.BH_FP64), ("mixed_fp32", B.BH_FP32), ("mixed_fp16_mfma", B.BH_FP16_MFMA)):
         t0 = time.perf_counter()
         mg = B.PMultigrid(op, M=M, degrees=[k, 1], min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2, prec=prec)
+        B.pmg_set_graphs(mg, USE_GRAPHS)
         ctx.sync()
         t_init = time.perf_counter() - t0
         fg = B.FlexGMRES(op=op, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=500)
@@ -173,6 +175,7 @@ def xdg_leg(B, ctx, out, n=48, seed=3):
     for name, prec in (("solve_fp32_precond", B.BH_FP32), ("solve_fp64_precond", B.BH_FP64)):
         t0 = time.perf_counter()
         P = B.XdgPrecond(X, coarse_degree=1, prec=prec)
+        B.pmg_set_graphs(P, USE_GRAPHS)
         ctx.sync()
         t_init = time.perf_counter() - t0
         S = B.XdgFlexGMRES(X, P, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=1000)
@@ -304,7 +307,11 @@ def main():
     ap.add_argument("--no-euler", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (torch) as the reference kernel for HBM counters")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="eager kernel launches in the multigrid (for kernel-trace profilers that cannot follow graphs)")
     args = ap.parse_args()
+    global USE_GRAPHS
+    USE_GRAPHS = not args.no_graphs
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU kernels; the multi-GPU "
               "sharding is a design, not a measured path). Run with --gpus 1.", file=sys.stderr)
@@ -396,6 +403,9 @@ def main():
                      "hbm_GBps": bytes32 / (ms * 1e-3) / 1e9, "hbm_frac": bytes32 / (ms * 1e-3) / PEAK_HBM,
                      "algorithmic_flop": flops, "TFLOPs": flops / (ms * 1e-3) / 1e12,
                      "compute_frac": flops / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}
+        if prec == B.BH_FP16_MFMA:
+            w_, x_, t_ = op.mfma_tuning()
+            out[name]["autotuned_launch"] = {"waves_per_workgroup": w_, "workgroups_per_xcd": x_, "tuning_ms": t_}
     del u32, y32
     if not args.no_solve:
         solve_leg(B, ctx, grid, op, n, k, out)

@@ -62,6 +62,7 @@ SIGNATURES = {
     "bh_sip_set_rhs": (_c_i, [_c_p, _c_p, _c_i]),
     "bh_op_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i, _c_i]),
     "bh_op_assemble": (_c_i, [_c_p, _PP, _c_p, _c_i]),
+    "bh_op_tuning": (_c_i, [_c_p, _c_p, ctypes.POINTER(_c_f)]),
     "bh_bsr_create": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p, _PP]),
     "bh_bsr_destroy": (_c_i, [_c_p]),
     "bh_bsr_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i64)]),
@@ -421,6 +422,13 @@ class SipOperator(_Owned):
         assert fu == fy
         _check(lib().bh_op_evaluate(self.h, alpha, up, beta, yp, 1 if affine else 0, prec, fu))
         return output
+
+    def mfma_tuning(self):
+        """(waves per workgroup, workgroups per XCD, ms) picked by the autotuner of the MFMA apply."""
+        cfg = np.zeros(2, np.int32)
+        ms = _c_f()
+        _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ctypes.byref(ms)))
+        return int(cfg[0]), int(cfg[1]), ms.value
 
     def ComputeMatrix(self):
         """Assemble on the device -> (BlockMsrMatrix, affine vector)."""

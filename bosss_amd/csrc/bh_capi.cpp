@@ -410,6 +410,13 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
     BH_API_END
 }
 
+int bh_op_tuning(bh_op* op, int* cfg, float* ms) {
+    BH_API_BEGIN
+    if (cfg) { cfg[0] = op->mfma_waves; cfg[1] = op->mfma_wg_per_xcd; }
+    if (ms) *ms = op->mfma_tuned_ms;
+    BH_API_END
+}
+
 int bh_op_assemble(bh_op* op, bh_bsr** out, double* affine, int flags) {
     BH_API_BEGIN
     bh_grid* g = op->grid;

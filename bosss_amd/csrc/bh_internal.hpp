@@ -149,6 +149,8 @@ struct bh_op {
     bh::DevBuf nu, nu32;
     bh::DevBuf tab64, tab32;    // device copies of the tables: G | T | Nn | vw | fw
     bh::DevBuf tab16;           // FP16 MFMA tables (built on first FP16-MFMA apply)
+    int mfma_waves = 0, mfma_wg_per_xcd = 0;   // autotuned launch shape of the MFMA apply (0: not yet)
+    float mfma_tuned_ms = 0.f;
     size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0;
     // boundary values and RHS -> affine vector
     std::vector<double> gbnd;

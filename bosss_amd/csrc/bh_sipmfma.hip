@@ -142,9 +142,7 @@ __device__ __forceinline__ int encode_cell(const int* idx, int n0, int n1) {
     return D == 2 ? idx[0] * n1 + idx[1] : idx[0] + idx[1] * n0 + idx[2] * n0 * n1;
 }
 
-constexpr int MFMA_WAVES = 8;
-
-template <int D, int K>
+template <int D, int K, int MFMA_WAVES>
 __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArgs a) {
     using L = MfmaLayout<D, K>;
     constexpr int Np = L::Np;
@@ -331,9 +329,23 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
     }
 }
 
+// Launch shape: waves per workgroup (4 or 8) and workgroups per XCD.  Picked
+// once per operator by timing the candidates on the operator's own grid
+// (autotuner); the choice is kept in the operator.
+template <int D, int K, int W>
+static void mfma_run(bh_op* op, const MfmaArgs& a, int per_xcd) {
+    hipLaunchKernelGGL((sip_apply_mfma_kernel<D, K, W>), dim3(8 * per_xcd), dim3(64 * W), 0, op->ctx->stream, a);
+    BH_HIP(hipGetLastError());
+}
+
+template <int D, int K>
+static void mfma_run_cfg(bh_op* op, const MfmaArgs& a, int waves, int per_xcd) {
+    if (waves == 4) mfma_run<D, K, 4>(op, a, per_xcd);
+    else mfma_run<D, K, 8>(op, a, per_xcd);
+}
+
 template <int D, int K>
 static void mfma_launch(bh_op* op, const MfmaArgs& a0) {
-    using L = MfmaLayout<D, K>;
     if (!op->tab16.p) {
         std::vector<uint16_t> h16;
         build_mfma_tables<D, K>(op->tab, h16);
@@ -343,12 +355,40 @@ static void mfma_launch(bh_op* op, const MfmaArgs& a0) {
     MfmaArgs a = a0;
     a.tab = (const _Float16*)op->tab16.p;
     const int ntiles = (a.J + 15) / 16;
-    // 8 XCDs x up to 64 workgroups each (2 per CU); each wave loops over tiles
-    int per_xcd = (ntiles + 8 * MFMA_WAVES - 1) / (8 * MFMA_WAVES);
-    if (per_xcd > 64) per_xcd = 64;
     a.tiles_per_xcd = (ntiles + 7) / 8;
-    hipLaunchKernelGGL((sip_apply_mfma_kernel<D, K>), dim3(8 * per_xcd), dim3(64 * MFMA_WAVES), 0, op->ctx->stream, a);
-    BH_HIP(hipGetLastError());
+    if (op->mfma_waves == 0) {
+        // autotune on scratch output (beta = 0, no affine): candidates (waves, workgroups per XCD)
+        DevBuf ys((size_t)a.J * basis_count(D, K) * sizeof(float));
+        MfmaArgs t = a;
+        t.y = ys.as<float>();
+        t.beta = 0.f;
+        t.affine = nullptr;
+        float best = 1e30f;
+        int bw = 8, bx = 64;
+        hipEvent_t e0, e1;
+        BH_HIP(hipEventCreate(&e0));
+        BH_HIP(hipEventCreate(&e1));
+        for (int waves : {4, 8})
+            for (int px : {32, 64, 128, 256}) {
+                const int need = (a.tiles_per_xcd + waves - 1) / waves;   // no more workgroups than tiles
+                const int pxe = px < need ? px : (need > 0 ? need : 1);
+                if (pxe != px && px != 32) continue;
+                mfma_run_cfg<D, K>(op, t, waves, pxe);
+                BH_HIP(hipEventRecord(e0, op->ctx->stream));
+                for (int r = 0; r < 3; ++r) mfma_run_cfg<D, K>(op, t, waves, pxe);
+                BH_HIP(hipEventRecord(e1, op->ctx->stream));
+                BH_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                BH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; bw = waves; bx = pxe; }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        op->mfma_waves = bw;
+        op->mfma_wg_per_xcd = bx;
+        op->mfma_tuned_ms = best / 3;
+    }
+    mfma_run_cfg<D, K>(op, a, op->mfma_waves, op->mfma_wg_per_xcd);
 }
 
 void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine) {
