@@ -135,6 +135,21 @@ def solve_leg(B, ctx, grid, op, n, k, out):
     out["time_to_solve"] = res
 
 
+def measured_traffic(n, k, kernel_prefix):
+    """HBM bytes per launch (read + write) of a kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (counters scaled by a copy
+    kernel of known size); None if not profiled for this workload."""
+    path = os.path.join(ROOT, "profiles", f"sip_apply_k{k}_{n}cubed_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    for name, v in d["kernels"].items():
+        if kernel_prefix in name:
+            return v["read_bytes"] + v["write_bytes"]
+    return None
+
+
 def xdg_leg(B, ctx, out, n=48, seed=3):
     """Two-phase XDG Poisson, sphere in a 48^3 grid (k = 2, diffusion 1 inside /
     1000 outside, agglomeration threshold 0.1): set-up (cut-cell rules,
@@ -366,7 +381,7 @@ def main():
         "config": {"workload": f"3D SIP Poisson {n}^3 hexes k={k} ({N} DOF), FP64 matrix-free operator apply",
                    "grid": f"{n}^3", "k": k, "dof": N},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "frac": achieved / PEAK_FP64, "traffic": measured_traffic(n, k, "sip_apply_kernel<3, 3, double>"),
                      "kernel": "sip_apply_kernel<3,3,double>", "kernel_ms": kern_ms,
                      "algorithmic_flop": flops, "algorithmic_bytes": hbm_bytes,
                      "hbm_frac_at_algorithmic_bytes": hbm_bytes / (kern_ms * 1e-3) / PEAK_HBM},
@@ -400,6 +415,8 @@ def main():
         ms = ctx.elapsed_ms(0, 1) / args.steps
         err = float(np.linalg.norm(y32.download().astype(np.float64) - ref) / np.linalg.norm(ref))
         out[name] = {"kernel_ms": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": bytes32,
+                     "traffic": measured_traffic(n, k, "sip_apply_mfma_kernel<3, 3, 8>" if prec == B.BH_FP16_MFMA
+                                                 else "sip_apply_kernel<3, 3, float>"),
                      "hbm_GBps": bytes32 / (ms * 1e-3) / 1e9, "hbm_frac": bytes32 / (ms * 1e-3) / PEAK_HBM,
                      "algorithmic_flop": flops, "TFLOPs": flops / (ms * 1e-3) / 1e12,
                      "compute_frac": flops / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}

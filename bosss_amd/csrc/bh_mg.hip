@@ -117,16 +117,17 @@ __global__ __launch_bounds__(256) void block_gemv_k(int nrows, const T* __restri
     }
 }
 
-// x = Ainv b (dense, row-major n x n), one wave per row
-template <typename T>
-__global__ __launch_bounds__(256) void dense_gemv_k(int n, const T* __restrict__ A, const T* __restrict__ b, T* __restrict__ x) {
+// x = Ainv b (dense, row-major n x n), one wave per row; the matrix and the
+// accumulation may be wider than the vectors (FP64 inverse in an FP32 cycle)
+template <typename T, typename M>
+__global__ __launch_bounds__(256) void dense_gemv_k(int n, const M* __restrict__ A, const T* __restrict__ b, T* __restrict__ x) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + wave;
     if (row >= n) return;
-    T s = 0;
-    for (int c = lane; c < n; c += 64) s += A[(size_t)row * n + c] * b[c];
+    M s = 0;
+    for (int c = lane; c < n; c += 64) s += A[(size_t)row * n + c] * (M)b[c];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) x[row] = s;
+    if (lane == 0) x[row] = (T)s;
 }
 
 template <typename T>
@@ -175,8 +176,8 @@ void block_gemv(bh_ctx* ctx, int nrows, int bs, const T* Dinv, const T* r, T* x,
 }
 
 template <typename T>
-void dense_gemv(bh_ctx* ctx, int n, const T* A, const T* b, T* x) {
-    hipLaunchKernelGGL((dense_gemv_k<T>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, A, b, x);
+void dense_gemv(bh_ctx* ctx, int n, const double* A, const T* b, T* x) {
+    hipLaunchKernelGGL((dense_gemv_k<T, double>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, A, b, x);
 }
 
 template <typename T>
@@ -190,7 +191,7 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void agg_restrict<T>(bh_ctx*, int, int, const int*, const int*, const T*, const T*, T*); \
     template void agg_prolong_add<T>(bh_ctx*, int, int, const int*, const T*, const T*, T*);         \
     template void block_gemv<T>(bh_ctx*, int, int, const T*, const T*, T*, double, bool);             \
-    template void dense_gemv<T>(bh_ctx*, int, const T*, const T*, T*);                               \
+    template void dense_gemv<T>(bh_ctx*, int, const double*, const T*, T*);                               \
     template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);
 BH_INST(float)
 BH_INST(double)

@@ -14,7 +14,8 @@ template <typename T> void p_prolong_add(bh_ctx*, int64_t J, int bsf, int bsc, c
 template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr, const int* cidx, const T* P, const T* rf, T* rc);
 template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const T* P, const T* xc, T* xf);
 template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
-template <typename T> void dense_gemv(bh_ctx*, int n, const T* A, const T* b, T* x);
+// coarsest solve: FP64 inverse and accumulation for either cycle precision
+template <typename T> void dense_gemv(bh_ctx*, int n, const double* A, const T* b, T* x);
 template <typename T> void vcopy(bh_ctx*, const T* s, T* d, int64_t n);
 void bsr_residual(bh_bsr* A, const void* b, const void* x, void* r, int prec);
 
@@ -47,7 +48,7 @@ struct PMultigrid : bh_solver {
     int kin = 0;
     std::vector<MgLevel> lv;
     int ncoarse = 0;
-    DevBuf Ainv64, Ainv32;
+    DevBuf Ainv64;                 // coarsest level: dense FP64 inverse (both cycle precisions)
     DevBuf in32, out32;
     bh_bsr* ownedAf = nullptr;
     // learned correction after the cycle (guarded by residual minimisation)

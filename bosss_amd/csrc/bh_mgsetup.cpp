@@ -352,10 +352,6 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
     dense_inverse(dense, ncoarse);
     Ainv64.alloc(dense.size() * sizeof(double));
     ctx->upload(Ainv64.p, dense.data(), dense.size() * sizeof(double));
-    if (f32) {
-        Ainv32.alloc(dense.size() * sizeof(float));
-        convert_f64_f32(ctx, Ainv64.as<double>(), Ainv32.as<float>(), (int64_t)dense.size());
-    }
     ctx->sync();
 }
 
@@ -379,7 +375,7 @@ void PMultigrid::vcycle(int l) {
     T* r = vec<T>(L.r);
     const bool f32 = sizeof(T) == 4;
     if (l + 1 == (int)lv.size()) {
-        dense_gemv<T>(ctx, ncoarse, (const T*)(f32 ? Ainv32.p : Ainv64.p), b, x);
+        dense_gemv<T>(ctx, ncoarse, Ainv64.as<double>(), b, x);
         return;
     }
     const T* Dinv = (const T*)(f32 ? L.dinv32.p : L.dinv64.p);
