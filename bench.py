@@ -422,7 +422,8 @@ def main():
                      "compute_frac": flops / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}
         if prec == B.BH_FP16_MFMA:
             w_, x_, t_ = op.mfma_tuning()
-            out[name]["autotuned_launch"] = {"waves_per_workgroup": w_, "workgroups_per_xcd": x_, "tuning_ms": t_}
+            out[name]["autotuned_launch"] = {"waves_per_workgroup": w_ % 100, "prefetch_neighbours": w_ >= 100,
+                                             "workgroups_per_xcd": x_, "tuning_ms": t_}
     del u32, y32
     if not args.no_solve:
         solve_leg(B, ctx, grid, op, n, k, out)

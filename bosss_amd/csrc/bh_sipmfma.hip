@@ -142,7 +142,7 @@ __device__ __forceinline__ int encode_cell(const int* idx, int n0, int n1) {
     return D == 2 ? idx[0] * n1 + idx[1] : idx[0] + idx[1] * n0 + idx[2] * n0 * n1;
 }
 
-template <int D, int K, int MFMA_WAVES>
+template <int D, int K, int MFMA_WAVES, bool PF>
 __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArgs a) {
     using L = MfmaLayout<D, K>;
     constexpr int Np = L::Np;
@@ -231,35 +231,60 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
 #pragma unroll
                 for (int i = 0; i < 4; ++i) yacc[mt][i] += kr[i] * yv[mt][i];
         }
-        // ---------------- faces
-#pragma unroll 1
+        // ---------------- faces.  PF: all six neighbour loads are issued first so their
+        // latencies overlap (more registers); otherwise each face loads when it starts.
+        constexpr int NPF = PF ? 2 * D : 1;
+        float uqr[NPF][8], nuQa[NPF], hqa[NPF], usQa[NPF];
+        h8_t bQa[NPF];
+        auto load_face = [&](int f, float* uq, float& nuq, float& hq) {
+            const int d = f >> 1, s = f & 1;
+            const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+            int nidx[3] = {idx[0], idx[1], idx[2]};
+            nidx[d] += isBnd ? 0 : (s ? 1 : -1);
+            const int cq = isBnd ? cc : encode_cell<D>(nidx, a.n0, a.n1);
+            hq = isBnd ? hP[d] : (float)a.width[d * a.nmax + nidx[d]];
+            nuq = isBnd ? nuP : a.nu[cq];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int n = 8 * grp + t;
+                uq[t] = (!isBnd && n < Np) ? a.u[(size_t)cq * Np + n] : 0.f;
+            }
+        };
+        auto to_h8 = [&](const float* uq, h8_t& bq, float& us) {
+            float mq = 0.f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) mq = fmaxf(mq, fabsf(uq[t]));
+            mq = lane_max4(mq);
+            us = mq > 0.f ? mq : 1.f;
+            const float r = 1.f / us;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) bq[t] = (_Float16)(uq[t] * r);
+        };
+        if constexpr (PF) {
+#pragma unroll
+            for (int f = 0; f < 2 * D; ++f) load_face(f, uqr[f], nuQa[f], hqa[f]);
+#pragma unroll
+            for (int f = 0; f < 2 * D; ++f) to_h8(uqr[f], bQa[f], usQa[f]);
+        }
+#pragma unroll (PF ? 2 * D : 1)
         for (int f = 0; f < 2 * D; ++f) {
+            const int fi = PF ? f : 0;
+            if constexpr (PF) {
+                __builtin_amdgcn_sched_barrier(0);   // one face at a time: bounds the register pressure
+            } else {
+                load_face(f, uqr[0], nuQa[0], hqa[0]);
+                to_h8(uqr[0], bQa[0], usQa[0]);
+            }
             const int d = f >> 1, s = f & 1;
             const float sgn = s ? 1.f : -1.f;
             const int fQ = 2 * d + (1 - s);
             const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
             const bool skip = isBnd && ((a.bc_neumann >> f) & 1);
-            int cq = cc;
-            float ihq = ih[d];
             int nidx[3] = {idx[0], idx[1], idx[2]};
-            if (!isBnd) {
-                nidx[d] += s ? 1 : -1;
-                cq = encode_cell<D>(nidx, a.n0, a.n1);
-                ihq = 1.f / (float)a.width[d * a.nmax + nidx[d]];
-            }
-            float uq[8];
-            float mq = 0.f;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int n = 8 * grp + t;
-                uq[t] = (!isBnd && n < Np) ? a.u[(size_t)cq * Np + n] : 0.f;
-                mq = fmaxf(mq, fabsf(uq[t]));
-            }
-            mq = lane_max4(mq);
-            const float usQ = mq > 0.f ? mq : 1.f, rusQ = 1.f / usQ;
-            h8_t bQ;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) bQ[t] = (_Float16)(uq[t] * rusQ);
+            nidx[d] += isBnd ? 0 : (s ? 1 : -1);
+            const float ihq = 1.f / hqa[fi];
+            const float usQ = usQa[fi];
+            const h8_t bQ = bQa[fi];
             const f4_t z = {0.f, 0.f, 0.f, 0.f};
             const f4_t dT = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((f * 2 + 0) * 16 + col) * 32 + 8 * grp]), bU, z, 0, 0, 0);
             const f4_t dN = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((f * 2 + 1) * 16 + col) * 32 + 8 * grp]), bU, z, 0, 0, 0);
@@ -269,7 +294,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
             const float gP = 2.f * ih[d], gQ = 2.f * ihq;
             const float sdQinv = sdPinv * sqrtf(hP[d] * ihq);
             const float wf = gP / sdPinv;              // dA * sdPinv = (2/h_d) * detJ * sdPinv, detJ = 1/sdPinv^2
-            const float nuQ = isBnd ? nuP : a.nu[cq];
+            const float nuQ = nuQa[fi];
             const float nuMax = fabsf(nuP) > fabsf(nuQ) ? nuP : nuQ;
             const float bP = 1.f + 0.5f * (float)((idx[d] == 0) + (idx[d] == nn[d] - 1));
             const float bQf = 1.f + 0.5f * (float)((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
@@ -332,16 +357,21 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
 // Launch shape: waves per workgroup (4 or 8) and workgroups per XCD.  Picked
 // once per operator by timing the candidates on the operator's own grid
 // (autotuner); the choice is kept in the operator.
-template <int D, int K, int W>
+template <int D, int K, int W, bool PF>
 static void mfma_run(bh_op* op, const MfmaArgs& a, int per_xcd) {
-    hipLaunchKernelGGL((sip_apply_mfma_kernel<D, K, W>), dim3(8 * per_xcd), dim3(64 * W), 0, op->ctx->stream, a);
+    hipLaunchKernelGGL((sip_apply_mfma_kernel<D, K, W, PF>), dim3(8 * per_xcd), dim3(64 * W), 0, op->ctx->stream, a);
     BH_HIP(hipGetLastError());
 }
 
+// cfg: waves per workgroup (4 / 8) + 100 if the neighbour loads are prefetched
 template <int D, int K>
-static void mfma_run_cfg(bh_op* op, const MfmaArgs& a, int waves, int per_xcd) {
-    if (waves == 4) mfma_run<D, K, 4>(op, a, per_xcd);
-    else mfma_run<D, K, 8>(op, a, per_xcd);
+static void mfma_run_cfg(bh_op* op, const MfmaArgs& a, int cfg, int per_xcd) {
+    switch (cfg) {
+        case 4: mfma_run<D, K, 4, false>(op, a, per_xcd); break;
+        case 8: mfma_run<D, K, 8, false>(op, a, per_xcd); break;
+        case 104: mfma_run<D, K, 4, true>(op, a, per_xcd); break;
+        default: mfma_run<D, K, 8, true>(op, a, per_xcd); break;
+    }
 }
 
 template <int D, int K>
@@ -368,19 +398,20 @@ static void mfma_launch(bh_op* op, const MfmaArgs& a0) {
         hipEvent_t e0, e1;
         BH_HIP(hipEventCreate(&e0));
         BH_HIP(hipEventCreate(&e1));
-        for (int waves : {4, 8})
+        for (int cfg : {4, 8, 104, 108})
             for (int px : {32, 64, 128, 256}) {
+                const int waves = cfg % 100;
                 const int need = (a.tiles_per_xcd + waves - 1) / waves;   // no more workgroups than tiles
                 const int pxe = px < need ? px : (need > 0 ? need : 1);
                 if (pxe != px && px != 32) continue;
-                mfma_run_cfg<D, K>(op, t, waves, pxe);
+                mfma_run_cfg<D, K>(op, t, cfg, pxe);
                 BH_HIP(hipEventRecord(e0, op->ctx->stream));
-                for (int r = 0; r < 3; ++r) mfma_run_cfg<D, K>(op, t, waves, pxe);
+                for (int r = 0; r < 3; ++r) mfma_run_cfg<D, K>(op, t, cfg, pxe);
                 BH_HIP(hipEventRecord(e1, op->ctx->stream));
                 BH_HIP(hipEventSynchronize(e1));
                 float ms = 0;
                 BH_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) { best = ms; bw = waves; bx = pxe; }
+                if (ms < best) { best = ms; bw = cfg; bx = pxe; }
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

@@ -103,8 +103,9 @@ int bh_sip_set_rhs(bh_op* op, const double* f, int flags);
 int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double* y, int affine,
                    int prec, int flags);
 /* No counterpart in the reference: the launch shape the autotuner picked for the
- * FP16-MFMA apply on its first call (cfg[0] waves per workgroup, cfg[1]
- * workgroups per XCD; 0 before the first call) and its timed duration (ms). */
+ * FP16-MFMA apply on its first call (cfg[0] waves per workgroup, plus 100 when
+ * the variant that prefetches all neighbour blocks won; cfg[1] workgroups per
+ * XCD; 0 before the first call) and its timed duration (ms). */
 int bh_op_tuning(bh_op* op, int* cfg, float* ms);
 /* IEvaluatorLinear.ComputeMatrix (F/DifferentialOperator.cs:1787-1860):
  * assemble the operator matrix on the device and return the affine vector
