@@ -62,7 +62,8 @@ SIGNATURES = {
     "bh_sip_set_rhs": (_c_i, [_c_p, _c_p, _c_i]),
     "bh_op_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i, _c_i]),
     "bh_op_assemble": (_c_i, [_c_p, _PP, _c_p, _c_i]),
-    "bh_op_tuning": (_c_i, [_c_p, _c_p, ctypes.POINTER(_c_f)]),
+    "bh_op_tuning": (_c_i, [_c_p, _c_p, _c_p]),
+    "bh_op_set_fp64_path": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_bsr_create": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p, _PP]),
     "bh_bsr_destroy": (_c_i, [_c_p]),
     "bh_bsr_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i64)]),
@@ -425,10 +426,23 @@ class SipOperator(_Owned):
 
     def mfma_tuning(self):
         """(waves per workgroup, workgroups per XCD, ms) picked by the autotuner of the MFMA apply."""
-        cfg = np.zeros(2, np.int32)
-        ms = _c_f()
-        _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ctypes.byref(ms)))
-        return int(cfg[0]), int(cfg[1]), ms.value
+        cfg = np.zeros(5, np.int32)
+        ms = (_c_f * 3)()
+        _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ms))
+        return int(cfg[0]), int(cfg[1]), ms[0]
+
+    def fp64_tuning(self):
+        """FP64 apply choice: ('vector' | 'mfma' | None, waves, workgroups per XCD, vector ms, MFMA ms)."""
+        cfg = np.zeros(5, np.int32)
+        ms = (_c_f * 3)()
+        _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ms))
+        path = {0: None, 1: "vector", 2: "mfma"}[int(cfg[2])]
+        return path, int(cfg[3]), int(cfg[4]), ms[1], ms[2]
+
+    def force_fp64_path(self, path, waves=8, wg_per_xcd=64):
+        """Pin the FP64 apply to 'vector' or 'mfma' (tests compare both against the oracle)."""
+        code = {"vector": 1, "mfma": 2}[path]
+        _check(lib().bh_op_set_fp64_path(self.h, code, int(waves), int(wg_per_xcd)))
 
     def ComputeMatrix(self):
         """Assemble on the device -> (BlockMsrMatrix, affine vector)."""

@@ -412,8 +412,30 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
 
 int bh_op_tuning(bh_op* op, int* cfg, float* ms) {
     BH_API_BEGIN
-    if (cfg) { cfg[0] = op->mfma_waves; cfg[1] = op->mfma_wg_per_xcd; }
-    if (ms) *ms = op->mfma_tuned_ms;
+    if (cfg) {
+        cfg[0] = op->mfma_waves;
+        cfg[1] = op->mfma_wg_per_xcd;
+        cfg[2] = op->f64_path;
+        cfg[3] = op->f64_waves;
+        cfg[4] = op->f64_wg_per_xcd;
+    }
+    if (ms) { ms[0] = op->mfma_tuned_ms; ms[1] = op->f64_ms_vector; ms[2] = op->f64_ms_mfma; }
+    BH_API_END
+}
+
+int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
+    BH_API_BEGIN
+    BH_CHECK(op, "bh_op_set_fp64_path: null operator");
+    BH_CHECK(path >= 0 && path <= 2, "bh_op_set_fp64_path: path must be 0 (autotune), 1 (vector) or 2 (MFMA)");
+    if (path == 2) {
+        BH_CHECK(bh::sip_apply_mfma64_supported(op->D, op->k),
+                 "bh_op_set_fp64_path: the FP64-MFMA apply supports D in {2, 3} and k in {2, 3}");
+        BH_CHECK(waves == 4 || waves == 8, "bh_op_set_fp64_path: waves must be 4 or 8");
+        BH_CHECK(wg_per_xcd >= 1 && wg_per_xcd <= 4096, "bh_op_set_fp64_path: wg_per_xcd out of range");
+        op->f64_waves = waves;
+        op->f64_wg_per_xcd = wg_per_xcd;
+    }
+    op->f64_path = path;
     BH_API_END
 }
 

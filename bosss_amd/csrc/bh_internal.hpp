@@ -151,6 +151,11 @@ struct bh_op {
     bh::DevBuf tab16;           // FP16 MFMA tables (built on first FP16-MFMA apply)
     int mfma_waves = 0, mfma_wg_per_xcd = 0;   // autotuned launch shape of the MFMA apply (0: not yet)
     float mfma_tuned_ms = 0.f;
+    // FP64 apply: vector kernel or FP64-MFMA kernel, picked by timing on first use
+    bh::DevBuf tab64m;          // FP64 MFMA tables
+    int f64_path = 0;           // 0: not tuned, 1: vector, 2: MFMA
+    int f64_waves = 8, f64_wg_per_xcd = 64;
+    float f64_ms_vector = 0.f, f64_ms_mfma = 0.f;
     size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0;
     // boundary values and RHS -> affine vector
     std::vector<double> gbnd;
@@ -182,6 +187,10 @@ namespace bh {
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec);
 // FP32 vectors, FP16 MFMA contractions with FP32 accumulation
 void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine);
+// FP64 vectors, FP64 MFMA contractions (same results as the vector kernel up to summation order)
+bool sip_apply_mfma64_supported(int D, int K);
+void sip_apply_mfma64(bh_op* op, double alpha, const double* u, double beta, double* y, const double* affine, int waves,
+                      int per_xcd);
 void sip_assemble(bh_op* op, bh_bsr* M);
 void bsr_spmv(bh_bsr* A, double alpha, const void* x, double beta, void* y, int prec);
 void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32);

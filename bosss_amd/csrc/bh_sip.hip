@@ -16,6 +16,7 @@
 // recomputed from the neighbour's modes, read through L2), so y is written
 // exactly once per apply: HBM traffic = u + y + nu.
 #include <cmath>
+#include <functional>
 
 #include "bh_internal.hpp"
 
@@ -340,9 +341,53 @@ static void dispatch_apply(bh_op* op, const SipArgs<R>& a) {
     throw Error(-3, "sip_apply: unsupported (D, k) = (" + std::to_string(D) + ", " + std::to_string(K) + ")");
 }
 
+// Autotuner of the FP64 apply: times the vector kernel and the FP64-MFMA kernel
+// (waves per workgroup x workgroups per XCD) on a scratch vector of this
+// operator and keeps the fastest; both compute the same quadrature.
+static void tune_fp64_path(bh_op* op) {
+    op->f64_path = 1;
+    if (!sip_apply_mfma64_supported(op->D, op->k)) return;
+    DevBuf us(op->N * sizeof(double)), ys(op->N * sizeof(double));
+    fill(op->ctx, us.as<double>(), 1.0, op->N);
+    hipEvent_t e0, e1;
+    BH_HIP(hipEventCreate(&e0));
+    BH_HIP(hipEventCreate(&e1));
+    auto time_it = [&](const std::function<void()>& f) {
+        f();
+        BH_HIP(hipEventRecord(e0, op->ctx->stream));
+        for (int r = 0; r < 3; ++r) f();
+        BH_HIP(hipEventRecord(e1, op->ctx->stream));
+        BH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        BH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return ms / 3;
+    };
+    const float tv = time_it([&] { dispatch_apply<double>(op, make_args<double>(op, us.p, ys.p, 1.0, 0.0, false)); });
+    float best = 1e30f;
+    int bw = 8, bx = 64;
+    for (int w : {4, 8})
+        for (int px : {32, 64, 128}) {
+            const float t = time_it([&] { sip_apply_mfma64(op, 1.0, us.as<double>(), 0.0, ys.as<double>(), nullptr, w, px); });
+            if (t < best) { best = t; bw = w; bx = px; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    op->f64_ms_vector = tv;
+    op->f64_ms_mfma = best;
+    op->f64_waves = bw;
+    op->f64_wg_per_xcd = bx;
+    op->f64_path = best < tv ? 2 : 1;
+}
+
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec) {
     if (add_affine && op->has_affine && !op->affine_valid) op->rebuild_affine();
     if (prec == BH_FP64) {
+        if (op->f64_path == 0) tune_fp64_path(op);
+        if (op->f64_path == 2) {
+            const double* aff = (add_affine && op->has_affine) ? op->affine.as<double>() : nullptr;
+            sip_apply_mfma64(op, alpha, (const double*)u, beta, (double*)y, aff, op->f64_waves, op->f64_wg_per_xcd);
+            return;
+        }
         dispatch_apply<double>(op, make_args<double>(op, u, y, alpha, beta, add_affine));
     } else if (prec == BH_FP32) {
         if (add_affine && op->has_affine) throw Error(-3, "FP32 evaluate supports the linear part only");

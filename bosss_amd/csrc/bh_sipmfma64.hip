@@ -1,0 +1,288 @@
+// FP64 matrix-free SIP apply on the matrix cores (gfx950): the drop-in
+// operator (SIPLaplace, L3-solution/BoSSS.Solution.NSECommon/ipLaplace.cs:100-250,
+// quadrature as F/NECQuadratureVolume.cs:377-870, F/NECQuadratureEdge.cs:561-1400)
+// with every basis-at-node and flux-to-modes contraction as
+// v_mfma_f64_16x16x4_f64.  Same results as the vector kernel up to summation order.
+//
+// One wave = 16 consecutive cells.  Lane layouts of the f64 16x16x4 product:
+// A[l & 15][l >> 4], B[l >> 4][l & 15], D[(l >> 4) + 4r][l & 15] (r = 0..3).
+//   basis at nodes  D[j][c] = sum_n G[j][n] u_c[n]   (rows j = table rows, K = modes in steps of 4)
+//   flux to modes   Y[c][m] += sum_j F[c][j] G[j][m]  (K = the 4 rows (l >> 4) + 4r of register r)
+// Register r of the first product is, per lane, the A operand of the second product
+// whose K slots are the rows {g + 4r}: the pointwise flux is formed in registers.
+// FP64 tables (volume gradient rows and face trace / normal-derivative rows,
+// row stride padded to an odd number of doubles against LDS bank conflicts)
+// are staged once per workgroup; workgroups loop over tiles, XCD by XCD.
+#include <cmath>
+#include <vector>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+constexpr int bnm(int n, int k) {
+    int r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+constexpr int pw(int b, int e) { return e == 0 ? 1 : b * pw(b, e - 1); }
+}  // namespace
+
+template <int D, int K>
+struct M64Layout {
+    static constexpr int Np = bnm(K + D, D);
+    static constexpr int NQ1 = K + 1;
+    static constexpr int NQV = pw(NQ1, D);
+    static constexpr int NQF = pw(NQ1, D - 1);
+    static constexpr int NK = (Np + 3) / 4;                 // K steps of the first product
+    static constexpr int NPK = NK * 4;                      // modes padded to the K step
+    static constexpr int LD = NPK | 1;                      // row stride (doubles): odd, >= Np
+    static constexpr int NVT = (D * NQV + 15) / 16;         // 16-row volume tiles
+    static constexpr int MT = (Np + 15) / 16;               // 16-column output tiles
+    static constexpr int G = 0;                             // [NVT*16][LD]  dpsi_n/dxi_d, rows j = d*NQV + q
+    static constexpr int F = G + NVT * 16 * LD;             // [2D][2][16][LD] trace / normal derivative at face node q
+    static constexpr int VW = F + 2 * D * 2 * 16 * LD;      // [NVT*16] volume weights per row
+    static constexpr int FW = VW + NVT * 16;                // [16] face weights
+    static constexpr int TOTAL = FW + 16;
+    static_assert(NQF <= 16 && Np <= 32, "FP64 MFMA apply: face nodes <= 16, Np <= 32");
+};
+
+template <int D, int K>
+static void build_tables64(const SipTables& t, std::vector<double>& h) {
+    using L = M64Layout<D, K>;
+    h.assign(L::TOTAL, 0.0);
+    for (int j = 0; j < D * L::NQV; ++j) {
+        const int d = j / L::NQV, q = j % L::NQV;
+        for (int n = 0; n < L::Np; ++n) h[L::G + (size_t)j * L::LD + n] = t.G[((size_t)q * D + d) * L::Np + n];
+        h[L::VW + j] = t.vw[q];
+    }
+    for (int f = 0; f < 2 * D; ++f)
+        for (int q = 0; q < L::NQF; ++q)
+            for (int n = 0; n < L::Np; ++n) {
+                h[L::F + (((size_t)f * 2 + 0) * 16 + q) * L::LD + n] = t.T[((size_t)f * L::NQF + q) * L::Np + n];
+                h[L::F + (((size_t)f * 2 + 1) * 16 + q) * L::LD + n] = t.Nn[((size_t)f * L::NQF + q) * L::Np + n];
+            }
+    for (int q = 0; q < L::NQF; ++q) h[L::FW + q] = t.fw[q];
+}
+
+struct M64Args {
+    int J, n0, n1, n2, nmax;
+    const double* __restrict__ width;
+    const double* __restrict__ nu;
+    const double* __restrict__ tab;
+    double pen;
+    int bc_neumann;
+    const double* __restrict__ u;
+    double* __restrict__ y;
+    const double* __restrict__ affine;
+    double alpha, beta;
+    int tiles_per_xcd;
+};
+
+template <int D>
+__device__ __forceinline__ void dec64(int c, int n0, int n1, int* idx) {
+    if (D == 2) { idx[0] = c / n1; idx[1] = c - idx[0] * n1; idx[2] = 0; }
+    else { int s = n0 * n1; idx[2] = c / s; int r = c - idx[2] * s; idx[1] = r / n0; idx[0] = r - idx[1] * n0; }
+}
+template <int D>
+__device__ __forceinline__ int enc64(const int* idx, int n0, int n1) {
+    return D == 2 ? idx[0] * n1 + idx[1] : idx[0] + idx[1] * n0 + idx[2] * n0 * n1;
+}
+
+#define BH_MFMA64(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+
+template <int D, int K, int W>
+__global__ __launch_bounds__(64 * W) void sip_apply_mfma64_kernel(M64Args a) {
+    using L = M64Layout<D, K>;
+    constexpr int Np = L::Np, NK = L::NK, LD = L::LD;
+    __shared__ double tab[L::TOTAL];
+    for (int i = threadIdx.x; i < L::TOTAL; i += blockDim.x) tab[i] = a.tab[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, grp = lane >> 4;
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    const int ntiles = (a.J + 15) / 16;
+    const int xcd = blockIdx.x & 7, nb_x = gridDim.x >> 3;
+    const int t0 = xcd * a.tiles_per_xcd, t1 = min(ntiles, t0 + a.tiles_per_xcd);
+    const d4_t z = {0, 0, 0, 0};
+    for (int tile = t0 + (blockIdx.x >> 3) * W + wave; tile < t1; tile += nb_x * W) {
+        const int c = tile * 16 + col;
+        const bool valid = c < a.J;
+        const int cc = valid ? c : a.J - 1;
+        int idx[3];
+        dec64<D>(cc, a.n0, a.n1, idx);
+        double h[3] = {1, 1, 1}, detJ = 1;
+        for (int d = 0; d < D; ++d) { h[d] = a.width[d * a.nmax + idx[d]]; detJ *= 0.5 * h[d]; }
+        const double nuP = a.nu[cc];
+        double bU[NK];
+#pragma unroll
+        for (int s = 0; s < NK; ++s) {
+            const int n = 4 * s + grp;
+            bU[s] = (valid && n < Np) ? a.u[(size_t)cc * Np + n] : 0.0;
+        }
+        d4_t yacc[L::MT];
+#pragma unroll
+        for (int mt = 0; mt < L::MT; ++mt) yacc[mt] = z;
+        // ---------------- volume
+        {
+            double s2[3] = {0, 0, 0};
+            for (int d = 0; d < D; ++d) s2[d] = -nuP * 4.0 / (h[d] * h[d]);
+#pragma unroll 2
+            for (int t = 0; t < L::NVT; ++t) {
+                d4_t g = z;
+#pragma unroll
+                for (int s = 0; s < NK; ++s) g = BH_MFMA64(tab[L::G + (t * 16 + col) * LD + 4 * s + grp], bU[s], g);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = t * 16 + grp + 4 * r;
+                    const int dj = j / L::NQV;
+                    const double fac = (dj == 0 ? s2[0] : (dj == 1 ? s2[1] : s2[2])) * tab[L::VW + j];
+                    const double F = j < D * L::NQV ? fac * g[r] : 0.0;
+#pragma unroll
+                    for (int mt = 0; mt < L::MT; ++mt) {
+                        const int m = col + 16 * mt;
+                        const double Bv = m < Np ? tab[L::G + j * LD + m] : 0.0;
+                        yacc[mt] = BH_MFMA64(F, Bv, yacc[mt]);
+                    }
+                }
+            }
+        }
+        // ---------------- faces
+        const double sdPinv = 1.0 / sqrt(detJ);
+        double cjP = 0;
+        for (int d = 0; d < D; ++d) cjP += (1.0 + 0.5 * ((idx[d] == 0) + (idx[d] == nn[d] - 1))) / h[d];
+#pragma unroll 1
+        for (int f = 0; f < 2 * D; ++f) {
+            const int d = f >> 1, s = f & 1;
+            const double sgn = s ? 1.0 : -1.0;
+            const int fQ = 2 * d + (1 - s);
+            const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+            const bool skip = isBnd && ((a.bc_neumann >> f) & 1);
+            int nidx[3] = {idx[0], idx[1], idx[2]};
+            nidx[d] += isBnd ? 0 : (s ? 1 : -1);
+            const int cq = isBnd ? cc : enc64<D>(nidx, a.n0, a.n1);
+            const double hq = isBnd ? h[d] : a.width[d * a.nmax + nidx[d]];
+            const double nuQ = isBnd ? nuP : a.nu[cq];
+            double bQ[NK];
+#pragma unroll
+            for (int t = 0; t < NK; ++t) {
+                const int n = 4 * t + grp;
+                bQ[t] = (!isBnd && n < Np) ? a.u[(size_t)cq * Np + n] : 0.0;
+            }
+            d4_t dT = z, dN = z, eT = z, eN = z;
+#pragma unroll
+            for (int t = 0; t < NK; ++t) {
+                dT = BH_MFMA64(tab[L::F + ((f * 2 + 0) * 16 + col) * LD + 4 * t + grp], bU[t], dT);
+                dN = BH_MFMA64(tab[L::F + ((f * 2 + 1) * 16 + col) * LD + 4 * t + grp], bU[t], dN);
+                eT = BH_MFMA64(tab[L::F + ((fQ * 2 + 0) * 16 + col) * LD + 4 * t + grp], bQ[t], eT);
+                eN = BH_MFMA64(tab[L::F + ((fQ * 2 + 1) * 16 + col) * LD + 4 * t + grp], bQ[t], eN);
+            }
+            const double gP = 2.0 / h[d], gQ = 2.0 / hq;
+            const double sdQinv = sdPinv * sqrt(h[d] / hq);
+            const double wf = gP / sdPinv;                 // dA * sdPinv
+            const double nuMax = fabs(nuP) > fabs(nuQ) ? nuP : nuQ;
+            const double bP = 1.0 + 0.5 * ((idx[d] == 0) + (idx[d] == nn[d] - 1));
+            const double bQf = 1.0 + 0.5 * ((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+            const double cjQ = cjP - bP / h[d] + bQf / hq;
+            const double mu = isBnd ? a.pen * cjP : a.pen * fmax(cjP, cjQ);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = grp + 4 * r;                 // face node of register r
+                const double vP = sdPinv * dT[r];
+                const double dP = sgn * gP * sdPinv * dN[r];
+                const double vQ = sdQinv * eT[r];
+                const double dQ = sgn * gQ * sdQinv * eN[r];
+                double cv, cd;
+                if (isBnd) {
+                    cv = nuP * dP - 2.0 * mu * nuP * vP;
+                    cd = nuP * vP;
+                } else {
+                    const double jump = vP - vQ;
+                    cv = 0.5 * (nuP * dP + nuQ * dQ) - mu * nuMax * jump;
+                    cd = 0.5 * nuP * jump;
+                }
+                const double wq = (skip || !valid) ? 0.0 : tab[L::FW + q] * wf;
+                const double cvw = wq * cv, cdw = wq * sgn * gP * cd;
+#pragma unroll
+                for (int mt = 0; mt < L::MT; ++mt) {
+                    const int m = col + 16 * mt;
+                    const double Tv = m < Np ? tab[L::F + ((f * 2 + 0) * 16 + q) * LD + m] : 0.0;
+                    const double Nv = m < Np ? tab[L::F + ((f * 2 + 1) * 16 + q) * LD + m] : 0.0;
+                    yacc[mt] = BH_MFMA64(cvw, Tv, yacc[mt]);
+                    yacc[mt] = BH_MFMA64(cdw, Nv, yacc[mt]);
+                }
+            }
+        }
+        // ---------------- output: lane holds Y[cell tile*16 + grp + 4r][m = col + 16*mt]
+#pragma unroll
+        for (int mt = 0; mt < L::MT; ++mt) {
+            const int m = col + 16 * mt;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = tile * 16 + grp + 4 * r;
+                if (co < a.J && m < Np) {
+                    const size_t o = (size_t)co * Np + m;
+                    double v = yacc[mt][r];
+                    if (a.affine) v += a.affine[o];
+                    a.y[o] = a.beta == 0.0 ? a.alpha * v : a.beta * a.y[o] + a.alpha * v;
+                }
+            }
+        }
+    }
+}
+
+template <int D, int K, int W>
+static void m64_run(bh_op* op, const M64Args& a, int per_xcd) {
+    hipLaunchKernelGGL((sip_apply_mfma64_kernel<D, K, W>), dim3(8 * per_xcd), dim3(64 * W), 0, op->ctx->stream, a);
+    BH_HIP(hipGetLastError());
+}
+
+template <int D, int K>
+static void m64_launch(bh_op* op, const M64Args& a0, int waves, int per_xcd) {
+    if (!op->tab64m.p) {
+        std::vector<double> h;
+        build_tables64<D, K>(op->tab, h);
+        op->tab64m.alloc(h.size() * sizeof(double));
+        op->ctx->upload(op->tab64m.p, h.data(), h.size() * sizeof(double));
+    }
+    M64Args a = a0;
+    a.tab = op->tab64m.as<double>();
+    const int ntiles = (a.J + 15) / 16;
+    a.tiles_per_xcd = (ntiles + 7) / 8;
+    const int need = (a.tiles_per_xcd + waves - 1) / waves;
+    if (per_xcd > need) per_xcd = need > 0 ? need : 1;
+    if (waves == 4) m64_run<D, K, 4>(op, a, per_xcd);
+    else m64_run<D, K, 8>(op, a, per_xcd);
+}
+
+bool sip_apply_mfma64_supported(int D, int K) { return (D == 3 && (K == 2 || K == 3)) || (D == 2 && (K == 2 || K == 3)); }
+
+void sip_apply_mfma64(bh_op* op, double alpha, const double* u, double beta, double* y, const double* affine, int waves,
+                      int per_xcd) {
+    bh_grid* g = op->grid;
+    M64Args a{};
+    a.J = g->J;
+    a.n0 = g->n[0]; a.n1 = g->n[1]; a.n2 = g->n[2];
+    a.nmax = g->nmax;
+    a.width = g->d_width.as<double>();
+    a.nu = op->nu.as<double>();
+    a.pen = op->penalty_base * op->penalty_deg;
+    a.bc_neumann = 0;
+    for (int f = 0; f < 2 * op->D; ++f) if (op->bc[f] == 1) a.bc_neumann |= 1 << f;
+    a.u = u;
+    a.y = y;
+    a.affine = affine;
+    a.alpha = alpha;
+    a.beta = beta;
+    const int D = op->D, K = op->k;
+    if (D == 3 && K == 3) return m64_launch<3, 3>(op, a, waves, per_xcd);
+    if (D == 3 && K == 2) return m64_launch<3, 2>(op, a, waves, per_xcd);
+    if (D == 2 && K == 3) return m64_launch<2, 3>(op, a, waves, per_xcd);
+    if (D == 2 && K == 2) return m64_launch<2, 2>(op, a, waves, per_xcd);
+    throw Error(-3, "sip_apply (FP64 MFMA): unsupported (D, k)");
+}
+
+}  // namespace bh

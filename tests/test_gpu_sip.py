@@ -81,6 +81,51 @@ def test_evaluate_fp64_matches_oracle(ctx, D, k, affine):
     assert rel(out2, ref2) < 1e-12
 
 
+MFMA64_CASES = [(2, 2), (2, 3), (3, 2), (3, 3)]
+
+
+@pytest.mark.parametrize("D,k", MFMA64_CASES)
+@pytest.mark.parametrize("affine", [False, True])
+@pytest.mark.parametrize("path,waves,per_xcd", [("vector", 8, 64), ("mfma", 4, 1), ("mfma", 8, 64)])
+def test_evaluate_fp64_paths_match_oracle(ctx, D, k, affine, path, waves, per_xcd):
+    """Both FP64 kernels (vector and FP64-MFMA, each pinned, several launch
+    shapes) within 1e-12 of the oracle, with Neumann sides, the affine part and
+    Evaluate(alpha, beta)."""
+    _say(f"case evaluate fp64 path={path} waves={waves} per_xcd={per_xcd} D={D} k={k} affine={affine}")
+    bc = [0, 1] * D if affine else None
+    og, gg, oo, go = _pair(ctx, D, k, 60 + 10 * D + k, bc=bc, affine=affine)
+    go.force_fp64_path(path, waves, per_xcd)
+    u = np.random.default_rng(15).standard_normal(oo.N)
+    assert rel(go.Evaluate(1.0, 0.0, np.zeros(go.N), u), oo.evaluate(u)) < 1e-12
+    y0 = np.random.default_rng(16).standard_normal(oo.N)
+    ref2 = oo.evaluate(u, alpha=-1.5, beta=0.25, y=y0.copy())
+    assert rel(go.Evaluate(-1.5, 0.25, y0.copy(), u), ref2) < 1e-12
+    assert go.fp64_tuning()[0] == path
+
+
+def test_evaluate_fp64_autotuned_ragged(ctx):
+    """Ragged 3D grid (13 x 11 x 9 cells, k = 3: last 16-cell tile partial, XCD
+    ranges uneven): the autotuned FP64 apply equals the vector kernel to 1e-13
+    and the oracle to 1e-12."""
+    _say("case evaluate fp64 autotuned ragged 13x11x9 k=3")
+    rng = np.random.default_rng(17)
+    nodes = [np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n)]) for n in (13, 11, 9)]
+    og = O.Grid(*nodes)
+    nu = log_uniform_nu(og.J, 17)
+    gg = B.Grid(ctx, *nodes)
+    oo = O.Sip(og, 3, nu)
+    go = B.SipOperator(gg, 3, nu)
+    u = rng.standard_normal(oo.N)
+    auto = go.Evaluate(1.0, 0.0, np.zeros(go.N), u, affine=False)
+    path, waves, per_xcd, ms_v, ms_m = go.fp64_tuning()
+    _say(f"  tuned: {path} waves={waves} per_xcd={per_xcd} vector {ms_v:.4f} ms, mfma {ms_m:.4f} ms")
+    assert path in ("vector", "mfma") and ms_v > 0 and ms_m > 0
+    go.force_fp64_path("vector")
+    vec = go.Evaluate(1.0, 0.0, np.zeros(go.N), u, affine=False)
+    assert rel(auto, vec) < 1e-13
+    assert rel(auto, oo.evaluate(u)) < 1e-12
+
+
 @pytest.mark.parametrize("D,k", [(2, 2), (3, 3)])
 def test_evaluate_device_pointers(ctx, D, k):
     _say(f"case evaluate device pointers D={D} k={k}")
