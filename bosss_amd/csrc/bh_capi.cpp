@@ -294,6 +294,7 @@ int bh_sip_create(bh_grid* g, int k, double penalty_base, const double* nu, cons
     op->off_N = t.size(); t.insert(t.end(), op->tab.Nn.begin(), op->tab.Nn.end());
     op->off_vw = t.size(); t.insert(t.end(), op->tab.vw.begin(), op->tab.vw.end());
     op->off_fw = t.size(); t.insert(t.end(), op->tab.fw.begin(), op->tab.fw.end());
+    op->off_K = t.size(); t.insert(t.end(), op->tab.Kv.begin(), op->tab.Kv.end());
     op->tab64.alloc(t.size() * sizeof(double));
     ctx->upload(op->tab64.p, t.data(), t.size() * sizeof(double));
     std::vector<float> t32(t.begin(), t.end());

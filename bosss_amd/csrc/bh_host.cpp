@@ -155,6 +155,14 @@ void SipTables::build(int D_, int k_, int nq1_) {
         for (int d = 0; d < D; ++d)
             for (int n = 0; n < Np; ++n) G[((size_t)q * D + d) * Np + n] = grad[n * D + d];
     }
+    Kv.assign((size_t)D * Np * Np, 0.0);
+    for (int d = 0; d < D; ++d)
+        for (int n = 0; n < Np; ++n)
+            for (int m = 0; m < Np; ++m) {
+                double s = 0;
+                for (int q = 0; q < nqv; ++q) s += vw[q] * G[((size_t)q * D + d) * Np + n] * G[((size_t)q * D + d) * Np + m];
+                Kv[((size_t)d * Np + n) * Np + m] = s;
+            }
     fw.assign(nqf, 1.0); fxi.assign((size_t)nqf * std::max(D - 1, 1), 0.0);
     for (int q = 0; q < nqf; ++q) {
         int r = q;

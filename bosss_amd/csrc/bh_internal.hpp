@@ -117,6 +117,9 @@ struct SipTables {
     std::vector<double> Nn;     // [2D][nqf][Np] d psi_n / d xi_d at face nodes
     std::vector<double> vxi;    // nqv*D reference volume nodes
     std::vector<double> fxi;    // nqf*(D-1) reference face-parameter nodes
+    // [D][Np][Np] reference stiffness per direction: Kv[d][n][m] = sum_q vw[q] G[q][d][n] G[q][d][m]
+    // (the volume integral of an axis-aligned box cell is sum_d (-nu 4/h_d^2) Kv[d], by the same rule)
+    std::vector<double> Kv;
     void build(int D, int k, int nq1);
 };
 }  // namespace bh
@@ -156,7 +159,7 @@ struct bh_op {
     int f64_path = 0;           // 0: not tuned, 1: vector, 2: MFMA
     int f64_waves = 8, f64_wg_per_xcd = 64;
     float f64_ms_vector = 0.f, f64_ms_mfma = 0.f;
-    size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0;
+    size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0, off_K = 0;
     // boundary values and RHS -> affine vector
     std::vector<double> gbnd;
     std::vector<int> bnd_slot;  // per edge: first slot or -1

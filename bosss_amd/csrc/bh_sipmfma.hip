@@ -2,22 +2,23 @@
 // sip_apply_kernel (SIPLaplace volume + interior/boundary face terms,
 // L3-solution/BoSSS.Solution.NSECommon/ipLaplace.cs:100-250, quadrature as in
 // F/NECQuadratureVolume.cs:377-870 and F/NECQuadratureEdge.cs:561-1400), with
-// FP32 vectors and the basis-at-node / flux-to-modes contractions as FP16
-// MFMA (v_mfma_f32_16x16x32_f16) with FP32 accumulation.
+// FP32 vectors and every contraction as FP16 MFMA (v_mfma_f32_16x16x32_f16)
+// with FP32 accumulation.
 //
 // One wave = a tile of 16 consecutive cells.  Every contraction is a small
 // GEMM with the cells as one dimension:
-//   basis at nodes:  D_t[j][c] = sum_n A_t[j][n] * u_c[n]     (A = table rows j, K = modes padded to 32)
-//   flux to modes:   Y[c][m] += sum_j F[c][j] * B[j][m]        (K = 32 rows j: two 16-row tiles)
-// The accumulator layout of the first product (lane = cell c, 4 rows j) is
-// the A-operand layout of the second (lane = row c, 8 consecutive k), so the
-// pointwise flux (scaling by nu, 4/h^2, weights; the SIP face flux from the
-// four traces of one face node, which sit in the same lane and register) is
-// computed in registers between the two MFMAs: no LDS round trip.
+//   volume:          Y[c][m] += sum_n (s_d(c) u_c[n]) Kv[d][n][m]   (reference stiffness per direction)
+//   face traces:     D[q][c] = sum_n T_f[q][n] * u_c[n]           (rows q = face nodes, K = modes padded to 32)
+//   flux to modes:   Y[c][m] += sum_k F[c][k] * B[k][m]           (K = 16 nodes x {trace, normal derivative})
+// The accumulator layout of a trace product (lane = cell c, 4 nodes) is the
+// A-operand layout of the flux product (lane = row c, 8 consecutive k), so the
+// SIP face flux (from the four traces of one face node, which sit in the same
+// lane and register) is computed in registers between the two MFMAs: no LDS
+// round trip.
 // FP16 range: each cell's modes are scaled by their max |u| before the cast,
 // and each cell's flux vector by its max |flux| before the second product;
 // the scales are applied to the FP32 results.
-// The tables (FP16, ~48 KB for 3D k = 3) are staged once per workgroup in LDS;
+// The tables (FP16, ~30 KB for 3D k = 3) are staged once per workgroup in LDS;
 // workgroups loop over tiles.
 #include <cmath>
 #include <cstring>
@@ -39,21 +40,18 @@ constexpr int binom_(int n, int k) {
 constexpr int ipow_(int b, int e) { return e == 0 ? 1 : b * ipow_(b, e - 1); }
 }  // namespace
 
-// Table layout (FP16, offsets in halves).  Volume rows are direction-major,
-// j = d*NQV + q, and carry the quadrature weight (GA = w_q * dpsi_n/dxi_d);
-// the face back-contraction rows carry the face weight (FB = w_q * trace).
+// Table layout (FP16, offsets in halves): the reference stiffness per direction
+// and the face tables; the face back-contraction rows carry the face weight
+// (FB = w_q * trace).
 template <int D, int K>
 struct MfmaLayout {
     static constexpr int Np = binom_(K + D, D);
     static constexpr int NQ1 = K + 1;
     static constexpr int NQV = ipow_(NQ1, D);
     static constexpr int NQF = ipow_(NQ1, D - 1);
-    static constexpr int NVT = ((D * NQV + 31) / 32) * 2;   // 16-row volume tiles, even count
-    static constexpr int NPAIR = NVT / 2;
     static constexpr int MT = (Np + 15) / 16;               // 16-column output tiles
-    static constexpr int GA = 0;                            // [NVT*16][32]   rows j, cols n
-    static constexpr int GB = GA + NVT * 16 * 32;           // [NPAIR][MT*16][32] (m, k-slot)
-    static constexpr int FA = GB + NPAIR * MT * 16 * 32;    // [2D][2][16][32] face trace / normal-derivative rows
+    static constexpr int KB = 0;                            // [D][MT*16][32] reference stiffness Kv[d] (m, k = n)
+    static constexpr int FA = KB + D * MT * 16 * 32;        // [2D][2][16][32] face trace / normal-derivative rows
     static constexpr int FB = FA + 2 * D * 2 * 16 * 32;     // [2D][MT*16][32]
     static constexpr int TOTAL = FB + 2 * D * MT * 16 * 32;
     static_assert(Np <= 32, "MFMA apply: Np <= 32");
@@ -61,29 +59,14 @@ struct MfmaLayout {
     static_assert(TOTAL % 8 == 0, "16-byte staging");
 };
 
-// k-slot s (0..7) of lane group g in a 32-deep K block built from two 16-row tiles
-__host__ __device__ constexpr int slot_row(int tile_pair, int g, int s) {
-    return s < 4 ? (2 * tile_pair) * 16 + 4 * g + s : (2 * tile_pair + 1) * 16 + 4 * g + (s - 4);
-}
-
 template <int D, int K>
 void build_mfma_tables(const SipTables& t, std::vector<uint16_t>& h16) {
     using L = MfmaLayout<D, K>;
     std::vector<float> f(L::TOTAL, 0.0f);
-    const int Np = L::Np, NQV = L::NQV, NQF = L::NQF;
-    auto G = [&](int j, int n) -> double {   // volume row j = d*NQV + q
-        if (j >= D * NQV || n >= Np) return 0.0;
-        const int d = j / NQV, q = j % NQV;
-        return t.G[((size_t)q * D + d) * Np + n];
-    };
-    auto W = [&](int j) -> double { return j < D * NQV ? t.vw[j % NQV] : 0.0; };
-    for (int j = 0; j < L::NVT * 16; ++j)
-        for (int n = 0; n < 32; ++n) f[L::GA + j * 32 + n] = (float)(W(j) * G(j, n));
-    for (int p = 0; p < L::NPAIR; ++p)
-        for (int m = 0; m < L::MT * 16; ++m)
-            for (int g = 0; g < 4; ++g)
-                for (int s = 0; s < 8; ++s)
-                    f[L::GB + ((size_t)p * L::MT * 16 + m) * 32 + 8 * g + s] = (float)G(slot_row(p, g, s), m);
+    const int Np = L::Np, NQF = L::NQF;
+    for (int d = 0; d < D; ++d)
+        for (int m = 0; m < Np; ++m)
+            for (int n = 0; n < Np; ++n) f[L::KB + ((size_t)d * L::MT * 16 + m) * 32 + n] = (float)t.Kv[((size_t)d * Np + n) * Np + m];
     auto T = [&](int fc, int q, int n) -> double { return (q < NQF && n < Np) ? t.T[((size_t)fc * NQF + q) * Np + n] : 0.0; };
     auto N = [&](int fc, int q, int n) -> double { return (q < NQF && n < Np) ? t.Nn[((size_t)fc * NQF + q) * Np + n] : 0.0; };
     auto FW = [&](int q) -> double { return q < NQF ? t.fw[q] : 0.0; };
@@ -190,37 +173,27 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
         f4_t yacc[L::MT];
 #pragma unroll
         for (int mt = 0; mt < L::MT; ++mt) yacc[mt] = f4_t{0.f, 0.f, 0.f, 0.f};
-        // ---------------- volume: -nu * sum_q w_q (4/h_d^2) dpsi_m/dxi_d dpsi_n/dxi_d u_n
+        // ---------------- volume: Y += sum_d diag(-nu 4/h_d^2) U Kv[d] (reference stiffness per
+        // direction: the volume rule of an axis-aligned box factorises); the FP16 A operand is
+        // the cell's scaled modes times -s2_d / max_d s2_d, the rest of the factor is applied
+        // to the FP32 result rows
         {
-            float s2[3] = {0.f, 0.f, 0.f}, s2max = 0.f;
-            for (int d = 0; d < D; ++d) { s2[d] = 4.f * ih[d] * ih[d]; s2max = fmaxf(s2max, s2[d]); }
-            const float r2 = 1.f / s2max;
-            const float rs0 = -s2[0] * r2, rs1 = -s2[1] * r2, rs2 = D == 3 ? -s2[2] * r2 : 0.f;
+            float s2max = 0.f;
+            for (int d = 0; d < D; ++d) s2max = fmaxf(s2max, 4.f * ih[d] * ih[d]);
             const float kap = usP * nuP * s2max;
             f4_t yv[L::MT];
 #pragma unroll
             for (int mt = 0; mt < L::MT; ++mt) yv[mt] = f4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-            for (int p = 0; p < L::NPAIR; ++p) {
-                const f4_t z = {0.f, 0.f, 0.f, 0.f};
-                const h8_t A0 = lds_h8(&tab[L::GA + ((2 * p) * 16 + col) * 32 + 8 * grp]);
-                const h8_t A1 = lds_h8(&tab[L::GA + ((2 * p + 1) * 16 + col) * 32 + 8 * grp]);
-                const f4_t d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, bU, z, 0, 0, 0);
-                const f4_t d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, bU, z, 0, 0, 0);
-                h8_t F;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int j0 = (2 * p) * 16 + 4 * grp + i, j1 = j0 + 16;
-                    const int e0 = j0 / L::NQV, e1 = j1 / L::NQV;      // row direction (rows >= D*NQV are zero)
-                    const float f0 = e0 == 0 ? rs0 : (e0 == 1 ? rs1 : rs2);
-                    const float f1 = e1 == 0 ? rs0 : (e1 == 1 ? rs1 : rs2);
-                    F[i] = (_Float16)(f0 * d0[i]);
-                    F[4 + i] = (_Float16)(f1 * d1[i]);
-                }
+            for (int d = 0; d < D; ++d) {
+                const float rs = -4.f * ih[d] * ih[d] / s2max;
+                h8_t Ad;
+#pragma unroll
+                for (int s = 0; s < 8; ++s) Ad[s] = (_Float16)(rs * (float)bU[s]);
 #pragma unroll
                 for (int mt = 0; mt < L::MT; ++mt) {
-                    const h8_t Bm = lds_h8(&tab[L::GB + ((size_t)p * L::MT * 16 + mt * 16 + col) * 32 + 8 * grp]);
-                    yv[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(F, Bm, yv[mt], 0, 0, 0);
+                    const h8_t Bm = lds_h8(&tab[L::KB + ((size_t)d * L::MT * 16 + mt * 16 + col) * 32 + 8 * grp]);
+                    yv[mt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ad, Bm, yv[mt], 0, 0, 0);
                 }
             }
             float kr[4];

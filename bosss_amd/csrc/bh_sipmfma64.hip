@@ -1,18 +1,20 @@
 // FP64 matrix-free SIP apply on the matrix cores (gfx950): the drop-in
 // operator (SIPLaplace, L3-solution/BoSSS.Solution.NSECommon/ipLaplace.cs:100-250,
 // quadrature as F/NECQuadratureVolume.cs:377-870, F/NECQuadratureEdge.cs:561-1400)
-// with every basis-at-node and flux-to-modes contraction as
-// v_mfma_f64_16x16x4_f64.  Same results as the vector kernel up to summation order.
+// with every contraction as v_mfma_f64_16x16x4_f64.  Same results as the
+// vector kernel up to summation order.
 //
 // One wave = 16 consecutive cells.  Lane layouts of the f64 16x16x4 product:
 // A[l & 15][l >> 4], B[l >> 4][l & 15], D[(l >> 4) + 4r][l & 15] (r = 0..3).
-//   basis at nodes  D[j][c] = sum_n G[j][n] u_c[n]   (rows j = table rows, K = modes in steps of 4)
-//   flux to modes   Y[c][m] += sum_j F[c][j] G[j][m]  (K = the 4 rows (l >> 4) + 4r of register r)
-// Register r of the first product is, per lane, the A operand of the second product
-// whose K slots are the rows {g + 4r}: the pointwise flux is formed in registers.
-// FP64 tables (volume gradient rows and face trace / normal-derivative rows,
-// row stride padded to an odd number of doubles against LDS bank conflicts)
-// are staged once per workgroup; workgroups loop over tiles, XCD by XCD.
+//   volume          Y[c][m] += sum_n (-nu_c 4/h_d^2 u_c[n]) Kv[d][n][m]  (reference stiffness
+//                   per direction: the volume rule of an axis-aligned box factorises)
+//   face traces     D[q][c] = sum_n T_f[q][n] u_c[n]   (rows q = face nodes, K = modes in steps of 4)
+//   flux to modes   Y[c][m] += sum_q F[c][q] T_f[q][m]  (K = the 4 nodes (l >> 4) + 4r of register r)
+// Register r of a trace product is, per lane, the A operand of the flux product
+// whose K slots are the nodes {g + 4r}: the pointwise flux is formed in registers.
+// FP64 tables (stiffness rows; face trace / normal-derivative rows with the row
+// stride padded to an odd number of doubles against LDS bank conflicts) are
+// staged once per workgroup; workgroups loop over tiles, XCD by XCD.
 #include <cmath>
 #include <vector>
 
@@ -40,12 +42,11 @@ struct M64Layout {
     static constexpr int NK = (Np + 3) / 4;                 // K steps of the first product
     static constexpr int NPK = NK * 4;                      // modes padded to the K step
     static constexpr int LD = NPK | 1;                      // row stride (doubles): odd, >= Np
-    static constexpr int NVT = (D * NQV + 15) / 16;         // 16-row volume tiles
     static constexpr int MT = (Np + 15) / 16;               // 16-column output tiles
-    static constexpr int G = 0;                             // [NVT*16][LD]  dpsi_n/dxi_d, rows j = d*NQV + q
-    static constexpr int F = G + NVT * 16 * LD;             // [2D][2][16][LD] trace / normal derivative at face node q
-    static constexpr int VW = F + 2 * D * 2 * 16 * LD;      // [NVT*16] volume weights per row
-    static constexpr int FW = VW + NVT * 16;                // [16] face weights
+    static constexpr int LDK = MT * 16;                     // row stride of the stiffness tables
+    static constexpr int KV = 0;                            // [D][NPK][LDK] reference stiffness Kv[d][n][m]
+    static constexpr int F = KV + D * NPK * LDK;            // [2D][2][16][LD] trace / normal derivative at face node q
+    static constexpr int FW = F + 2 * D * 2 * 16 * LD;      // [16] face weights
     static constexpr int TOTAL = FW + 16;
     static_assert(NQF <= 16 && Np <= 32, "FP64 MFMA apply: face nodes <= 16, Np <= 32");
 };
@@ -54,11 +55,10 @@ template <int D, int K>
 static void build_tables64(const SipTables& t, std::vector<double>& h) {
     using L = M64Layout<D, K>;
     h.assign(L::TOTAL, 0.0);
-    for (int j = 0; j < D * L::NQV; ++j) {
-        const int d = j / L::NQV, q = j % L::NQV;
-        for (int n = 0; n < L::Np; ++n) h[L::G + (size_t)j * L::LD + n] = t.G[((size_t)q * D + d) * L::Np + n];
-        h[L::VW + j] = t.vw[q];
-    }
+    for (int d = 0; d < D; ++d)
+        for (int n = 0; n < L::Np; ++n)
+            for (int m = 0; m < L::Np; ++m)
+                h[L::KV + ((size_t)d * L::NPK + n) * L::LDK + m] = t.Kv[((size_t)d * L::Np + n) * L::Np + m];
     for (int f = 0; f < 2 * D; ++f)
         for (int q = 0; q < L::NQF; ++q)
             for (int n = 0; n < L::Np; ++n) {
@@ -126,28 +126,17 @@ __global__ __launch_bounds__(64 * W) void sip_apply_mfma64_kernel(M64Args a) {
         d4_t yacc[L::MT];
 #pragma unroll
         for (int mt = 0; mt < L::MT; ++mt) yacc[mt] = z;
-        // ---------------- volume
-        {
-            double s2[3] = {0, 0, 0};
-            for (int d = 0; d < D; ++d) s2[d] = -nuP * 4.0 / (h[d] * h[d]);
-#pragma unroll 2
-            for (int t = 0; t < L::NVT; ++t) {
-                d4_t g = z;
+        // ---------------- volume: Y += sum_d diag(-nu 4/h_d^2) U Kv[d]  (A = scaled modes, B = Kv[d])
+#pragma unroll 1
+        for (int d = 0; d < D; ++d) {
+            const double hd = d == 0 ? h[0] : (d == 1 ? h[1] : h[2]);
+            const double cd = -nuP * 4.0 / (hd * hd);
 #pragma unroll
-                for (int s = 0; s < NK; ++s) g = BH_MFMA64(tab[L::G + (t * 16 + col) * LD + 4 * s + grp], bU[s], g);
+            for (int s = 0; s < NK; ++s) {
+                const double av = cd * bU[s];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = t * 16 + grp + 4 * r;
-                    const int dj = j / L::NQV;
-                    const double fac = (dj == 0 ? s2[0] : (dj == 1 ? s2[1] : s2[2])) * tab[L::VW + j];
-                    const double F = j < D * L::NQV ? fac * g[r] : 0.0;
-#pragma unroll
-                    for (int mt = 0; mt < L::MT; ++mt) {
-                        const int m = col + 16 * mt;
-                        const double Bv = m < Np ? tab[L::G + j * LD + m] : 0.0;
-                        yacc[mt] = BH_MFMA64(F, Bv, yacc[mt]);
-                    }
-                }
+                for (int mt = 0; mt < L::MT; ++mt)
+                    yacc[mt] = BH_MFMA64(av, tab[L::KV + (d * L::NPK + 4 * s + grp) * L::LDK + col + 16 * mt], yacc[mt]);
             }
         }
         // ---------------- faces
