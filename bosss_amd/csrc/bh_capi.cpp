@@ -114,6 +114,55 @@ void bh_op::rebuild_affine() {
     affine_valid = true;
 }
 
+namespace bh {
+// SIP operator of degree k on grid g (penalty_deg <= 0: the degree's own
+// penalty factor, ipLaplace.cs:100-110; a coarse p-level of a multigrid passes
+// the finest level's factor so that its operator is the Galerkin submatrix)
+std::unique_ptr<bh_op> make_sip_op(bh_grid* g, int k, double penalty_base, double penalty_deg, const double* nu,
+                                   const int* bc) {
+    BH_CHECK(k >= 1 && k <= (g->D == 2 ? 4 : 3), "bh_sip_create: supported degrees are 1..4 (2D), 1..3 (3D)");
+    auto op = std::make_unique<bh_op>();
+    op->ctx = g->ctx;
+    op->grid = g;
+    op->D = g->D;
+    op->k = k;
+    op->tab.build(g->D, k, k + 1);   // Gauss rule of order 2k (precision 2k+1), F/RefElement.cs:855-884
+    op->Np = op->tab.Np;
+    op->N = (int64_t)g->J * op->Np;
+    op->penalty_base = penalty_base;
+    const double D = g->D, p = k;
+    op->penalty_deg = penalty_deg > 0 ? penalty_deg : std::max((p + 1) * (p + D) / D, (p + 1.0) * (p + 1.0));
+    for (int f = 0; f < 2 * g->D; ++f) op->bc[f] = bc ? bc[f] : 0;
+    op->h_nu.assign(nu, nu + g->J);
+    for (double v : op->h_nu) BH_CHECK(std::isfinite(v), "Inf/NaN in diffusion coefficient");
+    bh_ctx* ctx = g->ctx;
+    op->nu.alloc(g->J * sizeof(double));
+    op->nu32.alloc(g->J * sizeof(float));
+    ctx->upload(op->nu.p, nu, g->J * sizeof(double));
+    convert_f64_f32(ctx, op->nu.as<double>(), op->nu32.as<float>(), g->J);
+    // tables
+    std::vector<double> t;
+    op->off_G = t.size(); t.insert(t.end(), op->tab.G.begin(), op->tab.G.end());
+    op->off_T = t.size(); t.insert(t.end(), op->tab.T.begin(), op->tab.T.end());
+    op->off_N = t.size(); t.insert(t.end(), op->tab.Nn.begin(), op->tab.Nn.end());
+    op->off_vw = t.size(); t.insert(t.end(), op->tab.vw.begin(), op->tab.vw.end());
+    op->off_fw = t.size(); t.insert(t.end(), op->tab.fw.begin(), op->tab.fw.end());
+    op->off_K = t.size(); t.insert(t.end(), op->tab.Kv.begin(), op->tab.Kv.end());
+    op->tab64.alloc(t.size() * sizeof(double));
+    ctx->upload(op->tab64.p, t.data(), t.size() * sizeof(double));
+    std::vector<float> t32(t.begin(), t.end());
+    op->tab32.alloc(t32.size() * sizeof(float));
+    ctx->upload(op->tab32.p, t32.data(), t32.size() * sizeof(float));
+    // boundary slots
+    op->bnd_slot.assign(g->e_in.size(), -1);
+    int slot = 0;
+    for (size_t e = 0; e < g->e_in.size(); ++e)
+        if (g->e_out[e] < 0) { op->bnd_slot[e] = slot; slot += op->tab.nqf; }
+    op->gbnd.assign(slot, 0.0);
+    return op;
+}
+}  // namespace bh
+
 extern "C" {
 
 int bh_last_error(char* buf, int len) {
@@ -267,46 +316,7 @@ int bh_grid_cj(bh_grid* g, double* cj) {
 // ------------------------------------------------------------------ SIP
 int bh_sip_create(bh_grid* g, int k, double penalty_base, const double* nu, const int* bc, bh_op** out) {
     BH_API_BEGIN
-    BH_CHECK(k >= 1 && k <= (g->D == 2 ? 4 : 3), "bh_sip_create: supported degrees are 1..4 (2D), 1..3 (3D)");
-    auto op = std::make_unique<bh_op>();
-    op->ctx = g->ctx;
-    op->grid = g;
-    op->D = g->D;
-    op->k = k;
-    op->tab.build(g->D, k, k + 1);   // Gauss rule of order 2k (precision 2k+1), F/RefElement.cs:855-884
-    op->Np = op->tab.Np;
-    op->N = (int64_t)g->J * op->Np;
-    op->penalty_base = penalty_base;
-    const double D = g->D, p = k;
-    op->penalty_deg = std::max((p + 1) * (p + D) / D, (p + 1.0) * (p + 1.0));  // ipLaplace.cs:100-110
-    for (int f = 0; f < 2 * g->D; ++f) op->bc[f] = bc ? bc[f] : 0;
-    op->h_nu.assign(nu, nu + g->J);
-    for (double v : op->h_nu) BH_CHECK(std::isfinite(v), "Inf/NaN in diffusion coefficient");
-    bh_ctx* ctx = g->ctx;
-    op->nu.alloc(g->J * sizeof(double));
-    op->nu32.alloc(g->J * sizeof(float));
-    ctx->upload(op->nu.p, nu, g->J * sizeof(double));
-    convert_f64_f32(ctx, op->nu.as<double>(), op->nu32.as<float>(), g->J);
-    // tables
-    std::vector<double> t;
-    op->off_G = t.size(); t.insert(t.end(), op->tab.G.begin(), op->tab.G.end());
-    op->off_T = t.size(); t.insert(t.end(), op->tab.T.begin(), op->tab.T.end());
-    op->off_N = t.size(); t.insert(t.end(), op->tab.Nn.begin(), op->tab.Nn.end());
-    op->off_vw = t.size(); t.insert(t.end(), op->tab.vw.begin(), op->tab.vw.end());
-    op->off_fw = t.size(); t.insert(t.end(), op->tab.fw.begin(), op->tab.fw.end());
-    op->off_K = t.size(); t.insert(t.end(), op->tab.Kv.begin(), op->tab.Kv.end());
-    op->tab64.alloc(t.size() * sizeof(double));
-    ctx->upload(op->tab64.p, t.data(), t.size() * sizeof(double));
-    std::vector<float> t32(t.begin(), t.end());
-    op->tab32.alloc(t32.size() * sizeof(float));
-    ctx->upload(op->tab32.p, t32.data(), t32.size() * sizeof(float));
-    // boundary slots
-    op->bnd_slot.assign(g->e_in.size(), -1);
-    int slot = 0;
-    for (size_t e = 0; e < g->e_in.size(); ++e)
-        if (g->e_out[e] < 0) { op->bnd_slot[e] = slot; slot += op->tab.nqf; }
-    op->gbnd.assign(slot, 0.0);
-    *out = op.release();
+    *out = make_sip_op(g, k, penalty_base, 0.0, nu, bc).release();
     BH_API_END
 }
 
@@ -724,8 +734,7 @@ int bh_pmg_learned_stats(bh_solver* s, int* acc, int* rej) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
     BH_CHECK(mg, "bh_pmg_learned_stats: solver is not a multigrid preconditioner");
-    if (acc) *acc = mg->learned_accepted;
-    if (rej) *rej = mg->learned_rejected;
+    mg->learned_stats(acc, rej);
     BH_API_END
 }
 

@@ -170,6 +170,11 @@ struct bh_op {
     void rebuild_affine();
 };
 
+namespace bh {
+std::unique_ptr<bh_op> make_sip_op(bh_grid* g, int k, double penalty_base, double penalty_deg, const double* nu,
+                                   const int* bc);
+}
+
 // ------------------------------------------------------------------ solvers
 struct bh_solver {
     virtual ~bh_solver() = default;
@@ -213,6 +218,11 @@ double nrm2(bh_ctx* ctx, const double* a, int64_t n);
 template <typename T> void multi_dot_t(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* h);
 template <typename T> double nrm2_t(bh_ctx* ctx, const T* a, int64_t n);
 template <typename T> void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
+// device-resident coefficients (graph-capturable): dout[k] = <w, V_k> (w null: |V_k|^2);
+// w += sign * (dscale ? dscale[0] : 1) * sum_k dcoef[k] V_k
+template <typename T> void multi_dot_to_dev(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* dout);
+template <typename T> void multi_axpy_from_dev(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* dcoef,
+                                               const double* dscale, double sign);
 // w -= sum_k h[k] V_k  (h host array, m <= 128)
 void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
 }  // namespace bh

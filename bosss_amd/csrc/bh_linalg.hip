@@ -438,12 +438,53 @@ void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int
     multi_axpy_t<double>(ctx, w, V, ldv, m, n, h, sign);
 }
 
+// ------------------------------------------------ device-resident variants
+// The coefficients stay in device memory: no host round trip, so a sequence
+// of them can be captured into a HIP graph.
+
+// dout[k] = <w, V_k> (w == nullptr: |V_k|^2), k < m
+template <typename T>
+void multi_dot_to_dev(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* dout) {
+    BH_CHECK(m <= 256, "multi_dot: too many vectors");
+    for (int k0 = 0; k0 < m; k0 += MD_M)
+        multi_dot_dev<T>(ctx, w, V + (size_t)k0 * ldv, ldv, std::min(MD_M, m - k0), n, dout + k0);
+}
+
+// w <- w + sign * scale * sum_k c[k] V_k with c (and scale, if given) read from device memory
+template <typename T>
+__global__ __launch_bounds__(256) void multi_axpy_dc_k(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int m,
+                                                       int64_t n, const double* __restrict__ c, const double* __restrict__ scale,
+                                                       double sign) {
+    __shared__ double cs[32];
+    const double f = sign * (scale ? scale[0] : 1.0);
+    if (threadIdx.x < m) cs[threadIdx.x] = f * c[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = (double)w[i];
+        for (int k = 0; k < m; ++k) s += cs[k] * (double)V[(size_t)k * ldv + i];
+        w[i] = (T)s;
+    }
+}
+
+template <typename T>
+void multi_axpy_from_dev(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* dcoef,
+                         const double* dscale, double sign) {
+    for (int k0 = 0; k0 < m; k0 += 32)
+        hipLaunchKernelGGL((multi_axpy_dc_k<T>), grid_for(n), dim3(256), 0, ctx->stream, w, V + (size_t)k0 * ldv, ldv,
+                           std::min(32, m - k0), n, dcoef + k0, dscale, sign);
+    BH_HIP(hipGetLastError());
+}
+
 template void multi_dot_t<float>(bh_ctx*, const float*, const float*, int64_t, int, int64_t, double*);
 template void multi_dot_t<double>(bh_ctx*, const double*, const double*, int64_t, int, int64_t, double*);
 template double nrm2_t<float>(bh_ctx*, const float*, int64_t);
 template double nrm2_t<double>(bh_ctx*, const double*, int64_t);
 template void multi_axpy_t<float>(bh_ctx*, float*, const float*, int64_t, int, int64_t, const double*, double);
 template void multi_axpy_t<double>(bh_ctx*, double*, const double*, int64_t, int, int64_t, const double*, double);
+template void multi_dot_to_dev<float>(bh_ctx*, const float*, const float*, int64_t, int, int64_t, double*);
+template void multi_dot_to_dev<double>(bh_ctx*, const double*, const double*, int64_t, int, int64_t, double*);
+template void multi_axpy_from_dev<float>(bh_ctx*, float*, const float*, int64_t, int, int64_t, const double*, const double*, double);
+template void multi_axpy_from_dev<double>(bh_ctx*, double*, const double*, int64_t, int, int64_t, const double*, const double*, double);
 
 }  // namespace bh
 

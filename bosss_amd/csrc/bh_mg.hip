@@ -135,7 +135,92 @@ __global__ void copy_k(const T* __restrict__ s, T* __restrict__ d, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
+// ---------------------------------------------------------------- K-cycle scalars
+// The inner flexible GMRES of the K-cycle keeps its Hessenberg matrix, Gram-
+// Schmidt coefficients and least-squares solution in device memory (layout
+// KcycleScalars), so the whole preconditioner needs no host round trip.
+
+// dst = src / sqrt(*s2); zero when sqrt(*s2) <= tiny (or not finite): a zero
+// Krylov direction then yields zero columns, which the least squares skip
+template <typename T>
+__global__ __launch_bounds__(256) void scale_inv_norm_k(const T* __restrict__ src, T* __restrict__ dst, int64_t n,
+                                                        const double* __restrict__ s2, double tiny) {
+    const double nv = sqrt(fmax(s2[0], 0.0));
+    const double f = (nv > tiny && nv < 1e300) ? 1.0 / nv : 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (T)(f * (double)src[i]);
+}
+
+// H[i][j] = h1[i] + h2[i] (i <= j), H[j+1][j] = |w|
+__global__ void kcycle_column_k(double* __restrict__ S, KcycleScalars L, int j) {
+    const int i = threadIdx.x;
+    if (i <= j) S[L.H + i * L.kin + j] = S[L.H1 + i] + S[L.H2 + i];
+    if (i == 0) S[L.H + (j + 1) * L.kin + j] = sqrt(fmax(S[L.W2], 0.0));
+}
+
+// min || beta e1 - H y || by Givens rotations (one thread: kin <= 16); a zero
+// diagonal after the rotations (a direction that broke down) gets y = 0
+__global__ void kcycle_lsq_k(double* __restrict__ S, KcycleScalars L) {
+    if (threadIdx.x != 0) return;
+    const int kin = L.kin;
+    double R[17 * 16], g[17];
+    for (int t = 0; t < (kin + 1) * kin; ++t) R[t] = S[L.H + t];
+    for (int t = 0; t <= kin; ++t) g[t] = 0.0;
+    g[0] = sqrt(fmax(S[L.B2], 0.0));
+    for (int j = 0; j < kin; ++j) {
+        const double a = R[j * kin + j], b = R[(j + 1) * kin + j];
+        const double rr = hypot(a, b);
+        const double c = rr > 0 ? a / rr : 1.0, s = rr > 0 ? b / rr : 0.0;
+        for (int k = j; k < kin; ++k) {
+            const double t1 = R[j * kin + k], t2 = R[(j + 1) * kin + k];
+            R[j * kin + k] = c * t1 + s * t2;
+            R[(j + 1) * kin + k] = -s * t1 + c * t2;
+        }
+        const double t1 = g[j], t2 = g[j + 1];
+        g[j] = c * t1 + s * t2;
+        g[j + 1] = -s * t1 + c * t2;
+    }
+    for (int r = kin - 1; r >= 0; --r) {
+        double a = g[r];
+        for (int c = r + 1; c < kin; ++c) a -= R[r * kin + c] * S[L.Y + c];
+        const double d = R[r * kin + r];
+        S[L.Y + r] = d != 0.0 ? a / d : 0.0;
+    }
+}
+
+// learned step: alpha = <q, r'> / <q, q> from st[0..1]; x += alpha e only when
+// alpha > 0 and finite; counters cnt[0] (taken) / cnt[1] (declined)
+template <typename T>
+__global__ __launch_bounds__(256) void guarded_step_k(const T* __restrict__ e, T* __restrict__ x, int64_t n,
+                                                      const double* __restrict__ st, int* __restrict__ cnt) {
+    const double qq = st[1];
+    const double alpha = qq > 0 ? st[0] / qq : 0.0;
+    const bool take = alpha > 0 && alpha < 1e300;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[take ? 0 : 1], 1);
+    if (!take) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (T)((double)x[i] + alpha * (double)e[i]);
+}
+
 // ------------------------------------------------------------------ launchers
+template <typename T>
+void scale_inv_norm(bh_ctx* ctx, const T* src, T* dst, int64_t n, const double* s2, double tiny) {
+    hipLaunchKernelGGL((scale_inv_norm_k<T>), grid1d(n), dim3(256), 0, ctx->stream, src, dst, n, s2, tiny);
+    BH_HIP(hipGetLastError());
+}
+void kcycle_column(bh_ctx* ctx, double* S, const KcycleScalars& L, int j) {
+    hipLaunchKernelGGL(kcycle_column_k, dim3(1), dim3(64), 0, ctx->stream, S, L, j);
+    BH_HIP(hipGetLastError());
+}
+void kcycle_lsq(bh_ctx* ctx, double* S, const KcycleScalars& L) {
+    hipLaunchKernelGGL(kcycle_lsq_k, dim3(1), dim3(64), 0, ctx->stream, S, L);
+    BH_HIP(hipGetLastError());
+}
+template <typename T>
+void guarded_step(bh_ctx* ctx, const T* e, T* x, int64_t n, const double* st, int* cnt) {
+    hipLaunchKernelGGL((guarded_step_k<T>), grid1d(n), dim3(256), 0, ctx->stream, e, x, n, st, cnt);
+    BH_HIP(hipGetLastError());
+}
 void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc) {
     hipLaunchKernelGGL(extract_subblocks_k, grid1d(nnzb * bsc * bsc), dim3(256), 0, ctx->stream, nnzb, bsf, bsc, vf, vc);
     BH_HIP(hipGetLastError());
@@ -192,7 +277,9 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void agg_prolong_add<T>(bh_ctx*, int, int, const int*, const T*, const T*, T*);         \
     template void block_gemv<T>(bh_ctx*, int, int, const T*, const T*, T*, double, bool);             \
     template void dense_gemv<T>(bh_ctx*, int, const double*, const T*, T*);                               \
-    template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);
+    template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);                                          \
+    template void scale_inv_norm<T>(bh_ctx*, const T*, T*, int64_t, const double*, double);           \
+    template void guarded_step<T>(bh_ctx*, const T*, T*, int64_t, const double*, int*);
 BH_INST(float)
 BH_INST(double)
 #undef BH_INST

@@ -19,6 +19,22 @@ template <typename T> void dense_gemv(bh_ctx*, int n, const double* A, const T* 
 template <typename T> void vcopy(bh_ctx*, const T* s, T* d, int64_t n);
 void bsr_residual(bh_bsr* A, const void* b, const void* x, void* r, int prec);
 
+// device-resident scalars of the K-cycle's inner flexible GMRES (offsets in doubles)
+struct KcycleScalars {
+    int kin = 0;
+    int B2 = 0, W2 = 1, H = 8, H1 = 0, H2 = 0, Y = 0, total = 0;
+    explicit KcycleScalars(int k = 0) : kin(k) {
+        H1 = H + (kin + 1) * kin;
+        H2 = H1 + kin + 1;
+        Y = H2 + kin + 1;
+        total = Y + kin;
+    }
+};
+template <typename T> void scale_inv_norm(bh_ctx*, const T* src, T* dst, int64_t n, const double* s2, double tiny);
+void kcycle_column(bh_ctx*, double* S, const KcycleScalars& L, int j);
+void kcycle_lsq(bh_ctx*, double* S, const KcycleScalars& L);
+template <typename T> void guarded_step(bh_ctx*, const T* e, T* x, int64_t n, const double* st, int* cnt);
+
 struct MgLevel {
     int J = 0, bs = 1;
     int64_t N = 0;
@@ -26,7 +42,8 @@ struct MgLevel {
     int deg = 0;                   // DG degree of the level
     bh_bsr* A = nullptr;           // level operator (Galerkin)
     bool ownA = false;
-    bh_op* op = nullptr;           // matrix-free operator (finest level only)
+    bh_op* op = nullptr;           // matrix-free operator (finest level; coarse p-levels of a matrix-free hierarchy)
+    std::unique_ptr<bh_op> ownop;  // coarse p-level operator owned by the hierarchy
     DevBuf dinv64, dinv32;
     // transfer to the next coarser level: 0 = degree truncation, 1 = aggregation
     int kind = 0;
@@ -35,7 +52,7 @@ struct MgLevel {
     // work vectors (precision of the cycle)
     DevBuf b, x, r;
     // K-cycle storage (inner flexible Krylov on this level): rhs, V, Z, w
-    DevBuf kb, kv, kz, kw;
+    DevBuf kb, kv, kz, kw, ks;     // ks: KcycleScalars in device memory
 };
 
 struct PMultigrid : bh_solver {
@@ -55,7 +72,8 @@ struct PMultigrid : bh_solver {
     struct Mlp* learned = nullptr;
     bh_op* fine_op = nullptr;
     DevBuf lX, ls, lY, le, lrq;     // features, scales, outputs, correction, [r' | A e]
-    int learned_accepted = 0, learned_rejected = 0;
+    DevBuf lstat;                   // <q, r'>, <q, q> (doubles), then int counters taken / declined
+    void learned_stats(int* acc, int* rej);
     template <typename T> void learned_correction();
     void attach_learned(struct Mlp* m);
     ~PMultigrid() override;
@@ -69,6 +87,8 @@ struct PMultigrid : bh_solver {
     // K-cycle, or the inner cycle of the K-cycle) are captured once into a HIP
     // graph and replayed: one launch instead of one per kernel of every level
     bool use_graphs = true;
+    bool capturing = false;        // inside a capture: nested cycles are recorded, not captured again
+    template <typename T> void cycle_body(int l);   // cycle from level l (+ learned step on level 0)
     hipGraphExec_t gexec[2][16] = {};
     int gcalls[2][16] = {};
     template <typename T> void vcycle_g(int l);
@@ -76,6 +96,8 @@ struct PMultigrid : bh_solver {
     template <typename T> void kcycle(int l);
     template <typename T> void solve_level(int l) { if (l == kcycle_level && kin > 0) kcycle<T>(l); else vcycle<T>(l); }
     template <typename T> void residual(int l, const T* b, const T* x, T* r);
+    template <typename T> void apply(int l, const T* x, T* y);   // y = A_l x
+    int op_prec(int l, bool f32) const { return f32 ? ((l == 0 && mfma) ? BH_FP16_MFMA : BH_FP32) : BH_FP64; }
     template <typename T> T* vec(DevBuf& d) { return (T*)d.p; }
 };
 

@@ -43,22 +43,31 @@ void PMultigrid::drop_graphs() {
 // captures the cycle into a graph; later calls replay it.  The cycle's buffers
 // are fixed per level, so the captured kernel arguments stay valid.
 template <typename T>
+void PMultigrid::cycle_body(int l) {
+    vcycle<T>(l);
+    if (l == 0 && learned) learned_correction<T>();
+}
+
+template <typename T>
 void PMultigrid::vcycle_g(int l) {
     const int ti = sizeof(T) == 4 ? 1 : 0;
-    if (!use_graphs || l >= 16 || gcalls[ti][l]++ == 0) {
-        vcycle<T>(l);
+    if (!use_graphs || capturing || l >= 16 || gcalls[ti][l]++ == 0) {
+        cycle_body<T>(l);
         return;
     }
     if (!gexec[ti][l]) {
         hipGraph_t g = nullptr;
         BH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        capturing = true;
         try {
-            vcycle<T>(l);
+            cycle_body<T>(l);
         } catch (...) {
+            capturing = false;
             (void)hipStreamEndCapture(ctx->stream, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
+        capturing = false;
         BH_HIP(hipStreamEndCapture(ctx->stream, &g));
         const hipError_t e = hipGraphInstantiate(&gexec[ti][l], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -176,6 +185,14 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
             L.A = make_bsr(ctx, J, L.bs, Af->h_rowptr, Af->h_col);
             L.ownA = true;
             extract_subblocks(ctx, Af->nnzb, Af->bs, L.bs, Af->vals.as<double>(), L.A->vals.as<double>());
+            if (mfop && degrees[i] >= 1) {
+                // the truncated-basis Galerkin operator of a SIP matrix is the SIP operator of the
+                // lower degree with the finest level's penalty factor (orthonormal hierarchical
+                // basis; both quadratures are exact for it): apply it matrix-free as well
+                L.ownop = make_sip_op(mfop->grid, degrees[i], mfop->penalty_base, mfop->penalty_deg,
+                                      mfop->h_nu.data(), mfop->bc);
+                L.op = L.ownop.get();
+            }
         }
         L.kind = 0;
         lv.push_back(std::move(L));
@@ -317,7 +334,7 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
     for (size_t l = 0; l + 1 < lv.size(); ++l) {
         MgLevel& L = lv[l];
         bsr_invert_diag(L.A, L.dinv64, f32 ? &L.dinv32 : nullptr);
-        if (f32) { L.dinv64.release(); if (!(l == 0 && L.op)) L.A->ensure_fp32(); }
+        if (f32) { L.dinv64.release(); if (!L.op) L.A->ensure_fp32(); }
     }
     for (auto& L : lv) {
         L.b.alloc(L.N * es);
@@ -331,6 +348,9 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         K.kv.alloc((size_t)(kin + 1) * K.N * es);
         K.kz.alloc((size_t)kin * K.N * es);
         K.kw.alloc(K.N * es);
+        BH_CHECK(kin <= 16, "multigrid: at most 16 inner K-cycle steps");
+        K.ks.alloc(KcycleScalars(kin).total * sizeof(double));
+        fill(ctx, K.ks.as<double>(), 0.0, KcycleScalars(kin).total);
     }
     // coarsest: dense inverse
     MgLevel& Cc = lv.back();
@@ -361,10 +381,18 @@ void PMultigrid::residual(int l, const T* b, const T* x, T* r) {
     const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
     if (L.op) {
         vcopy<T>(ctx, b, r, L.N);
-        sip_apply(L.op, -1.0, x, 1.0, r, false, (p == BH_FP32 && mfma) ? BH_FP16_MFMA : p);
+        sip_apply(L.op, -1.0, x, 1.0, r, false, op_prec(l, p == BH_FP32));
     } else {
         bsr_residual(L.A, b, x, r, p);
     }
+}
+
+template <typename T>
+void PMultigrid::apply(int l, const T* x, T* y) {
+    MgLevel& L = lv[l];
+    const bool f32 = sizeof(T) == 4;
+    if (L.op) sip_apply(L.op, 1.0, x, 0.0, y, false, op_prec(l, f32));
+    else bsr_spmv(L.A, 1.0, x, 0.0, y, f32 ? BH_FP32 : BH_FP64);
 }
 
 template <typename T>
@@ -410,69 +438,44 @@ static void axpby_t(bh_ctx* ctx, double a, const T* x, double b, T* y, int64_t n
 // accelerated coarse solve of the reference's K-cycle / OrthonormalizationMultigrid
 // configurations (AS/OrthonormalizationMultigrid.cs:797-1100, AS/PmgConfig.cs:80-200),
 // standing in for the direct low-order solve of the p-two-grid (AS/LevelPmg.cs:411).
+// Classical Gram-Schmidt applied twice; every scalar (norms, Hessenberg entries,
+// Givens least squares, the combination weights) stays in device memory, so the
+// K-cycle has no host synchronisation and is captured with the rest of the cycle.
 template <typename T>
 void PMultigrid::kcycle(int l) {
     MgLevel& L = lv[l];
     const int64_t n = L.N;
-    const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
     T* kb = vec<T>(L.kb);
     T* V = vec<T>(L.kv);
     T* Z = vec<T>(L.kz);
     T* w = vec<T>(L.kw);
+    double* S = L.ks.as<double>();
+    const KcycleScalars K(kin);
     vcopy<T>(ctx, vec<T>(L.b), kb, n);
-    const double beta = nrm2_t<T>(ctx, kb, n);
-    if (!(beta > 0) || !std::isfinite(beta)) {
-        axpby_t<T>(ctx, 0.0, kb, 0.0, vec<T>(L.x), n);
-        return;
-    }
-    axpby_t<T>(ctx, 1.0 / beta, kb, 0.0, V, n);
-    std::vector<double> H((size_t)(kin + 1) * kin, 0.0), h1(kin + 1), h2(kin + 1);
-    int used = 0;
+    multi_dot_to_dev<T>(ctx, nullptr, kb, 0, 1, n, S + K.B2);
+    scale_inv_norm<T>(ctx, kb, V, n, S + K.B2, 0.0);
     for (int j = 0; j < kin; ++j) {
         vcopy<T>(ctx, V + (size_t)j * n, vec<T>(L.b), n);
         vcycle_g<T>(l);
         vcopy<T>(ctx, vec<T>(L.x), Z + (size_t)j * n, n);
-        bsr_spmv(L.A, 1.0, Z + (size_t)j * n, 0.0, w, p);
-        multi_dot_t<T>(ctx, w, V, n, j + 1, n, h1.data());
-        multi_axpy_t<T>(ctx, w, V, n, j + 1, n, h1.data(), -1.0);
-        multi_dot_t<T>(ctx, w, V, n, j + 1, n, h2.data());
-        multi_axpy_t<T>(ctx, w, V, n, j + 1, n, h2.data(), -1.0);
-        for (int i = 0; i <= j; ++i) H[(size_t)i * kin + j] = h1[i] + h2[i];
-        const double nw = nrm2_t<T>(ctx, w, n);
-        H[(size_t)(j + 1) * kin + j] = nw;
-        used = j + 1;
-        if (!(nw > 1e-300)) break;
-        if (j + 1 < kin) axpby_t<T>(ctx, 1.0 / nw, w, 0.0, V + (size_t)(j + 1) * n, n);
+        apply<T>(l, Z + (size_t)j * n, w);
+        multi_dot_to_dev<T>(ctx, w, V, n, j + 1, n, S + K.H1);
+        multi_axpy_from_dev<T>(ctx, w, V, n, j + 1, n, S + K.H1, nullptr, -1.0);
+        multi_dot_to_dev<T>(ctx, w, V, n, j + 1, n, S + K.H2);
+        multi_axpy_from_dev<T>(ctx, w, V, n, j + 1, n, S + K.H2, nullptr, -1.0);
+        multi_dot_to_dev<T>(ctx, nullptr, w, 0, 1, n, S + K.W2);
+        kcycle_column(ctx, S, K, j);
+        if (j + 1 < kin) scale_inv_norm<T>(ctx, w, V + (size_t)(j + 1) * n, n, S + K.W2, 1e-300);
     }
-    // least squares min || beta e1 - H y || by Givens rotations (host, tiny)
-    std::vector<double> g(used + 1, 0.0), y(used, 0.0);
-    g[0] = beta;
-    std::vector<double> R(H);
-    for (int j = 0; j < used; ++j) {
-        const double a = R[(size_t)j * kin + j], b = R[(size_t)(j + 1) * kin + j];
-        const double rr = std::hypot(a, b);
-        const double c = rr > 0 ? a / rr : 1.0, s = rr > 0 ? b / rr : 0.0;
-        for (int k = j; k < used; ++k) {
-            const double t1 = R[(size_t)j * kin + k], t2 = R[(size_t)(j + 1) * kin + k];
-            R[(size_t)j * kin + k] = c * t1 + s * t2;
-            R[(size_t)(j + 1) * kin + k] = -s * t1 + c * t2;
-        }
-        const double t1 = g[j], t2 = g[j + 1];
-        g[j] = c * t1 + s * t2;
-        g[j + 1] = -s * t1 + c * t2;
-    }
-    for (int r = used - 1; r >= 0; --r) {
-        double a = g[r];
-        for (int c = r + 1; c < used; ++c) a -= R[(size_t)r * kin + c] * y[c];
-        y[r] = a / R[(size_t)r * kin + r];
-    }
-    axpby_t<T>(ctx, 0.0, kb, 0.0, vec<T>(L.x), n);
-    multi_axpy_t<T>(ctx, vec<T>(L.x), Z, n, used, n, y.data(), 1.0);
+    kcycle_lsq(ctx, S, K);
+    if (sizeof(T) == 4) fill32(ctx, (float*)vec<T>(L.x), 0.f, n);
+    else fill(ctx, (double*)vec<T>(L.x), 0.0, n);
+    multi_axpy_from_dev<T>(ctx, vec<T>(L.x), Z, n, kin, n, S + K.Y, nullptr, 1.0);
 }
 
 void PMultigrid::attach_learned(Mlp* m) {
     learned = m;
-    learned_accepted = learned_rejected = 0;
+    drop_graphs();   // the captured cycle includes (or not) the learned step
     if (!m) return;
     BH_CHECK(fine_op, "learned correction needs the fine-level operator");
     BH_CHECK(m->out == fine_op->Np && m->in == fine_op->Np + 2 * fine_op->D, "learned correction: network widths do not match the operator");
@@ -483,6 +486,8 @@ void PMultigrid::attach_learned(Mlp* m) {
     lY.alloc((size_t)J * m->out_p * sizeof(float));
     le.alloc(N * es);
     lrq.alloc(2 * N * es);
+    lstat.alloc(4 * sizeof(double));
+    fill(ctx, lstat.as<double>(), 0.0, 4);   // also zeroes the two int counters in the third double
 }
 
 // After the cycle (x = x_mg in lv[0].x): r' = b - A x_mg, e = network(r'),
@@ -502,34 +507,32 @@ void PMultigrid::learned_correction() {
     learned->infer(fine_op->grid->J, lX.as<float>(), lY.as<float>());
     mlp_correction(fine_op, ls.as<float>(), lY.as<float>(), learned->out_p, e, p);
     sip_apply(fine_op, 1.0, e, 0.0, q, false, p);
-    double h[2];
-    multi_dot_t<T>(ctx, q, rq, N, 2, N, h);   // <q, r'>, <q, q>
-    const double alpha = h[1] > 0 ? h[0] / h[1] : 0.0;
-    if (alpha > 0 && std::isfinite(alpha)) {
-        axpby_t<T>(ctx, alpha, e, 1.0, vec<T>(L.x), N);
-        learned_accepted++;
-    } else {
-        learned_rejected++;
-    }
+    double* st = lstat.as<double>();
+    multi_dot_to_dev<T>(ctx, q, rq, N, 2, N, st);   // <q, r'>, <q, q>
+    guarded_step<T>(ctx, e, vec<T>(L.x), N, st, (int*)(st + 2));
 }
 
+// The cycle from the finest level, the K-cycle inside it and the guarded learned
+// step contain no host synchronisation: one graph per preconditioner call.
 void PMultigrid::precondition(double* x, const double* b) {
     MgLevel& L = lv[0];
-    const bool sync_free = kcycle_level < 0 || kin == 0;   // no K-cycle: the whole cycle is one graph
     if (prec == BH_FP32) {
         convert_f64_f32(ctx, b, vec<float>(L.b), N);
-        if (sync_free) vcycle_g<float>(0);
-        else vcycle<float>(0);
-        if (learned) learned_correction<float>();
+        vcycle_g<float>(0);
         convert_f32_f64(ctx, vec<float>(L.x), x, N);
     } else {
         copy(ctx, vec<double>(L.b), b, N);
-        if (sync_free) vcycle_g<double>(0);
-        else vcycle<double>(0);
-        if (learned) learned_correction<double>();
+        vcycle_g<double>(0);
         copy(ctx, x, vec<double>(L.x), N);
     }
     iterations++;
+}
+
+void PMultigrid::learned_stats(int* acc, int* rej) {
+    int c[2] = {0, 0};
+    if (learned && lstat.p) ctx->download(c, lstat.as<double>() + 2, sizeof(c));
+    if (acc) *acc = c[0];
+    if (rej) *rej = c[1];
 }
 
 void PMultigrid::solve_dev(double* x, const double* b) {
@@ -550,6 +553,8 @@ template void PMultigrid::vcycle<float>(int);
 template void PMultigrid::vcycle<double>(int);
 template void PMultigrid::vcycle_g<float>(int);
 template void PMultigrid::vcycle_g<double>(int);
+template void PMultigrid::cycle_body<float>(int);
+template void PMultigrid::cycle_body<double>(int);
 template void PMultigrid::kcycle<float>(int);
 template void PMultigrid::kcycle<double>(int);
 
