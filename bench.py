@@ -77,8 +77,10 @@ def sine_rhs_coeffs(n, k, seed, nmodes=16, kmax=6):
 
 
 def solve_leg(B, ctx, grid, op, n, k, out):
-    """time-to-solve: FP64 FGMRES to 1e-8 true relative residual with the
-    p/h-multigrid K-cycle preconditioner in FP64 (plain) vs FP32 (mixed)."""
+    """time-to-solve: FP64 FGMRES to 1e-8 relative residual with the p/h-multigrid
+    K-cycle preconditioner: the FP64 drop-in on the assembled matrix (plain), the
+    FP64 matrix-free variant, and the mixed-precision variants (FP32 multigrid;
+    FP16-MFMA level 0; with the learned correction)."""
     N = op.N
     f = sine_rhs_coeffs(n, k, seed=16)
     b = B.DeviceArray.from_host(ctx, f)
@@ -108,19 +110,43 @@ def solve_leg(B, ctx, grid, op, n, k, out):
         res[name] = {"ms": ms, "algorithmic_bytes": int(nbytes), "GBps": nbytes / (ms * 1e-3) / 1e9,
                      "hbm_frac": nbytes / (ms * 1e-3) / PEAK_HBM}
         del xv, yv
-    for name, prec in (("plain_fp64", B.BH_FP64), ("mixed_fp32", B.BH_FP32), ("mixed_fp16_mfma", B.BH_FP16_MFMA)):
+    # one multigrid configuration for every variant (the fastest one found for both
+    # precisions: single pre/post block-Jacobi sweep, 6-step K-cycle on the k = 1 level)
+    mgp = dict(degrees=[k, 1], min_coarse=64, omega=0.8, pre=1, post=1, kcycle_inner=6)
+    res["multigrid"] = dict(mgp)
+    net = None
+    variants = (
+        # the drop-in: assembled BlockMsrMatrix SpMV for the Krylov operator and every
+        # multigrid residual, FP64 throughout (the reference's method on the GPU)
+        ("plain_fp64", B.BH_FP64, False, False),
+        # same precision, matrix-free operator applies (FP64 MFMA on level 0, k = 1 kernel on level 1)
+        ("fp64_matrix_free", B.BH_FP64, True, False),
+        ("mixed_fp32", B.BH_FP32, True, False),
+        ("mixed_fp16_mfma", B.BH_FP16_MFMA, True, False),
+        ("mixed_fp16_mfma_learned", B.BH_FP16_MFMA, True, True),
+    )
+    for name, prec, mfree, use_net in variants:
+        if use_net and net is None:
+            net, res["learned_training"] = train_network(B, ctx, k)
         t0 = time.perf_counter()
-        mg = B.PMultigrid(op, M=M, degrees=[k, 1], min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2, prec=prec)
+        mg = B.PMultigrid(op, M=M, prec=prec, matrix_free=mfree, **mgp)
         B.pmg_set_graphs(mg, USE_GRAPHS)
+        if use_net:
+            B.pmg_set_learned(mg, net)
         ctx.sync()
         t_init = time.perf_counter() - t0
-        fg = B.FlexGMRES(op=op, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=500)
+        fg = (B.FlexGMRES(op=op, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=500)
+              if mfree else
+              B.FlexGMRES(M=M, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=500))
         x = B.DeviceArray.from_host(ctx, np.zeros(N))
-        ctx.sync()
-        t0 = time.perf_counter()
-        fg.Solve(x, b)
-        ctx.sync()
-        t_run = time.perf_counter() - t0
+        t_run = 1e30
+        for rep in range(2):   # the first solve also captures the graphs and runs the autotuners
+            x.upload(np.zeros(N))
+            ctx.sync()
+            t0 = time.perf_counter()
+            fg.Solve(x, b)
+            ctx.sync()
+            t_run = min(t_run, time.perf_counter() - t0)
         st = fg.stats()
         # independent FP64 check of the true residual
         r = B.DeviceArray(ctx, N)
@@ -129,10 +155,42 @@ def solve_leg(B, ctx, grid, op, n, k, out):
         relres = float(np.linalg.norm(rr) / np.linalg.norm(f))
         res[name] = {"solver_init_s": t_init, "solver_run_s": t_run, "iterations": st["iterations"],
                      "converged": st["converged"], "true_rel_residual": relres}
+        if use_net:
+            acc, rej = B.pmg_learned_stats(mg)
+            active, gain = B.pmg_learned_info(mg)
+            res[name].update({"learned_steps_taken": acc, "learned_steps_declined": rej,
+                              "learned_still_active": active,
+                              "learned_mean_removed_residual_sq_fraction": gain,
+                              "learned_fallback": "probe 8 calls, switch off below 1e-2 mean removed |r|^2 fraction"})
         del fg, mg, x, r
-    res["speedup_mixed_vs_plain"] = res["plain_fp64"]["solver_run_s"] / res["mixed_fp32"]["solver_run_s"]
-    res["speedup_mfma_vs_plain"] = res["plain_fp64"]["solver_run_s"] / res["mixed_fp16_mfma"]["solver_run_s"]
+    base = res["plain_fp64"]["solver_run_s"]
+    for name, _, _, _ in variants[1:]:
+        res["speedup_" + name + "_vs_plain"] = base / res[name]["solver_run_s"]
+    res["speedup_mixed_fp16_mfma_vs_fp64_matrix_free"] = (res["fp64_matrix_free"]["solver_run_s"] /
+                                                          res["mixed_fp16_mfma"]["solver_run_s"])
     out["time_to_solve"] = res
+
+
+def train_network(B, ctx, k, n=32, nrhs=8, steps=2000):
+    """Per-cell correction network trained on (residual, error) pairs of plain FP32
+    multigrid iterations on a smaller instance of the same problem class (the
+    features are scaled per cell, so the network transfers across grid sizes)."""
+    from bosss_amd import learned
+    t0 = time.perf_counter()
+    x = np.linspace(0.0, 1.0, n + 1)
+    g = B.Grid(ctx, x, x, x)
+    nu = 10.0 ** np.random.default_rng(77).uniform(-2, 2, g.J)
+    op = B.SipOperator(g, k, nu)
+    mg = B.PMultigrid(op, degrees=[k, 1], min_coarse=64, kcycle_inner=2, prec=B.BH_FP32)
+    B.pmg_set_graphs(mg, USE_GRAPHS)
+    rng = np.random.default_rng(7)
+    X, T, ns, _, _ = learned.harvest_pairs(op, mg, [rng.standard_normal(op.N) for _ in range(nrhs)], iters_per_rhs=5)
+    net = B.CellMLP(ctx, op.Np + 2 * 3, op.Np, seed=1)
+    hist, t_train = learned.train(net, X, T, ns * g.J, steps=steps, batch=16384, lr=2e-3)
+    info = {"grid": f"{n}^3", "pairs": int(ns), "rows": int(ns * g.J), "steps": steps, "loss_first": float(hist[0]),
+            "loss_last50": float(np.mean(hist[-50:])), "train_s": t_train, "total_s": time.perf_counter() - t0}
+    del mg, op, g, X, T
+    return net, info
 
 
 def measured_traffic(n, k, kernel_prefix):

@@ -86,6 +86,8 @@ SIGNATURES = {
     "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_set_learned_fallback": (_c_i, [_c_p, _c_i, _c_d]),
+    "bh_pmg_learned_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_d)]),
     "bh_euler_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _PP]),
     "bh_euler_destroy": (_c_i, [_c_p]),
     "bh_euler_info": (_c_i, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i)]),
@@ -591,6 +593,18 @@ def pmg_learned_stats(pmg):
     a, r = _c_i(), _c_i()
     _check(lib().bh_pmg_learned_stats(pmg.h, ctypes.byref(a), ctypes.byref(r)))
     return a.value, r.value
+
+
+def pmg_set_learned_fallback(pmg, probe=8, min_gain=1e-2):
+    """After `probe` calls, a mean removed |r'|^2 fraction below min_gain switches the network off."""
+    _check(lib().bh_pmg_set_learned_fallback(pmg.h, int(probe), float(min_gain)))
+
+
+def pmg_learned_info(pmg):
+    """(still active, mean removed |r'|^2 fraction per step over the probe window)"""
+    a, g = _c_i(), _c_d()
+    _check(lib().bh_pmg_learned_info(pmg.h, ctypes.byref(a), ctypes.byref(g)))
+    return bool(a.value), g.value
 
 
 def pmg_set_graphs(pmg, enable):

@@ -737,6 +737,23 @@ int bh_pmg_learned_stats(bh_solver* s, int* acc, int* rej) {
     mg->learned_stats(acc, rej);
     BH_API_END
 }
+int bh_pmg_set_learned_fallback(bh_solver* s, int probe, double min_gain) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_set_learned_fallback: solver is not a multigrid preconditioner");
+    BH_CHECK(probe >= 0 && std::isfinite(min_gain), "bh_pmg_set_learned_fallback: need probe >= 0 and a finite min_gain");
+    mg->learned_probe = probe;
+    mg->learned_min_gain = min_gain;
+    BH_API_END
+}
+int bh_pmg_learned_info(bh_solver* s, int* active, double* mean_gain) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_learned_info: solver is not a multigrid preconditioner");
+    if (active) *active = (mg->learned && !mg->learned_off) ? 1 : 0;
+    if (mean_gain) *mean_gain = mg->learned_mean_gain;
+    BH_API_END
+}
 
 int bh_pmg_set_graphs(bh_solver* s, int enable) {
     BH_API_BEGIN

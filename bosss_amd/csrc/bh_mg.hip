@@ -88,32 +88,48 @@ __global__ void agg_prolong_add_k(int Jf, int bs, const int* __restrict__ parent
     }
 }
 
-// x (+)= omega * Dinv r, one wave per block row, groups of BS lanes over the
-// columns of the (column-major) inverse block.
+// x (+)= omega * Dinv r.  A workgroup takes R consecutive block rows: their
+// inverse blocks (contiguous, column-major) and residual pieces are staged in
+// LDS with coalesced loads by all 256 threads, then thread (row, i) forms
+// entry i of its row's product.  R = 256 / BS, capped so that the staged
+// blocks stay within 32 KB.
+template <typename T, int BS>
+struct BlockGemvShape {
+    static constexpr int R0 = 256 / BS;
+    static constexpr int RC = 32768 / (BS * BS * (int)sizeof(T));
+    static constexpr int R = R0 < RC ? (R0 > 0 ? R0 : 1) : (RC > 0 ? RC : 1);
+};
+
 template <typename T, int BS, bool ADD>
 __global__ __launch_bounds__(256) void block_gemv_k(int nrows, const T* __restrict__ Dinv, const T* __restrict__ r,
                                                     T* __restrict__ x, T omega) {
-    constexpr int G = 64 / BS;
-    __shared__ T red[4][64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + wave;
-    const bool active = row < nrows;
-    const int g = lane / BS, ri = lane - g * BS;
-    T acc = 0;
-    if (active && g < G) {
-        const T* __restrict__ Di = Dinv + (size_t)row * BS * BS;
-        const T* __restrict__ rr = r + (size_t)row * BS;
-#pragma unroll 4
-        for (int c = g; c < BS; c += G) acc += Di[c * BS + ri] * rr[c];
+    constexpr int R = BlockGemvShape<T, BS>::R;
+    __shared__ __attribute__((aligned(16))) T sD[R * BS * BS];
+    __shared__ T sr[R * BS];
+    const int row0 = blockIdx.x * R;
+    const int nr = min(R, nrows - row0);
+    const T* __restrict__ Dg = Dinv + (size_t)row0 * BS * BS;
+    const T* __restrict__ rg = r + (size_t)row0 * BS;
+    // 16-byte loads where the staged range allows them (the block start is a multiple of
+    // R*BS*BS elements from an allocation start, so 16-byte aligned when that range is)
+    constexpr int V16 = 16 / (int)sizeof(T);
+    if ((R * BS * BS) % V16 == 0 && (nr * BS * BS) % V16 == 0) {
+        const int nv = nr * BS * BS / V16;
+        for (int t = threadIdx.x; t < nv; t += blockDim.x) ((uint4*)sD)[t] = ((const uint4*)Dg)[t];
+    } else {
+        for (int t = threadIdx.x; t < nr * BS * BS; t += blockDim.x) sD[t] = Dg[t];
     }
-    red[wave][lane] = acc;
+    for (int t = threadIdx.x; t < nr * BS; t += blockDim.x) sr[t] = rg[t];
     __syncthreads();
-    if (active && g == 0) {
-        T s = acc;
-#pragma unroll
-        for (int gg = 1; gg < G; ++gg) s += red[wave][gg * BS + ri];
-        const size_t o = (size_t)row * BS + ri;
-        x[o] = ADD ? x[o] + omega * s : omega * s;
+    const int row = threadIdx.x / BS, i = threadIdx.x - row * BS;
+    if (row < nr) {
+        const T* __restrict__ Di = sD + row * BS * BS;
+        const T* __restrict__ ri = sr + row * BS;
+        T acc = 0;
+#pragma unroll 4
+        for (int c = 0; c < BS; ++c) acc += Di[c * BS + i] * ri[c];
+        const size_t o = (size_t)(row0 + row) * BS + i;
+        x[o] = ADD ? x[o] + omega * acc : omega * acc;
     }
 }
 
@@ -188,15 +204,20 @@ __global__ void kcycle_lsq_k(double* __restrict__ S, KcycleScalars L) {
     }
 }
 
-// learned step: alpha = <q, r'> / <q, q> from st[0..1]; x += alpha e only when
-// alpha > 0 and finite; counters cnt[0] (taken) / cnt[1] (declined)
+// learned step: alpha = <q, r'> / <q, q> from st[0..2] = <q, r'>, <q, q>, <r', r'>;
+// x += alpha e only when alpha > 0 and finite.  The fraction of |r'|^2 the step
+// removes, <q, r'>^2 / (<q, q> <r', r'>), is summed into st[3]; counters
+// cnt[0] (taken) / cnt[1] (declined).
 template <typename T>
 __global__ __launch_bounds__(256) void guarded_step_k(const T* __restrict__ e, T* __restrict__ x, int64_t n,
-                                                      const double* __restrict__ st, int* __restrict__ cnt) {
-    const double qq = st[1];
-    const double alpha = qq > 0 ? st[0] / qq : 0.0;
+                                                      double* __restrict__ st, int* __restrict__ cnt) {
+    const double qr = st[0], qq = st[1], rr = st[2];
+    const double alpha = qq > 0 ? qr / qq : 0.0;
     const bool take = alpha > 0 && alpha < 1e300;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[take ? 0 : 1], 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&cnt[take ? 0 : 1], 1);
+        if (take && rr > 0) st[3] += qr * qr / (qq * rr);
+    }
     if (!take) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         x[i] = (T)((double)x[i] + alpha * (double)e[i]);
@@ -217,7 +238,7 @@ void kcycle_lsq(bh_ctx* ctx, double* S, const KcycleScalars& L) {
     BH_HIP(hipGetLastError());
 }
 template <typename T>
-void guarded_step(bh_ctx* ctx, const T* e, T* x, int64_t n, const double* st, int* cnt) {
+void guarded_step(bh_ctx* ctx, const T* e, T* x, int64_t n, double* st, int* cnt) {
     hipLaunchKernelGGL((guarded_step_k<T>), grid1d(n), dim3(256), 0, ctx->stream, e, x, n, st, cnt);
     BH_HIP(hipGetLastError());
 }
@@ -245,12 +266,14 @@ void agg_prolong_add(bh_ctx* ctx, int Jf, int bs, const int* parent, const T* P,
 
 template <typename T>
 void block_gemv(bh_ctx* ctx, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add) {
-    dim3 blk(256), grd((nrows + 3) / 4);
+    dim3 blk(256);
 #define BH_CASE(B)                                                                                                \
     case B:                                                                                                       \
+        grd = dim3((nrows + BlockGemvShape<T, B>::R - 1) / BlockGemvShape<T, B>::R);                           \
         if (add) hipLaunchKernelGGL((block_gemv_k<T, B, true>), grd, blk, 0, ctx->stream, nrows, Dinv, r, x, (T)omega); \
         else hipLaunchKernelGGL((block_gemv_k<T, B, false>), grd, blk, 0, ctx->stream, nrows, Dinv, r, x, (T)omega);    \
         break;
+    dim3 grd;
     switch (bs) {
         BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
         BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)
@@ -279,7 +302,7 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void dense_gemv<T>(bh_ctx*, int, const double*, const T*, T*);                               \
     template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);                                          \
     template void scale_inv_norm<T>(bh_ctx*, const T*, T*, int64_t, const double*, double);           \
-    template void guarded_step<T>(bh_ctx*, const T*, T*, int64_t, const double*, int*);
+    template void guarded_step<T>(bh_ctx*, const T*, T*, int64_t, double*, int*);
 BH_INST(float)
 BH_INST(double)
 #undef BH_INST

@@ -33,7 +33,7 @@ struct KcycleScalars {
 template <typename T> void scale_inv_norm(bh_ctx*, const T* src, T* dst, int64_t n, const double* s2, double tiny);
 void kcycle_column(bh_ctx*, double* S, const KcycleScalars& L, int j);
 void kcycle_lsq(bh_ctx*, double* S, const KcycleScalars& L);
-template <typename T> void guarded_step(bh_ctx*, const T* e, T* x, int64_t n, const double* st, int* cnt);
+template <typename T> void guarded_step(bh_ctx*, const T* e, T* x, int64_t n, double* st, int* cnt);
 
 struct MgLevel {
     int J = 0, bs = 1;
@@ -72,8 +72,15 @@ struct PMultigrid : bh_solver {
     struct Mlp* learned = nullptr;
     bh_op* fine_op = nullptr;
     DevBuf lX, ls, lY, le, lrq;     // features, scales, outputs, correction, [r' | A e]
-    DevBuf lstat;                   // <q, r'>, <q, q> (doubles), then int counters taken / declined
+    DevBuf lstat;                   // <q, r'>, <q, q>, <r', r'>, sum of removed |r'|^2 fractions, int counters
     void learned_stats(int* acc, int* rej);
+    // fallback: after `learned_probe` preconditioner calls, a mean removed fraction of
+    // |r'|^2 below `learned_min_gain` switches the learned step off (plain multigrid)
+    int learned_probe = 8;
+    double learned_min_gain = 1e-2;
+    int learned_calls = 0;
+    bool learned_off = false;
+    double learned_mean_gain = 0.0;
     template <typename T> void learned_correction();
     void attach_learned(struct Mlp* m);
     ~PMultigrid() override;

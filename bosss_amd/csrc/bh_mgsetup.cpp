@@ -45,7 +45,7 @@ void PMultigrid::drop_graphs() {
 template <typename T>
 void PMultigrid::cycle_body(int l) {
     vcycle<T>(l);
-    if (l == 0 && learned) learned_correction<T>();
+    if (l == 0 && learned && !learned_off) learned_correction<T>();
 }
 
 template <typename T>
@@ -486,8 +486,11 @@ void PMultigrid::attach_learned(Mlp* m) {
     lY.alloc((size_t)J * m->out_p * sizeof(float));
     le.alloc(N * es);
     lrq.alloc(2 * N * es);
-    lstat.alloc(4 * sizeof(double));
-    fill(ctx, lstat.as<double>(), 0.0, 4);   // also zeroes the two int counters in the third double
+    lstat.alloc(6 * sizeof(double));
+    fill(ctx, lstat.as<double>(), 0.0, 6);   // also zeroes the two int counters in the fifth double
+    learned_calls = 0;
+    learned_off = false;
+    learned_mean_gain = 0.0;
 }
 
 // After the cycle (x = x_mg in lv[0].x): r' = b - A x_mg, e = network(r'),
@@ -506,10 +509,11 @@ void PMultigrid::learned_correction() {
     mlp_features(fine_op, rp, p, lX.as<float>(), ls.as<float>(), learned->in_p);
     learned->infer(fine_op->grid->J, lX.as<float>(), lY.as<float>());
     mlp_correction(fine_op, ls.as<float>(), lY.as<float>(), learned->out_p, e, p);
-    sip_apply(fine_op, 1.0, e, 0.0, q, false, p);
+    sip_apply(fine_op, 1.0, e, 0.0, q, false, op_prec(0, p == BH_FP32));
     double* st = lstat.as<double>();
-    multi_dot_to_dev<T>(ctx, q, rq, N, 2, N, st);   // <q, r'>, <q, q>
-    guarded_step<T>(ctx, e, vec<T>(L.x), N, st, (int*)(st + 2));
+    multi_dot_to_dev<T>(ctx, q, rq, N, 2, N, st);        // <q, r'>, <q, q>
+    multi_dot_to_dev<T>(ctx, rp, rp, 0, 1, N, st + 2);   // <r', r'>
+    guarded_step<T>(ctx, e, vec<T>(L.x), N, st, (int*)(st + 4));
 }
 
 // The cycle from the finest level, the K-cycle inside it and the guarded learned
@@ -526,11 +530,20 @@ void PMultigrid::precondition(double* x, const double* b) {
         copy(ctx, x, vec<double>(L.x), N);
     }
     iterations++;
+    if (learned && !learned_off && learned_probe > 0 && ++learned_calls == learned_probe) {
+        double st[6];
+        ctx->download(st, lstat.p, sizeof(st));
+        learned_mean_gain = st[3] / learned_probe;
+        if (!(learned_mean_gain >= learned_min_gain)) {
+            learned_off = true;   // the plain multigrid takes over
+            drop_graphs();
+        }
+    }
 }
 
 void PMultigrid::learned_stats(int* acc, int* rej) {
     int c[2] = {0, 0};
-    if (learned && lstat.p) ctx->download(c, lstat.as<double>() + 2, sizeof(c));
+    if (learned && lstat.p) ctx->download(c, lstat.as<double>() + 4, sizeof(c));
     if (acc) *acc = c[0];
     if (rej) *rej = c[1];
 }

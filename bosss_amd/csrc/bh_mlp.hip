@@ -36,34 +36,38 @@ __host__ __device__ inline float bf16_to_f32(uint16_t b) {
 // ------------------------------------------------------------------ features
 // X[j][:] as described above (fp32, row stride IN_P), scale s[j].
 template <typename T, int D>
-__global__ void mlp_features_k(int J, int Np, int n0, int n1, int n2, const T* __restrict__ r, const double* __restrict__ nu,
-                               int in_p, float* __restrict__ X, float* __restrict__ s) {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < J; j += gridDim.x * blockDim.x) {
-        double nrm = 0;
-        for (int m = 0; m < Np; ++m) { double v = (double)r[(size_t)j * Np + m]; nrm += v * v; }
-        const double sc = sqrt(nrm) + 1e-30;
-        float* x = X + (size_t)j * in_p;
-        for (int m = 0; m < Np; ++m) x[m] = (float)((double)r[(size_t)j * Np + m] / sc);
-        int idx[3];
-        if (D == 2) { idx[0] = j / n1; idx[1] = j - idx[0] * n1; idx[2] = 0; }
-        else { int sl = n0 * n1; idx[2] = j / sl; int rr = j - idx[2] * sl; idx[1] = rr / n0; idx[0] = rr - idx[1] * n0; }
-        const int nn[3] = {n0, n1, n2};
-        const double lnu = log10(nu[j]);
-        int f = 0;
-        for (int d = 0; d < D; ++d)
-            for (int side = 0; side < 2; ++side, ++f) {
-                int q[3] = {idx[0], idx[1], idx[2]};
-                q[d] += side ? 1 : -1;
-                float v;
-                if (q[d] < 0 || q[d] >= nn[d]) v = 2.0f;
-                else {
-                    const int c = D == 2 ? q[0] * n1 + q[1] : q[0] + q[1] * n0 + q[2] * n0 * n1;
-                    v = (float)(log10(nu[c]) - lnu);
-                }
-                x[Np + f] = v;
+__global__ __launch_bounds__(256) void mlp_features_k(int J, int Np, int n0, int n1, int n2, const T* __restrict__ r,
+                                                      const double* __restrict__ nu, int in_p, float* __restrict__ X,
+                                                      float* __restrict__ s) {
+    // one segment of in_p lanes (16 or 32) per cell: lane t of the segment owns
+    // feature slot t (coalesced reads of the residual row and writes of the
+    // feature row); the residual norm is a butterfly inside the segment
+    const int segs = 256 / in_p;
+    const int t = threadIdx.x & (in_p - 1);
+    for (int j = blockIdx.x * segs + threadIdx.x / in_p; j < J; j += gridDim.x * segs) {
+        const double v = t < Np ? (double)r[(size_t)j * Np + t] : 0.0;
+        double sq = v * v;
+        for (int off = in_p >> 1; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        const double sc = sqrt(sq) + 1e-30;
+        float out = 0.f;
+        if (t < Np) {
+            out = (float)(v / sc);
+        } else if (t < Np + 2 * D) {
+            const int f = t - Np, d = f >> 1, side = f & 1;
+            int q[3];
+            if (D == 2) { q[0] = j / n1; q[1] = j - q[0] * n1; q[2] = 0; }
+            else { const int sl = n0 * n1; q[2] = j / sl; const int rr = j - q[2] * sl; q[1] = rr / n0; q[0] = rr - q[1] * n0; }
+            const int nn[3] = {n0, n1, n2};
+            q[d] += side ? 1 : -1;
+            if (q[d] < 0 || q[d] >= nn[d]) {
+                out = 2.0f;
+            } else {
+                const int c = D == 2 ? q[0] * n1 + q[1] : q[0] + q[1] * n0 + q[2] * n0 * n1;
+                out = log10f((float)nu[c]) - log10f((float)nu[j]);
             }
-        for (int t = Np + 2 * D; t < in_p; ++t) x[t] = 0.f;
-        s[j] = (float)sc;
+        }
+        X[(size_t)j * in_p + t] = out;
+        if (t == 0) s[j] = (float)sc;
     }
 }
 
@@ -116,11 +120,11 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 // one 8-byte read; activations go through a per-wave LDS tile (bf16, row-major
 // [cell][feature]) so an A fragment is one 8-byte read.
 template <int INP, int H, int OUTP>
-__global__ __launch_bounds__(128) void mlp_infer_k(int J, const float* __restrict__ X, const uint16_t* __restrict__ W1t,
+__global__ __launch_bounds__(512) void mlp_infer_k(int J, const float* __restrict__ X, const uint16_t* __restrict__ W1t,
                                                    const float* __restrict__ b1, const uint16_t* __restrict__ W2t,
                                                    const float* __restrict__ b2, const uint16_t* __restrict__ W3t,
                                                    const float* __restrict__ b3, float* __restrict__ Y) {
-    constexpr int WAVES = 2;
+    constexpr int WAVES = 8;   // the weights (48 KB) are staged once per workgroup of 8 waves
     __shared__ uint16_t sW1[H * INP];
     __shared__ uint16_t sW2[H * H];
     __shared__ uint16_t sW3[OUTP * H];
@@ -384,12 +388,14 @@ void Mlp::infer(int J, const float* X, float* Y) {
     const float* P = params.as<float>();
     const uint16_t* Wb = wbf.as<uint16_t>();
     const int groups = (J + 15) / 16;
-    int blocks = (groups + 1) / 2;
-    if (blocks > 4096) blocks = 4096;
+    // persistent workgroups (2 per CU) loop over the 16-cell groups: the weight
+    // staging is paid once per workgroup, not once per handful of cells
+    int blocks = (groups + 7) / 8;
+    if (blocks > 512) blocks = 512;
     if (blocks < 1) blocks = 1;
 #define BH_CASE(I, O)                                                                                            \
     if (in_p == I && out_p == O) {                                                                               \
-        hipLaunchKernelGGL((mlp_infer_k<I, 128, O>), dim3(blocks), dim3(128), 0, ctx->stream, J, X, Wb, P + off_b1(), \
+        hipLaunchKernelGGL((mlp_infer_k<I, 128, O>), dim3(blocks), dim3(512), 0, ctx->stream, J, X, Wb, P + off_b1(), \
                            Wb + hid * in_p, P + off_b2(), Wb + hid * in_p + hid * hid, P + off_b3(), Y);            \
         BH_HIP(hipGetLastError());                                                                               \
         return;                                                                                                  \
@@ -435,7 +441,9 @@ void Mlp::adam(float lr) {
 void mlp_features(bh_op* op, const void* r, int prec, float* X, float* s, int in_p) {
     bh_grid* g = op->grid;
     const int J = g->J;
-    dim3 grd((J + 255) / 256 > 4096 ? 4096 : (J + 255) / 256), blk(256);
+    BH_CHECK((in_p == 16 || in_p == 32) && op->Np + 2 * op->D <= in_p, "mlp_features: input width must be 16 or 32");
+    const int segs = 256 / in_p;
+    dim3 grd((J + segs - 1) / segs > 8192 ? 8192 : (J + segs - 1) / segs), blk(256);
     if (op->D == 2) {
         if (prec == BH_FP32) hipLaunchKernelGGL((mlp_features_k<float, 2>), grd, blk, 0, op->ctx->stream, J, op->Np, g->n[0], g->n[1], g->n[2], (const float*)r, op->nu.as<double>(), in_p, X, s);
         else hipLaunchKernelGGL((mlp_features_k<double, 2>), grd, blk, 0, op->ctx->stream, J, op->Np, g->n[0], g->n[1], g->n[2], (const double*)r, op->nu.as<double>(), in_p, X, s);

@@ -94,7 +94,7 @@ __device__ __forceinline__ void face_term(const SipArgs<R>& a, int f, const R* u
     const R* __restrict__ TQ = a.T + (size_t)fQ * NQF * Np;
     const R* __restrict__ NQ = a.Nn + (size_t)fQ * NQF * Np;
     const R nuMax = (fabs((double)nuP) > fabs((double)nuQ)) ? nuP : nuQ;
-#pragma unroll 1
+#pragma unroll (Np <= 10 ? NQF : 1)
     for (int q = 0; q < NQF; ++q) {
         R vP = 0, dP = 0, vQ = 0, dQ = 0;
         if (SELF) {
@@ -134,7 +134,7 @@ __device__ __forceinline__ void face_term(const SipArgs<R>& a, int f, const R* u
 template <int D, int K, typename R>
 __device__ __forceinline__ void volume_term(const SipArgs<R>& a, const R* uP, const R* s2, R nuP, R* y) {
     constexpr int Np = binom(K + D, D);
-#pragma unroll 1
+#pragma unroll (Np <= 10 ? D : 1)
     for (int d = 0; d < D; ++d) {
         const R* __restrict__ Kd = a.Kv + (size_t)d * Np * Np;
         const R cd = -nuP * (d == 0 ? s2[0] : (d == 1 ? s2[1] : s2[D - 1]));
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     const R cjP = cell_cj<D>(a, idx, h);
     const R sdPinv = R(1) / sqrt(detJ);
     const R pen = R(a.pen);
-#pragma unroll 1
+#pragma unroll (Np <= 4 ? 2 * D : 1)
     for (int f = 0; f < 2 * D; ++f) {
         const int d = f >> 1, s = f & 1;
         const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);

@@ -204,9 +204,18 @@ int bh_mlp_adam_step(bh_mlp* mlp, float lr);
  * target map of an error e -> T[J][out_p] with the same scales */
 int bh_mlp_features(bh_op* op, const double* r, float* X, float* s, int in_p, int flags);
 int bh_mlp_targets(bh_op* op, const float* s, const double* e, float* T, int out_p, int flags);
-/* attach (mlp != NULL) or detach the learned correction of a bh_pmg_create solver */
+/* attach (mlp != NULL) or detach the learned correction of a bh_pmg_create solver:
+ * after each cycle, e = network(r'), and x += alpha e with the residual-minimising
+ * alpha = <A e, r'> / <A e, A e> when alpha > 0 (counted as accepted, else rejected) */
 int bh_pmg_set_learned(bh_solver* pmg, bh_mlp* mlp);
 int bh_pmg_learned_stats(bh_solver* pmg, int* accepted, int* rejected);
+/* No counterpart in the reference.  Fallback of the learned correction: after
+ * `probe` preconditioner calls (0: never), a mean removed fraction of |r'|^2 per
+ * step below `min_gain` switches the network off and the plain multigrid takes
+ * over (defaults 8 and 1e-2).  bh_pmg_learned_info reports whether it is still
+ * active and the measured mean gain (0 before the probe window ends). */
+int bh_pmg_set_learned_fallback(bh_solver* pmg, int probe, double min_gain);
+int bh_pmg_learned_info(bh_solver* pmg, int* active, double* mean_gain);
 
 /* ---------------------------------------------------------------- compressible Euler
  * Explicit DG for the Euler equations on a periodic Cartesian grid: the CNS
