@@ -86,6 +86,8 @@ SIGNATURES = {
     "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_set_schwarz": (_c_i, [_c_p, _c_i]),
+    "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_learned_fallback": (_c_i, [_c_p, _c_i, _c_d]),
     "bh_pmg_learned_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_d)]),
     "bh_euler_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _PP]),
@@ -607,6 +609,11 @@ def pmg_learned_info(pmg):
     return bool(a.value), g.value
 
 
+def pmg_set_schwarz(pmg, enable=True):
+    """Additive Schwarz on the multigrid aggregates (no overlap) as the smoother of the aggregated levels."""
+    _check(lib().bh_pmg_set_schwarz(pmg.h, 1 if enable else 0))
+
+
 def pmg_set_graphs(pmg, enable):
     """HIP-graph replay of the synchronisation-free V-cycles on (default) / off."""
     _check(lib().bh_pmg_set_graphs(pmg.h, 1 if enable else 0))
@@ -769,6 +776,11 @@ class XdgPrecond(_Solver):
         _check(lib().bh_xdg_precond_create(xdg.h, coarse_degree, min_coarse, omega, pre, post, kcycle_inner, prec,
                                            ctypes.byref(h)))
         super().__init__(xdg.ctx, h.value, xdg.N, (xdg,))
+
+
+def xdg_precond_set_fp64_blocks(precond, on=True):
+    """FP32 XDG preconditioner: cell blocks applied with their FP64 inverse."""
+    _check(lib().bh_xdg_precond_set_fp64_blocks(precond.h, 1 if on else 0))
 
 
 class XdgFlexGMRES(_Solver):

@@ -755,6 +755,25 @@ int bh_pmg_learned_info(bh_solver* s, int* active, double* mean_gain) {
     BH_API_END
 }
 
+int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
+    BH_API_BEGIN
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(s);
+    BH_CHECK(xp, "bh_xdg_precond_set_fp64_blocks: solver is not an XDG preconditioner");
+    xp->Dc->fp64_inverse = on != 0;
+    BH_API_END
+}
+
+int bh_pmg_set_schwarz(bh_solver* s, int enable) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(s);
+    BH_CHECK(mg || xp, "bh_pmg_set_schwarz: solver is not a multigrid preconditioner");
+    if (xp) mg = xp->cmg.get();
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    mg->set_schwarz(enable != 0);
+    BH_API_END
+}
+
 int bh_pmg_set_graphs(bh_solver* s, int enable) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);

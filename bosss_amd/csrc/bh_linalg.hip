@@ -162,6 +162,22 @@ void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32) {
     BH_CHECK(h == 0, "block inversion: singular diagonal block in row " + std::to_string(h - 1));
 }
 
+// Inverses of `nblocks` dense column-major n x n blocks (n <= 64), same kernel
+void invert_dense_blocks(bh_ctx* ctx, int nblocks, int n, const double* blocks, double* inv64, float* inv32) {
+    BH_CHECK(n <= 64, "dense block inversion supports n <= 64");
+    std::vector<int64_t> iota(nblocks);
+    for (int i = 0; i < nblocks; ++i) iota[i] = i;
+    DevBuf idx(nblocks * sizeof(int64_t)), st(sizeof(int));
+    ctx->upload(idx.p, iota.data(), iota.size() * sizeof(int64_t));
+    BH_HIP(hipMemsetAsync(st.p, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(invert_blocks_kernel, dim3(nblocks), dim3(64), (size_t)n * 2 * n * sizeof(double), ctx->stream, nblocks,
+                       n, idx.as<int64_t>(), blocks, inv64, inv32, st.as<int>());
+    BH_HIP(hipGetLastError());
+    int h = 0;
+    ctx->download(&h, st.p, sizeof(int));
+    BH_CHECK(h == 0, "dense block inversion: singular block " + std::to_string(h - 1));
+}
+
 // ------------------------------------------------------------ block Jacobi
 // One fused sweep: xout = xin + omega * Dinv (b - A xin).  Algebraically the
 // reference's update x = omega*Dinv*(b - A x + D x) + (1-omega) x

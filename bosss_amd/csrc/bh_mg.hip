@@ -151,6 +151,62 @@ __global__ void copy_k(const T* __restrict__ s, T* __restrict__ d, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
+// ---------------------------------------------------------------- aggregate Schwarz
+// Additive Schwarz over the multigrid aggregates without overlap (the MG-aggregate
+// blocking of AS/Schwarz.cs): the block of aggregate I is A restricted to the DOFs
+// of its children (slot s = position in the children list, DOF s*bs + i), stored
+// dense and column-major with BA = cmax*bs rows; slots beyond the aggregate's child
+// count are identity, so every block stays invertible.
+__global__ void gather_agg_blocks_k(int nagg, int bs, int cmax, const int* __restrict__ cptr, const int* __restrict__ cidx,
+                                    const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+                                    const double* __restrict__ vals, double* __restrict__ out) {
+    const int BA = cmax * bs;
+    const int64_t tot = (int64_t)nagg * cmax * cmax;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(t / (cmax * cmax));
+        const int su = (int)(t - (int64_t)I * cmax * cmax);
+        const int s = su / cmax, u = su - s * cmax;
+        const int nc = cptr[I + 1] - cptr[I];
+        double* __restrict__ blk = out + (size_t)I * BA * BA;
+        if (s < nc && u < nc) {
+            const int rs = cidx[cptr[I] + s], cu = cidx[cptr[I] + u];
+            int64_t p = -1;
+            for (int64_t q = rowptr[rs]; q < rowptr[rs + 1]; ++q)
+                if (col[q] == cu) { p = q; break; }
+            for (int j = 0; j < bs; ++j)
+                for (int i = 0; i < bs; ++i)
+                    blk[(size_t)(u * bs + j) * BA + s * bs + i] = p >= 0 ? vals[(size_t)p * bs * bs + (size_t)j * bs + i] : 0.0;
+        } else {
+            for (int j = 0; j < bs; ++j)
+                for (int i = 0; i < bs; ++i) blk[(size_t)(u * bs + j) * BA + s * bs + i] = (s == u && i == j) ? 1.0 : 0.0;
+        }
+    }
+}
+
+// x (+)= omega * sum_I R_I^T Ainv_I R_I r: thread (aggregate I, block row) with the
+// aggregate's residual gathered through its children list (each DOF in one block)
+template <typename T, bool ADD>
+__global__ __launch_bounds__(256) void schwarz_apply_k(int nagg, int bs, int cmax, const int* __restrict__ cptr,
+                                                       const int* __restrict__ cidx, const T* __restrict__ Ainv,
+                                                       const T* __restrict__ r, T* __restrict__ x, T omega) {
+    const int BA = cmax * bs;
+    const int64_t tot = (int64_t)nagg * BA;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(t / BA), row = (int)(t - (int64_t)I * BA);
+        const int c0 = cptr[I], nc = cptr[I + 1] - c0;
+        const int s = row / bs, i = row - s * bs;
+        if (s >= nc) continue;
+        const T* __restrict__ Ai = Ainv + (size_t)I * BA * BA;
+        T acc = 0;
+        for (int u = 0; u < nc; ++u) {
+            const T* __restrict__ ru = r + (size_t)cidx[c0 + u] * bs;
+            for (int j = 0; j < bs; ++j) acc += Ai[(size_t)(u * bs + j) * BA + row] * ru[j];
+        }
+        const size_t o = (size_t)cidx[c0 + s] * bs + i;
+        x[o] = ADD ? x[o] + omega * acc : omega * acc;
+    }
+}
+
 // ---------------------------------------------------------------- K-cycle scalars
 // The inner flexible GMRES of the K-cycle keeps its Hessenberg matrix, Gram-
 // Schmidt coefficients and least-squares solution in device memory (layout
@@ -224,6 +280,21 @@ __global__ __launch_bounds__(256) void guarded_step_k(const T* __restrict__ e, T
 }
 
 // ------------------------------------------------------------------ launchers
+void gather_agg_blocks(bh_ctx* ctx, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const bh_bsr* A,
+                       double* out) {
+    hipLaunchKernelGGL(gather_agg_blocks_k, grid1d((int64_t)nagg * cmax * cmax), dim3(256), 0, ctx->stream, nagg, bs, cmax,
+                       cptr, cidx, A->rowptr.as<int64_t>(), A->col.as<int>(), A->vals.as<double>(), out);
+    BH_HIP(hipGetLastError());
+}
+template <typename T>
+void schwarz_apply(bh_ctx* ctx, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const T* Ainv, const T* r,
+                   T* x, double omega, bool add) {
+    const int64_t tot = (int64_t)nagg * cmax * bs;
+    if (add) hipLaunchKernelGGL((schwarz_apply_k<T, true>), grid1d(tot), dim3(256), 0, ctx->stream, nagg, bs, cmax, cptr, cidx, Ainv, r, x, (T)omega);
+    else hipLaunchKernelGGL((schwarz_apply_k<T, false>), grid1d(tot), dim3(256), 0, ctx->stream, nagg, bs, cmax, cptr, cidx, Ainv, r, x, (T)omega);
+    BH_HIP(hipGetLastError());
+}
+
 template <typename T>
 void scale_inv_norm(bh_ctx* ctx, const T* src, T* dst, int64_t n, const double* s2, double tiny) {
     hipLaunchKernelGGL((scale_inv_norm_k<T>), grid1d(n), dim3(256), 0, ctx->stream, src, dst, n, s2, tiny);
@@ -302,6 +373,7 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void dense_gemv<T>(bh_ctx*, int, const double*, const T*, T*);                               \
     template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);                                          \
     template void scale_inv_norm<T>(bh_ctx*, const T*, T*, int64_t, const double*, double);           \
+    template void schwarz_apply<T>(bh_ctx*, int, int, int, const int*, const int*, const T*, const T*, T*, double, bool); \
     template void guarded_step<T>(bh_ctx*, const T*, T*, int64_t, double*, int*);
 BH_INST(float)
 BH_INST(double)

@@ -31,6 +31,11 @@ struct KcycleScalars {
     }
 };
 template <typename T> void scale_inv_norm(bh_ctx*, const T* src, T* dst, int64_t n, const double* s2, double tiny);
+// aggregate Schwarz blocks: dense column-major (cmax*bs)^2 per aggregate, then x (+)= omega sum_I R_I^T Ainv_I R_I r
+void gather_agg_blocks(bh_ctx*, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const bh_bsr* A, double* out);
+template <typename T> void schwarz_apply(bh_ctx*, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const T* Ainv,
+                                         const T* r, T* x, double omega, bool add);
+void invert_dense_blocks(bh_ctx*, int nblocks, int n, const double* blocks, double* inv64, float* inv32);
 void kcycle_column(bh_ctx*, double* S, const KcycleScalars& L, int j);
 void kcycle_lsq(bh_ctx*, double* S, const KcycleScalars& L);
 template <typename T> void guarded_step(bh_ctx*, const T* e, T* x, int64_t n, double* st, int* cnt);
@@ -53,10 +58,14 @@ struct MgLevel {
     DevBuf b, x, r;
     // K-cycle storage (inner flexible Krylov on this level): rhs, V, Z, w
     DevBuf kb, kv, kz, kw, ks;     // ks: KcycleScalars in device memory
+    // additive Schwarz smoother over this level's aggregates (sw_cmax > 0: in use)
+    int sw_cmax = 0, sw_nagg = 0;
+    DevBuf sw64, sw32;
 };
 
 struct PMultigrid : bh_solver {
     int prec = BH_FP64;
+    int dim = 3;                   // space dimension of the grid
     bool mfma = false;             // FP32 cycle: matrix-free level 0 on the FP16 matrix cores
     double omega = 1.0;
     int pre = 1, post = 1;
@@ -104,6 +113,8 @@ struct PMultigrid : bh_solver {
     template <typename T> void solve_level(int l) { if (l == kcycle_level && kin > 0) kcycle<T>(l); else vcycle<T>(l); }
     template <typename T> void residual(int l, const T* b, const T* x, T* r);
     template <typename T> void apply(int l, const T* x, T* y);   // y = A_l x
+    template <typename T> void smooth(int l, const T* r, T* x, bool add);   // x (+)= omega M_l^-1 r
+    void set_schwarz(bool on);     // aggregate-block Schwarz on every level that has aggregates
     int op_prec(int l, bool f32) const { return f32 ? ((l == 0 && mfma) ? BH_FP16_MFMA : BH_FP32) : BH_FP64; }
     template <typename T> T* vec(DevBuf& d) { return (T*)d.p; }
 };

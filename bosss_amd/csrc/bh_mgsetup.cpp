@@ -163,6 +163,7 @@ void PMultigrid::build(bh_op* op, bh_bsr* Af, const std::vector<int>& degrees, i
 void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bh_op* mfop) {
     ctx = c;
     const int D = grid->D;
+    dim = D;
     N = (int64_t)grid->J * basis_count(D, degrees[0]);
     matrix_free = mfop != nullptr;
     for (size_t i = 1; i < degrees.size(); ++i)
@@ -406,12 +407,11 @@ void PMultigrid::vcycle(int l) {
         dense_gemv<T>(ctx, ncoarse, Ainv64.as<double>(), b, x);
         return;
     }
-    const T* Dinv = (const T*)(f32 ? L.dinv32.p : L.dinv64.p);
-    // pre-smoothing from a zero guess: the first sweep is x = omega Dinv b
-    block_gemv<T>(ctx, L.J, L.bs, Dinv, b, x, omega, false);
+    // pre-smoothing from a zero guess: the first sweep is x = omega M^-1 b
+    smooth<T>(l, b, x, false);
     for (int s = 1; s < pre; ++s) {
         residual<T>(l, b, x, r);
-        block_gemv<T>(ctx, L.J, L.bs, Dinv, r, x, omega, true);
+        smooth<T>(l, r, x, true);
     }
     residual<T>(l, b, x, r);
     MgLevel& C = lv[l + 1];
@@ -423,7 +423,45 @@ void PMultigrid::vcycle(int l) {
     else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), P, vec<T>(C.x), x);
     for (int s = 0; s < post; ++s) {
         residual<T>(l, b, x, r);
-        block_gemv<T>(ctx, L.J, L.bs, Dinv, r, x, omega, true);
+        smooth<T>(l, r, x, true);
+    }
+}
+
+// smoother of level l: block-Jacobi on the cell blocks (AS/BlockJacobi.cs:196-255), or
+// additive Schwarz on the aggregates of the level when enabled (AS/Schwarz.cs, MG-aggregate
+// blocking, no overlap)
+template <typename T>
+void PMultigrid::smooth(int l, const T* r, T* x, bool add) {
+    MgLevel& L = lv[l];
+    const bool f32 = sizeof(T) == 4;
+    if (L.sw_cmax > 0)
+        schwarz_apply<T>(ctx, L.sw_nagg, L.bs, L.sw_cmax, L.cptr.as<int>(), L.cidx.as<int>(),
+                         (const T*)(f32 ? L.sw32.p : L.sw64.p), r, x, omega, add);
+    else
+        block_gemv<T>(ctx, L.J, L.bs, (const T*)(f32 ? L.dinv32.p : L.dinv64.p), r, x, omega, add);
+}
+
+void PMultigrid::set_schwarz(bool on) {
+    drop_graphs();
+    const int D = dim;
+    for (size_t l = 0; l + 1 < lv.size(); ++l) {
+        MgLevel& L = lv[l];
+        L.sw_cmax = 0;
+        L.sw64.release();
+        L.sw32.release();
+        if (!on || L.kind != 1) continue;
+        const int cmax = 1 << D;
+        const int BA = cmax * L.bs;
+        if (BA > 64) continue;   // block inversion kernel limit: such levels keep block-Jacobi
+        const int nagg = lv[l + 1].J;
+        DevBuf dense((size_t)nagg * BA * BA * sizeof(double));
+        gather_agg_blocks(ctx, nagg, L.bs, cmax, L.cptr.as<int>(), L.cidx.as<int>(), L.A, dense.as<double>());
+        L.sw64.alloc((size_t)nagg * BA * BA * sizeof(double));
+        if (prec == BH_FP32) L.sw32.alloc((size_t)nagg * BA * BA * sizeof(float));
+        invert_dense_blocks(ctx, nagg, BA, dense.as<double>(), L.sw64.as<double>(), prec == BH_FP32 ? L.sw32.as<float>() : nullptr);
+        if (prec == BH_FP32) L.sw64.release();
+        L.sw_cmax = cmax;
+        L.sw_nagg = nagg;
     }
 }
 
@@ -567,6 +605,8 @@ template void PMultigrid::vcycle<double>(int);
 template void PMultigrid::vcycle_g<float>(int);
 template void PMultigrid::vcycle_g<double>(int);
 template void PMultigrid::cycle_body<float>(int);
+template void PMultigrid::smooth<float>(int, const float*, float*, bool);
+template void PMultigrid::smooth<double>(int, const double*, double*, bool);
 template void PMultigrid::cycle_body<double>(int);
 template void PMultigrid::kcycle<float>(int);
 template void PMultigrid::kcycle<double>(int);

@@ -422,30 +422,30 @@ __global__ __launch_bounds__(64) void vbsr_invert_kernel(int nrows, const int* _
 }
 
 // x (+)= omega * Dinv r per variable block
-template <typename T>
+template <typename T, typename M>
 __global__ __launch_bounds__(256) void vblock_gemv_kernel(int nrows, const int* __restrict__ rsz, const int64_t* __restrict__ roff,
-                                                          const int64_t* __restrict__ doff, const T* __restrict__ Dinv,
-                                                          const T* __restrict__ r, T* __restrict__ x, T omega, int add) {
-    __shared__ T red[4][64];
+                                                          const int64_t* __restrict__ doff, const M* __restrict__ Dinv,
+                                                          const T* __restrict__ r, T* __restrict__ x, M omega, int add) {
+    __shared__ M red[4][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + wave;
     const bool active = row < nrows;
     const int R = active ? rsz[row] : 1;
     const int G = R > 0 ? 64 / R : 1;
     const int g = R > 0 ? lane / R : 64, ri = R > 0 ? lane - g * R : 0;
-    T acc = 0;
+    M acc = 0;
     if (active && R > 0 && g < G) {
-        const T* Di = Dinv + doff[row];
+        const M* Di = Dinv + doff[row];
         const T* rr = r + roff[row];
-        for (int c = g; c < R; c += G) acc += Di[(int64_t)c * R + ri] * rr[c];
+        for (int c = g; c < R; c += G) acc += Di[(int64_t)c * R + ri] * (M)rr[c];
     }
     red[wave][lane] = acc;
     __syncthreads();
     if (active && R > 0 && g == 0) {
-        T s = acc;
+        M s = acc;
         for (int gg = 1; gg < G; ++gg) s += red[wave][gg * R + ri];
         const int64_t o = roff[row] + ri;
-        x[o] = add ? x[o] + omega * s : omega * s;
+        x[o] = (T)(add ? (M)x[o] + omega * s : omega * s);
     }
 }
 
@@ -583,12 +583,19 @@ void Vbsr::invert_diag(bool want32) {
 
 void Vbsr::block_gemv(const void* r, void* x, double omega, bool add, int prec) {
     dim3 blk(256), grd((nrows + 3) / 4);
-    if (prec == BH_FP32)
-        hipLaunchKernelGGL((vblock_gemv_kernel<float>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
+    if (prec == BH_FP32 && fp64_inverse && dinv64.p)
+        // FP32 vectors, FP64 inverse blocks and accumulation: cut-cell blocks that couple
+        // species at high contrast are too ill-conditioned for FP32-rounded inverses
+        hipLaunchKernelGGL((vblock_gemv_kernel<float, double>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(),
+                           roff_d.as<int64_t>(), doff_d.as<int64_t>(), dinv64.as<double>(), (const float*)r, (float*)x, omega,
+                           add ? 1 : 0);
+    else if (prec == BH_FP32)
+        hipLaunchKernelGGL((vblock_gemv_kernel<float, float>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
                            doff_d.as<int64_t>(), dinv32.as<float>(), (const float*)r, (float*)x, (float)omega, add ? 1 : 0);
     else
-        hipLaunchKernelGGL((vblock_gemv_kernel<double>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(), roff_d.as<int64_t>(),
-                           doff_d.as<int64_t>(), dinv64.as<double>(), (const double*)r, (double*)x, omega, add ? 1 : 0);
+        hipLaunchKernelGGL((vblock_gemv_kernel<double, double>), grd, blk, 0, ctx->stream, nrows, rsz_d.as<int>(),
+                           roff_d.as<int64_t>(), doff_d.as<int64_t>(), dinv64.as<double>(), (const double*)r, (double*)x, omega,
+                           add ? 1 : 0);
     BH_HIP(hipGetLastError());
 }
 

@@ -30,6 +30,7 @@ struct Vbsr {
     void invert_diag(bool want32);
     // x (+)= omega Dinv r
     void block_gemv(const void* r, void* x, double omega, bool add, int prec);
+    bool fp64_inverse = false;   // FP32 vectors with the FP64 inverse (see block_gemv)
     int64_t find(int r, int c) const {
         for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) if (col[p] == c) return p;
         return -1;

@@ -215,6 +215,14 @@ int bh_pmg_learned_stats(bh_solver* pmg, int* accepted, int* rejected);
  * over (defaults 8 and 1e-2).  bh_pmg_learned_info reports whether it is still
  * active and the measured mean gain (0 before the probe window ends). */
 int bh_pmg_set_learned_fallback(bh_solver* pmg, int probe, double min_gain);
+/* Schwarz (AS/Schwarz.cs:56-1454) with the multigrid-aggregate blocking and no
+ * overlap: on every level that is aggregated (the k = 1 level and the h-levels
+ * below it), the smoother becomes x (+)= omega sum_I R_I^T A_I^-1 R_I r over
+ * the 2^D-cell aggregates I instead of block-Jacobi over cells (enable = 1), or
+ * back to block-Jacobi (0).  Levels whose aggregate blocks exceed 64 DOF keep
+ * block-Jacobi.  Applies to bh_pmg_create and to the coarse hierarchy of
+ * bh_xdg_precond_create. */
+int bh_pmg_set_schwarz(bh_solver* pmg, int enable);
 int bh_pmg_learned_info(bh_solver* pmg, int* active, double* mean_gain);
 
 /* ---------------------------------------------------------------- compressible Euler
@@ -291,6 +299,11 @@ int bh_xdg_extend(bh_xdg* x, const double* ured, double* ufull);
  * by the aggregation multigrid (K-cycle with kcycle_inner inner steps) */
 int bh_xdg_precond_create(bh_xdg* x, int coarse_degree, int min_coarse, double omega, int pre, int post,
                           int kcycle_inner, int prec, bh_solver** solver);
+/* No counterpart in the reference: in an FP32 preconditioner, apply the cell
+ * diagonal blocks with their FP64 inverse and FP64 accumulation (vectors stay
+ * FP32); cut cells that couple two species at high contrast have blocks too
+ * ill-conditioned for FP32-rounded inverses.  on = 1 / 0. */
+int bh_xdg_precond_set_fp64_blocks(bh_solver* solver, int on);
 /* FGMRES on the agglomerated system (AS/FlexGMRES.cs), FP64 */
 int bh_xdg_fgmres_create(bh_xdg* x, bh_solver* precond, int restart, double tol, int maxit, bh_solver** solver);
 

@@ -72,6 +72,8 @@ def lib():
             "orc_pmg_vcycle_from": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_level_spmv": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_set": (None, [_p, _d, _i, _i]),
+            "orc_pmg_set_schwarz": (_i, [_p, _i]),
+            "orc_pmg_smooth": (None, [_p, _i, _p, _p]),
             "orc_euler_create": (_p, [_p, _i, _i, _d, _d]),
             "orc_euler_free": (None, [_p]),
             "orc_euler_rhs": (None, [_p, _dp, _dp]),
@@ -312,6 +314,17 @@ class Pmg:
 
     def set(self, omega, pre, post):
         lib().orc_pmg_set(self.h, omega, pre, post)
+
+    def smooth(self, level, r):
+        """x = omega M^-1 r with the level's smoother (block-Jacobi or aggregate Schwarz)."""
+        x = np.zeros(self.sizes[level])
+        lib().orc_pmg_smooth(self.h, level, np.ascontiguousarray(r, np.float64).ctypes.data, x.ctypes.data)
+        return x
+
+    def set_schwarz(self, on=True):
+        """Additive Schwarz over the aggregates (no overlap) instead of block-Jacobi on aggregated levels."""
+        if lib().orc_pmg_set_schwarz(self.h, 1 if on else 0) != 0:
+            raise ArithmeticException_("Schwarz block inversion failed")
 
     def __del__(self):
         try:

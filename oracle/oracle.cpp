@@ -807,9 +807,16 @@ struct MgLev {
     int kind = 0;               // transfer to next: 0 truncation, 1 aggregation
     std::vector<int> parent;
     std::vector<double> P;      // J x bs x bs, [fine mode][coarse mode]
+    // additive Schwarz over the aggregates (AS/Schwarz.cs with multigrid-aggregate
+    // blocks, no overlap): children per aggregate (ascending cell index) and the
+    // inverse of A restricted to their DOFs, padded to cmax children with identity
+    int sw_cmax = 0;
+    std::vector<std::vector<int>> sw_children;
+    std::vector<double> sw_inv;  // nagg x (cmax*bs)^2, row-major
 };
 
 struct Pmg {
+    int D = 3;
     std::vector<MgLev> lv;
     std::vector<double> Ainv;   // coarsest dense inverse (row-major)
     int nc = 0;
@@ -843,6 +850,7 @@ Pmg* pmg_build(const Bsr& Af, const Grid& g, const int* deg, int ndeg, int min_c
     Pmg* P = new Pmg();
     P->omega = omega; P->pre = pre; P->post = post;
     int D = g.D;
+    P->D = D;
     for (int l = 0; l < ndeg; ++l) {
         MgLev L;
         int bs = basis_count(D, deg[l]);
@@ -950,8 +958,61 @@ Pmg* pmg_build(const Bsr& Af, const Grid& g, const int* deg, int ndeg, int min_c
     return P;
 }
 
+void pmg_set_schwarz(Pmg& P, bool on) {
+    for (size_t l = 0; l + 1 < P.lv.size(); ++l) {
+        MgLev& L = P.lv[l];
+        L.sw_cmax = 0;
+        L.sw_children.clear();
+        L.sw_inv.clear();
+        if (!on || L.kind != 1) continue;
+        const int bs = L.A.bs, cmax = 1 << P.D, BA = cmax * bs;
+        if (BA > 64) continue;   // as the product: such levels keep block-Jacobi
+        const int Jc = P.lv[l + 1].A.nrows;
+        L.sw_children.assign(Jc, {});
+        for (int c = 0; c < L.A.nrows; ++c) L.sw_children[L.parent[c]].push_back(c);
+        L.sw_inv.assign((size_t)Jc * BA * BA, 0.0);
+        for (int I = 0; I < Jc; ++I) {
+            const std::vector<int>& ch = L.sw_children[I];
+            double* B = &L.sw_inv[(size_t)I * BA * BA];
+            for (int s = 0; s < cmax; ++s)
+                for (int u = 0; u < cmax; ++u)
+                    for (int i = 0; i < bs; ++i)
+                        for (int j = 0; j < bs; ++j) {
+                            double v = (s == u && i == j) ? 1.0 : 0.0;
+                            if (s < (int)ch.size() && u < (int)ch.size()) {
+                                v = 0.0;
+                                for (int64_t p = L.A.rowptr[ch[s]]; p < L.A.rowptr[ch[s] + 1]; ++p)
+                                    if (L.A.col[p] == ch[u]) v = L.A.val[(size_t)p * bs * bs + i * bs + j];
+                            }
+                            B[(size_t)(s * bs + i) * BA + u * bs + j] = v;
+                        }
+            if (!invert_block(B, BA)) throw std::runtime_error("singular Schwarz block");
+        }
+        L.sw_cmax = cmax;
+    }
+}
+
 static void bgemv(const MgLev& L, const double* r, double* x, double omega, bool add) {
     int bs = L.A.bs;
+    if (L.sw_cmax > 0) {
+        const int BA = L.sw_cmax * bs;
+        std::vector<double> rr(BA);
+        for (size_t I = 0; I < L.sw_children.size(); ++I) {
+            const std::vector<int>& ch = L.sw_children[I];
+            std::fill(rr.begin(), rr.end(), 0.0);
+            for (size_t s = 0; s < ch.size(); ++s)
+                for (int i = 0; i < bs; ++i) rr[s * bs + i] = r[(size_t)ch[s] * bs + i];
+            const double* B = &L.sw_inv[I * BA * BA];
+            for (size_t s = 0; s < ch.size(); ++s)
+                for (int i = 0; i < bs; ++i) {
+                    double v = 0;
+                    for (int c = 0; c < BA; ++c) v += B[(s * bs + i) * BA + c] * rr[c];
+                    double& xo = x[(size_t)ch[s] * bs + i];
+                    xo = (add ? xo : 0.0) + omega * v;
+                }
+        }
+        return;
+    }
     for (int row = 0; row < L.A.nrows; ++row) {
         const double* Di = &L.Dinv[(size_t)row * bs * bs];
         for (int i = 0; i < bs; ++i) {
@@ -1363,6 +1424,14 @@ void orc_pmg_level_spmv(void* p, int l, const double* x, double* y) {
 void orc_pmg_set(void* p, double omega, int pre, int post) {
     Pmg* P = (Pmg*)p;
     P->omega = omega; P->pre = pre; P->post = post;
+}
+// one smoother application on level l: x = omega M_l^-1 r
+void orc_pmg_smooth(void* p, int l, const double* r, double* x) {
+    Pmg* P = (Pmg*)p;
+    bgemv(P->lv[l], r, x, P->omega, false);
+}
+int orc_pmg_set_schwarz(void* p, int on) {
+    try { pmg_set_schwarz(*(Pmg*)p, on != 0); return 0; } catch (...) { return -1; }
 }
 // one V-cycle from a zero guess: x = V(b)
 int orc_pmg_vcycle(void* mp, void* gp, const int* deg, int ndeg, int min_coarse, double omega, int pre, int post,

@@ -49,6 +49,28 @@ def test_vcycle_matches_oracle(ctx, matrix_free, degrees):
     assert rel(out, ref) < 1e-11
 
 
+@pytest.mark.parametrize("matrix_free", [False, True])
+@pytest.mark.parametrize("prec,tol", [(B.BH_FP64, 1e-11), (B.BH_FP32, 1e-4)])
+def test_vcycle_schwarz_matches_oracle(ctx, matrix_free, prec, tol):
+    """Aggregate-block additive Schwarz smoothing on the k = 1 level and the h-levels
+    (AS/Schwarz.cs with multigrid-aggregate blocks, no overlap): device V-cycle vs the
+    oracle cycle with the same smoother."""
+    _say(f"case vcycle schwarz matrix_free={matrix_free} prec={prec}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(5).standard_normal(oo.N)
+    P = O.Pmg(Ao, og, [3, 1], min_coarse=8, omega=0.7, pre=2, post=1)
+    P.set_schwarz(True)
+    ref = P.vcycle(b)
+    mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, omega=0.7, pre=2, post=1, kcycle_inner=0,
+                      matrix_free=matrix_free, prec=prec)
+    B.pmg_set_schwarz(mg, True)
+    out = mg.Solve(np.zeros(oo.N), b)
+    assert rel(out, ref) < tol
+    B.pmg_set_schwarz(mg, False)
+    P.set_schwarz(False)
+    assert rel(mg.Solve(np.zeros(oo.N), b), P.vcycle(b)) < tol
+
+
 def _kcycle_numpy(P, level, b, kin):
     """kin flexible-GMRES steps on level `level` (CGS2), V-cycle preconditioned."""
     n = b.size

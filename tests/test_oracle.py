@@ -284,3 +284,56 @@ def test_bjacobi_and_gmres_against_dense():
     Dr = Dinv @ (b - M @ xg)
     assert np.linalg.norm(Dr) < 1e-12 * np.linalg.norm(b) + 1e-12 + 1e-14 * np.linalg.norm(Dinv @ b)
     assert np.linalg.norm(xg - xe) < 1e-6 * np.linalg.norm(xe)
+
+
+def _agg_parent_3d(n):
+    """parent aggregate of each cell of an n[0] x n[1] x n[2] grid (cell i + j nx + k nx ny)."""
+    nc = [(m + 1) // 2 for m in n]
+    par = []
+    for k in range(n[2]):
+        for j in range(n[1]):
+            for i in range(n[0]):
+                par.append(i // 2 + (j // 2) * nc[0] + (k // 2) * nc[0] * nc[1])
+    return np.array(par)
+
+
+def test_schwarz_smoother_against_dense():
+    """Additive Schwarz over the multigrid aggregates, no overlap (AS/Schwarz.cs with
+    MG-aggregate blocks): x = omega sum_I R_I^T (R_I A R_I^T)^-1 R_I r on the k = 1
+    level, against numpy on the dense level matrix (odd grid: partial aggregates)."""
+    nodes = nodes_3d(5, 4, 3, jitter=0.3, seed=8)
+    g = O.Grid(*nodes)
+    op = O.Sip(g, 1, log_uniform_nu(g.J, 8))
+    A, _ = op.assemble()
+    P = O.Pmg(A, g, [1], min_coarse=4, omega=0.7, pre=1, post=1)
+    P.set_schwarz(True)
+    M = A.dense()
+    par = _agg_parent_3d([5, 4, 3])
+    r = np.random.default_rng(4).standard_normal(op.N)
+    x = np.zeros(op.N)
+    Np = op.Np
+    for I in np.unique(par):
+        dofs = np.concatenate([np.arange(c * Np, (c + 1) * Np) for c in np.flatnonzero(par == I)])
+        x[dofs] = 0.7 * np.linalg.solve(M[np.ix_(dofs, dofs)], r[dofs])
+    xo = P.smooth(0, r)
+    assert np.abs(xo - x).max() < 1e-12 * np.abs(x).max()
+    P.set_schwarz(False)   # back to block-Jacobi: cell blocks
+    xb = np.zeros(op.N)
+    for c in range(g.J):
+        s = slice(c * Np, (c + 1) * Np)
+        xb[s] = 0.7 * np.linalg.solve(M[s, s], r[s])
+    assert np.abs(P.smooth(0, r) - xb).max() < 1e-12 * np.abs(xb).max()
+
+
+def test_schwarz_single_aggregate_is_exact_solve():
+    """A 2x2x2 grid is one aggregate: with omega = 1 the Schwarz smoother is A^-1, so one
+    pre-smoothing sweep already solves and the V-cycle returns A^-1 b."""
+    x = np.linspace(0.0, 1.0, 3)
+    g = O.Grid(x, x, x)
+    op = O.Sip(g, 1, log_uniform_nu(g.J, 2))
+    A, _ = op.assemble()
+    P = O.Pmg(A, g, [1], min_coarse=1, omega=1.0, pre=1, post=0)
+    P.set_schwarz(True)
+    b = np.random.default_rng(1).standard_normal(op.N)
+    xe = np.linalg.solve(A.dense(), b)
+    assert np.abs(P.vcycle(b) - xe).max() < 1e-10 * np.abs(xe).max()
