@@ -245,6 +245,7 @@ static void launch_rhs(EulerOp* e, const R* U, R* out) {
 }
 
 void EulerOp::rhs(const void* U, void* out, int prec) {
+    if (use_mfma && euler_rhs_mfma(this, U, out, prec)) return;
 #define BH_CASE(DD, KK, QQ)                                                                                 \
     if (D == DD && k == KK && nq1 == QQ) {                                                                  \
         if (prec == BH_FP32) launch_rhs<float, DD, KK, QQ>(this, (const float*)U, (float*)out);             \

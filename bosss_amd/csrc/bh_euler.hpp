@@ -4,6 +4,10 @@
 
 namespace bh {
 
+struct EulerOp;
+// MFMA right-hand side (bh_eulermfma.hip); false if (D, k, nodes) has no MFMA kernel
+bool euler_rhs_mfma(EulerOp* e, const void* U, void* out, int prec);
+
 struct EulerOp {
     bh_ctx* ctx;
     bh_grid* g;
@@ -11,6 +15,8 @@ struct EulerOp {
     int64_t N;
     double gamma, MS;
     DevBuf tab64, tab32, work;
+    DevBuf tabm64, tabm32;    // MFMA kernel tables (built on first use)
+    bool use_mfma = true;     // MFMA kernel (16 cells per wave) instead of one wave per cell
     EulerOp(bh_grid* g, int k, int nq1, double gamma, double mach);
     void rhs(const void* U, void* out, int prec);
     void tvd3(void* U, double dt, int steps, int prec);

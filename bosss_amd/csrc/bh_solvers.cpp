@@ -115,6 +115,29 @@ static void cgs2(bh_ctx* ctx, double* w, const double* V, int64_t N, int i, doub
     for (int k = 0; k < i; ++k) hcol[k] = h1[k] + h2[k];
 }
 
+// Classical Gram-Schmidt with selective reorthogonalisation (the Daniel-Gragg-
+// Kaufman-Stewart criterion): w is stored in slot i of V, so one multi-dot pass
+// gives <w, V_k> (k < i) and |w|^2.  After the projection, |w'|^2 = |w|^2 - sum h^2;
+// only when that drops below |w|^2 / 2 (cancellation: w nearly in span V) is a
+// second pass made.  Returns the projected |w| (explicit norm).
+static double cgs_selective(bh_ctx* ctx, double* V, int64_t N, int i, double* hcol, int* passes) {
+    double* w = V + (size_t)i * N;
+    std::vector<double> h1(i + 1), h2(i);
+    multi_dot(ctx, w, V, N, i + 1, N, h1.data());
+    const double w2 = h1[i];
+    multi_axpy(ctx, w, V, N, i, N, h1.data(), -1.0);
+    double proj = 0;
+    for (int k = 0; k < i; ++k) { hcol[k] = h1[k]; proj += h1[k] * h1[k]; }
+    *passes = 1;
+    if (!(w2 - proj > 0.5 * w2)) {
+        multi_dot(ctx, w, V, N, i, N, h2.data());
+        multi_axpy(ctx, w, V, N, i, N, h2.data(), -1.0);
+        for (int k = 0; k < i; ++k) hcol[k] += h2[k];
+        *passes = 2;
+    }
+    return nrm2(ctx, w, N);
+}
+
 static void back_solve(const std::vector<double>& H, int ld, int i, const double* s, double* y) {
     for (int r = i - 1; r >= 0; --r) {
         double a = s[r];
@@ -223,9 +246,9 @@ void GmresSolver::solve_flexible(double* X, const double* B) {
     double* Vp = V.as<double>();
     double* Zp = Z.as<double>();
     double* rp = r.as<double>();
-    double* wp = w.as<double>();
     iterations = 0;
     converged = 0;
+    reorth_passes = 0;
     const double bn = nrm2(ctx, B, N);
     const double target = term.tol * (bn > 0 ? bn : 1.0);
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), s(m + 2), y, hcol(m + 1);
@@ -245,12 +268,14 @@ void GmresSolver::solve_flexible(double* X, const double* B) {
         for (i = 1; i <= m; ++i) {
             iterations++;
             precond(Vp + (size_t)(i - 1) * N, Zp + (size_t)(i - 1) * N);
-            A.apply(Zp + (size_t)(i - 1) * N, wp);
-            cgs2(ctx, wp, Vp, N, i, hcol.data());
+            double* vi = Vp + (size_t)i * N;   // w = A z_{i-1}, orthogonalised in place in slot i
+            A.apply(Zp + (size_t)(i - 1) * N, vi);
+            int passes = 1;
+            const double nw = cgs_selective(ctx, Vp, N, i, hcol.data(), &passes);
+            reorth_passes += passes - 1;
             for (int k = 0; k < i; ++k) H[k * m + i - 1] = hcol[k];
-            const double nw = nrm2(ctx, wp, N);
             H[i * m + i - 1] = nw;
-            if (nw > 0) axpby(ctx, 1.0 / nw, wp, 0.0, Vp + (size_t)i * N, N);
+            if (nw > 0) axpby(ctx, 1.0 / nw, vi, 0.0, vi, N);
             for (int k = 1; k <= i - 1; ++k) {
                 double t = cs[k - 1] * H[(k - 1) * m + i - 1] + sn[k - 1] * H[k * m + i - 1];
                 H[k * m + i - 1] = -sn[k - 1] * H[(k - 1) * m + i - 1] + cs[k - 1] * H[k * m + i - 1];

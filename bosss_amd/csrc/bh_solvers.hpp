@@ -46,6 +46,7 @@ struct GmresSolver : bh_solver {
     Termination term;
     DevBuf V, Z, r, z, w;
     std::vector<double>* history = nullptr;
+    int reorth_passes = 0;   // flexible solve: Arnoldi steps that needed the second Gram-Schmidt pass
     GmresSolver(LinOp A, bh_solver* P, int m, double tol, int maxit, bool flexible);
     void solve_dev(double* x, const double* b) override;
     void solve_flexible(double* x, const double* b);
