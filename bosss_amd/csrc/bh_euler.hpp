@@ -14,9 +14,8 @@ struct EulerOp {
     int D, k, nq1, Np, nv;
     int64_t N;
     double gamma, MS;
-    DevBuf tab64, tab32, work;
+    DevBuf work;
     DevBuf tabm64, tabm32;    // MFMA kernel tables (built on first use)
-    bool use_mfma = true;     // MFMA kernel (16 cells per wave) instead of one wave per cell
     EulerOp(bh_grid* g, int k, int nq1, double gamma, double mach);
     void rhs(const void* U, void* out, int prec);
     void tvd3(void* U, double dt, int steps, int prec);
