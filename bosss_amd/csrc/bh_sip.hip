@@ -93,7 +93,7 @@ __device__ __forceinline__ void face_term(const SipArgs<R>& a, int f, const R* u
     const R* __restrict__ NP = a.Nn + (size_t)f * NQF * Np;
     const R* __restrict__ TQ = a.T + (size_t)fQ * NQF * Np;
     const R* __restrict__ NQ = a.Nn + (size_t)fQ * NQF * Np;
-    const R nuMax = (fabs((double)nuP) > fabs((double)nuQ)) ? nuP : nuQ;
+    const R nuMax = (fabs(nuP) > fabs(nuQ)) ? nuP : nuQ;
 #pragma unroll (Np <= 10 ? NQF : 1)
     for (int q = 0; q < NQF; ++q) {
         R vP = 0, dP = 0, vQ = 0, dQ = 0;
@@ -171,18 +171,24 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     R detJ = 1;
     for (int d = 0; d < D; ++d) detJ *= R(0.5) * h[d];
     const R nuP = a.nu[c];
-    R s2[D];
-    for (int d = 0; d < D; ++d) s2[d] = R(4) / (h[d] * h[d]);
+    // reciprocal widths once; cj = sum_d (1 + boundary faces_d / 2) / h_d (the cell_cj formula),
+    // and the neighbour's cj differs from P's only in direction d
+    R ih[3] = {1, 1, 1}, s2[D], cjP = 0;
+    for (int d = 0; d < D; ++d) {
+        ih[d] = R(1) / h[d];
+        s2[d] = R(4) * ih[d] * ih[d];
+        cjP += (R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
+    }
     volume_term<D, K, R>(a, uP, s2, nuP, y);
-    const R cjP = cell_cj<D>(a, idx, h);
     const R sdPinv = R(1) / sqrt(detJ);
     const R pen = R(a.pen);
 #pragma unroll (Np <= 4 ? 2 * D : 1)
     for (int f = 0; f < 2 * D; ++f) {
         const int d = f >> 1, s = f & 1;
         const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
-        const R dA = detJ / (R(0.5) * h[d]);
-        const R gP = R(2) / h[d];
+        const R ihd = d == 0 ? ih[0] : (d == 1 ? ih[1] : ih[2]);
+        const R dA = R(2) * detJ * ihd;
+        const R gP = R(2) * ihd;
         if (isBnd) {
             if ((a.bc_neumann >> f) & 1) continue;
             R uQ[1] = {0};
@@ -191,15 +197,16 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
             int nidx[3] = {idx[0], idx[1], idx[2]};
             nidx[d] += s ? 1 : -1;
             const int cq = encode<D>(nidx, a.n0, a.n1);
-            R hq[3] = {h[0], h[1], h[2]};
-            hq[d] = R(a.width[d * a.nmax + nidx[d]]);
-            const R detQ = detJ / h[d] * hq[d];
-            const R cjQ = cell_cj<D>(a, nidx, hq);
+            const R hd = d == 0 ? h[0] : (d == 1 ? h[1] : h[2]);
+            const R ihq = R(1) / R(a.width[d * a.nmax + nidx[d]]);
+            const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
+            const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+            const R cjQ = cjP - bP * ihd + bQ * ihq;
             R uQ[Np];
             const R* uq = a.u + (size_t)cq * Np;
 #pragma unroll
             for (int n = 0; n < Np; ++n) uQ[n] = uq[n];
-            face_term<D, K, R, true, true>(a, f, uP, uQ, sdPinv, R(1) / sqrt(detQ), gP, R(2) / hq[d], dA,
+            face_term<D, K, R, true, true>(a, f, uP, uQ, sdPinv, sdPinv * sqrt(hd * ihq), gP, R(2) * ihq, dA,
                                            nuP, a.nu[cq], pen * (cjP > cjQ ? cjP : cjQ), false, y);
         }
     }
