@@ -93,16 +93,25 @@ __device__ __forceinline__ int enc64(const int* idx, int n0, int n1) {
 }
 
 #define BH_MFMA64(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+// 4 blocks of 4x4x4: lane 16k + 4b + i holds A_b[i][k] (the same lanes as the 16x16x4 A
+// operand of row 4b + i, slot k), lane 16k + 4b + j holds B_b[k][j], and lane 16g + 4b + j
+// receives D_b[g][j].  Used for output modes 16..19 (cells 4b..4b+3 per block): a quarter of
+// the work of a 16-column tile that would be three-quarters padding.
+#define BH_MFMA4(a, b, c) __builtin_amdgcn_mfma_f64_4x4x4f64((a), (b), (c), 0, 0, 0)
 
 template <int D, int K, int W>
 __global__ __launch_bounds__(64 * W) void sip_apply_mfma64_kernel(M64Args a) {
     using L = M64Layout<D, K>;
     constexpr int Np = L::Np, NK = L::NK, LD = L::LD;
+    constexpr int TAIL = (Np > 16 && Np <= 20) ? Np - 16 : 0;   // modes 16.. on the 4x4x4 blocks
+    constexpr int MTF = TAIL ? 1 : L::MT;                         // full 16-column tiles
     __shared__ double tab[L::TOTAL];
     for (int i = threadIdx.x; i < L::TOTAL; i += blockDim.x) tab[i] = a.tab[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, grp = lane >> 4;
+    const int tcol = 16 + (col & 3);           // tail mode of this lane's B operand and output
+    const bool tlive = TAIL > 0 && tcol < Np;
     const int nn[3] = {a.n0, a.n1, a.n2};
     const int ntiles = (a.J + 15) / 16;
     const int xcd = blockIdx.x & 7, nb_x = gridDim.x >> 3;
@@ -123,9 +132,10 @@ __global__ __launch_bounds__(64 * W) void sip_apply_mfma64_kernel(M64Args a) {
             const int n = 4 * s + grp;
             bU[s] = (valid && n < Np) ? a.u[(size_t)cc * Np + n] : 0.0;
         }
-        d4_t yacc[L::MT];
+        d4_t yacc[MTF];
 #pragma unroll
-        for (int mt = 0; mt < L::MT; ++mt) yacc[mt] = z;
+        for (int mt = 0; mt < MTF; ++mt) yacc[mt] = z;
+        double ytail = 0;
         // ---------------- volume: Y += sum_d diag(-nu 4/h_d^2) U Kv[d]  (A = scaled modes, B = Kv[d])
 #pragma unroll 1
         for (int d = 0; d < D; ++d) {
@@ -134,9 +144,10 @@ __global__ __launch_bounds__(64 * W) void sip_apply_mfma64_kernel(M64Args a) {
 #pragma unroll
             for (int s = 0; s < NK; ++s) {
                 const double av = cd * bU[s];
+                const int krow = L::KV + (d * L::NPK + 4 * s + grp) * L::LDK;
 #pragma unroll
-                for (int mt = 0; mt < L::MT; ++mt)
-                    yacc[mt] = BH_MFMA64(av, tab[L::KV + (d * L::NPK + 4 * s + grp) * L::LDK + col + 16 * mt], yacc[mt]);
+                for (int mt = 0; mt < MTF; ++mt) yacc[mt] = BH_MFMA64(av, tab[krow + col + 16 * mt], yacc[mt]);
+                if (TAIL) ytail = BH_MFMA4(av, tlive ? tab[krow + tcol] : 0.0, ytail);
             }
         }
         // ---------------- faces
@@ -195,19 +206,34 @@ __global__ __launch_bounds__(64 * W) void sip_apply_mfma64_kernel(M64Args a) {
                 }
                 const double wq = (skip || !valid) ? 0.0 : tab[L::FW + q] * wf;
                 const double cvw = wq * cv, cdw = wq * sgn * gP * cd;
+                const int trow = L::F + ((f * 2 + 0) * 16 + q) * LD, nrow = L::F + ((f * 2 + 1) * 16 + q) * LD;
 #pragma unroll
-                for (int mt = 0; mt < L::MT; ++mt) {
+                for (int mt = 0; mt < MTF; ++mt) {
                     const int m = col + 16 * mt;
-                    const double Tv = m < Np ? tab[L::F + ((f * 2 + 0) * 16 + q) * LD + m] : 0.0;
-                    const double Nv = m < Np ? tab[L::F + ((f * 2 + 1) * 16 + q) * LD + m] : 0.0;
+                    const double Tv = m < Np ? tab[trow + m] : 0.0;
+                    const double Nv = m < Np ? tab[nrow + m] : 0.0;
                     yacc[mt] = BH_MFMA64(cvw, Tv, yacc[mt]);
                     yacc[mt] = BH_MFMA64(cdw, Nv, yacc[mt]);
+                }
+                if (TAIL) {
+                    ytail = BH_MFMA4(cvw, tlive ? tab[trow + tcol] : 0.0, ytail);
+                    ytail = BH_MFMA4(cdw, tlive ? tab[nrow + tcol] : 0.0, ytail);
                 }
             }
         }
         // ---------------- output: lane holds Y[cell tile*16 + grp + 4r][m = col + 16*mt]
+        // and, for the tail, Y[cell tile*16 + 4*(col/4) + grp][m = 16 + col%4]
+        if (TAIL && tlive) {
+            const int co = tile * 16 + 4 * (col >> 2) + grp;
+            if (co < a.J) {
+                const size_t o = (size_t)co * Np + tcol;
+                double v = ytail;
+                if (a.affine) v += a.affine[o];
+                a.y[o] = a.beta == 0.0 ? a.alpha * v : a.beta * a.y[o] + a.alpha * v;
+            }
+        }
 #pragma unroll
-        for (int mt = 0; mt < L::MT; ++mt) {
+        for (int mt = 0; mt < MTF; ++mt) {
             const int m = col + 16 * mt;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
