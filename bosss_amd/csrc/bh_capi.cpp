@@ -439,8 +439,8 @@ int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
     BH_CHECK(op, "bh_op_set_fp64_path: null operator");
     BH_CHECK(path >= 0 && path <= 2, "bh_op_set_fp64_path: path must be 0 (autotune), 1 (vector) or 2 (MFMA)");
     if (path == 2) {
-        BH_CHECK(bh::sip_apply_mfma64_supported(op->D, op->k),
-                 "bh_op_set_fp64_path: the FP64-MFMA apply supports D in {2, 3} and k in {2, 3}");
+        BH_CHECK(bh::sip_apply_mfma64_supported(op->D, op->k, op->N),
+                 "bh_op_set_fp64_path: the FP64-MFMA apply supports D in {2, 3}, k in {2, 3} and vectors below 4 GiB");
         BH_CHECK(waves == 4 || waves == 8, "bh_op_set_fp64_path: waves must be 4 or 8");
         BH_CHECK(wg_per_xcd >= 1 && wg_per_xcd <= 4096, "bh_op_set_fp64_path: wg_per_xcd out of range");
         op->f64_waves = waves;

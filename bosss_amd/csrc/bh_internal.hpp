@@ -196,7 +196,7 @@ void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, boo
 // FP32 vectors, FP16 MFMA contractions with FP32 accumulation
 void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine);
 // FP64 vectors, FP64 MFMA contractions (same results as the vector kernel up to summation order)
-bool sip_apply_mfma64_supported(int D, int K);
+bool sip_apply_mfma64_supported(int D, int K, int64_t N);   // N: DOF (arrays below 4 GiB)
 void sip_apply_mfma64(bh_op* op, double alpha, const double* u, double beta, double* y, const double* affine, int waves,
                       int per_xcd);
 void sip_assemble(bh_op* op, bh_bsr* M);

@@ -351,7 +351,7 @@ static void dispatch_apply(bh_op* op, const SipArgs<R>& a) {
 // operator and keeps the fastest; both compute the same quadrature.
 static void tune_fp64_path(bh_op* op) {
     op->f64_path = 1;
-    if (!sip_apply_mfma64_supported(op->D, op->k)) return;
+    if (!sip_apply_mfma64_supported(op->D, op->k, op->N)) return;
     DevBuf us(op->N * sizeof(double)), ys(op->N * sizeof(double));
     fill(op->ctx, us.as<double>(), 1.0, op->N);
     hipEvent_t e0, e1;
