@@ -178,6 +178,97 @@ void invert_dense_blocks(bh_ctx* ctx, int nblocks, int n, const double* blocks, 
     BH_CHECK(h == 0, "dense block inversion: singular block " + std::to_string(h - 1));
 }
 
+// Gauss-Jordan inverse with partial pivoting of one dense row-major n x n matrix
+// (the coarsest multigrid level, n up to a few thousand): per column c one pivot
+// search, one row swap + scale of the pivot row (both single-workgroup), and one
+// grid-wide rank-1 elimination of every other row of [A | Inv].  The host never
+// waits between columns; a zero pivot is flagged and reported at the end.
+__global__ __launch_bounds__(1024) void gj_pivot_k(const double* __restrict__ A, int n, int c, int* __restrict__ st) {
+    __shared__ double bv[1024];
+    __shared__ int bi[1024];
+    double best = -1.0;
+    int bidx = n;
+    for (int r = c + (int)threadIdx.x; r < n; r += blockDim.x) {
+        const double v = fabs(A[(size_t)r * n + c]);
+        if (v > best) { best = v; bidx = r; }   // rows ascend per thread: the first maximum wins
+    }
+    bv[threadIdx.x] = best;
+    bi[threadIdx.x] = bidx;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const double o = bv[threadIdx.x + off];
+            const int oi = bi[threadIdx.x + off];
+            if (o > bv[threadIdx.x] || (o == bv[threadIdx.x] && oi < bi[threadIdx.x])) {
+                bv[threadIdx.x] = o;
+                bi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        st[1] = bi[0];
+        if (!(bv[0] > 0.0) && st[0] == 0) st[0] = c + 1;
+    }
+}
+
+// swap rows c and p of A and Inv, scale row c by 1 / pivot, keep column c (the
+// elimination factors) in fcol with a zero at row c
+__global__ __launch_bounds__(1024) void gj_swap_scale_k(double* __restrict__ A, double* __restrict__ Inv, int n, int c,
+                                                        const int* __restrict__ st, double* __restrict__ fcol) {
+    const int p = st[1];
+    const double pv = A[(size_t)p * n + c], cv = A[(size_t)c * n + c];
+    const double rp = st[0] ? 0.0 : 1.0 / pv;
+    // factors first (rows other than c and p are not written by this kernel; row p receives row c)
+    for (int r = threadIdx.x; r < n; r += blockDim.x) fcol[r] = r == c ? 0.0 : (r == p ? cv : A[(size_t)r * n + c]);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * n; k += blockDim.x) {
+        double* M = k < n ? A : Inv;
+        const int kk = k < n ? k : k - n;
+        const double vp = M[(size_t)p * n + kk], vc = M[(size_t)c * n + kk];
+        M[(size_t)c * n + kk] = vp * rp;
+        if (p != c) M[(size_t)p * n + kk] = vc;
+    }
+}
+
+// [A | Inv][r][:] -= fcol[r] * [A | Inv][c][:] for r != c (row c is read-only here)
+__global__ __launch_bounds__(256) void gj_eliminate_k(double* __restrict__ A, double* __restrict__ Inv, int n, int c,
+                                                      const double* __restrict__ fcol) {
+    const int r = blockIdx.y;
+    const double f = fcol[r];
+    if (f == 0.0) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * n; k += gridDim.x * blockDim.x) {
+        double* M = k < n ? A : Inv;
+        const int kk = k < n ? k : k - n;
+        M[(size_t)r * n + kk] -= f * M[(size_t)c * n + kk];
+    }
+}
+
+__global__ void gj_identity_k(double* __restrict__ I, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)n * n) I[i] = (i / n == i % n) ? 1.0 : 0.0;
+}
+
+void dense_inverse_dev(bh_ctx* ctx, double* A, double* Inv, int n) {
+    BH_CHECK(n >= 1 && n <= 16384, "dense inverse: n out of range");
+    DevBuf st(2 * sizeof(int)), fcol((size_t)n * sizeof(double));
+    const int zero2[2] = {0, 0};
+    ctx->upload(st.p, zero2, sizeof(zero2));
+    const int64_t nn = (int64_t)n * n;
+    hipLaunchKernelGGL(gj_identity_k, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, ctx->stream, Inv, n);
+    BH_HIP(hipGetLastError());
+    const dim3 eg((unsigned)std::min<int64_t>((2 * (int64_t)n + 255) / 256, 64), (unsigned)n);
+    for (int c = 0; c < n; ++c) {
+        hipLaunchKernelGGL(gj_pivot_k, dim3(1), dim3(1024), 0, ctx->stream, A, n, c, st.as<int>());
+        hipLaunchKernelGGL(gj_swap_scale_k, dim3(1), dim3(1024), 0, ctx->stream, A, Inv, n, c, st.as<int>(), fcol.as<double>());
+        hipLaunchKernelGGL(gj_eliminate_k, eg, dim3(256), 0, ctx->stream, A, Inv, n, c, fcol.as<double>());
+    }
+    BH_HIP(hipGetLastError());
+    int h[2] = {0, 0};
+    ctx->download(h, st.p, sizeof(h));
+    BH_CHECK(h[0] == 0, "multigrid: singular coarsest-level operator (zero pivot in column " + std::to_string(h[0] - 1) + ")");
+}
+
 // ------------------------------------------------------------ block Jacobi
 // One fused sweep: xout = xin + omega * Dinv (b - A xin).  Algebraically the
 // reference's update x = omega*Dinv*(b - A x + D x) + (1-omega) x

@@ -36,6 +36,8 @@ void gather_agg_blocks(bh_ctx*, int nagg, int bs, int cmax, const int* cptr, con
 template <typename T> void schwarz_apply(bh_ctx*, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const T* Ainv,
                                          const T* r, T* x, double omega, bool add);
 void invert_dense_blocks(bh_ctx*, int nblocks, int n, const double* blocks, double* inv64, float* inv32);
+// Gauss-Jordan inverse (partial pivoting) of a dense row-major n x n device matrix; A is overwritten
+void dense_inverse_dev(bh_ctx*, double* A, double* Inv, int n);
 void kcycle_column(bh_ctx*, double* S, const KcycleScalars& L, int j);
 void kcycle_lsq(bh_ctx*, double* S, const KcycleScalars& L);
 template <typename T> void guarded_step(bh_ctx*, const T* e, T* x, int64_t n, double* st, int* cnt);

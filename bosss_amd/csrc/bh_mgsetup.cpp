@@ -92,28 +92,6 @@ static bh_bsr* make_bsr(bh_ctx* ctx, int nrows, int bs, const std::vector<int64_
     return M;
 }
 
-// Gauss-Jordan inverse of a dense row-major matrix (coarsest level)
-static void dense_inverse(std::vector<double>& A, int n) {
-    std::vector<double> I(n * (size_t)n, 0.0);
-    for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1;
-    for (int c = 0; c < n; ++c) {
-        int p = c;
-        for (int r = c + 1; r < n; ++r) if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)p * n + c])) p = r;
-        BH_CHECK(A[(size_t)p * n + c] != 0.0, "multigrid: singular coarsest-level operator");
-        if (p != c)
-            for (int k = 0; k < n; ++k) { std::swap(A[(size_t)p * n + k], A[(size_t)c * n + k]); std::swap(I[(size_t)p * n + k], I[(size_t)c * n + k]); }
-        const double piv = 1.0 / A[(size_t)c * n + c];
-        for (int k = 0; k < n; ++k) { A[(size_t)c * n + k] *= piv; I[(size_t)c * n + k] *= piv; }
-        for (int r = 0; r < n; ++r) {
-            if (r == c) continue;
-            const double f = A[(size_t)r * n + c];
-            if (f == 0.0) continue;
-            for (int k = 0; k < n; ++k) { A[(size_t)r * n + k] -= f * A[(size_t)c * n + k]; I[(size_t)r * n + k] -= f * I[(size_t)c * n + k]; }
-        }
-    }
-    A.swap(I);
-}
-
 static int cell_of(int D, const int* n, int i, int j, int k) {
     return D == 2 ? i * n[1] + j : i + j * n[0] + k * n[0] * n[1];
 }
@@ -370,9 +348,12 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
                         dense[(size_t)(r * bs + i) * ncoarse + c * bs + j] = v[(size_t)p * bs * bs + (size_t)j * bs + i];
             }
     }
-    dense_inverse(dense, ncoarse);
-    Ainv64.alloc(dense.size() * sizeof(double));
-    ctx->upload(Ainv64.p, dense.data(), dense.size() * sizeof(double));
+    {
+        DevBuf Ad(dense.size() * sizeof(double));
+        ctx->upload(Ad.p, dense.data(), dense.size() * sizeof(double));
+        Ainv64.alloc(dense.size() * sizeof(double));
+        dense_inverse_dev(ctx, Ad.as<double>(), Ainv64.as<double>(), ncoarse);
+    }
     ctx->sync();
 }
 
