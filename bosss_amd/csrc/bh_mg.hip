@@ -33,6 +33,35 @@ __global__ void extract_subblocks_k(int64_t nnzb, int bsf, int bsc, const double
     }
 }
 
+// Galerkin coarse blocks by gathering (fixed summation order, no atomics):
+// Ac[q](m, n) = sum over the fine blocks p of q of sum_{r,c} P_i[r][m] A_p(r, c) P_j[c][n]
+// (i, j: fine row and column of p; P row-major per fine cell; blocks column-major)
+__global__ __launch_bounds__(256) void galerkin_gather_k(int64_t nq, int bs, const int64_t* __restrict__ qptr,
+                                                         const int* __restrict__ plist, const int* __restrict__ prow,
+                                                         const int* __restrict__ fcol, const double* __restrict__ fv,
+                                                         const double* __restrict__ P, double* __restrict__ cv) {
+    const int E = bs * bs;
+    const int64_t tot = nq * E;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = t / E;
+        const int e = (int)(t - q * E);
+        const int n = e / bs, m = e - n * bs;
+        double s = 0;
+        for (int64_t k = qptr[q]; k < qptr[q + 1]; ++k) {
+            const int64_t p = plist[k];
+            const double* A = fv + p * E;
+            const double* Pi = P + (int64_t)prow[k] * E;
+            const double* Pj = P + (int64_t)fcol[p] * E;
+            for (int c = 0; c < bs; ++c) {
+                double t1 = 0;
+                for (int r = 0; r < bs; ++r) t1 += Pi[r * bs + m] * A[c * bs + r];   // (P_i^T A_p)(m, c)
+                s += t1 * Pj[c * bs + n];
+            }
+        }
+        cv[t] = s;
+    }
+}
+
 template <typename T>
 __global__ void p_restrict_k(int64_t J, int bsf, int bsc, const T* __restrict__ rf, T* __restrict__ rc) {
     const int64_t tot = J * bsc;
@@ -311,6 +340,12 @@ void kcycle_lsq(bh_ctx* ctx, double* S, const KcycleScalars& L) {
 template <typename T>
 void guarded_step(bh_ctx* ctx, const T* e, T* x, int64_t n, double* st, int* cnt) {
     hipLaunchKernelGGL((guarded_step_k<T>), grid1d(n), dim3(256), 0, ctx->stream, e, x, n, st, cnt);
+    BH_HIP(hipGetLastError());
+}
+void galerkin_gather(bh_ctx* ctx, int64_t nq, int bs, const int64_t* qptr, const int* plist, const int* prow,
+                     const int* fcol, const double* fv, const double* P, double* cv) {
+    hipLaunchKernelGGL(galerkin_gather_k, grid1d(nq * bs * bs), dim3(256), 0, ctx->stream, nq, bs, qptr, plist, prow, fcol, fv,
+                       P, cv);
     BH_HIP(hipGetLastError());
 }
 void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc) {

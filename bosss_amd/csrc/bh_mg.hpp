@@ -9,6 +9,9 @@ namespace bh {
 struct Mlp;
 
 void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc);
+// coarse Galerkin blocks: cv[q] = sum_{k in qptr[q]..qptr[q+1]} P_{prow[k]}^T fv[plist[k]] P_{fcol[plist[k]]}
+void galerkin_gather(bh_ctx* ctx, int64_t nq, int bs, const int64_t* qptr, const int* plist, const int* prow, const int* fcol,
+                     const double* fv, const double* P, double* cv);
 template <typename T> void p_restrict(bh_ctx*, int64_t J, int bsf, int bsc, const T* rf, T* rc);
 template <typename T> void p_prolong_add(bh_ctx*, int64_t J, int bsf, int bsc, const T* xc, T* xf);
 template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr, const int* cidx, const T* P, const T* rf, T* rc);

@@ -242,46 +242,51 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
             std::vector<int> fillp(cptr.begin(), cptr.end() - 1);
             for (int c = 0; c < F.J; ++c) cidx[fillp[parent[c]]++] = c;
         }
-        // Galerkin A_c = P^T A P (blocks: Ac[I][J] += P_i^T A_ij P_j), row-major blocks on the host
-        std::vector<double> fv((size_t)F.A->nnzb * bs * bs);
-        ctx->download(fv.data(), F.A->vals.p, fv.size() * sizeof(double));   // column-major blocks
-        std::vector<std::map<int, std::vector<double>>> rows(Jc);
-        std::vector<double> t1(bs * bs), t2(bs * bs);
-        for (int i = 0; i < F.J; ++i) {
-            const double* Pi = &P[(size_t)i * bs * bs];
-            for (int64_t p = F.A->h_rowptr[i]; p < F.A->h_rowptr[i + 1]; ++p) {
-                const int j = F.A->h_col[p];
-                const double* Aij = &fv[(size_t)p * bs * bs];   // [col][row]
-                const double* Pj = &P[(size_t)j * bs * bs];
-                // t1 = A_ij P_j  (row-major)
-                for (int r = 0; r < bs; ++r)
-                    for (int n = 0; n < bs; ++n) {
-                        double s = 0;
-                        for (int c = 0; c < bs; ++c) s += Aij[(size_t)c * bs + r] * Pj[c * bs + n];
-                        t1[r * bs + n] = s;
-                    }
-                // t2 = P_i^T t1
-                for (int m = 0; m < bs; ++m)
-                    for (int n = 0; n < bs; ++n) {
-                        double s = 0;
-                        for (int r = 0; r < bs; ++r) s += Pi[r * bs + m] * t1[r * bs + n];
-                        t2[m * bs + n] = s;
-                    }
-                auto& blk = rows[parent[i]][parent[j]];
-                if (blk.empty()) blk.assign(bs * bs, 0.0);
-                for (int t = 0; t < bs * bs; ++t) blk[t] += t2[t];
-            }
-        }
+        // Galerkin A_c = P^T A P (blocks: Ac[I][J] = sum P_i^T A_ij P_j over the children i of I,
+        // j of J).  Host: the coarse sparsity and, per coarse block, the list of its fine
+        // blocks (integer work); device: the block products, gathered in that fixed order.
         std::vector<int64_t> rp(Jc + 1, 0);
         std::vector<int> col;
-        std::vector<double> val;
-        for (int I = 0; I < Jc; ++I) {
-            for (auto& kv : rows[I]) {
-                col.push_back(kv.first);
-                for (int c = 0; c < bs; ++c)          // to column-major
-                    for (int r = 0; r < bs; ++r) val.push_back(kv.second[r * bs + c]);
+        const std::vector<int64_t>& frp = F.A->h_rowptr;
+        const std::vector<int>& fcl = F.A->h_col;
+        std::vector<int> qof((size_t)F.A->nnzb);
+        {
+            std::vector<int> mark(Jc, -1), pos(Jc), rowc;
+            for (int I = 0; I < Jc; ++I) {
+                rowc.clear();
+                for (int t = cptr[I]; t < cptr[I + 1]; ++t) {
+                    const int i = cidx[t];
+                    for (int64_t p = frp[i]; p < frp[i + 1]; ++p) {
+                        const int Jb = parent[fcl[p]];
+                        if (mark[Jb] != I) { mark[Jb] = I; rowc.push_back(Jb); }
+                    }
+                }
+                std::sort(rowc.begin(), rowc.end());
+                for (size_t t = 0; t < rowc.size(); ++t) pos[rowc[t]] = (int)t;
+                for (int t = cptr[I]; t < cptr[I + 1]; ++t) {
+                    const int i = cidx[t];
+                    for (int64_t p = frp[i]; p < frp[i + 1]; ++p) qof[p] = (int)(rp[I] + pos[parent[fcl[p]]]);
+                }
+                col.insert(col.end(), rowc.begin(), rowc.end());
+                rp[I + 1] = (int64_t)col.size();
             }
-            rp[I + 1] = (int64_t)col.size();
+        }
+        const int64_t nq = rp[Jc];
+        std::vector<int64_t> qptr(nq + 1, 0);
+        for (int64_t p = 0; p < F.A->nnzb; ++p) qptr[qof[p] + 1]++;
+        for (int64_t q = 0; q < nq; ++q) qptr[q + 1] += qptr[q];
+        std::vector<int> plist((size_t)F.A->nnzb), prow((size_t)F.A->nnzb);
+        {
+            std::vector<int64_t> fillq(qptr.begin(), qptr.end() - 1);
+            for (int I = 0; I < Jc; ++I)                       // children in order, then their blocks
+                for (int t = cptr[I]; t < cptr[I + 1]; ++t) {
+                    const int i = cidx[t];
+                    for (int64_t p = frp[i]; p < frp[i + 1]; ++p) {
+                        const int64_t k = fillq[qof[p]]++;
+                        plist[k] = (int)p;
+                        prow[k] = i;
+                    }
+                }
         }
         F.kind = 1;
         F.cptr.alloc(cptr.size() * sizeof(int));
@@ -304,7 +309,15 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         C.h = hc;
         C.A = make_bsr(ctx, Jc, bs, rp, col);
         C.ownA = true;
-        ctx->upload(C.A->vals.p, val.data(), val.size() * sizeof(double));
+        {
+            DevBuf dq((nq + 1) * sizeof(int64_t)), dl(plist.size() * sizeof(int)), dr(prow.size() * sizeof(int));
+            ctx->upload(dq.p, qptr.data(), qptr.size() * sizeof(int64_t));
+            ctx->upload(dl.p, plist.data(), plist.size() * sizeof(int));
+            ctx->upload(dr.p, prow.data(), prow.size() * sizeof(int));
+            galerkin_gather(ctx, nq, bs, dq.as<int64_t>(), dl.as<int>(), dr.as<int>(), F.A->col.as<int>(),
+                            F.A->vals.as<double>(), F.P64.as<double>(), C.A->vals.as<double>());
+            ctx->sync();
+        }
         lv.push_back(std::move(C));
     }
     // smoother inverses and work vectors
