@@ -420,28 +420,51 @@ void axpby32(bh_ctx* ctx, float a, const float* x, float b, float* y, int64_t n)
 }
 
 // ------------------------------------------------------------ reductions
-// Multi-dot h[k] = <w, V_k>, k < m (m <= 16 per launch): each thread keeps m
-// partial sums, a wave64 butterfly (__shfl_xor) reduces them, the 4 waves of
-// a block combine in LDS, and a second kernel sums the per-block partials in
-// a fixed order (deterministic, no atomics).
+// Multi-dot h[k] = <w, V_k>, k < M (M <= 16 per launch, a template parameter):
+// each thread keeps M partial sums over 16-byte vector loads, a wave64
+// butterfly (__shfl_xor) and LDS combine them per workgroup, and a second
+// kernel sums the per-workgroup partials in a fixed order (deterministic, no
+// atomics).  (A last-workgroup-finishes variant in one launch needs a
+// device-scope fence per workgroup, which costs more than the second launch.)
 constexpr int MD_BLOCKS = 1024;
 constexpr int MD_M = 16;
 
-// T: vector precision; partial sums are accumulated in double.
-template <typename T, int M>
-__global__ __launch_bounds__(256) void multidot_k(const T* __restrict__ w, const T* __restrict__ V, int64_t ldv,
-                                                  int m, int64_t n, double* __restrict__ part) {
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { typedef float __attribute__((ext_vector_type(4))) type; static constexpr int L = 4; };
+template <> struct Vec16<double> { typedef double __attribute__((ext_vector_type(2))) type; static constexpr int L = 2; };
+
+// T: vector precision; partial sums are accumulated in double.  VEC: every row
+// of V and w 16-byte aligned and n a multiple of the vector width.
+template <typename T, int M, bool VEC>
+__global__ __launch_bounds__(256) void multidot_k(const T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int64_t n,
+                                                  double* __restrict__ part) {
     double acc[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) acc[k] = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double wi = w ? (double)w[i] : 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if constexpr (VEC) {
+        using VT = typename Vec16<T>::type;
+        constexpr int L = Vec16<T>::L;
+        for (int64_t i = t0; i < n / L; i += stride) {
+            VT wv{};
+            if (w) wv = reinterpret_cast<const VT*>(w)[i];
 #pragma unroll
-        for (int k = 0; k < M; ++k)
-            if (k < m) {
-                const double v = (double)V[(size_t)k * ldv + i];
-                acc[k] += (w ? wi : v) * v;   // w == nullptr: squared norms of V_k
+            for (int k = 0; k < M; ++k) {
+                const VT v = reinterpret_cast<const VT*>(V + (size_t)k * ldv)[i];
+#pragma unroll
+                for (int e = 0; e < L; ++e) acc[k] += (double)(w ? wv[e] : v[e]) * (double)v[e];   // w == nullptr: |V_k|^2
             }
+        }
+    } else {
+        for (int64_t i = t0; i < n; i += stride) {
+            const double wi = w ? (double)w[i] : 0.0;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const double v = (double)V[(size_t)k * ldv + i];
+                acc[k] += (w ? wi : v) * v;
+            }
+        }
     }
     __shared__ double sm[4][M];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -458,6 +481,7 @@ __global__ __launch_bounds__(256) void multidot_k(const T* __restrict__ w, const
     }
 }
 
+// out[k] = sum_b part[b][k], b < nb, in a fixed order (one workgroup per k)
 __global__ __launch_bounds__(256) void reduce_parts_k(const double* __restrict__ part, int nb, int M, double* __restrict__ out) {
     const int k = blockIdx.x;
     double s = 0;
@@ -477,12 +501,37 @@ static void ensure_red(bh_ctx* ctx) {
 }
 
 template <typename T>
+static bool vec16_ok(const T* w, const T* V, int64_t ldv, int m, int64_t n) {
+    constexpr int L = Vec16<T>::L;
+    const bool al = ((uintptr_t)V & 15) == 0 && (!w || ((uintptr_t)w & 15) == 0);
+    return al && n % L == 0 && (m == 1 || ldv % L == 0);
+}
+
+template <typename T, int M>
+static void md_launch(bh_ctx* ctx, dim3 g, bool vec, const T* w, const T* V, int64_t ldv, int64_t n, double* dout) {
+    if (vec)
+        hipLaunchKernelGGL((multidot_k<T, M, true>), g, dim3(256), 0, ctx->stream, w, V, ldv, n, ctx->red.as<double>());
+    else
+        hipLaunchKernelGGL((multidot_k<T, M, false>), g, dim3(256), 0, ctx->stream, w, V, ldv, n, ctx->red.as<double>());
+    hipLaunchKernelGGL(reduce_parts_k, dim3(M), dim3(256), 0, ctx->stream, ctx->red.as<double>(), (int)g.x, M, dout);
+}
+
+template <typename T>
 static void multi_dot_dev(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* dout) {
     ensure_red(ctx);
-    int64_t nb = (n + 255) / 256;
+    const bool vec = vec16_ok<T>(w, V, ldv, m, n);
+    const int64_t units = vec ? n / Vec16<T>::L : n;
+    int64_t nb = (units + 511) / 512;       // about two units per thread before the grid is full
     if (nb > MD_BLOCKS) nb = MD_BLOCKS;
-    hipLaunchKernelGGL((multidot_k<T, MD_M>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, w, V, ldv, m, n, ctx->red.as<double>());
-    hipLaunchKernelGGL(reduce_parts_k, dim3(m), dim3(256), 0, ctx->stream, ctx->red.as<double>(), (int)nb, MD_M, dout);
+    if (nb < 1) nb = 1;
+    const dim3 g((unsigned)nb);
+    switch (m) {
+#define BH_MD(MM) case MM: md_launch<T, MM>(ctx, g, vec, w, V, ldv, n, dout); break;
+        BH_MD(1) BH_MD(2) BH_MD(3) BH_MD(4) BH_MD(5) BH_MD(6) BH_MD(7) BH_MD(8)
+        BH_MD(9) BH_MD(10) BH_MD(11) BH_MD(12) BH_MD(13) BH_MD(14) BH_MD(15) BH_MD(16)
+#undef BH_MD
+        default: throw Error(-3, "multi_dot: 1..16 vectors per launch");
+    }
     BH_HIP(hipGetLastError());
 }
 
@@ -518,16 +567,55 @@ double dot(bh_ctx* ctx, const double* a, const double* b, int64_t n) {
 }
 double nrm2(bh_ctx* ctx, const double* a, int64_t n) { return nrm2_t<double>(ctx, a, n); }
 
+// w <- w + sum_k c[k] V_k for m <= 32 vectors, 16-byte vector loads when VEC
+template <typename T, bool VEC>
+__device__ __forceinline__ void axpy_rows(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int m, int64_t n,
+                                          const double* c) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if constexpr (VEC) {
+        using VT = typename Vec16<T>::type;
+        constexpr int L = Vec16<T>::L;
+        for (int64_t i = t0; i < n / L; i += stride) {
+            const VT x = reinterpret_cast<const VT*>(w)[i];
+            double s[L];
+#pragma unroll
+            for (int e = 0; e < L; ++e) s[e] = (double)x[e];
+            for (int k = 0; k < m; ++k) {
+                const VT v = reinterpret_cast<const VT*>(V + (size_t)k * ldv)[i];
+#pragma unroll
+                for (int e = 0; e < L; ++e) s[e] += c[k] * (double)v[e];
+            }
+            VT y;
+#pragma unroll
+            for (int e = 0; e < L; ++e) y[e] = (T)s[e];
+            reinterpret_cast<VT*>(w)[i] = y;
+        }
+    } else {
+        for (int64_t i = t0; i < n; i += stride) {
+            double s = (double)w[i];
+            for (int k = 0; k < m; ++k) s += c[k] * (double)V[(size_t)k * ldv + i];
+            w[i] = (T)s;
+        }
+    }
+}
+
 // w <- w + sign * sum_k h[k] V_k   (coefficients passed by value, m <= 32 per launch)
 struct Coefs { double c[32]; };
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void multi_axpy_k(T* __restrict__ w, const T* __restrict__ V, int64_t ldv,
                                                     int m, int64_t n, Coefs h) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double s = (double)w[i];
-        for (int k = 0; k < m; ++k) s += h.c[k] * (double)V[(size_t)k * ldv + i];
-        w[i] = (T)s;
-    }
+    __shared__ double cs[32];
+    if (threadIdx.x < 32) cs[threadIdx.x] = h.c[threadIdx.x];
+    __syncthreads();
+    axpy_rows<T, VEC>(w, V, ldv, m, n, cs);
+}
+
+static dim3 grid_vec(int64_t units) {
+    int64_t b = (units + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
 }
 
 template <typename T>
@@ -536,7 +624,11 @@ void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, 
         Coefs c{};
         int mm = std::min(32, m - k0);
         for (int k = 0; k < mm; ++k) c.c[k] = sign * h[k0 + k];
-        hipLaunchKernelGGL((multi_axpy_k<T>), grid_for(n), dim3(256), 0, ctx->stream, w, V + (size_t)k0 * ldv, ldv, mm, n, c);
+        const T* Vk = V + (size_t)k0 * ldv;
+        if (vec16_ok<T>(w, Vk, ldv, mm, n))
+            hipLaunchKernelGGL((multi_axpy_k<T, true>), grid_vec(n / Vec16<T>::L), dim3(256), 0, ctx->stream, w, Vk, ldv, mm, n, c);
+        else
+            hipLaunchKernelGGL((multi_axpy_k<T, false>), grid_vec(n), dim3(256), 0, ctx->stream, w, Vk, ldv, mm, n, c);
     }
     BH_HIP(hipGetLastError());
 }
@@ -558,7 +650,7 @@ void multi_dot_to_dev(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, i
 }
 
 // w <- w + sign * scale * sum_k c[k] V_k with c (and scale, if given) read from device memory
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void multi_axpy_dc_k(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int m,
                                                        int64_t n, const double* __restrict__ c, const double* __restrict__ scale,
                                                        double sign) {
@@ -566,19 +658,22 @@ __global__ __launch_bounds__(256) void multi_axpy_dc_k(T* __restrict__ w, const 
     const double f = sign * (scale ? scale[0] : 1.0);
     if (threadIdx.x < m) cs[threadIdx.x] = f * c[threadIdx.x];
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double s = (double)w[i];
-        for (int k = 0; k < m; ++k) s += cs[k] * (double)V[(size_t)k * ldv + i];
-        w[i] = (T)s;
-    }
+    axpy_rows<T, VEC>(w, V, ldv, m, n, cs);
 }
 
 template <typename T>
 void multi_axpy_from_dev(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* dcoef,
                          const double* dscale, double sign) {
-    for (int k0 = 0; k0 < m; k0 += 32)
-        hipLaunchKernelGGL((multi_axpy_dc_k<T>), grid_for(n), dim3(256), 0, ctx->stream, w, V + (size_t)k0 * ldv, ldv,
-                           std::min(32, m - k0), n, dcoef + k0, dscale, sign);
+    for (int k0 = 0; k0 < m; k0 += 32) {
+        const int mm = std::min(32, m - k0);
+        const T* Vk = V + (size_t)k0 * ldv;
+        if (vec16_ok<T>(w, Vk, ldv, mm, n))
+            hipLaunchKernelGGL((multi_axpy_dc_k<T, true>), grid_vec(n / Vec16<T>::L), dim3(256), 0, ctx->stream, w, Vk, ldv,
+                               mm, n, dcoef + k0, dscale, sign);
+        else
+            hipLaunchKernelGGL((multi_axpy_dc_k<T, false>), grid_vec(n), dim3(256), 0, ctx->stream, w, Vk, ldv, mm, n,
+                               dcoef + k0, dscale, sign);
+    }
     BH_HIP(hipGetLastError());
 }
 
