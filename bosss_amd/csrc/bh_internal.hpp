@@ -7,6 +7,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/bosss_hip.h"
@@ -99,6 +100,39 @@ struct bh_grid {
         else { idx[0] = c % n[0]; idx[1] = (c / n[0]) % n[1]; idx[2] = c / (n[0] * n[1]); }
     }
 };
+
+// ------------------------------------------------------------------ device helpers
+namespace bh {
+// f(std::integral_constant<int, i>) for i = 0..N-1, unrolled at compile time
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_(F& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_(f, std::make_integer_sequence<int, N>{});
+}
+// exponent in direction d of mode n of the degree-k basis (order of basis_exponents:
+// degree, then e_0, then e_1)
+__host__ __device__ constexpr int mode_exponent(int D, int k, int n, int d) {
+    int i = 0;
+    for (int deg = 0; deg <= k; ++deg) {
+        if (D == 2) {
+            for (int a0 = 0; a0 <= deg; ++a0) {
+                if (i == n) return d == 0 ? a0 : deg - a0;
+                ++i;
+            }
+        } else {
+            for (int a0 = 0; a0 <= deg; ++a0)
+                for (int b = 0; b <= deg - a0; ++b) {
+                    if (i == n) return d == 0 ? a0 : (d == 1 ? b : deg - a0 - b);
+                    ++i;
+                }
+        }
+    }
+    return 0;
+}
+}  // namespace bh
 
 // ------------------------------------------------------------------ tables
 // Reference-element tables of the orthonormal basis at the Gauss nodes.

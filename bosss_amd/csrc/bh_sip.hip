@@ -127,25 +127,117 @@ __device__ __forceinline__ void face_term(const SipArgs<R>& a, int f, const R* u
     }
 }
 
+// Both faces of direction d (f = 2d: '-', f = 2d+1: '+') of cell P, for the apply.
+// On the Legendre tensor basis the '+' face rows are the '-' face rows times
+// (-1)^{e_d(n)} (values) and -(-1)^{e_d(n)} (normal derivatives, d/dxi_d), so
+// P's traces on both faces come from one split into even and odd modes in d and
+// the flux of both faces goes back through the '-' rows once.  The flux is the
+// same as face_term's, written as linear combinations of the raw traces
+// (t = T u, n = N u per side; the scalings sdinv of the two cells, the normal
+// sign and 2/h are folded into per-face coefficients, dA sdPinv^2 = 2/h_d).
+// uM / uQ: modes of the '-' / '+' neighbour (zero on a boundary side).
+template <int D, int K, typename R, int d>
+__device__ __forceinline__ void face_pair(const SipArgs<R>& a, const R* uP, const R* uM, const R* uQ, const bool* bnd,
+                                          const bool* skip, R gP, const R* gQ, const R* rq, R nuP, const R* nuQ,
+                                          const R* mu, R* y) {
+    constexpr int Np = binom(K + D, D);
+    constexpr int NQF = ipow(K + 1, D - 1);
+    const R* __restrict__ T = a.T + (size_t)(2 * d) * NQF * Np;
+    const R* __restrict__ Nr = a.Nn + (size_t)(2 * d) * NQF * Np;
+    R c1[2], c2[2], c3[2], c4[2], c5[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const R sgn = s ? R(1) : R(-1);
+        const R G = skip[s] ? R(0) : gP;
+        const R A = G * nuP * sgn * gP;
+        const R nuMax = (fabs(nuP) > fabs(nuQ[s])) ? nuP : nuQ[s];
+        c1[s] = (bnd[s] ? R(1) : R(0.5)) * A;
+        c3[s] = -G * mu[s] * (bnd[s] ? R(2) * nuP : nuMax);
+        c2[s] = bnd[s] ? R(0) : R(0.5) * G * nuQ[s] * sgn * gQ[s] * rq[s];
+        c4[s] = bnd[s] ? R(0) : -c3[s] * rq[s];
+        c5[s] = bnd[s] ? R(0) : R(-0.5) * A * rq[s];
+    }
+#pragma unroll (Np <= 4 ? NQF : 1)
+    for (int q = 0; q < NQF; ++q) {
+        R tE = 0, tO = 0, nE = 0, nO = 0, mE = 0, mO = 0, mnE = 0, mnO = 0, pt = 0, pn = 0;
+        static_for<Np>([&](auto NN) {
+            constexpr int n = NN;
+            constexpr int e = mode_exponent(D, K, n, d);   // d/dxi_d of a mode constant in xi_d is zero
+            const R tv = T[q * Np + n];
+            if constexpr (e & 1) {
+                tO += tv * uP[n]; mO += tv * uM[n];
+            } else {
+                tE += tv * uP[n]; mE += tv * uM[n];
+            }
+            pt += tv * uQ[n];
+            if constexpr (e > 0) {
+                const R nv = Nr[q * Np + n];
+                if constexpr (e & 1) {
+                    nO += nv * uP[n]; mnO += nv * uM[n];
+                } else {
+                    nE += nv * uP[n]; mnE += nv * uM[n];
+                }
+                pn += nv * uQ[n];
+            }
+        });
+        // '-' face: P's t, n = tE + tO, nE + nO; the '-' neighbour's '+' face: mE - mO, -(mnE - mnO)
+        const R cvm = c1[0] * (nE + nO) + c2[0] * (mnO - mnE) + c3[0] * (tE + tO) + c4[0] * (mE - mO);
+        const R cdm = c1[0] * (tE + tO) + c5[0] * (mE - mO);
+        // '+' face: P's t, n = tE - tO, nO - nE; the '+' neighbour's '-' face: pt, pn
+        const R cvp = c1[1] * (nO - nE) + c2[1] * pn + c3[1] * (tE - tO) + c4[1] * pt;
+        const R cdp = c1[1] * (tE - tO) + c5[1] * pt;
+        const R w = a.fw[q];
+        const R TEc = w * (cvm + cvp), TOc = w * (cvm - cvp), NEc = w * (cdm - cdp), NOc = w * (cdm + cdp);
+        static_for<Np>([&](auto MM) {
+            constexpr int m = MM;
+            constexpr int e = mode_exponent(D, K, m, d);
+            y[m] = fma(T[q * Np + m], (e & 1) ? TOc : TEc, y[m]);
+            if constexpr (e > 0) y[m] = fma(Nr[q * Np + m], (e & 1) ? NOc : NEc, y[m]);
+        });
+    }
+}
+
 // Volume integral -nu grad u . grad v of an axis-aligned box cell: the metric
 // is diagonal and constant, so the Gauss sum over the volume nodes factorises
 // into the per-direction reference stiffness Kv[d] = sum_q w_q G_d(q) G_d(q)^T
 // (computed once from the same rule): y += sum_d (-nu 4/h_d^2) Kv[d] u.
+// On the orthonormal Legendre tensor basis Kv[d][m][n] vanishes unless the
+// exponents agree in every other direction and both are >= 1 and of equal
+// parity in d (the rule is exact for these products): only those entries are
+// summed.
+__host__ __device__ constexpr bool kv_nonzero(int D, int k, int m, int n, int d) {
+    for (int dd = 0; dd < D; ++dd) {
+        const int em = mode_exponent(D, k, m, dd), en = mode_exponent(D, k, n, dd);
+        if (dd == d) {
+            if (em == 0 || en == 0 || ((em + en) & 1)) return false;
+        } else if (em != en) {
+            return false;
+        }
+    }
+    return true;
+}
+
 template <int D, int K, typename R>
 __device__ __forceinline__ void volume_term(const SipArgs<R>& a, const R* uP, const R* s2, R nuP, R* y) {
     constexpr int Np = binom(K + D, D);
-#pragma unroll (Np <= 10 ? D : 1)
-    for (int d = 0; d < D; ++d) {
+    static_for<D>([&](auto DD) {
+        constexpr int d = DD;
         const R* __restrict__ Kd = a.Kv + (size_t)d * Np * Np;
-        const R cd = -nuP * (d == 0 ? s2[0] : (d == 1 ? s2[1] : s2[D - 1]));
-#pragma unroll
-        for (int m = 0; m < Np; ++m) {
+        const R cd = -nuP * s2[d];
+        static_for<Np>([&](auto MM) {
+            constexpr int m = MM;
             R acc = 0;
-#pragma unroll
-            for (int n = 0; n < Np; ++n) acc += Kd[m * Np + n] * uP[n];
-            y[m] += cd * acc;
-        }
-    }
+            bool any = false;
+            static_for<Np>([&](auto NN) {
+                constexpr int n = NN;
+                if constexpr (kv_nonzero(D, K, m, n, d)) {
+                    acc += Kd[m * Np + n] * uP[n];
+                    any = true;
+                }
+            });
+            if (any) y[m] += cd * acc;
+        });
+    });
 }
 
 // geometry of cell with logical index idx
@@ -168,8 +260,6 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     const R* up = a.u + (size_t)c * Np;
 #pragma unroll
     for (int n = 0; n < Np; ++n) { uP[n] = up[n]; y[n] = 0; }
-    R detJ = 1;
-    for (int d = 0; d < D; ++d) detJ *= R(0.5) * h[d];
     const R nuP = a.nu[c];
     // reciprocal widths once; cj = sum_d (1 + boundary faces_d / 2) / h_d (the cell_cj formula),
     // and the neighbour's cj differs from P's only in direction d
@@ -180,36 +270,42 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
         cjP += (R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
     }
     volume_term<D, K, R>(a, uP, s2, nuP, y);
-    const R sdPinv = R(1) / sqrt(detJ);
     const R pen = R(a.pen);
-#pragma unroll (Np <= 4 ? 2 * D : 1)
-    for (int f = 0; f < 2 * D; ++f) {
-        const int d = f >> 1, s = f & 1;
-        const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
-        const R ihd = d == 0 ? ih[0] : (d == 1 ? ih[1] : ih[2]);
-        const R dA = R(2) * detJ * ihd;
-        const R gP = R(2) * ihd;
-        if (isBnd) {
-            if ((a.bc_neumann >> f) & 1) continue;
-            R uQ[1] = {0};
-            face_term<D, K, R, true, false>(a, f, uP, uQ, sdPinv, R(0), gP, R(0), dA, nuP, nuP, pen * cjP, true, y);
-        } else {
-            int nidx[3] = {idx[0], idx[1], idx[2]};
-            nidx[d] += s ? 1 : -1;
-            const int cq = encode<D>(nidx, a.n0, a.n1);
-            const R hd = d == 0 ? h[0] : (d == 1 ? h[1] : h[2]);
-            const R ihq = R(1) / R(a.width[d * a.nmax + nidx[d]]);
-            const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
-            const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
-            const R cjQ = cjP - bP * ihd + bQ * ihq;
-            R uQ[Np];
-            const R* uq = a.u + (size_t)cq * Np;
+    static_for<D>([&](auto DD) {
+        constexpr int d = DD;
+        const R ihd = ih[d];
+        const R hd = h[d];
+        bool bnd[2], skip[2];
+        R gQ[2], rq[2], nuQ[2], mu[2];
+        R uM[Np], uQ[Np];
 #pragma unroll
-            for (int n = 0; n < Np; ++n) uQ[n] = uq[n];
-            face_term<D, K, R, true, true>(a, f, uP, uQ, sdPinv, sdPinv * sqrt(hd * ihq), gP, R(2) * ihq, dA,
-                                           nuP, a.nu[cq], pen * (cjP > cjQ ? cjP : cjQ), false, y);
+        for (int s = 0; s < 2; ++s) {
+            R* un = s ? uQ : uM;
+            bnd[s] = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+            skip[s] = bnd[s] && ((a.bc_neumann >> (2 * d + s)) & 1);
+            if (bnd[s]) {
+                gQ[s] = 0; rq[s] = 0; nuQ[s] = nuP; mu[s] = pen * cjP;
+#pragma unroll
+                for (int n = 0; n < Np; ++n) un[n] = 0;
+            } else {
+                int nidx[3] = {idx[0], idx[1], idx[2]};
+                nidx[d] += s ? 1 : -1;
+                const int cq = encode<D>(nidx, a.n0, a.n1);
+                const R ihq = R(1) / R(a.width[d * a.nmax + nidx[d]]);
+                const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
+                const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+                const R cjQ = cjP - bP * ihd + bQ * ihq;
+                gQ[s] = R(2) * ihq;
+                rq[s] = sqrt(hd * ihq);                 // sdQinv / sdPinv
+                nuQ[s] = a.nu[cq];
+                mu[s] = pen * (cjP > cjQ ? cjP : cjQ);
+                const R* uq = a.u + (size_t)cq * Np;
+#pragma unroll
+                for (int n = 0; n < Np; ++n) un[n] = uq[n];
+            }
         }
-    }
+        face_pair<D, K, R, d>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y);
+    });
     R* yp = a.y + (size_t)c * Np;
     const R al = R(a.alpha), be = R(a.beta);
     if (a.affine) {

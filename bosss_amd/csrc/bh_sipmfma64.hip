@@ -223,15 +223,6 @@ __device__ __forceinline__ double ldg(const double* __restrict__ p, int i) {
     return *(const double*)((const char*)p + (unsigned)i * 8u);
 }
 
-template <typename F, int... I>
-__device__ __forceinline__ void static_for_(F& f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-    static_for_(f, std::make_integer_sequence<int, N>{});
-}
-
 // Row-local lane move of a double (DPP): lanes whose source lies outside their
 // 16-lane row keep `old`.  0x100 + n: lane i reads lane i + n; 0x110 + n: lane i - n.
 template <int CTRL>
