@@ -24,16 +24,36 @@ USE_GRAPHS = True        # HIP-graph replay of the multigrid V-cycles (--no-grap
 
 
 def sip_flops(n, D, k):
-    """Algorithmic FLOPs of one matrix-free SIP apply: volume 2*D*Np^2 per
-    cell (the per-direction reference stiffness times the modes: the volume
-    rule of an axis-aligned box factorises) and 16*Np*Nqf per face (values and
-    normal derivatives on both sides + contraction back)."""
+    """Algorithmic FLOPs of one matrix-free SIP apply: the multiply-adds the
+    exact quadrature needs on the orthonormal Legendre tensor basis, without its
+    structural zeros.  Volume: 2 per nonzero of the per-direction reference
+    stiffness Kv[d] (the volume rule of an axis-aligned box factorises; Kv[d][m][n]
+    is nonzero only for equal exponents outside d and exponents >= 1 of equal
+    parity in d).  Faces of direction d: 8*Nqf*(Np + nd) (value and normal
+    derivative on both sides, and the contraction back; nd modes have a nonzero
+    d-derivative)."""
     from math import comb
     Np = comb(k + D, D)
     Nqf = (k + 1) ** (D - 1)
     J = n ** D
-    F = D * (n - 1) * n ** (D - 1) + 2 * D * n ** (D - 1)
-    return J * 2 * D * Np * Np + F * 16 * Np * Nqf
+    ex = []
+    for deg in range(k + 1):
+        if D == 2:
+            ex += [(a, deg - a) for a in range(deg + 1)]
+        else:
+            ex += [(a, b, deg - a - b) for a in range(deg + 1) for b in range(deg - a + 1)]
+    assert len(ex) == Np
+    vol = 0
+    faces = 0
+    Fd = (n - 1) * n ** (D - 1) + 2 * n ** (D - 1)     # faces per direction, boundary included
+    for d in range(D):
+        for em in ex:
+            for en in ex:
+                if em[d] >= 1 and en[d] >= 1 and (em[d] + en[d]) % 2 == 0 and all(em[e] == en[e] for e in range(D) if e != d):
+                    vol += 2
+        nd = sum(1 for e in ex if e[d] >= 1)
+        faces += Fd * 8 * Nqf * (Np + nd)
+    return J * vol + faces
 
 
 def legendre_1d(k, xi):
@@ -480,6 +500,7 @@ def main():
         "data": "synthetic (seeded N(0,1) DOF vector, log-uniform per-cell diffusion)",
         "config": {"workload": f"3D SIP Poisson {n}^3 hexes k={k} ({N} DOF), FP64 matrix-free operator apply",
                    "grid": f"{n}^3", "k": k, "dof": N},
+        # compute-bound: FP64 peak, the same for vector FMA and matrix cores on MI355X
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": measured_traffic(n, k, kname),
                      "kernel": kname, "kernel_ms": kern_ms,
