@@ -136,10 +136,26 @@ __device__ __forceinline__ void face_term(const SipArgs<R>& a, int f, const R* u
 // (t = T u, n = N u per side; the scalings sdinv of the two cells, the normal
 // sign and 2/h are folded into per-face coefficients, dA sdPinv^2 = 2/h_d).
 // uM / uQ: modes of the '-' / '+' neighbour (zero on a boundary side).
-template <int D, int K, typename R, int d>
+// FAST (d is the direction of consecutive cells, one wave = 64 consecutive cells):
+// the neighbours' traces are the adjacent lanes' own traces (ds_bpermute), and
+// lanes 4q..4q+3 of hv hold, for face node q, the value and derivative traces of
+// the cell before the wave on its '+' face and of the cell after it on its '-'
+// face; uM / uQ are not read.
+template <typename R>
+__device__ __forceinline__ R lane_value(R v, int l) {
+    if constexpr (sizeof(R) == 8) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    }
+}
+
+template <int D, int K, typename R, int d, bool FAST>
 __device__ __forceinline__ void face_pair(const SipArgs<R>& a, const R* uP, const R* uM, const R* uQ, const bool* bnd,
                                           const bool* skip, R gP, const R* gQ, const R* rq, R nuP, const R* nuQ,
-                                          const R* mu, R* y) {
+                                          const R* mu, R* y, R hv, int lane) {
     constexpr int Np = binom(K + D, D);
     constexpr int NQF = ipow(K + 1, D - 1);
     const R* __restrict__ T = a.T + (size_t)(2 * d) * NQF * Np;
@@ -165,27 +181,47 @@ __device__ __forceinline__ void face_pair(const SipArgs<R>& a, const R* uP, cons
             constexpr int e = mode_exponent(D, K, n, d);   // d/dxi_d of a mode constant in xi_d is zero
             const R tv = T[q * Np + n];
             if constexpr (e & 1) {
-                tO += tv * uP[n]; mO += tv * uM[n];
+                tO += tv * uP[n];
+                if constexpr (!FAST) mO += tv * uM[n];
             } else {
-                tE += tv * uP[n]; mE += tv * uM[n];
+                tE += tv * uP[n];
+                if constexpr (!FAST) mE += tv * uM[n];
             }
-            pt += tv * uQ[n];
+            if constexpr (!FAST) pt += tv * uQ[n];
             if constexpr (e > 0) {
                 const R nv = Nr[q * Np + n];
                 if constexpr (e & 1) {
-                    nO += nv * uP[n]; mnO += nv * uM[n];
+                    nO += nv * uP[n];
+                    if constexpr (!FAST) mnO += nv * uM[n];
                 } else {
-                    nE += nv * uP[n]; mnE += nv * uM[n];
+                    nE += nv * uP[n];
+                    if constexpr (!FAST) mnE += nv * uM[n];
                 }
-                pn += nv * uQ[n];
+                if constexpr (!FAST) pn += nv * uQ[n];
             }
         });
-        // '-' face: P's t, n = tE + tO, nE + nO; the '-' neighbour's '+' face: mE - mO, -(mnE - mnO)
-        const R cvm = c1[0] * (nE + nO) + c2[0] * (mnO - mnE) + c3[0] * (tE + tO) + c4[0] * (mE - mO);
-        const R cdm = c1[0] * (tE + tO) + c5[0] * (mE - mO);
-        // '+' face: P's t, n = tE - tO, nO - nE; the '+' neighbour's '-' face: pt, pn
-        const R cvp = c1[1] * (nO - nE) + c2[1] * pn + c3[1] * (tE - tO) + c4[1] * pt;
-        const R cdp = c1[1] * (tE - tO) + c5[1] * pt;
+        // the '-' neighbour's '+' face (t, n) and the '+' neighbour's '-' face (t, n)
+        R mt, mn, qt, qn;
+        if constexpr (FAST) {
+            const R hTm = lane_value(hv, 4 * q), hNm = lane_value(hv, 4 * q + 1);
+            const R hTp = lane_value(hv, 4 * q + 2), hNp = lane_value(hv, 4 * q + 3);
+            const int lm = (lane + 63) & 63, lp = (lane + 1) & 63;
+            mt = __shfl(lane == 63 ? hTm : tE - tO, lm, 64);   // lane 0 receives the cell before the wave
+            mn = __shfl(lane == 63 ? hNm : nO - nE, lm, 64);
+            qt = __shfl(lane == 0 ? hTp : tE + tO, lp, 64);    // lane 63 receives the cell after it
+            qn = __shfl(lane == 0 ? hNp : nE + nO, lp, 64);
+        } else {
+            mt = mE - mO;
+            mn = mnO - mnE;
+            qt = pt;
+            qn = pn;
+        }
+        // '-' face: P's t, n = tE + tO, nE + nO
+        const R cvm = c1[0] * (nE + nO) + c2[0] * mn + c3[0] * (tE + tO) + c4[0] * mt;
+        const R cdm = c1[0] * (tE + tO) + c5[0] * mt;
+        // '+' face: P's t, n = tE - tO, nO - nE
+        const R cvp = c1[1] * (nO - nE) + c2[1] * qn + c3[1] * (tE - tO) + c4[1] * qt;
+        const R cdp = c1[1] * (tE - tO) + c5[1] * qt;
         const R w = a.fw[q];
         const R TEc = w * (cvm + cvp), TOc = w * (cvm - cvp), NEc = w * (cdm - cdp), NOc = w * (cdm + cdp);
         static_for<Np>([&](auto MM) {
@@ -249,8 +285,13 @@ __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S
 template <int D, int K, typename R>
 __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     constexpr int Np = binom(K + D, D);
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.J) return;
+    constexpr int NQF = ipow(K + 1, D - 1);
+    constexpr int DF = D == 2 ? 1 : 0;       // direction of consecutive cells
+    const int c0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63);   // first cell of the wave
+    const int lane = threadIdx.x & 63;
+    const int craw = c0 + lane;
+    const bool valid = craw < a.J;
+    const int c = valid ? craw : a.J - 1;   // every lane stays active for the lane exchanges
     int idx[3];
     decode<D>(c, a.n0, a.n1, idx);
     const int nn[3] = {a.n0, a.n1, a.n2};
@@ -275,9 +316,30 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
         constexpr int d = DD;
         const R ihd = ih[d];
         const R hd = h[d];
+        constexpr bool FAST = d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
         bool bnd[2], skip[2];
         R gQ[2], rq[2], nuQ[2], mu[2];
-        R uM[Np], uQ[Np];
+        R uM[FAST ? 1 : Np], uQ[FAST ? 1 : Np];
+        R hv = 0;
+        if constexpr (FAST) {
+            // traces of the cells before and after the wave, node q = lane / 4:
+            // kind 0 / 1: value / derivative of cell c0 - 1 on its '+' face (the '-' face
+            // rows times (-1)^e, and times -(-1)^e for the derivative); kind 2 / 3: value /
+            // derivative of cell c0 + 64 on its '-' face
+            const int hq = lane >> 2, kind = lane & 3;
+            if (hq < NQF) {
+                const int hc = kind < 2 ? max(c0 - 1, 0) : min(c0 + 64, a.J - 1);
+                const R* __restrict__ row = (kind & 1 ? a.Nn : a.T) + ((size_t)(2 * d) * NQF + hq) * Np;
+                const R* __restrict__ uh = a.u + (size_t)hc * Np;
+                static_for<Np>([&](auto NN) {
+                    constexpr int n = NN;
+                    constexpr int e = mode_exponent(D, K, n, d);
+                    const R v = row[n] * uh[n];
+                    hv += (kind < 2 && (e & 1)) ? -v : v;
+                });
+                if (kind == 1) hv = -hv;
+            }
+        }
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             R* un = s ? uQ : uM;
@@ -285,8 +347,10 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
             skip[s] = bnd[s] && ((a.bc_neumann >> (2 * d + s)) & 1);
             if (bnd[s]) {
                 gQ[s] = 0; rq[s] = 0; nuQ[s] = nuP; mu[s] = pen * cjP;
+                if constexpr (!FAST) {
 #pragma unroll
-                for (int n = 0; n < Np; ++n) un[n] = 0;
+                    for (int n = 0; n < Np; ++n) un[n] = 0;
+                }
             } else {
                 int nidx[3] = {idx[0], idx[1], idx[2]};
                 nidx[d] += s ? 1 : -1;
@@ -299,13 +363,16 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
                 rq[s] = sqrt(hd * ihq);                 // sdQinv / sdPinv
                 nuQ[s] = a.nu[cq];
                 mu[s] = pen * (cjP > cjQ ? cjP : cjQ);
-                const R* uq = a.u + (size_t)cq * Np;
+                if constexpr (!FAST) {
+                    const R* uq = a.u + (size_t)cq * Np;
 #pragma unroll
-                for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                    for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                }
             }
         }
-        face_pair<D, K, R, d>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y);
+        face_pair<D, K, R, d, FAST>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y, hv, lane);
     });
+    if (!valid) return;
     R* yp = a.y + (size_t)c * Np;
     const R al = R(a.alpha), be = R(a.beta);
     if (a.affine) {
