@@ -274,8 +274,17 @@ int bh_grid_cartesian(bh_ctx* ctx, int D, const double* x, int nx1, const double
     BH_CHECK(J < (1LL << 31), "grid too large");
     g->J = (int)J;
     build_grid_edges(g.get());
-    std::vector<double> w(3 * (size_t)g->nmax, 0.0);
-    for (int d = 0; d < D; ++d) std::copy(g->width[d].begin(), g->width[d].end(), w.begin() + (size_t)d * g->nmax);
+    // [4][3][nmax]: h, 1/h, sqrt(h), 1/sqrt(h) per direction (the kernels' per-face scalings without
+    // divisions or square roots)
+    std::vector<double> w(4 * 3 * (size_t)g->nmax, 0.0);
+    for (int d = 0; d < D; ++d)
+        for (int i = 0; i < g->n[d]; ++i) {
+            const double h = g->width[d][i];
+            w[(0 * 3 + d) * (size_t)g->nmax + i] = h;
+            w[(1 * 3 + d) * (size_t)g->nmax + i] = 1.0 / h;
+            w[(2 * 3 + d) * (size_t)g->nmax + i] = std::sqrt(h);
+            w[(3 * 3 + d) * (size_t)g->nmax + i] = 1.0 / std::sqrt(h);
+        }
     if (ctx) {   // a grid without context serves host-side set-up only (cut-cell rules, agglomeration)
         g->d_width.alloc(w.size() * sizeof(double));
         ctx->upload(g->d_width.p, w.data(), w.size() * sizeof(double));

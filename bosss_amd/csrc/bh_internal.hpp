@@ -90,7 +90,7 @@ struct bh_grid {
     // edges (own enumeration: per cell, per direction: '-' boundary face, then '+' face)
     std::vector<int> e_in, e_out, e_dir, e_side;
     std::vector<double> cj;
-    bh::DevBuf d_width;     // 3 * nmax doubles: widths per direction (padded)
+    bh::DevBuf d_width;     // [4][3][nmax] doubles: widths, 1/width, sqrt(width), 1/sqrt(width) per direction (padded)
     int nmax = 1;
     int cell_index(int i, int j, int k) const {
         return D == 2 ? i * n[1] + j : i + j * n[0] + k * n[0] * n[1];

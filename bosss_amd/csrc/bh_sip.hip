@@ -27,7 +27,7 @@ struct SipArgs {
     int J;
     int n0, n1, n2;
     int nmax;
-    const double* __restrict__ width;  // [3][nmax]
+    const double* __restrict__ width;  // [4][3][nmax]: h, 1/h, sqrt(h), 1/sqrt(h)
     const R* __restrict__ nu;
     const R* __restrict__ G;    // [nqv][D][Np]
     const R* __restrict__ T;    // [2D][nqf][Np]
@@ -295,8 +295,6 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     int idx[3];
     decode<D>(c, a.n0, a.n1, idx);
     const int nn[3] = {a.n0, a.n1, a.n2};
-    R h[3];                      // geometry in the arithmetic precision of the apply
-    cell_geom<D>(a, idx, h);
     R uP[Np], y[Np];
     const R* up = a.u + (size_t)c * Np;
 #pragma unroll
@@ -306,7 +304,7 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     // and the neighbour's cj differs from P's only in direction d
     R ih[3] = {1, 1, 1}, s2[D], cjP = 0;
     for (int d = 0; d < D; ++d) {
-        ih[d] = R(1) / h[d];
+        ih[d] = R(a.width[(3 + d) * a.nmax + idx[d]]);
         s2[d] = R(4) * ih[d] * ih[d];
         cjP += (R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
     }
@@ -315,7 +313,6 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     static_for<D>([&](auto DD) {
         constexpr int d = DD;
         const R ihd = ih[d];
-        const R hd = h[d];
         constexpr bool FAST = d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
         bool bnd[2], skip[2];
         R gQ[2], rq[2], nuQ[2], mu[2];
@@ -355,12 +352,12 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
                 int nidx[3] = {idx[0], idx[1], idx[2]};
                 nidx[d] += s ? 1 : -1;
                 const int cq = encode<D>(nidx, a.n0, a.n1);
-                const R ihq = R(1) / R(a.width[d * a.nmax + nidx[d]]);
+                const R ihq = R(a.width[(3 + d) * a.nmax + nidx[d]]);
                 const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
                 const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
                 const R cjQ = cjP - bP * ihd + bQ * ihq;
                 gQ[s] = R(2) * ihq;
-                rq[s] = sqrt(hd * ihq);                 // sdQinv / sdPinv
+                rq[s] = R(a.width[(6 + d) * a.nmax + idx[d]] * a.width[(9 + d) * a.nmax + nidx[d]]);   // sdQinv / sdPinv
                 nuQ[s] = a.nu[cq];
                 mu[s] = pen * (cjP > cjQ ? cjP : cjQ);
                 if constexpr (!FAST) {
