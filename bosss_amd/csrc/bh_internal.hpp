@@ -226,13 +226,16 @@ struct bh_solver {
 
 namespace bh {
 // kernels / launchers (defined in the .hip files)
-void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec);
+// y = alpha A u + beta y (+ affine); with b0: y = b0 + alpha A u (the residual form, one pass)
+void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec,
+               const void* b0 = nullptr);
 // FP32 vectors, FP16 MFMA contractions with FP32 accumulation
-void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine);
+void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine,
+                    const float* b0 = nullptr);
 // FP64 vectors, FP64 MFMA contractions (same results as the vector kernel up to summation order)
 bool sip_apply_mfma64_supported(int D, int K, int64_t N);   // N: DOF (arrays below 4 GiB)
 void sip_apply_mfma64(bh_op* op, double alpha, const double* u, double beta, double* y, const double* affine, int waves,
-                      int per_xcd);
+                      int per_xcd, const double* b0 = nullptr);
 void sip_assemble(bh_op* op, bh_bsr* M);
 void bsr_spmv(bh_bsr* A, double alpha, const void* x, double beta, void* y, int prec);
 void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32);

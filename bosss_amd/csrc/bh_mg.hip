@@ -39,7 +39,8 @@ __global__ void extract_subblocks_k(int64_t nnzb, int bsf, int bsc, const double
 __global__ __launch_bounds__(256) void galerkin_gather_k(int64_t nq, int bs, const int64_t* __restrict__ qptr,
                                                          const int* __restrict__ plist, const int* __restrict__ prow,
                                                          const int* __restrict__ fcol, const double* __restrict__ fv,
-                                                         const double* __restrict__ P, double* __restrict__ cv) {
+                                                         const int* __restrict__ pidx, const double* __restrict__ P,
+                                                         double* __restrict__ cv) {
     const int E = bs * bs;
     const int64_t tot = nq * E;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
@@ -50,8 +51,8 @@ __global__ __launch_bounds__(256) void galerkin_gather_k(int64_t nq, int bs, con
         for (int64_t k = qptr[q]; k < qptr[q + 1]; ++k) {
             const int64_t p = plist[k];
             const double* A = fv + p * E;
-            const double* Pi = P + (int64_t)prow[k] * E;
-            const double* Pj = P + (int64_t)fcol[p] * E;
+            const double* Pi = P + (int64_t)pidx[prow[k]] * E;
+            const double* Pj = P + (int64_t)pidx[fcol[p]] * E;
             for (int c = 0; c < bs; ++c) {
                 double t1 = 0;
                 for (int r = 0; r < bs; ++r) t1 += Pi[r * bs + m] * A[c * bs + r];   // (P_i^T A_p)(m, c)
@@ -88,14 +89,15 @@ __global__ void p_prolong_add_k(int64_t J, int bsf, int bsc, const T* __restrict
 // (fine cell, mode).
 template <typename T>
 __global__ void agg_restrict_k(int Jc, int bs, const int* __restrict__ cptr, const int* __restrict__ cidx,
-                               const T* __restrict__ P, const T* __restrict__ rf, T* __restrict__ rc) {
+                               const int* __restrict__ pidx, const T* __restrict__ P, const T* __restrict__ rf,
+                               T* __restrict__ rc) {
     const int64_t tot = (int64_t)Jc * bs;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
         const int I = (int)(t / bs), n = (int)(t - (int64_t)I * bs);
         T s = 0;
         for (int p = cptr[I]; p < cptr[I + 1]; ++p) {
             const int i = cidx[p];
-            const T* Pi = P + (size_t)i * bs * bs;
+            const T* Pi = P + (size_t)pidx[i] * bs * bs;
             const T* ri = rf + (size_t)i * bs;
             for (int m = 0; m < bs; ++m) s += Pi[m * bs + n] * ri[m];
         }
@@ -104,12 +106,12 @@ __global__ void agg_restrict_k(int Jc, int bs, const int* __restrict__ cptr, con
 }
 
 template <typename T>
-__global__ void agg_prolong_add_k(int Jf, int bs, const int* __restrict__ parent, const T* __restrict__ P,
-                                  const T* __restrict__ xc, T* __restrict__ xf) {
+__global__ void agg_prolong_add_k(int Jf, int bs, const int* __restrict__ parent, const int* __restrict__ pidx,
+                                  const T* __restrict__ P, const T* __restrict__ xc, T* __restrict__ xf) {
     const int64_t tot = (int64_t)Jf * bs;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(t / bs), m = (int)(t - (int64_t)i * bs);
-        const T* Pi = P + (size_t)i * bs * bs + (size_t)m * bs;
+        const T* Pi = P + (size_t)pidx[i] * bs * bs + (size_t)m * bs;
         const T* xcI = xc + (size_t)parent[i] * bs;
         T s = 0;
         for (int n = 0; n < bs; ++n) s += Pi[n] * xcI[n];
@@ -162,15 +164,33 @@ __global__ __launch_bounds__(256) void block_gemv_k(int nrows, const T* __restri
     }
 }
 
-// x = Ainv b (dense, row-major n x n), one wave per row; the matrix and the
-// accumulation may be wider than the vectors (FP64 inverse in an FP32 cycle)
-template <typename T, typename M>
+// x = Ainv b (dense, row-major n x n), one wave per row, accumulated in double.
+// VEC (n a multiple of 4): each lane reads 4 consecutive entries per step, two
+// steps in flight.
+template <typename T, typename M, bool VEC>
 __global__ __launch_bounds__(256) void dense_gemv_k(int n, const M* __restrict__ A, const T* __restrict__ b, T* __restrict__ x) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + wave;
     if (row >= n) return;
-    M s = 0;
-    for (int c = lane; c < n; c += 64) s += A[(size_t)row * n + c] * (M)b[c];
+    const M* __restrict__ Ar = A + (size_t)row * n;
+    double s0 = 0, s1 = 0;
+    if constexpr (VEC) {
+        int c = 4 * lane;
+        for (; c + 256 < n; c += 512) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                s0 += (double)Ar[c + e] * (double)b[c + e];
+                s1 += (double)Ar[c + 256 + e] * (double)b[c + 256 + e];
+            }
+        }
+        if (c < n) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s0 += (double)Ar[c + e] * (double)b[c + e];
+        }
+    } else {
+        for (int c = lane; c < n; c += 64) s0 += (double)Ar[c] * (double)b[c];
+    }
+    double s = s0 + s1;
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) x[row] = (T)s;
 }
@@ -343,9 +363,9 @@ void guarded_step(bh_ctx* ctx, const T* e, T* x, int64_t n, double* st, int* cnt
     BH_HIP(hipGetLastError());
 }
 void galerkin_gather(bh_ctx* ctx, int64_t nq, int bs, const int64_t* qptr, const int* plist, const int* prow,
-                     const int* fcol, const double* fv, const double* P, double* cv) {
+                     const int* fcol, const double* fv, const int* pidx, const double* P, double* cv) {
     hipLaunchKernelGGL(galerkin_gather_k, grid1d(nq * bs * bs), dim3(256), 0, ctx->stream, nq, bs, qptr, plist, prow, fcol, fv,
-                       P, cv);
+                       pidx, P, cv);
     BH_HIP(hipGetLastError());
 }
 void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc) {
@@ -362,12 +382,14 @@ void p_prolong_add(bh_ctx* ctx, int64_t J, int bsf, int bsc, const T* xc, T* xf)
     hipLaunchKernelGGL((p_prolong_add_k<T>), grid1d(J * bsc), dim3(256), 0, ctx->stream, J, bsf, bsc, xc, xf);
 }
 template <typename T>
-void agg_restrict(bh_ctx* ctx, int Jc, int bs, const int* cptr, const int* cidx, const T* P, const T* rf, T* rc) {
-    hipLaunchKernelGGL((agg_restrict_k<T>), grid1d((int64_t)Jc * bs), dim3(256), 0, ctx->stream, Jc, bs, cptr, cidx, P, rf, rc);
+void agg_restrict(bh_ctx* ctx, int Jc, int bs, const int* cptr, const int* cidx, const int* pidx, const T* P, const T* rf, T* rc) {
+    hipLaunchKernelGGL((agg_restrict_k<T>), grid1d((int64_t)Jc * bs), dim3(256), 0, ctx->stream, Jc, bs, cptr, cidx, pidx, P, rf,
+                       rc);
 }
 template <typename T>
-void agg_prolong_add(bh_ctx* ctx, int Jf, int bs, const int* parent, const T* P, const T* xc, T* xf) {
-    hipLaunchKernelGGL((agg_prolong_add_k<T>), grid1d((int64_t)Jf * bs), dim3(256), 0, ctx->stream, Jf, bs, parent, P, xc, xf);
+void agg_prolong_add(bh_ctx* ctx, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc, T* xf) {
+    hipLaunchKernelGGL((agg_prolong_add_k<T>), grid1d((int64_t)Jf * bs), dim3(256), 0, ctx->stream, Jf, bs, parent, pidx, P, xc,
+                       xf);
 }
 
 template <typename T>
@@ -389,9 +411,12 @@ void block_gemv(bh_ctx* ctx, int nrows, int bs, const T* Dinv, const T* r, T* x,
     BH_HIP(hipGetLastError());
 }
 
-template <typename T>
-void dense_gemv(bh_ctx* ctx, int n, const double* A, const T* b, T* x) {
-    hipLaunchKernelGGL((dense_gemv_k<T, double>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, A, b, x);
+template <typename T, typename M>
+void dense_gemv(bh_ctx* ctx, int n, const M* A, const T* b, T* x) {
+    if (n % 4 == 0)
+        hipLaunchKernelGGL((dense_gemv_k<T, M, true>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, A, b, x);
+    else
+        hipLaunchKernelGGL((dense_gemv_k<T, M, false>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, A, b, x);
 }
 
 template <typename T>
@@ -402,10 +427,11 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
 #define BH_INST(T)                                                                                   \
     template void p_restrict<T>(bh_ctx*, int64_t, int, int, const T*, T*);                           \
     template void p_prolong_add<T>(bh_ctx*, int64_t, int, int, const T*, T*);                        \
-    template void agg_restrict<T>(bh_ctx*, int, int, const int*, const int*, const T*, const T*, T*); \
-    template void agg_prolong_add<T>(bh_ctx*, int, int, const int*, const T*, const T*, T*);         \
+    template void agg_restrict<T>(bh_ctx*, int, int, const int*, const int*, const int*, const T*, const T*, T*); \
+    template void agg_prolong_add<T>(bh_ctx*, int, int, const int*, const int*, const T*, const T*, T*); \
     template void block_gemv<T>(bh_ctx*, int, int, const T*, const T*, T*, double, bool);             \
-    template void dense_gemv<T>(bh_ctx*, int, const double*, const T*, T*);                               \
+    template void dense_gemv<T, double>(bh_ctx*, int, const double*, const T*, T*);                       \
+    template void dense_gemv<T, float>(bh_ctx*, int, const float*, const T*, T*);                         \
     template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);                                          \
     template void scale_inv_norm<T>(bh_ctx*, const T*, T*, int64_t, const double*, double);           \
     template void schwarz_apply<T>(bh_ctx*, int, int, int, const int*, const int*, const T*, const T*, T*, double, bool); \

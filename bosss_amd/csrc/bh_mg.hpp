@@ -9,16 +9,20 @@ namespace bh {
 struct Mlp;
 
 void extract_subblocks(bh_ctx* ctx, int64_t nnzb, int bsf, int bsc, const double* vf, double* vc);
-// coarse Galerkin blocks: cv[q] = sum_{k in qptr[q]..qptr[q+1]} P_{prow[k]}^T fv[plist[k]] P_{fcol[plist[k]]}
+// coarse Galerkin blocks: cv[q] = sum_{k in qptr[q]..qptr[q+1]} P_{prow[k]}^T fv[plist[k]] P_{fcol[plist[k]]},
+// with the P block of fine cell i at P + pidx[i] * bs * bs
 void galerkin_gather(bh_ctx* ctx, int64_t nq, int bs, const int64_t* qptr, const int* plist, const int* prow, const int* fcol,
-                     const double* fv, const double* P, double* cv);
+                     const double* fv, const int* pidx, const double* P, double* cv);
 template <typename T> void p_restrict(bh_ctx*, int64_t J, int bsf, int bsc, const T* rf, T* rc);
 template <typename T> void p_prolong_add(bh_ctx*, int64_t J, int bsf, int bsc, const T* xc, T* xf);
-template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr, const int* cidx, const T* P, const T* rf, T* rc);
-template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const T* P, const T* xc, T* xf);
+// P: the distinct prolongation blocks; pidx[i]: block of fine cell i
+template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr, const int* cidx, const int* pidx, const T* P,
+                                       const T* rf, T* rc);
+template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc,
+                                          T* xf);
 template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
 // coarsest solve: FP64 inverse and accumulation for either cycle precision
-template <typename T> void dense_gemv(bh_ctx*, int n, const double* A, const T* b, T* x);
+template <typename T, typename M> void dense_gemv(bh_ctx*, int n, const M* A, const T* b, T* x);
 template <typename T> void vcopy(bh_ctx*, const T* s, T* d, int64_t n);
 void bsr_residual(bh_bsr* A, const void* b, const void* x, void* r, int prec);
 
@@ -57,12 +61,13 @@ struct MgLevel {
     DevBuf dinv64, dinv32;
     // transfer to the next coarser level: 0 = degree truncation, 1 = aggregation
     int kind = 0;
-    DevBuf cptr, cidx, parent, P64, P32;   // aggregation: children CSR, parents, P_i blocks
+    DevBuf cptr, cidx, parent, pidx;       // aggregation: children CSR, parents, P block index per fine cell
+    DevBuf P64, P32;                       // the distinct prolongation blocks (few on a regular grid)
     std::vector<double> lo, h;     // cell boxes (J x D), for the aggregation prolongation
     // work vectors (precision of the cycle)
     DevBuf b, x, r;
     // K-cycle storage (inner flexible Krylov on this level): rhs, V, Z, w
-    DevBuf kb, kv, kz, kw, ks;     // ks: KcycleScalars in device memory
+    DevBuf kv, kz, kw, ks;         // ks: KcycleScalars in device memory
     // additive Schwarz smoother over this level's aggregates (sw_cmax > 0: in use)
     int sw_cmax = 0, sw_nagg = 0;
     DevBuf sw64, sw32;
@@ -79,7 +84,8 @@ struct PMultigrid : bh_solver {
     int kin = 0;
     std::vector<MgLevel> lv;
     int ncoarse = 0;
-    DevBuf Ainv64;                 // coarsest level: dense FP64 inverse (both cycle precisions)
+    DevBuf Ainv64;                 // coarsest level: dense FP64 inverse (FP64 cycle)
+    DevBuf Ainv32;                 // its FP32 copy (FP32 cycle)
     DevBuf in32, out32;
     bh_bsr* ownedAf = nullptr;
     // learned correction after the cycle (guarded by residual minimisation)
@@ -103,7 +109,8 @@ struct PMultigrid : bh_solver {
     void build_grid(bh_ctx* ctx, bh_grid* grid, bh_bsr* Af, const std::vector<int>& degrees, int min_coarse, bh_op* mfop);
     void solve_dev(double* x, const double* b) override;        // x <- x + V(b - A x)
     void precondition(double* x, const double* b) override;     // x <- V(b)
-    template <typename T> void vcycle(int l);
+    template <typename T> void vcycle(int l, const T* b, T* x);   // x = (V-cycle from level l) b; scratch: level l's r and coarser
+    template <typename T> void vcycle(int l) { vcycle<T>(l, vec<T>(lv[l].b), vec<T>(lv[l].x)); }
     // V-cycles that contain no host synchronisation (the whole cycle without a
     // K-cycle, or the inner cycle of the K-cycle) are captured once into a HIP
     // graph and replayed: one launch instead of one per kernel of every level

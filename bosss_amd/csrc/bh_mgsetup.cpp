@@ -215,7 +215,8 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         // prolongation blocks
         // (the block depends only on the fine box relative to the coarse box:
         // cache by that relative geometry, quantised far below round-off of the setup)
-        std::vector<double> P((size_t)F.J * bs * bs);
+        std::vector<double> P;                  // the distinct blocks
+        std::vector<int> pidx(F.J);             // block of each fine cell
         std::map<std::vector<long long>, int> seen;
         for (int c = 0; c < F.J; ++c) {
             const double* lf = &F.lo[(size_t)c * D];
@@ -229,10 +230,13 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
             }
             auto it = seen.find(key);
             if (it != seen.end()) {
-                std::copy(&P[(size_t)it->second * bs * bs], &P[(size_t)(it->second + 1) * bs * bs], &P[(size_t)c * bs * bs]);
+                pidx[c] = it->second;
             } else {
-                projection_block(D, deg, lf, hf, lc, hcc, &P[(size_t)c * bs * bs]);
-                seen.emplace(std::move(key), c);
+                const int u = (int)(P.size() / ((size_t)bs * bs));
+                P.resize(P.size() + (size_t)bs * bs);
+                projection_block(D, deg, lf, hf, lc, hcc, &P[(size_t)u * bs * bs]);
+                seen.emplace(std::move(key), u);
+                pidx[c] = u;
             }
         }
         std::vector<int> cptr(Jc + 1, 0), cidx(F.J);
@@ -292,11 +296,13 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         F.cptr.alloc(cptr.size() * sizeof(int));
         F.cidx.alloc(cidx.size() * sizeof(int));
         F.parent.alloc(parent.size() * sizeof(int));
+        F.pidx.alloc(pidx.size() * sizeof(int));
         F.P64.alloc(P.size() * sizeof(double));
         F.P32.alloc(P.size() * sizeof(float));
         ctx->upload(F.cptr.p, cptr.data(), cptr.size() * sizeof(int));
         ctx->upload(F.cidx.p, cidx.data(), cidx.size() * sizeof(int));
         ctx->upload(F.parent.p, parent.data(), parent.size() * sizeof(int));
+        ctx->upload(F.pidx.p, pidx.data(), pidx.size() * sizeof(int));
         ctx->upload(F.P64.p, P.data(), P.size() * sizeof(double));
         convert_f64_f32(ctx, F.P64.as<double>(), F.P32.as<float>(), (int64_t)P.size());
         MgLevel C;
@@ -315,7 +321,7 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
             ctx->upload(dl.p, plist.data(), plist.size() * sizeof(int));
             ctx->upload(dr.p, prow.data(), prow.size() * sizeof(int));
             galerkin_gather(ctx, nq, bs, dq.as<int64_t>(), dl.as<int>(), dr.as<int>(), F.A->col.as<int>(),
-                            F.A->vals.as<double>(), F.P64.as<double>(), C.A->vals.as<double>());
+                            F.A->vals.as<double>(), F.pidx.as<int>(), F.P64.as<double>(), C.A->vals.as<double>());
             ctx->sync();
         }
         lv.push_back(std::move(C));
@@ -336,7 +342,6 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
     if (kcycle_level >= (int)lv.size() - 1) kcycle_level = -1;   // the coarsest level is solved directly
     if (kcycle_level >= 0 && kin > 0) {
         MgLevel& K = lv[kcycle_level];
-        K.kb.alloc(K.N * es);
         K.kv.alloc((size_t)(kin + 1) * K.N * es);
         K.kz.alloc((size_t)kin * K.N * es);
         K.kw.alloc(K.N * es);
@@ -367,6 +372,10 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         Ainv64.alloc(dense.size() * sizeof(double));
         dense_inverse_dev(ctx, Ad.as<double>(), Ainv64.as<double>(), ncoarse);
     }
+    if (prec == BH_FP32) {
+        Ainv32.alloc(dense.size() * sizeof(float));
+        convert_f64_f32(ctx, Ainv64.as<double>(), Ainv32.as<float>(), (int64_t)dense.size());
+    }
     ctx->sync();
 }
 
@@ -375,8 +384,7 @@ void PMultigrid::residual(int l, const T* b, const T* x, T* r) {
     MgLevel& L = lv[l];
     const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
     if (L.op) {
-        vcopy<T>(ctx, b, r, L.N);
-        sip_apply(L.op, -1.0, x, 1.0, r, false, op_prec(l, p == BH_FP32));
+        sip_apply(L.op, -1.0, x, 0.0, r, false, op_prec(l, p == BH_FP32), b);   // r = b - A x in one pass
     } else {
         bsr_residual(L.A, b, x, r, p);
     }
@@ -391,14 +399,13 @@ void PMultigrid::apply(int l, const T* x, T* y) {
 }
 
 template <typename T>
-void PMultigrid::vcycle(int l) {
+void PMultigrid::vcycle(int l, const T* b, T* x) {
     MgLevel& L = lv[l];
-    T* b = vec<T>(L.b);
-    T* x = vec<T>(L.x);
     T* r = vec<T>(L.r);
     const bool f32 = sizeof(T) == 4;
     if (l + 1 == (int)lv.size()) {
-        dense_gemv<T>(ctx, ncoarse, Ainv64.as<double>(), b, x);
+        if (sizeof(T) == 4 && Ainv32.p) dense_gemv<T, float>(ctx, ncoarse, Ainv32.as<float>(), b, x);
+        else dense_gemv<T, double>(ctx, ncoarse, Ainv64.as<double>(), b, x);
         return;
     }
     // pre-smoothing from a zero guess: the first sweep is x = omega M^-1 b
@@ -411,10 +418,10 @@ void PMultigrid::vcycle(int l) {
     MgLevel& C = lv[l + 1];
     const T* P = (const T*)(f32 ? L.P32.p : L.P64.p);
     if (L.kind == 0) p_restrict<T>(ctx, L.J, L.bs, C.bs, r, vec<T>(C.b));
-    else agg_restrict<T>(ctx, C.J, L.bs, L.cptr.as<int>(), L.cidx.as<int>(), P, r, vec<T>(C.b));
+    else agg_restrict<T>(ctx, C.J, L.bs, L.cptr.as<int>(), L.cidx.as<int>(), L.pidx.as<int>(), P, r, vec<T>(C.b));
     solve_level<T>(l + 1);
     if (L.kind == 0) p_prolong_add<T>(ctx, L.J, L.bs, C.bs, vec<T>(C.x), x);
-    else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), P, vec<T>(C.x), x);
+    else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), L.pidx.as<int>(), P, vec<T>(C.x), x);
     for (int s = 0; s < post; ++s) {
         residual<T>(l, b, x, r);
         smooth<T>(l, r, x, true);
@@ -477,19 +484,15 @@ template <typename T>
 void PMultigrid::kcycle(int l) {
     MgLevel& L = lv[l];
     const int64_t n = L.N;
-    T* kb = vec<T>(L.kb);
     T* V = vec<T>(L.kv);
     T* Z = vec<T>(L.kz);
     T* w = vec<T>(L.kw);
     double* S = L.ks.as<double>();
     const KcycleScalars K(kin);
-    vcopy<T>(ctx, vec<T>(L.b), kb, n);
-    multi_dot_to_dev<T>(ctx, nullptr, kb, 0, 1, n, S + K.B2);
-    scale_inv_norm<T>(ctx, kb, V, n, S + K.B2, 0.0);
+    multi_dot_to_dev<T>(ctx, nullptr, vec<T>(L.b), 0, 1, n, S + K.B2);
+    scale_inv_norm<T>(ctx, vec<T>(L.b), V, n, S + K.B2, 0.0);
     for (int j = 0; j < kin; ++j) {
-        vcopy<T>(ctx, V + (size_t)j * n, vec<T>(L.b), n);
-        vcycle_g<T>(l);
-        vcopy<T>(ctx, vec<T>(L.x), Z + (size_t)j * n, n);
+        vcycle<T>(l, V + (size_t)j * n, Z + (size_t)j * n);   // Z_j = (V-cycle from this level) V_j
         apply<T>(l, Z + (size_t)j * n, w);
         multi_dot_to_dev<T>(ctx, w, V, n, j + 1, n, S + K.H1);
         multi_axpy_from_dev<T>(ctx, w, V, n, j + 1, n, S + K.H1, nullptr, -1.0);
@@ -594,8 +597,8 @@ void PMultigrid::solve_dev(double* x, const double* b) {
     converged = 1;
 }
 
-template void PMultigrid::vcycle<float>(int);
-template void PMultigrid::vcycle<double>(int);
+template void PMultigrid::vcycle<float>(int, const float*, float*);
+template void PMultigrid::vcycle<double>(int, const double*, double*);
 template void PMultigrid::vcycle_g<float>(int);
 template void PMultigrid::vcycle_g<double>(int);
 template void PMultigrid::cycle_body<float>(int);

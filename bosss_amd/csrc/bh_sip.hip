@@ -41,6 +41,7 @@ struct SipArgs {
     R* __restrict__ y;
     const double* __restrict__ affine;
     double alpha, beta;
+    const R* __restrict__ b0;   // if set: y = b0 + alpha * (A u)  (residual form; beta ignored)
 };
 
 constexpr int binom(int n, int k) {
@@ -377,7 +378,11 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
 #pragma unroll
         for (int n = 0; n < Np; ++n) y[n] += R(af[n]);
     }
-    if (a.beta == 0.0) {
+    if (a.b0) {
+        const R* bp = a.b0 + (size_t)c * Np;
+#pragma unroll
+        for (int n = 0; n < Np; ++n) yp[n] = bp[n] + al * y[n];
+    } else if (a.beta == 0.0) {
 #pragma unroll
         for (int n = 0; n < Np; ++n) yp[n] = al * y[n];
     } else {
@@ -544,21 +549,26 @@ static void tune_fp64_path(bh_op* op) {
     op->f64_path = best < tv ? 2 : 1;
 }
 
-void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec) {
+void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec, const void* b0) {
     if (add_affine && op->has_affine && !op->affine_valid) op->rebuild_affine();
     if (prec == BH_FP64) {
         if (op->f64_path == 0) tune_fp64_path(op);
         if (op->f64_path == 2) {
             const double* aff = (add_affine && op->has_affine) ? op->affine.as<double>() : nullptr;
-            sip_apply_mfma64(op, alpha, (const double*)u, beta, (double*)y, aff, op->f64_waves, op->f64_wg_per_xcd);
+            sip_apply_mfma64(op, alpha, (const double*)u, beta, (double*)y, aff, op->f64_waves, op->f64_wg_per_xcd,
+                             (const double*)b0);
             return;
         }
-        dispatch_apply<double>(op, make_args<double>(op, u, y, alpha, beta, add_affine));
+        SipArgs<double> a = make_args<double>(op, u, y, alpha, beta, add_affine);
+        a.b0 = (const double*)b0;
+        dispatch_apply<double>(op, a);
     } else if (prec == BH_FP32) {
         if (add_affine && op->has_affine) throw Error(-3, "FP32 evaluate supports the linear part only");
-        dispatch_apply<float>(op, make_args<float>(op, u, y, alpha, beta, false));
+        SipArgs<float> a = make_args<float>(op, u, y, alpha, beta, false);
+        a.b0 = (const float*)b0;
+        dispatch_apply<float>(op, a);
     } else if (prec == BH_FP16_MFMA) {
-        sip_apply_mfma(op, alpha, (const float*)u, beta, (float*)y, add_affine);
+        sip_apply_mfma(op, alpha, (const float*)u, beta, (float*)y, add_affine, (const float*)b0);
     } else {
         throw Error(-3, "sip_apply: precision not supported by this entry point");
     }

@@ -105,6 +105,7 @@ struct MfmaArgs {
     const double* __restrict__ affine;
     float alpha, beta;
     int tiles_per_xcd;   // tiles are split into 8 contiguous ranges, one per XCD
+    const float* __restrict__ b0;   // if set: y = b0 + alpha * (A u)
 };
 
 __device__ __forceinline__ h8_t lds_h8(const _Float16* p) { return *(const h8_t*)p; }
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
                     const size_t o = (size_t)co * Np + m;
                     float v = yacc[mt][i];
                     if (a.affine) v += (float)a.affine[o];
-                    a.y[o] = a.beta == 0.f ? a.alpha * v : a.beta * a.y[o] + a.alpha * v;
+                    a.y[o] = a.b0 ? a.b0[o] + a.alpha * v : (a.beta == 0.f ? a.alpha * v : a.beta * a.y[o] + a.alpha * v);
                 }
             }
         }
@@ -365,6 +366,7 @@ static void mfma_launch(bh_op* op, const MfmaArgs& a0) {
         MfmaArgs t = a;
         t.y = ys.as<float>();
         t.beta = 0.f;
+        t.b0 = nullptr;
         t.affine = nullptr;
         float best = 1e30f;
         int bw = 8, bx = 64;
@@ -395,7 +397,7 @@ static void mfma_launch(bh_op* op, const MfmaArgs& a0) {
     mfma_run_cfg<D, K>(op, a, op->mfma_waves, op->mfma_wg_per_xcd);
 }
 
-void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine) {
+void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float* y, bool add_affine, const float* b0) {
     if (add_affine && op->has_affine && !op->affine_valid) op->rebuild_affine();
     bh_grid* g = op->grid;
     MfmaArgs a{};
@@ -412,6 +414,7 @@ void sip_apply_mfma(bh_op* op, double alpha, const float* u, double beta, float*
     a.affine = (add_affine && op->has_affine) ? op->affine.as<double>() : nullptr;
     a.alpha = (float)alpha;
     a.beta = (float)beta;
+    a.b0 = b0;
     const int D = op->D, K = op->k;
     if (D == 3 && K == 3) return mfma_launch<3, 3>(op, a);
     if (D == 3 && K == 2) return mfma_launch<3, 2>(op, a);

@@ -206,6 +206,7 @@ struct M64Args {
     const double* __restrict__ affine;
     double alpha, beta;
     int tiles_per_xcd;
+    const double* __restrict__ b0;   // if set: y = b0 + alpha * (A u)
 };
 
 template <int D>
@@ -472,7 +473,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 8 ?
                 const size_t o = (size_t)co * Np + tcol;
                 double v = ytail;
                 if (a.affine) v += a.affine[o];
-                a.y[o] = a.beta == 0.0 ? a.alpha * v : a.beta * a.y[o] + a.alpha * v;
+                a.y[o] = a.b0 ? a.b0[o] + a.alpha * v : (a.beta == 0.0 ? a.alpha * v : a.beta * a.y[o] + a.alpha * v);
             }
         }
 #pragma unroll
@@ -485,7 +486,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 8 ?
                     const size_t o = (size_t)co * Np + m;
                     double v = yacc[mt][r];
                     if (a.affine) v += a.affine[o];
-                    a.y[o] = a.beta == 0.0 ? a.alpha * v : a.beta * a.y[o] + a.alpha * v;
+                    a.y[o] = a.b0 ? a.b0[o] + a.alpha * v : (a.beta == 0.0 ? a.alpha * v : a.beta * a.y[o] + a.alpha * v);
                 }
             }
         }
@@ -523,7 +524,7 @@ bool sip_apply_mfma64_supported(int D, int K, int64_t N) {
 }
 
 void sip_apply_mfma64(bh_op* op, double alpha, const double* u, double beta, double* y, const double* affine, int waves,
-                      int per_xcd) {
+                      int per_xcd, const double* b0) {
     bh_grid* g = op->grid;
     M64Args a{};
     a.J = g->J;
@@ -538,6 +539,7 @@ void sip_apply_mfma64(bh_op* op, double alpha, const double* u, double beta, dou
     a.affine = affine;
     a.alpha = alpha;
     a.beta = beta;
+    a.b0 = b0;
     const int D = op->D, K = op->k;
     if (D == 3 && K == 3) return m64_launch<3, 3>(op, a, waves, per_xcd);
     if (D == 3 && K == 2) return m64_launch<3, 2>(op, a, waves, per_xcd);
