@@ -262,6 +262,9 @@ template <typename T> void multi_axpy_from_dev(bh_ctx* ctx, T* w, const T* V, in
                                                const double* dscale, double sign);
 // w -= sum_k h[k] V_k  (h host array, m <= 128)
 void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign);
+// w <- scale (w - sum_{k<m} c_k V_k), m <= 31, in one pass; with dots: dots[k] = <V_k, w_new>, *wn2 = |w_new|^2 (host)
+void axpy_scale_dots(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* c, double scale,
+                     double* dots, double* wn2);
 }  // namespace bh
 
 // extern "C" wrappers: translate exceptions into error codes + last-error text

@@ -495,7 +495,7 @@ __global__ __launch_bounds__(256) void reduce_parts_k(const double* __restrict__
 }
 
 static void ensure_red(bh_ctx* ctx) {
-    if (!ctx->red.p) ctx->red.alloc((size_t)MD_BLOCKS * MD_M * sizeof(double));
+    if (!ctx->red.p) ctx->red.alloc((size_t)MD_BLOCKS * 32 * sizeof(double));   // multi-dot (16) and fused passes (32)
     if (!ctx->red_out.p) ctx->red_out.alloc(256 * sizeof(double));
     if (!ctx->host_red) BH_HIP(hipHostMalloc((void**)&ctx->host_red, 256 * sizeof(double), hipHostMallocDefault));
 }
@@ -635,6 +635,90 @@ void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, 
 
 void multi_axpy(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* h, double sign) {
     multi_axpy_t<double>(ctx, w, V, ldv, m, n, h, sign);
+}
+
+// ------------------------------------------------ fused Gram-Schmidt passes (FP64)
+// w <- scale * (w - sum_k c_k V_k) for M <= 31 vectors (M a template parameter, so
+// the M loads of an element issue together), each element's V values held in
+// registers; DOT: the same pass also forms the partial sums of <V_k, w_new>
+// (k < M) and <w_new, w_new> (column M), reduced by reduce_parts_k.  One read of
+// V where a projection followed by a multi-dot needs two.
+template <bool DOT, int M>
+__global__ __launch_bounds__(256) void axpy_dot_k(double* __restrict__ w, const double* __restrict__ V, int64_t ldv, int64_t n,
+                                                  Coefs c, double scale, double* __restrict__ part) {
+    double acc[DOT ? M + 1 : 1];
+#pragma unroll
+    for (int k = 0; k < (DOT ? M + 1 : 1); ++k) acc[k] = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        double v[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) v[k] = V[(size_t)k * ldv + i];
+        double s = w[i];
+#pragma unroll
+        for (int k = 0; k < M; ++k) s -= c.c[k] * v[k];
+        s *= scale;
+        w[i] = s;
+        if constexpr (DOT) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) acc[k] += v[k] * s;
+            acc[M] += s * s;
+        }
+    }
+    if constexpr (DOT) {
+        __shared__ double sm[4][M + 1];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k <= M; ++k) {
+            double x = acc[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if (lane == 0) sm[wave][k] = x;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x <= M) {
+            const int k = threadIdx.x;
+            part[(size_t)blockIdx.x * (M + 1) + k] = sm[0][k] + sm[1][k] + sm[2][k] + sm[3][k];
+        }
+    }
+}
+
+template <int M>
+static void axpy_dot_launch(bh_ctx* ctx, bool dot, double* w, const double* V, int64_t ldv, int64_t n, const Coefs& c,
+                            double scale, int nb) {
+    if (dot) {
+        hipLaunchKernelGGL((axpy_dot_k<true, M>), dim3(nb), dim3(256), 0, ctx->stream, w, V, ldv, n, c, scale,
+                           ctx->red.as<double>());
+        hipLaunchKernelGGL(reduce_parts_k, dim3(M + 1), dim3(256), 0, ctx->stream, ctx->red.as<double>(), nb, M + 1,
+                           ctx->red_out.as<double>());
+    } else {
+        hipLaunchKernelGGL((axpy_dot_k<false, M>), dim3(nb), dim3(256), 0, ctx->stream, w, V, ldv, n, c, scale,
+                           ctx->red.as<double>());
+    }
+}
+
+void axpy_scale_dots(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m, int64_t n, const double* c, double scale,
+                     double* dots, double* wn2) {
+    BH_CHECK(m >= 1 && m <= 31, "axpy_scale_dots: 1..31 vectors");
+    ensure_red(ctx);
+    Coefs cf{};
+    for (int k = 0; k < m; ++k) cf.c[k] = c[k];
+    int64_t nb = (n + 1023) / 1024;
+    if (nb > MD_BLOCKS) nb = MD_BLOCKS;
+    if (nb < 1) nb = 1;
+    const bool dot = dots != nullptr;
+    switch (m) {
+#define BH_AD(MM) case MM: axpy_dot_launch<MM>(ctx, dot, w, V, ldv, n, cf, scale, (int)nb); break;
+        BH_AD(1) BH_AD(2) BH_AD(3) BH_AD(4) BH_AD(5) BH_AD(6) BH_AD(7) BH_AD(8) BH_AD(9) BH_AD(10) BH_AD(11) BH_AD(12)
+        BH_AD(13) BH_AD(14) BH_AD(15) BH_AD(16) BH_AD(17) BH_AD(18) BH_AD(19) BH_AD(20) BH_AD(21) BH_AD(22) BH_AD(23)
+        BH_AD(24) BH_AD(25) BH_AD(26) BH_AD(27) BH_AD(28) BH_AD(29) BH_AD(30) BH_AD(31)
+#undef BH_AD
+    }
+    BH_HIP(hipGetLastError());
+    if (!dot) return;
+    BH_HIP(hipMemcpyAsync(ctx->host_red, ctx->red_out.p, (m + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < m; ++k) dots[k] = ctx->host_red[k];
+    *wn2 = ctx->host_red[m];
 }
 
 // ------------------------------------------------ device-resident variants

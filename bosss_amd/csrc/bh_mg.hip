@@ -119,6 +119,50 @@ __global__ void agg_prolong_add_k(int Jf, int bs, const int* __restrict__ parent
     }
 }
 
+// The same transfers with the block size known at compile time: one thread per
+// coarse cell (restriction) or fine cell (prolongation) moves whole blocks.
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void agg_restrict_bs_k(int Jc, const int* __restrict__ cptr, const int* __restrict__ cidx,
+                                                         const int* __restrict__ pidx, const T* __restrict__ P,
+                                                         const T* __restrict__ rf, T* __restrict__ rc) {
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < Jc; I += gridDim.x * blockDim.x) {
+        T acc[BS] = {};
+        const int p0 = cptr[I], p1 = cptr[I + 1];
+        for (int p = p0; p < p1; ++p) {
+            const int i = cidx[p];
+            const T* __restrict__ Pi = P + (size_t)pidx[i] * BS * BS;
+            T r[BS];
+#pragma unroll
+            for (int m = 0; m < BS; ++m) r[m] = rf[(size_t)i * BS + m];
+#pragma unroll
+            for (int m = 0; m < BS; ++m)
+#pragma unroll
+                for (int n = 0; n < BS; ++n) acc[n] += Pi[m * BS + n] * r[m];
+        }
+#pragma unroll
+        for (int n = 0; n < BS; ++n) rc[(size_t)I * BS + n] = acc[n];
+    }
+}
+
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void agg_prolong_add_bs_k(int Jf, const int* __restrict__ parent, const int* __restrict__ pidx,
+                                                            const T* __restrict__ P, const T* __restrict__ xc,
+                                                            T* __restrict__ xf) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Jf; i += gridDim.x * blockDim.x) {
+        const T* __restrict__ Pi = P + (size_t)pidx[i] * BS * BS;
+        const T* __restrict__ xcI = xc + (size_t)parent[i] * BS;
+        T c[BS], y[BS];
+#pragma unroll
+        for (int n = 0; n < BS; ++n) { c[n] = xcI[n]; y[n] = xf[(size_t)i * BS + n]; }
+#pragma unroll
+        for (int m = 0; m < BS; ++m)
+#pragma unroll
+            for (int n = 0; n < BS; ++n) y[m] += Pi[m * BS + n] * c[n];
+#pragma unroll
+        for (int m = 0; m < BS; ++m) xf[(size_t)i * BS + m] = y[m];
+    }
+}
+
 // x (+)= omega * Dinv r.  A workgroup takes R consecutive block rows: their
 // inverse blocks (contiguous, column-major) and residual pieces are staged in
 // LDS with coalesced loads by all 256 threads, then thread (row, i) forms
@@ -383,13 +427,23 @@ void p_prolong_add(bh_ctx* ctx, int64_t J, int bsf, int bsc, const T* xc, T* xf)
 }
 template <typename T>
 void agg_restrict(bh_ctx* ctx, int Jc, int bs, const int* cptr, const int* cidx, const int* pidx, const T* P, const T* rf, T* rc) {
-    hipLaunchKernelGGL((agg_restrict_k<T>), grid1d((int64_t)Jc * bs), dim3(256), 0, ctx->stream, Jc, bs, cptr, cidx, pidx, P, rf,
-                       rc);
+    if (bs == 4)
+        hipLaunchKernelGGL((agg_restrict_bs_k<T, 4>), grid1d(Jc), dim3(256), 0, ctx->stream, Jc, cptr, cidx, pidx, P, rf, rc);
+    else if (bs == 3)
+        hipLaunchKernelGGL((agg_restrict_bs_k<T, 3>), grid1d(Jc), dim3(256), 0, ctx->stream, Jc, cptr, cidx, pidx, P, rf, rc);
+    else
+        hipLaunchKernelGGL((agg_restrict_k<T>), grid1d((int64_t)Jc * bs), dim3(256), 0, ctx->stream, Jc, bs, cptr, cidx, pidx, P,
+                           rf, rc);
 }
 template <typename T>
 void agg_prolong_add(bh_ctx* ctx, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc, T* xf) {
-    hipLaunchKernelGGL((agg_prolong_add_k<T>), grid1d((int64_t)Jf * bs), dim3(256), 0, ctx->stream, Jf, bs, parent, pidx, P, xc,
-                       xf);
+    if (bs == 4)
+        hipLaunchKernelGGL((agg_prolong_add_bs_k<T, 4>), grid1d(Jf), dim3(256), 0, ctx->stream, Jf, parent, pidx, P, xc, xf);
+    else if (bs == 3)
+        hipLaunchKernelGGL((agg_prolong_add_bs_k<T, 3>), grid1d(Jf), dim3(256), 0, ctx->stream, Jf, parent, pidx, P, xc, xf);
+    else
+        hipLaunchKernelGGL((agg_prolong_add_k<T>), grid1d((int64_t)Jf * bs), dim3(256), 0, ctx->stream, Jf, bs, parent, pidx, P,
+                           xc, xf);
 }
 
 template <typename T>

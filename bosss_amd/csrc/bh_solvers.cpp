@@ -119,23 +119,51 @@ static void cgs2(bh_ctx* ctx, double* w, const double* V, int64_t N, int i, doub
 // Kaufman-Stewart criterion): w is stored in slot i of V, so one multi-dot pass
 // gives <w, V_k> (k < i) and |w|^2.  After the projection, |w'|^2 = |w|^2 - sum h^2;
 // only when that drops below |w|^2 / 2 (cancellation: w nearly in span V) is a
-// second pass made.  Returns the projected |w| (explicit norm).
+// second pass made.  The criterion is known before the projection, so the
+// projection, the second pass's multi-dot and the normalisation are fused:
+//   no second pass:  w <- (w - V h1) / sqrt(|w|^2 - |h1|^2)          (one pass)
+//   second pass:     w <- w - V h1 with h2 = V^T w', |w'|^2 in the same pass;
+//                    w <- (w' - V h2) / sqrt(|w'|^2 - |h2|^2)          (two passes)
+// (the norms by Pythagoras, well conditioned in both cases).  Returns the norm;
+// w is left normalised.  More than 31 vectors: separate passes, explicit norm.
 static double cgs_selective(bh_ctx* ctx, double* V, int64_t N, int i, double* hcol, int* passes) {
     double* w = V + (size_t)i * N;
     std::vector<double> h1(i + 1), h2(i);
     multi_dot(ctx, w, V, N, i + 1, N, h1.data());
     const double w2 = h1[i];
-    multi_axpy(ctx, w, V, N, i, N, h1.data(), -1.0);
     double proj = 0;
     for (int k = 0; k < i; ++k) { hcol[k] = h1[k]; proj += h1[k] * h1[k]; }
-    *passes = 1;
-    if (!(w2 - proj > 0.5 * w2)) {
-        multi_dot(ctx, w, V, N, i, N, h2.data());
-        multi_axpy(ctx, w, V, N, i, N, h2.data(), -1.0);
-        for (int k = 0; k < i; ++k) hcol[k] += h2[k];
-        *passes = 2;
+    const bool second = !(w2 - proj > 0.5 * w2);
+    *passes = second ? 2 : 1;
+    if (i >= 1 && i <= 31) {
+        if (!second) {
+            const double nw = std::sqrt(w2 - proj);
+            axpy_scale_dots(ctx, w, V, N, i, N, h1.data(), nw > 0 ? 1.0 / nw : 1.0, nullptr, nullptr);
+            return nw;
+        }
+        double n2 = 0;
+        axpy_scale_dots(ctx, w, V, N, i, N, h1.data(), 1.0, h2.data(), &n2);
+        double p2 = 0;
+        for (int k = 0; k < i; ++k) { hcol[k] += h2[k]; p2 += h2[k] * h2[k]; }
+        const double nn = n2 - p2;
+        if (nn > 1e-4 * n2) {
+            const double nw = std::sqrt(nn);
+            axpy_scale_dots(ctx, w, V, N, i, N, h2.data(), 1.0 / nw, nullptr, nullptr);
+            return nw;
+        }
+        // w' still nearly in span V: project again, explicit norm
+        axpy_scale_dots(ctx, w, V, N, i, N, h2.data(), 1.0, nullptr, nullptr);
+    } else {
+        multi_axpy(ctx, w, V, N, i, N, h1.data(), -1.0);
+        if (second) {
+            multi_dot(ctx, w, V, N, i, N, h2.data());
+            multi_axpy(ctx, w, V, N, i, N, h2.data(), -1.0);
+            for (int k = 0; k < i; ++k) hcol[k] += h2[k];
+        }
     }
-    return nrm2(ctx, w, N);
+    const double nw = nrm2(ctx, w, N);
+    if (nw > 0) axpby(ctx, 1.0 / nw, w, 0.0, w, N);
+    return nw;
 }
 
 static void back_solve(const std::vector<double>& H, int ld, int i, const double* s, double* y) {
@@ -274,8 +302,7 @@ void GmresSolver::solve_flexible(double* X, const double* B) {
             const double nw = cgs_selective(ctx, Vp, N, i, hcol.data(), &passes);
             reorth_passes += passes - 1;
             for (int k = 0; k < i; ++k) H[k * m + i - 1] = hcol[k];
-            H[i * m + i - 1] = nw;
-            if (nw > 0) axpby(ctx, 1.0 / nw, vi, 0.0, vi, N);
+            H[i * m + i - 1] = nw;   // vi is normalised
             for (int k = 1; k <= i - 1; ++k) {
                 double t = cs[k - 1] * H[(k - 1) * m + i - 1] + sn[k - 1] * H[k * m + i - 1];
                 H[k * m + i - 1] = -sn[k - 1] * H[(k - 1) * m + i - 1] + cs[k - 1] * H[k * m + i - 1];
