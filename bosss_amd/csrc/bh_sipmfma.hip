@@ -148,12 +148,11 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
         const int cc = valid ? c : a.J - 1;
         int idx[3];
         decode_cell<D>(cc, a.n0, a.n1, idx);
-        float hP[3] = {1.f, 1.f, 1.f}, ih[3] = {1.f, 1.f, 1.f};
-        float sdPinv = 1.f, cjP = 0.f;
+        // widths from the grid table [4][3][nmax]: h, 1/h, sqrt(h), 1/sqrt(h)
+        float ih[3] = {1.f, 1.f, 1.f};
+        float cjP = 0.f;
         for (int d = 0; d < D; ++d) {
-            hP[d] = (float)a.width[d * a.nmax + idx[d]];
-            ih[d] = 1.f / hP[d];
-            sdPinv *= sqrtf(2.f * ih[d]);
+            ih[d] = (float)a.width[(3 + d) * a.nmax + idx[d]];
             cjP += (1.f + 0.5f * (float)((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
         }
         const float nuP = a.nu[cc];
@@ -208,15 +207,15 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
         // ---------------- faces.  PF: all six neighbour loads are issued first so their
         // latencies overlap (more registers); otherwise each face loads when it starts.
         constexpr int NPF = PF ? 2 * D : 1;
-        float uqr[NPF][8], nuQa[NPF], hqa[NPF], usQa[NPF];
+        float uqr[NPF][8], nuQa[NPF], ihqa[NPF], usQa[NPF];
         h8_t bQa[NPF];
-        auto load_face = [&](int f, float* uq, float& nuq, float& hq) {
+        auto load_face = [&](int f, float* uq, float& nuq, float& ihq) {
             const int d = f >> 1, s = f & 1;
             const bool isBnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
             int nidx[3] = {idx[0], idx[1], idx[2]};
             nidx[d] += isBnd ? 0 : (s ? 1 : -1);
             const int cq = isBnd ? cc : encode_cell<D>(nidx, a.n0, a.n1);
-            hq = isBnd ? hP[d] : (float)a.width[d * a.nmax + nidx[d]];
+            ihq = (float)a.width[(3 + d) * a.nmax + nidx[d]];
             nuq = isBnd ? nuP : a.nu[cq];
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -236,7 +235,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
         };
         if constexpr (PF) {
 #pragma unroll
-            for (int f = 0; f < 2 * D; ++f) load_face(f, uqr[f], nuQa[f], hqa[f]);
+            for (int f = 0; f < 2 * D; ++f) load_face(f, uqr[f], nuQa[f], ihqa[f]);
 #pragma unroll
             for (int f = 0; f < 2 * D; ++f) to_h8(uqr[f], bQa[f], usQa[f]);
         }
@@ -246,7 +245,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
             if constexpr (PF) {
                 __builtin_amdgcn_sched_barrier(0);   // one face at a time: bounds the register pressure
             } else {
-                load_face(f, uqr[0], nuQa[0], hqa[0]);
+                load_face(f, uqr[0], nuQa[0], ihqa[0]);
                 to_h8(uqr[0], bQa[0], usQa[0]);
             }
             const int d = f >> 1, s = f & 1;
@@ -256,7 +255,7 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
             const bool skip = isBnd && ((a.bc_neumann >> f) & 1);
             int nidx[3] = {idx[0], idx[1], idx[2]};
             nidx[d] += isBnd ? 0 : (s ? 1 : -1);
-            const float ihq = 1.f / hqa[fi];
+            const float ihq = ihqa[fi];
             const float usQ = usQa[fi];
             const h8_t bQ = bQa[fi];
             const f4_t z = {0.f, 0.f, 0.f, 0.f};
@@ -266,32 +265,30 @@ __global__ __launch_bounds__(64 * MFMA_WAVES) void sip_apply_mfma_kernel(MfmaArg
             const f4_t eN = __builtin_amdgcn_mfma_f32_16x16x32_f16(lds_h8(&tab[L::FA + ((fQ * 2 + 1) * 16 + col) * 32 + 8 * grp]), bQ, z, 0, 0, 0);
             // face scalars of this lane's cell (float; only direction d differs for Q)
             const float gP = 2.f * ih[d], gQ = 2.f * ihq;
-            const float sdQinv = sdPinv * sqrtf(hP[d] * ihq);
-            const float wf = gP / sdPinv;              // dA * sdPinv = (2/h_d) * detJ * sdPinv, detJ = 1/sdPinv^2
+            const float rq = (float)(a.width[(6 + d) * a.nmax + idx[d]] * a.width[(9 + d) * a.nmax + nidx[d]]);   // sdQinv / sdPinv
             const float nuQ = nuQa[fi];
             const float nuMax = fabsf(nuP) > fabsf(nuQ) ? nuP : nuQ;
             const float bP = 1.f + 0.5f * (float)((idx[d] == 0) + (idx[d] == nn[d] - 1));
             const float bQf = 1.f + 0.5f * (float)((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
             const float cjQ = cjP - bP * ih[d] + bQf * ihq;
             const float mu = isBnd ? a.pen * cjP : a.pen * fmaxf(cjP, cjQ);
+            // the SIP flux as linear combinations of the FP16-scaled traces (scales usP, usQ and
+            // the per-side 1/sqrt(det J), normal sign and 2/h folded into the coefficients;
+            // dA sdPinv^2 = 2/h_d):
+            //   cvw = c1 dN + c2 eN + c3 dT + c4 eT,  cdw = c1 dT + c5 eT
+            const float G = (skip || !valid) ? 0.f : gP;
+            const float A = G * nuP * sgn * gP;
+            const float c3u = -G * mu * (isBnd ? 2.f * nuP : nuMax);
+            const float c1 = (isBnd ? 1.f : 0.5f) * A * usP;
+            const float c3 = c3u * usP;
+            const float c2 = isBnd ? 0.f : 0.5f * G * nuQ * sgn * gQ * rq * usQ;
+            const float c4 = isBnd ? 0.f : -c3u * rq * usQ;
+            const float c5 = isBnd ? 0.f : -0.5f * A * rq * usQ;
             float cvw[4], cdw[4], sm = 0.f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float vP = sdPinv * usP * dT[i];
-                const float dP = sgn * gP * sdPinv * usP * dN[i];
-                const float vQ = sdQinv * usQ * eT[i];
-                const float dQ = sgn * gQ * sdQinv * usQ * eN[i];
-                float cv, cd;
-                if (isBnd) {
-                    cv = nuP * dP - 2.f * mu * nuP * vP;
-                    cd = nuP * vP;
-                } else {
-                    const float jump = vP - vQ;
-                    cv = 0.5f * (nuP * dP + nuQ * dQ) - mu * nuMax * jump;
-                    cd = 0.5f * nuP * jump;
-                }
-                cvw[i] = (skip || !valid) ? 0.f : wf * cv;
-                cdw[i] = (skip || !valid) ? 0.f : wf * sgn * gP * cd;
+                cvw[i] = c1 * dN[i] + c2 * eN[i] + c3 * dT[i] + c4 * eT[i];
+                cdw[i] = c1 * dT[i] + c5 * eT[i];
                 sm = fmaxf(sm, fmaxf(fabsf(cvw[i]), fabsf(cdw[i])));
             }
             sm = lane_max4(sm);
