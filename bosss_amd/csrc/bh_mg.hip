@@ -208,6 +208,59 @@ __global__ __launch_bounds__(256) void block_gemv_k(int nrows, const T* __restri
     }
 }
 
+// The same smoother step with the inverse blocks stored in FP16, each block
+// scaled by its largest entry (scale[row], FP32): half the bytes of the FP32
+// blocks, FP32 vectors and FP32 accumulation.
+template <int BS, bool ADD>
+__global__ __launch_bounds__(256) void block_gemv_h_k(int nrows, const _Float16* __restrict__ Dinv, const float* __restrict__ scale,
+                                                      const float* __restrict__ r, float* __restrict__ x, float omega) {
+    constexpr int R = BlockGemvShape<float, BS>::R;
+    __shared__ __attribute__((aligned(16))) _Float16 sD[R * BS * BS + 8];
+    __shared__ float sr[R * BS];
+    const int row0 = blockIdx.x * R;
+    const int nr = min(R, nrows - row0);
+    const _Float16* __restrict__ Dg = Dinv + (size_t)row0 * BS * BS;
+    const float* __restrict__ rg = r + (size_t)row0 * BS;
+    if ((R * BS * BS) % 8 == 0 && (nr * BS * BS) % 8 == 0) {
+        const int nv = nr * BS * BS / 8;
+        for (int t = threadIdx.x; t < nv; t += blockDim.x) ((uint4*)sD)[t] = ((const uint4*)Dg)[t];
+    } else {
+        for (int t = threadIdx.x; t < nr * BS * BS; t += blockDim.x) sD[t] = Dg[t];
+    }
+    for (int t = threadIdx.x; t < nr * BS; t += blockDim.x) sr[t] = rg[t];
+    __syncthreads();
+    const int row = threadIdx.x / BS, i = threadIdx.x - row * BS;
+    if (row < nr) {
+        const _Float16* __restrict__ Di = sD + row * BS * BS;
+        const float* __restrict__ ri = sr + row * BS;
+        float acc = 0;
+#pragma unroll 4
+        for (int c = 0; c < BS; ++c) acc += (float)Di[c * BS + i] * ri[c];
+        const size_t o = (size_t)(row0 + row) * BS + i;
+        const float f = omega * scale[row0 + row];
+        x[o] = ADD ? x[o] + f * acc : f * acc;
+    }
+}
+
+// FP16 copy of FP32 blocks: per block, the largest |entry| is the scale, entries / scale in FP16
+__global__ __launch_bounds__(256) void blocks_to_f16_k(int nblocks, int bb, const float* __restrict__ src, _Float16* __restrict__ dst,
+                                                       float* __restrict__ scale) {
+    __shared__ float sm[4];
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const float* s = src + (size_t)b * bb;
+        float m = 0.f;
+        for (int t = threadIdx.x; t < bb; t += blockDim.x) m = fmaxf(m, fabsf(s[t]));
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+        const float sc = m > 0.f ? m : 1.f, rs = 1.f / sc;
+        for (int t = threadIdx.x; t < bb; t += blockDim.x) dst[(size_t)b * bb + t] = (_Float16)(s[t] * rs);
+        if (threadIdx.x == 0) scale[b] = sc;
+    }
+}
+
 // x = Ainv b (dense, row-major n x n), one wave per row, accumulated in double.
 // VEC (n a multiple of 4): each lane reads 4 consecutive entries per step, two
 // steps in flight.
@@ -460,6 +513,29 @@ void block_gemv(bh_ctx* ctx, int nrows, int bs, const T* Dinv, const T* r, T* x,
         BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
         BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)
         default: throw Error(-3, "block_gemv: unsupported block size");
+    }
+#undef BH_CASE
+    BH_HIP(hipGetLastError());
+}
+
+void blocks_to_f16(bh_ctx* ctx, int nblocks, int bs, const float* src, _Float16* dst, float* scale) {
+    hipLaunchKernelGGL(blocks_to_f16_k, dim3((unsigned)std::min(nblocks, 16384)), dim3(256), 0, ctx->stream, nblocks, bs * bs, src,
+                       dst, scale);
+    BH_HIP(hipGetLastError());
+}
+
+void block_gemv_h(bh_ctx* ctx, int nrows, int bs, const _Float16* Dinv, const float* scale, const float* r, float* x, double omega,
+                  bool add) {
+    dim3 blk(256), grd;
+#define BH_CASE(B)                                                                                                   \
+    case B:                                                                                                          \
+        grd = dim3((nrows + BlockGemvShape<float, B>::R - 1) / BlockGemvShape<float, B>::R);                        \
+        if (add) hipLaunchKernelGGL((block_gemv_h_k<B, true>), grd, blk, 0, ctx->stream, nrows, Dinv, scale, r, x, (float)omega); \
+        else hipLaunchKernelGGL((block_gemv_h_k<B, false>), grd, blk, 0, ctx->stream, nrows, Dinv, scale, r, x, (float)omega); \
+        break;
+    switch (bs) {
+        BH_CASE(10) BH_CASE(15) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(35)
+        default: throw Error(-3, "block_gemv (FP16 blocks): unsupported block size");
     }
 #undef BH_CASE
     BH_HIP(hipGetLastError());

@@ -21,6 +21,10 @@ template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr
 template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc,
                                           T* xf);
 template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
+// FP16 inverse blocks scaled per block (FP32 vectors): bs in {10, 15, 20, 21, 28, 35}
+void blocks_to_f16(bh_ctx*, int nblocks, int bs, const float* src, _Float16* dst, float* scale);
+void block_gemv_h(bh_ctx*, int nrows, int bs, const _Float16* Dinv, const float* scale, const float* r, float* x, double omega,
+                  bool add);
 // coarsest solve: FP64 inverse and accumulation for either cycle precision
 template <typename T, typename M> void dense_gemv(bh_ctx*, int n, const M* A, const T* b, T* x);
 template <typename T> void vcopy(bh_ctx*, const T* s, T* d, int64_t n);
@@ -59,6 +63,7 @@ struct MgLevel {
     bh_op* op = nullptr;           // matrix-free operator (finest level; coarse p-levels of a matrix-free hierarchy)
     std::unique_ptr<bh_op> ownop;  // coarse p-level operator owned by the hierarchy
     DevBuf dinv64, dinv32;
+    DevBuf dinv16, dscale;         // FP16-MFMA cycle, level 0: FP16 inverse blocks and their FP32 scales
     // transfer to the next coarser level: 0 = degree truncation, 1 = aggregation
     int kind = 0;
     DevBuf cptr, cidx, parent, pidx;       // aggregation: children CSR, parents, P block index per fine cell

@@ -333,6 +333,13 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         MgLevel& L = lv[l];
         bsr_invert_diag(L.A, L.dinv64, f32 ? &L.dinv32 : nullptr);
         if (f32) { L.dinv64.release(); if (!L.op) L.A->ensure_fp32(); }
+        // FP16-MFMA cycle: the fine level's smoother blocks in FP16 as well (half the bytes of
+        // the largest smoother sweep; the blocks are scaled per block)
+        if (f32 && mfma && l == 0 && (L.bs == 10 || L.bs == 15 || L.bs == 20 || L.bs == 21 || L.bs == 28 || L.bs == 35)) {
+            L.dinv16.alloc((size_t)L.J * L.bs * L.bs * sizeof(_Float16));
+            L.dscale.alloc((size_t)L.J * sizeof(float));
+            blocks_to_f16(ctx, L.J, L.bs, L.dinv32.as<float>(), L.dinv16.as<_Float16>(), L.dscale.as<float>());
+        }
     }
     for (auto& L : lv) {
         L.b.alloc(L.N * es);
@@ -438,6 +445,8 @@ void PMultigrid::smooth(int l, const T* r, T* x, bool add) {
     if (L.sw_cmax > 0)
         schwarz_apply<T>(ctx, L.sw_nagg, L.bs, L.sw_cmax, L.cptr.as<int>(), L.cidx.as<int>(),
                          (const T*)(f32 ? L.sw32.p : L.sw64.p), r, x, omega, add);
+    else if (f32 && L.dinv16.p)
+        block_gemv_h(ctx, L.J, L.bs, L.dinv16.as<_Float16>(), L.dscale.as<float>(), (const float*)r, (float*)x, omega, add);
     else
         block_gemv<T>(ctx, L.J, L.bs, (const T*)(f32 ? L.dinv32.p : L.dinv64.p), r, x, omega, add);
 }
