@@ -133,6 +133,20 @@ def test_fp32_cycle_close_to_fp64(ctx):
     assert rel(x32, x64) < 1e-4
 
 
+def test_fp16_mfma_cycle_close_to_fp64(ctx):
+    """FP16-MFMA cycle (FP16 level-0 apply and FP16 level-0 smoother blocks, each
+    block scaled by its largest entry) against the FP64 cycle: within the FP16
+    rounding of the preconditioner."""
+    _say("case fp16-mfma cycle")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(6).standard_normal(oo.N)
+    m64 = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=2, prec=B.BH_FP64, matrix_free=True)
+    m16 = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=2, prec=B.BH_FP16_MFMA, matrix_free=True)
+    x64 = m64.Solve(np.zeros(oo.N), b)
+    x16 = m16.Solve(np.zeros(oo.N), b)
+    assert rel(x16, x64) < 5e-3
+
+
 @pytest.mark.parametrize("kin", [0, 2])
 def test_graph_replay_matches_eager(ctx, kin):
     """The HIP-graph replay of the synchronisation-free V-cycles gives the same

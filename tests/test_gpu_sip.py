@@ -258,9 +258,12 @@ def test_soft_gmres_block_jacobi_matches_oracle(ctx):
     assert rel(xg, xo) < 1e-7
 
 
-@pytest.mark.parametrize("matrix_free", [False, True])
-def test_flex_gmres_true_residual(ctx, matrix_free):
-    _say(f"case flex gmres matrix_free={matrix_free}")
+@pytest.mark.parametrize("matrix_free,restart", [(False, 30), (True, 30), (True, 45)])
+def test_flex_gmres_true_residual(ctx, matrix_free, restart):
+    """FGMRES + block-Jacobi to 1e-10 true residual.  Restart 30 runs the fused
+    Gram-Schmidt passes for every step; restart 45 also the path for more than
+    31 basis vectors (separate passes)."""
+    _say(f"case flex gmres matrix_free={matrix_free} restart={restart}")
     nodes = nodes_3d(6, 6, 6)
     og = O.Grid(*nodes)
     nu = log_uniform_nu(og.J, 4)
@@ -271,10 +274,12 @@ def test_flex_gmres_true_residual(ctx, matrix_free):
     b = np.random.default_rng(3).standard_normal(oo.N)
     bj = B.BlockJacobi(Ag, omega=1.0, NoOfIterations=1)
     fg = B.FlexGMRES(M=None if matrix_free else Ag, op=go if matrix_free else None, Precond=bj,
-                     MaxKrylovDim=30, ConvergenceCriterion=1e-10)
+                     MaxKrylovDim=restart, ConvergenceCriterion=1e-10)
     x = fg.Solve(np.zeros(oo.N), b)
     st = fg.stats()
+    _say("  stats", st)
     assert st["converged"]
+    assert restart <= 31 or st["iterations"] > 32
     r = b - oo.evaluate(x)  # oracle operator: no affine part set
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b) * 1.0001
 
