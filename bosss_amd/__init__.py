@@ -87,6 +87,7 @@ SIGNATURES = {
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_schwarz": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_set_diag_equilib": (_c_i, [_c_p]),
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_learned_fallback": (_c_i, [_c_p, _c_i, _c_d]),
     "bh_pmg_learned_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_d)]),
@@ -612,6 +613,12 @@ def pmg_learned_info(pmg):
 def pmg_set_schwarz(pmg, enable=True):
     """Additive Schwarz on the multigrid aggregates (no overlap) as the smoother of the aggregated levels."""
     _check(lib().bh_pmg_set_schwarz(pmg.h, 1 if enable else 0))
+
+
+def pmg_set_diag_equilib(pmg):
+    """DiagBlockEquilib: every level but the coarsest in the basis where its diagonal blocks
+    are the identity (assembled hierarchies only; irreversible)."""
+    _check(lib().bh_pmg_set_diag_equilib(pmg.h))
 
 
 def pmg_set_graphs(pmg, enable):

@@ -783,6 +783,15 @@ int bh_pmg_set_schwarz(bh_solver* s, int enable) {
     BH_API_END
 }
 
+int bh_pmg_set_diag_equilib(bh_solver* s) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_set_diag_equilib: solver is not a multigrid preconditioner");
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    mg->set_equilib();
+    BH_API_END
+}
+
 int bh_pmg_set_graphs(bh_solver* s, int enable) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);

@@ -261,6 +261,172 @@ __global__ __launch_bounds__(256) void blocks_to_f16_k(int nblocks, int bb, cons
     }
 }
 
+// ---------------------------------------------------------------- block equilibration
+// Per diagonal block D_j (symmetric): D = C S C^T with C lower triangular, C_kk =
+// sqrt(|pivot_k|) and S = diag(sign(pivot_k)), and L = C^-1, so that L D L^T = S
+// (the reference's SymmInv, MultigridOperator_ChangeOfBasis.cs:624-700: rows and
+// columns scaled by 1/sqrt(|M_ii|), diagonal +-1).  One workgroup per block, the block
+// in LDS; C and L column-major, zero above the diagonal; S per row.  status: 1 + the
+// first block with a zero pivot.
+__global__ __launch_bounds__(64) void chol_inv_blocks_k(int nrows, int bs, const int64_t* __restrict__ diag_idx,
+                                                        const double* __restrict__ vals, double* __restrict__ Cout,
+                                                        double* __restrict__ Lout, double* __restrict__ Sout,
+                                                        int* __restrict__ status) {
+    extern __shared__ double S[];   // M: bs x bs row-major (becomes C), X: bs x bs row-major (becomes L), sg: bs signs
+    double* M = S;
+    double* X = S + bs * bs;
+    double* sg = S + 2 * bs * bs;
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= nrows) return;
+    const int64_t bi = diag_idx[r];
+    for (int t = lane; t < bs * bs; t += 64) {
+        const int i = t / bs, j = t - i * bs;
+        M[t] = bi >= 0 ? vals[(size_t)bi * bs * bs + (size_t)j * bs + i] : 0.0;
+        X[t] = 0.0;
+    }
+    __syncthreads();
+    bool ok = true;
+    for (int k = 0; k < bs; ++k) {
+        const double p = M[k * bs + k];
+        if (p == 0.0 || !(fabs(p) < INFINITY)) { ok = false; break; }
+        const double sk = p > 0.0 ? 1.0 : -1.0, ck = sqrt(fabs(p));
+        __syncthreads();
+        for (int i = k + 1 + lane; i < bs; i += 64) M[i * bs + k] /= sk * ck;
+        if (lane == 0) { M[k * bs + k] = ck; sg[k] = sk; }
+        __syncthreads();
+        for (int t = lane; t < bs * bs; t += 64) {
+            const int i = t / bs, j = t - i * bs;
+            if (j > k && i >= j) M[i * bs + j] -= M[i * bs + k] * sk * M[j * bs + k];
+        }
+        __syncthreads();
+    }
+    if (!ok) { if (lane == 0) atomicCAS(status, 0, r + 1); return; }
+    // L = C^-1 (lower triangular): column j by lane j, forward substitution
+    for (int j = lane; j < bs; j += 64) {
+        for (int i = j; i < bs; ++i) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = j; k < i; ++k) s -= M[i * bs + k] * X[k * bs + j];
+            X[i * bs + j] = s / M[i * bs + i];
+        }
+    }
+    __syncthreads();
+    for (int t = lane; t < bs * bs; t += 64) {
+        const int i = t / bs, j = t - i * bs;   // store column-major, upper parts zero
+        Cout[(size_t)r * bs * bs + (size_t)j * bs + i] = (j <= i) ? M[t] : 0.0;
+        Lout[(size_t)r * bs * bs + (size_t)j * bs + i] = (j <= i) ? X[t] : 0.0;
+    }
+    for (int i = lane; i < bs; i += 64) Sout[(size_t)r * bs + i] = sg[i];
+}
+
+// x (+)= omega S r, S a diagonal of signs (block-Jacobi on an equilibrated level)
+template <typename T>
+__global__ void sign_smooth_k(int64_t n, T omega, const T* __restrict__ S, const T* __restrict__ r, T* __restrict__ x, int add) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (add ? x[i] : T(0)) + omega * S[i] * r[i];
+}
+
+// BSR block p (row i, column j): A_p <- L_i A_p L_j^T (column-major blocks), in place
+__global__ __launch_bounds__(64) void block_sandwich_k(int64_t nnzb, int bs, const int* __restrict__ prow, const int* __restrict__ col,
+                                                       const double* __restrict__ Lb, double* __restrict__ vals) {
+    extern __shared__ double S[];   // Li, A, Lj, T (bs x bs each, column-major)
+    double* Li = S;
+    double* A = S + bs * bs;
+    double* Lj = S + 2 * bs * bs;
+    double* T = S + 3 * bs * bs;
+    const int lane = threadIdx.x;
+    for (int64_t p = blockIdx.x; p < nnzb; p += gridDim.x) {
+        const int i = prow[p], j = col[p];
+        __syncthreads();
+        for (int t = lane; t < bs * bs; t += 64) {
+            Li[t] = Lb[(size_t)i * bs * bs + t];
+            A[t] = vals[(size_t)p * bs * bs + t];
+            Lj[t] = Lb[(size_t)j * bs * bs + t];
+        }
+        __syncthreads();
+        // T = A Lj^T:  T(r, c) = sum_k A(r, k) Lj(c, k)
+        for (int t = lane; t < bs * bs; t += 64) {
+            const int c = t / bs, rr = t - c * bs;
+            double s = 0;
+            for (int k = 0; k <= c; ++k) s += A[k * bs + rr] * Lj[k * bs + c];
+            T[t] = s;
+        }
+        __syncthreads();
+        // A_p = Li T:  (r, c) = sum_k Li(r, k) T(k, c)
+        for (int t = lane; t < bs * bs; t += 64) {
+            const int c = t / bs, rr = t - c * bs;
+            double s = 0;
+            for (int k = 0; k <= rr; ++k) s += Li[k * bs + rr] * T[c * bs + k];
+            vals[(size_t)p * bs * bs + t] = s;
+        }
+    }
+}
+
+// transformed transfer blocks, row-major [fine mode][coarse mode] per fine cell i:
+// Pt_i = C_i^T P_i L_I^T with I = parent(i); P_i from the distinct blocks (pidx) or, when
+// P == nullptr, the degree truncation [I; 0]; L == nullptr: the coarse level is not
+// transformed (identity)
+__global__ __launch_bounds__(256) void transfer_blocks_k(int Jf, int bsf, int bsc, const double* __restrict__ C,
+                                                         const int* __restrict__ pidx, const double* __restrict__ P,
+                                                         const int* __restrict__ parent, const double* __restrict__ L,
+                                                         double* __restrict__ Pt, float* __restrict__ Pt32) {
+    const int64_t tot = (int64_t)Jf * bsf * bsc;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / (bsf * bsc));
+        const int e = (int)(t - (int64_t)i * bsf * bsc);
+        const int m = e / bsc, n = e - m * bsc;
+        const double* Ci = C + (size_t)i * bsf * bsf;                     // column-major: C(a, m) = Ci[m * bsf + a]
+        const double* Pi = P ? P + (size_t)pidx[i] * bsf * bsc : nullptr;  // row-major: P(a, b) = Pi[a * bsc + b]
+        const int I = parent ? parent[i] : i;
+        const double* LI = L ? L + (size_t)I * bsc * bsc : nullptr;        // column-major: L(n, b) = LI[b * bsc + n]
+        double s = 0;
+        for (int a = m; a < bsf; ++a) {                                    // C^T(m, a) = C(a, m), zero for a < m
+            const double cam = Ci[m * bsf + a];
+            double pl = 0;
+            for (int b = 0; b < bsc; ++b) {
+                const double pab = Pi ? Pi[a * bsc + b] : (a == b ? 1.0 : 0.0);
+                const double lnb = LI ? (b <= n ? LI[b * bsc + n] : 0.0) : (b == n ? 1.0 : 0.0);
+                pl += pab * lnb;
+            }
+            s += cam * pl;
+        }
+        Pt[t] = s;
+        if (Pt32) Pt32[t] = (float)s;
+    }
+}
+
+// transfers with one rectangular block per fine cell (row-major [fine mode][coarse mode]):
+// rc_I = sum_{i in I} Pt_i^T r_i,  xf_i += Pt_i xc_{parent(i)}
+template <typename T>
+__global__ void rect_restrict_k(int Jc, int bsf, int bsc, const int* __restrict__ cptr, const int* __restrict__ cidx,
+                                const T* __restrict__ Pt, const T* __restrict__ rf, T* __restrict__ rc) {
+    const int64_t tot = (int64_t)Jc * bsc;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(t / bsc), n = (int)(t - (int64_t)I * bsc);
+        T s = 0;
+        for (int p = cptr[I]; p < cptr[I + 1]; ++p) {
+            const int i = cidx[p];
+            const T* Pi = Pt + (size_t)i * bsf * bsc;
+            const T* ri = rf + (size_t)i * bsf;
+            for (int m = 0; m < bsf; ++m) s += Pi[m * bsc + n] * ri[m];
+        }
+        rc[t] = s;
+    }
+}
+
+template <typename T>
+__global__ void rect_prolong_add_k(int Jf, int bsf, int bsc, const int* __restrict__ parent, const T* __restrict__ Pt,
+                                   const T* __restrict__ xc, T* __restrict__ xf) {
+    const int64_t tot = (int64_t)Jf * bsf;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / bsf), m = (int)(t - (int64_t)i * bsf);
+        const T* Pi = Pt + (size_t)i * bsf * bsc + (size_t)m * bsc;
+        const T* xcI = xc + (size_t)parent[i] * bsc;
+        T s = 0;
+        for (int n = 0; n < bsc; ++n) s += Pi[n] * xcI[n];
+        xf[t] += s;
+    }
+}
+
 // x = Ainv b (dense, row-major n x n), one wave per row, accumulated in double.
 // VEC (n a multiple of 4): each lane reads 4 consecutive entries per step, two
 // steps in flight.
@@ -540,6 +706,69 @@ void block_gemv_h(bh_ctx* ctx, int nrows, int bs, const _Float16* Dinv, const fl
 #undef BH_CASE
     BH_HIP(hipGetLastError());
 }
+
+void chol_inv_blocks(bh_bsr* A, DevBuf& C, DevBuf& L, DevBuf& S) {
+    A->ensure_diag();
+    const int bs = A->bs;
+    BH_CHECK(bs <= 40, "block equilibration supports block sizes <= 40");
+    C.alloc((size_t)A->nrows * bs * bs * sizeof(double));
+    L.alloc((size_t)A->nrows * bs * bs * sizeof(double));
+    S.alloc((size_t)A->nrows * bs * sizeof(double));
+    DevBuf st(sizeof(int));
+    const int zero = 0;
+    A->ctx->upload(st.p, &zero, sizeof(int));
+    hipLaunchKernelGGL(chol_inv_blocks_k, dim3(A->nrows), dim3(64), ((size_t)2 * bs * bs + bs) * sizeof(double), A->ctx->stream,
+                       A->nrows, bs, A->diag_idx.as<int64_t>(), A->vals.as<double>(), C.as<double>(), L.as<double>(),
+                       S.as<double>(), st.as<int>());
+    BH_HIP(hipGetLastError());
+    int h = 0;
+    A->ctx->download(&h, st.p, sizeof(int));
+    BH_CHECK(h == 0, "block equilibration: diagonal block " + std::to_string(h - 1) + " has a zero pivot");
+}
+
+template <typename T>
+void sign_smooth(bh_ctx* ctx, int64_t n, double omega, const T* S, const T* r, T* x, bool add) {
+    hipLaunchKernelGGL((sign_smooth_k<T>), grid1d(n), dim3(256), 0, ctx->stream, n, (T)omega, S, r, x, add ? 1 : 0);
+}
+template void sign_smooth<float>(bh_ctx*, int64_t, double, const float*, const float*, float*, bool);
+template void sign_smooth<double>(bh_ctx*, int64_t, double, const double*, const double*, double*, bool);
+
+void block_sandwich(bh_bsr* A, const double* L) {
+    std::vector<int> prow((size_t)A->nnzb);
+    for (int r = 0; r < A->nrows; ++r)
+        for (int64_t p = A->h_rowptr[r]; p < A->h_rowptr[r + 1]; ++p) prow[p] = r;
+    DevBuf dp(std::max<size_t>(prow.size(), 1) * sizeof(int));
+    A->ctx->upload(dp.p, prow.data(), prow.size() * sizeof(int));
+    const int bs = A->bs;
+    const unsigned g = (unsigned)std::min<int64_t>(A->nnzb, 65536);
+    hipLaunchKernelGGL(block_sandwich_k, dim3(g > 0 ? g : 1), dim3(64), (size_t)4 * bs * bs * sizeof(double), A->ctx->stream,
+                       A->nnzb, bs, dp.as<int>(), A->col.as<int>(), L, A->vals.as<double>());
+    BH_HIP(hipGetLastError());
+    A->ctx->sync();
+    A->vals32.release();   // stale FP32 copy
+}
+
+void transfer_blocks(bh_ctx* ctx, int Jf, int bsf, int bsc, const double* C, const int* pidx, const double* P, const int* parent,
+                     const double* L, double* Pt, float* Pt32) {
+    hipLaunchKernelGGL(transfer_blocks_k, grid1d((int64_t)Jf * bsf * bsc), dim3(256), 0, ctx->stream, Jf, bsf, bsc, C, pidx, P,
+                       parent, L, Pt, Pt32);
+    BH_HIP(hipGetLastError());
+}
+
+template <typename T>
+void rect_restrict(bh_ctx* ctx, int Jc, int bsf, int bsc, const int* cptr, const int* cidx, const T* Pt, const T* rf, T* rc) {
+    hipLaunchKernelGGL((rect_restrict_k<T>), grid1d((int64_t)Jc * bsc), dim3(256), 0, ctx->stream, Jc, bsf, bsc, cptr, cidx, Pt,
+                       rf, rc);
+}
+template <typename T>
+void rect_prolong_add(bh_ctx* ctx, int Jf, int bsf, int bsc, const int* parent, const T* Pt, const T* xc, T* xf) {
+    hipLaunchKernelGGL((rect_prolong_add_k<T>), grid1d((int64_t)Jf * bsf), dim3(256), 0, ctx->stream, Jf, bsf, bsc, parent, Pt,
+                       xc, xf);
+}
+template void rect_restrict<float>(bh_ctx*, int, int, int, const int*, const int*, const float*, const float*, float*);
+template void rect_restrict<double>(bh_ctx*, int, int, int, const int*, const int*, const double*, const double*, double*);
+template void rect_prolong_add<float>(bh_ctx*, int, int, int, const int*, const float*, const float*, float*);
+template void rect_prolong_add<double>(bh_ctx*, int, int, int, const int*, const double*, const double*, double*);
 
 template <typename T, typename M>
 void dense_gemv(bh_ctx* ctx, int n, const M* A, const T* b, T* x) {

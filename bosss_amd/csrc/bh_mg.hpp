@@ -21,6 +21,18 @@ template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr
 template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc,
                                           T* xf);
 template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
+// block equilibration (DiagBlockEquilib): factors of the diagonal blocks D = C S C^T (S signs) and
+// L = C^-1 (L D L^T = S), the transform A_ij <- L_i A_ij L_j^T, transfer blocks Pt_i = C_i^T P_i L_I^T,
+// and the smoother x (+)= omega S r of an equilibrated level
+void chol_inv_blocks(bh_bsr* A, DevBuf& C, DevBuf& L, DevBuf& S);
+template <typename T> void sign_smooth(bh_ctx*, int64_t n, double omega, const T* S, const T* r, T* x, bool add);
+void block_sandwich(bh_bsr* A, const double* L);
+void transfer_blocks(bh_ctx*, int Jf, int bsf, int bsc, const double* C, const int* pidx, const double* P, const int* parent,
+                     const double* L, double* Pt, float* Pt32);
+template <typename T> void rect_restrict(bh_ctx*, int Jc, int bsf, int bsc, const int* cptr, const int* cidx, const T* Pt,
+                                         const T* rf, T* rc);
+template <typename T> void rect_prolong_add(bh_ctx*, int Jf, int bsf, int bsc, const int* parent, const T* Pt, const T* xc,
+                                            T* xf);
 // FP16 inverse blocks scaled per block (FP32 vectors): bs in {10, 15, 20, 21, 28, 35}
 void blocks_to_f16(bh_ctx*, int nblocks, int bs, const float* src, _Float16* dst, float* scale);
 void block_gemv_h(bh_ctx*, int nrows, int bs, const _Float16* Dinv, const float* scale, const float* r, float* x, double omega,
@@ -76,6 +88,10 @@ struct MgLevel {
     // additive Schwarz smoother over this level's aggregates (sw_cmax > 0: in use)
     int sw_cmax = 0, sw_nagg = 0;
     DevBuf sw64, sw32;
+    // block equilibration (level in the transformed basis x~ = L^-T x): the inverse
+    // factors L (and L^T, level 0 only, for the output) and the transfer blocks to the next level
+    DevBuf eqL64, eqL32, eqLt64, eqLt32, eqP64, eqP32, eqcptr, eqcidx, eqparent;
+    DevBuf eqS64, eqS32;           // signs of the equilibrated diagonal blocks (L D L^T = diag(eqS))
 };
 
 struct PMultigrid : bh_solver {
@@ -85,6 +101,8 @@ struct PMultigrid : bh_solver {
     double omega = 1.0;
     int pre = 1, post = 1;
     bool matrix_free = true;
+    bool equilib = false;          // DiagBlockEquilib: every level but the coarsest in its block-equilibrated basis
+    bh_bsr* eqA0 = nullptr;        // the transformed finest matrix (the caller's matrix stays untouched)
     int kcycle_level = -1;         // level solved by `kin` inner FGMRES steps (K-cycle), -1: none
     int kin = 0;
     std::vector<MgLevel> lv;
@@ -132,6 +150,7 @@ struct PMultigrid : bh_solver {
     template <typename T> void apply(int l, const T* x, T* y);   // y = A_l x
     template <typename T> void smooth(int l, const T* r, T* x, bool add);   // x (+)= omega M_l^-1 r
     void set_schwarz(bool on);     // aggregate-block Schwarz on every level that has aggregates
+    void set_equilib();            // DiagBlockEquilib (assembled hierarchies; irreversible)
     int op_prec(int l, bool f32) const { return f32 ? ((l == 0 && mfma) ? BH_FP16_MFMA : BH_FP32) : BH_FP64; }
     template <typename T> T* vec(DevBuf& d) { return (T*)d.p; }
 };

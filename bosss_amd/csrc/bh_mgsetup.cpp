@@ -27,6 +27,7 @@ PMultigrid::~PMultigrid() {
     drop_graphs();
     for (auto& L : lv)
         if (L.ownA) delete L.A;
+    delete eqA0;
     delete ownedAf;
 }
 
@@ -424,10 +425,13 @@ void PMultigrid::vcycle(int l, const T* b, T* x) {
     residual<T>(l, b, x, r);
     MgLevel& C = lv[l + 1];
     const T* P = (const T*)(f32 ? L.P32.p : L.P64.p);
-    if (L.kind == 0) p_restrict<T>(ctx, L.J, L.bs, C.bs, r, vec<T>(C.b));
+    const T* Pt = (const T*)(f32 ? L.eqP32.p : L.eqP64.p);
+    if (equilib) rect_restrict<T>(ctx, C.J, L.bs, C.bs, L.eqcptr.as<int>(), L.eqcidx.as<int>(), Pt, r, vec<T>(C.b));
+    else if (L.kind == 0) p_restrict<T>(ctx, L.J, L.bs, C.bs, r, vec<T>(C.b));
     else agg_restrict<T>(ctx, C.J, L.bs, L.cptr.as<int>(), L.cidx.as<int>(), L.pidx.as<int>(), P, r, vec<T>(C.b));
     solve_level<T>(l + 1);
-    if (L.kind == 0) p_prolong_add<T>(ctx, L.J, L.bs, C.bs, vec<T>(C.x), x);
+    if (equilib) rect_prolong_add<T>(ctx, L.J, L.bs, C.bs, L.eqparent.as<int>(), Pt, vec<T>(C.x), x);
+    else if (L.kind == 0) p_prolong_add<T>(ctx, L.J, L.bs, C.bs, vec<T>(C.x), x);
     else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), L.pidx.as<int>(), P, vec<T>(C.x), x);
     for (int s = 0; s < post; ++s) {
         residual<T>(l, b, x, r);
@@ -442,13 +446,113 @@ template <typename T>
 void PMultigrid::smooth(int l, const T* r, T* x, bool add) {
     MgLevel& L = lv[l];
     const bool f32 = sizeof(T) == 4;
-    if (L.sw_cmax > 0)
+    if (equilib && L.eqL64.p) {
+        // block-equilibrated level: the diagonal blocks are diag(S) (signs), block-Jacobi is x (+)= omega S r
+        sign_smooth<T>(ctx, L.N, omega, (const T*)(f32 ? L.eqS32.p : L.eqS64.p), r, x, add);
+    } else if (L.sw_cmax > 0)
         schwarz_apply<T>(ctx, L.sw_nagg, L.bs, L.sw_cmax, L.cptr.as<int>(), L.cidx.as<int>(),
                          (const T*)(f32 ? L.sw32.p : L.sw64.p), r, x, omega, add);
     else if (f32 && L.dinv16.p)
         block_gemv_h(ctx, L.J, L.bs, L.dinv16.as<_Float16>(), L.dscale.as<float>(), (const float*)r, (float*)x, omega, add);
     else
         block_gemv<T>(ctx, L.J, L.bs, (const T*)(f32 ? L.dinv32.p : L.dinv64.p), r, x, omega, add);
+}
+
+// DiagBlockEquilib (AS/MultigridOperator_ChangeOfBasis.cs:819-828, SymmInv :624): every level
+// but the coarsest runs in the basis in which its diagonal blocks are diagonal matrices of
+// signs (the identity for an SPD operator, minus it for the SIP form's sign).  Per level,
+// L_j = C_j^-1 with D_j = C_j S_j C_j^T; the level matrix becomes L A L^T (blocks
+// L_i A_ij L_j^T), vectors x~ = C^T x, right-hand sides b~ = L b, the transfers
+// Pt_i = C_i^T P_i L_I^T (and Pt^T), and block-Jacobi is x~ (+)= omega S r~.  The coarsest
+// level stays in the original basis (L = I).  For assembled hierarchies; one-way.
+void PMultigrid::set_equilib() {
+    if (equilib) return;
+    BH_CHECK(!matrix_free, "multigrid: DiagBlockEquilib needs the assembled hierarchy (matrix_free = 0)");
+    BH_CHECK(!learned, "multigrid: DiagBlockEquilib and the learned correction are exclusive");
+    for (auto& L : lv) BH_CHECK(L.sw_cmax == 0, "multigrid: DiagBlockEquilib and the Schwarz smoother are exclusive");
+    drop_graphs();
+    const int nl = (int)lv.size();
+    std::vector<DevBuf> C(nl);
+    for (int l = 0; l + 1 < nl; ++l) {
+        chol_inv_blocks(lv[l].A, C[l], lv[l].eqL64, lv[l].eqS64);
+        if (prec == BH_FP32) {
+            lv[l].eqS32.alloc((size_t)lv[l].N * sizeof(float));
+            convert_f64_f32(ctx, lv[l].eqS64.as<double>(), lv[l].eqS32.as<float>(), lv[l].N);
+        }
+    }
+    // transfer blocks from the original P and the factors
+    for (int l = 0; l + 1 < nl; ++l) {
+        MgLevel& F = lv[l];
+        MgLevel& Cc = lv[l + 1];
+        const size_t nb = (size_t)F.J * F.bs * Cc.bs;
+        F.eqP64.alloc(nb * sizeof(double));
+        F.eqP32.alloc(nb * sizeof(float));
+        const double* Lc = (l + 2 < nl) ? Cc.eqL64.as<double>() : nullptr;   // the coarsest level is not transformed
+        std::vector<int> cp, ci, par;
+        if (F.kind == 1) {
+            transfer_blocks(ctx, F.J, F.bs, Cc.bs, C[l].as<double>(), F.pidx.as<int>(), F.P64.as<double>(), F.parent.as<int>(), Lc,
+                            F.eqP64.as<double>(), F.eqP32.as<float>());
+        } else {
+            transfer_blocks(ctx, F.J, F.bs, Cc.bs, C[l].as<double>(), nullptr, nullptr, nullptr, Lc, F.eqP64.as<double>(),
+                            F.eqP32.as<float>());
+        }
+        // children lists / parents for the rectangular transfers (degree truncation: identity)
+        if (F.kind == 1) {
+            F.eqcptr.alloc(((size_t)Cc.J + 1) * sizeof(int));
+            F.eqcidx.alloc((size_t)F.J * sizeof(int));
+            F.eqparent.alloc((size_t)F.J * sizeof(int));
+            copy_bytes(ctx, F.eqcptr.p, F.cptr.p, ((size_t)Cc.J + 1) * sizeof(int));
+            copy_bytes(ctx, F.eqcidx.p, F.cidx.p, (size_t)F.J * sizeof(int));
+            copy_bytes(ctx, F.eqparent.p, F.parent.p, (size_t)F.J * sizeof(int));
+        } else {
+            cp.resize(F.J + 1);
+            ci.resize(F.J);
+            for (int i = 0; i <= F.J; ++i) cp[i] = i;
+            for (int i = 0; i < F.J; ++i) ci[i] = i;
+            F.eqcptr.alloc(cp.size() * sizeof(int));
+            F.eqcidx.alloc(ci.size() * sizeof(int));
+            F.eqparent.alloc(ci.size() * sizeof(int));
+            ctx->upload(F.eqcptr.p, cp.data(), cp.size() * sizeof(int));
+            ctx->upload(F.eqcidx.p, ci.data(), ci.size() * sizeof(int));
+            ctx->upload(F.eqparent.p, ci.data(), ci.size() * sizeof(int));
+        }
+    }
+    // level matrices in the transformed basis (the finest as a copy: it belongs to the caller)
+    for (int l = 0; l + 1 < nl; ++l) {
+        MgLevel& L = lv[l];
+        if (l == 0) {
+            bh_bsr* A0 = make_bsr(ctx, L.A->nrows, L.A->bs, L.A->h_rowptr, L.A->h_col);
+            copy_bytes(ctx, A0->vals.p, L.A->vals.p, (size_t)L.A->nnzb * L.A->bs * L.A->bs * sizeof(double));
+            eqA0 = A0;
+            L.A = A0;
+        }
+        block_sandwich(L.A, L.eqL64.as<double>());
+        if (prec == BH_FP32) L.A->ensure_fp32();
+        const size_t nb = (size_t)L.J * L.bs * L.bs;
+        if (prec == BH_FP32) {
+            L.eqL32.alloc(nb * sizeof(float));
+            convert_f64_f32(ctx, L.eqL64.as<double>(), L.eqL32.as<float>(), (int64_t)nb);
+        }
+    }
+    // level 0: L^T for the output transform
+    {
+        MgLevel& L = lv[0];
+        const int bs = L.bs;
+        const size_t nb = (size_t)L.J * bs * bs;
+        std::vector<double> h(nb), t(nb);
+        ctx->download(h.data(), L.eqL64.p, nb * sizeof(double));
+        for (int j = 0; j < L.J; ++j)
+            for (int c = 0; c < bs; ++c)
+                for (int r = 0; r < bs; ++r) t[(size_t)j * bs * bs + (size_t)c * bs + r] = h[(size_t)j * bs * bs + (size_t)r * bs + c];
+        L.eqLt64.alloc(nb * sizeof(double));
+        ctx->upload(L.eqLt64.p, t.data(), nb * sizeof(double));
+        if (prec == BH_FP32) {
+            L.eqLt32.alloc(nb * sizeof(float));
+            convert_f64_f32(ctx, L.eqLt64.as<double>(), L.eqLt32.as<float>(), (int64_t)nb);
+        }
+    }
+    ctx->sync();
+    equilib = true;
 }
 
 void PMultigrid::set_schwarz(bool on) {
@@ -518,6 +622,7 @@ void PMultigrid::kcycle(int l) {
 }
 
 void PMultigrid::attach_learned(Mlp* m) {
+    BH_CHECK(!m || !equilib, "learned correction and DiagBlockEquilib are exclusive");
     learned = m;
     drop_graphs();   // the captured cycle includes (or not) the learned step
     if (!m) return;
@@ -565,13 +670,23 @@ void PMultigrid::learned_correction() {
 void PMultigrid::precondition(double* x, const double* b) {
     MgLevel& L = lv[0];
     if (prec == BH_FP32) {
-        convert_f64_f32(ctx, b, vec<float>(L.b), N);
-        vcycle_g<float>(0);
-        convert_f32_f64(ctx, vec<float>(L.x), x, N);
+        if (equilib) {   // b~ = L b and x = L^T x~ through the level's residual buffer
+            convert_f64_f32(ctx, b, vec<float>(L.r), N);
+            block_gemv<float>(ctx, L.J, L.bs, L.eqL32.as<float>(), vec<float>(L.r), vec<float>(L.b), 1.0, false);
+            vcycle_g<float>(0);
+            block_gemv<float>(ctx, L.J, L.bs, L.eqLt32.as<float>(), vec<float>(L.x), vec<float>(L.r), 1.0, false);
+            convert_f32_f64(ctx, vec<float>(L.r), x, N);
+        } else {
+            convert_f64_f32(ctx, b, vec<float>(L.b), N);
+            vcycle_g<float>(0);
+            convert_f32_f64(ctx, vec<float>(L.x), x, N);
+        }
     } else {
-        copy(ctx, vec<double>(L.b), b, N);
+        if (equilib) block_gemv<double>(ctx, L.J, L.bs, L.eqL64.as<double>(), b, vec<double>(L.b), 1.0, false);
+        else copy(ctx, vec<double>(L.b), b, N);
         vcycle_g<double>(0);
-        copy(ctx, x, vec<double>(L.x), N);
+        if (equilib) block_gemv<double>(ctx, L.J, L.bs, L.eqLt64.as<double>(), vec<double>(L.x), x, 1.0, false);
+        else copy(ctx, x, vec<double>(L.x), N);
     }
     iterations++;
     if (learned && !learned_off && learned_probe > 0 && ++learned_calls == learned_probe) {

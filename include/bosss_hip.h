@@ -223,6 +223,15 @@ int bh_pmg_set_learned_fallback(bh_solver* pmg, int probe, double min_gain);
  * block-Jacobi.  Applies to bh_pmg_create and to the coarse hierarchy of
  * bh_xdg_precond_create. */
 int bh_pmg_set_schwarz(bh_solver* pmg, int enable);
+/* DiagBlockEquilib (AS/MultigridOperator_ChangeOfBasis.cs:819-828; the factor of
+ * SymmInv, :624): every multigrid level but the coarsest is taken into the basis
+ * in which its diagonal blocks are diagonals of signs (D_j = C_j S_j C_j^T, level
+ * matrix L A L^T with L = C^-1, transfers C^T P L^T), so block-Jacobi there is
+ * x~ (+)= omega S r~ and the K-cycle's inner products are those of that basis.
+ * The preconditioner's input and output stay in the caller's basis.  For
+ * hierarchies created with matrix_free = 0; irreversible; exclusive with the
+ * Schwarz smoother and the learned correction. */
+int bh_pmg_set_diag_equilib(bh_solver* pmg);
 int bh_pmg_learned_info(bh_solver* pmg, int* active, double* mean_gain);
 
 /* ---------------------------------------------------------------- compressible Euler

@@ -122,6 +122,35 @@ def test_kcycle_matches_numpy_restatement(ctx):
     assert rel(out, x) < 1e-9
 
 
+@pytest.mark.parametrize("degrees", [[3, 1], [3, 2, 1, 0]])
+def test_vcycle_diag_equilib_matches_oracle(ctx, degrees):
+    """DiagBlockEquilib (every level but the coarsest in the basis where its diagonal
+    blocks are diagonals of signs; minus the identity for the SIP form): with
+    block-Jacobi smoothing the V-cycle is the same operator on the caller's vectors
+    (a similarity per level: L A L^T, C^T P L^T, x~ += omega S r~), so it equals the
+    oracle's untransformed cycle."""
+    _say(f"case vcycle diag-equilib degrees={degrees}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(7).standard_normal(oo.N)
+    ref, nl = Ao.pmg_vcycle(og, b, degrees, min_coarse=8, omega=0.7, pre=2, post=1)
+    mg = B.PMultigrid(go, degrees=degrees, min_coarse=8, omega=0.7, pre=2, post=1, kcycle_inner=0, matrix_free=False)
+    B.pmg_set_diag_equilib(mg)
+    out = mg.Solve(np.zeros(oo.N), b)
+    assert nl >= 3
+    assert rel(out, ref) < 1e-10
+    m32 = B.PMultigrid(go, degrees=degrees, min_coarse=8, omega=0.7, pre=2, post=1, kcycle_inner=0, matrix_free=False,
+                       prec=B.BH_FP32)
+    B.pmg_set_diag_equilib(m32)
+    assert rel(m32.Solve(np.zeros(oo.N), b), ref) < 1e-4
+
+
+def test_diag_equilib_rejects_matrix_free(ctx):
+    og, gg, oo, go, Ao = _problem(ctx, n=(3, 3, 3))
+    mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=0, matrix_free=True)
+    with pytest.raises(B.BoSSSHipError):
+        B.pmg_set_diag_equilib(mg)
+
+
 def test_fp32_cycle_close_to_fp64(ctx):
     _say("case fp32 cycle")
     og, gg, oo, go, Ao = _problem(ctx)
@@ -167,10 +196,12 @@ def test_graph_replay_matches_eager(ctx, kin):
         assert np.array_equal(xg, xe), f"replay {rep}"
 
 
-@pytest.mark.parametrize("prec", [B.BH_FP64, B.BH_FP32, B.BH_FP16_MFMA])
-def test_fgmres_multigrid_solve(ctx, prec):
-    """FGMRES(30) + p/h multigrid (K-cycle) to 1e-8 true FP64 residual, 16^3 k=3, log-uniform nu."""
-    _say(f"case fgmres+mg prec={prec}")
+@pytest.mark.parametrize("prec,equilib", [(B.BH_FP64, False), (B.BH_FP32, False), (B.BH_FP16_MFMA, False),
+                                          (B.BH_FP64, True), (B.BH_FP32, True)])
+def test_fgmres_multigrid_solve(ctx, prec, equilib):
+    """FGMRES(30) + p/h multigrid (K-cycle) to 1e-8 true FP64 residual, 16^3 k=3, log-uniform nu
+    (also with the block-equilibrated assembled hierarchy)."""
+    _say(f"case fgmres+mg prec={prec} equilib={equilib}")
     n = 16
     x = np.linspace(0, 1, n + 1)
     og = O.Grid(x, x, x)
@@ -180,6 +211,9 @@ def test_fgmres_multigrid_solve(ctx, prec):
     oo = O.Sip(og, 3, nu)
     b = np.random.default_rng(5).standard_normal(go.N)
     mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=64, kcycle_inner=2, prec=prec)
+    if equilib:
+        mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=64, kcycle_inner=2, prec=prec, matrix_free=False)
+        B.pmg_set_diag_equilib(mg)
     fg = B.FlexGMRES(op=go, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=300)
     xs = fg.Solve(np.zeros(go.N), b)
     st = fg.stats()
