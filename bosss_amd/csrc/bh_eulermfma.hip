@@ -102,44 +102,48 @@ struct EulerMfmaArgs {
     R* __restrict__ out;
 };
 
-// HLLC as OptimizedHLLC*Flux.InnerEdgeFlux (normal = sgn * e_dn); same algebra as hllc() of the per-node kernel
+// HLLC as OptimizedHLLC*Flux.InnerEdgeFlux (normal = sgn * e_dn); same algebra as hllc() of the per-node kernel,
+// with each denominator inverted once (7 divisions per node instead of 20; rounding-level differences)
 template <int D, typename R>
 __device__ __forceinline__ void hllc_m(R gamma, R MS, const R* Ui, const R* Uo, int dn, R sgn, R* F) {
     const R rIn = Ui[0], rOut = Uo[0];
+    const R irIn = R(1) / rIn, irOut = R(1) / rOut;
+    const R iMS = R(1) / MS, igam = R(1) / gamma, iMach = sqrt(gamma * iMS);   // uniform: hoisted by the compiler
     R m2In = 0, m2Out = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) { m2In += Ui[d + 1] * Ui[d + 1]; m2Out += Uo[d + 1] * Uo[d + 1]; }
     const R uIn = dn == 0 ? Ui[1] : (dn == 1 ? Ui[2] : Ui[D == 3 ? 3 : 2]);
     const R uOut = dn == 0 ? Uo[1] : (dn == 1 ? Uo[2] : Uo[D == 3 ? 3 : 2]);
-    const R vnIn = sgn * uIn / rIn, vnOut = sgn * uOut / rOut;
+    const R vnIn = sgn * uIn * irIn, vnOut = sgn * uOut * irOut;
     const R EIn = Ui[D + 1], EOut = Uo[D + 1];
-    const R pIn = (gamma - R(1)) * (EIn - MS * R(0.5) * m2In / rIn);
-    const R pOut = (gamma - R(1)) * (EOut - MS * R(0.5) * m2Out / rOut);
-    const R Mach = sqrt(MS / gamma);
-    const R aIn = sqrt(pIn / rIn) / Mach, aOut = sqrt(pOut / rOut) / Mach;
+    const R pIn = (gamma - R(1)) * (EIn - MS * R(0.5) * m2In * irIn);
+    const R pOut = (gamma - R(1)) * (EOut - MS * R(0.5) * m2Out * irOut);
+    const R aIn = sqrt(pIn * irIn) * iMach, aOut = sqrt(pOut * irOut) * iMach;
     const R rMean = R(0.5) * (rIn + rOut), pMean = R(0.5) * (pIn + pOut), aMean = R(0.5) * (aIn + aOut);
     const R pStar = fmax(R(0), pMean - R(0.5) * MS * (vnOut - vnIn) * aMean * rMean);
-    const R qIn = pStar > pIn ? sqrt(R(1) + R(0.5) * (gamma + R(1)) * (pStar / pIn - R(1)) / gamma) : R(1);
-    const R qOut = pStar > pOut ? sqrt(R(1) + R(0.5) * (gamma + R(1)) * (pStar / pOut - R(1)) / gamma) : R(1);
+    const R hg = R(0.5) * (gamma + R(1)) * igam;
+    const R qIn = pStar > pIn ? sqrt(R(1) + hg * (pStar / pIn - R(1))) : R(1);
+    const R qOut = pStar > pOut ? sqrt(R(1) + hg * (pStar / pOut - R(1))) : R(1);
     const R sIn = vnIn - aIn * qIn, sOut = vnOut + aOut * qOut;
     const R cIn = rIn * (sIn - vnIn), cOut = rOut * (sOut - vnOut);
-    const R sStar = (cOut * vnOut - cIn * vnIn + (pIn - pOut) / MS) / (cOut - cIn);
+    const R sStar = (cOut * vnOut - cIn * vnIn + (pIn - pOut) * iMS) / (cOut - cIn);
     const bool left = sStar > R(0);
-    const R rS = left ? rIn : rOut, vnS = left ? vnIn : vnOut, pS = left ? pIn : pOut, ES = left ? EIn : EOut;
+    const R rS = left ? rIn : rOut, irS = left ? irIn : irOut, vnS = left ? vnIn : vnOut, pS = left ? pIn : pOut;
+    const R ES = left ? EIn : EOut;
     const R sK = left ? sIn : sOut;
     R Us[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) Us[c] = left ? Ui[c + 1] : Uo[c + 1];
     F[0] = vnS * rS;
 #pragma unroll
-    for (int c = 0; c < D; ++c) F[c + 1] = vnS * Us[c] + (c == dn ? sgn * pS / MS : R(0));
+    for (int c = 0; c < D; ++c) F[c + 1] = vnS * Us[c] + (c == dn ? sgn * pS * iMS : R(0));
     F[D + 1] = vnS * (ES + pS);
     if (left ? (sIn <= R(0)) : (sOut >= R(0))) {
         const R f = (sK - vnS) / (sK - sStar);
         F[0] += sK * (rS * f - rS);
 #pragma unroll
         for (int c = 0; c < D; ++c)
-            F[c + 1] += sK * (rS * f * (Us[c] / rS + (c == dn ? sgn * (sStar - vnS) : R(0))) - Us[c]);
+            F[c + 1] += sK * (rS * f * (Us[c] * irS + (c == dn ? sgn * (sStar - vnS) : R(0))) - Us[c]);
         const R factor = rS * sStar * MS + pS / (sK - vnS);
         F[D + 1] += sK * (f * (ES + factor * (sStar - vnS)) - ES);
     }
@@ -167,7 +171,12 @@ __global__ __launch_bounds__(64 * W, 2) void euler_rhs_mfma_kernel(EulerMfmaArgs
         if (D == 2) { idx[0] = cc / a.n1; idx[1] = cc - idx[0] * a.n1; idx[2] = 0; }
         else { const int sl = a.n0 * a.n1; idx[2] = cc / sl; const int rr = cc - idx[2] * sl; idx[1] = rr / a.n0; idx[0] = rr - idx[1] * a.n0; }
         R h[3] = {1, 1, 1}, detJ = 1;
-        for (int d = 0; d < D; ++d) { h[d] = (R)a.width[d * a.nmax + idx[d]]; detJ *= R(0.5) * h[d]; }
+        R ih2[3] = {2, 2, 2};    // 2 / h_d from the grid's reciprocal widths
+        for (int d = 0; d < D; ++d) {
+            h[d] = (R)a.width[d * a.nmax + idx[d]];
+            ih2[d] = R(2) * (R)a.width[(3 + d) * a.nmax + idx[d]];
+            detJ *= R(0.5) * h[d];
+        }
         const R sc = R(1) / sqrt(detJ);
         // own modes in the B-operand layout: variable v, mode 4s + grp
         R bU[NV][NK];
@@ -200,20 +209,22 @@ __global__ __launch_bounds__(64 * W, 2) void euler_rhs_mfma_kernel(EulerMfmaArgs
                 for (int v = 0; v < NV; ++v) Uq[v] = Ut[v][r] * sc;
                 const bool live = valid && q < L::NQV;
                 const R rho = live ? Uq[0] : R(1), E = Uq[D + 1];
+                const R irho = R(1) / rho;
                 R m2 = 0;
 #pragma unroll
                 for (int d = 0; d < D; ++d) m2 += Uq[d + 1] * Uq[d + 1];
-                const R p = (a.gamma - R(1)) * (E - a.MS * R(0.5) * m2 / rho);
+                const R p = (a.gamma - R(1)) * (E - a.MS * R(0.5) * m2 * irho);
+                const R pMS = p / a.MS;
                 const R w = live ? tab[L::VW + q] * detJ * sc : R(0);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const R wd = w * R(2) / h[d];
-                    const R ud = Uq[d + 1] / rho;
+                    const R wd = w * ih2[d];
+                    const R ud = Uq[d + 1] * irho;
                     const R Bg = tab[L::GT + (d * L::NVT * 16 + q) * 16 + col];
                     y[0] = M::mma(wd * Uq[d + 1], Bg, y[0]);
 #pragma unroll
                     for (int cc2 = 0; cc2 < D; ++cc2)
-                        y[cc2 + 1] = M::mma(wd * (Uq[cc2 + 1] * ud + (cc2 == d ? p / a.MS : R(0))), Bg, y[cc2 + 1]);
+                        y[cc2 + 1] = M::mma(wd * (Uq[cc2 + 1] * ud + (cc2 == d ? pMS : R(0))), Bg, y[cc2 + 1]);
                     y[D + 1] = M::mma(wd * ud * (E + p), Bg, y[D + 1]);
                 }
             }
