@@ -700,7 +700,7 @@ void block_gemv_h(bh_ctx* ctx, int nrows, int bs, const _Float16* Dinv, const fl
         else hipLaunchKernelGGL((block_gemv_h_k<B, false>), grd, blk, 0, ctx->stream, nrows, Dinv, scale, r, x, (float)omega); \
         break;
     switch (bs) {
-        BH_CASE(10) BH_CASE(15) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(35)
+        BH_CASE(3) BH_CASE(4) BH_CASE(6) BH_CASE(10) BH_CASE(15) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(35)
         default: throw Error(-3, "block_gemv (FP16 blocks): unsupported block size");
     }
 #undef BH_CASE

@@ -334,9 +334,11 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         MgLevel& L = lv[l];
         bsr_invert_diag(L.A, L.dinv64, f32 ? &L.dinv32 : nullptr);
         if (f32) { L.dinv64.release(); if (!L.op) L.A->ensure_fp32(); }
-        // FP16-MFMA cycle: the fine level's smoother blocks in FP16 as well (half the bytes of
-        // the largest smoother sweep; the blocks are scaled per block)
-        if (f32 && mfma && l == 0 && (L.bs == 10 || L.bs == 15 || L.bs == 20 || L.bs == 21 || L.bs == 28 || L.bs == 35)) {
+        // FP16-MFMA cycle: the fine level's smoother blocks in FP16 as well (half the bytes of the
+        // largest smoother sweep; the blocks are scaled per block).  Small blocks (the k = 1 level)
+        // stay FP32: there the FP16 sweep measured slower.
+        const bool hbs = L.bs == 10 || L.bs == 15 || L.bs == 20 || L.bs == 21 || L.bs == 28 || L.bs == 35;
+        if (f32 && mfma && l == 0 && hbs) {
             L.dinv16.alloc((size_t)L.J * L.bs * L.bs * sizeof(_Float16));
             L.dscale.alloc((size_t)L.J * sizeof(float));
             blocks_to_f16(ctx, L.J, L.bs, L.dinv32.as<float>(), L.dinv16.as<_Float16>(), L.dscale.as<float>());
