@@ -258,6 +258,12 @@ template <typename T> void multi_axpy_t(bh_ctx* ctx, T* w, const T* V, int64_t l
 // device-resident coefficients (graph-capturable): dout[k] = <w, V_k> (w null: |V_k|^2);
 // w += sign * (dscale ? dscale[0] : 1) * sum_k dcoef[k] V_k
 template <typename T> void multi_dot_to_dev(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, double* dout);
+// K-cycle Gram-Schmidt passes with coefficients in device memory (m <= 16):
+//   kc_project_dots:  w <- w - sum c_k V_k;  res[k] = <V_k, w> (k < m), res[m] = |w|^2
+//   kc_project_scale: out <- (w - sum c_k V_k) / sqrt(n2 - sum c_k^2), *nnout = n2 - sum c_k^2
+template <typename T> void kc_project_dots(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* c, double* res);
+template <typename T> void kc_project_scale(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, const double* c,
+                                            const double* n2, double* nnout, T* out);
 template <typename T> void multi_axpy_from_dev(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* dcoef,
                                                const double* dscale, double sign);
 // w -= sum_k h[k] V_k  (h host array, m <= 128)

@@ -721,6 +721,123 @@ void axpy_scale_dots(bh_ctx* ctx, double* w, const double* V, int64_t ldv, int m
     *wn2 = ctx->host_red[m];
 }
 
+// ------------------------------------------------ K-cycle Gram-Schmidt passes (device coefficients)
+// Classical Gram-Schmidt twice with the second projection's dots, the norm and the
+// normalisation fused (M <= 16 vectors, coefficients read from device memory, so the
+// passes can be captured into a graph):
+//   kc_axpy_dot:    w <- w - sum_k c_k V_k, partials of <V_k, w> (k < M) and |w|^2
+//   kc_axpy_scale:  out <- (w - sum_k c_k V_k) / sqrt(nn), nn = n2 - sum_k c_k^2 (Pythagoras),
+//                   nn also stored (workgroup 0) for the Hessenberg column
+template <typename T, int M>
+__global__ __launch_bounds__(256) void kc_axpy_dot_k(T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int64_t n,
+                                                     const double* __restrict__ c, double* __restrict__ part) {
+    __shared__ double cs[M];
+    __shared__ double sm[4][M + 1];
+    if ((int)threadIdx.x < M) cs[threadIdx.x] = c[threadIdx.x];
+    __syncthreads();
+    double acc[M + 1];
+#pragma unroll
+    for (int k = 0; k <= M; ++k) acc[k] = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) v[k] = (double)V[(size_t)k * ldv + i];
+        double s = (double)w[i];
+#pragma unroll
+        for (int k = 0; k < M; ++k) s -= cs[k] * v[k];
+        const T st = (T)s;
+        w[i] = st;
+        const double ws = (double)st;
+#pragma unroll
+        for (int k = 0; k < M; ++k) acc[k] += v[k] * ws;
+        acc[M] += ws * ws;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k <= M; ++k) {
+        double x = acc[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) sm[wave][k] = x;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x <= M) {
+        const int k = threadIdx.x;
+        part[(size_t)blockIdx.x * (M + 1) + k] = sm[0][k] + sm[1][k] + sm[2][k] + sm[3][k];
+    }
+}
+
+template <typename T, int M>
+__global__ __launch_bounds__(256) void kc_axpy_scale_k(const T* __restrict__ w, const T* __restrict__ V, int64_t ldv, int64_t n,
+                                                       const double* __restrict__ c, const double* __restrict__ n2,
+                                                       double* __restrict__ nnout, T* __restrict__ out) {
+    __shared__ double cs[M];
+    __shared__ double sc;
+    if ((int)threadIdx.x < M) cs[threadIdx.x] = c[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double p = 0;
+        for (int k = 0; k < M; ++k) p += cs[k] * cs[k];
+        const double nn = n2[0] - p;
+        if (blockIdx.x == 0) nnout[0] = nn;
+        const double nv = sqrt(fmax(nn, 0.0));
+        sc = (nv > 1e-300 && nv < 1e300) ? 1.0 / nv : 0.0;
+    }
+    __syncthreads();
+    const double f = sc;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = (double)w[i];
+#pragma unroll
+        for (int k = 0; k < M; ++k) s -= cs[k] * (double)V[(size_t)k * ldv + i];
+        out[i] = (T)(s * f);
+    }
+}
+
+template <typename T, int M>
+static void kc_launch(bh_ctx* ctx, bool dots, T* w, const T* V, int64_t ldv, int64_t n, const double* c, double* res,
+                      const double* n2, double* nnout, T* out) {
+    int64_t nb = (n + 1023) / 1024;
+    if (nb > MD_BLOCKS) nb = MD_BLOCKS;
+    if (nb < 1) nb = 1;
+    if (dots) {
+        hipLaunchKernelGGL((kc_axpy_dot_k<T, M>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, w, V, ldv, n, c,
+                           ctx->red.as<double>());
+        hipLaunchKernelGGL(reduce_parts_k, dim3(M + 1), dim3(256), 0, ctx->stream, ctx->red.as<double>(), (int)nb, M + 1, res);
+    } else {
+        hipLaunchKernelGGL((kc_axpy_scale_k<T, M>), dim3((unsigned)nb), dim3(256), 0, ctx->stream, w, V, ldv, n, c, n2, nnout,
+                           out);
+    }
+}
+
+template <typename T>
+static void kc_dispatch(bh_ctx* ctx, bool dots, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* c, double* res,
+                        const double* n2, double* nnout, T* out) {
+    ensure_red(ctx);
+    switch (m) {
+#define BH_KC(MM) case MM: kc_launch<T, MM>(ctx, dots, w, V, ldv, n, c, res, n2, nnout, out); break;
+        BH_KC(1) BH_KC(2) BH_KC(3) BH_KC(4) BH_KC(5) BH_KC(6) BH_KC(7) BH_KC(8)
+        BH_KC(9) BH_KC(10) BH_KC(11) BH_KC(12) BH_KC(13) BH_KC(14) BH_KC(15) BH_KC(16)
+#undef BH_KC
+        default: throw Error(-3, "K-cycle Gram-Schmidt: 1..16 vectors");
+    }
+    BH_HIP(hipGetLastError());
+}
+
+template <typename T>
+void kc_project_dots(bh_ctx* ctx, T* w, const T* V, int64_t ldv, int m, int64_t n, const double* c, double* res) {
+    kc_dispatch<T>(ctx, true, w, V, ldv, m, n, c, res, nullptr, nullptr, nullptr);
+}
+template <typename T>
+void kc_project_scale(bh_ctx* ctx, const T* w, const T* V, int64_t ldv, int m, int64_t n, const double* c, const double* n2,
+                      double* nnout, T* out) {
+    kc_dispatch<T>(ctx, false, const_cast<T*>(w), V, ldv, m, n, c, nullptr, n2, nnout, out);
+}
+template void kc_project_dots<float>(bh_ctx*, float*, const float*, int64_t, int, int64_t, const double*, double*);
+template void kc_project_dots<double>(bh_ctx*, double*, const double*, int64_t, int, int64_t, const double*, double*);
+template void kc_project_scale<float>(bh_ctx*, const float*, const float*, int64_t, int, int64_t, const double*, const double*,
+                                      double*, float*);
+template void kc_project_scale<double>(bh_ctx*, const double*, const double*, int64_t, int, int64_t, const double*, const double*,
+                                       double*, double*);
+
 // ------------------------------------------------ device-resident variants
 // The coefficients stay in device memory: no host round trip, so a sequence
 // of them can be captured into a HIP graph.

@@ -609,13 +609,12 @@ void PMultigrid::kcycle(int l) {
     for (int j = 0; j < kin; ++j) {
         vcycle<T>(l, V + (size_t)j * n, Z + (size_t)j * n);   // Z_j = (V-cycle from this level) V_j
         apply<T>(l, Z + (size_t)j * n, w);
+        // classical Gram-Schmidt twice, passes fused: H1 = V^T w; w <- w - V H1 with H2 = V^T w and
+        // |w|^2 (in slot H2[j+1]) from the same pass; V_{j+1} = (w - V H2) / |.| (norm by Pythagoras)
         multi_dot_to_dev<T>(ctx, w, V, n, j + 1, n, S + K.H1);
-        multi_axpy_from_dev<T>(ctx, w, V, n, j + 1, n, S + K.H1, nullptr, -1.0);
-        multi_dot_to_dev<T>(ctx, w, V, n, j + 1, n, S + K.H2);
-        multi_axpy_from_dev<T>(ctx, w, V, n, j + 1, n, S + K.H2, nullptr, -1.0);
-        multi_dot_to_dev<T>(ctx, nullptr, w, 0, 1, n, S + K.W2);
+        kc_project_dots<T>(ctx, w, V, n, j + 1, n, S + K.H1, S + K.H2);
+        kc_project_scale<T>(ctx, w, V, n, j + 1, n, S + K.H2, S + K.H2 + (j + 1), S + K.W2, V + (size_t)(j + 1) * n);
         kcycle_column(ctx, S, K, j);
-        if (j + 1 < kin) scale_inv_norm<T>(ctx, w, V + (size_t)(j + 1) * n, n, S + K.W2, 1e-300);
     }
     kcycle_lsq(ctx, S, K);
     if (sizeof(T) == 4) fill32(ctx, (float*)vec<T>(L.x), 0.f, n);
