@@ -1,6 +1,7 @@
 // XDG (extended DG) two-phase Poisson with a spherical level set: cut-cell
 // quadrature, agglomeration, element lists and the variable-block matrix.
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "bh_internal.hpp"
@@ -26,6 +27,9 @@ struct Sphere {
 // Cut-cell rules for a sphere and an axis-aligned box (bh_xdgsetup.cpp).
 // species: -1 = inside (phi < 0, "A"), +1 = outside ("B").
 void sphere_volume_rule(int D, const double* lo, const double* hi, const Sphere& S, int species, int q, QRule& out);
+// f(i) for i in [0, n) on up to 16 host threads (f must write disjoint slots
+// and allocate little: page faults of freshly allocated memory serialise)
+void parallel_for(int n, const std::function<void(int)>& f);
 void sphere_interface_rule(int D, const double* lo, const double* hi, const Sphere& S, int q, QRule& out);
 // face in the plane x_dir = lo[dir] (lo[dir] == hi[dir])
 void sphere_face_rule(int D, const double* lo, const double* hi, int dir, const Sphere& S, int species, int q, QRule& out);

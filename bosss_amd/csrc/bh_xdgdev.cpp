@@ -258,6 +258,22 @@ void xdg_build(bh_xdg* X) {
         std::vector<double> uval;
         std::vector<std::vector<int>> dcols(J);
         std::vector<double> dval;
+        {
+            // capacity bounds first: the value arrays reach hundreds of MB, and
+            // growing them by doubling would copy them several times
+            size_t nb = 0, nd = 0;
+            for (int r = 0; r < J; ++r) {
+                const int R = p.bsz_red[r];
+                if (R == 0) continue;
+                nd += (size_t)R * R;
+                size_t cu = rows[r].count(r) ? 0 : R / Np;
+                for (auto& kv : rows[r]) cu += p.bsz_red[kv.first] / Np;
+                nb += (size_t)(R / Np) * cu;
+            }
+            ucol.reserve(nb);
+            uval.reserve(nb * Np * Np);
+            dval.reserve(nd);
+        }
         for (int r = 0; r < J; ++r) {
             const int R = p.bsz_red[r];
             if (R == 0) continue;

@@ -15,8 +15,12 @@
 // direction is not uniformly valid (silhouette or tangency inside) are
 // subdivided.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <exception>
 #include <functional>
+#include <mutex>
+#include <thread>
 
 #include "bh_xdg.hpp"
 
@@ -295,6 +299,34 @@ void sphere_rule_rec(int D, const double* lo, const double* hi, const Sphere& S,
 
 }  // namespace
 
+// The rules of different cells and faces are independent; every result lands
+// in its own slot and is consumed afterwards in index order, so the output does
+// not depend on the thread count.
+void parallel_for(int n, const std::function<void(int)>& f) {
+    const int nt = std::max(1, std::min({(int)std::thread::hardware_concurrency(), 16, n / 8}));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::mutex em;
+    std::exception_ptr err;   // the first exception of any thread, rethrown on the caller's
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            try {
+                for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(em);
+                if (!err) err = std::current_exception();
+                next.store(n);
+            }
+        });
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
 void sphere_volume_rule(int D, const double* lo, const double* hi, const Sphere& S, int species, int q, QRule& out) {
     Gauss1 G(q);
     sphere_rule_rec(D, lo, hi, S, species, 0, G, 0, out);
@@ -381,6 +413,9 @@ void XdgProblem::setup() {
     std::vector<int> vol_rule0[2], vol_rulen[2], if_r0(J, 0), if_rn(J, 0);
     for (int s = 0; s < 2; ++s) { vol_rule0[s].assign(J, 0); vol_rulen[s].assign(J, 0); }
     ncut = 0;
+    // pass 1: classify by the box's distance range to the centre; candidates get
+    // their species and interface rules computed in parallel
+    std::vector<int> cand_of(J, -1), cand;
     for (int c = 0; c < J; ++c) {
         double lo[3], hi[3];
         box(c, lo, hi);
@@ -395,12 +430,26 @@ void XdgProblem::setup() {
         const double r2 = S.r * S.r;
         if (dmax2 <= r2) { spmask[c] = 1; vol[0][c] = V; }
         else if (dmin2 >= r2) { spmask[c] = 2; vol[1][c] = V; }
-        else {
-            // candidate cut cell: compute the species rules
-            QRule RA, RB, RI;
-            sphere_volume_rule(D, lo, hi, S, -1, q, RA);
-            sphere_volume_rule(D, lo, hi, S, +1, q, RB);
-            sphere_interface_rule(D, lo, hi, S, q, RI);
+        else { cand_of[c] = (int)cand.size(); cand.push_back(c); }
+    }
+    std::vector<QRule> crule(3 * cand.size());
+    parallel_for((int)cand.size(), [&](int i) {
+        double lo[3], hi[3];
+        box(cand[i], lo, hi);
+        sphere_volume_rule(D, lo, hi, S, -1, q, crule[3 * i]);
+        sphere_volume_rule(D, lo, hi, S, +1, q, crule[3 * i + 1]);
+        sphere_interface_rule(D, lo, hi, S, q, crule[3 * i + 2]);
+    });
+    // pass 2, in cell order: species volumes, masks and rule slots
+    for (int c = 0; c < J; ++c) {
+        double lo[3], hi[3];
+        box(c, lo, hi);
+        double V = 1;
+        for (int d = 0; d < D; ++d) V *= hi[d] - lo[d];
+        if (cand_of[c] >= 0) {
+            const QRule& RA = crule[3 * cand_of[c]];
+            const QRule& RB = crule[3 * cand_of[c] + 1];
+            const QRule& RI = crule[3 * cand_of[c] + 2];
             double va = 0, vb = 0, ai = 0;
             for (double w : RA.w) va += w;
             for (double w : RB.w) vb += w;
@@ -424,6 +473,8 @@ void XdgProblem::setup() {
     // faces: per species, area and rule; (cell, dir) '+' faces and boundary faces
     struct FaceInfo { int a, b, dir, side; double area[2]; int r0[2], rn[2]; bool tensor[2]; };
     std::vector<FaceInfo> faces;
+    struct CutFace { int face; double lo[3], hi[3]; };
+    std::vector<CutFace> cut_faces;
     std::vector<double> edge_sum[2];
     for (int s = 0; s < 2; ++s) edge_sum[s].assign(J, 0.0);
     for (int c = 0; c < J; ++c) {
@@ -458,22 +509,36 @@ void XdgProblem::setup() {
                 if (dmax2 <= r2) { F.area[0] = A; F.tensor[0] = true; }
                 else if (dmin2 >= r2) { F.area[1] = A; F.tensor[1] = true; }
                 else {
-                    for (int sp = 0; sp < 2; ++sp) {
-                        QRule R;
-                        sphere_face_rule(D, fl, fh, d, S, sp == 0 ? -1 : +1, q, R);
-                        double a = 0;
-                        for (double w : R.w) a += w;
-                        F.area[sp] = a;
-                        if (a > 0) push_rule(R, F.r0[sp], F.rn[sp]);
-                    }
-                }
-                for (int sp = 0; sp < 2; ++sp) {
-                    edge_sum[sp][c] += F.area[sp];
-                    if (F.b >= 0) edge_sum[sp][F.b] += F.area[sp];
+                    CutFace cf;
+                    cf.face = (int)faces.size();
+                    for (int e = 0; e < 3; ++e) { cf.lo[e] = fl[e]; cf.hi[e] = fh[e]; }
+                    cut_faces.push_back(cf);
                 }
                 faces.push_back(F);
             }
     }
+    // species rules of the cut faces, in parallel, then areas and rule slots in face order
+    std::vector<QRule> frule(2 * cut_faces.size());
+    parallel_for((int)cut_faces.size(), [&](int i) {
+        const CutFace& cf = cut_faces[i];
+        for (int sp = 0; sp < 2; ++sp)
+            sphere_face_rule(D, cf.lo, cf.hi, faces[cf.face].dir, S, sp == 0 ? -1 : +1, q, frule[2 * i + sp]);
+    });
+    for (size_t i = 0; i < cut_faces.size(); ++i) {
+        FaceInfo& F = faces[cut_faces[i].face];
+        for (int sp = 0; sp < 2; ++sp) {
+            const QRule& R = frule[2 * i + sp];
+            double a = 0;
+            for (double w : R.w) a += w;
+            F.area[sp] = a;
+            if (a > 0) push_rule(R, F.r0[sp], F.rn[sp]);
+        }
+    }
+    for (const FaceInfo& F : faces)
+        for (int sp = 0; sp < 2; ++sp) {
+            edge_sum[sp][F.a] += F.area[sp];
+            if (F.b >= 0) edge_sum[sp][F.b] += F.area[sp];
+        }
     // face index of every (cell, local face 2d+s)
     std::vector<int> cell_face((size_t)J * 6, -1);
     for (size_t fi = 0; fi < faces.size(); ++fi) {
