@@ -56,6 +56,34 @@ def sip_flops(n, D, k):
     return J * vol + faces
 
 
+def sip_flops_modal(n, D, k):
+    """Algorithmic FLOPs of the apply in modal face form: the volume as in
+    sip_flops; per face 8*(Np + nd) (value and d-derivative traces of both cells
+    along the normal, and the contraction back) plus 24 per face mode e' (the
+    flux terms of both sides), Nf = C(k + D - 1, D - 1) face modes."""
+    from math import comb
+    Np = comb(k + D, D)
+    Nf = comb(k + D - 1, D - 1)
+    J = n ** D
+    ex = []
+    for deg in range(k + 1):
+        if D == 2:
+            ex += [(a, deg - a) for a in range(deg + 1)]
+        else:
+            ex += [(a, b, deg - a - b) for a in range(deg + 1) for b in range(deg - a + 1)]
+    vol = 0
+    faces = 0
+    Fd = (n - 1) * n ** (D - 1) + 2 * n ** (D - 1)
+    for d in range(D):
+        for em in ex:
+            for en in ex:
+                if em[d] >= 1 and en[d] >= 1 and (em[d] + en[d]) % 2 == 0 and all(em[e] == en[e] for e in range(D) if e != d):
+                    vol += 2
+        nd = sum(1 for e in ex if e[d] >= 1)
+        faces += Fd * (8 * (Np + nd) + 24 * Nf)
+    return J * vol + faces
+
+
 def legendre_1d(k, xi):
     """orthonormal 1D Legendre sqrt(n+1/2) P_n at xi, shape (len(xi), k+1)."""
     P = np.zeros((len(xi), k + 1))
@@ -478,12 +506,28 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = N * args.steps / wall
 
-    flops = sip_flops(n, D, k)
-    achieved = flops / (kern_ms * 1e-3)
     hbm_bytes = 2 * N * 8 + grid.J * 8
-    # the FP64 apply picks its kernel (vector FMA or FP64 MFMA) by timing both on first use
+    # the FP64 apply picks its kernel (vector FMA, node by node or modal, or FP64 MFMA) by timing on first use
     f64_path, f64_w, f64_x, f64_ms_v, f64_ms_m = op.fp64_tuning()
-    kname = f"sip_apply_mfma64_kernel<{D}, {k}, {f64_w}>" if f64_path == "mfma" else f"sip_apply_kernel<{D}, {k}, double>"
+    modal = f64_path == "modal"
+    flops = sip_flops_modal(n, D, k) if modal else sip_flops(n, D, k)
+    achieved = flops / (kern_ms * 1e-3)
+    kname = (f"sip_apply_mfma64_kernel<{D}, {k}, {f64_w}>" if f64_path == "mfma"
+             else f"sip_apply_kernel<{D}, {k}, double, {'true' if modal else 'false'}>")
+    if modal:
+        # the modal form needs 1/8 of the quadrature's FLOPs: HBM is the bound
+        gbps = hbm_bytes / (kern_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": gbps, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                    "frac": gbps * 1e9 / PEAK_HBM, "traffic": measured_traffic(n, k, kname),
+                    "kernel": kname, "kernel_ms": kern_ms, "algorithmic_bytes": hbm_bytes,
+                    "algorithmic_flop": flops, "fp64_frac_at_algorithmic_flop": achieved / PEAK_FP64}
+    else:
+        # compute-bound: FP64 peak, the same for vector FMA and matrix cores on MI355X
+        roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                    "frac": achieved / PEAK_FP64, "traffic": measured_traffic(n, k, kname),
+                    "kernel": kname, "kernel_ms": kern_ms,
+                    "algorithmic_flop": flops, "algorithmic_bytes": hbm_bytes,
+                    "hbm_frac_at_algorithmic_bytes": hbm_bytes / (kern_ms * 1e-3) / PEAK_HBM}
 
     out = {
         "metric": "DG op-apply DOF/s; time-to-1e-8 residual, mixed+learned vs plain FP64 MG",
@@ -500,12 +544,7 @@ def main():
         "data": "synthetic (seeded N(0,1) DOF vector, log-uniform per-cell diffusion)",
         "config": {"workload": f"3D SIP Poisson {n}^3 hexes k={k} ({N} DOF), FP64 matrix-free operator apply",
                    "grid": f"{n}^3", "k": k, "dof": N},
-        # compute-bound: FP64 peak, the same for vector FMA and matrix cores on MI355X
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": measured_traffic(n, k, kname),
-                     "kernel": kname, "kernel_ms": kern_ms,
-                     "algorithmic_flop": flops, "algorithmic_bytes": hbm_bytes,
-                     "hbm_frac_at_algorithmic_bytes": hbm_bytes / (kern_ms * 1e-3) / PEAK_HBM},
+        "roofline": roofline,
         "fp64_autotune": {"chosen": f64_path, "vector_ms": f64_ms_v, "mfma64_ms": f64_ms_m,
                           "mfma64_waves_per_workgroup": f64_w, "mfma64_workgroups_per_xcd": f64_x},
     }
@@ -537,12 +576,15 @@ def main():
         ctx.sync()
         ms = ctx.elapsed_ms(0, 1) / args.steps
         err = float(np.linalg.norm(y32.download().astype(np.float64) - ref) / np.linalg.norm(ref))
+        # the FP16-MFMA kernel evaluates the face rule node by node; the FP32 vector
+        # kernel takes the FP64 tuner's face form
+        fl = sip_flops_modal(n, D, k) if (modal and prec == B.BH_FP32) else sip_flops(n, D, k)
         out[name] = {"kernel_ms": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": bytes32,
                      "traffic": measured_traffic(n, k, "sip_apply_mfma_kernel<3, 3, 8>" if prec == B.BH_FP16_MFMA
-                                                 else "sip_apply_kernel<3, 3, float>"),
+                                                 else f"sip_apply_kernel<3, 3, float, {'true' if modal else 'false'}>"),
                      "hbm_GBps": bytes32 / (ms * 1e-3) / 1e9, "hbm_frac": bytes32 / (ms * 1e-3) / PEAK_HBM,
-                     "algorithmic_flop": flops, "TFLOPs": flops / (ms * 1e-3) / 1e12,
-                     "compute_frac": flops / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}
+                     "algorithmic_flop": fl, "TFLOPs": fl / (ms * 1e-3) / 1e12,
+                     "compute_frac": fl / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}
         if prec == B.BH_FP16_MFMA:
             w_, x_, t_ = op.mfma_tuning()
             out[name]["autotuned_launch"] = {"waves_per_workgroup": w_ % 100, "prefetch_neighbours": w_ >= 100,

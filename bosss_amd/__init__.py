@@ -437,16 +437,17 @@ class SipOperator(_Owned):
         return int(cfg[0]), int(cfg[1]), ms[0]
 
     def fp64_tuning(self):
-        """FP64 apply choice: ('vector' | 'mfma' | None, waves, workgroups per XCD, vector ms, MFMA ms)."""
+        """FP64 apply choice: ('vector' | 'modal' | 'mfma' | None, waves, workgroups per XCD, vector ms, MFMA ms)."""
         cfg = np.zeros(5, np.int32)
         ms = (_c_f * 3)()
         _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ms))
-        path = {0: None, 1: "vector", 2: "mfma"}[int(cfg[2])]
+        path = {0: None, 1: "vector", 2: "mfma", 3: "modal"}[int(cfg[2])]
         return path, int(cfg[3]), int(cfg[4]), ms[1], ms[2]
 
     def force_fp64_path(self, path, waves=8, wg_per_xcd=64):
-        """Pin the FP64 apply to 'vector' or 'mfma' (tests compare both against the oracle)."""
-        code = {"vector": 1, "mfma": 2}[path]
+        """Pin the FP64 apply to 'vector' (face rule node by node), 'modal' (the vector
+        kernel with modal face terms) or 'mfma'; tests compare each against the oracle."""
+        code = {"vector": 1, "mfma": 2, "modal": 3}[path]
         _check(lib().bh_op_set_fp64_path(self.h, code, int(waves), int(wg_per_xcd)))
 
     def ComputeMatrix(self):

@@ -446,7 +446,9 @@ int bh_op_tuning(bh_op* op, int* cfg, float* ms) {
 int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
     BH_API_BEGIN
     BH_CHECK(op, "bh_op_set_fp64_path: null operator");
-    BH_CHECK(path >= 0 && path <= 2, "bh_op_set_fp64_path: path must be 0 (autotune), 1 (vector) or 2 (MFMA)");
+    BH_CHECK(path >= 0 && path <= 3,
+             "bh_op_set_fp64_path: path must be 0 (autotune), 1 (vector, face rule node by node), 2 (MFMA) or 3 "
+             "(vector, modal face terms)");
     if (path == 2) {
         BH_CHECK(bh::sip_apply_mfma64_supported(op->D, op->k, op->N),
                  "bh_op_set_fp64_path: the FP64-MFMA apply supports D in {2, 3}, k in {2, 3} and vectors below 4 GiB");
@@ -456,6 +458,7 @@ int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
         op->f64_wg_per_xcd = wg_per_xcd;
     }
     op->f64_path = path;
+    if (path == 1 || path == 3) op->modal_faces = path == 3;
     BH_API_END
 }
 

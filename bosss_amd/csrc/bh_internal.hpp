@@ -193,6 +193,7 @@ struct bh_op {
     int f64_path = 0;           // 0: not tuned, 1: vector, 2: MFMA
     int f64_waves = 8, f64_wg_per_xcd = 64;
     float f64_ms_vector = 0.f, f64_ms_mfma = 0.f;
+    bool modal_faces = true;    // vector apply: face terms in modal form (false: node by node); FP64 path 3 / 1
     size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0, off_K = 0;
     // boundary values and RHS -> affine vector
     std::vector<double> gbnd;

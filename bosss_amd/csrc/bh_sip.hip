@@ -234,6 +234,103 @@ __device__ __forceinline__ void face_pair(const SipArgs<R>& a, const R* uP, cons
     }
 }
 
+// Modal form of face_pair.  On an axis-aligned box the face rule of k + 1 Gauss
+// points per direction integrates every SIP face product exactly, and on the
+// face x_d = -1 / +1 the basis function of exponents (e', j) (e' the exponents
+// across the face, j the one along d) is p_j(-1 / +1) Phi_e'; its d-derivative
+// is p_j'(-1 / +1) Phi_e', with p_j = sqrt(j + 1/2) P_j, so p_j(+-1) = (+-1)^j s_j
+// and p_j'(+-1) = (+-1)^(j+1) s_j j(j+1)/2.  The Phi_e' are orthonormal on the
+// face, so the node sums of face_pair reduce to sums over e': per face mode the
+// traces are A = sum_j p_j(+-1) u_(e',j) and B = sum_j p_j'(+-1) u_(e',j), the
+// flux terms cvm, cdm, cvp, cdp become per-mode scalars, and
+// y_(e',j) += p_j(-1) CVM + p_j(+1) CVP + p_j'(-1) CDM + p_j'(+1) CDP.
+// The result equals the quadrature up to rounding, with about 4 Np multiply-adds
+// per side instead of 4 Np per face node.  A face mode is addressed by its root:
+// the mode with the same e' and j = 0.
+__host__ __device__ constexpr int face_root(int D, int k, int n, int d) {
+    const int Np = binom(k + D, D);
+    for (int m = 0; m < Np; ++m) {
+        if (mode_exponent(D, k, m, d) != 0) continue;
+        bool same = true;
+        for (int dd = 0; dd < D; ++dd)
+            if (dd != d && mode_exponent(D, k, m, dd) != mode_exponent(D, k, n, dd)) same = false;
+        if (same) return m;
+    }
+    return -1;
+}
+// s_j = sqrt(j + 1/2), correctly rounded
+__host__ __device__ constexpr double sqrt_half(int j) {
+    return j == 0 ? 0.7071067811865476 : j == 1 ? 1.224744871391589 : j == 2 ? 1.5811388300841898
+         : j == 3 ? 1.8708286933869707 : j == 4 ? 2.1213203435596424 : j == 5 ? 2.345207879911715
+         : 2.5495097567963922;
+}
+
+template <int D, int K, typename R, int d>
+__device__ __forceinline__ void face_pair_modal(const R* uP, const R* uM, const R* uQ, const bool* bnd,
+                                                const bool* skip, R gP, const R* gQ, const R* rq, R nuP,
+                                                const R* nuQ, const R* mu, R* y) {
+    constexpr int Np = binom(K + D, D);
+    static_assert(K <= 6, "sqrt_half covers j <= 6");
+    R c1[2], c2[2], c3[2], c4[2], c5[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const R sgn = s ? R(1) : R(-1);
+        const R G = skip[s] ? R(0) : gP;
+        const R A = G * nuP * sgn * gP;
+        const R nuMax = (fabs(nuP) > fabs(nuQ[s])) ? nuP : nuQ[s];
+        c1[s] = (bnd[s] ? R(1) : R(0.5)) * A;
+        c3[s] = -G * mu[s] * (bnd[s] ? R(2) * nuP : nuMax);
+        c2[s] = bnd[s] ? R(0) : R(0.5) * G * nuQ[s] * sgn * gQ[s] * rq[s];
+        c4[s] = bnd[s] ? R(0) : -c3[s] * rq[s];
+        c5[s] = bnd[s] ? R(0) : R(-0.5) * A * rq[s];
+    }
+    static_for<Np>([&](auto FF) {
+        constexpr int f = FF;
+        if constexpr (mode_exponent(D, K, f, d) == 0) {   // one face mode e' per root f
+            // The empty volatile asm statements keep their order, and they pin this face
+            // mode's inputs after the previous mode's outputs.  The face modes then run one
+            // after another, and the registers hold one mode's temporaries at a time.
+            // traces: P on both faces, M on its '+' face, Q on its '-' face
+            R Am = 0, Ap = 0, Bm = 0, Bp = 0, AM = 0, BM = 0, AQ = 0, BQ = 0;
+            static_for<Np>([&](auto NN) {
+                constexpr int n = NN;
+                if constexpr (face_root(D, K, n, d) == f) {
+                    constexpr int j = mode_exponent(D, K, n, d);
+                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+                    constexpr double odd = (j & 1) ? -1.0 : 1.0;
+                    R vP = uP[n], vM = uM[n], vQ = uQ[n];
+                    asm volatile("" : "+v"(vP), "+v"(vM), "+v"(vQ));
+                    Ap = fma(R(sv), vP, Ap);
+                    Am = fma(R(odd * sv), vP, Am);
+                    AM = fma(R(sv), vM, AM);
+                    AQ = fma(R(odd * sv), vQ, AQ);
+                    if constexpr (j > 0) {
+                        Bp = fma(R(sd), vP, Bp);
+                        Bm = fma(R(-odd * sd), vP, Bm);
+                        BM = fma(R(sd), vM, BM);
+                        BQ = fma(R(-odd * sd), vQ, BQ);
+                    }
+                }
+            });
+            const R CVM = c1[0] * Bm + c2[0] * BM + c3[0] * Am + c4[0] * AM;
+            const R CDM = c1[0] * Am + c5[0] * AM;
+            const R CVP = c1[1] * Bp + c2[1] * BQ + c3[1] * Ap + c4[1] * AQ;
+            const R CDP = c1[1] * Ap + c5[1] * AQ;
+            const R VE = CVM + CVP, VO = CVP - CVM, DE = CDP - CDM, DO = CDP + CDM;
+            static_for<Np>([&](auto MM) {
+                constexpr int m = MM;
+                if constexpr (face_root(D, K, m, d) == f) {
+                    constexpr int j = mode_exponent(D, K, m, d);
+                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+                    y[m] = fma(R(sv), (j & 1) ? VO : VE, y[m]);
+                    if constexpr (j > 0) y[m] = fma(R(sd), (j & 1) ? DO : DE, y[m]);
+                    asm volatile("" : "+v"(y[m]));
+                }
+            });
+        }
+    });
+}
+
 // Volume integral -nu grad u . grad v of an axis-aligned box cell: the metric
 // is diagonal and constant, so the Gauss sum over the volume nodes factorises
 // into the per-direction reference stiffness Kv[d] = sum_q w_q G_d(q) G_d(q)^T
@@ -283,8 +380,10 @@ __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S
     for (int d = 0; d < D; ++d) h[d] = S(a.width[d * a.nmax + idx[d]]);
 }
 
-template <int D, int K, typename R>
-__global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
+// MODAL: every direction loads its neighbours' coefficients and uses the modal
+// face form (face_pair_modal); otherwise the face rule is evaluated node by node.
+template <int D, int K, typename R, bool MODAL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sip_apply_kernel(SipArgs<R> a) {
     constexpr int Np = binom(K + D, D);
     constexpr int NQF = ipow(K + 1, D - 1);
     constexpr int DF = D == 2 ? 1 : 0;       // direction of consecutive cells
@@ -314,7 +413,7 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
     static_for<D>([&](auto DD) {
         constexpr int d = DD;
         const R ihd = ih[d];
-        constexpr bool FAST = d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
+        constexpr bool FAST = !MODAL && d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
         bool bnd[2], skip[2];
         R gQ[2], rq[2], nuQ[2], mu[2];
         R uM[FAST ? 1 : Np], uQ[FAST ? 1 : Np];
@@ -368,7 +467,14 @@ __global__ __launch_bounds__(256) void sip_apply_kernel(SipArgs<R> a) {
                 }
             }
         }
-        face_pair<D, K, R, d, FAST>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y, hv, lane);
+        if constexpr (MODAL) {
+            face_pair_modal<D, K, R, d>(uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y);
+            // one direction's neighbour coefficients live at a time (the scheduler
+            // would otherwise hoist every direction's loads and spill)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        else
+            face_pair<D, K, R, d, FAST>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y, hv, lane);
     });
     if (!valid) return;
     R* yp = a.y + (size_t)c * Np;
@@ -497,7 +603,10 @@ static SipArgs<R> make_args(bh_op* op, const void* u, void* y, double alpha, dou
 template <typename R, int D, int K>
 static void launch_apply(bh_op* op, const SipArgs<R>& a) {
     dim3 blk(256), grd((a.J + 255) / 256);
-    hipLaunchKernelGGL((sip_apply_kernel<D, K, R>), grd, blk, 0, op->ctx->stream, a);
+    if (op->modal_faces)
+        hipLaunchKernelGGL((sip_apply_kernel<D, K, R, true>), grd, blk, 0, op->ctx->stream, a);
+    else
+        hipLaunchKernelGGL((sip_apply_kernel<D, K, R, false>), grd, blk, 0, op->ctx->stream, a);
     BH_HIP(hipGetLastError());
 }
 
@@ -511,11 +620,13 @@ static void dispatch_apply(bh_op* op, const SipArgs<R>& a) {
     throw Error(-3, "sip_apply: unsupported (D, k) = (" + std::to_string(D) + ", " + std::to_string(K) + ")");
 }
 
-// Autotuner of the FP64 apply: times the vector kernel and the FP64-MFMA kernel
-// (waves per workgroup x workgroups per XCD) on a scratch vector of this
-// operator and keeps the fastest; both compute the same quadrature.
+// Autotuner of the FP64 apply: times the vector kernel (face terms node by node
+// and in modal form) and the FP64-MFMA kernel (waves per workgroup x workgroups
+// per XCD) on a scratch vector of this operator and keeps the fastest; all
+// compute the same integrals, up to rounding.
 static void tune_fp64_path(bh_op* op) {
-    op->f64_path = 1;
+    op->f64_path = 3;   // the modal vector kernel, unless timing says otherwise
+    op->modal_faces = true;
     if (!sip_apply_mfma64_supported(op->D, op->k, op->N)) return;
     DevBuf us(op->N * sizeof(double)), ys(op->N * sizeof(double));
     fill(op->ctx, us.as<double>(), 1.0, op->N);
@@ -532,7 +643,14 @@ static void tune_fp64_path(bh_op* op) {
         BH_HIP(hipEventElapsedTime(&ms, e0, e1));
         return ms / 3;
     };
-    const float tv = time_it([&] { dispatch_apply<double>(op, make_args<double>(op, us.p, ys.p, 1.0, 0.0, false)); });
+    // the vector kernel in both face forms (node by node, modal)
+    auto vec = [&] { dispatch_apply<double>(op, make_args<double>(op, us.p, ys.p, 1.0, 0.0, false)); };
+    op->modal_faces = false;
+    const float tnode = time_it(vec);
+    op->modal_faces = true;
+    const float tmodal = time_it(vec);
+    op->modal_faces = tmodal <= tnode;
+    const float tv = std::min(tnode, tmodal);
     float best = 1e30f;
     int bw = 8, bx = 64;
     for (int w : {4, 8})
@@ -546,7 +664,7 @@ static void tune_fp64_path(bh_op* op) {
     op->f64_ms_mfma = best;
     op->f64_waves = bw;
     op->f64_wg_per_xcd = bx;
-    op->f64_path = best < tv ? 2 : 1;
+    op->f64_path = best < tv ? 2 : (op->modal_faces ? 3 : 1);
 }
 
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec, const void* b0) {

@@ -106,14 +106,17 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
  * FP16-MFMA apply on its first call (cfg[0] waves per workgroup, plus 100 when
  * the variant that prefetches all neighbour blocks won; cfg[1] workgroups per
  * XCD; 0 before the first call) and its timed duration (ms) in ms[0].
- * FP64 apply (picked on its first call): cfg[2] = 0 not yet, 1 vector kernel,
- * 2 FP64-MFMA kernel; cfg[3], cfg[4] the MFMA kernel's waves per workgroup and
+ * FP64 apply (picked on its first call): cfg[2] = 0 not yet, 1 vector kernel
+ * with the face rule evaluated node by node, 2 FP64-MFMA kernel, 3 vector kernel
+ * with the face terms in modal form (the FP32 vector apply follows the 1 / 3
+ * choice, modal before any FP64 call); cfg[3], cfg[4] the MFMA kernel's waves per workgroup and
  * workgroups per XCD; ms[1], ms[2] the timed vector and best MFMA durations.
  * cfg holds 5 ints, ms 3 floats. */
 int bh_op_tuning(bh_op* op, int* cfg, float* ms);
 /* No counterpart in the reference: pin the FP64 apply to the vector kernel
- * (path 1) or the FP64-MFMA kernel (path 2, launch shape waves in {4, 8} per
- * workgroup, wg_per_xcd >= 1) instead of timing both on the first call;
+ * (path 1: face rule node by node; path 3: modal face terms) or the FP64-MFMA
+ * kernel (path 2, launch shape waves in {4, 8} per workgroup, wg_per_xcd >= 1)
+ * instead of timing them on the first call;
  * path 0 re-arms the autotuner.  Both kernels give the same result up to
  * summation order. */
 int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd);

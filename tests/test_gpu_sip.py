@@ -86,7 +86,7 @@ MFMA64_CASES = [(2, 2), (2, 3), (3, 2), (3, 3)]
 
 @pytest.mark.parametrize("D,k", MFMA64_CASES)
 @pytest.mark.parametrize("affine", [False, True])
-@pytest.mark.parametrize("path,waves,per_xcd", [("vector", 8, 64), ("mfma", 4, 1), ("mfma", 8, 64)])
+@pytest.mark.parametrize("path,waves,per_xcd", [("vector", 8, 64), ("modal", 8, 64), ("mfma", 4, 1), ("mfma", 8, 64)])
 def test_evaluate_fp64_paths_match_oracle(ctx, D, k, affine, path, waves, per_xcd):
     """Both FP64 kernels (vector and FP64-MFMA, each pinned, several launch
     shapes) within 1e-12 of the oracle, with Neumann sides, the affine part and
@@ -119,11 +119,46 @@ def test_evaluate_fp64_autotuned_ragged(ctx):
     auto = go.Evaluate(1.0, 0.0, np.zeros(go.N), u, affine=False)
     path, waves, per_xcd, ms_v, ms_m = go.fp64_tuning()
     _say(f"  tuned: {path} waves={waves} per_xcd={per_xcd} vector {ms_v:.4f} ms, mfma {ms_m:.4f} ms")
-    assert path in ("vector", "mfma") and ms_v > 0 and ms_m > 0
+    assert path in ("vector", "modal", "mfma") and ms_v > 0 and ms_m > 0
     go.force_fp64_path("vector")
     vec = go.Evaluate(1.0, 0.0, np.zeros(go.N), u, affine=False)
     assert rel(auto, vec) < 1e-13
     assert rel(auto, oo.evaluate(u)) < 1e-12
+
+
+@pytest.mark.parametrize("D,k", [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1), (3, 2), (3, 3)])
+def test_evaluate_modal_faces(ctx, D, k):
+    """The vector kernel with the face terms in modal form equals the one that
+    evaluates the face rule node by node to 1e-13 and the oracle to 1e-12 in FP64
+    (ragged stretched grid, Neumann sides, affine part, Evaluate(alpha, beta)), and
+    its FP32 apply is within 1e-5 of the FP64 result."""
+    _say(f"case evaluate modal faces D={D} k={k}")
+    rng = np.random.default_rng(80 + 10 * D + k)
+    cells = (23, 17) if D == 2 else (13, 11, 9)
+    nodes = [np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n)]) for n in cells]
+    og = O.Grid(*nodes)
+    nu = log_uniform_nu(og.J, 81)
+    gg = B.Grid(ctx, *nodes)
+    bc = [0, 1] * D
+    oo = O.Sip(og, k, nu, bc=bc)
+    go = B.SipOperator(gg, k, nu, bc=bc)
+    gv = _bnd_values(oo, 82)
+    oo.set_bnd_values(gv)
+    go.set_bnd_values(gv)
+    u = rng.standard_normal(oo.N)
+    y0 = rng.standard_normal(oo.N)
+    go.force_fp64_path("vector")
+    node = go.Evaluate(-1.5, 0.25, y0.copy(), u)
+    go.force_fp64_path("modal")
+    modal = go.Evaluate(-1.5, 0.25, y0.copy(), u)
+    assert go.fp64_tuning()[0] == "modal"
+    assert rel(modal, node) < 1e-13
+    assert rel(modal, oo.evaluate(u, alpha=-1.5, beta=0.25, y=y0.copy())) < 1e-12
+    ud = B.DeviceArray.from_host(ctx, u.astype(np.float32))
+    yd = B.DeviceArray(ctx, go.N, np.float32)
+    go.Evaluate(1.0, 0.0, yd, ud, affine=False, prec=B.BH_FP32)
+    ref = go.Evaluate(1.0, 0.0, np.zeros(go.N), u.astype(np.float32).astype(np.float64), affine=False)
+    assert rel(yd.download().astype(np.float64), ref) < 1e-5
 
 
 @pytest.mark.parametrize("D,k", [(2, 2), (3, 3)])
