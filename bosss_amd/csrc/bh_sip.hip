@@ -331,6 +331,66 @@ __device__ __forceinline__ void face_pair_modal(const R* uP, const R* uM, const 
     });
 }
 
+// One side of face_pair_modal: side s = 0 is the '-' face (neighbour M, its '+' face),
+// s = 1 the '+' face (neighbour Q, its '-' face).  The two sides are independent
+// once P's traces are formed per side, so one neighbour's coefficients are live
+// at a time.
+template <int D, int K, typename R, int d, int s>
+__device__ __forceinline__ void face_side_modal(const R* uP, const R* un, bool bnd, bool skip, R gP, R gQ, R rq,
+                                                R nuP, R nuQ, R mu, R* y) {
+    constexpr int Np = binom(K + D, D);
+    static_assert(K <= 6, "sqrt_half covers j <= 6");
+    const R sgn = s ? R(1) : R(-1);
+    const R G = skip ? R(0) : gP;
+    const R A = G * nuP * sgn * gP;
+    const R nuMax = (fabs(nuP) > fabs(nuQ)) ? nuP : nuQ;
+    const R c1 = (bnd ? R(1) : R(0.5)) * A;
+    const R c3 = -G * mu * (bnd ? R(2) * nuP : nuMax);
+    const R c2 = bnd ? R(0) : R(0.5) * G * nuQ * sgn * gQ * rq;
+    const R c4 = bnd ? R(0) : -c3 * rq;
+    const R c5 = bnd ? R(0) : R(-0.5) * A * rq;
+    static_for<Np>([&](auto FF) {
+        constexpr int f = FF;
+        if constexpr (mode_exponent(D, K, f, d) == 0) {
+            // P's value / d-derivative trace on face s, the neighbour's on the same face
+            R Ao = 0, Bo = 0, An = 0, Bn = 0;
+            static_for<Np>([&](auto NN) {
+                constexpr int n = NN;
+                if constexpr (face_root(D, K, n, d) == f) {
+                    constexpr int j = mode_exponent(D, K, n, d);
+                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+                    constexpr double odd = (j & 1) ? -1.0 : 1.0;
+                    // p_j(+1) = s_j, p_j'(+1) = sd_j; p_j(-1) = (-1)^j s_j, p_j'(-1) = -(-1)^j sd_j
+                    constexpr double vo = s ? sv : odd * sv, dov = s ? sd : -odd * sd;   // own side
+                    constexpr double vn = s ? odd * sv : sv, dnv = s ? -odd * sd : sd;   // neighbour's side
+                    R vP = uP[n], vN = un[n];
+                    asm volatile("" : "+v"(vP), "+v"(vN));
+                    Ao = fma(R(vo), vP, Ao);
+                    An = fma(R(vn), vN, An);
+                    if constexpr (j > 0) {
+                        Bo = fma(R(dov), vP, Bo);
+                        Bn = fma(R(dnv), vN, Bn);
+                    }
+                }
+            });
+            const R CV = c1 * Bo + c2 * Bn + c3 * Ao + c4 * An;
+            const R CD = c1 * Ao + c5 * An;
+            static_for<Np>([&](auto MM) {
+                constexpr int m = MM;
+                if constexpr (face_root(D, K, m, d) == f) {
+                    constexpr int j = mode_exponent(D, K, m, d);
+                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+                    constexpr double odd = (j & 1) ? -1.0 : 1.0;
+                    constexpr double vo = s ? sv : odd * sv, dov = s ? sd : -odd * sd;
+                    y[m] = fma(R(vo), CV, y[m]);
+                    if constexpr (j > 0) y[m] = fma(R(dov), CD, y[m]);
+                    asm volatile("" : "+v"(y[m]));
+                }
+            });
+        }
+    });
+}
+
 // Volume integral -nu grad u . grad v of an axis-aligned box cell: the metric
 // is diagonal and constant, so the Gauss sum over the volume nodes factorises
 // into the per-direction reference stiffness Kv[d] = sum_q w_q G_d(q) G_d(q)^T
@@ -383,7 +443,7 @@ __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S
 // MODAL: every direction loads its neighbours' coefficients and uses the modal
 // face form (face_pair_modal); otherwise the face rule is evaluated node by node.
 template <int D, int K, typename R, bool MODAL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sip_apply_kernel(SipArgs<R> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODAL ? 3 : 2))) void sip_apply_kernel(SipArgs<R> a) {
     constexpr int Np = binom(K + D, D);
     constexpr int NQF = ipow(K + 1, D - 1);
     constexpr int DF = D == 2 ? 1 : 0;       // direction of consecutive cells
@@ -413,6 +473,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
     static_for<D>([&](auto DD) {
         constexpr int d = DD;
         const R ihd = ih[d];
+        if constexpr (MODAL) {
+            // modal faces, one side at a time
+            static_for<2>([&](auto SS) {
+                constexpr int s = SS;
+                const bool bnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+                const bool skip = bnd && ((a.bc_neumann >> (2 * d + s)) & 1);
+                R gQ = 0, rq = 0, nuQ = nuP, mu = pen * cjP;
+                R un[Np];
+                if (bnd) {
+#pragma unroll
+                    for (int n = 0; n < Np; ++n) un[n] = 0;
+                } else {
+                    int nidx[3] = {idx[0], idx[1], idx[2]};
+                    nidx[d] += s ? 1 : -1;
+                    const int cq = encode<D>(nidx, a.n0, a.n1);
+                    const R ihq = R(a.width[(3 + d) * a.nmax + nidx[d]]);
+                    const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
+                    const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+                    const R cjQ = cjP - bP * ihd + bQ * ihq;
+                    gQ = R(2) * ihq;
+                    rq = R(a.width[(6 + d) * a.nmax + idx[d]] * a.width[(9 + d) * a.nmax + nidx[d]]);
+                    nuQ = a.nu[cq];
+                    mu = pen * (cjP > cjQ ? cjP : cjQ);
+                    const R* uq = a.u + (size_t)cq * Np;
+#pragma unroll
+                    for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                }
+                face_side_modal<D, K, R, d, s>(uP, un, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y);
+            });
+            return;
+        }
         constexpr bool FAST = !MODAL && d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
         bool bnd[2], skip[2];
         R gQ[2], rq[2], nuQ[2], mu[2];
