@@ -98,6 +98,7 @@ struct PMultigrid : bh_solver {
     int prec = BH_FP64;
     int dim = 3;                   // space dimension of the grid
     bool mfma = false;             // FP32 cycle: matrix-free level 0 on the FP16 matrix cores
+    bool l0_mfma_apply = true;     // ... for its operator apply too (timed at set-up against the FP32 kernel)
     double omega = 1.0;
     int pre = 1, post = 1;
     bool matrix_free = true;
@@ -151,7 +152,9 @@ struct PMultigrid : bh_solver {
     template <typename T> void smooth(int l, const T* r, T* x, bool add);   // x (+)= omega M_l^-1 r
     void set_schwarz(bool on);     // aggregate-block Schwarz on every level that has aggregates
     void set_equilib();            // DiagBlockEquilib (assembled hierarchies; irreversible)
-    int op_prec(int l, bool f32) const { return f32 ? ((l == 0 && mfma) ? BH_FP16_MFMA : BH_FP32) : BH_FP64; }
+    int op_prec(int l, bool f32) const {
+        return f32 ? ((l == 0 && mfma && l0_mfma_apply) ? BH_FP16_MFMA : BH_FP32) : BH_FP64;
+    }
     template <typename T> T* vec(DevBuf& d) { return (T*)d.p; }
 };
 

@@ -349,6 +349,31 @@ void PMultigrid::build_grid(bh_ctx* c, bh_grid* grid, bh_bsr* Af, const std::vec
         L.x.alloc(L.N * es);
         L.r.alloc(L.N * es);
     }
+    // FP16-MFMA cycle: the level-0 operator apply on the FP16 matrix cores or the FP32
+    // vector kernel (modal face form), whichever is faster on this operator; the FP16
+    // smoother blocks stay either way
+    l0_mfma_apply = true;
+    if (f32 && mfma && lv[0].op) {
+        MgLevel& L = lv[0];
+        fill32(ctx, L.x.as<float>(), 1.0f, L.N);
+        hipEvent_t e0, e1;
+        BH_HIP(hipEventCreate(&e0));
+        BH_HIP(hipEventCreate(&e1));
+        auto time_it = [&](int p) {
+            sip_apply(L.op, 1.0, L.x.p, 0.0, L.r.p, false, p);   // first call: the kernel's own tuner
+            BH_HIP(hipEventRecord(e0, ctx->stream));
+            for (int i = 0; i < 3; ++i) sip_apply(L.op, 1.0, L.x.p, 0.0, L.r.p, false, p);
+            BH_HIP(hipEventRecord(e1, ctx->stream));
+            BH_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            BH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
+        const float t16 = time_it(BH_FP16_MFMA), t32 = time_it(BH_FP32);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        l0_mfma_apply = t16 < t32;
+    }
     if (kcycle_level >= (int)lv.size() - 1) kcycle_level = -1;   // the coarsest level is solved directly
     if (kcycle_level >= 0 && kin > 0) {
         MgLevel& K = lv[kcycle_level];
