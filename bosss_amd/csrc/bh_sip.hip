@@ -443,7 +443,7 @@ __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S
 // MODAL: every direction loads its neighbours' coefficients and uses the modal
 // face form (face_pair_modal); otherwise the face rule is evaluated node by node.
 template <int D, int K, typename R, bool MODAL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODAL ? 3 : 2))) void sip_apply_kernel(SipArgs<R> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sip_apply_kernel(SipArgs<R> a) {
     constexpr int Np = binom(K + D, D);
     constexpr int NQF = ipow(K + 1, D - 1);
     constexpr int DF = D == 2 ? 1 : 0;       // direction of consecutive cells
