@@ -265,76 +265,10 @@ __host__ __device__ constexpr double sqrt_half(int j) {
          : 2.5495097567963922;
 }
 
-template <int D, int K, typename R, int d>
-__device__ __forceinline__ void face_pair_modal(const R* uP, const R* uM, const R* uQ, const bool* bnd,
-                                                const bool* skip, R gP, const R* gQ, const R* rq, R nuP,
-                                                const R* nuQ, const R* mu, R* y) {
-    constexpr int Np = binom(K + D, D);
-    static_assert(K <= 6, "sqrt_half covers j <= 6");
-    R c1[2], c2[2], c3[2], c4[2], c5[2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const R sgn = s ? R(1) : R(-1);
-        const R G = skip[s] ? R(0) : gP;
-        const R A = G * nuP * sgn * gP;
-        const R nuMax = (fabs(nuP) > fabs(nuQ[s])) ? nuP : nuQ[s];
-        c1[s] = (bnd[s] ? R(1) : R(0.5)) * A;
-        c3[s] = -G * mu[s] * (bnd[s] ? R(2) * nuP : nuMax);
-        c2[s] = bnd[s] ? R(0) : R(0.5) * G * nuQ[s] * sgn * gQ[s] * rq[s];
-        c4[s] = bnd[s] ? R(0) : -c3[s] * rq[s];
-        c5[s] = bnd[s] ? R(0) : R(-0.5) * A * rq[s];
-    }
-    static_for<Np>([&](auto FF) {
-        constexpr int f = FF;
-        if constexpr (mode_exponent(D, K, f, d) == 0) {   // one face mode e' per root f
-            // The empty volatile asm statements keep their order, and they pin this face
-            // mode's inputs after the previous mode's outputs.  The face modes then run one
-            // after another, and the registers hold one mode's temporaries at a time.
-            // traces: P on both faces, M on its '+' face, Q on its '-' face
-            R Am = 0, Ap = 0, Bm = 0, Bp = 0, AM = 0, BM = 0, AQ = 0, BQ = 0;
-            static_for<Np>([&](auto NN) {
-                constexpr int n = NN;
-                if constexpr (face_root(D, K, n, d) == f) {
-                    constexpr int j = mode_exponent(D, K, n, d);
-                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
-                    constexpr double odd = (j & 1) ? -1.0 : 1.0;
-                    R vP = uP[n], vM = uM[n], vQ = uQ[n];
-                    asm volatile("" : "+v"(vP), "+v"(vM), "+v"(vQ));
-                    Ap = fma(R(sv), vP, Ap);
-                    Am = fma(R(odd * sv), vP, Am);
-                    AM = fma(R(sv), vM, AM);
-                    AQ = fma(R(odd * sv), vQ, AQ);
-                    if constexpr (j > 0) {
-                        Bp = fma(R(sd), vP, Bp);
-                        Bm = fma(R(-odd * sd), vP, Bm);
-                        BM = fma(R(sd), vM, BM);
-                        BQ = fma(R(-odd * sd), vQ, BQ);
-                    }
-                }
-            });
-            const R CVM = c1[0] * Bm + c2[0] * BM + c3[0] * Am + c4[0] * AM;
-            const R CDM = c1[0] * Am + c5[0] * AM;
-            const R CVP = c1[1] * Bp + c2[1] * BQ + c3[1] * Ap + c4[1] * AQ;
-            const R CDP = c1[1] * Ap + c5[1] * AQ;
-            const R VE = CVM + CVP, VO = CVP - CVM, DE = CDP - CDM, DO = CDP + CDM;
-            static_for<Np>([&](auto MM) {
-                constexpr int m = MM;
-                if constexpr (face_root(D, K, m, d) == f) {
-                    constexpr int j = mode_exponent(D, K, m, d);
-                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
-                    y[m] = fma(R(sv), (j & 1) ? VO : VE, y[m]);
-                    if constexpr (j > 0) y[m] = fma(R(sd), (j & 1) ? DO : DE, y[m]);
-                    asm volatile("" : "+v"(y[m]));
-                }
-            });
-        }
-    });
-}
-
-// One side of face_pair_modal: side s = 0 is the '-' face (neighbour M, its '+' face),
-// s = 1 the '+' face (neighbour Q, its '-' face).  The two sides are independent
-// once P's traces are formed per side, so one neighbour's coefficients are live
-// at a time.
+// The modal face terms of one side of direction d: s = 0 is the '-' face (neighbour
+// M, its '+' face), s = 1 the '+' face (neighbour Q, its '-' face).  The coefficients
+// c1..c5 are those of face_pair.  The two sides are independent, so the kernel runs
+// them one after the other with one neighbour's coefficients live at a time.
 template <int D, int K, typename R, int d, int s>
 __device__ __forceinline__ void face_side_modal(const R* uP, const R* un, bool bnd, bool skip, R gP, R gQ, R rq,
                                                 R nuP, R nuQ, R mu, R* y) {
@@ -441,7 +375,7 @@ __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S
 }
 
 // MODAL: every direction loads its neighbours' coefficients and uses the modal
-// face form (face_pair_modal); otherwise the face rule is evaluated node by node.
+// face form (face_side_modal); otherwise the face rule is evaluated node by node.
 template <int D, int K, typename R, bool MODAL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sip_apply_kernel(SipArgs<R> a) {
     constexpr int Np = binom(K + D, D);
@@ -504,7 +438,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
             });
             return;
         }
-        constexpr bool FAST = !MODAL && d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
+        constexpr bool FAST = d == DF && NQF <= 16 && Np > 4;   // at k = 1 the exchange costs what it saves
         bool bnd[2], skip[2];
         R gQ[2], rq[2], nuQ[2], mu[2];
         R uM[FAST ? 1 : Np], uQ[FAST ? 1 : Np];
@@ -558,14 +492,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
                 }
             }
         }
-        if constexpr (MODAL) {
-            face_pair_modal<D, K, R, d>(uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y);
-            // one direction's neighbour coefficients live at a time (the scheduler
-            // would otherwise hoist every direction's loads and spill)
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        else
-            face_pair<D, K, R, d, FAST>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y, hv, lane);
+        face_pair<D, K, R, d, FAST>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y, hv, lane);
     });
     if (!valid) return;
     R* yp = a.y + (size_t)c * Np;
