@@ -898,6 +898,7 @@ int bh_xdg_create(bh_grid* g, int k, const double* center, double radius, double
     BH_API_BEGIN
     BH_CHECK(g && center && out, "bh_xdg_create: null argument");
     BH_CHECK(k >= 1 && k <= 3, "bh_xdg_create: degree must be 1..3");
+    BH_CHECK(!(g->D == 3 && k == 3), "bh_xdg_create: D = 3 supports k = 1, 2 (the element kernels hold 4 Np^2 <= 448 entries per wave)");
     BH_CHECK(radius > 0, "bh_xdg_create: radius must be positive");
     BH_CHECK(agg_threshold >= 0 && agg_threshold < 0.5, "bh_xdg_create: agglomeration threshold must be in [0, 0.5)");
     auto X = std::make_unique<bh_xdg>();

@@ -20,6 +20,7 @@
 #include <exception>
 #include <functional>
 #include <mutex>
+#include <stdexcept>
 #include <thread>
 
 #include "bh_xdg.hpp"
@@ -546,55 +547,187 @@ void XdgProblem::setup() {
         cell_face[(size_t)F.a * 6 + 2 * F.dir + F.side] = (int)fi;
         if (F.b >= 0) cell_face[(size_t)F.b * 6 + 2 * F.dir + 0] = (int)fi;
     }
-    // agglomeration (X/AgglomerationAlgorithm.cs:393-460 sources, :2064-2140 direct targets)
+    // agglomeration, per species (X/AgglomerationAlgorithm.cs, one rank, static
+    // interface): sources :393-460; "sources cannot be targets" :1721-1726; direct
+    // targets SearchDirectAgglomeration_Mk3 :2064-2200; chains
+    // SearchChainAgglomeration_Mk3 :1815-2055; groups SearchGroupAgglomeration_Mk3
+    // :2232-2330 with AgglomerationGroup.AddNeighbors :72-111; self pairs removed
     agg_pairs.clear();
     n_failed = 0;
     const double EmptyEdge = 1e-10;
+    auto center = [&](int c, double* x) {
+        double lo[3], hi[3];
+        box(c, lo, hi);
+        for (int d = 0; d < 3; ++d) x[d] = 0.5 * (lo[d] + hi[d]);
+    };
     for (int sp = 0; sp < 2; ++sp) {
-        std::vector<char> source(J, 0);
-        for (int c = 0; c < J; ++c)
-            if (cut[c]) {
-                double f = std::min(1.0, std::max(0.0, frac[sp][c]));
-                if (f <= agg_threshold) source[c] = 1;
-            }
+        std::vector<char> source(J, 0), cand(J, 0);
         for (int c = 0; c < J; ++c) {
-            if (!source[c]) continue;
+            double lo[3], hi[3];
+            box(c, lo, hi);
+            double V = 1;
+            for (int d = 0; d < D; ++d) V *= hi[d] - lo[d];
+            const bool present = (spmask[c] >> sp) & 1;
+            const double f = std::min(1.0, std::max(0.0, std::max(vol[sp][c], 0.0) / V));
+            source[c] = cut[c] && present && f <= agg_threshold;
+            cand[c] = present && !source[c];
+        }
+        // neighbours and species cut-edge areas per face, per direction '-' then '+'
+        auto neighbours = [&](int c, int* nbs, double* areas) {
             int idx[3];
             g->cell_ijk(c, idx);
-            // neighbours in the reference's face order (per direction: '-' then '+')
-            int nbs[6], fidx[6], nnb = 0;
-            double areas[6];
+            int t = 0;
             for (int d = 0; d < D; ++d)
-                for (int s = 0; s < 2; ++s) {
+                for (int s = 0; s < 2; ++s, ++t) {
                     int nb[3] = {idx[0], idx[1], idx[2]};
                     nb[d] += s ? 1 : -1;
                     const bool inside = nb[d] >= 0 && nb[d] < g->n[d];
-                    nbs[nnb] = inside ? g->cell_index(nb[0], nb[1], nb[2]) : -1;
-                    fidx[nnb] = 2 * d + s;
-                    areas[nnb] = 0;
-                    ++nnb;
+                    nbs[t] = inside ? g->cell_index(nb[0], nb[1], nb[2]) : -1;
+                    const int fi = cell_face[(size_t)c * 6 + 2 * d + s];
+                    areas[t] = fi >= 0 ? faces[fi].area[sp] : 0.0;
                 }
-            // cut-edge areas of species sp on the faces of c
-            for (int i = 0; i < nnb; ++i) {
-                const int fi = cell_face[(size_t)c * 6 + fidx[i]];
-                areas[i] = fi >= 0 ? faces[fi].area[sp] : 0.0;
+            return t;
+        };
+        struct Pair { int src, tgt; double frac; double pos[3]; };
+        std::vector<Pair> pairs;
+        std::vector<int> pair_of(J, -1);          // pair index per source, visible pairs only
+        auto add_pairs = [&](const std::vector<Pair>& ps) {
+            for (const Pair& q : ps) {
+                if (q.src != q.tgt && pair_of[q.src] >= 0) throw std::runtime_error("agglomeration: a cell is mapped to two targets");
+                pair_of[q.src] = (int)pairs.size();
+                pairs.push_back(q);
             }
+        };
+        std::vector<int> need;
+        int failed = 0;
+        // direct pass
+        for (int c = 0; c < J; ++c) {
+            if (!source[c]) continue;
+            int nbs[6];
+            double areas[6];
+            const int nn = neighbours(c, nbs, areas);
             bool nonEmpty = false;
-            for (int i = 0; i < nnb; ++i) if (nbs[i] >= 0 && areas[i] > EmptyEdge) nonEmpty = true;
+            for (int i = 0; i < nn; ++i) if (nbs[i] >= 0 && areas[i] > EmptyEdge) nonEmpty = true;
             if (vol[sp][c] <= 0) nonEmpty = false;
             double fmax = -1.0;
             int tgt = -1;
-            for (int i = 0; i < nnb; ++i) {
+            for (int i = 0; i < nn; ++i) {
                 const int nb = nbs[i];
                 if (nb < 0) continue;
                 if (areas[i] <= EmptyEdge && nonEmpty) continue;
-                if (!((spmask[nb] >> sp) & 1) || source[nb]) continue;   // AggCandidates: species present, not a source
+                if (!cand[nb]) continue;
                 if (frac[sp][nb] > fmax) { fmax = frac[sp][nb]; tgt = nb; }
             }
-            if (tgt < 0) { ++n_failed; continue; }
-            agg_target[sp][c] = tgt;
-            agg_pairs.push_back(c);
-            agg_pairs.push_back(tgt);
+            if (tgt < 0) {
+                bool chainable = false;
+                for (int i = 0; i < nn; ++i) if (nbs[i] >= 0 && source[nbs[i]]) chainable = true;
+                if (chainable) need.push_back(c);
+                else ++failed;
+                continue;
+            }
+            Pair q{c, tgt, fmax, {0, 0, 0}};
+            center(tgt, q.pos);
+            add_pairs({q});
+        }
+        auto remove_need = [&](int c) { need.erase(std::find(need.begin(), need.end(), c)); };
+        // chain pass: connect waiting sources to the final target of a paired neighbour
+        auto chain = [&]() {
+            struct WEdge { double dist, frac; int cell, pair; };
+            auto before = [](const WEdge& a, const WEdge& b) { return a.dist < b.dist || (a.dist == b.dist && a.frac > b.frac); };
+            bool update = true;
+            while (!need.empty() && update) {
+                update = false;
+                std::vector<Pair> loop;
+                std::vector<WEdge> weighted;
+                std::vector<int> immediate;
+                for (int c : need) {
+                    int nbs[6];
+                    double areas[6], cc[3];
+                    const int nn = neighbours(c, nbs, areas);
+                    center(c, cc);
+                    int nsrc = 0, npaired = 0;
+                    std::vector<WEdge> wc;
+                    for (int i = 0; i < nn; ++i) {
+                        const int nb = nbs[i];
+                        if (nb < 0) continue;
+                        if (source[nb]) ++nsrc;
+                        const int pi = pair_of[nb];
+                        if (pi < 0) continue;
+                        double s2 = 0;
+                        for (int d = 0; d < D; ++d) s2 += (pairs[pi].pos[d] - cc[d]) * (pairs[pi].pos[d] - cc[d]);
+                        wc.push_back({std::sqrt(s2), pairs[pi].frac, c, pi});
+                        ++npaired;
+                    }
+                    if (npaired == nsrc && npaired > 0) {
+                        std::stable_sort(wc.begin(), wc.end(), before);
+                        const Pair& tp = pairs[wc[0].pair];
+                        Pair q{c, tp.tgt, tp.frac, {tp.pos[0], tp.pos[1], tp.pos[2]}};
+                        loop.push_back(q);
+                        immediate.push_back(c);
+                    } else {
+                        weighted.insert(weighted.end(), wc.begin(), wc.end());
+                    }
+                }
+                for (int c : immediate) { remove_need(c); update = true; }
+                std::stable_sort(weighted.begin(), weighted.end(), before);
+                if (!weighted.empty() && !need.empty()) {
+                    const Pair& tp = pairs[weighted[0].pair];
+                    Pair q{weighted[0].cell, tp.tgt, tp.frac, {tp.pos[0], tp.pos[1], tp.pos[2]}};
+                    loop.push_back(q);
+                    remove_need(weighted[0].cell);
+                    update = true;
+                }
+                add_pairs(loop);
+            }
+        };
+        // neighbours that join a group: interior faces with a non-empty cut edge,
+        // or all interior faces when every cut edge is empty
+        auto group_neighbours = [&](int c, std::vector<int>& out) {
+            int nbs[6];
+            double areas[6];
+            const int nn = neighbours(c, nbs, areas);
+            bool nonEmpty = false;
+            for (int i = 0; i < nn; ++i) if (nbs[i] >= 0 && areas[i] > EmptyEdge) nonEmpty = true;
+            for (int i = 0; i < nn; ++i)
+                if (nbs[i] >= 0 && (areas[i] > EmptyEdge || !nonEmpty)) out.push_back(nbs[i]);
+        };
+        chain();
+        while (!need.empty()) {
+            int gt = need[0];
+            for (int c : need)
+                if (vol[sp][c] > vol[sp][gt] || (vol[sp][c] == vol[sp][gt] && c < gt)) gt = c;
+            std::vector<int> members{gt}, nbl;
+            group_neighbours(gt, nbl);
+            remove_need(gt);
+            for (;;) {
+                std::vector<int> inter;
+                for (int c : nbl)
+                    if (std::find(inter.begin(), inter.end(), c) == inter.end() &&
+                        std::find(need.begin(), need.end(), c) != need.end()) inter.push_back(c);
+                if (inter.empty()) break;
+                for (int c : inter) {
+                    if (std::find(members.begin(), members.end(), c) == members.end()) {
+                        members.push_back(c);
+                        group_neighbours(c, nbl);
+                    }
+                    remove_need(c);
+                }
+            }
+            std::vector<Pair> gp;
+            for (int c : members) {
+                Pair q{c, gt, vol[sp][gt], {0, 0, 0}};
+                center(gt, q.pos);
+                gp.push_back(q);
+            }
+            add_pairs(gp);
+            chain();
+        }
+        n_failed += failed;
+        for (const Pair& q : pairs) {
+            if (q.src == q.tgt) continue;
+            agg_target[sp][q.src] = q.tgt;
+            agg_pairs.push_back(q.src);
+            agg_pairs.push_back(q.tgt);
             agg_pairs.push_back(sp);
         }
     }
@@ -605,8 +738,13 @@ void XdgProblem::setup() {
         surf[sp].assign(J, 0.0);
         volA[sp] = vol[sp];
         for (int c = 0; c < J; ++c) surf[sp][c] = ((spmask[c] >> sp) & 1) ? iface_area[c] : 0.0;
+        // agglomeration edges: faces between two cells of one group, target and
+        // sources (CellAgglomerator.cs:418-446)
+        std::vector<int> gid(J, -1);
+        for (int c = 0; c < J; ++c)
+            if (agg_target[sp][c] >= 0) { gid[c] = agg_target[sp][c]; gid[agg_target[sp][c]] = agg_target[sp][c]; }
         for (auto& F : faces) {
-            const bool aggEdge = F.b >= 0 && (agg_target[sp][F.a] == F.b || agg_target[sp][F.b] == F.a);
+            const bool aggEdge = F.b >= 0 && gid[F.a] >= 0 && gid[F.a] == gid[F.b];
             if (aggEdge) continue;
             surf[sp][F.a] += F.area[sp];
             if (F.b >= 0) surf[sp][F.b] += F.area[sp];

@@ -269,7 +269,11 @@ int bh_euler_tvd3(bh_euler* e, void* U, double dt, int steps, int prec, int flag
  * DOF layout: cell-major, per cell the present species blocks (A then B),
  * Np modes each; the agglomerated ("reduced") system drops source blocks. */
 typedef struct bh_xdg bh_xdg;
-/* quad_order: Gauss points per 1D piece of the cut-cell rules (<= 0: k + 1) */
+/* quad_order: Gauss points per 1D piece of the cut-cell rules (<= 0: k + 1).
+ * Supported degrees: D = 2, k = 1..3; D = 3, k = 1..2 (other values are refused
+ * before any set-up work).  Agglomeration: direct, chain and group passes
+ * (X/AgglomerationAlgorithm.cs:1699-2330); a source with no source neighbour and
+ * no target fails the call. */
 int bh_xdg_create(bh_grid* grid, int k, const double* center, double radius, double nu_in, double nu_out,
                   double penalty_base, double agg_threshold, int quad_order, bh_xdg** x);
 int bh_xdg_destroy(bh_xdg* x);

@@ -74,6 +74,10 @@ def lib():
             "orc_pmg_set": (None, [_p, _d, _i, _i]),
             "orc_pmg_set_schwarz": (_i, [_p, _i]),
             "orc_pmg_smooth": (None, [_p, _i, _p, _p]),
+            "orc_pmg_level_info": (None, [_p, _i, _ip]),
+            "orc_pmg_level_transfer": (_i, [_p, _i, _ip, _dp]),
+            "orc_pmg_level_boxes": (_i, [_p, _i, _dp, _dp]),
+            "orc_pmg_level_matrix": (_p, [_p, _i]),
             "orc_euler_create": (_p, [_p, _i, _i, _d, _d]),
             "orc_euler_free": (None, [_p]),
             "orc_euler_rhs": (None, [_p, _dp, _dp]),
@@ -325,6 +329,35 @@ class Pmg:
         """Additive Schwarz over the aggregates (no overlap) instead of block-Jacobi on aggregated levels."""
         if lib().orc_pmg_set_schwarz(self.h, 1 if on else 0) != 0:
             raise ArithmeticException_("Schwarz block inversion failed")
+
+    def level_info(self, level):
+        """dict(nrows, bs, kind (0 truncation / 1 aggregation to the next level), deg, n)."""
+        info = np.zeros(7, np.int32)
+        lib().orc_pmg_level_info(self.h, level, info)
+        return dict(nrows=int(info[0]), bs=int(info[1]), kind=int(info[2]), deg=int(info[3]),
+                    n=[int(v) for v in info[4:7]])
+
+    def level_transfer(self, level):
+        """(parent aggregate per cell, blocks P[cell][fine mode][coarse mode]) of an aggregation level."""
+        inf = self.level_info(level)
+        par = np.zeros(inf["nrows"], np.int32)
+        P = np.zeros((inf["nrows"], inf["bs"], inf["bs"]))
+        if lib().orc_pmg_level_transfer(self.h, level, par, P.reshape(-1)) != 0:
+            raise ValueError("level %d is not an aggregation level" % level)
+        return par, P
+
+    def level_boxes(self, level, D):
+        """(lower corners, widths) of the level's cells, J x D each."""
+        inf = self.level_info(level)
+        lo = np.zeros((inf["nrows"], D))
+        h = np.zeros((inf["nrows"], D))
+        if lib().orc_pmg_level_boxes(self.h, level, lo.reshape(-1), h.reshape(-1)) != 0:
+            raise ValueError("level %d carries no cell boxes" % level)
+        return lo, h
+
+    def level_matrix(self, level):
+        """the level matrix as a free-standing oracle Bsr."""
+        return Bsr(handle=lib().orc_pmg_level_matrix(self.h, level))
 
     def __del__(self):
         try:

@@ -1433,6 +1433,31 @@ void orc_pmg_smooth(void* p, int l, const double* r, double* x) {
 int orc_pmg_set_schwarz(void* p, int on) {
     try { pmg_set_schwarz(*(Pmg*)p, on != 0); return 0; } catch (...) { return -1; }
 }
+// level l: info = {block rows, block size, transfer kind to l+1, degree, n0, n1, n2}
+void orc_pmg_level_info(void* p, int l, int* info) {
+    const MgLev& L = ((Pmg*)p)->lv[l];
+    info[0] = L.A.nrows; info[1] = L.A.bs; info[2] = L.kind; info[3] = L.deg;
+    info[4] = L.n[0]; info[5] = L.n[1]; info[6] = L.n[2];
+}
+// aggregation transfer of level l (kind 1): parent aggregate per cell and the
+// blocks P_i[fine mode][coarse mode]; returns 0, or -1 for a truncation level
+int orc_pmg_level_transfer(void* p, int l, int* parent, double* Pb) {
+    const MgLev& L = ((Pmg*)p)->lv[l];
+    if (L.kind != 1 || L.parent.empty()) return -1;
+    std::copy(L.parent.begin(), L.parent.end(), parent);
+    std::copy(L.P.begin(), L.P.end(), Pb);
+    return 0;
+}
+// cell boxes of level l (J x D lower corners and widths); -1 where unknown
+int orc_pmg_level_boxes(void* p, int l, double* lo, double* h) {
+    const MgLev& L = ((Pmg*)p)->lv[l];
+    if (L.lo.empty()) return -1;
+    std::copy(L.lo.begin(), L.lo.end(), lo);
+    std::copy(L.h.begin(), L.h.end(), h);
+    return 0;
+}
+// a copy of the level matrix as a free-standing BSR handle
+void* orc_pmg_level_matrix(void* p, int l) { return new Bsr(((Pmg*)p)->lv[l].A); }
 // one V-cycle from a zero guess: x = V(b)
 int orc_pmg_vcycle(void* mp, void* gp, const int* deg, int ndeg, int min_coarse, double omega, int pre, int post,
                    const double* b, double* x, int* nlevels) {

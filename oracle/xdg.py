@@ -8,8 +8,8 @@ Three independent parts:
   adaptive 1D quadrature (scipy.integrate.quad) for the 3D volume and surface
   pieces -- this pins the cut-cell volumes, face areas and interface areas of
   the product's dimension-reduction rules without sharing any code with them;
-* the agglomeration of X/AgglomerationAlgorithm.cs (sources :393-460,
-  direct targets SearchDirectAgglomeration_Mk3 :2064-2200) and the length
+* the agglomeration of X/AgglomerationAlgorithm.cs (sources :393-460, direct
+  targets :2064-2200, chains :1815-2055, groups :2232-2330) and the length
   scales of X/MultiphaseAgglomerator.cs:870-1030, from those volumes/areas;
 * the bilinear forms XLaplace_Bulk (SIPLaplace with species diffusion and
   species length scales) and XLaplace_Interface in SIP mode
@@ -244,57 +244,194 @@ class Geometry:
         hi = np.array([self.nodes[d][idx[d] + 1] for d in range(self.D)])
         return lo, hi
 
+    def center(self, j):
+        lo, hi = self.box(j)
+        return 0.5 * (lo + hi)
+
+    def _neighbours(self, j, s):
+        """(neighbour or -1, cut-edge area of species s) per face of cell j, in the
+        order per direction: lower face, then upper face (no periodic faces here)."""
+        idx = self.ijk(j)
+        out = []
+        for d in range(self.D):
+            for side in (0, 1):
+                nb = list(idx)
+                nb[d] += 1 if side else -1
+                jn = self.index(*nb) if 0 <= nb[d] < self.n[d] else -1
+                fi = self.cell_face[j, 2 * d + side]
+                out.append((jn, self.faces[fi][4][s]))
+        return out
+
     def agglomerate(self, threshold):
-        """Per species: sources (cut cells, fraction <= threshold, AgglomerationAlgorithm.cs:437-460)
-        and direct targets (SearchDirectAgglomeration_Mk3, :2095-2200).  Returns the pairs
-        (source, target, species) in discovery order, the per-species target arrays and the
-        sources that found no target."""
+        """Agglomeration map per species, X/AgglomerationAlgorithm.cs restated for one
+        MPI rank and a static interface:
+
+        * sources: cut cells of the species with clipped volume fraction <= threshold
+          (FindAgglomerationSources, :393-460); candidates: cells where the species is
+          present and which are no source ("make sure source cells cannot be target
+          cells", FindAgglomerationTargets_Mk3, :1721-1726);
+        * direct pass (SearchDirectAgglomeration_Mk3, :2064-2200): the face neighbour
+          among the candidates with the largest raw volume fraction (strict >, face
+          order), skipping empty cut edges when a non-empty one exists; a source
+          without a target goes to the chain list when a neighbour is a source, else
+          it fails;
+        * chain pass (SearchChainAgglomeration_Mk3, :1815-2055): repeatedly connect a
+          waiting source to the final target of a paired neighbour source, directly
+          when all its neighbour sources are paired (nearest target, then largest
+          target fraction), else one connection per sweep from the globally best
+          edge (distance ascending, fraction descending, stable order);
+        * group pass (SearchGroupAgglomeration_Mk3, :2232-2330, global mapping): the
+          waiting source with the largest species volume (then the lowest index)
+          becomes a target and takes every waiting source connected to it through
+          non-empty edges (AgglomerationGroup.AddNeighbors, :72-111); then the chain
+          pass runs again; the self pairs are removed at the end.
+
+        Returns the pairs (source, target, species) in the order the passes find
+        them, the per-species target arrays and the (source, species) that found no
+        target."""
         D, J = self.D, self.J
-        pairs, failed = [], []
+        pairs_out, failed_out = [], []
         target = -np.ones((2, J), np.int64)
         for s in range(2):
-            frac = np.clip(np.maximum(self.vol[s], 0.0) / self.cellvol, 0.0, 1.0)
+            vol = self.vol[s]
             present = ((self.spmask >> s) & 1) == 1
-            sources = [j for j in range(J) if self.cut[j] and present[j] and frac[j] <= threshold]
+            fsrc = np.clip(np.maximum(vol, 0.0) / self.cellvol, 0.0, 1.0)
+            is_src = self.cut & present & (fsrc <= threshold)
+            cand = present & ~is_src
+            sources = [j for j in range(J) if is_src[j]]
+            pairs = []        # [source, target, fracTarget, posTarget]
+            need, failed = [], []
+            # direct pass
             for j in sources:
-                idx = self.ijk(j)
-                nbs = []
-                for d in range(D):
-                    for side in (0, 1):
-                        nb = list(idx)
-                        nb[d] += 1 if side else -1
-                        jn = self.index(*nb) if 0 <= nb[d] < self.n[d] else -1
-                        fi = self.cell_face[j, 2 * d + side]
-                        nbs.append((jn, self.faces[fi][4][s]))
+                nbs = self._neighbours(j, s)
                 non_empty = any(jn >= 0 and a > EMPTY_EDGE for jn, a in nbs)
-                if self.vol[s, j] <= 0:
+                if vol[j] <= 0:
                     non_empty = False
                 best, tgt = -1.0, -1
                 for jn, a in nbs:
                     if jn < 0 or (a <= EMPTY_EDGE and non_empty):
                         continue
-                    if not present[jn]:      # AggCandidates = species mask
+                    if not cand[jn]:
                         continue
-                    if frac[jn] > best:
-                        best, tgt = frac[jn], jn
+                    f = vol[jn] / self.cellvol[jn]
+                    if f > best:
+                        best, tgt = f, jn
                 if tgt < 0:
-                    failed.append((j, s))
-                    continue
-                target[s, j] = tgt
-                pairs.append((j, tgt, s))
-        return np.array(pairs, np.int64).reshape(-1, 3), target, failed
+                    if any(jn >= 0 and is_src[jn] for jn, _ in nbs):
+                        need.append(j)
+                    else:
+                        failed.append(j)
+                else:
+                    pairs.append([j, tgt, best, self.center(tgt)])
+
+            def pair_of(src):
+                hits = [p for p in pairs if p[0] == src]
+                assert len(hits) <= 1, "a cell is mapped to two different target cells"
+                return hits[0] if hits else None
+
+            def chain():
+                update = True
+                while need and update:
+                    update = False
+                    loop_pairs, weighted, immediate = [], [], []
+                    for j in need:
+                        nsrc = npaired = 0
+                        wj = []
+                        cj = self.center(j)
+                        for jn, a in self._neighbours(j, s):
+                            if jn < 0:
+                                continue
+                            if is_src[jn]:
+                                nsrc += 1
+                            tp = pair_of(jn)
+                            if tp is not None:
+                                dist = math.sqrt(float(((tp[3] - cj) ** 2).sum()))
+                                wj.append((dist, tp[2], j, jn, tp))
+                                npaired += 1
+                        if npaired == nsrc and npaired > 0:
+                            e = sorted(wj, key=lambda t: (t[0], -t[1]))[0]
+                            loop_pairs.append([j, e[4][1], e[4][2], e[4][3]])
+                            immediate.append(j)
+                        else:
+                            weighted.extend(wj)
+                    for j in immediate:
+                        need.remove(j)
+                        update = True
+                    weighted.sort(key=lambda t: (t[0], -t[1]))
+                    if weighted and need:
+                        e = weighted[0]
+                        loop_pairs.append([e[2], e[4][1], e[4][2], e[4][3]])
+                        need.remove(e[2])
+                        update = True
+                    pairs.extend(loop_pairs)
+
+            def group_neighbours(j):
+                nbs = self._neighbours(j, s)
+                non_empty = any(jn >= 0 and a > EMPTY_EDGE for jn, a in nbs)
+                return [jn for jn, a in nbs if jn >= 0 and (a > EMPTY_EDGE or not non_empty)]
+
+            chain()
+            while need:
+                gt = min(need, key=lambda p: (-vol[p], p))
+                members = [gt]
+                nbl = group_neighbours(gt)
+                need.remove(gt)
+                while True:
+                    seen = set()
+                    inter = []
+                    for c in nbl:
+                        if c in seen:
+                            continue
+                        seen.add(c)
+                        if c in need:
+                            inter.append(c)
+                    if not inter:
+                        break
+                    for c in inter:
+                        if c not in members:
+                            members.append(c)
+                            nbl = nbl + group_neighbours(c)
+                        need.remove(c)
+                pos = self.center(gt)
+                pairs.extend([[c, gt, vol[gt], pos] for c in members])
+                chain()
+            failed.extend(need)
+            pairs = [p for p in pairs if p[0] != p[1]]
+            for p in pairs:
+                target[s, p[0]] = p[1]
+                pairs_out.append((p[0], p[1], s))
+            failed_out.extend((j, s) for j in failed)
+        return np.array(pairs_out, np.int64).reshape(-1, 3), target, failed_out
+
+    def agg_groups(self, target, s):
+        """cells of each agglomerated group of species s: target -> [target, sources...]."""
+        groups = {}
+        for j in range(self.J):
+            t = target[s, j]
+            if t >= 0:
+                groups.setdefault(int(t), [int(t)]).append(j)
+        return groups
 
     def length_scales(self, target):
         """Agglomerated species volume / (interface area + cut-edge areas without the
-        agglomeration edges) (MultiphaseAgglomerator.cs:870-1030); direct pairs only."""
+        agglomeration edges) (MultiphaseAgglomerator.cs:870-1030).  Agglomeration
+        edges are the faces between two cells of one agglomerated group, target and
+        sources (CellAgglomerator.cs:418-446); sources add their volume and surface
+        to their target, which hands the sums back to them."""
         J = self.J
         out = np.zeros((2, J))
         for s in range(2):
             present = ((self.spmask >> s) & 1) == 1
+            gid = -np.ones(J, np.int64)
+            for j in range(J):
+                t = target[s, j]
+                if t >= 0:
+                    gid[j] = t
+                    gid[t] = t
             surf = np.where(present, self.iface, 0.0)
             vol = self.vol[s].copy()
             for (a, b, d, side, area) in self.faces:
-                if b >= 0 and (target[s, a] == b or target[s, b] == a):
+                if b >= 0 and gid[a] >= 0 and gid[a] == gid[b]:
                     continue
                 surf[a] += area[s]
                 if b >= 0:

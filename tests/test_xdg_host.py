@@ -101,30 +101,123 @@ def test_cut_rule_moments():
 
 
 # ---------------------------------------------------------------- agglomeration
-@pytest.mark.parametrize("D,n,c,r,thr", [(3, 12, [0.5, 0.47, 0.52], 0.27, 0.1), (3, 16, [0.43, 0.55, 0.51], 0.22, 0.1),
-                                          (2, 24, [0.52, 0.47], 0.31, 0.1), (2, 24, [0.52, 0.47], 0.31, 0.3)])
-def test_agglomeration_bit_exact(D, n, c, r, thr):
-    """Agglomeration pairs (source, target, species) equal the oracle's, which
-    derives them from closed-form volumes and face areas (integer work: exact)."""
-    x, g, X = _product(D, n, c, r, 6, agg=thr)
+def _check_agglomeration(G, got, target, thr):
+    """Invariants of the map (X/AgglomerationAlgorithm.cs, CellAgglomerator.cs): every
+    source is a cut cell of its species with clipped fraction <= thr and appears once;
+    no target is a source (chains end at the final target); a target whose fraction is
+    <= thr is a group target (its group holds only sources); every group (target and
+    its sources) is face-connected; direct pairs (face neighbours) have a target with
+    fraction > thr."""
+    frac = np.clip(G.vol / G.cellvol, 0.0, 1.0)
+    for sp in range(2):
+        mine = got[got[:, 2] == sp]
+        srcs = set(mine[:, 0].tolist())
+        assert len(srcs) == len(mine)
+        for (s_, t_, _) in mine:
+            assert G.cut[s_] and frac[sp, s_] <= thr
+            assert t_ not in srcs
+        for t_, cells in G.agg_groups(target, sp).items():
+            seen, todo = {t_}, [t_]
+            while todo:
+                c = todo.pop()
+                idx = np.array(G.ijk(c))
+                for o in cells:
+                    if o not in seen and np.abs(np.array(G.ijk(o)) - idx).sum() == 1:
+                        seen.add(o)
+                        todo.append(o)
+            assert seen == set(cells)
+            if frac[sp, t_] <= thr:
+                assert all(G.cut[c] for c in cells)
+        for (s_, t_, _) in mine:
+            if np.abs(np.array(G.ijk(s_)) - np.array(G.ijk(t_))).sum() == 1 and len(G.agg_groups(target, sp)[t_]) == 2:
+                assert frac[sp, t_] > thr or G.cut[t_]
+
+
+def _compare(D, n, c, r, thr, q=6, expect_nonadjacent=None, len_rtol=1e-7):
+    x, g, X = _product(D, n, c, r, q, agg=thr)
     G = OX.Geometry([x] * D, c, r)
     pairs, target, failed = G.agglomerate(thr)
     assert not failed
     got = X.agglomeration()
     assert np.array_equal(got, pairs)
-    # invariants: sources are cut with fraction <= thr, targets are face neighbours with fraction > thr
-    frac = G.vol / G.cellvol
-    for (s_, t_, sp) in got:
-        assert G.cut[s_] and frac[sp, s_] <= thr
-        assert frac[sp, t_] > thr
-        d = np.abs(np.array(G.ijk(s_)) - np.array(G.ijk(t_)))
-        assert d.sum() == 1
-    # length scales from the oracle's own areas
+    _check_agglomeration(G, got, target, thr)
+    nonadj = sum(1 for (s_, t_, sp) in got if np.abs(np.array(G.ijk(s_)) - np.array(G.ijk(t_))).sum() != 1)
+    if expect_nonadjacent is not None:
+        assert (nonadj > 0) == expect_nonadjacent
+    # length scales from the oracle's own areas (agglomeration edges: faces inside a group)
     cp = X.cells()
     L = G.length_scales(target)
     ok = L > 0
-    assert np.allclose(cp["len"][ok], L[ok], rtol=1e-7)
+    assert np.allclose(cp["len"][ok], L[ok], rtol=len_rtol)
     assert np.all(cp["len"][~ok] == 0)
+    return got, G
+
+
+@pytest.mark.parametrize("D,n,c,r,thr", [(3, 12, [0.5, 0.47, 0.52], 0.27, 0.1), (3, 16, [0.43, 0.55, 0.51], 0.22, 0.1),
+                                          (2, 24, [0.52, 0.47], 0.31, 0.1), (2, 24, [0.52, 0.47], 0.31, 0.3)])
+def test_agglomeration_bit_exact(D, n, c, r, thr):
+    """Agglomeration pairs (source, target, species) equal the oracle's, which
+    derives them from closed-form volumes and face areas (integer work: exact)."""
+    _compare(D, n, c, r, thr)
+
+
+@pytest.mark.parametrize("seed,n,thr", [(1, 10, 0.3), (2, 10, 0.3), (3, 12, 0.4), (1, 12, 0.4)])
+def test_agglomeration_chains_bit_exact(seed, n, thr):
+    """Spheres on coarse grids with a high threshold: sources whose every candidate
+    neighbour is a source, connected through chains (SearchChainAgglomeration_Mk3,
+    AgglomerationAlgorithm.cs:1815-2055) to the final target of a paired neighbour."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.4, 0.6, 3)
+    r = float(rng.uniform(0.2, 0.3))
+    _compare(3, n, c, r, thr, q=10, expect_nonadjacent=True)
+
+
+@pytest.mark.parametrize("D,n,c,r,members", [(2, 10, [0.503, 0.502], 0.025, 4), (3, 10, [0.503, 0.502, 0.5015], 0.04, 8),
+                                              (3, 8, [0.502, 0.5035, 0.5625], 0.05, 4)])
+def test_agglomeration_group(D, n, c, r, members):
+    """A small ball near a grid vertex (or edge): its 2^D (or 4) cut cells are all
+    sources of the inside species and no candidate touches them, so no direct or
+    chain target exists; the group pass (SearchGroupAgglomeration_Mk3,
+    AgglomerationAlgorithm.cs:2232-2330) makes the cell with the largest inside
+    volume the target of the others (the centre is off the vertex so that the
+    volumes differ well above the rules' accuracy)."""
+    got, G = _compare(D, n, c, r, 0.1, q=10)
+    cut = np.flatnonzero(G.cut)
+    assert len(cut) == members
+    inside = got[got[:, 2] == 0]
+    assert len(inside) == members - 1
+    big = int(cut[np.argmax(G.vol[0][cut])])
+    assert set(inside[:, 1].tolist()) == {big}
+    assert set(inside[:, 0].tolist()) == set(cut.tolist()) - {big}
+
+
+def test_agglomeration_oracle_rule_sources_are_no_targets():
+    """Hand-built: a 2D circle on the face between two cells, inside fractions 0.043
+    and 0.083 (both sources) and nothing else inside.  The larger source is no
+    candidate ("make sure source cells cannot be target cells",
+    AgglomerationAlgorithm.cs:1721-1726), so neither finds a direct target and the
+    group pass makes the larger the target of the smaller.  Without the rule each
+    would take the other as its direct target (a cycle)."""
+    x = np.linspace(0.0, 1.0, 11)
+    c, r = [0.605, 0.55], 0.02
+    G = OX.Geometry([x, x], c, r)
+    frac = G.vol[0] / G.cellvol
+    cut = np.flatnonzero(G.cut)
+    assert len(cut) == 2 and np.all(frac[cut] <= 0.1) and frac[cut[0]] != frac[cut[1]]
+    pairs, target, failed = G.agglomerate(0.1)
+    big, small = (cut[0], cut[1]) if frac[cut[0]] > frac[cut[1]] else (cut[1], cut[0])
+    assert pairs[pairs[:, 2] == 0].tolist() == [[small, big, 0]]
+    _compare(2, 10, c, r, 0.1, q=10)
+
+
+def test_agglomeration_c3_sphere_48():
+    """The seeded 48^3 sphere of the XDG benchmark workload (k = 2, threshold 0.1):
+    pairs bit-exact with the oracle."""
+    rng = np.random.default_rng(3)
+    c = rng.uniform(0.4, 0.6, 3)
+    r = float(rng.uniform(0.2, 0.3))
+    got, G = _compare(3, 48, c, r, 0.1, q=3, len_rtol=1e-3)   # the workload default rule (k + 1 points): 1e-4 accurate
+    assert len(got) > 1000
 
 
 def test_dof_maps():
