@@ -80,6 +80,7 @@ SIGNATURES = {
     "bh_mlp_set_params": (_c_i, [_c_p, _c_p]),
     "bh_mlp_forward": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_i]),
     "bh_mlp_gradient": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, ctypes.POINTER(_c_d), _c_i]),
+    "bh_mlp_gradient_noalloc": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_p]),
     "bh_mlp_adam_step": (_c_i, [_c_p, _c_f]),
     "bh_mlp_features": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
     "bh_mlp_targets": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
@@ -574,6 +575,13 @@ class CellMLP(_Owned):
             T = np.ascontiguousarray(T, np.float32)
             _check(lib().bh_mlp_gradient(self.h, X.shape[0], X.ctypes.data, T.ctypes.data, _ptr(g), ctypes.byref(loss), 0))
         return loss.value, g
+
+    def gradient_noalloc(self, X, T, grad=None, loss=None):
+        """bh_mlp_gradient_noalloc: device arrays only; grad (n_params FP32) and
+        loss (one FP64) are optional device outputs.  Stream-ordered, no sync."""
+        n = X.n // self.in_p
+        _check(lib().bh_mlp_gradient_noalloc(self.h, n, X.ptr, T.ptr, grad.ptr if grad is not None else None,
+                                             loss.ptr if loss is not None else None))
 
     def adam_step(self, lr):
         _check(lib().bh_mlp_adam_step(self.h, lr))

@@ -180,6 +180,7 @@ int bh_init(int device, bh_ctx** out) {
     BH_HIP(hipGetDeviceCount(&n));
     BH_CHECK(device >= 0 && device < n, "bh_init: no such HIP device");
     BH_HIP(hipSetDevice(device));
+    mlp_noalloc_setup(device);   // device address of the allocation-free workspace on this GPU
     bh_ctx* c = new bh_ctx();
     c->device = device;
     BH_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -701,6 +702,14 @@ int bh_mlp_gradient(bh_mlp* w, int n, const float* X, const float* T, float* gra
     const double l = m->gradient(n, nullptr, (const float*)sx.dptr, (const float*)st.dptr);
     if (loss) *loss = l;
     if (grad) m->ctx->download(grad, m->grad.p, m->nw * sizeof(float));
+    BH_API_END
+}
+int bh_mlp_gradient_noalloc(bh_mlp* w, int n, const float* X, const float* T, float* grad, double* loss) {
+    BH_API_BEGIN
+    BH_CHECK(w && w->m, "bh_mlp_gradient_noalloc: null network");
+    BH_CHECK(n >= 1, "bh_mlp_gradient_noalloc: empty batch");
+    BH_CHECK(X && T, "bh_mlp_gradient_noalloc: X and T must be device pointers");
+    w->m->gradient_noalloc(n, X, T, grad, loss);
     BH_API_END
 }
 int bh_mlp_adam_step(bh_mlp* w, float lr) {

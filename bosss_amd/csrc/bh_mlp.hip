@@ -12,8 +12,11 @@
 //   face neighbours (boundary: +2) , 0-padding ]
 // Output y_j maps to the correction e_j = s_j * (h_j^2 / nu_j) * y_j.
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "bh_internal.hpp"
+
 #include "bh_mlp.hpp"
 
 namespace bh {
@@ -405,27 +408,94 @@ void Mlp::infer(int J, const float* X, float* Y) {
     throw Error(-3, "MLP: unsupported widths");
 }
 
-double Mlp::gradient(int B, const int* idx, const float* X, const float* Tg) {
-    int nb = B < 256 ? B : 256;
-    if (nb < 1) nb = 1;
-    if (part.bytes < (size_t)nb * nw * sizeof(float)) part.alloc((size_t)256 * nw * sizeof(float));
-    if (lossp.bytes < 256 * sizeof(float)) lossp.alloc(256 * sizeof(float));
+// One training pass: mlp_train_k writes every entry of the nb per-block gradient
+// partials and the nb loss partials, and mlp_reduce_k sums the partials in block
+// order into grad (no atomics, so results do not depend on scheduling).
+void Mlp::launch_gradient(int B, int nb, const int* idx, const float* X, const float* Tg, float* part, float* lossp) {
     const float* P = params.as<float>();
 #define BH_CASE(I, O)                                                                                                 \
     if (in_p == I && out_p == O) {                                                                                    \
         hipLaunchKernelGGL((mlp_train_k<I, 128, O>), dim3(nb), dim3(128), 0, ctx->stream, B, idx, X, Tg, out, P + off_W1(), \
-                           P + off_b1(), P + off_W2(), P + off_b2(), P + off_W3(), P + off_b3(), part.as<float>(),   \
-                           lossp.as<float>());                                                                        \
+                           P + off_b1(), P + off_W2(), P + off_b2(), P + off_W3(), P + off_b3(), part, lossp);        \
     } else
     BH_CASE(16, 16) BH_CASE(16, 32) BH_CASE(32, 16) BH_CASE(32, 32) { throw Error(-3, "MLP: unsupported widths"); }
 #undef BH_CASE
     BH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(mlp_reduce_k, dim3((nw + 255) / 256), dim3(256), 0, ctx->stream, nb, nw, part.as<float>(), grad.as<float>());
+    hipLaunchKernelGGL(mlp_reduce_k, dim3((nw + 255) / 256), dim3(256), 0, ctx->stream, nb, nw, part, grad.as<float>());
+    BH_HIP(hipGetLastError());
+}
+
+double Mlp::gradient(int B, const int* idx, const float* X, const float* Tg) {
+    const int nb = gradient_blocks(B);
+    if (part.bytes < (size_t)nb * nw * sizeof(float)) part.alloc((size_t)256 * nw * sizeof(float));
+    if (lossp.bytes < 256 * sizeof(float)) lossp.alloc(256 * sizeof(float));
+    launch_gradient(B, nb, idx, X, Tg, part.as<float>(), lossp.as<float>());
     std::vector<float> lp(nb);
     ctx->download(lp.data(), lossp.p, nb * sizeof(float));
     double l = 0;
     for (float v : lp) l += v;
     return l / ((double)B * out);
+}
+
+// ------------------------------------------------------------------ allocation-free gradient
+// Fixed workspace of gradient_noalloc: the per-block gradient partials (at most
+// 256 blocks of the largest parameter count the widths allow, in_p = out_p = 32)
+// followed by the per-block loss partials.  One copy per GPU, 25.5 MB, held until
+// the process ends.  In host code `scratch` is only a stand-in for the symbol:
+// the kernels receive the device address that hipGetSymbolAddress returns for the
+// current GPU (noalloc_ptr), never the array's name.
+constexpr size_t kNoallocBlocks = 256;
+constexpr size_t kNoallocMaxParams = 128 * 32 + 128 + 128 * 128 + 128 + 32 * 128 + 32;
+constexpr size_t kNoallocFloats = kNoallocBlocks * kNoallocMaxParams + kNoallocBlocks;
+static __device__ __align__(256) float scratch[kNoallocFloats];
+
+constexpr int kMaxDevices = 64;
+static float* noalloc_ptr[kMaxDevices] = {};
+static std::mutex noalloc_mu;
+
+size_t mlp_noalloc_capacity() { return kNoallocFloats; }
+
+// Called once per context from bh_init, with the context's device current.
+void mlp_noalloc_setup(int device) {
+    BH_CHECK(device >= 0 && device < kMaxDevices, "bh_init: device index beyond the allocation-free workspace table");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_ptr[device]) return;
+    void* p = nullptr;
+    BH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BH_CHECK(p != nullptr, "bh_init: no device address for the allocation-free workspace");
+    noalloc_ptr[device] = (float*)p;
+}
+
+__global__ void mlp_loss_k(int nb, const float* __restrict__ lossp, double denom, double* __restrict__ loss) {
+    // the same additions, in the same order, as the host sum of gradient()
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double l = 0;
+        for (int b = 0; b < nb; ++b) l += lossp[b];
+        *loss = l / denom;
+    }
+}
+
+void Mlp::gradient_noalloc(int B, const float* X, const float* Tg, float* grad_out, double* loss_dev) {
+    const int nb = gradient_blocks(B);
+    const size_t need = (size_t)nb * nw + (size_t)nb;
+    BH_CHECK(need <= kNoallocFloats, "bh_mlp_gradient_noalloc: " + std::to_string(nb) + " blocks x " + std::to_string(nw) +
+                                         " parameters exceed the fixed workspace of " + std::to_string(kNoallocFloats) +
+                                         " floats");
+    float* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(noalloc_mu);
+        base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? noalloc_ptr[ctx->device] : nullptr;
+    }
+    BH_CHECK(base != nullptr, "bh_mlp_gradient_noalloc: the workspace of this GPU is not set up (bh_init)");
+    float* part = base;
+    float* lossp = base + (size_t)nb * nw;
+    launch_gradient(B, nb, nullptr, X, Tg, part, lossp);
+    if (grad_out)
+        BH_HIP(hipMemcpyAsync(grad_out, grad.p, (size_t)nw * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+    if (loss_dev) {
+        hipLaunchKernelGGL(mlp_loss_k, dim3(1), dim3(64), 0, ctx->stream, nb, lossp, (double)B * out, loss_dev);
+        BH_HIP(hipGetLastError());
+    }
 }
 
 void Mlp::adam(float lr) {

@@ -201,7 +201,19 @@ int bh_mlp_forward(bh_mlp* mlp, int ncells, const float* X, float* Y, int flags)
 /* mean-squared-error loss over the first n_out outputs and its gradient
  * (FP32, weights rounded to BF16 as in inference); grad may be NULL */
 int bh_mlp_gradient(bh_mlp* mlp, int ncells, const float* X, const float* T, float* grad, double* loss, int flags);
-/* one Adam step with the gradient of the last bh_mlp_gradient call */
+/* No counterpart in the reference.  Allocation-free variant of bh_mlp_gradient,
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight training loops).  X [ncells][in_p], T [ncells][out_p], grad [n_params]
+ * and loss (one double) are device pointers; grad and loss may be NULL.  The
+ * gradient also lands in the network's own buffer for bh_mlp_adam_step.  The
+ * per-block partials live in a fixed workspace of 25.5 MB per GPU (256 blocks x
+ * 24,864 parameters, the largest network the widths allow) that the library
+ * holds until the process ends; a network that does not fit is rejected.  No
+ * allocation, no memset and no host synchronisation: results are stream-ordered
+ * on the context's stream.  Every call on one GPU shares that workspace, so calls
+ * that overlap on one GPU (several contexts or host threads) race. */
+int bh_mlp_gradient_noalloc(bh_mlp* mlp, int ncells, const float* X, const float* T, float* grad, double* loss);
+/* one Adam step with the gradient of the last bh_mlp_gradient(_noalloc) call */
 int bh_mlp_adam_step(bh_mlp* mlp, float lr);
 /* feature map of a residual r (N values, FP64) on op's grid -> X[J][in_p], scales s[J];
  * target map of an error e -> T[J][out_p] with the same scales */
