@@ -465,8 +465,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-xdg", action="store_true")
     ap.add_argument("--no-euler", action="store_true")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="also run a plain 256 MiB device copy (torch) as the reference kernel for HBM counters")
+    ap.add_argument("--copy-probe", action="store_true",  # kept for old command lines: the probe always runs
+                    help="no effect: the timed 1 GiB device copy (copy16_k) always runs")
     ap.add_argument("--no-graphs", action="store_true",
                     help="eager kernel launches in the multigrid (for kernel-trace profilers that cannot follow graphs)")
     args = ap.parse_args()
@@ -509,11 +509,15 @@ def main():
     hbm_bytes = 2 * N * 8 + grid.J * 8
     # the FP64 apply picks its kernel (vector FMA, node by node or modal, or FP64 MFMA) by timing on first use
     f64_path, f64_w, f64_x, f64_ms_v, f64_ms_m = op.fp64_tuning()
-    modal = f64_path == "modal"
+    modal = f64_path in ("modal", "tiled")
     flops = sip_flops_modal(n, D, k) if modal else sip_flops(n, D, k)
     achieved = flops / (kern_ms * 1e-3)
-    kname = (f"sip_apply_mfma64_kernel<{D}, {k}, {f64_w}>" if f64_path == "mfma"
-             else f"sip_apply_kernel<{D}, {k}, double, {'true' if modal else 'false'}>")
+    if f64_path == "mfma":
+        kname = f"sip_apply_mfma64_kernel<{D}, {k}, {f64_w}>"
+    elif f64_path == "tiled":
+        kname = f"sip_apply_tile_kernel<{D}, {k}, double, {f64_w}>"
+    else:
+        kname = f"sip_apply_kernel<{D}, {k}, double, {'true' if modal else 'false'}>"
     if modal:
         # the modal form needs 1/8 of the quadrature's FLOPs: HBM is the bound
         gbps = hbm_bytes / (kern_ms * 1e-3) / 1e9
@@ -548,16 +552,27 @@ def main():
         "fp64_autotune": {"chosen": f64_path, "vector_ms": f64_ms_v, "mfma64_ms": f64_ms_m,
                           "mfma64_waves_per_workgroup": f64_w, "mfma64_workgroups_per_xcd": f64_x},
     }
-    if args.copy_probe:
-        # known-size copy kernel (copy16_k): FETCH_SIZE / WRITE_SIZE of it calibrate the counters
-        nbytes = 256 * 1024 * 1024
-        src = B.DeviceArray.from_host(ctx, np.ones(nbytes // 4, np.float32))
-        dst = B.DeviceArray(ctx, nbytes // 4, np.float32)
-        for _ in range(3):
-            dst.copy_from(src)
-        ctx.sync()
-        out["copy_probe"] = {"kernel": "copy16_k", "bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}
-        del src, dst
+    # known-size copy kernel (copy16_k), timed: the HBM rate a plain streaming kernel
+    # reaches on this GPU (read + write bytes per second), beside the 8 TB/s peak; its
+    # FETCH_SIZE / WRITE_SIZE also calibrate the rocprofv3 traffic counters
+    nbytes = 1024 * 1024 * 1024
+    src = B.DeviceArray.from_host(ctx, np.ones(nbytes // 4, np.float32))
+    dst = B.DeviceArray(ctx, nbytes // 4, np.float32)
+    dst.copy_from(src)
+    ctx.sync()
+    reps = 10
+    ctx.record(2)
+    for _ in range(reps):
+        dst.copy_from(src)
+    ctx.record(3)
+    ctx.sync()
+    cms = ctx.elapsed_ms(2, 3) / reps
+    out["copy_probe"] = {"kernel": "copy16_k", "bytes_read": nbytes, "bytes_written": nbytes, "launches": reps + 1,
+                         "ms": cms, "GBps": 2 * nbytes / (cms * 1e-3) / 1e9,
+                         "frac_of_peak": 2 * nbytes / (cms * 1e-3) / PEAK_HBM}
+    out["roofline"]["frac_of_measured_copy"] = out["roofline"].get("achieved", 0) * 1e9 / (2 * nbytes / (cms * 1e-3)) \
+        if out["roofline"]["unit"] == "GB/s" else None
+    del src, dst
     # reduced-precision applies of the same operator (preconditioner use): FP32
     # vectors; vector FP32 FMA vs FP16 MFMA contractions with FP32 accumulation
     u64 = u.download()

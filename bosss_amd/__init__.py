@@ -64,6 +64,7 @@ SIGNATURES = {
     "bh_op_assemble": (_c_i, [_c_p, _PP, _c_p, _c_i]),
     "bh_op_tuning": (_c_i, [_c_p, _c_p, _c_p]),
     "bh_op_set_fp64_path": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
+    "bh_op_set_fp32_path": (_c_i, [_c_p, _c_i, _c_i]),
     "bh_bsr_create": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p, _PP]),
     "bh_bsr_destroy": (_c_i, [_c_p]),
     "bh_bsr_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i64)]),
@@ -442,14 +443,20 @@ class SipOperator(_Owned):
         cfg = np.zeros(5, np.int32)
         ms = (_c_f * 3)()
         _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ms))
-        path = {0: None, 1: "vector", 2: "mfma", 3: "modal"}[int(cfg[2])]
+        path = {0: None, 1: "vector", 2: "mfma", 3: "modal", 4: "tiled"}[int(cfg[2])]
         return path, int(cfg[3]), int(cfg[4]), ms[1], ms[2]
 
     def force_fp64_path(self, path, waves=8, wg_per_xcd=64):
         """Pin the FP64 apply to 'vector' (face rule node by node), 'modal' (the vector
-        kernel with modal face terms) or 'mfma'; tests compare each against the oracle."""
-        code = {"vector": 1, "mfma": 2, "modal": 3}[path]
+        kernel with modal face terms), 'tiled' (modal, traces exchanged; waves = 2, 4
+        or 8 per workgroup) or 'mfma'; tests compare each against the oracle."""
+        code = {"vector": 1, "mfma": 2, "modal": 3, "tiled": 4}[path]
         _check(lib().bh_op_set_fp64_path(self.h, code, int(waves), int(wg_per_xcd)))
+
+    def force_fp32_path(self, path, waves=4):
+        """Pin the FP32 apply to 'vector', 'modal' or 'tiled' (waves = 2, 4 or 8)."""
+        code = {"vector": 1, "modal": 3, "tiled": 4}[path]
+        _check(lib().bh_op_set_fp32_path(self.h, code, int(waves)))
 
     def ComputeMatrix(self):
         """Assemble on the device -> (BlockMsrMatrix, affine vector)."""

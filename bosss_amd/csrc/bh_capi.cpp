@@ -437,7 +437,7 @@ int bh_op_tuning(bh_op* op, int* cfg, float* ms) {
         cfg[0] = op->mfma_waves;
         cfg[1] = op->mfma_wg_per_xcd;
         cfg[2] = op->f64_path;
-        cfg[3] = op->f64_waves;
+        cfg[3] = op->f64_path == 4 ? op->f64_tile_w : op->f64_waves;
         cfg[4] = op->f64_wg_per_xcd;
     }
     if (ms) { ms[0] = op->mfma_tuned_ms; ms[1] = op->f64_ms_vector; ms[2] = op->f64_ms_mfma; }
@@ -447,9 +447,9 @@ int bh_op_tuning(bh_op* op, int* cfg, float* ms) {
 int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
     BH_API_BEGIN
     BH_CHECK(op, "bh_op_set_fp64_path: null operator");
-    BH_CHECK(path >= 0 && path <= 3,
-             "bh_op_set_fp64_path: path must be 0 (autotune), 1 (vector, face rule node by node), 2 (MFMA) or 3 "
-             "(vector, modal face terms)");
+    BH_CHECK(path >= 0 && path <= 4,
+             "bh_op_set_fp64_path: path must be 0 (autotune), 1 (vector, face rule node by node), 2 (MFMA), 3 "
+             "(vector, modal face terms) or 4 (vector, tiled modal face terms)");
     if (path == 2) {
         BH_CHECK(bh::sip_apply_mfma64_supported(op->D, op->k, op->N),
                  "bh_op_set_fp64_path: the FP64-MFMA apply supports D in {2, 3}, k in {2, 3} and vectors below 4 GiB");
@@ -458,8 +458,23 @@ int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
         op->f64_waves = waves;
         op->f64_wg_per_xcd = wg_per_xcd;
     }
+    if (path == 4) {
+        BH_CHECK(waves == 2 || waves == 4 || waves == 8, "bh_op_set_fp64_path: the tiled kernel takes 2, 4 or 8 waves");
+        op->f64_tile_w = waves;
+    }
     op->f64_path = path;
-    if (path == 1 || path == 3) op->modal_faces = path == 3;
+    BH_API_END
+}
+
+int bh_op_set_fp32_path(bh_op* op, int path, int waves) {
+    BH_API_BEGIN
+    BH_CHECK(op, "bh_op_set_fp32_path: null operator");
+    BH_CHECK(path == 0 || path == 1 || path == 3 || path == 4,
+             "bh_op_set_fp32_path: path must be 0 (autotune), 1 (face rule node by node), 3 (modal face terms) or 4 "
+             "(tiled modal face terms)");
+    if (path == 4) BH_CHECK(waves == 2 || waves == 4 || waves == 8, "bh_op_set_fp32_path: the tiled kernel takes 2, 4 or 8 waves");
+    op->f32_path = path;
+    op->f32_tile_w = path == 4 ? waves : 0;
     BH_API_END
 }
 

@@ -190,10 +190,16 @@ struct bh_op {
     float mfma_tuned_ms = 0.f;
     // FP64 apply: vector kernel or FP64-MFMA kernel, picked by timing on first use
     bh::DevBuf tab64m;          // FP64 MFMA tables
-    int f64_path = 0;           // 0: not tuned, 1: vector, 2: MFMA
+    // 0: not tuned; vector kernel 1: face rule node by node, 3: modal faces,
+    // 4: tiled modal faces (f64_tile_w waves per workgroup); 2: FP64 MFMA
+    int f64_path = 0;
+    int f64_tile_w = 0;
     int f64_waves = 8, f64_wg_per_xcd = 64;
     float f64_ms_vector = 0.f, f64_ms_mfma = 0.f;
-    bool modal_faces = true;    // vector apply: face terms in modal form (false: node by node); FP64 path 3 / 1
+    // FP32 apply (multigrid levels), tuned separately on its first use: form 1, 3 or 4
+    int f32_path = 0;
+    int f32_tile_w = 0;
+    float f32_ms = 0.f;
     size_t off_G = 0, off_T = 0, off_N = 0, off_vw = 0, off_fw = 0, off_K = 0;
     // boundary values and RHS -> affine vector
     std::vector<double> gbnd;

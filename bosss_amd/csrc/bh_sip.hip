@@ -17,6 +17,7 @@
 // exactly once per apply: HBM traffic = u + y + nu.
 #include <cmath>
 #include <functional>
+#include <type_traits>
 
 #include "bh_internal.hpp"
 
@@ -270,7 +271,7 @@ __host__ __device__ constexpr double sqrt_half(int j) {
 // c1..c5 are those of face_pair.  The two sides are independent, so the kernel runs
 // them one after the other with one neighbour's coefficients live at a time.
 template <int D, int K, typename R, int d, int s>
-__device__ __forceinline__ void face_side_modal(const R* uP, const R* un, bool bnd, bool skip, R gP, R gQ, R rq,
+__device__ __forceinline__ void face_side_modal(R* uP, R* un, bool bnd, bool skip, R gP, R gQ, R rq,
                                                 R nuP, R nuQ, R mu, R* y) {
     constexpr int Np = binom(K + D, D);
     static_assert(K <= 6, "sqrt_half covers j <= 6");
@@ -297,13 +298,12 @@ __device__ __forceinline__ void face_side_modal(const R* uP, const R* un, bool b
                     // p_j(+1) = s_j, p_j'(+1) = sd_j; p_j(-1) = (-1)^j s_j, p_j'(-1) = -(-1)^j sd_j
                     constexpr double vo = s ? sv : odd * sv, dov = s ? sd : -odd * sd;   // own side
                     constexpr double vn = s ? odd * sv : sv, dnv = s ? -odd * sd : sd;   // neighbour's side
-                    R vP = uP[n], vN = un[n];
-                    asm volatile("" : "+v"(vP), "+v"(vN));
-                    Ao = fma(R(vo), vP, Ao);
-                    An = fma(R(vn), vN, An);
+                    asm volatile("" : "+v"(uP[n]), "+v"(un[n]));   // in place: no copies
+                    Ao = fma(R(vo), uP[n], Ao);
+                    An = fma(R(vn), un[n], An);
                     if constexpr (j > 0) {
-                        Bo = fma(R(dov), vP, Bo);
-                        Bn = fma(R(dnv), vN, Bn);
+                        Bo = fma(R(dov), uP[n], Bo);
+                        Bn = fma(R(dnv), un[n], Bn);
                     }
                 }
             });
@@ -368,10 +368,115 @@ __device__ __forceinline__ void volume_term(const SipArgs<R>& a, const R* uP, co
     });
 }
 
+// The same volume term with Kv from its closed form: on the orthonormal Legendre
+// basis p_j = sqrt(j + 1/2) P_j, int_{-1}^{1} p_a' p_b' = sqrt((a + 1/2)(b + 1/2)) m (m + 1)
+// with m = min(a, b) for a, b >= 1 and a + b even (from P_n' = sum over k = n-1, n-3, ...
+// of (2k + 1) P_k), and the k + 1 point Gauss rule integrates these products exactly,
+// so Kv[d][m][n] is that value when the exponents agree in the other directions.
+// The entries become compile-time constants: no table loads, no registers held.
+__host__ __device__ constexpr double kv_closed(int a, int b) {
+    const int m = a < b ? a : b;
+    return sqrt_half(a) * sqrt_half(b) * (double)(m * (m + 1));
+}
+template <int D, int K, typename R>
+__device__ __forceinline__ void volume_term_cf(R* uP, const R* s2, R nuP, R* y) {
+    constexpr int Np = binom(K + D, D);
+    static_for<D>([&](auto DD) {
+        constexpr int d = DD;
+        const R cd = -nuP * s2[d];
+        static_for<Np>([&](auto MM) {
+            constexpr int m = MM;
+            R acc = 0;
+            bool any = false;
+            static_for<Np>([&](auto NN) {
+                constexpr int n = NN;
+                if constexpr (kv_nonzero(D, K, m, n, d)) {
+                    constexpr double kv = kv_closed(mode_exponent(D, K, m, d), mode_exponent(D, K, n, d));
+                    acc = fma(R(kv), uP[n], acc);
+                    any = true;
+                }
+            });
+            if (any) {
+                y[m] = fma(cd, acc, y[m]);
+                asm volatile("" : "+v"(y[m]));
+            }
+        });
+    });
+}
+
 // geometry of cell with logical index idx
 template <int D, typename R, typename S>
 __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S* h) {
     for (int d = 0; d < D; ++d) h[d] = S(a.width[d * a.nmax + idx[d]]);
+}
+
+// Writes the results of a wave's 64 consecutive cells (lane l: cell c0 + l, its Np
+// values y; the first nvalid lanes valid) as 16-byte chunks that consecutive lanes
+// write to consecutive addresses: each lane stores its values in a per-wave LDS buffer
+// (cell stride Np + 1, so the lanes' rows fall into different banks), and the wave
+// reads them back in chunk order.  One lane per cell writing its own Np values would
+// issue Np stores per wave that each touch 64 scattered rows.
+// out = alpha * (y + affine) + (b0 ? b0 : beta * out), element by element.
+template <int Np, typename R>
+__device__ __forceinline__ void store_cells_coalesced(R* __restrict__ buf, const R* y, int lane, size_t c0, int nvalid,
+                                                      const SipArgs<R>& a) {
+#pragma unroll
+    for (int n = 0; n < Np; ++n) buf[lane * (Np + 1) + n] = y[n];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int CH = 16 / sizeof(R);             // elements per chunk
+    constexpr int NCH = (64 * Np + CH - 1) / CH;   // chunks per wave
+    const int nel = nvalid * Np;
+    R* __restrict__ yp = a.y + c0 * Np;
+    const R* __restrict__ bp = a.b0 ? a.b0 + c0 * Np : nullptr;
+    const double* __restrict__ af = a.affine ? a.affine + c0 * Np : nullptr;
+    const R al = R(a.alpha), be = R(a.beta);
+    const bool full_chunks = ((c0 * Np) % CH) == 0;   // chunk-aligned block start
+#pragma unroll
+    for (int i = 0; i < (NCH + 63) / 64; ++i) {
+        const int q = lane + 64 * i;
+        const int e0 = q * CH;
+        if (e0 >= nel) break;
+        R v[CH];
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+            const int e = e0 + t;
+            const int cell = e / Np, m = e - cell * Np;
+            v[t] = e < nel ? buf[cell * (Np + 1) + m] : R(0);
+        }
+        if (af) {
+#pragma unroll
+            for (int t = 0; t < CH; ++t) if (e0 + t < nel) v[t] += R(af[e0 + t]);
+        }
+        if (full_chunks && e0 + CH <= nel) {
+            using V = std::conditional_t<sizeof(R) == 8, double2, float4>;
+            V o;
+            R* ov = reinterpret_cast<R*>(&o);
+            if (bp) {
+                const V b = *reinterpret_cast<const V*>(bp + e0);
+                const R* bv = reinterpret_cast<const R*>(&b);
+#pragma unroll
+                for (int t = 0; t < CH; ++t) ov[t] = bv[t] + al * v[t];
+            } else if (a.beta == 0.0) {
+#pragma unroll
+                for (int t = 0; t < CH; ++t) ov[t] = al * v[t];
+            } else {
+                const V b = *reinterpret_cast<const V*>(yp + e0);
+                const R* bv = reinterpret_cast<const R*>(&b);
+#pragma unroll
+                for (int t = 0; t < CH; ++t) ov[t] = be * bv[t] + al * v[t];
+            }
+            *reinterpret_cast<V*>(yp + e0) = o;
+        } else {
+#pragma unroll
+            for (int t = 0; t < CH; ++t) {
+                const int e = e0 + t;
+                if (e >= nel) break;
+                yp[e] = bp ? bp[e] + al * v[t] : (a.beta == 0.0 ? al * v[t] : be * yp[e] + al * v[t]);
+            }
+        }
+    }
 }
 
 // MODAL: every direction loads its neighbours' coefficients and uses the modal
@@ -402,38 +507,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
         s2[d] = R(4) * ih[d] * ih[d];
         cjP += (R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
     }
-    volume_term<D, K, R>(a, uP, s2, nuP, y);
+    if constexpr (MODAL) volume_term_cf<D, K, R>(uP, s2, nuP, y);
+    else volume_term<D, K, R>(a, uP, s2, nuP, y);
     const R pen = R(a.pen);
     static_for<D>([&](auto DD) {
         constexpr int d = DD;
         const R ihd = ih[d];
         if constexpr (MODAL) {
-            // modal faces, one side at a time
+            // modal faces, one side at a time.  A boundary side reads the cell itself as its
+            // "neighbour": the flux coefficients of the neighbour's traces (c2, c4, c5) are zero
+            // there, so any finite values do, and the loads need no branch or zero fill.
             static_for<2>([&](auto SS) {
                 constexpr int s = SS;
                 const bool bnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
                 const bool skip = bnd && ((a.bc_neumann >> (2 * d + s)) & 1);
-                R gQ = 0, rq = 0, nuQ = nuP, mu = pen * cjP;
+                int nidx[3] = {idx[0], idx[1], idx[2]};
+                nidx[d] += bnd ? 0 : (s ? 1 : -1);
+                const int cq = encode<D>(nidx, a.n0, a.n1);
+                const R ihq = R(a.width[(3 + d) * a.nmax + nidx[d]]);
+                const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
+                const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+                const R cjQ = cjP - bP * ihd + bQ * ihq;   // = cjP on a boundary side
+                const R gQ = R(2) * ihq;
+                const R rq = R(a.width[(6 + d) * a.nmax + idx[d]] * a.width[(9 + d) * a.nmax + nidx[d]]);
+                const R nuQ = a.nu[cq];
+                const R mu = pen * (cjP > cjQ ? cjP : cjQ);
                 R un[Np];
-                if (bnd) {
+                const R* uq = a.u + (size_t)cq * Np;
 #pragma unroll
-                    for (int n = 0; n < Np; ++n) un[n] = 0;
-                } else {
-                    int nidx[3] = {idx[0], idx[1], idx[2]};
-                    nidx[d] += s ? 1 : -1;
-                    const int cq = encode<D>(nidx, a.n0, a.n1);
-                    const R ihq = R(a.width[(3 + d) * a.nmax + nidx[d]]);
-                    const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
-                    const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
-                    const R cjQ = cjP - bP * ihd + bQ * ihq;
-                    gQ = R(2) * ihq;
-                    rq = R(a.width[(6 + d) * a.nmax + idx[d]] * a.width[(9 + d) * a.nmax + nidx[d]]);
-                    nuQ = a.nu[cq];
-                    mu = pen * (cjP > cjQ ? cjP : cjQ);
-                    const R* uq = a.u + (size_t)cq * Np;
-#pragma unroll
-                    for (int n = 0; n < Np; ++n) un[n] = uq[n];
-                }
+                for (int n = 0; n < Np; ++n) un[n] = uq[n];
                 face_side_modal<D, K, R, d, s>(uP, un, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y);
             });
             return;
@@ -494,25 +596,320 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void s
         }
         face_pair<D, K, R, d, FAST>(a, uP, uM, uQ, bnd, skip, R(2) * ihd, gQ, rq, nuP, nuQ, mu, y, hv, lane);
     });
-    if (!valid) return;
-    R* yp = a.y + (size_t)c * Np;
-    const R al = R(a.alpha), be = R(a.beta);
-    if (a.affine) {
-        const double* af = a.affine + (size_t)c * Np;
-#pragma unroll
-        for (int n = 0; n < Np; ++n) y[n] += R(af[n]);
-    }
-    if (a.b0) {
-        const R* bp = a.b0 + (size_t)c * Np;
-#pragma unroll
-        for (int n = 0; n < Np; ++n) yp[n] = bp[n] + al * y[n];
-    } else if (a.beta == 0.0) {
-#pragma unroll
-        for (int n = 0; n < Np; ++n) yp[n] = al * y[n];
+    __shared__ R sbuf[4][64 * (Np + 1)];
+    const int nvalid = a.J - c0 < 64 ? a.J - c0 : 64;
+    store_cells_coalesced<Np, R>(sbuf[threadIdx.x >> 6], y, lane, (size_t)c0, nvalid, a);
+}
+
+// ------------------------------------------------------------------ tiled modal apply
+// The modal face form with the neighbours' traces exchanged instead of reloaded.
+// Direction roles: DF, the direction of consecutive cells, runs along the 64 lanes
+// of a wave; DT runs along the W waves of a workgroup (one row of up to 64 cells
+// each); DZ (3D) is the remaining direction.  Per face mode e' a face needs only
+// the value and d-derivative traces (A, B) of both cells, so:
+//  * DF: every lane forms its own traces on both faces and takes the neighbours'
+//    from the adjacent lanes (DPP wave shifts); the cells just before and after
+//    the wave (rows longer than 64 cells) are traced once, across its lanes;
+//  * DT: every wave writes its traces on both DT faces to LDS; after one workgroup
+//    barrier, a wave reads the adjacent waves' traces, and only the first and last
+//    wave of the workgroup load a neighbour row from memory;
+//  * DZ: the neighbours' modes are loaded, as in sip_apply_kernel.
+// Workgroups are mapped to tiles XCD-contiguously (tile = (b mod 8) * per_xcd +
+// b / 8, the hardware dispatching workgroup b to XCD b mod 8), so the DZ
+// neighbour rows of a tile are read by the same XCD's L2 shortly before or after.
+// Same integrals and the same flux as face_side_modal; results equal
+// sip_apply_kernel<.., MODAL = true> up to rounding.
+
+// index of face mode f (a root: exponent 0 in d) among the roots of direction d
+__host__ __device__ constexpr int face_slot(int D, int k, int f, int d) {
+    int r = 0;
+    for (int m = 0; m < f; ++m) if (mode_exponent(D, k, m, d) == 0) ++r;
+    return r;
+}
+
+// value / d-derivative traces of face mode f on face `side` (0: '-', 1: '+') of
+// direction d: A = sum_j p_j(+-1) u_(e',j), B = sum_j p_j'(+-1) u_(e',j)
+template <int D, int K, typename R, int d, int side, int f>
+__device__ __forceinline__ void modal_trace(R* u, R& A, R& B) {
+    constexpr int Np = binom(K + D, D);
+    A = 0;
+    B = 0;
+    static_for<Np>([&](auto NN) {
+        constexpr int n = NN;
+        if constexpr (face_root(D, K, n, d) == f) {
+            constexpr int j = mode_exponent(D, K, n, d);
+            constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+            constexpr double odd = (j & 1) ? -1.0 : 1.0;
+            constexpr double cv = side ? sv : odd * sv, cdv = side ? sd : -odd * sd;
+            asm volatile("" : "+v"(u[n]));   // one face mode's inputs at a time, pinned in place
+            A = fma(R(cv), u[n], A);
+            if constexpr (j > 0) B = fma(R(cdv), u[n], B);
+        }
+    });
+}
+
+// y_(e',j) += p_j(+-1) CV + p_j'(+-1) CD on face `side` of direction d
+template <int D, int K, typename R, int d, int side, int f>
+__device__ __forceinline__ void modal_scatter(R CV, R CD, R* y) {
+    constexpr int Np = binom(K + D, D);
+    static_for<Np>([&](auto MM) {
+        constexpr int m = MM;
+        if constexpr (face_root(D, K, m, d) == f) {
+            constexpr int j = mode_exponent(D, K, m, d);
+            constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+            constexpr double odd = (j & 1) ? -1.0 : 1.0;
+            constexpr double cv = side ? sv : odd * sv, cdv = side ? sd : -odd * sd;
+            y[m] = fma(R(cv), CV, y[m]);
+            if constexpr (j > 0) y[m] = fma(R(cdv), CD, y[m]);
+            asm volatile("" : "+v"(y[m]));
+        }
+    });
+}
+
+// SIP flux coefficients of one side (those of face_pair / face_side_modal):
+// CV = c1 Bo + c2 Bn + c3 Ao + c4 An, CD = c1 Ao + c5 An (o: own, n: neighbour)
+template <typename R>
+struct SideCoef {
+    R c1, c2, c3, c4, c5;
+};
+template <typename R, int s>
+__device__ __forceinline__ SideCoef<R> side_coef(bool bnd, bool skip, R gP, R gQ, R rq, R nuP, R nuQ, R mu) {
+    const R sgn = s ? R(1) : R(-1);
+    const R G = skip ? R(0) : gP;
+    const R A = G * nuP * sgn * gP;
+    const R nuMax = (fabs(nuP) > fabs(nuQ)) ? nuP : nuQ;
+    SideCoef<R> c;
+    c.c1 = (bnd ? R(1) : R(0.5)) * A;
+    c.c3 = -G * mu * (bnd ? R(2) * nuP : nuMax);
+    c.c2 = bnd ? R(0) : R(0.5) * G * nuQ * sgn * gQ * rq;
+    c.c4 = bnd ? R(0) : -c.c3 * rq;
+    c.c5 = bnd ? R(0) : R(-0.5) * A * rq;
+    return c;
+}
+
+// Face data of side s of direction d of the cell idx: boundary / Neumann flags and,
+// for an interior face, the neighbour's index, 2/h, sqrt-width ratio and penalty.
+template <int D, typename R, int d, int s>
+struct SideGeom {
+    bool bnd, skip;
+    int cq;
+    R gQ, rq, mu;
+};
+// A boundary side gets the cell itself as its neighbour (cq = own index, the penalty of
+// the own cell): the flux coefficients that multiply the neighbour's traces are zero
+// there, so reading finite values of the cell itself needs no branch or zero fill.
+template <int D, typename R, int d, int s>
+__device__ __forceinline__ SideGeom<D, R, d, s> side_geom(const SipArgs<R>& a, const int* idx, const int* nn, R cjP,
+                                                          R ihd, R pen) {
+    SideGeom<D, R, d, s> g;
+    g.bnd = s ? (idx[d] == nn[d] - 1) : (idx[d] == 0);
+    g.skip = g.bnd && ((a.bc_neumann >> (2 * d + s)) & 1);
+    int nidx[3] = {idx[0], idx[1], idx[2]};
+    nidx[d] += g.bnd ? 0 : (s ? 1 : -1);
+    g.cq = encode<D>(nidx, a.n0, a.n1);
+    const R ihq = R(a.width[(3 + d) * a.nmax + nidx[d]]);
+    const R bP = R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1));
+    const R bQ = R(1) + R(0.5) * R((nidx[d] == 0) + (nidx[d] == nn[d] - 1));
+    const R cjQ = cjP - bP * ihd + bQ * ihq;   // = cjP on a boundary side
+    g.gQ = R(2) * ihq;
+    g.rq = R(a.width[(6 + d) * a.nmax + idx[d]] * a.width[(9 + d) * a.nmax + nidx[d]]);
+    g.mu = pen * (cjP > cjQ ? cjP : cjQ);
+    return g;
+}
+
+// whole-wave lane shift of a value (DPP): WF_SR1 (0x138) gives lane l the value of
+// lane l - 1, WF_SL1 (0x130) that of lane l + 1; the lane without a source keeps old
+template <int CTRL, typename R>
+__device__ __forceinline__ R wave_shift(R old, R src) {
+    if constexpr (sizeof(R) == 8) {
+        const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+        const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     } else {
-#pragma unroll
-        for (int n = 0; n < Np; ++n) yp[n] = be * yp[n] + al * y[n];
+        return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, 0xf, 0xf, false));
     }
+}
+
+// One side of direction d from the neighbour's traces: nb(f, fs, An, Bn) supplies them.
+template <int D, int K, typename R, int d, int s, typename NB>
+__device__ __forceinline__ void face_side_traces(R* uP, const SideCoef<R>& c, NB&& nb, R* y) {
+    constexpr int Np = binom(K + D, D);
+    static_for<Np>([&](auto FF) {
+        constexpr int f = FF;
+        if constexpr (mode_exponent(D, K, f, d) == 0) {
+            constexpr int fs = face_slot(D, K, f, d);
+            R Ao, Bo, An, Bn;
+            modal_trace<D, K, R, d, s, f>(uP, Ao, Bo);
+            nb(std::integral_constant<int, f>{}, std::integral_constant<int, fs>{}, An, Bn);
+            asm volatile("" : "+v"(Ao), "+v"(Bo), "+v"(An), "+v"(Bn));
+            const R CV = c.c1 * Bo + c.c2 * Bn + c.c3 * Ao + c.c4 * An;
+            const R CD = c.c1 * Ao + c.c5 * An;
+            modal_scatter<D, K, R, d, s, f>(CV, CD, y);
+        }
+    });
+}
+
+template <int D, int K, typename R, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void sip_apply_tile_kernel(SipArgs<R> a, int nseg, int nrb, int ntiles, int per_xcd) {
+    constexpr int Np = binom(K + D, D);
+    constexpr int NFM = binom(K + D - 1, D - 1);   // face modes
+    constexpr int DF = D == 2 ? 1 : 0;             // consecutive cells: lanes
+    constexpr int DT = D == 2 ? 0 : 1;             // rows of the workgroup: LDS
+    __shared__ R tr[W][2][2 * NFM][64];            // [wave][face '-' / '+'][A, B per face mode][lane]
+    const int t = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (t >= ntiles) return;                       // whole workgroup, before any barrier
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int seg = t % nseg, rest = t / nseg;
+    const int rb = rest % nrb, z = rest / nrb;
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    const int x0 = seg * 64, x = x0 + lane, r = rb * W + w;
+    const bool valid = x < nn[DF] && r < nn[DT];
+    int idx[3] = {0, 0, 0};
+    idx[DF] = x < nn[DF] ? x : nn[DF] - 1;         // every lane stays active for the exchanges
+    idx[DT] = r < nn[DT] ? r : nn[DT] - 1;
+    if (D == 3) idx[2] = z;
+    const int c = encode<D>(idx, a.n0, a.n1);
+    R uP[Np], y[Np];
+    const R* up = a.u + (size_t)c * Np;
+#pragma unroll
+    for (int n = 0; n < Np; ++n) { uP[n] = up[n]; y[n] = 0; }
+    const R nuP = a.nu[c];
+    R ih[3] = {1, 1, 1}, s2[D], cjP = 0;
+    for (int d = 0; d < D; ++d) {
+        ih[d] = R(a.width[(3 + d) * a.nmax + idx[d]]);
+        s2[d] = R(4) * ih[d] * ih[d];
+        cjP += (R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
+    }
+    const R pen = R(a.pen);
+    // 1. own traces on both DT faces -> LDS
+    static_for<Np>([&](auto FF) {
+        constexpr int f = FF;
+        if constexpr (mode_exponent(D, K, f, DT) == 0) {
+            constexpr int fs = face_slot(D, K, f, DT);
+            R Am, Bm, Ap, Bp;
+            modal_trace<D, K, R, DT, 0, f>(uP, Am, Bm);
+            modal_trace<D, K, R, DT, 1, f>(uP, Ap, Bp);
+            tr[w][0][2 * fs][lane] = Am;
+            tr[w][0][2 * fs + 1][lane] = Bm;
+            tr[w][1][2 * fs][lane] = Ap;
+            tr[w][1][2 * fs + 1][lane] = Bp;
+        }
+    });
+    // 2. volume
+    volume_term_cf<D, K, R>(uP, s2, nuP, y);
+    // 3. DF faces from the adjacent lanes
+    {
+        constexpr int d = DF;
+        const R ihd = ih[d];
+        const auto gm = side_geom<D, R, d, 0>(a, idx, nn, cjP, ihd, pen);
+        const auto gp = side_geom<D, R, d, 1>(a, idx, nn, cjP, ihd, pen);
+        const R nuM = a.nu[gm.cq], nuQ = a.nu[gp.cq];
+        const SideCoef<R> cm = side_coef<R, 0>(gm.bnd, gm.skip, R(2) * ihd, gm.gQ, gm.rq, nuP, nuM, gm.mu);
+        const SideCoef<R> cp = side_coef<R, 1>(gp.bnd, gp.skip, R(2) * ihd, gp.gQ, gp.rq, nuP, nuQ, gp.mu);
+        // traces of the cells before and after the wave: lanes 2 fs + {0, 1} (A, B of face
+        // mode slot fs) for the cell before (its '+' face), lanes 32 + 2 fs + {0, 1} for the
+        // cell after (its '-' face)
+        const bool halo = x0 > 0 || x0 + 64 < nn[d];   // wave-uniform
+        R hv = 0;
+        if (halo) {
+            const int hl = lane & 31;
+            const bool after = lane >= 32;
+            const int hx = after ? x0 + 64 : x0 - 1;
+            if (hl < 2 * NFM && hx >= 0 && hx < nn[d]) {
+                int hidx[3] = {idx[0], idx[1], idx[2]};
+                hidx[d] = hx;
+                const R* uh = a.u + (size_t)encode<D>(hidx, a.n0, a.n1) * Np;
+                const int fsl = hl >> 1, kind = hl & 1;
+                static_for<Np>([&](auto NN) {
+                    constexpr int n = NN;
+                    constexpr int j = mode_exponent(D, K, n, d);
+                    constexpr int fsn = face_slot(D, K, face_root(D, K, n, d), d);
+                    constexpr double sv = sqrt_half(j), sd = sqrt_half(j) * (j * (j + 1) / 2);
+                    constexpr double odd = (j & 1) ? -1.0 : 1.0;
+                    if (fsn == fsl && (kind == 0 || j > 0)) {
+                        const R cf = kind ? (after ? R(-odd * sd) : R(sd)) : (after ? R(odd * sv) : R(sv));
+                        hv = fma(cf, uh[n], hv);
+                    }
+                });
+            }
+        }
+        static_for<Np>([&](auto FF) {
+            constexpr int f = FF;
+            if constexpr (mode_exponent(D, K, f, d) == 0) {
+                constexpr int fs = face_slot(D, K, f, d);
+                R Am, Bm, Ap, Bp;
+                modal_trace<D, K, R, d, 0, f>(uP, Am, Bm);
+                modal_trace<D, K, R, d, 1, f>(uP, Ap, Bp);
+                const R hmA = halo ? lane_value(hv, 2 * fs) : R(0), hmB = halo ? lane_value(hv, 2 * fs + 1) : R(0);
+                const R hpA = halo ? lane_value(hv, 32 + 2 * fs) : R(0), hpB = halo ? lane_value(hv, 33 + 2 * fs) : R(0);
+                // '-' neighbour (lane - 1): its '+' traces; '+' neighbour (lane + 1): its '-' traces
+                R AnM = wave_shift<0x138>(hmA, Ap), BnM = wave_shift<0x138>(hmB, Bp);
+                R AnP = wave_shift<0x130>(hpA, Am), BnP = wave_shift<0x130>(hpB, Bm);
+                const R CVm = cm.c1 * Bm + cm.c2 * BnM + cm.c3 * Am + cm.c4 * AnM;
+                const R CDm = cm.c1 * Am + cm.c5 * AnM;
+                const R CVp = cp.c1 * Bp + cp.c2 * BnP + cp.c3 * Ap + cp.c4 * AnP;
+                const R CDp = cp.c1 * Ap + cp.c5 * AnP;
+                modal_scatter<D, K, R, d, 0, f>(CVm, CDm, y);
+                modal_scatter<D, K, R, d, 1, f>(CVp, CDp, y);
+            }
+        });
+    }
+    // 4. DZ faces (3D): the neighbours' modes from memory, one side at a time
+    if constexpr (D == 3) {
+        constexpr int d = 2;
+        const R ihd = ih[d];
+        static_for<2>([&](auto SS) {
+            constexpr int s = SS;
+            const auto g = side_geom<D, R, d, s>(a, idx, nn, cjP, ihd, pen);
+            R un[Np];
+            const R nuQ = a.nu[g.cq];
+            const R* uq = a.u + (size_t)g.cq * Np;
+#pragma unroll
+            for (int n = 0; n < Np; ++n) un[n] = uq[n];
+            const SideCoef<R> cf = side_coef<R, s>(g.bnd, g.skip, R(2) * ihd, g.gQ, g.rq, nuP, nuQ, g.mu);
+            face_side_traces<D, K, R, d, s>(uP, cf, [&](auto F, auto, R& An, R& Bn) {
+                modal_trace<D, K, R, d, 1 - s, decltype(F)::value>(un, An, Bn);
+            }, y);
+        });
+    }
+    // 5. DT faces: the adjacent waves' traces from LDS, the workgroup's edge rows from memory
+    __syncthreads();
+    {
+        constexpr int d = DT;
+        const R ihd = ih[d];
+        static_for<2>([&](auto SS) {
+            constexpr int s = SS;
+            const auto g = side_geom<D, R, d, s>(a, idx, nn, cjP, ihd, pen);
+            const R nuQ = a.nu[g.cq];
+            const SideCoef<R> cf = side_coef<R, s>(g.bnd, g.skip, R(2) * ihd, g.gQ, g.rq, nuP, nuQ, g.mu);
+            const int wn = s ? w + 1 : w - 1;
+            if (g.bnd || (wn >= 0 && wn < W)) {   // wave-uniform
+                const int wr = g.bnd ? w : wn;
+                face_side_traces<D, K, R, d, s>(uP, cf, [&](auto, auto FS, R& An, R& Bn) {
+                    constexpr int fs = decltype(FS)::value;
+                    An = tr[wr][1 - s][2 * fs][lane];   // on a boundary side: finite, multiplied by zero
+                    Bn = tr[wr][1 - s][2 * fs + 1][lane];
+                }, y);
+            } else {
+                R un[Np];
+                const R* uq = a.u + (size_t)g.cq * Np;
+#pragma unroll
+                for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                face_side_traces<D, K, R, d, s>(uP, cf, [&](auto F, auto, R& An, R& Bn) {
+                    modal_trace<D, K, R, d, 1 - s, decltype(F)::value>(un, An, Bn);
+                }, y);
+            }
+        });
+    }
+    // 6. results: the wave's row of cells is contiguous in memory; after the barrier no wave
+    // reads the trace buffers any more, so each wave stages its results in its own
+    static_assert(4 * NFM >= Np + 1, "trace buffer too small to stage the results");
+    __syncthreads();
+    const int nvalid = r < nn[DT] ? (nn[DF] - x0 < 64 ? nn[DF] - x0 : 64) : 0;
+    int idx0[3] = {idx[0], idx[1], idx[2]};
+    idx0[DF] = x0;
+    store_cells_coalesced<Np, R>(&tr[w][0][0][0], y, lane, (size_t)encode<D>(idx0, a.n0, a.n1), nvalid, a);
 }
 
 // Assembly: thread = (cell r, trial mode n).  Column n of the diagonal block
@@ -618,57 +1015,108 @@ static SipArgs<R> make_args(bh_op* op, const void* u, void* y, double alpha, dou
     return a;
 }
 
+// Vector-kernel forms: 1 = face rule node by node, 3 = modal faces, 4 = tiled modal
+// faces (tile_w waves per workgroup).
+template <typename R, int D, int K, int W>
+static void launch_tile(bh_op* op, const SipArgs<R>& a) {
+    const int DF = D == 2 ? 1 : 0, DT = D == 2 ? 0 : 1;
+    const int nn[3] = {a.n0, a.n1, a.n2};
+    const int nseg = (nn[DF] + 63) / 64, nrb = (nn[DT] + W - 1) / W, nz = D == 3 ? nn[2] : 1;
+    const int64_t nt = (int64_t)nseg * nrb * nz;
+    BH_CHECK(nt < (int64_t)1 << 30, "sip_apply: grid too large for the tiled kernel");
+    const int ntiles = (int)nt, per_xcd = (ntiles + 7) / 8;
+    hipLaunchKernelGGL((sip_apply_tile_kernel<D, K, R, W>), dim3(8 * per_xcd), dim3(64 * W), 0, op->ctx->stream, a, nseg,
+                       nrb, ntiles, per_xcd);
+}
+
 template <typename R, int D, int K>
-static void launch_apply(bh_op* op, const SipArgs<R>& a) {
+static void launch_apply(bh_op* op, const SipArgs<R>& a, int form, int tile_w) {
     dim3 blk(256), grd((a.J + 255) / 256);
-    if (op->modal_faces)
+    if (form == 4) {
+        if (tile_w == 2) launch_tile<R, D, K, 2>(op, a);
+        else if (tile_w == 8) launch_tile<R, D, K, 8>(op, a);
+        else launch_tile<R, D, K, 4>(op, a);
+    } else if (form == 3) {
         hipLaunchKernelGGL((sip_apply_kernel<D, K, R, true>), grd, blk, 0, op->ctx->stream, a);
-    else
+    } else {
         hipLaunchKernelGGL((sip_apply_kernel<D, K, R, false>), grd, blk, 0, op->ctx->stream, a);
+    }
     BH_HIP(hipGetLastError());
 }
 
 template <typename R>
-static void dispatch_apply(bh_op* op, const SipArgs<R>& a) {
+static void dispatch_apply(bh_op* op, const SipArgs<R>& a, int form, int tile_w) {
     const int D = op->D, K = op->k;
-#define BH_CASE(DD, KK) if (D == DD && K == KK) return launch_apply<R, DD, KK>(op, a);
+#define BH_CASE(DD, KK) if (D == DD && K == KK) return launch_apply<R, DD, KK>(op, a, form, tile_w);
     BH_CASE(2, 1) BH_CASE(2, 2) BH_CASE(2, 3) BH_CASE(2, 4)
     BH_CASE(3, 1) BH_CASE(3, 2) BH_CASE(3, 3)
 #undef BH_CASE
     throw Error(-3, "sip_apply: unsupported (D, k) = (" + std::to_string(D) + ", " + std::to_string(K) + ")");
 }
 
-// Autotuner of the FP64 apply: times the vector kernel (face terms node by node
-// and in modal form) and the FP64-MFMA kernel (waves per workgroup x workgroups
-// per XCD) on a scratch vector of this operator and keeps the fastest; all
-// compute the same integrals, up to rounding.
-static void tune_fp64_path(bh_op* op) {
-    op->f64_path = 3;   // the modal vector kernel, unless timing says otherwise
-    op->modal_faces = true;
-    if (!sip_apply_mfma64_supported(op->D, op->k, op->N)) return;
-    DevBuf us(op->N * sizeof(double)), ys(op->N * sizeof(double));
-    fill(op->ctx, us.as<double>(), 1.0, op->N);
-    hipEvent_t e0, e1;
-    BH_HIP(hipEventCreate(&e0));
-    BH_HIP(hipEventCreate(&e1));
-    auto time_it = [&](const std::function<void()>& f) {
+namespace {
+struct ApplyTimer {
+    hipStream_t st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    explicit ApplyTimer(hipStream_t s) : st(s) {
+        BH_HIP(hipEventCreate(&e0));
+        BH_HIP(hipEventCreate(&e1));
+    }
+    ~ApplyTimer() {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    float operator()(const std::function<void()>& f) {
         f();
-        BH_HIP(hipEventRecord(e0, op->ctx->stream));
+        BH_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < 3; ++r) f();
-        BH_HIP(hipEventRecord(e1, op->ctx->stream));
+        BH_HIP(hipEventRecord(e1, st));
         BH_HIP(hipEventSynchronize(e1));
         float ms = 0;
         BH_HIP(hipEventElapsedTime(&ms, e0, e1));
         return ms / 3;
-    };
-    // the vector kernel in both face forms (node by node, modal)
-    auto vec = [&] { dispatch_apply<double>(op, make_args<double>(op, us.p, ys.p, 1.0, 0.0, false)); };
-    op->modal_faces = false;
-    const float tnode = time_it(vec);
-    op->modal_faces = true;
-    const float tmodal = time_it(vec);
-    op->modal_faces = tmodal <= tnode;
-    const float tv = std::min(tnode, tmodal);
+    }
+};
+}  // namespace
+
+// Times the vector-kernel forms of precision R (modal, tiled modal with 2, 4 and 8
+// waves per workgroup, and node by node when with_node) on scratch vectors of this
+// operator; returns the fastest form and its time.
+template <typename R>
+static float tune_vector_forms(bh_op* op, bool with_node, int& form, int& tile_w) {
+    DevBuf us(op->N * sizeof(R)), ys(op->N * sizeof(R));
+    if (sizeof(R) == 8) fill(op->ctx, us.as<double>(), 1.0, op->N);
+    else fill32(op->ctx, us.as<float>(), 1.0f, op->N);
+    ApplyTimer time_it(op->ctx->stream);
+    const SipArgs<R> a = make_args<R>(op, us.p, ys.p, 1.0, 0.0, false);
+    float best = time_it([&] { dispatch_apply<R>(op, a, 3, 0); });
+    form = 3;
+    tile_w = 0;
+    for (int w : {2, 4, 8}) {
+        const float t = time_it([&] { dispatch_apply<R>(op, a, 4, w); });
+        if (t < best) { best = t; form = 4; tile_w = w; }
+    }
+    if (with_node) {
+        const float t = time_it([&] { dispatch_apply<R>(op, a, 1, 0); });
+        if (t < best) { best = t; form = 1; tile_w = 0; }
+    }
+    return best;
+}
+
+// Autotuner of the FP64 apply: times the vector kernel (face terms node by node,
+// modal, tiled modal) and the FP64-MFMA kernel (waves per workgroup x workgroups
+// per XCD) on a scratch vector of this operator and keeps the fastest; all
+// compute the same integrals, up to rounding.
+static void tune_fp64_path(bh_op* op) {
+    int form = 3, tw = 0;
+    const float tv = tune_vector_forms<double>(op, true, form, tw);
+    op->f64_path = form;
+    op->f64_tile_w = tw;
+    op->f64_ms_vector = tv;
+    if (!sip_apply_mfma64_supported(op->D, op->k, op->N)) return;
+    DevBuf us(op->N * sizeof(double)), ys(op->N * sizeof(double));
+    fill(op->ctx, us.as<double>(), 1.0, op->N);
+    ApplyTimer time_it(op->ctx->stream);
     float best = 1e30f;
     int bw = 8, bx = 64;
     for (int w : {4, 8})
@@ -676,13 +1124,18 @@ static void tune_fp64_path(bh_op* op) {
             const float t = time_it([&] { sip_apply_mfma64(op, 1.0, us.as<double>(), 0.0, ys.as<double>(), nullptr, w, px); });
             if (t < best) { best = t; bw = w; bx = px; }
         }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    op->f64_ms_vector = tv;
     op->f64_ms_mfma = best;
     op->f64_waves = bw;
     op->f64_wg_per_xcd = bx;
-    op->f64_path = best < tv ? 2 : (op->modal_faces ? 3 : 1);
+    if (best < tv) op->f64_path = 2;
+}
+
+// The FP32 apply (multigrid levels) is tuned on its own, over the modal forms.
+static void tune_fp32_path(bh_op* op) {
+    int form = 3, tw = 0;
+    op->f32_ms = tune_vector_forms<float>(op, false, form, tw);
+    op->f32_path = form;
+    op->f32_tile_w = tw;
 }
 
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec, const void* b0) {
@@ -697,12 +1150,13 @@ void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, boo
         }
         SipArgs<double> a = make_args<double>(op, u, y, alpha, beta, add_affine);
         a.b0 = (const double*)b0;
-        dispatch_apply<double>(op, a);
+        dispatch_apply<double>(op, a, op->f64_path, op->f64_tile_w);
     } else if (prec == BH_FP32) {
         if (add_affine && op->has_affine) throw Error(-3, "FP32 evaluate supports the linear part only");
+        if (op->f32_path == 0) tune_fp32_path(op);
         SipArgs<float> a = make_args<float>(op, u, y, alpha, beta, false);
         a.b0 = (const float*)b0;
-        dispatch_apply<float>(op, a);
+        dispatch_apply<float>(op, a, op->f32_path, op->f32_tile_w);
     } else if (prec == BH_FP16_MFMA) {
         sip_apply_mfma(op, alpha, (const float*)u, beta, (float*)y, add_affine, (const float*)b0);
     } else {

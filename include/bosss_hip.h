@@ -108,18 +108,24 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
  * XCD; 0 before the first call) and its timed duration (ms) in ms[0].
  * FP64 apply (picked on its first call): cfg[2] = 0 not yet, 1 vector kernel
  * with the face rule evaluated node by node, 2 FP64-MFMA kernel, 3 vector kernel
- * with the face terms in modal form (the FP32 vector apply follows the 1 / 3
- * choice, modal before any FP64 call); cfg[3], cfg[4] the MFMA kernel's waves per workgroup and
- * workgroups per XCD; ms[1], ms[2] the timed vector and best MFMA durations.
+ * with the face terms in modal form, 4 the same with the neighbours' traces
+ * exchanged across lanes and waves (tiled); cfg[3] the chosen kernel's waves per
+ * workgroup (MFMA or tiled), cfg[4] the MFMA kernel's workgroups per XCD; ms[1],
+ * ms[2] the timed best vector and best MFMA durations.  The FP32 apply is tuned
+ * separately, over the modal forms, on its own first call.
  * cfg holds 5 ints, ms 3 floats. */
 int bh_op_tuning(bh_op* op, int* cfg, float* ms);
 /* No counterpart in the reference: pin the FP64 apply to the vector kernel
- * (path 1: face rule node by node; path 3: modal face terms) or the FP64-MFMA
- * kernel (path 2, launch shape waves in {4, 8} per workgroup, wg_per_xcd >= 1)
- * instead of timing them on the first call;
- * path 0 re-arms the autotuner.  Both kernels give the same result up to
- * summation order. */
+ * (path 1: face rule node by node; path 3: modal face terms; path 4: tiled modal
+ * face terms, waves in {2, 4, 8} per workgroup) or the FP64-MFMA kernel (path 2,
+ * launch shape waves in {4, 8} per workgroup, wg_per_xcd >= 1) instead of timing
+ * them on the first call; path 0 re-arms the autotuner.  All kernels give the
+ * same result up to summation order. */
 int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd);
+/* No counterpart in the reference: pin the FP32 apply (multigrid levels) to path 1
+ * (face rule node by node), 3 (modal face terms) or 4 (tiled modal face terms, waves
+ * in {2, 4, 8} per workgroup); path 0 re-arms its own autotuner. */
+int bh_op_set_fp32_path(bh_op* op, int path, int waves);
 /* IEvaluatorLinear.ComputeMatrix (F/DifferentialOperator.cs:1787-1860):
  * assemble the operator matrix on the device and return the affine vector
  * (N values, host or device per flags; may be NULL). */

@@ -86,7 +86,8 @@ MFMA64_CASES = [(2, 2), (2, 3), (3, 2), (3, 3)]
 
 @pytest.mark.parametrize("D,k", MFMA64_CASES)
 @pytest.mark.parametrize("affine", [False, True])
-@pytest.mark.parametrize("path,waves,per_xcd", [("vector", 8, 64), ("modal", 8, 64), ("mfma", 4, 1), ("mfma", 8, 64)])
+@pytest.mark.parametrize("path,waves,per_xcd", [("vector", 8, 64), ("modal", 8, 64), ("mfma", 4, 1), ("mfma", 8, 64),
+                                               ("tiled", 2, 0), ("tiled", 4, 0), ("tiled", 8, 0)])
 def test_evaluate_fp64_paths_match_oracle(ctx, D, k, affine, path, waves, per_xcd):
     """Both FP64 kernels (vector and FP64-MFMA, each pinned, several launch
     shapes) within 1e-12 of the oracle, with Neumann sides, the affine part and
@@ -119,7 +120,7 @@ def test_evaluate_fp64_autotuned_ragged(ctx):
     auto = go.Evaluate(1.0, 0.0, np.zeros(go.N), u, affine=False)
     path, waves, per_xcd, ms_v, ms_m = go.fp64_tuning()
     _say(f"  tuned: {path} waves={waves} per_xcd={per_xcd} vector {ms_v:.4f} ms, mfma {ms_m:.4f} ms")
-    assert path in ("vector", "modal", "mfma") and ms_v > 0 and ms_m > 0
+    assert path in ("vector", "modal", "tiled", "mfma") and ms_v > 0 and ms_m > 0
     go.force_fp64_path("vector")
     vec = go.Evaluate(1.0, 0.0, np.zeros(go.N), u, affine=False)
     assert rel(auto, vec) < 1e-13
@@ -159,6 +160,47 @@ def test_evaluate_modal_faces(ctx, D, k):
     go.Evaluate(1.0, 0.0, yd, ud, affine=False, prec=B.BH_FP32)
     ref = go.Evaluate(1.0, 0.0, np.zeros(go.N), u.astype(np.float32).astype(np.float64), affine=False)
     assert rel(yd.download().astype(np.float64), ref) < 1e-5
+
+
+@pytest.mark.parametrize("D,k", CASES)
+def test_evaluate_tiled_long_rows(ctx, D, k):
+    """The tiled modal kernel (neighbour traces from the adjacent lanes and, through
+    LDS, from the adjacent waves) on ragged stretched grids whose rows of consecutive
+    cells are longer than one wave (70 and 133 cells: the cells just before and after
+    a wave come from the halo lanes) and whose row count is not a multiple of the
+    waves per workgroup: FP64 equals the node-by-node kernel to 1e-13 and the oracle to
+    1e-12 for 2, 4 and 8 waves per workgroup, with Neumann sides, the affine part and
+    Evaluate(alpha, beta); the FP32 tiled apply is within 1e-5 of FP64."""
+    _say(f"case evaluate tiled long rows D={D} k={k}")
+    rng = np.random.default_rng(120 + 10 * D + k)
+    cells = (11, 133) if D == 2 else (70, 9, 3)
+    nodes = [np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n)]) for n in cells]
+    og = O.Grid(*nodes)
+    nu = log_uniform_nu(og.J, 121)
+    gg = B.Grid(ctx, *nodes)
+    bc = [1, 0] * D
+    oo = O.Sip(og, k, nu, bc=bc)
+    go = B.SipOperator(gg, k, nu, bc=bc)
+    gv = _bnd_values(oo, 122)
+    oo.set_bnd_values(gv)
+    go.set_bnd_values(gv)
+    u = rng.standard_normal(oo.N)
+    y0 = rng.standard_normal(oo.N)
+    ref = oo.evaluate(u, alpha=-1.5, beta=0.25, y=y0.copy())
+    go.force_fp64_path("vector")
+    node = go.Evaluate(-1.5, 0.25, y0.copy(), u)
+    for w in (2, 4, 8):
+        go.force_fp64_path("tiled", waves=w)
+        tiled = go.Evaluate(-1.5, 0.25, y0.copy(), u)
+        assert go.fp64_tuning()[:2] == ("tiled", w)
+        assert rel(tiled, node) < 1e-13, w
+        assert rel(tiled, ref) < 1e-12, w
+        go.force_fp32_path("tiled", waves=w)
+        ud = B.DeviceArray.from_host(ctx, u.astype(np.float32))
+        yd = B.DeviceArray(ctx, go.N, np.float32)
+        go.Evaluate(1.0, 0.0, yd, ud, affine=False, prec=B.BH_FP32)
+        ref32 = go.Evaluate(1.0, 0.0, np.zeros(go.N), u.astype(np.float32).astype(np.float64), affine=False)
+        assert rel(yd.download().astype(np.float64), ref32) < 1e-5, w
 
 
 @pytest.mark.parametrize("D,k", [(2, 2), (3, 3)])
