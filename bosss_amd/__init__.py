@@ -45,6 +45,7 @@ SIGNATURES = {
     "bh_free": (_c_i, [_c_p, _c_p]),
     "bh_memcpy_h2d": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
     "bh_memcpy_d2h": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
+    "bh_dot": (_c_i, [_c_p, _c_i64, _c_p, _c_p, ctypes.POINTER(_c_d)]),
     "bh_memcpy_d2d": (_c_i, [_c_p, _c_p, _c_p, ctypes.c_size_t]),
     "bh_event_record": (_c_i, [_c_p, _c_i]),
     "bh_event_elapsed": (_c_i, [_c_p, _c_i, _c_i, ctypes.POINTER(_c_f)]),
@@ -84,6 +85,9 @@ SIGNATURES = {
     "bh_mlp_gradient_noalloc": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_p]),
     "bh_mlp_adam_step": (_c_i, [_c_p, _c_f]),
     "bh_mlp_features": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
+    "bh_mlp_backward": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_i]),
+    "bh_mlp_correction": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
+    "bh_mlp_upstream": (_c_i, [_c_p, _c_p, _c_p, _c_d, _c_p, _c_i, _c_i]),
     "bh_mlp_targets": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
     "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
@@ -396,6 +400,7 @@ class SipOperator(_Owned):
         self.grid = grid
         self.ctx = grid.ctx
         nu = np.ascontiguousarray(np.broadcast_to(nu, (grid.J,)), np.float64)
+        self.nu = nu
         bc = np.ascontiguousarray(bc if bc is not None else np.zeros(2 * grid.D), np.int32)
         h = ctypes.c_void_p()
         _check(lib().bh_sip_create(grid.h, k, penalty_base, nu.ctypes.data, bc.ctypes.data, ctypes.byref(h)))
@@ -590,6 +595,20 @@ class CellMLP(_Owned):
         _check(lib().bh_mlp_gradient_noalloc(self.h, n, X.ptr, T.ptr, grad.ptr if grad is not None else None,
                                              loss.ptr if loss is not None else None))
 
+    def backward(self, X, dY):
+        """bh_mlp_backward: parameter gradient from an upstream gradient dL/dY.
+        Device arrays: the gradient stays in the network for adam_step (returns
+        None); host arrays: also returned."""
+        if isinstance(X, (DeviceArray, _DeviceView)):
+            n = X.n // self.in_p
+            _check(lib().bh_mlp_backward(self.h, n, X.ptr, dY.ptr, None, BH_DEVICE_PTR))
+            return None
+        X = np.ascontiguousarray(X, np.float32)
+        dY = np.ascontiguousarray(dY, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        _check(lib().bh_mlp_backward(self.h, X.shape[0], X.ctypes.data, dY.ctypes.data, g.ctypes.data, 0))
+        return g
+
     def adam_step(self, lr):
         _check(lib().bh_mlp_adam_step(self.h, lr))
 
@@ -597,6 +616,23 @@ class CellMLP(_Owned):
 def mlp_features(op, r, X, s, in_p):
     """device feature map of a residual (all DeviceArrays)."""
     _check(lib().bh_mlp_features(op.h, r.ptr, X.ptr, s.ptr, in_p, BH_DEVICE_PTR))
+
+
+def dot(ctx, x, y):
+    """x . y of two FP64 device arrays (bh_dot)."""
+    out = _c_d()
+    _check(lib().bh_dot(ctx.h, x.n, x.ptr, y.ptr, ctypes.byref(out)))
+    return out.value
+
+
+def mlp_correction(op, s, Y, e, out_p):
+    """device correction e = s h^2 / nu * Y (all DeviceArrays)."""
+    _check(lib().bh_mlp_correction(op.h, s.ptr, Y.ptr, e.ptr, out_p, BH_DEVICE_PTR))
+
+
+def mlp_upstream(op, s, g, c, dY, out_p):
+    """device upstream gradient dY = c s h^2 / nu * g (all DeviceArrays)."""
+    _check(lib().bh_mlp_upstream(op.h, s.ptr, g.ptr, float(c), dY.ptr, out_p, BH_DEVICE_PTR))
 
 
 def mlp_targets(op, s, e, T, out_p):

@@ -235,6 +235,12 @@ int bh_memcpy_d2d(bh_ctx* c, void* dst, const void* src, size_t bytes) {
     copy_bytes(c, dst, src, bytes);
     BH_API_END
 }
+int bh_dot(bh_ctx* c, int64_t n, const double* x, const double* y, double* result) {
+    BH_API_BEGIN
+    BH_CHECK(c && x && y && result && n >= 0, "bh_dot: null argument or negative length");
+    multi_dot(c, x, y, n, 1, n, result);   // deterministic two-pass reduction, synchronises
+    BH_API_END
+}
 int bh_event_record(bh_ctx* c, int slot) {
     BH_API_BEGIN
     BH_CHECK(slot >= 0 && slot < 8, "event slot out of range");
@@ -734,7 +740,7 @@ int bh_mlp_adam_step(bh_mlp* w, float lr) {
 }
 int bh_mlp_features(bh_op* op, const double* r, float* X, float* s, int in_p, int flags) {
     BH_API_BEGIN
-    BH_CHECK(in_p >= op->Np + 2 * op->D, "bh_mlp_features: in_p too small");
+    BH_CHECK(in_p >= op->Np + 4 * op->D, "bh_mlp_features: in_p too small (Np + 4 D features)");
     const bool dev = flags & BH_DEVICE_PTR;
     const int J = op->grid->J;
     Staged sr(op->ctx, r, op->N * sizeof(double), dev, true);
@@ -756,6 +762,47 @@ int bh_mlp_targets(bh_op* op, const float* s, const double* e, float* T, int out
     Staged st(op->ctx, T, (size_t)J * out_p * sizeof(float), dev, false);
     mlp_targets(op, (const float*)ss.dptr, (const double*)se.dptr, out_p, (float*)st.dptr);
     st.download(T, (size_t)J * out_p * sizeof(float), dev);
+    if (!dev) op->ctx->sync();
+    BH_API_END
+}
+int bh_mlp_backward(bh_mlp* w, int n, const float* X, const float* dY, float* grad, int flags) {
+    BH_API_BEGIN
+    Mlp* m = w->m;
+    BH_CHECK(n >= 1, "bh_mlp_backward: empty batch");
+    const bool dev = flags & BH_DEVICE_PTR;
+    Staged sx(m->ctx, X, (size_t)n * m->in_p * sizeof(float), dev, true);
+    Staged sd(m->ctx, dY, (size_t)n * m->out_p * sizeof(float), dev, true);
+    m->backward(n, (const float*)sx.dptr, (const float*)sd.dptr);
+    if (grad) {
+        if (dev) BH_HIP(hipMemcpyAsync(grad, m->grad.p, m->nw * sizeof(float), hipMemcpyDeviceToDevice, m->ctx->stream));
+        else m->ctx->download(grad, m->grad.p, m->nw * sizeof(float));
+    }
+    if (!dev) m->ctx->sync();
+    BH_API_END
+}
+int bh_mlp_correction(bh_op* op, const float* s, const float* Y, double* e, int out_p, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(out_p >= op->Np && (out_p == 16 || out_p == 32), "bh_mlp_correction: out_p must be 16 or 32 and >= Np");
+    const bool dev = flags & BH_DEVICE_PTR;
+    const int J = op->grid->J;
+    Staged ss(op->ctx, s, (size_t)J * sizeof(float), dev, true);
+    Staged sy(op->ctx, Y, (size_t)J * out_p * sizeof(float), dev, true);
+    Staged se(op->ctx, e, op->N * sizeof(double), dev, false);
+    mlp_correction(op, (const float*)ss.dptr, (const float*)sy.dptr, out_p, se.dptr, BH_FP64);
+    se.download(e, op->N * sizeof(double), dev);
+    if (!dev) op->ctx->sync();
+    BH_API_END
+}
+int bh_mlp_upstream(bh_op* op, const float* s, const double* g, double c, float* dY, int out_p, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(out_p >= op->Np, "bh_mlp_upstream: out_p too small");
+    const bool dev = flags & BH_DEVICE_PTR;
+    const int J = op->grid->J;
+    Staged ss(op->ctx, s, (size_t)J * sizeof(float), dev, true);
+    Staged sg(op->ctx, g, op->N * sizeof(double), dev, true);
+    Staged sd(op->ctx, dY, (size_t)J * out_p * sizeof(float), dev, false);
+    mlp_upstream(op, (const float*)ss.dptr, (const double*)sg.dptr, c, out_p, (float*)sd.dptr);
+    sd.download(dY, (size_t)J * out_p * sizeof(float), dev);
     if (!dev) op->ctx->sync();
     BH_API_END
 }
@@ -787,7 +834,18 @@ int bh_pmg_learned_info(bh_solver* s, int* active, double* mean_gain) {
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
     BH_CHECK(mg, "bh_pmg_learned_info: solver is not a multigrid preconditioner");
     if (active) *active = (mg->learned && !mg->learned_off) ? 1 : 0;
-    if (mean_gain) *mean_gain = mg->learned_mean_gain;
+    if (mean_gain) {
+        // mean over the probe window once it is closed, else over the calls so far
+        *mean_gain = mg->learned_mean_gain;
+        if (mg->learned && mg->lstat.p && (mg->learned_probe == 0 || mg->learned_calls < mg->learned_probe)) {
+            double st[8];
+            mg->ctx->download(st, mg->lstat.p, sizeof(st));
+            int c[2];
+            std::memcpy(c, &st[7], sizeof(c));
+            const int calls = c[0] + c[1];
+            *mean_gain = calls > 0 ? st[6] / calls : 0.0;
+        }
+    }
     BH_API_END
 }
 

@@ -576,19 +576,46 @@ __global__ void kcycle_lsq_k(double* __restrict__ S, KcycleScalars L) {
 // x += alpha e only when alpha > 0 and finite.  The fraction of |r'|^2 the step
 // removes, <q, r'>^2 / (<q, q> <r', r'>), is summed into st[3]; counters
 // cnt[0] (taken) / cnt[1] (declined).
+// The learned step as the reference's orthonormalisation multigrid adds a correction
+// (AS/OrthonormalizationMultigrid.cs:149-300) with a history of one: the cycle's
+// correction z1 (x on entry, A z1 = q1) and the network's e (A e = q2) are combined
+// as x = g1 z1 + g2 e minimising |b - A x|, i.e. A e orthonormalised against A z1
+// and both added with their optimal coefficients.  st: [q1.q1, q1.q2, q2.q2, b.q1,
+// b.q2, b.b, sum of gains, counters].  The gain is the fraction of the optimally
+// scaled cycle's residual |r|^2 that e removes in addition.  When A e is (nearly)
+// in the span of A z1 (|q2 - proj q2| <= 1e-5 |q2|, the reference's cancellation
+// test) or nothing is gained, x stays the cycle's result.
 template <typename T>
-__global__ __launch_bounds__(256) void guarded_step_k(const T* __restrict__ e, T* __restrict__ x, int64_t n,
-                                                      double* __restrict__ st, int* __restrict__ cnt) {
-    const double qr = st[0], qq = st[1], rr = st[2];
-    const double alpha = qq > 0 ? qr / qq : 0.0;
-    const bool take = alpha > 0 && alpha < 1e300;
+__global__ __launch_bounds__(256) void orth2_step_k(const T* __restrict__ e, T* __restrict__ x, int64_t n,
+                                                    double* __restrict__ st, int* __restrict__ cnt) {
+    const double g11 = st[0], g12 = st[1], g22 = st[2], f1 = st[3], f2 = st[4], bb = st[5];
+    bool take = g11 > 0 && g22 > 0 && isfinite(g11) && isfinite(g22) && isfinite(g12);
+    double c1 = 1.0, c2 = 0.0, gain = 0.0;
+    if (take) {
+        const double perp = g22 - g12 * g12 / g11;   // |q2 - proj_q1 q2|^2
+        take = perp > 1e-10 * g22;
+        if (take) {
+            c2 = (f2 - g12 * f1 / g11) / perp;
+            c1 = (f1 - g12 * c2) / g11;
+            const double r_mg = bb - f1 * f1 / g11;  // residual^2 of the best multiple of z1
+            const double r_new = bb - (c1 * f1 + c2 * f2);
+            gain = r_mg > 0 ? 1.0 - r_new / r_mg : 0.0;
+            take = gain > 0 && isfinite(c1) && isfinite(c2);
+        }
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&cnt[take ? 0 : 1], 1);
-        if (take && rr > 0) st[3] += qr * qr / (qq * rr);
+        if (take) st[6] += gain;
     }
     if (!take) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] = (T)((double)x[i] + alpha * (double)e[i]);
+        x[i] = (T)(c1 * (double)x[i] + c2 * (double)e[i]);
+}
+
+// q1 = b - r (the image A z1 of the cycle's correction, from its residual)
+template <typename T>
+__global__ __launch_bounds__(256) void sub_k(const T* __restrict__ b, const T* __restrict__ r, T* __restrict__ q, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) q[i] = b[i] - r[i];
 }
 
 // ------------------------------------------------------------------ launchers
@@ -621,8 +648,13 @@ void kcycle_lsq(bh_ctx* ctx, double* S, const KcycleScalars& L) {
     BH_HIP(hipGetLastError());
 }
 template <typename T>
-void guarded_step(bh_ctx* ctx, const T* e, T* x, int64_t n, double* st, int* cnt) {
-    hipLaunchKernelGGL((guarded_step_k<T>), grid1d(n), dim3(256), 0, ctx->stream, e, x, n, st, cnt);
+void orth2_step(bh_ctx* ctx, const T* e, T* x, int64_t n, double* st, int* cnt) {
+    hipLaunchKernelGGL((orth2_step_k<T>), grid1d(n), dim3(256), 0, ctx->stream, e, x, n, st, cnt);
+    BH_HIP(hipGetLastError());
+}
+template <typename T>
+void sub_vec(bh_ctx* ctx, const T* b, const T* r, T* q, int64_t n) {
+    hipLaunchKernelGGL((sub_k<T>), grid1d(n), dim3(256), 0, ctx->stream, b, r, q, n);
     BH_HIP(hipGetLastError());
 }
 void galerkin_gather(bh_ctx* ctx, int64_t nq, int bs, const int64_t* qptr, const int* plist, const int* prow,
@@ -794,7 +826,8 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void vcopy<T>(bh_ctx*, const T*, T*, int64_t);                                          \
     template void scale_inv_norm<T>(bh_ctx*, const T*, T*, int64_t, const double*, double);           \
     template void schwarz_apply<T>(bh_ctx*, int, int, int, const int*, const int*, const T*, const T*, T*, double, bool); \
-    template void guarded_step<T>(bh_ctx*, const T*, T*, int64_t, double*, int*);
+    template void orth2_step<T>(bh_ctx*, const T*, T*, int64_t, double*, int*); \
+    template void sub_vec<T>(bh_ctx*, const T*, const T*, T*, int64_t);
 BH_INST(float)
 BH_INST(double)
 #undef BH_INST

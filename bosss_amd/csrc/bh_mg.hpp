@@ -63,7 +63,8 @@ void invert_dense_blocks(bh_ctx*, int nblocks, int n, const double* blocks, doub
 void dense_inverse_dev(bh_ctx*, double* A, double* Inv, int n);
 void kcycle_column(bh_ctx*, double* S, const KcycleScalars& L, int j);
 void kcycle_lsq(bh_ctx*, double* S, const KcycleScalars& L);
-template <typename T> void guarded_step(bh_ctx*, const T* e, T* x, int64_t n, double* st, int* cnt);
+template <typename T> void orth2_step(bh_ctx*, const T* e, T* x, int64_t n, double* st, int* cnt);
+template <typename T> void sub_vec(bh_ctx*, const T* b, const T* r, T* q, int64_t n);
 
 struct MgLevel {
     int J = 0, bs = 1;
@@ -112,11 +113,12 @@ struct PMultigrid : bh_solver {
     DevBuf Ainv32;                 // its FP32 copy (FP32 cycle)
     DevBuf in32, out32;
     bh_bsr* ownedAf = nullptr;
-    // learned correction after the cycle (guarded by residual minimisation)
+    // learned correction after the cycle: residual minimisation over the cycle's and the
+    // network's corrections (the orthonormalisation multigrid's step with a history of one)
     struct Mlp* learned = nullptr;
     bh_op* fine_op = nullptr;
-    DevBuf lX, ls, lY, le, lrq;     // features, scales, outputs, correction, [r' | A e]
-    DevBuf lstat;                   // <q, r'>, <q, q>, <r', r'>, sum of removed |r'|^2 fractions, int counters
+    DevBuf lX, ls, lY, le, lrq;     // features, scales, outputs, correction, [r' | A z1 | A e]
+    DevBuf lstat;                   // Gram entries and right-hand sides, sum of gains, int counters
     void learned_stats(int* acc, int* rej);
     // fallback: after `learned_probe` preconditioner calls, a mean removed fraction of
     // |r'|^2 below `learned_min_gain` switches the learned step off (plain multigrid)

@@ -653,42 +653,48 @@ void PMultigrid::attach_learned(Mlp* m) {
     drop_graphs();   // the captured cycle includes (or not) the learned step
     if (!m) return;
     BH_CHECK(fine_op, "learned correction needs the fine-level operator");
-    BH_CHECK(m->out == fine_op->Np && m->in == fine_op->Np + 2 * fine_op->D, "learned correction: network widths do not match the operator");
+    BH_CHECK(m->out == fine_op->Np && m->in == fine_op->Np + 4 * fine_op->D,
+             "learned correction: network widths do not match the operator (Np + 4 D inputs, Np outputs)");
     const int J = fine_op->grid->J;
     const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
     lX.alloc((size_t)J * m->in_p * sizeof(float));
     ls.alloc((size_t)J * sizeof(float));
     lY.alloc((size_t)J * m->out_p * sizeof(float));
     le.alloc(N * es);
-    lrq.alloc(2 * N * es);
-    lstat.alloc(6 * sizeof(double));
-    fill(ctx, lstat.as<double>(), 0.0, 6);   // also zeroes the two int counters in the fifth double
+    lrq.alloc(3 * N * es);
+    lstat.alloc(8 * sizeof(double));
+    fill(ctx, lstat.as<double>(), 0.0, 8);   // also zeroes the two int counters in the eighth double
     learned_calls = 0;
     learned_off = false;
     learned_mean_gain = 0.0;
 }
 
-// After the cycle (x = x_mg in lv[0].x): r' = b - A x_mg, e = network(r'),
-// q = A e, alpha = <q, r'> / <q, q>.  The step x += alpha e minimises the
-// residual along e; it is taken only when alpha > 0 (the network's direction
-// agrees with the residual), otherwise the plain multigrid result stands.
+// After the cycle (x = z1, the cycle's correction from zero, in lv[0].x):
+// r' = b - A z1, q1 = A z1 = b - r', e = network(features of r'), q2 = A e; then
+// x = g1 z1 + g2 e with the residual-minimising g (orth2_step).  The network sees
+// the residual the cycle leaves, and only a direction that reduces it is added.
 template <typename T>
 void PMultigrid::learned_correction() {
     MgLevel& L = lv[0];
     const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
     T* rq = vec<T>(lrq);
     T* rp = rq;
-    T* q = rq + N;
+    T* q1 = rq + N;
+    T* q2 = rq + 2 * N;
     T* e = vec<T>(le);
-    residual<T>(0, vec<T>(L.b), vec<T>(L.x), rp);
+    const T* b = vec<T>(L.b);
+    residual<T>(0, b, vec<T>(L.x), rp);
+    sub_vec<T>(ctx, b, rp, q1, N);
     mlp_features(fine_op, rp, p, lX.as<float>(), ls.as<float>(), learned->in_p);
     learned->infer(fine_op->grid->J, lX.as<float>(), lY.as<float>());
     mlp_correction(fine_op, ls.as<float>(), lY.as<float>(), learned->out_p, e, p);
-    sip_apply(fine_op, 1.0, e, 0.0, q, false, op_prec(0, p == BH_FP32));
+    sip_apply(fine_op, 1.0, e, 0.0, q2, false, op_prec(0, p == BH_FP32));
     double* st = lstat.as<double>();
-    multi_dot_to_dev<T>(ctx, q, rq, N, 2, N, st);        // <q, r'>, <q, q>
-    multi_dot_to_dev<T>(ctx, rp, rp, 0, 1, N, st + 2);   // <r', r'>
-    guarded_step<T>(ctx, e, vec<T>(L.x), N, st, (int*)(st + 4));
+    multi_dot_to_dev<T>(ctx, q1, q1, N, 2, N, st);       // q1.q1, q1.q2
+    multi_dot_to_dev<T>(ctx, q2, q2, 0, 1, N, st + 2);   // q2.q2
+    multi_dot_to_dev<T>(ctx, b, q1, N, 2, N, st + 3);    // b.q1, b.q2
+    multi_dot_to_dev<T>(ctx, b, b, 0, 1, N, st + 5);     // b.b
+    orth2_step<T>(ctx, e, vec<T>(L.x), N, st, (int*)(st + 7));
 }
 
 // The cycle from the finest level, the K-cycle inside it and the guarded learned
@@ -716,9 +722,9 @@ void PMultigrid::precondition(double* x, const double* b) {
     }
     iterations++;
     if (learned && !learned_off && learned_probe > 0 && ++learned_calls == learned_probe) {
-        double st[6];
+        double st[8];
         ctx->download(st, lstat.p, sizeof(st));
-        learned_mean_gain = st[3] / learned_probe;
+        learned_mean_gain = st[6] / learned_probe;
         if (!(learned_mean_gain >= learned_min_gain)) {
             learned_off = true;   // the plain multigrid takes over
             drop_graphs();
@@ -728,7 +734,7 @@ void PMultigrid::precondition(double* x, const double* b) {
 
 void PMultigrid::learned_stats(int* acc, int* rej) {
     int c[2] = {0, 0};
-    if (learned && lstat.p) ctx->download(c, lstat.as<double>() + 4, sizeof(c));
+    if (learned && lstat.p) ctx->download(c, lstat.as<double>() + 7, sizeof(c));
     if (acc) *acc = c[0];
     if (rej) *rej = c[1];
 }

@@ -7,10 +7,12 @@
 // multigrid (the acceptance rule of OrthonormalizationMultigrid.AddSol /
 // MinimizeResidual, AS/OrthonormalizationMultigrid.cs:149-215).
 //
-// Feature map per cell j (k-th mode residual r_jk, diffusion nu):
-//   s_j = ||r_j|| + tiny,  x_j = [ r_j / s_j , log10(nu_nb / nu_j) for the 2D
-//   face neighbours (boundary: +2) , 0-padding ]
-// Output y_j maps to the correction e_j = s_j * (h_j^2 / nu_j) * y_j.
+// Feature map per cell j over its face adjacency (k-th mode residual r_jk,
+// mean residual mode r_n0 of the 2D face neighbours n, diffusion nu):
+//   s_j = sqrt(||r_j||^2 + sum_n r_n0^2) + tiny,
+//   x_j = [ r_j / s_j , r_n0 / s_j for the 2D face neighbours (boundary: 0) ,
+//           log10(nu_n / nu_j) for the same neighbours (boundary: +2) , 0-padding ]
+// (Np + 4D inputs).  Output y_j maps to the correction e_j = s_j * (h_j^2 / nu_j) * y_j.
 #include <cmath>
 #include <mutex>
 #include <string>
@@ -44,30 +46,33 @@ __global__ __launch_bounds__(256) void mlp_features_k(int J, int Np, int n0, int
                                                       float* __restrict__ s) {
     // one segment of in_p lanes (16 or 32) per cell: lane t of the segment owns
     // feature slot t (coalesced reads of the residual row and writes of the
-    // feature row); the residual norm is a butterfly inside the segment
+    // feature row); the patch norm is a butterfly inside the segment
     const int segs = 256 / in_p;
     const int t = threadIdx.x & (in_p - 1);
+    const int nn[3] = {n0, n1, n2};
     for (int j = blockIdx.x * segs + threadIdx.x / in_p; j < J; j += gridDim.x * segs) {
-        const double v = t < Np ? (double)r[(size_t)j * Np + t] : 0.0;
+        int q[3];
+        if (D == 2) { q[0] = j / n1; q[1] = j - q[0] * n1; q[2] = 0; }
+        else { const int sl = n0 * n1; q[2] = j / sl; const int rr = j - q[2] * sl; q[1] = rr / n0; q[0] = rr - q[1] * n0; }
+        // the neighbour across face f = 2d + side of slots Np + f and Np + 2D + f (-1: boundary)
+        int nbc = -1;
+        if (t >= Np && t < Np + 4 * D) {
+            const int f = (t - Np) % (2 * D), d = f >> 1, side = f & 1;
+            int qq[3] = {q[0], q[1], q[2]};
+            qq[d] += side ? 1 : -1;
+            if (qq[d] >= 0 && qq[d] < nn[d]) nbc = D == 2 ? qq[0] * n1 + qq[1] : qq[0] + qq[1] * n0 + qq[2] * n0 * n1;
+        }
+        double v = 0.0;
+        if (t < Np) v = (double)r[(size_t)j * Np + t];
+        else if (t < Np + 2 * D && nbc >= 0) v = (double)r[(size_t)nbc * Np];
         double sq = v * v;
         for (int off = in_p >> 1; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
         const double sc = sqrt(sq) + 1e-30;
         float out = 0.f;
-        if (t < Np) {
+        if (t < Np + 2 * D) {
             out = (float)(v / sc);
-        } else if (t < Np + 2 * D) {
-            const int f = t - Np, d = f >> 1, side = f & 1;
-            int q[3];
-            if (D == 2) { q[0] = j / n1; q[1] = j - q[0] * n1; q[2] = 0; }
-            else { const int sl = n0 * n1; q[2] = j / sl; const int rr = j - q[2] * sl; q[1] = rr / n0; q[0] = rr - q[1] * n0; }
-            const int nn[3] = {n0, n1, n2};
-            q[d] += side ? 1 : -1;
-            if (q[d] < 0 || q[d] >= nn[d]) {
-                out = 2.0f;
-            } else {
-                const int c = D == 2 ? q[0] * n1 + q[1] : q[0] + q[1] * n0 + q[2] * n0 * n1;
-                out = log10f((float)nu[c]) - log10f((float)nu[j]);
-            }
+        } else if (t < Np + 4 * D) {
+            out = nbc < 0 ? 2.0f : log10f((float)nu[nbc]) - log10f((float)nu[j]);
         }
         X[(size_t)j * in_p + t] = out;
         if (t == 0) s[j] = (float)sc;
@@ -108,6 +113,26 @@ __global__ void mlp_targets_k(int J, int Np, int n0, int n1, const double* __res
         for (int d = 0; d < D; ++d) vol *= width[d * nmax + idx[d]];
         const double h2 = pow(vol, 2.0 / D);
         Tg[t] = (float)(e[(size_t)j * Np + m] / ((double)s[j] * h2 / nu[j]));
+    }
+}
+
+// upstream gradient of a loss L(e) through the correction map e_j = sigma_j y_j
+// (sigma_j = s_j h_j^2 / nu_j): dY[j][m] = c * sigma_j * g[j][m], with g = dL/de / c
+template <int D>
+__global__ void mlp_upstream_k(int J, int Np, int n0, int n1, const double* __restrict__ width, int nmax,
+                               const double* __restrict__ nu, const float* __restrict__ s, const double* __restrict__ g,
+                               double c, int out_p, float* __restrict__ dY) {
+    const int64_t tot = (int64_t)J * out_p;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(t / out_p), m = (int)(t - (int64_t)j * out_p);
+        if (m >= Np) { dY[t] = 0.f; continue; }
+        int idx[3];
+        if (D == 2) { idx[0] = j / n1; idx[1] = j - idx[0] * n1; idx[2] = 0; }
+        else { int sl = n0 * n1; idx[2] = j / sl; int rr = j - idx[2] * sl; idx[1] = rr / n0; idx[0] = rr - idx[1] * n0; }
+        double vol = 1;
+        for (int d = 0; d < D; ++d) vol *= width[d * nmax + idx[d]];
+        const double h2 = pow(vol, 2.0 / D);
+        dY[t] = (float)(c * (double)s[j] * h2 / nu[j] * g[(size_t)j * Np + m]);
     }
 }
 
@@ -231,7 +256,10 @@ __global__ __launch_bounds__(512) void mlp_infer_k(int J, const float* __restric
 // written as per-block partials (reduced by mlp_reduce_k), so there are no
 // atomics.  Weights used in the forward are the BF16-rounded values (as in
 // inference); the activations stay FP32.
-template <int INP, int H, int OUTP>
+// UPSTREAM: Tg holds dLoss/dy per cell (the loss is defined outside the network,
+// e.g. the residual after the correction); the kernel back-propagates it and
+// reports no loss.  Otherwise Tg holds targets of the mean-squared error.
+template <int INP, int H, int OUTP, bool UPSTREAM>
 __global__ __launch_bounds__(H) void mlp_train_k(int B, const int* __restrict__ idx, const float* __restrict__ X,
                                                  const float* __restrict__ Tg, int nout, const float* __restrict__ W1,
                                                  const float* __restrict__ b1, const float* __restrict__ W2,
@@ -268,10 +296,14 @@ __global__ __launch_bounds__(H) void mlp_train_k(int B, const int* __restrict__ 
             float o = b3[u];
             for (int i = 0; i < H; ++i) o += bf16_to_f32(f32_to_bf16(W3[u * H + i])) * h2[i];
             y[u] = o;
-            const float t = u < nout ? Tg[(size_t)c * OUTP + u] : o;
-            const float diff = o - t;
-            dy[u] = 2.f * diff / (float)(B * nout);
-            lacc += diff * diff;
+            if (UPSTREAM) {
+                dy[u] = u < nout ? Tg[(size_t)c * OUTP + u] : 0.f;
+            } else {
+                const float t = u < nout ? Tg[(size_t)c * OUTP + u] : o;
+                const float diff = o - t;
+                dy[u] = 2.f * diff / (float)(B * nout);
+                lacc += diff * diff;
+            }
         }
         __syncthreads();
         // backward
@@ -411,12 +443,19 @@ void Mlp::infer(int J, const float* X, float* Y) {
 // One training pass: mlp_train_k writes every entry of the nb per-block gradient
 // partials and the nb loss partials, and mlp_reduce_k sums the partials in block
 // order into grad (no atomics, so results do not depend on scheduling).
-void Mlp::launch_gradient(int B, int nb, const int* idx, const float* X, const float* Tg, float* part, float* lossp) {
+void Mlp::launch_gradient(int B, int nb, const int* idx, const float* X, const float* Tg, float* part, float* lossp,
+                          bool upstream) {
     const float* P = params.as<float>();
 #define BH_CASE(I, O)                                                                                                 \
     if (in_p == I && out_p == O) {                                                                                    \
-        hipLaunchKernelGGL((mlp_train_k<I, 128, O>), dim3(nb), dim3(128), 0, ctx->stream, B, idx, X, Tg, out, P + off_W1(), \
-                           P + off_b1(), P + off_W2(), P + off_b2(), P + off_W3(), P + off_b3(), part, lossp);        \
+        if (upstream)                                                                                                 \
+            hipLaunchKernelGGL((mlp_train_k<I, 128, O, true>), dim3(nb), dim3(128), 0, ctx->stream, B, idx, X, Tg, out,  \
+                               P + off_W1(), P + off_b1(), P + off_W2(), P + off_b2(), P + off_W3(), P + off_b3(), part,  \
+                               lossp);                                                                                \
+        else                                                                                                          \
+            hipLaunchKernelGGL((mlp_train_k<I, 128, O, false>), dim3(nb), dim3(128), 0, ctx->stream, B, idx, X, Tg, out, \
+                               P + off_W1(), P + off_b1(), P + off_W2(), P + off_b2(), P + off_W3(), P + off_b3(), part,  \
+                               lossp);                                                                                \
     } else
     BH_CASE(16, 16) BH_CASE(16, 32) BH_CASE(32, 16) BH_CASE(32, 32) { throw Error(-3, "MLP: unsupported widths"); }
 #undef BH_CASE
@@ -429,12 +468,22 @@ double Mlp::gradient(int B, const int* idx, const float* X, const float* Tg) {
     const int nb = gradient_blocks(B);
     if (part.bytes < (size_t)nb * nw * sizeof(float)) part.alloc((size_t)256 * nw * sizeof(float));
     if (lossp.bytes < 256 * sizeof(float)) lossp.alloc(256 * sizeof(float));
-    launch_gradient(B, nb, idx, X, Tg, part.as<float>(), lossp.as<float>());
+    launch_gradient(B, nb, idx, X, Tg, part.as<float>(), lossp.as<float>(), false);
     std::vector<float> lp(nb);
     ctx->download(lp.data(), lossp.p, nb * sizeof(float));
     double l = 0;
     for (float v : lp) l += v;
     return l / ((double)B * out);
+}
+
+// Gradient of a loss defined outside the network from its upstream gradient dY
+// [B][out_p] (first `out` columns used), into grad; same kernels and partials as
+// gradient().
+void Mlp::backward(int B, const float* X, const float* dY) {
+    const int nb = gradient_blocks(B);
+    if (part.bytes < (size_t)nb * nw * sizeof(float)) part.alloc((size_t)256 * nw * sizeof(float));
+    if (lossp.bytes < 256 * sizeof(float)) lossp.alloc(256 * sizeof(float));
+    launch_gradient(B, nb, nullptr, X, dY, part.as<float>(), lossp.as<float>(), true);
 }
 
 // ------------------------------------------------------------------ allocation-free gradient
@@ -489,7 +538,7 @@ void Mlp::gradient_noalloc(int B, const float* X, const float* Tg, float* grad_o
     BH_CHECK(base != nullptr, "bh_mlp_gradient_noalloc: the workspace of this GPU is not set up (bh_init)");
     float* part = base;
     float* lossp = base + (size_t)nb * nw;
-    launch_gradient(B, nb, nullptr, X, Tg, part, lossp);
+    launch_gradient(B, nb, nullptr, X, Tg, part, lossp, false);
     if (grad_out)
         BH_HIP(hipMemcpyAsync(grad_out, grad.p, (size_t)nw * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
     if (loss_dev) {
@@ -511,7 +560,8 @@ void Mlp::adam(float lr) {
 void mlp_features(bh_op* op, const void* r, int prec, float* X, float* s, int in_p) {
     bh_grid* g = op->grid;
     const int J = g->J;
-    BH_CHECK((in_p == 16 || in_p == 32) && op->Np + 2 * op->D <= in_p, "mlp_features: input width must be 16 or 32");
+    BH_CHECK((in_p == 16 || in_p == 32) && op->Np + 4 * op->D <= in_p,
+             "mlp_features: input width must be 16 or 32 and hold Np + 4 D features");
     const int segs = 256 / in_p;
     dim3 grd((J + segs - 1) / segs > 8192 ? 8192 : (J + segs - 1) / segs), blk(256);
     if (op->D == 2) {
@@ -536,6 +586,16 @@ void mlp_correction(bh_op* op, const float* s, const float* Y, int out_p, void* 
         if (prec == BH_FP32) hipLaunchKernelGGL((mlp_apply_scale_k<float, 3>), grd, blk, 0, op->ctx->stream, g->J, op->Np, g->n[0], g->n[1], w, g->nmax, op->nu.as<double>(), s, Y, out_p, (float*)e);
         else hipLaunchKernelGGL((mlp_apply_scale_k<double, 3>), grd, blk, 0, op->ctx->stream, g->J, op->Np, g->n[0], g->n[1], w, g->nmax, op->nu.as<double>(), s, Y, out_p, (double*)e);
     }
+    BH_HIP(hipGetLastError());
+}
+
+void mlp_upstream(bh_op* op, const float* s, const double* g, double c, int out_p, float* dY) {
+    bh_grid* gr = op->grid;
+    const int64_t tot = (int64_t)gr->J * out_p;
+    dim3 grd((unsigned)std::min<int64_t>((tot + 255) / 256, 8192)), blk(256);
+    const double* w = gr->d_width.as<double>();
+    if (op->D == 2) hipLaunchKernelGGL((mlp_upstream_k<2>), grd, blk, 0, op->ctx->stream, gr->J, op->Np, gr->n[0], gr->n[1], w, gr->nmax, op->nu.as<double>(), s, g, c, out_p, dY);
+    else hipLaunchKernelGGL((mlp_upstream_k<3>), grd, blk, 0, op->ctx->stream, gr->J, op->Np, gr->n[0], gr->n[1], w, gr->nmax, op->nu.as<double>(), s, g, c, out_p, dY);
     BH_HIP(hipGetLastError());
 }
 

@@ -24,11 +24,14 @@ struct Mlp {
     void refresh_bf16();
     void infer(int J, const float* X, float* Y);                          // Y[J][out_p]
     static int gradient_blocks(int B) { return B < 1 ? 1 : (B < 256 ? B : 256); }
-    void launch_gradient(int B, int nb, const int* idx, const float* X, const float* Tg, float* part, float* lossp);
+    void launch_gradient(int B, int nb, const int* idx, const float* X, const float* Tg, float* part, float* lossp,
+                         bool upstream);
     double gradient(int B, const int* idx, const float* X, const float* Tg);  // -> grad (device), returns loss
     // the same gradient with its partials in the fixed per-GPU workspace; no
     // allocation and no host synchronisation (grad_out, loss_dev: device, may be null)
     void gradient_noalloc(int B, const float* X, const float* Tg, float* grad_out, double* loss_dev);
+    // parameter gradient (-> grad) of a loss defined outside the network, from dL/dY [B][out_p]
+    void backward(int B, const float* X, const float* dY);
     void adam(float lr);
 };
 
@@ -37,6 +40,7 @@ size_t mlp_noalloc_capacity();        // floats in the allocation-free workspace
 void mlp_features(bh_op* op, const void* r, int prec, float* X, float* s, int in_p);
 void mlp_correction(bh_op* op, const float* s, const float* Y, int out_p, void* e, int prec);
 void mlp_targets(bh_op* op, const float* s, const double* e, int out_p, float* Tg);
+void mlp_upstream(bh_op* op, const float* s, const double* g, double c, int out_p, float* dY);
 
 }  // namespace bh
 

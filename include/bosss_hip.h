@@ -58,6 +58,9 @@ int bh_memcpy_d2h(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* device-to-device copy on the context stream by a plain copy kernel (16-byte
  * aligned pointers); also the known-traffic kernel that calibrates HBM counters */
 int bh_memcpy_d2d(bh_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* BLAS.ddot (IL/BLAS.cs:1216-1358) on device vectors: *result = x . y (FP64,
+ * fixed-order reduction, returns after the result is on the host) */
+int bh_dot(bh_ctx* ctx, int64_t n, const double* x, const double* y, double* result);
 /* elapsed device time (ms) of the last `bh_timed_*` region on the context stream */
 int bh_event_record(bh_ctx* ctx, int slot);
 int bh_event_elapsed(bh_ctx* ctx, int slot0, int slot1, float* ms);
@@ -221,20 +224,39 @@ int bh_mlp_gradient(bh_mlp* mlp, int ncells, const float* X, const float* T, flo
 int bh_mlp_gradient_noalloc(bh_mlp* mlp, int ncells, const float* X, const float* T, float* grad, double* loss);
 /* one Adam step with the gradient of the last bh_mlp_gradient(_noalloc) call */
 int bh_mlp_adam_step(bh_mlp* mlp, float lr);
-/* feature map of a residual r (N values, FP64) on op's grid -> X[J][in_p], scales s[J];
+/* parameter gradient of a loss defined outside the network, from its upstream
+ * gradient dY[ncells][out_p] = dLoss/dY (first n_out columns used; FP32 forward
+ * with BF16-rounded weights, as bh_mlp_gradient); grad may be NULL, the gradient
+ * also lands in the network's buffer for bh_mlp_adam_step */
+int bh_mlp_backward(bh_mlp* mlp, int ncells, const float* X, const float* dY, float* grad, int flags);
+/* feature map of a residual r (N values, FP64) on op's grid -> X[J][in_p], scales s[J]:
+ * per cell its Np residual modes, the mean residual mode of its 2D face neighbours
+ * (0 on the boundary), both over the patch norm s, and log10 of the neighbours'
+ * diffusion ratios (2 on the boundary): Np + 4D inputs, in_p 16 or 32;
  * target map of an error e -> T[J][out_p] with the same scales */
 int bh_mlp_features(bh_op* op, const double* r, float* X, float* s, int in_p, int flags);
 int bh_mlp_targets(bh_op* op, const float* s, const double* e, float* T, int out_p, int flags);
+/* correction of network outputs Y[J][out_p]: e = s_j h_j^2 / nu_j * Y (N values, FP64) */
+int bh_mlp_correction(bh_op* op, const float* s, const float* Y, double* e, int out_p, int flags);
+/* upstream gradient through that map: dY[j][m] = c * s_j h_j^2 / nu_j * g[j*Np + m]
+ * (g = dLoss/de / c, N values FP64; columns m >= Np zero) */
+int bh_mlp_upstream(bh_op* op, const float* s, const double* g, double c, float* dY, int out_p, int flags);
 /* attach (mlp != NULL) or detach the learned correction of a bh_pmg_create solver:
- * after each cycle, e = network(r'), and x += alpha e with the residual-minimising
- * alpha = <A e, r'> / <A e, A e> when alpha > 0 (counted as accepted, else rejected) */
+ * after each cycle (correction z1 from zero, residual r' = b - A z1), e = network(r'),
+ * and x = g1 z1 + g2 e with the coefficients minimising |b - A x|: A e orthonormalised
+ * against A z1 and both added with their optimal coefficients, as
+ * OrthonormalizationMultigrid.AddSol / MinimizeResidual do with their history
+ * (AS/OrthonormalizationMultigrid.cs:149-300).  Counted as accepted when e removes
+ * residual beyond the best multiple of z1, else rejected (x = z1, the plain cycle). */
 int bh_pmg_set_learned(bh_solver* pmg, bh_mlp* mlp);
 int bh_pmg_learned_stats(bh_solver* pmg, int* accepted, int* rejected);
 /* No counterpart in the reference.  Fallback of the learned correction: after
  * `probe` preconditioner calls (0: never), a mean removed fraction of |r'|^2 per
  * step below `min_gain` switches the network off and the plain multigrid takes
  * over (defaults 8 and 1e-2).  bh_pmg_learned_info reports whether it is still
- * active and the measured mean gain (0 before the probe window ends). */
+ * active and the mean gain: the fraction of the residual^2 of the best multiple of
+ * the cycle's correction that the network's direction removes (over the probe
+ * window once it is closed, else over the calls so far). */
 int bh_pmg_set_learned_fallback(bh_solver* pmg, int probe, double min_gain);
 /* Schwarz (AS/Schwarz.cs:56-1454) with the multigrid-aggregate blocking and no
  * overlap: on every level that is aggregated (the k = 1 level and the h-levels

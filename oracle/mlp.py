@@ -111,3 +111,32 @@ def grad64(p, X, T, n_out, in_p, out_p):
     dh1 = (dh2 @ W2) * (a1 > 0)
     return np.concatenate([(dh1.T @ X).ravel(), dh1.sum(0), (dh2.T @ h1).ravel(), dh2.sum(0),
                            (dy.T @ h2).ravel(), dy.sum(0)])
+
+
+def backward_upstream(p, X, dY, n_out, in_p, out_p, dtype=np.float32, round_weights=True):
+    """Parameter gradient for a given upstream gradient dY = dL/dy (first n_out
+    columns), as the device's bh_mlp_backward: FP32 forward with BF16-rounded
+    weights, the same back-propagation as loss_grad with dy = dY."""
+    W1, b1, W2, b2, W3, b3 = [a.astype(dtype) for a in unpack(np.asarray(p, np.float32), in_p, out_p)]
+    if round_weights:
+        W1, W2, W3 = (bf16(W).astype(dtype) for W in (W1, W2, W3))
+    X = np.asarray(X, dtype)
+    a1 = X @ W1.T + b1
+    h1 = np.maximum(a1, 0)
+    a2 = h1 @ W2.T + b2
+    h2 = np.maximum(a2, 0)
+    dy = np.zeros((X.shape[0], out_p), dtype)
+    dy[:, :n_out] = np.asarray(dY, dtype)[:, :n_out]
+    dh2 = (dy @ W3) * (a2 > 0)
+    dh1 = (dh2 @ W2) * (a1 > 0)
+    return pack(dh1.T @ X, dh1.sum(0), dh2.T @ h1, dh2.sum(0), dy.T @ h2, dy.sum(0))
+
+
+def linear_loss64(p, X, dY, n_out, in_p, out_p):
+    """float64 L = sum(dY * y) with unrounded weights: its gradient is the
+    back-propagation of the upstream gradient dY (finite-difference reference)."""
+    W1, b1, W2, b2, W3, b3 = [a.astype(np.float64) for a in unpack(np.asarray(p, np.float64), in_p, out_p)]
+    h1 = np.maximum(X @ W1.T + b1, 0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0)
+    y = h2 @ W3.T + b3
+    return float(np.sum(np.asarray(dY, np.float64)[:, :n_out] * y[:, :n_out]))

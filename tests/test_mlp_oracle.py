@@ -55,3 +55,32 @@ def test_fp32_gradient_matches_float64_formulas():
     g64 = M.grad64(p, X.astype(np.float64), T.astype(np.float64), n_out, in_p, out_p)
     assert np.linalg.norm(g32 - g64) <= 1e-5 * np.linalg.norm(g64)
     assert abs(l32 - M.loss64(p, X.astype(np.float64), T.astype(np.float64), n_out, in_p, out_p)) < 1e-6
+
+
+def test_upstream_backward_matches_finite_differences():
+    """The back-propagation of a given upstream gradient dY is the gradient of the
+    linear functional sum(dY * y): central differences of the float64 forward."""
+    n_in, n_out = 22, 10
+    p, in_p, out_p = _init(n_in, n_out, 4)
+    rng = np.random.default_rng(5)
+    X = np.zeros((20, in_p))
+    X[:, :n_in] = rng.standard_normal((20, n_in))
+    dY = np.zeros((20, out_p))
+    dY[:, :n_out] = rng.standard_normal((20, n_out))
+    g = M.backward_upstream(p, X, dY, n_out, in_p, out_p, dtype=np.float64, round_weights=False)
+    idx = rng.choice(p.size, 60, replace=False)
+    eps = 1e-6
+    for i in idx:
+        e = np.zeros(p.size)
+        e[i] = eps
+        fd = (M.linear_loss64(p + e, X, dY, n_out, in_p, out_p) - M.linear_loss64(p - e, X, dY, n_out, in_p, out_p)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd)
+    # with the MSE's upstream gradient 2 (y - T) / (B n_out) it is the MSE gradient
+    T = np.zeros((20, out_p))
+    T[:, :n_out] = rng.standard_normal((20, n_out))
+    W1, b1, W2, b2, W3, b3 = M.unpack(np.asarray(p, np.float64), in_p, out_p)
+    y = np.maximum(np.maximum(X @ W1.T + b1, 0) @ W2.T + b2, 0) @ W3.T + b3
+    dmse = np.zeros_like(y)
+    dmse[:, :n_out] = 2 * (y[:, :n_out] - T[:, :n_out]) / (20 * n_out)
+    g2 = M.backward_upstream(p, X, dmse, n_out, in_p, out_p, dtype=np.float64, round_weights=False)
+    assert np.allclose(g2, M.grad64(p, X, T, n_out, in_p, out_p), rtol=1e-6, atol=1e-9)   # pack() stores FP32
