@@ -436,6 +436,28 @@ def xdg_leg(B, ctx, out, n=48, seed=3):
         res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
                      "hbm_frac": nbytes / (ms * 1e-3) / PEAK_HBM}
         del xv, yv
+    # matrix-free FP64 evaluate of the full (cell, species) operator over the element list
+    # (volume, face and interface items in colour groups, tensor and cut rules)
+    uf = B.DeviceArray.from_host(ctx, rng.standard_normal(X.N_full))
+    yf = B.DeviceArray(ctx, X.N_full)
+    for _ in range(3):
+        X.Evaluate(1.0, 0.0, yf, uf)
+    ctx.sync()
+    reps = 20
+    ctx.record(2)
+    for _ in range(reps):
+        X.Evaluate(1.0, 0.0, yf, uf)
+    ctx.record(3)
+    ctx.sync()
+    ms = ctx.elapsed_ms(2, 3) / reps
+    nel = X.nelem
+    # u + y, the element records (64 B each) and the cut rules' nodes, weights and normals
+    nbytes = 2 * X.N_full * 8 + nel * 64 + X.nnodes * 56
+    res["evaluate_fp64"] = {"ms": ms, "dof_per_s": X.N_full / (ms * 1e-3), "elements": nel, "colour_groups": 14,
+                            "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
+                            "hbm_frac": nbytes / (ms * 1e-3) / PEAK_HBM,
+                            "note": "one wave per element; the basis is evaluated at every node (compute-bound)"}
+    del uf, yf
     b = X.rhs(1)
     # Schwarz on the aggregates of the coarse hierarchy (both precisions); in FP32 the
     # cut-cell blocks are applied with their FP64 inverse (ill-conditioned at 1:1000)

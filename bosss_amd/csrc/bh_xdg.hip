@@ -13,8 +13,10 @@
 // (a few hundred nodes) -- both kinds are items of the same launch.  At each
 // node the lanes evaluate the orthonormal basis (and gradients) of both sides
 // into LDS, then every lane accumulates its (test, trial) entries of the
-// element matrix; the element matrix is added to the global blocks with FP64
-// atomics (elements sharing a block are few: the order changes round-off only).
+// element matrix; the element matrix is added to the global blocks with plain
+// adds: the elements run in colour groups (XDG_COLOURS) within which no two touch
+// the same entry, one launch per group in a fixed order, so the result is
+// bitwise reproducible.
 #include <cmath>
 
 #include "bh_internal.hpp"
@@ -30,6 +32,8 @@ struct XArgs {
     int nstride;
     const XElem* __restrict__ elems;
     int nelem;
+    const int* __restrict__ perm;       // element indices of the colour group [start, start + count)
+    int start, count;
     const double* __restrict__ nx;
     const double* __restrict__ nw;
     const double* __restrict__ nn;
@@ -44,39 +48,62 @@ struct XArgs {
     double* __restrict__ vals;
 };
 
+template <int D>
 __device__ __forceinline__ void cell_box(const XArgs& a, int c, double* lo, double* h) {
     int idx[3];
-    if (a.D == 2) { idx[0] = c / a.n1; idx[1] = c - idx[0] * a.n1; idx[2] = 0; }
+    if (D == 2) { idx[0] = c / a.n1; idx[1] = c - idx[0] * a.n1; idx[2] = 0; }
     else { int sl = a.n0 * a.n1; idx[2] = c / sl; int r = c - idx[2] * sl; idx[1] = r / a.n0; idx[0] = r - idx[1] * a.n0; }
-    for (int d = 0; d < a.D; ++d) {
+    lo[2] = 0;
+    h[2] = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
         lo[d] = a.nodes1[d * a.nstride + idx[d]];
         h[d] = a.nodes1[d * a.nstride + idx[d] + 1] - lo[d];
     }
 }
 
-// basis value / gradient of all modes at physical x in cell (lo, h); lane-parallel over modes
-__device__ void basis_at(const XArgs& a, const double* lo, const double* h, const double* x, int m, double* val, double* grad) {
-    double P[3][8], dP[3][8];
+// basis value / gradient of mode m at physical x in cell (lo, h); lane-parallel over modes.
+// Degrees k <= 3: the 1D Legendre values and derivatives are held in fixed-size
+// register arrays and picked by unrolled selects (a runtime index into a local
+// array would place it in per-lane scratch memory).
+constexpr int XMAXK = 3;
+__device__ __forceinline__ double pick4(const double* v, int e) {
+    return e == 0 ? v[0] : e == 1 ? v[1] : e == 2 ? v[2] : v[3];
+}
+template <int D>
+__device__ __forceinline__ void basis_at(const XArgs& a, const double* lo, const double* h, const double* x, int m, double* val,
+                                         double* grad) {
+    double P[3][XMAXK + 1], dP[3][XMAXK + 1];
     double detJ = 1;
-    for (int d = 0; d < a.D; ++d) {
-        const double xi = 2.0 * (x[d] - lo[d]) / h[d] - 1.0;
-        detJ *= 0.5 * h[d];
-        P[d][0] = 1; dP[d][0] = 0;
-        if (a.k > 0) { P[d][1] = xi; dP[d][1] = 1; }
-        for (int n = 2; n <= a.k; ++n) {
-            P[d][n] = ((2.0 * n - 1.0) * xi * P[d][n - 1] - (n - 1.0) * P[d][n - 2]) / n;
-            dP[d][n] = n * P[d][n - 1] + xi * dP[d][n - 1];
-        }
-        for (int n = 0; n <= a.k; ++n) { const double s = sqrt(n + 0.5); P[d][n] *= s; dP[d][n] *= s; }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double xi = d < D ? 2.0 * (x[d] - lo[d]) / h[d] - 1.0 : 0.0;
+        if (d < D) detJ *= 0.5 * h[d];
+        // orthonormal Legendre p_n = sqrt(n + 1/2) P_n and derivatives, n <= 3
+        const double P2 = 1.5 * xi * xi - 0.5, P3 = (2.5 * xi * xi - 1.5) * xi;
+        P[d][0] = 0.7071067811865476;
+        P[d][1] = 1.224744871391589 * xi;
+        P[d][2] = 1.5811388300841898 * P2;
+        P[d][3] = 1.8708286933869707 * P3;
+        dP[d][0] = 0.0;
+        dP[d][1] = 1.224744871391589;
+        dP[d][2] = 1.5811388300841898 * 3.0 * xi;
+        dP[d][3] = 1.8708286933869707 * (7.5 * xi * xi - 1.5);
     }
     const double sc = 1.0 / sqrt(detJ);
-    const int* e = a.ex + m * a.D;
-    double v = sc;
-    for (int d = 0; d < a.D; ++d) v *= P[d][e[d]];
-    *val = v;
-    for (int d = 0; d < a.D; ++d) {
+    const int* e = a.ex + m * D;
+    double pv[3], dv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        pv[d] = d < D ? pick4(P[d], e[d]) : 1.0;
+        dv[d] = d < D ? pick4(dP[d], e[d]) : 0.0;
+    }
+    *val = sc * pv[0] * pv[1] * pv[2];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
         double g = sc * 2.0 / h[d];
-        for (int dd = 0; dd < a.D; ++dd) g *= (dd == d) ? dP[dd][e[dd]] : P[dd][e[dd]];
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) g *= (dd == d) ? dv[dd] : pv[dd];
         grad[d] = g;
     }
 }
@@ -92,16 +119,17 @@ __device__ __forceinline__ int spos(const XArgs& a, int c, int sp) { return (sp 
 
 constexpr int XMAXNP = 20;
 
+template <int D>
 __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
     __shared__ double sv[2][XMAXNP], sd[2][XMAXNP], sg[XMAXNP][3];
     __shared__ double node[4];
     const int lane = threadIdx.x;
-    const int Np = a.Np, D = a.D;
-    for (int e = blockIdx.x; e < a.nelem; e += gridDim.x) {
-        const XElem E = a.elems[e];
+    const int Np = a.Np;
+    for (int ei = blockIdx.x; ei < a.count; ei += gridDim.x) {
+        const XElem E = a.elems[a.perm[a.start + ei]];
         double loP[3], hP[3], loQ[3], hQ[3];
-        cell_box(a, E.cellP, loP, hP);
-        cell_box(a, E.cellQ, loQ, hQ);
+        cell_box<D>(a, E.cellP, loP, hP);
+        cell_box<D>(a, E.cellQ, loQ, hQ);
         const int nblk = E.type == 0 ? 1 : (E.type == 2 ? 1 : 4);
         const int nent = nblk * Np * Np;
         // entries owned by this lane: t = lane + 64 j
@@ -146,22 +174,25 @@ __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
             __syncthreads();
             const double x[3] = {node[0], node[1], node[2]};
             const double w = node[3];
-            double nrm[3] = {0, 0, 0};
-            if (E.type == 3) for (int d = 0; d < 3; ++d) nrm[d] = a.nn[(size_t)(E.r0 + qn) * 3 + d];
-            else if (E.type == 1) nrm[E.dir] = 1.0;
-            else if (E.type == 2) nrm[E.dir] = E.side ? 1.0 : -1.0;
+            double nrm[3];
+            const double fsgn = E.type == 1 ? 1.0 : (E.side ? 1.0 : -1.0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                nrm[d] = E.type == 3 ? a.nn[(size_t)(E.r0 + qn) * 3 + d] : (E.type != 0 && d == E.dir ? fsgn : 0.0);
             // basis of both sides at the node
             if (lane < 2 * Np) {
                 const int side = lane / Np, m = lane - side * Np;
                 double v, g[3];
-                basis_at(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
+                basis_at<D>(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
                 double dn = 0;
+#pragma unroll
                 for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
                 sv[side][m] = v;
                 sd[side][m] = dn;
                 if (side == 0) for (int d = 0; d < 3; ++d) sg[m][d] = d < D ? g[d] : 0.0;
             }
             __syncthreads();
+#pragma unroll
             for (int j = 0; j < 7; ++j) {
                 const int t = lane + 64 * j;
                 if (t >= nent) break;
@@ -189,6 +220,7 @@ __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
             __syncthreads();
         }
         // scatter into the global blocks
+#pragma unroll
         for (int j = 0; j < 7; ++j) {
             const int t = lane + 64 * j;
             if (t >= nent) break;
@@ -207,7 +239,7 @@ __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
             if (p < 0) continue;
             const int R = a.bsz[rc];
             const int row = spos(a, rc, rsp) * Np + m, colj = spos(a, cc, csp) * Np + n;
-            atomicAdd(&a.vals[a.voff[p] + (int64_t)colj * R + row], acc[j]);
+            a.vals[a.voff[p] + (int64_t)colj * R + row] += acc[j];   // one element of the group per entry
         }
         __syncthreads();
     }
@@ -216,8 +248,9 @@ __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
 // ------------------------------------------------------------------ matrix-free evaluate
 // Node qn of element E: physical point and weight (tensor Gauss rule for uncut
 // items, the item's own node list for cut items).
-__device__ void element_node(const XArgs& a, const XElem& E, int qn, const double* loP, const double* hP, double* x,
-                             double* w) {
+template <int D>
+__device__ __forceinline__ void element_node(const XArgs& a, const XElem& E, int qn, const double* loP, const double* hP,
+                                             double* x, double* w) {
     x[0] = x[1] = x[2] = 0;
     if (E.rn > 0) {
         for (int d = 0; d < 3; ++d) x[d] = a.nx[(size_t)(E.r0 + qn) * 3 + d];
@@ -226,7 +259,8 @@ __device__ void element_node(const XArgs& a, const XElem& E, int qn, const doubl
     }
     int r = qn;
     double ww = 1;
-    for (int d = 0; d < a.D; ++d) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
         if (E.type != 0 && d == E.dir) {
             x[d] = (E.type == 2 && E.side == 0) ? loP[d] : loP[d] + hP[d];
             continue;
@@ -251,18 +285,19 @@ struct XEvalArgs {
 // the linear XLaplace forms): one wave per element, the same element list and rules as the assembly.
 // Per node, the lanes evaluate both sides' basis into LDS, every lane forms the traces of u (sums over the
 // modes in LDS), and lane m (P side) / Np + m (Q side) accumulates the test-function m contribution; one
-// FP64 atomic add per (element, test function) at the end.
+// plain add per (element, test function) at the end (colour groups, as in the assembly: deterministic).
+template <int D>
 __global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
     const XArgs& a = e.g;
     __shared__ double sv[2][XMAXNP], sd[2][XMAXNP], sg[XMAXNP][3], su[2][XMAXNP];
     __shared__ double node[4];
     const int lane = threadIdx.x;
-    const int Np = a.Np, D = a.D;
-    for (int ei = blockIdx.x; ei < a.nelem; ei += gridDim.x) {
-        const XElem E = a.elems[ei];
+    const int Np = a.Np;
+    for (int ei = blockIdx.x; ei < a.count; ei += gridDim.x) {
+        const XElem E = a.elems[a.perm[a.start + ei]];
         double loP[3], hP[3], loQ[3], hQ[3];
-        cell_box(a, E.cellP, loP, hP);
-        cell_box(a, E.cellQ, loQ, hQ);
+        cell_box<D>(a, E.cellP, loP, hP);
+        cell_box<D>(a, E.cellQ, loQ, hQ);
         const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
         const int64_t oQ = e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np;
         const bool twoSided = E.type == 1 || E.type == 3;
@@ -276,21 +311,23 @@ __global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
         for (int qn = 0; qn < nnodes; ++qn) {
             if (lane == 0) {
                 double x[3], w;
-                element_node(a, E, qn, loP, hP, x, &w);
+                element_node<D>(a, E, qn, loP, hP, x, &w);
                 node[0] = x[0]; node[1] = x[1]; node[2] = x[2]; node[3] = w;
             }
             __syncthreads();
             const double x[3] = {node[0], node[1], node[2]};
             const double w = node[3];
-            double nrm[3] = {0, 0, 0};
-            if (E.type == 3) for (int d = 0; d < 3; ++d) nrm[d] = a.nn[(size_t)(E.r0 + qn) * 3 + d];
-            else if (E.type == 1) nrm[E.dir] = 1.0;
-            else if (E.type == 2) nrm[E.dir] = E.side ? 1.0 : -1.0;
+            double nrm[3];
+            const double fsgn = E.type == 1 ? 1.0 : (E.side ? 1.0 : -1.0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                nrm[d] = E.type == 3 ? a.nn[(size_t)(E.r0 + qn) * 3 + d] : (E.type != 0 && d == E.dir ? fsgn : 0.0);
             if (lane < 2 * Np) {
                 const int side = lane / Np, m = lane - side * Np;
                 double v, g[3];
-                basis_at(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
+                basis_at<D>(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
                 double dn = 0;
+#pragma unroll
                 for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
                 sv[side][m] = v;
                 sd[side][m] = dn;
@@ -327,8 +364,8 @@ __global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
             }
             __syncthreads();
         }
-        if (lane < Np) atomicAdd(&e.y[oP + lane], e.alpha * acc);
-        else if (lane < 2 * Np && twoSided) atomicAdd(&e.y[oQ + lane - Np], e.alpha * acc);
+        if (lane < Np) e.y[oP + lane] += e.alpha * acc;
+        else if (lane < 2 * Np && twoSided) e.y[oQ + lane - Np] += e.alpha * acc;
         __syncthreads();
     }
 }
@@ -615,10 +652,16 @@ void xdg_assemble(XdgDevice& X, Vbsr& M) {
     a.rowptr = M.rowptr_d.as<int64_t>(); a.col = M.col_d.as<int>(); a.voff = M.voff_d.as<int64_t>(); a.bsz = M.rsz_d.as<int>();
     a.vals = M.vals.as<double>();
     BH_CHECK(P.Np <= XMAXNP && 4 * P.Np * P.Np <= 7 * 64, "XDG assembly: degree too high");
-    int blocks = a.nelem < 65536 ? a.nelem : 65536;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(xdg_assemble_kernel, dim3(blocks), dim3(64), 0, X.ctx->stream, a);
-    BH_HIP(hipGetLastError());
+    a.perm = X.perm.as<int>();
+    for (int c = 0; c < XDG_COLOURS; ++c) {
+        a.start = X.colour_off[c];
+        a.count = X.colour_off[c + 1] - a.start;
+        if (a.count == 0) continue;
+        const int blocks = a.count < 65536 ? a.count : 65536;
+        if (P.D == 2) hipLaunchKernelGGL(xdg_assemble_kernel<2>, dim3(blocks), dim3(64), 0, X.ctx->stream, a);
+        else hipLaunchKernelGGL(xdg_assemble_kernel<3>, dim3(blocks), dim3(64), 0, X.ctx->stream, a);
+        BH_HIP(hipGetLastError());
+    }
 }
 
 void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y) {
@@ -642,10 +685,16 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
     e.y = y;
     e.alpha = alpha;
     BH_CHECK(2 * P.Np <= 64 && P.Np <= XMAXNP, "XDG evaluate: degree too high");
-    int blocks = a.nelem < 65536 ? a.nelem : 65536;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(xdg_evaluate_kernel, dim3(blocks), dim3(64), 0, X.ctx->stream, e);
-    BH_HIP(hipGetLastError());
+    a.perm = X.perm.as<int>();
+    for (int c = 0; c < XDG_COLOURS; ++c) {
+        a.start = X.colour_off[c];
+        a.count = X.colour_off[c + 1] - a.start;
+        if (a.count == 0) continue;
+        const int blocks = a.count < 65536 ? a.count : 65536;
+        if (P.D == 2) hipLaunchKernelGGL(xdg_evaluate_kernel<2>, dim3(blocks), dim3(64), 0, X.ctx->stream, e);
+        else hipLaunchKernelGGL(xdg_evaluate_kernel<3>, dim3(blocks), dim3(64), 0, X.ctx->stream, e);
+        BH_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace bh

@@ -29,6 +29,35 @@ void XdgDevice::upload() {
     ctx->upload(nodes1.p, n1.data(), n1.size() * sizeof(double));
     elems.alloc(std::max<size_t>(p.elems.size(), 1) * sizeof(XElem));
     ctx->upload(elems.p, p.elems.data(), p.elems.size() * sizeof(XElem));
+    // colour groups (see XDG_COLOURS), each with its cut items (own rules, more nodes) first
+    auto colour = [&](const XElem& E) {
+        if (E.type == 0) return 0;
+        if (E.type == 3) return 1;
+        if (E.type == 2) return 2 + 2 * E.dir + (E.side ? 1 : 0);
+        int iP[3], iQ[3];
+        g->cell_ijk(E.cellP, iP);
+        g->cell_ijk(E.cellQ, iQ);
+        return 8 + 2 * E.dir + (std::min(iP[E.dir], iQ[E.dir]) & 1);
+    };
+    std::vector<int> cnt(2 * XDG_COLOURS + 1, 0);
+    const int ne = (int)p.elems.size();
+    std::vector<int> key(ne);
+    for (int e = 0; e < ne; ++e) {
+        const int c = colour(p.elems[e]);
+        BH_CHECK(c >= 0 && c < XDG_COLOURS, "XDG element colour out of range");
+        key[e] = 2 * c + (p.elems[e].rn > 0 ? 0 : 1);
+        cnt[key[e] + 1]++;
+    }
+    for (int i = 0; i < 2 * XDG_COLOURS; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> pm(std::max(ne, 1));
+    {
+        std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < ne; ++e) pm[pos[key[e]]++] = e;   // stable: the original order within a key
+    }
+    colour_off.assign(XDG_COLOURS + 1, 0);
+    for (int c = 0; c <= XDG_COLOURS; ++c) colour_off[c] = cnt[2 * c];
+    perm.alloc(pm.size() * sizeof(int));
+    ctx->upload(perm.p, pm.data(), pm.size() * sizeof(int));
     auto up = [&](DevBuf& b, const std::vector<double>& v) {
         b.alloc(std::max<size_t>(v.size(), 1) * sizeof(double));
         ctx->upload(b.p, v.data(), v.size() * sizeof(double));

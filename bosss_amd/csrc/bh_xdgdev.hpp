@@ -37,11 +37,21 @@ struct Vbsr {
     }
 };
 
+// Elements are processed in groups ("colours") within which no two elements add
+// to the same (cell, species) rows or matrix entries: volume items, interface
+// items, boundary faces per (direction, side), interior faces per (direction,
+// parity of the lower cell's index).  The groups run one after another in a fixed
+// order, so every entry receives its contributions in the same order on every
+// run: plain adds, no atomics, bitwise reproducible.
+constexpr int XDG_COLOURS = 14;
+
 struct XdgDevice {
     bh_ctx* ctx = nullptr;
     XdgProblem* P = nullptr;
     int nstride = 0;
     DevBuf nodes1, elems, nx, nw, nn, ex, gx, gw, spmask, offf;
+    DevBuf perm;                            // element indices grouped by colour (cut items first in each)
+    std::vector<int> colour_off;            // [XDG_COLOURS + 1] offsets into perm
     void upload();
 };
 
