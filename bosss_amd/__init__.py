@@ -67,6 +67,7 @@ SIGNATURES = {
     "bh_op_set_fp64_path": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_op_set_fp32_path": (_c_i, [_c_p, _c_i, _c_i]),
     "bh_bsr_create": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p, _PP]),
+    "bh_bsr_create_v": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _PP]),
     "bh_bsr_destroy": (_c_i, [_c_p]),
     "bh_bsr_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i64)]),
     "bh_bsr_export": (_c_i, [_c_p, _c_p, _c_p, _c_p]),
@@ -116,6 +117,8 @@ SIGNATURES = {
     "bh_xdg_extend": (_c_i, [_c_p, _c_p, _c_p]),
     "bh_xdg_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i]),
     "bh_xdg_precond_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
+    "bh_xdg_precond_create_v": (_c_i, [_c_p, _c_p, _c_i, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
+    "bh_xdg_dof_frame": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p]),
     "bh_xdg_fgmres_create": (_c_i, [_c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
     "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
@@ -375,6 +378,23 @@ class BlockMsrMatrix(_Owned):
         _check(lib().bh_bsr_info(self.h, ctypes.byref(n), ctypes.byref(bs_), ctypes.byref(nnzb)))
         self.nrows, self.bs, self.nnzb = n.value, bs_.value, nnzb.value
         self.N = self.nrows * self.bs
+
+    @classmethod
+    def variable(cls, ctx, rowptr, col, blk_i0, blk_len, vals):
+        """Variable block sizes (bh_bsr_create_v): block row r of length blk_len[r]
+        at offset blk_i0[r] of the vector layout; vals = every stored block,
+        row-major, in rowptr order.  Vectors have max(blk_i0 + blk_len) entries."""
+        rp = np.ascontiguousarray(rowptr, np.int64)
+        cl = np.ascontiguousarray(col, np.int32)
+        i0 = np.ascontiguousarray(blk_i0, np.int64)
+        ln = np.ascontiguousarray(blk_len, np.int32)
+        vv = np.ascontiguousarray(vals, np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().bh_bsr_create_v(ctx.h, len(ln), rp.ctypes.data, cl.ctypes.data, i0.ctypes.data, ln.ctypes.data,
+                                     vv.ctypes.data, ctypes.byref(h)))
+        M = cls(ctx, handle=h.value)
+        M.N = int((i0 + ln).max()) if len(ln) else 0
+        return M
 
     def export(self):
         rp = np.zeros(self.nrows + 1, np.int64)
@@ -820,6 +840,15 @@ class XdgPoisson(_Owned):
         _check(lib().bh_xdg_evaluate(self.h, alpha, up, beta, yp, fu))
         return output
 
+    def dof_frame(self, which=1):
+        """Per cell: offset in the XDG constant frame (j * 2 Np), length, and offset
+        in the compressed layout of the full (0) or agglomerated (1) system."""
+        i0 = np.zeros(self.J, np.int64)
+        ln = np.zeros(self.J, np.int32)
+        c0 = np.zeros(self.J, np.int64)
+        _check(lib().bh_xdg_dof_frame(self.h, which, i0.ctypes.data, ln.ctypes.data, c0.ctypes.data))
+        return i0, ln, c0
+
     def extend(self, u):
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros(self.N_full)
@@ -835,6 +864,18 @@ class XdgPrecond(_Solver):
         _check(lib().bh_xdg_precond_create(xdg.h, coarse_degree, min_coarse, omega, pre, post, kcycle_inner, prec,
                                            ctypes.byref(h)))
         super().__init__(xdg.ctx, h.value, xdg.N, (xdg,))
+
+
+class XdgPrecondV(_Solver):
+    """The two-level XDG preconditioner for a caller's variable-block matrix of
+    cell blocks in the caller's vector layout (bh_xdg_precond_create_v)."""
+
+    def __init__(self, grid, M, Np, coarse_degree=1, min_coarse=64, omega=0.8, pre=2, post=2, kcycle_inner=2,
+                 prec=BH_FP32):
+        h = ctypes.c_void_p()
+        _check(lib().bh_xdg_precond_create_v(grid.h, M.h, Np, coarse_degree, min_coarse, omega, pre, post,
+                                             kcycle_inner, prec, ctypes.byref(h)))
+        super().__init__(M.ctx, h.value, M.N, (grid, M))
 
 
 def xdg_precond_set_fp64_blocks(precond, on=True):

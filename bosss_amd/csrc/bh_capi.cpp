@@ -35,6 +35,48 @@ struct Staged {
     }
 };
 
+// A solver in the caller's layout (frame) of a variable-block matrix around a solver in
+// its compressed layout: gathers b and x, runs the inner solver, scatters x back with
+// the unused frame slots set to zero.
+struct FrameSolver : bh_solver {
+    bh_bsr* M = nullptr;
+    std::unique_ptr<bh_solver> inner;
+    DevBuf xc, bc;
+    FrameSolver(bh_bsr* m, bh_solver* in) : M(m), inner(in) {
+        ctx = m->ctx;
+        N = m->fr_N;
+        xc.alloc(std::max<int64_t>(in->N, 1) * sizeof(double));
+        bc.alloc(std::max<int64_t>(in->N, 1) * sizeof(double));
+    }
+    void sync_stats() {
+        iterations = inner->iterations;
+        converged = inner->converged;
+        res0 = inner->res0;
+        res = inner->res;
+    }
+    void solve_dev(double* x, const double* b) override {
+        frame_gather<double>(ctx, inner->N, M->fr_fwd.as<int64_t>(), b, bc.as<double>());
+        frame_gather<double>(ctx, inner->N, M->fr_fwd.as<int64_t>(), x, xc.as<double>());
+        inner->solve_dev(xc.as<double>(), bc.as<double>());
+        frame_scatter<double>(ctx, N, M->fr_inv.as<int64_t>(), xc.as<double>(), x);
+        sync_stats();
+    }
+    void precondition(double* x, const double* b) override {
+        frame_gather<double>(ctx, inner->N, M->fr_fwd.as<int64_t>(), b, bc.as<double>());
+        inner->precondition(xc.as<double>(), bc.as<double>());
+        frame_scatter<double>(ctx, N, M->fr_inv.as<int64_t>(), xc.as<double>(), x);
+        sync_stats();
+    }
+};
+
+// wraps a compressed-layout solver of M for the caller's layout (when the two differ)
+bh_solver* in_frame(bh_bsr* M, bh_solver* s) { return (M && M->var && M->fr_gaps) ? new FrameSolver(M, s) : s; }
+// the compressed-layout solver behind a caller-layout one
+bh_solver* compressed(bh_solver* s) {
+    FrameSolver* f = dynamic_cast<FrameSolver*>(s);
+    return f ? f->inner.get() : s;
+}
+
 void build_grid_edges(bh_grid* g) {
     const int D = g->D;
     for (int c = 0; c < g->J; ++c) {
@@ -564,6 +606,82 @@ int bh_bsr_create(bh_ctx* ctx, int nrows, int bs, const int64_t* rowptr, const i
     BH_API_END
 }
 
+}  // extern "C"
+
+bh_bsr::~bh_bsr() { delete var; }
+
+extern "C" {
+
+int bh_bsr_create_v(bh_ctx* ctx, int nrows, const int64_t* rowptr, const int* col, const int64_t* blk_i0,
+                    const int* blk_len, const double* vals, bh_bsr** out) {
+    BH_API_BEGIN
+    BH_CHECK(ctx && rowptr && col && blk_i0 && blk_len && out && nrows >= 0, "bh_bsr_create_v: null argument");
+    BH_CHECK(rowptr[0] == 0, "bh_bsr_create_v: rowptr[0] must be 0");
+    std::vector<int> sizes(blk_len, blk_len + nrows);
+    int64_t frN = 0;
+    for (int r = 0; r < nrows; ++r) {
+        BH_CHECK(sizes[r] >= 0 && sizes[r] <= 64, "bh_bsr_create_v: block sizes must be in 0..64");
+        BH_CHECK(blk_i0[r] >= 0, "bh_bsr_create_v: negative block offset");
+        BH_CHECK(rowptr[r + 1] >= rowptr[r], "bh_bsr_create_v: rowptr not monotone");
+        frN = std::max<int64_t>(frN, blk_i0[r] + sizes[r]);
+    }
+    // the rows may not overlap in the caller's layout
+    std::vector<int> order(nrows);
+    for (int r = 0; r < nrows; ++r) order[r] = r;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return blk_i0[a] < blk_i0[b]; });
+    for (int i = 1; i < nrows; ++i)
+        BH_CHECK(blk_i0[order[i - 1]] + sizes[order[i - 1]] <= blk_i0[order[i]] || sizes[order[i - 1]] == 0,
+                 "bh_bsr_create_v: block rows overlap in the vector layout");
+    const int64_t nnzb = rowptr[nrows];
+    std::vector<std::vector<int>> cols(nrows);
+    for (int r = 0; r < nrows; ++r)
+        for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+            BH_CHECK(col[p] >= 0 && col[p] < nrows, "bh_bsr_create_v: column out of range");
+            cols[r].push_back(col[p]);
+        }
+    auto M = std::make_unique<bh_bsr>();
+    M->ctx = ctx;
+    M->nrows = nrows;
+    M->bs = 0;
+    M->nnzb = nnzb;
+    M->h_rowptr.assign(rowptr, rowptr + nrows + 1);
+    M->h_col.assign(col, col + nnzb);
+    M->var = new Vbsr();
+    Vbsr& V = *M->var;
+    V.build(ctx, nrows, sizes, cols);
+    // values: the caller's blocks (row-major, every block listed) -> the kept blocks, column-major
+    std::vector<double> v(std::max<int64_t>(V.nvals, 1), 0.0);
+    int64_t src = 0, q = 0;
+    for (int r = 0; r < nrows; ++r)
+        for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+            const int R = sizes[r], C = sizes[col[p]];
+            if (R > 0 && C > 0) {
+                double* dst = &v[V.voff[q++]];
+                for (int i = 0; i < R; ++i)
+                    for (int j = 0; j < C; ++j) dst[(size_t)j * R + i] = vals[src + (size_t)i * C + j];
+            }
+            src += (int64_t)R * C;
+        }
+    ctx->upload(V.vals.p, v.data(), V.nvals * sizeof(double));
+    // layouts: compressed row r at V.roff[r], caller's at blk_i0[r]
+    M->fr_N = frN;
+    M->fr_gaps = frN != V.N;
+    for (int r = 0; r < nrows && !M->fr_gaps; ++r) M->fr_gaps = sizes[r] > 0 && blk_i0[r] != V.roff[r];
+    std::vector<int64_t> fwd(std::max<int64_t>(V.N, 1)), inv(std::max<int64_t>(frN, 1), -1);
+    for (int r = 0; r < nrows; ++r)
+        for (int i = 0; i < sizes[r]; ++i) {
+            fwd[V.roff[r] + i] = blk_i0[r] + i;
+            inv[blk_i0[r] + i] = V.roff[r] + i;
+        }
+    M->fr_fwd.alloc(fwd.size() * sizeof(int64_t));
+    M->fr_inv.alloc(inv.size() * sizeof(int64_t));
+    ctx->upload(M->fr_fwd.p, fwd.data(), fwd.size() * sizeof(int64_t));
+    ctx->upload(M->fr_inv.p, inv.data(), inv.size() * sizeof(int64_t));
+    ctx->sync();
+    *out = M.release();
+    BH_API_END
+}
+
 int bh_bsr_destroy(bh_bsr* M) {
     BH_API_BEGIN
     if (M) (void)hipStreamSynchronize(M->ctx->stream);
@@ -574,13 +692,14 @@ int bh_bsr_destroy(bh_bsr* M) {
 int bh_bsr_info(bh_bsr* M, int* nrows, int* bs, int64_t* nnzb) {
     BH_API_BEGIN
     if (nrows) *nrows = M->nrows;
-    if (bs) *bs = M->bs;
+    if (bs) *bs = M->bs;   // 0: variable blocks
     if (nnzb) *nnzb = M->nnzb;
     BH_API_END
 }
 
 int bh_bsr_export(bh_bsr* M, int64_t* rowptr, int* col, double* vals) {
     BH_API_BEGIN
+    BH_CHECK(!M->var, "bh_bsr_export: uniform-block matrices only");
     const int bs = M->bs;
     if (rowptr) std::copy(M->h_rowptr.begin(), M->h_rowptr.end(), rowptr);
     if (col) std::copy(M->h_col.begin(), M->h_col.end(), col);
@@ -598,6 +717,35 @@ int bh_bsr_spmv(bh_bsr* M, double alpha, const void* x, double beta, void* y, in
     BH_API_BEGIN
     const bool dev = flags & BH_DEVICE_PTR;
     bh_ctx* ctx = M->ctx;
+    if (M->var) {
+        // variable blocks: vectors in the caller's layout; unused slots are zero rows
+        Vbsr& V = *M->var;
+        BH_CHECK(prec == BH_FP64 || (prec == BH_FP32 && dev), "bh_bsr_spmv: FP64, or FP32 on device float vectors");
+        const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
+        Staged sx(ctx, x, M->fr_N * es, dev, true);
+        Staged sy(ctx, y, M->fr_N * es, dev, beta != 0.0);
+        if (!M->fr_gaps) {
+            V.spmv(alpha, sx.dptr, beta, sy.dptr, prec);
+        } else {
+            if (M->fr_a.bytes < (size_t)V.N * es) { M->fr_a.alloc(std::max<int64_t>(V.N, 1) * es); M->fr_b.alloc(std::max<int64_t>(V.N, 1) * es); }
+            const int64_t* fw = M->fr_fwd.as<int64_t>();
+            const int64_t* iv = M->fr_inv.as<int64_t>();
+            if (prec == BH_FP32) {
+                frame_gather<float>(ctx, V.N, fw, (const float*)sx.dptr, M->fr_a.as<float>());
+                if (beta != 0.0) frame_gather<float>(ctx, V.N, fw, (const float*)sy.dptr, M->fr_b.as<float>());
+                V.spmv(alpha, M->fr_a.p, beta, M->fr_b.p, BH_FP32);
+                frame_scatter<float>(ctx, M->fr_N, iv, M->fr_b.as<float>(), (float*)sy.dptr);
+            } else {
+                frame_gather<double>(ctx, V.N, fw, (const double*)sx.dptr, M->fr_a.as<double>());
+                if (beta != 0.0) frame_gather<double>(ctx, V.N, fw, (const double*)sy.dptr, M->fr_b.as<double>());
+                V.spmv(alpha, M->fr_a.p, beta, M->fr_b.p, BH_FP64);
+                frame_scatter<double>(ctx, M->fr_N, iv, M->fr_b.as<double>(), (double*)sy.dptr);
+            }
+        }
+        sy.download(y, M->fr_N * es, dev);
+        if (!dev) ctx->sync();
+        return 0;
+    }
     const int64_t N = (int64_t)M->nrows * M->bs;
     if (prec == BH_FP64) {
         Staged sx(ctx, x, N * sizeof(double), dev, true);
@@ -616,6 +764,7 @@ int bh_bsr_spmv(bh_bsr* M, double alpha, const void* x, double beta, void* y, in
 int bh_bjacobi_create(bh_bsr* M, double omega, int iterations, int prec, bh_solver** out) {
     BH_API_BEGIN
     BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_bjacobi_create: precision must be FP64 or FP32");
+    BH_CHECK(!M->var, "bh_bjacobi_create: uniform-block matrices only (variable blocks: bh_xdg_precond_create_v)");
     *out = new BlockJacobiSolver(M, omega, iterations, prec);
     BH_API_END
 }
@@ -625,11 +774,17 @@ static int make_gmres(bh_bsr* M, bh_op* op, bh_solver* P, int restart, double to
     BH_CHECK(M || op, "GMRES needs a matrix or an operator");
     BH_CHECK(restart >= 1 && restart <= 256, "GMRES restart length must be in 1..256");
     LinOp A;
-    A.M = M;
-    A.op = M ? nullptr : op;
-    A.N = M ? (int64_t)M->nrows * M->bs : op->N;
-    if (P) BH_CHECK(P->N == A.N, "preconditioner size mismatch");
-    *out = new GmresSolver(A, P, restart, tol, maxit, flex);
+    if (M && M->var) {
+        A.V = M->var;
+        A.N = M->var->N;
+    } else {
+        A.M = M;
+        A.op = M ? nullptr : op;
+        A.N = M ? (int64_t)M->nrows * M->bs : op->N;
+    }
+    bh_solver* Pc = P ? compressed(P) : nullptr;
+    if (Pc) BH_CHECK(Pc->N == A.N, "preconditioner size mismatch");
+    *out = in_frame(M, new GmresSolver(A, Pc, restart, tol, maxit, flex));
     BH_API_END
 }
 
@@ -655,6 +810,7 @@ int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int
     mg->post = post;
     mg->kin = kcycle_inner;
     mg->kcycle_level = kcycle_inner > 0 ? ndeg - 1 : -1;
+    BH_CHECK(!fine_mat || !fine_mat->var, "bh_pmg_create: uniform-block fine matrix only");
     bh_bsr* Af = fine_mat;
     if (!Af) {
         BH_CHECK(bh_op_assemble(op, &Af, nullptr, 0) == 0, "bh_pmg_create: assembly failed: " + bh::last_error());
@@ -851,7 +1007,7 @@ int bh_pmg_learned_info(bh_solver* s, int* active, double* mean_gain) {
 
 int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
     BH_API_BEGIN
-    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(s);
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(compressed(s));
     BH_CHECK(xp, "bh_xdg_precond_set_fp64_blocks: solver is not an XDG preconditioner");
     xp->Dc->fp64_inverse = on != 0;
     BH_API_END
@@ -860,7 +1016,7 @@ int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
 int bh_pmg_set_schwarz(bh_solver* s, int enable) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
-    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(s);
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(compressed(s));
     BH_CHECK(mg || xp, "bh_pmg_set_schwarz: solver is not a multigrid preconditioner");
     if (xp) mg = xp->cmg.get();
     (void)hipStreamSynchronize(mg->ctx->stream);
@@ -880,7 +1036,7 @@ int bh_pmg_set_diag_equilib(bh_solver* s) {
 int bh_pmg_set_graphs(bh_solver* s, int enable) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
-    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(s);
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(compressed(s));
     BH_CHECK(mg || xp, "bh_pmg_set_graphs: solver is not a multigrid preconditioner");
     if (xp) mg = xp->cmg.get();
     (void)hipStreamSynchronize(mg->ctx->stream);
@@ -1178,6 +1334,45 @@ int bh_xdg_precond_create(bh_xdg* X, int coarse_degree, int min_coarse, double o
     P->post = post;
     P->build(X->grid, coarse_degree, min_coarse, kcycle_inner);
     *out = P.release();
+    BH_API_END
+}
+
+int bh_xdg_precond_create_v(bh_grid* g, bh_bsr* M, int Np, int coarse_degree, int min_coarse, double omega, int pre,
+                            int post, int kcycle_inner, int prec, bh_solver** out) {
+    BH_API_BEGIN
+    BH_CHECK(g && M && out, "bh_xdg_precond_create_v: null argument");
+    BH_CHECK(M->var, "bh_xdg_precond_create_v: needs a variable-block matrix (bh_bsr_create_v)");
+    BH_CHECK(M->nrows == g->J, "bh_xdg_precond_create_v: one block row per grid cell expected");
+    BH_CHECK(g->ctx, "bh_xdg_precond_create_v: the grid needs a device context");
+    BH_CHECK(prec == BH_FP64 || prec == BH_FP32, "bh_xdg_precond_create_v: precision must be FP64 or FP32");
+    BH_CHECK(pre >= 1 && post >= 0, "bh_xdg_precond_create_v: need pre >= 1, post >= 0");
+    BH_CHECK(Np >= 1 && coarse_degree >= 0, "bh_xdg_precond_create_v: invalid Np or coarse degree");
+    BH_CHECK(basis_count(g->D, coarse_degree) <= Np, "bh_xdg_precond_create_v: coarse degree above the DG degree");
+    auto P = std::make_unique<XdgPrecond>();
+    P->Av = M->var;
+    P->Dc = M->var;
+    P->prec = prec;
+    P->Np = Np;
+    P->omega = omega;
+    P->pre = pre;
+    P->post = post;
+    P->build(g, coarse_degree, min_coarse, kcycle_inner);
+    *out = in_frame(M, P.release());
+    BH_API_END
+}
+
+int bh_xdg_dof_frame(bh_xdg* X, int which, int64_t* frame_i0, int* len, int64_t* compressed_i0) {
+    BH_API_BEGIN
+    BH_CHECK(which == 0 || which == 1, "bh_xdg_dof_frame: which must be 0 (full) or 1 (agglomerated)");
+    const XdgProblem& p = *X->P;
+    const int J = p.g->J;
+    const std::vector<int64_t>& off = which ? p.off_red : p.off_full;
+    const int nmax = 2 * p.Np;   // Np times the number of species (XDGBasis.MaximalLength)
+    for (int j = 0; j < J; ++j) {
+        if (frame_i0) frame_i0[j] = (int64_t)j * nmax;
+        if (len) len[j] = (int)(off[j + 1] - off[j]);
+        if (compressed_i0) compressed_i0[j] = off[j];
+    }
     BH_API_END
 }
 

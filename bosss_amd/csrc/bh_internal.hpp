@@ -159,9 +159,13 @@ struct SipTables {
 }  // namespace bh
 
 // ------------------------------------------------------------------ BSR
+namespace bh {
+struct Vbsr;
+}
+
 struct bh_bsr {
     bh_ctx* ctx = nullptr;
-    int nrows = 0, bs = 0;
+    int nrows = 0, bs = 0;      // bs = 0: variable blocks (var)
     int64_t nnzb = 0;
     std::vector<int64_t> h_rowptr;
     std::vector<int> h_col;
@@ -169,8 +173,21 @@ struct bh_bsr {
     bh::DevBuf vals;            // FP64, nnzb blocks, each column-major
     bh::DevBuf vals32;          // FP32 copy (lazy)
     bh::DevBuf diag_idx;        // int64 per row: index of the diagonal block or -1
+    // variable-block matrix (bh_bsr_create_v): the blocks in compressed row order
+    // (row r at the prefix sum of the sizes) and the caller's layout ("frame": row r
+    // at blk_i0[r], possibly with unused slots between rows, as the XDG constant frame)
+    bh::Vbsr* var = nullptr;
+    int64_t fr_N = 0;           // frame length
+    bool fr_gaps = false;       // frame differs from the compressed layout
+    bh::DevBuf fr_fwd;          // int64 per compressed DOF: its frame index
+    bh::DevBuf fr_inv;          // int64 per frame DOF: its compressed index or -1
+    bh::DevBuf fr_a, fr_b;      // compressed work vectors (FP64, or FP32 when used so)
     void ensure_fp32();
     void ensure_diag();
+    bh_bsr() = default;
+    bh_bsr(const bh_bsr&) = delete;
+    bh_bsr& operator=(const bh_bsr&) = delete;
+    ~bh_bsr();
 };
 
 // ------------------------------------------------------------------ SIP operator

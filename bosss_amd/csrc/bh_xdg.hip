@@ -17,6 +17,7 @@
 // adds: the elements run in colour groups (XDG_COLOURS) within which no two touch
 // the same entry, one launch per group in a fixed order, so the result is
 // bitwise reproducible.
+#include <algorithm>
 #include <cmath>
 
 #include "bh_internal.hpp"
@@ -509,6 +510,35 @@ __global__ void xdg_prolong_k(int nrows, int Np, int Npc, const int* __restrict_
     const T v = xc[t];
     for (int q = 0; q < ns; ++q) x[roff[c] + q * Np + m] += v;
 }
+
+template <typename T>
+__global__ void frame_gather_k(int64_t nc, const int64_t* __restrict__ fwd, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[fwd[i]];
+}
+template <typename T>
+__global__ void frame_scatter_k(int64_t nf, const int64_t* __restrict__ inv, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nf; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = inv[k];
+        dst[k] = i >= 0 ? src[i] : T(0);
+    }
+}
+template <typename T>
+void frame_gather(bh_ctx* ctx, int64_t nc, const int64_t* fwd, const T* src_f, T* dst_c) {
+    const int64_t b = std::min<int64_t>((nc + 255) / 256, 65536);
+    hipLaunchKernelGGL((frame_gather_k<T>), dim3((unsigned)std::max<int64_t>(b, 1)), dim3(256), 0, ctx->stream, nc, fwd, src_f, dst_c);
+    BH_HIP(hipGetLastError());
+}
+template <typename T>
+void frame_scatter(bh_ctx* ctx, int64_t nf, const int64_t* inv, const T* src_c, T* dst_f) {
+    const int64_t b = std::min<int64_t>((nf + 255) / 256, 65536);
+    hipLaunchKernelGGL((frame_scatter_k<T>), dim3((unsigned)std::max<int64_t>(b, 1)), dim3(256), 0, ctx->stream, nf, inv, src_c, dst_f);
+    BH_HIP(hipGetLastError());
+}
+template void frame_gather<double>(bh_ctx*, int64_t, const int64_t*, const double*, double*);
+template void frame_gather<float>(bh_ctx*, int64_t, const int64_t*, const float*, float*);
+template void frame_scatter<double>(bh_ctx*, int64_t, const int64_t*, const double*, double*);
+template void frame_scatter<float>(bh_ctx*, int64_t, const int64_t*, const float*, float*);
 
 template <typename T>
 void xdg_restrict(bh_ctx* ctx, const Vbsr& A, int Np, int Npc, const T* r, T* rc) {

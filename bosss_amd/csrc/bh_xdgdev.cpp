@@ -462,30 +462,52 @@ void xdg_extend(const bh_xdg* X, const double* ured, double* ufull) {
 XdgPrecond::~XdgPrecond() { delete Ac; }
 
 void XdgPrecond::build(bh_grid* g, int degc, int min_coarse, int kcycle_inner) {
-    ctx = A->ctx;
-    N = (int64_t)A->nrows * A->bs;
+    ctx = Av ? Av->ctx : A->ctx;
+    N = Av ? Av->N : (int64_t)A->nrows * A->bs;
     const int D = g->D, J = g->J;
     Npc = basis_count(D, degc);
-    BH_CHECK(Npc <= Np && A->bs == Np, "XDG preconditioner: coarse degree above the DG degree");
-    BH_CHECK(Dc->N == N, "XDG preconditioner: cell blocks do not match the matrix");
+    BH_CHECK(Npc <= Np && (Av || A->bs == Np), "XDG preconditioner: coarse degree above the DG degree");
+    BH_CHECK(Dc->N == N && Dc->nrows == J, "XDG preconditioner: cell blocks do not match the matrix");
+    for (int c = 0; c < J; ++c)
+        BH_CHECK(Dc->rsz[c] % Np == 0 && Dc->rsz[c] <= 2 * Np, "XDG preconditioner: cell blocks must hold 0, Np or 2 Np values");
     // block inverses of the smoother (cell blocks, species coupled)
     Dc->invert_diag(prec == BH_FP32);
-    if (prec == BH_FP32) A->ensure_fp32();
-    // coarse matrix Ac = R^T A R (species merged, first Npc modes of every unit)
-    std::vector<int> cell_of(A->nrows);
-    for (int c = 0, u = 0; c < J; ++c)
-        for (int a = 0; a < Dc->rsz[c] / Np; ++a) cell_of[u++] = c;
-    std::vector<double> v((size_t)A->nnzb * Np * Np);
-    ctx->download(v.data(), A->vals.p, v.size() * sizeof(double));
+    if (prec == BH_FP32) {
+        if (Av) Av->ensure_fp32();
+        else A->ensure_fp32();
+    }
+    // coarse matrix Ac = R^T A R (species merged, first Npc modes of every species block)
     std::vector<std::map<int, std::vector<double>>> crow(J);
-    for (int u = 0; u < A->nrows; ++u)
-        for (int64_t bp = A->h_rowptr[u]; bp < A->h_rowptr[u + 1]; ++bp) {
-            const double* B = &v[(size_t)bp * Np * Np];   // column-major Np x Np
-            auto& blk = crow[cell_of[u]][cell_of[A->h_col[bp]]];
-            if (blk.empty()) blk.assign((size_t)Npc * Npc, 0.0);
-            for (int n = 0; n < Npc; ++n)
-                for (int m = 0; m < Npc; ++m) blk[(size_t)n * Npc + m] += B[(size_t)n * Np + m];
-        }
+    if (Av) {
+        std::vector<double> v(std::max<int64_t>(Av->nvals, 1));
+        ctx->download(v.data(), Av->vals.p, Av->nvals * sizeof(double));
+        for (int r = 0; r < J; ++r)
+            for (int64_t p = Av->rowptr[r]; p < Av->rowptr[r + 1]; ++p) {
+                const int c = Av->col[p], R = Av->rsz[r], C = Av->rsz[c];
+                const double* B = &v[Av->voff[p]];   // column-major R x C
+                auto& blk = crow[r][c];
+                if (blk.empty()) blk.assign((size_t)Npc * Npc, 0.0);
+                for (int a = 0; a < R / Np; ++a)
+                    for (int b = 0; b < C / Np; ++b)
+                        for (int n = 0; n < Npc; ++n)
+                            for (int m = 0; m < Npc; ++m)
+                                blk[(size_t)n * Npc + m] += B[(size_t)(b * Np + n) * R + (a * Np + m)];
+            }
+    } else {
+        std::vector<int> cell_of(A->nrows);
+        for (int c = 0, u = 0; c < J; ++c)
+            for (int a = 0; a < Dc->rsz[c] / Np; ++a) cell_of[u++] = c;
+        std::vector<double> v((size_t)A->nnzb * Np * Np);
+        ctx->download(v.data(), A->vals.p, v.size() * sizeof(double));
+        for (int u = 0; u < A->nrows; ++u)
+            for (int64_t bp = A->h_rowptr[u]; bp < A->h_rowptr[u + 1]; ++bp) {
+                const double* B = &v[(size_t)bp * Np * Np];   // column-major Np x Np
+                auto& blk = crow[cell_of[u]][cell_of[A->h_col[bp]]];
+                if (blk.empty()) blk.assign((size_t)Npc * Npc, 0.0);
+                for (int n = 0; n < Npc; ++n)
+                    for (int m = 0; m < Npc; ++m) blk[(size_t)n * Npc + m] += B[(size_t)n * Np + m];
+            }
+    }
     std::vector<int64_t> rp(J + 1, 0);
     std::vector<int> col;
     std::vector<double> cv;
@@ -525,6 +547,13 @@ void XdgPrecond::build(bh_grid* g, int degc, int min_coarse, int kcycle_inner) {
 }
 
 template <typename T>
+void XdgPrecond::residual(const T* bb, const T* xx, T* rr) {
+    const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
+    if (Av) Av->spmv(-1.0, xx, 0.0, rr, p, bb);
+    else bsr_residual(A, bb, xx, rr, p);
+}
+
+template <typename T>
 void XdgPrecond::cycle() {
     const int p = sizeof(T) == 4 ? BH_FP32 : BH_FP64;
     T* bb = (T*)b.p;
@@ -532,16 +561,16 @@ void XdgPrecond::cycle() {
     T* rr = (T*)r.p;
     Dc->block_gemv(bb, xx, omega, false, p);
     for (int s = 1; s < pre; ++s) {
-        bsr_residual(A, bb, xx, rr, p);
+        residual<T>(bb, xx, rr);
         Dc->block_gemv(rr, xx, omega, true, p);
     }
-    bsr_residual(A, bb, xx, rr, p);
+    residual<T>(bb, xx, rr);
     MgLevel& C = cmg->lv[0];
     xdg_restrict<T>(ctx, *Dc, Np, Npc, rr, (T*)C.b.p);
     cmg->solve_level<T>(0);
     xdg_prolong_add<T>(ctx, *Dc, Np, Npc, (const T*)C.x.p, xx);
     for (int s = 0; s < post; ++s) {
-        bsr_residual(A, bb, xx, rr, p);
+        residual<T>(bb, xx, rr);
         Dc->block_gemv(rr, xx, omega, true, p);
     }
 }
@@ -561,7 +590,7 @@ void XdgPrecond::precondition(double* xo, const double* bi) {
 
 void XdgPrecond::solve_dev(double* xo, const double* bi) {
     DevBuf rr(N * sizeof(double)), c(N * sizeof(double));
-    bsr_residual(A, bi, xo, rr.p, BH_FP64);
+    residual<double>(bi, xo, rr.as<double>());
     precondition(c.as<double>(), rr.as<double>());
     axpby(ctx, 1.0, c.as<double>(), 1.0, xo, N);
     converged = 1;

@@ -56,6 +56,10 @@ struct XdgDevice {
 };
 
 void xdg_assemble(XdgDevice& X, Vbsr& M);
+// frame <-> compressed layouts of a variable-block matrix: dst_c[i] = src_f[fwd[i]];
+// dst_f[k] = inv[k] >= 0 ? src_c[inv[k]] : 0 (unused frame slots become zero)
+template <typename T> void frame_gather(bh_ctx*, int64_t nc, const int64_t* fwd, const T* src_f, T* dst_c);
+template <typename T> void frame_scatter(bh_ctx*, int64_t nf, const int64_t* inv, const T* src_c, T* dst_f);
 // y <- beta y + alpha A u on the full (cell, species) DOF layout, matrix-free
 void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y);
 // coarse-space transfer: per cell, the first Npc modes of every species block are
@@ -69,6 +73,7 @@ template <typename T> void xdg_prolong_add(bh_ctx*, const Vbsr& A, int Np, int N
 // multigrid of the SIP path (aggregation levels + K-cycle + dense coarsest).
 struct XdgPrecond : bh_solver {
     bh_bsr* A = nullptr;                   // agglomerated matrix (species units)
+    Vbsr* Av = nullptr;                    // or: a caller's variable-block matrix of cell blocks (then Dc == Av)
     Vbsr* Dc = nullptr;                    // its cell diagonal blocks (smoother; cell layout of the transfers)
     int prec = BH_FP32;
     int Np = 0, Npc = 0;
@@ -79,6 +84,7 @@ struct XdgPrecond : bh_solver {
     DevBuf b, x, r, xc;                    // work vectors in the cycle precision
     ~XdgPrecond() override;
     void build(bh_grid* g, int degc, int min_coarse, int kcycle_inner);
+    template <typename T> void residual(const T* b, const T* x, T* r);   // r = b - A x
     template <typename T> void cycle();
     void precondition(double* x, const double* b) override;
     void solve_dev(double* x, const double* b) override;   // x <- x + M(b - A x)

@@ -140,6 +140,18 @@ int bh_op_assemble(bh_op* op, bh_bsr** mat, double* affine, int flags);
  * nnzb blocks of bs*bs doubles, each block row-major. */
 int bh_bsr_create(bh_ctx* ctx, int nrows, int bs, const int64_t* rowptr, const int* col,
                   const double* vals, bh_bsr** mat);
+/* BlockMsrMatrix with variable block sizes (P/BlockMsrMatrix.cs:191-340, block row r
+ * of length blk_len[r] at row/column offset blk_i0[r] of the vector layout, as the
+ * XDG constant frame X/XDGBasis.cs:46-53 places cell j at j * Np * (species count)
+ * with the present species' blocks first and unused slots after them).  rowptr and
+ * col as in bh_bsr_create; vals lists every stored block (blk_len[r] x blk_len[c],
+ * row-major) in rowptr order; blocks of length 0 are skipped.  Internally the rows
+ * are stored back to back ("compressed": row r at the prefix sum of the lengths);
+ * SpMV and solvers take and return vectors in the caller's layout, whose unused
+ * slots read as zero rows and are written as zero.  Lengths 0..64, rows must not
+ * overlap.  bh_bsr_info reports bs = 0 for such a matrix. */
+int bh_bsr_create_v(bh_ctx* ctx, int nrows, const int64_t* rowptr, const int* col, const int64_t* blk_i0,
+                    const int* blk_len, const double* vals, bh_bsr** mat);
 int bh_bsr_destroy(bh_bsr* mat);
 int bh_bsr_info(bh_bsr* mat, int* nrows, int* bs, int64_t* nnzb);
 int bh_bsr_export(bh_bsr* mat, int64_t* rowptr, int* col, double* vals);
@@ -157,7 +169,8 @@ int bh_bjacobi_create(bh_bsr* mat, double omega, int iterations, int prec, bh_so
 /* SoftGMRES (AS/SoftGMRES.cs:75-470), left preconditioned, restart m; tol as
  * ConvergenceCriterion of IterativeSolverConfig.DefaultTermination
  * (AS/IterativeSolverConfig.cs:48-58).  precond may be NULL.  The operator is
- * the matrix `mat`, or the matrix-free SIP operator `op` when mat is NULL. */
+ * the matrix `mat`, or the matrix-free SIP operator `op` when mat is NULL; a
+ * variable-block `mat` (bh_bsr_create_v) makes the solver work on its layout. */
 int bh_gmres_create(bh_bsr* mat, bh_op* op, bh_solver* precond, int restart, double tol, int maxit,
                     bh_solver** solver);
 /* FlexGMRES (AS/FlexGMRES.cs): right preconditioned, flexible (the
@@ -355,6 +368,17 @@ int bh_xdg_extend(bh_xdg* x, const double* ured, double* ufull);
  * by the aggregation multigrid (K-cycle with kcycle_inner inner steps) */
 int bh_xdg_precond_create(bh_xdg* x, int coarse_degree, int min_coarse, double omega, int pre, int post,
                           int kcycle_inner, int prec, bh_solver** solver);
+/* The same two-level preconditioner for a caller's variable-block matrix of cell
+ * blocks (bh_bsr_create_v, one block row per cell of g, each of length 0, Np or 2 Np:
+ * the species blocks of the cell, species A first) in the caller's vector layout;
+ * use it with bh_gmres_create on the same matrix (flexible = 1). */
+int bh_xdg_precond_create_v(bh_grid* g, bh_bsr* mat, int Np, int coarse_degree, int min_coarse, double omega,
+                            int pre, int post, int kcycle_inner, int prec, bh_solver** solver);
+/* The DOF layouts of the full (which = 0) or agglomerated (1) XDG system: per cell,
+ * its offset in the XDG constant frame (X/XDGBasis.cs:46-53: j * 2 Np), its length
+ * (present, non-agglomerated species times Np) and its offset in this library's
+ * compressed layout (bh_xdg_matrix's dof_off); any pointer may be NULL. */
+int bh_xdg_dof_frame(bh_xdg* x, int which, int64_t* frame_i0, int* len, int64_t* compressed_i0);
 /* No counterpart in the reference: in an FP32 preconditioner, apply the cell
  * diagonal blocks with their FP64 inverse and FP64 accumulation (vectors stay
  * FP32); cut cells that couple two species at high contrast have blocks too
