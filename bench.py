@@ -598,10 +598,13 @@ def euler_leg(B, ctx, out, n=96, k=2, steps=20, seed=4):
     out["euler"] = res
 
 
-def cpu_baseline(k, target_s=12.0):
+def cpu_baseline(k, target_s=12.0, threads=None):
     """Oracle (CPU restatement of the reference's quadrature evaluate) on a
-    bounded sample: 3D 24^3 cells, same k and coefficient law."""
+    bounded sample: 3D 24^3 cells, same k and coefficient law; all host threads
+    the oracle's OpenMP runtime offers, or `threads`."""
     import oracle as O
+    if threads is not None:
+        O.lib().orc_set_threads(int(threads))
     n = 24
     x = np.linspace(0.0, 1.0, n + 1)
     g = O.Grid(x, x, x)
@@ -758,12 +761,20 @@ def main():
         ctx.sync()
         ms = ctx.elapsed_ms(0, 1) / args.steps
         err = float(np.linalg.norm(y32.download().astype(np.float64) - ref) / np.linalg.norm(ref))
+        if prec == B.BH_FP16_MFMA:
+            w16 = op.mfma_tuning()[0]      # waves, + 100 for the neighbour-prefetch variant
+            kname32 = f"sip_apply_mfma_kernel<{D}, {k}, {w16 % 100}, {'true' if w16 >= 100 else 'false'}>"
+            modal32 = False
+        else:
+            p32, w32, _ = op.fp32_tuning()
+            modal32 = p32 in ("modal", "tiled")
+            kname32 = (f"sip_apply_tile_kernel<{D}, {k}, float, {w32}>" if p32 == "tiled"
+                       else f"sip_apply_kernel<{D}, {k}, float, {'true' if modal32 else 'false'}>")
         # the FP16-MFMA kernel evaluates the face rule node by node; the FP32 vector
-        # kernel takes the FP64 tuner's face form
-        fl = sip_flops_modal(n, D, k) if (modal and prec == B.BH_FP32) else sip_flops(n, D, k)
+        # kernel the form its own tuner picked
+        fl = sip_flops_modal(n, D, k) if modal32 else sip_flops(n, D, k)
         out[name] = {"kernel_ms": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": bytes32,
-                     "traffic": measured_traffic(n, k, "sip_apply_mfma_kernel<3, 3, 8>" if prec == B.BH_FP16_MFMA
-                                                 else f"sip_apply_kernel<3, 3, float, {'true' if modal else 'false'}>"),
+                     "kernel": kname32, "traffic": measured_traffic(n, k, kname32),
                      "hbm_GBps": bytes32 / (ms * 1e-3) / 1e9, "hbm_frac": bytes32 / (ms * 1e-3) / PEAK_HBM,
                      "algorithmic_flop": fl, "TFLOPs": fl / (ms * 1e-3) / 1e12,
                      "compute_frac": fl / (ms * 1e-3) / mfma_peak, "rel_l2_vs_fp64": err}
@@ -783,6 +794,8 @@ def main():
         euler_leg(B, ctx, out)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(k)
+        # the same sample on one host thread (the reference's evaluate is single-threaded per MPI rank)
+        out["cpu_baseline"]["single_core"] = cpu_baseline(k, target_s=6.0, threads=1)
     print(json.dumps(out), flush=True)
     ctx.close()
 

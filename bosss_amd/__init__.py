@@ -64,6 +64,7 @@ SIGNATURES = {
     "bh_op_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i, _c_i]),
     "bh_op_assemble": (_c_i, [_c_p, _PP, _c_p, _c_i]),
     "bh_op_tuning": (_c_i, [_c_p, _c_p, _c_p]),
+    "bh_op_tuning_fp32": (_c_i, [_c_p, _c_p, _c_p, _c_p]),
     "bh_op_set_fp64_path": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_op_set_fp32_path": (_c_i, [_c_p, _c_i, _c_i]),
     "bh_bsr_create": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_p, _c_p, _PP]),
@@ -470,6 +471,12 @@ class SipOperator(_Owned):
         _check(lib().bh_op_tuning(self.h, cfg.ctypes.data, ms))
         path = {0: None, 1: "vector", 2: "mfma", 3: "modal", 4: "tiled"}[int(cfg[2])]
         return path, int(cfg[3]), int(cfg[4]), ms[1], ms[2]
+
+    def fp32_tuning(self):
+        """FP32 apply choice: ('vector' | 'modal' | 'tiled' | None, waves per workgroup, ms)."""
+        p, w, ms = _c_i(), _c_i(), _c_f()
+        _check(lib().bh_op_tuning_fp32(self.h, ctypes.byref(p), ctypes.byref(w), ctypes.byref(ms)))
+        return {0: None, 1: "vector", 3: "modal", 4: "tiled"}[p.value], w.value, ms.value
 
     def force_fp64_path(self, path, waves=8, wg_per_xcd=64):
         """Pin the FP64 apply to 'vector' (face rule node by node), 'modal' (the vector

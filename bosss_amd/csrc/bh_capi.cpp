@@ -492,6 +492,15 @@ int bh_op_tuning(bh_op* op, int* cfg, float* ms) {
     BH_API_END
 }
 
+int bh_op_tuning_fp32(bh_op* op, int* path, int* waves, float* ms) {
+    BH_API_BEGIN
+    BH_CHECK(op, "bh_op_tuning_fp32: null operator");
+    if (path) *path = op->f32_path;
+    if (waves) *waves = op->f32_tile_w;
+    if (ms) *ms = op->f32_ms;
+    BH_API_END
+}
+
 int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
     BH_API_BEGIN
     BH_CHECK(op, "bh_op_set_fp64_path: null operator");

@@ -10,6 +10,8 @@
 // group-uniform gathers served by L1/L2.  Group partial sums are combined in
 // LDS.  This is the successor of the reference's cell-blocked CSR kernel
 // blockMultiply2 (ilPSP.LinSolvers.monkey/cuda-src/CudaMatrixKernelDP.cu:57).
+#include <type_traits>
+
 #include "bh_internal.hpp"
 
 namespace bh {
@@ -52,13 +54,82 @@ __global__ __launch_bounds__(256) void bsr_spmv_kernel(int nrows, const int64_t*
     }
 }
 
+// The same stream with 16-byte value loads, for block sizes whose columns are whole
+// 16-byte units (bs * sizeof(T) a multiple of 16): a lane reads VW = 16 / sizeof(T)
+// consecutive rows of one column, L = bs / VW lanes cover a column, and the G = 64 / L
+// groups read G consecutive columns per step (bs = 20 FP64: 10 lanes per column, 6
+// columns = 960 contiguous bytes per load instruction).  Each lane keeps VW partial
+// sums; the groups' partials are combined in LDS.
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void bsr_spmv_vec_kernel(int nrows, const int64_t* __restrict__ rowptr,
+                                                           const int* __restrict__ col, const T* __restrict__ vals,
+                                                           const T* __restrict__ x, T* __restrict__ y, T alpha, T beta,
+                                                           const T* __restrict__ b_sub) {
+    constexpr int VW = 16 / sizeof(T);
+    static_assert(BS % VW == 0, "columns must be whole 16-byte units");
+    constexpr int L = BS / VW, G = 64 / L;
+    using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    __shared__ T red[4][VW][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wave;
+    const bool active = r < nrows;
+    const int g = lane / L, li = lane - g * L;
+    T acc[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) acc[v] = 0;
+    if (active && g < G) {
+        const int64_t p0 = rowptr[r], p1 = rowptr[r + 1];
+        const int ncol = (int)(p1 - p0) * BS;
+        const T* __restrict__ base = vals + (size_t)p0 * BS * BS + li * VW;
+#pragma unroll 4
+        for (int q = g; q < ncol; q += G) {
+            const int b = q / BS, cc = q - b * BS;
+            const T xv = x[(size_t)col[p0 + b] * BS + cc];
+            const V a = *reinterpret_cast<const V*>(base + (size_t)q * BS);
+            const T* av = reinterpret_cast<const T*>(&a);
+#pragma unroll
+            for (int v = 0; v < VW; ++v) acc[v] += av[v] * xv;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < VW; ++v) red[wave][v][lane] = acc[v];
+    __syncthreads();
+    if (active && g == 0) {
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+            T s = acc[v];
+#pragma unroll
+            for (int gg = 1; gg < G; ++gg) s += red[wave][v][gg * L + li];
+            const size_t o = (size_t)r * BS + li * VW + v;
+            T out = alpha * s;
+            if (b_sub) out = b_sub[o] - s;      // residual form: y = b - A x
+            else if (beta != T(0)) out += beta * y[o];
+            y[o] = out;
+        }
+    }
+}
+
+template <typename T, int BS>
+static void launch_spmv_bs(dim3 grd, hipStream_t st, int nrows, const int64_t* rp, const int* cl, const T* vals, const T* x,
+                           T* y, T alpha, T beta, const T* bsub) {
+    // 16-byte loads where a column is whole 16-byte units and spans at least 5 lanes:
+    // with fewer lanes per column, more than 12 groups would leave the final LDS
+    // reduction (serial over the groups) longer than the stream it saves
+    if constexpr ((BS * sizeof(T)) % 16 == 0 && BS / (16 / sizeof(T)) >= 5)
+        hipLaunchKernelGGL((bsr_spmv_vec_kernel<T, BS>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
+    else
+        hipLaunchKernelGGL((bsr_spmv_kernel<T, BS>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
+}
+
 template <typename T>
 static void launch_spmv(bh_bsr* A, T alpha, const T* x, T beta, T* y, const T* bsub, const T* vals) {
-    dim3 blk(256), grd((A->nrows + 3) / 4);
+    dim3 grd((A->nrows + 3) / 4);
     hipStream_t st = A->ctx->stream;
     const int64_t* rp = A->rowptr.as<int64_t>();
     const int* cl = A->col.as<int>();
-#define BH_CASE(B) case B: hipLaunchKernelGGL((bsr_spmv_kernel<T, B>), grd, blk, 0, st, A->nrows, rp, cl, vals, x, y, alpha, beta, bsub); break;
+    // the 16-byte value loads need a 16-byte aligned value array (device allocations are)
+    BH_CHECK(((uintptr_t)vals & 15) == 0, "bsr_spmv: value array not 16-byte aligned");
+#define BH_CASE(B) case B: launch_spmv_bs<T, B>(grd, st, A->nrows, rp, cl, vals, x, y, alpha, beta, bsub); break;
     switch (A->bs) {
         BH_CASE(1) BH_CASE(2) BH_CASE(3) BH_CASE(4) BH_CASE(5) BH_CASE(6) BH_CASE(8) BH_CASE(10)
         BH_CASE(15) BH_CASE(16) BH_CASE(20) BH_CASE(21) BH_CASE(28) BH_CASE(32) BH_CASE(35) BH_CASE(50) BH_CASE(56)

@@ -118,6 +118,10 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
  * separately, over the modal forms, on its own first call.
  * cfg holds 5 ints, ms 3 floats. */
 int bh_op_tuning(bh_op* op, int* cfg, float* ms);
+/* No counterpart in the reference: the FP32 apply's choice (picked on its first
+ * call): path 0 not yet, 1 face rule node by node, 3 modal, 4 tiled modal with
+ * `waves` waves per workgroup; ms its timed duration.  Any pointer may be NULL. */
+int bh_op_tuning_fp32(bh_op* op, int* path, int* waves, float* ms);
 /* No counterpart in the reference: pin the FP64 apply to the vector kernel
  * (path 1: face rule node by node; path 3: modal face terms; path 4: tiled modal
  * face terms, waves in {2, 4, 8} per workgroup) or the FP64-MFMA kernel (path 2,
