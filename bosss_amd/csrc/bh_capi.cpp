@@ -519,6 +519,7 @@ int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd) {
         BH_CHECK(waves == 2 || waves == 4 || waves == 8, "bh_op_set_fp64_path: the tiled kernel takes 2, 4 or 8 waves");
         op->f64_tile_w = waves;
     }
+    if (path == 0) bh::sip_forget_tuning(op, BH_FP64);
     op->f64_path = path;
     BH_API_END
 }
@@ -530,6 +531,7 @@ int bh_op_set_fp32_path(bh_op* op, int path, int waves) {
              "bh_op_set_fp32_path: path must be 0 (autotune), 1 (face rule node by node), 3 (modal face terms) or 4 "
              "(tiled modal face terms)");
     if (path == 4) BH_CHECK(waves == 2 || waves == 4 || waves == 8, "bh_op_set_fp32_path: the tiled kernel takes 2, 4 or 8 waves");
+    if (path == 0) bh::sip_forget_tuning(op, BH_FP32);
     op->f32_path = path;
     op->f32_tile_w = path == 4 ? waves : 0;
     BH_API_END

@@ -251,6 +251,8 @@ struct bh_solver {
 namespace bh {
 // kernels / launchers (defined in the .hip files)
 // y = alpha A u + beta y (+ affine); with b0: y = b0 + alpha A u (the residual form, one pass)
+// forget the process-wide tuning decision for op's shape and precision (re-arms the tuner)
+void sip_forget_tuning(bh_op* op, int prec);
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec,
                const void* b0 = nullptr);
 // FP32 vectors, FP16 MFMA contractions with FP32 accumulation

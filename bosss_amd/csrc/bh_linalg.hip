@@ -109,13 +109,70 @@ __global__ __launch_bounds__(256) void bsr_spmv_vec_kernel(int nrows, const int6
     }
 }
 
+// Small blocks (bs <= 6: the k = 1 levels and the aggregation levels): one group of
+// bs lanes per block row, RPW = 64 / bs block rows per wave.  Lane j of a group reads
+// column j of each block of its row (bs consecutive values), multiplies it by x_j of
+// the block's column and keeps bs partial sums; the group's bs lanes then exchange
+// their partials so that lane i ends with row i, and consecutive lanes write
+// consecutive rows.  A wave thus covers RPW rows at once instead of one, and the
+// dependent loads (row pointer, block columns, x) are paid once per RPW rows.
+template <typename T, int BS>
+__global__ __launch_bounds__(256) void bsr_spmv_small_kernel(int nrows, const int64_t* __restrict__ rowptr,
+                                                             const int* __restrict__ col, const T* __restrict__ vals,
+                                                             const T* __restrict__ x, T* __restrict__ y, T alpha, T beta,
+                                                             const T* __restrict__ b_sub) {
+    constexpr int RPW = 64 / BS;
+    const int lane = threadIdx.x & 63;
+    const int q = lane / BS, j = lane - q * BS;
+    const int r = (blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW + q;
+    const bool active = q < RPW && r < nrows;
+    T acc[BS];
+#pragma unroll
+    for (int i = 0; i < BS; ++i) acc[i] = 0;
+    if (active) {
+        const int64_t p0 = rowptr[r], p1 = rowptr[r + 1];
+#pragma unroll 4
+        for (int64_t p = p0; p < p1; ++p) {
+            const T xj = x[(size_t)col[p] * BS + j];
+            const T* __restrict__ cj = vals + (size_t)p * BS * BS + (size_t)j * BS;
+            T v[BS];
+#pragma unroll
+            for (int i = 0; i < BS; ++i) v[i] = cj[i];
+#pragma unroll
+            for (int i = 0; i < BS; ++i) acc[i] += v[i] * xj;
+        }
+    }
+    // row i of the group = sum over its lanes j of acc_j[i]; lane i keeps it
+    const int base = lane - j;
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+        T t = 0;
+#pragma unroll
+        for (int jj = 0; jj < BS; ++jj) t += __shfl(acc[i], base + jj, 64);
+        if (j == i) s = t;
+    }
+    if (active) {
+        const size_t o = (size_t)r * BS + j;
+        T out = alpha * s;
+        if (b_sub) out = b_sub[o] - s;          // residual form: y = b - A x
+        else if (beta != T(0)) out += beta * y[o];
+        y[o] = out;
+    }
+}
+
 template <typename T, int BS>
 static void launch_spmv_bs(dim3 grd, hipStream_t st, int nrows, const int64_t* rp, const int* cl, const T* vals, const T* x,
                            T* y, T alpha, T beta, const T* bsub) {
-    // 16-byte loads where a column is whole 16-byte units and spans at least 5 lanes:
-    // with fewer lanes per column, more than 12 groups would leave the final LDS
-    // reduction (serial over the groups) longer than the stream it saves
-    if constexpr ((BS * sizeof(T)) % 16 == 0 && BS / (16 / sizeof(T)) >= 5)
+    // small blocks: several rows per wave; 16-byte loads where a column is whole 16-byte
+    // units and spans at least 5 lanes (with fewer lanes per column, more than 12 groups
+    // would leave the final LDS reduction, serial over the groups, longer than the
+    // stream it saves); else the one-row-per-wave stream
+    if constexpr (BS <= 6) {
+        constexpr int RPW = 64 / BS;
+        const dim3 g2((unsigned)((nrows + 4 * RPW - 1) / (4 * RPW)));
+        hipLaunchKernelGGL((bsr_spmv_small_kernel<T, BS>), g2, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
+    } else if constexpr ((BS * sizeof(T)) % 16 == 0 && BS / (16 / sizeof(T)) >= 5)
         hipLaunchKernelGGL((bsr_spmv_vec_kernel<T, BS>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
     else
         hipLaunchKernelGGL((bsr_spmv_kernel<T, BS>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);

@@ -15,8 +15,11 @@
 // Each cell evaluates its own side of all 2D faces (the neighbour's trace is
 // recomputed from the neighbour's modes, read through L2), so y is written
 // exactly once per apply: HBM traffic = u + y + nu.
+#include <array>
 #include <cmath>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "bh_internal.hpp"
@@ -775,6 +778,32 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
 #pragma unroll
     for (int n = 0; n < Np; ++n) { uP[n] = up[n]; y[n] = 0; }
     const R nuP = a.nu[c];
+    // Small cells (k = 1, and k = 2 in FP32): the neighbour rows that come from memory
+    // (both DZ sides; the DT side of the workgroup's first or last wave) are loaded
+    // here, so their latency overlaps the DF and DT work instead of following it.  A
+    // boundary side loads the cell itself (finite, multiplied by zero flux weights).
+    constexpr bool PRE = Np * (int)sizeof(R) <= 40;
+    R zq[2][PRE && D == 3 ? Np : 1], tq[PRE ? Np : 1];
+    if constexpr (PRE) {
+        if constexpr (D == 3) {
+            static_for<2>([&](auto SS) {
+                constexpr int s = SS;
+                int nidx[3] = {idx[0], idx[1], idx[2]};
+                nidx[2] += (s ? idx[2] == nn[2] - 1 : idx[2] == 0) ? 0 : (s ? 1 : -1);
+                const R* uq = a.u + (size_t)encode<D>(nidx, a.n0, a.n1) * Np;
+#pragma unroll
+                for (int n = 0; n < Np; ++n) zq[s][n] = uq[n];
+            });
+        }
+        if (w == 0 || w == W - 1) {   // wave-uniform
+            const int s = w == 0 ? 0 : 1;
+            int nidx[3] = {idx[0], idx[1], idx[2]};
+            nidx[DT] += (s ? idx[DT] == nn[DT] - 1 : idx[DT] == 0) ? 0 : (s ? 1 : -1);
+            const R* uq = a.u + (size_t)encode<D>(nidx, a.n0, a.n1) * Np;
+#pragma unroll
+            for (int n = 0; n < Np; ++n) tq[n] = uq[n];
+        }
+    }
     R ih[3] = {1, 1, 1}, s2[D], cjP = 0;
     for (int d = 0; d < D; ++d) {
         ih[d] = R(a.width[(3 + d) * a.nmax + idx[d]]);
@@ -864,9 +893,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
             const auto g = side_geom<D, R, d, s>(a, idx, nn, cjP, ihd, pen);
             R un[Np];
             const R nuQ = a.nu[g.cq];
-            const R* uq = a.u + (size_t)g.cq * Np;
+            if constexpr (PRE) {
 #pragma unroll
-            for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                for (int n = 0; n < Np; ++n) un[n] = zq[s][n];
+            } else {
+                const R* uq = a.u + (size_t)g.cq * Np;
+#pragma unroll
+                for (int n = 0; n < Np; ++n) un[n] = uq[n];
+            }
             const SideCoef<R> cf = side_coef<R, s>(g.bnd, g.skip, R(2) * ihd, g.gQ, g.rq, nuP, nuQ, g.mu);
             face_side_traces<D, K, R, d, s>(uP, cf, [&](auto F, auto, R& An, R& Bn) {
                 modal_trace<D, K, R, d, 1 - s, decltype(F)::value>(un, An, Bn);
@@ -893,9 +927,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
                 }, y);
             } else {
                 R un[Np];
-                const R* uq = a.u + (size_t)g.cq * Np;
+                if constexpr (PRE) {
 #pragma unroll
-                for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                    for (int n = 0; n < Np; ++n) un[n] = tq[n];
+                } else {
+                    const R* uq = a.u + (size_t)g.cq * Np;
+#pragma unroll
+                    for (int n = 0; n < Np; ++n) un[n] = uq[n];
+                }
                 face_side_traces<D, K, R, d, s>(uP, cf, [&](auto F, auto, R& An, R& Bn) {
                     modal_trace<D, K, R, d, 1 - s, decltype(F)::value>(un, An, Bn);
                 }, y);
@@ -1103,11 +1142,64 @@ static float tune_vector_forms(bh_op* op, bool with_node, int& form, int& tile_w
     return best;
 }
 
+static void tune_fp64_path_timed(bh_op* op);
+
+// Tuning decisions are kept per (device, dimension, degree, grid shape, precision) for
+// the process: an operator of a shape already tuned takes the earlier choice without
+// timing again, so every operator of one shape runs the same kernel and equal inputs
+// give bitwise equal outputs across operator instances (the forms agree only up to
+// rounding).  Re-arming the tuner of an operator (path 0) forgets its shape's entry.
+namespace {
+struct TuneRec {
+    int path = 0, tile_w = 0, waves = 0, wg_per_xcd = 0;
+    float ms_vector = 0.f, ms_mfma = 0.f;
+};
+using TuneKey = std::array<int, 7>;
+std::mutex tune_mu;
+std::map<TuneKey, TuneRec> tune_cache;
+TuneKey tune_key(const bh_op* op, int prec) {
+    const bh_grid* g = op->grid;
+    return {op->ctx->device, op->D, op->k, g->n[0], g->n[1], g->n[2], prec};
+}
+}  // namespace
+
+void sip_forget_tuning(bh_op* op, int prec) {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache.erase(tune_key(op, prec));
+}
+
 // Autotuner of the FP64 apply: times the vector kernel (face terms node by node,
 // modal, tiled modal) and the FP64-MFMA kernel (waves per workgroup x workgroups
 // per XCD) on a scratch vector of this operator and keeps the fastest; all
 // compute the same integrals, up to rounding.
 static void tune_fp64_path(bh_op* op) {
+    {
+        std::lock_guard<std::mutex> lk(tune_mu);
+        auto it = tune_cache.find(tune_key(op, BH_FP64));
+        if (it != tune_cache.end()) {
+            const TuneRec& t = it->second;
+            op->f64_path = t.path;
+            op->f64_tile_w = t.tile_w;
+            op->f64_waves = t.waves;
+            op->f64_wg_per_xcd = t.wg_per_xcd;
+            op->f64_ms_vector = t.ms_vector;
+            op->f64_ms_mfma = t.ms_mfma;
+            return;
+        }
+    }
+    tune_fp64_path_timed(op);
+    TuneRec t;
+    t.path = op->f64_path;
+    t.tile_w = op->f64_tile_w;
+    t.waves = op->f64_waves;
+    t.wg_per_xcd = op->f64_wg_per_xcd;
+    t.ms_vector = op->f64_ms_vector;
+    t.ms_mfma = op->f64_ms_mfma;
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache[tune_key(op, BH_FP64)] = t;
+}
+
+static void tune_fp64_path_timed(bh_op* op) {
     int form = 3, tw = 0;
     const float tv = tune_vector_forms<double>(op, true, form, tw);
     op->f64_path = form;
@@ -1132,10 +1224,23 @@ static void tune_fp64_path(bh_op* op) {
 
 // The FP32 apply (multigrid levels) is tuned on its own, over the modal forms.
 static void tune_fp32_path(bh_op* op) {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tune_cache.find(tune_key(op, BH_FP32));
+    if (it != tune_cache.end()) {
+        op->f32_path = it->second.path;
+        op->f32_tile_w = it->second.tile_w;
+        op->f32_ms = it->second.ms_vector;
+        return;
+    }
     int form = 3, tw = 0;
     op->f32_ms = tune_vector_forms<float>(op, false, form, tw);
     op->f32_path = form;
     op->f32_tile_w = tw;
+    TuneRec t;
+    t.path = form;
+    t.tile_w = tw;
+    t.ms_vector = op->f32_ms;
+    tune_cache[tune_key(op, BH_FP32)] = t;
 }
 
 void sip_apply(bh_op* op, double alpha, const void* u, double beta, void* y, bool add_affine, int prec, const void* b0) {

@@ -115,7 +115,10 @@ int bh_op_evaluate(bh_op* op, double alpha, const double* u, double beta, double
  * exchanged across lanes and waves (tiled); cfg[3] the chosen kernel's waves per
  * workgroup (MFMA or tiled), cfg[4] the MFMA kernel's workgroups per XCD; ms[1],
  * ms[2] the timed best vector and best MFMA durations.  The FP32 apply is tuned
- * separately, over the modal forms, on its own first call.
+ * separately, over the modal forms, on its own first call.  A decision holds for
+ * every operator of the same device, dimension, degree, grid shape and precision
+ * in the process (later operators of that shape take it without timing), so equal
+ * inputs give bitwise equal outputs across operators.
  * cfg holds 5 ints, ms 3 floats. */
 int bh_op_tuning(bh_op* op, int* cfg, float* ms);
 /* No counterpart in the reference: the FP32 apply's choice (picked on its first
@@ -126,7 +129,8 @@ int bh_op_tuning_fp32(bh_op* op, int* path, int* waves, float* ms);
  * (path 1: face rule node by node; path 3: modal face terms; path 4: tiled modal
  * face terms, waves in {2, 4, 8} per workgroup) or the FP64-MFMA kernel (path 2,
  * launch shape waves in {4, 8} per workgroup, wg_per_xcd >= 1) instead of timing
- * them on the first call; path 0 re-arms the autotuner.  All kernels give the
+ * them on the first call; path 0 re-arms the autotuner (and forgets the decision
+ * kept for this operator's shape).  All kernels give the
  * same result up to summation order. */
 int bh_op_set_fp64_path(bh_op* op, int path, int waves, int wg_per_xcd);
 /* No counterpart in the reference: pin the FP32 apply (multigrid levels) to path 1
