@@ -92,6 +92,9 @@ SIGNATURES = {
     "bh_mlp_upstream": (_c_i, [_c_p, _c_p, _c_p, _c_d, _c_p, _c_i, _c_i]),
     "bh_mlp_targets": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_i, _c_i]),
     "bh_pmg_set_learned": (_c_i, [_c_p, _c_p]),
+    "bh_orthomg_create": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i, _c_i, _c_i, _c_d, _c_i, _PP]),
+    "bh_orthomg_set_learned": (_c_i, [_c_p, _c_p]),
+    "bh_orthomg_info": (_c_i, [_c_p, _c_p]),
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_schwarz": (_c_i, [_c_p, _c_i]),
@@ -732,6 +735,31 @@ class EulerOperator(_Owned):
         up, fu = _vec_args(U)
         _check(lib().bh_euler_tvd3(self.h, up, dt, steps, prec, fu))
         return U
+
+
+class OrthonormalizationMultigrid(_Solver):
+    """OrthonormalizationMultigrid (bh_orthomg_create) on the finest level of a
+    PMultigrid: pre-smoother, coarse-grid and post-smoother corrections (and an
+    optional learned one), each orthonormalised in the A-image against all earlier
+    corrections of the solve and added with the residual-minimising coefficient."""
+
+    def __init__(self, mg, op, M=None, pre=1, NoOfPostSmootherSweeps=1, coarse_reps=1, max_hist=64,
+                 ConvergenceCriterion=1e-10, MaxSolverIterations=100):
+        h = ctypes.c_void_p()
+        _check(lib().bh_orthomg_create(mg.h, M.h if M else None, op.h, pre, NoOfPostSmootherSweeps, coarse_reps,
+                                       max_hist, ConvergenceCriterion, MaxSolverIterations, ctypes.byref(h)))
+        super().__init__(op.ctx, h.value, op.N, (mg, op, M))
+        self._net = None
+
+    def set_learned(self, net):
+        _check(lib().bh_orthomg_set_learned(self.h, net.h if net is not None else None))
+        self._net = net
+
+    def info(self):
+        """dict(added, cancellations, random, restarts) over the solver's life."""
+        v = np.zeros(4, np.int32)
+        _check(lib().bh_orthomg_info(self.h, v.ctypes.data))
+        return dict(added=int(v[0]), cancellations=int(v[1]), random=int(v[2]), restarts=int(v[3]))
 
 
 class XdgPoisson(_Owned):

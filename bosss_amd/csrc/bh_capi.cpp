@@ -9,6 +9,7 @@
 #include "bh_mlp.hpp"
 #include "bh_euler.hpp"
 #include "bh_xdgdev.hpp"
+#include "bh_ortho.hpp"
 
 namespace bh {
 const std::string& last_error();
@@ -980,6 +981,47 @@ int bh_pmg_set_learned(bh_solver* s, bh_mlp* w) {
     mg->attach_learned(w ? w->m : nullptr);
     BH_API_END
 }
+int bh_orthomg_create(bh_solver* mgs, bh_bsr* M, bh_op* op, int pre, int post_sweeps, int coarse_reps, int max_hist,
+                      double tol, int maxit, bh_solver** out) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(mgs);
+    BH_CHECK(mg && op && out, "bh_orthomg_create: needs a multigrid preconditioner (bh_pmg_create) and the operator");
+    BH_CHECK(!mg->equilib, "bh_orthomg_create: the block-equilibrated hierarchy is not supported");
+    BH_CHECK(mg->lv.size() >= 2, "bh_orthomg_create: the multigrid needs a coarser level");
+    BH_CHECK(!M || !M->var, "bh_orthomg_create: uniform-block matrices only");
+    BH_CHECK(pre == 0 || pre == 1, "bh_orthomg_create: pre must be 0 or 1");
+    BH_CHECK(post_sweeps >= 0 && coarse_reps >= 0 && pre + post_sweeps + coarse_reps > 0,
+             "bh_orthomg_create: need at least one correction per iteration");
+    BH_CHECK(max_hist >= 1 && max_hist <= 128, "bh_orthomg_create: max_hist must be in 1..128");
+    BH_CHECK(maxit >= 1 && tol >= 0, "bh_orthomg_create: need maxit >= 1 and tol >= 0");
+    LinOp A;
+    A.M = M;
+    A.op = M ? nullptr : op;
+    A.N = M ? (int64_t)M->nrows * M->bs : op->N;
+    BH_CHECK(A.N == mg->N && op->N == mg->N, "bh_orthomg_create: operator and multigrid sizes differ");
+    *out = new OrthoMG(mg, A, op, pre, post_sweeps, coarse_reps, max_hist, tol, maxit);
+    BH_API_END
+}
+
+int bh_orthomg_set_learned(bh_solver* s, bh_mlp* w) {
+    BH_API_BEGIN
+    OrthoMG* om = dynamic_cast<OrthoMG*>(s);
+    BH_CHECK(om, "bh_orthomg_set_learned: solver is not an orthonormalization multigrid");
+    om->attach_learned(w ? w->m : nullptr);
+    BH_API_END
+}
+
+int bh_orthomg_info(bh_solver* s, int* info) {
+    BH_API_BEGIN
+    OrthoMG* om = dynamic_cast<OrthoMG*>(s);
+    BH_CHECK(om && info, "bh_orthomg_info: solver is not an orthonormalization multigrid");
+    info[0] = om->n_added;
+    info[1] = om->n_cancel;
+    info[2] = om->n_random;
+    info[3] = om->n_restart;
+    BH_API_END
+}
+
 int bh_pmg_learned_stats(bh_solver* s, int* acc, int* rej) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);

@@ -269,6 +269,7 @@ void bjacobi_sweep(bh_bsr* A, const void* dinv, double omega, const void* b, con
 void convert_f64_f32(bh_ctx* ctx, const double* src, float* dst, int64_t n);
 void convert_f32_f64(bh_ctx* ctx, const float* src, double* dst, int64_t n);
 void fill(bh_ctx* ctx, double* x, double v, int64_t n);
+void fill_random(bh_ctx* ctx, double* x, int64_t n, uint64_t seed);   // uniform [0, 1)
 void fill32(bh_ctx* ctx, float* x, float v, int64_t n);
 void copy(bh_ctx* ctx, double* dst, const double* src, int64_t n);
 void copy_bytes(bh_ctx* ctx, void* dst, const void* src, size_t bytes);

@@ -510,6 +510,21 @@ void convert_f64_f32(bh_ctx* ctx, const double* s, float* d, int64_t n) {
 void convert_f32_f64(bh_ctx* ctx, const float* s, double* d, int64_t n) {
     hipLaunchKernelGGL(cvt_f2d, grid_for(n), dim3(256), 0, ctx->stream, s, d, n);
 }
+// x_i uniform in [0, 1) from a counter-based hash of (seed, i) (splitmix64 finaliser)
+__global__ void fill_random_k(double* __restrict__ x, int64_t n, uint64_t seed) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + (uint64_t)i * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        x[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+}
+void fill_random(bh_ctx* ctx, double* x, int64_t n, uint64_t seed) {
+    hipLaunchKernelGGL(fill_random_k, grid_for(n), dim3(256), 0, ctx->stream, x, n, seed);
+    BH_HIP(hipGetLastError());
+}
+
 void fill(bh_ctx* ctx, double* x, double v, int64_t n) {
     hipLaunchKernelGGL((fill_k<double>), grid_for(n), dim3(256), 0, ctx->stream, x, v, n);
 }

@@ -152,6 +152,8 @@ struct PMultigrid : bh_solver {
     template <typename T> void residual(int l, const T* b, const T* x, T* r);
     template <typename T> void apply(int l, const T* x, T* y);   // y = A_l x
     template <typename T> void smooth(int l, const T* r, T* x, bool add);   // x (+)= omega M_l^-1 r
+    template <typename T> void restrict_to(int l, const T* r);     // lv[l+1].b = R_l r
+    template <typename T> void prolong_add(int l, T* x);           // x += P_l lv[l+1].x
     void set_schwarz(bool on);     // aggregate-block Schwarz on every level that has aggregates
     void set_equilib();            // DiagBlockEquilib (assembled hierarchies; irreversible)
     int op_prec(int l, bool f32) const {

@@ -450,20 +450,37 @@ void PMultigrid::vcycle(int l, const T* b, T* x) {
         smooth<T>(l, r, x, true);
     }
     residual<T>(l, b, x, r);
+    restrict_to<T>(l, r);
+    solve_level<T>(l + 1);
+    prolong_add<T>(l, x);
+    for (int s = 0; s < post; ++s) {
+        residual<T>(l, b, x, r);
+        smooth<T>(l, r, x, true);
+    }
+}
+
+template <typename T>
+void PMultigrid::restrict_to(int l, const T* r) {
+    MgLevel& L = lv[l];
     MgLevel& C = lv[l + 1];
+    const bool f32 = sizeof(T) == 4;
     const T* P = (const T*)(f32 ? L.P32.p : L.P64.p);
     const T* Pt = (const T*)(f32 ? L.eqP32.p : L.eqP64.p);
     if (equilib) rect_restrict<T>(ctx, C.J, L.bs, C.bs, L.eqcptr.as<int>(), L.eqcidx.as<int>(), Pt, r, vec<T>(C.b));
     else if (L.kind == 0) p_restrict<T>(ctx, L.J, L.bs, C.bs, r, vec<T>(C.b));
     else agg_restrict<T>(ctx, C.J, L.bs, L.cptr.as<int>(), L.cidx.as<int>(), L.pidx.as<int>(), P, r, vec<T>(C.b));
-    solve_level<T>(l + 1);
+}
+
+template <typename T>
+void PMultigrid::prolong_add(int l, T* x) {
+    MgLevel& L = lv[l];
+    MgLevel& C = lv[l + 1];
+    const bool f32 = sizeof(T) == 4;
+    const T* P = (const T*)(f32 ? L.P32.p : L.P64.p);
+    const T* Pt = (const T*)(f32 ? L.eqP32.p : L.eqP64.p);
     if (equilib) rect_prolong_add<T>(ctx, L.J, L.bs, C.bs, L.eqparent.as<int>(), Pt, vec<T>(C.x), x);
     else if (L.kind == 0) p_prolong_add<T>(ctx, L.J, L.bs, C.bs, vec<T>(C.x), x);
     else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), L.pidx.as<int>(), P, vec<T>(C.x), x);
-    for (int s = 0; s < post; ++s) {
-        residual<T>(l, b, x, r);
-        smooth<T>(l, r, x, true);
-    }
 }
 
 // smoother of level l: block-Jacobi on the cell blocks (AS/BlockJacobi.cs:196-255), or
@@ -753,6 +770,12 @@ void PMultigrid::solve_dev(double* x, const double* b) {
     converged = 1;
 }
 
+template void PMultigrid::restrict_to<float>(int, const float*);
+template void PMultigrid::restrict_to<double>(int, const double*);
+template void PMultigrid::prolong_add<float>(int, float*);
+template void PMultigrid::prolong_add<double>(int, double*);
+template void PMultigrid::residual<float>(int, const float*, const float*, float*);
+template void PMultigrid::residual<double>(int, const double*, const double*, double*);
 template void PMultigrid::vcycle<float>(int, const float*, float*);
 template void PMultigrid::vcycle<double>(int, const double*, double*);
 template void PMultigrid::vcycle_g<float>(int);

@@ -37,6 +37,7 @@ typedef struct bh_grid bh_grid;
 typedef struct bh_op bh_op;
 typedef struct bh_bsr bh_bsr;
 typedef struct bh_solver bh_solver;
+typedef struct bh_mlp bh_mlp;
 
 #define BH_DEVICE_PTR 1 /* vector arguments are device pointers */
 
@@ -211,6 +212,35 @@ int bh_pmg_create(bh_op* op, bh_bsr* fine_mat, const int* degrees, int ndeg, int
  * bh_pmg_create or bh_xdg_precond_create preconditioner (the whole cycle without
  * a K-cycle, else the K-cycle's inner cycle).  Results are identical either way. */
 int bh_pmg_set_graphs(bh_solver* pmg, int enable);
+/* OrthonormalizationMultigrid (AS/OrthonormalizationMultigrid.cs:797-1080) on the
+ * finest level of the multigrid `pmg`: per iteration the pre-smoother's correction
+ * (pre = 1; `pmg`'s level-0 smoother with its `pre` sweeps from a zero guess), the
+ * coarse-grid correction `coarse_reps` times (restriction, `pmg` from level 1,
+ * prolongation) and `post_sweeps` post-smoother corrections (`post` sweeps each);
+ * each correction c is orthonormalised in the A-image against all earlier ones of
+ * the solve (AddSol, :219-340: |A c| normalised, classical Gram-Schmidt in two
+ * passes, re-orthonormalisation from the rescaled remainder on severe cancellation,
+ * a random vector when nothing is left) and added with the residual-minimising
+ * coefficient (MinimizeResidual, :149-215; the residual recomputed as b - A x after
+ * each correction); a residual increase beyond 20 % fails with status -5 (the
+ * reference's ArithmeticException).
+ * A cancellation in a post-smoother correction skips the next pre-smoother
+ * (:1047-1055).  Termination as IterativeSolverConfig.DefaultTermination
+ * (tol, maxit).  The operator is `mat` (FP64 BSR), or the matrix-free `op` when
+ * mat is NULL; `op` also supplies the learned correction's features.  max_hist
+ * (1..128) bounds the history (FP64 vector pairs); when it is full the
+ * minimisation restarts from the current iterate (the reference keeps every
+ * correction of a solve). */
+int bh_orthomg_create(bh_solver* pmg, bh_bsr* mat, bh_op* op, int pre, int post_sweeps, int coarse_reps,
+                      int max_hist, double tol, int maxit, bh_solver** solver);
+/* No counterpart in the reference: attach (NULL: detach) a network (bh_mlp_create
+ * with Np + 4D inputs and Np outputs) whose correction of the current residual is
+ * added after the post-smoother corrections of every iteration, orthonormalised
+ * and residual-minimised like them. */
+int bh_orthomg_set_learned(bh_solver* solver, bh_mlp* mlp);
+/* info[4]: corrections added, severe cancellations, random directions, history
+ * restarts (over the solver's life). */
+int bh_orthomg_info(bh_solver* solver, int* info);
 /* ---------------------------------------------------------------- learned correction
  * No counterpart in the reference: a per-cell MLP (in -> 128 -> 128 -> out,
  * ReLU; BF16 weights, FP32 accumulation on MFMA for inference; FP32 backward
@@ -220,7 +250,6 @@ int bh_pmg_set_graphs(bh_solver* pmg, int enable);
  * Parameter layout (FP32, flat): W1[128][in_p], b1[128], W2[128][128], b2[128],
  * W3[out_p][128], b3[out_p], in_p/out_p = in/out rounded up to 16 (padding zero).
  * Feature / target arrays are row-major [cell][in_p] / [cell][out_p] FP32. */
-typedef struct bh_mlp bh_mlp;
 int bh_mlp_create(bh_ctx* ctx, int n_in, int n_hidden, int n_out, uint64_t seed, bh_mlp** mlp);
 int bh_mlp_destroy(bh_mlp* mlp);
 int bh_mlp_info(bh_mlp* mlp, int* n_params, int* in_p, int* out_p);
