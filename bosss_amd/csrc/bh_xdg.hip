@@ -540,6 +540,120 @@ template void frame_gather<float>(bh_ctx*, int64_t, const int64_t*, const float*
 template void frame_scatter<double>(bh_ctx*, int64_t, const int64_t*, const double*, double*);
 template void frame_scatter<float>(bh_ctx*, int64_t, const int64_t*, const float*, float*);
 
+// ------------------------------------------------------------------ agglomeration Q^T A Q
+// A contribution is one species sub-block S of a full-matrix block (Np x Np, element
+// (m, n) at fv[src + n * rf + m], the full block column-major with rf rows), mapped to
+// the reduced unit block of its row and column targets, with the extension blocks of
+// the row / column species when they are agglomeration sources (E^T S, S E).
+
+// flags[i] = 1 when contribution i's sub-block holds a nonzero
+__global__ void qaq_flags_k(int64_t n, int Np, const int64_t* __restrict__ src, const int* __restrict__ rf,
+                            const double* __restrict__ fv, unsigned char* __restrict__ flags) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* S = fv + src[i];
+    const int R = rf[i];
+    bool nz = false;
+    for (int c = 0; c < Np && !nz; ++c)
+        for (int r = 0; r < Np; ++r)
+            if (S[(size_t)c * R + r] != 0.0) { nz = true; break; }
+    flags[i] = nz ? 1 : 0;
+}
+
+// One workgroup (64 lanes) per reduced unit block b: the sum over its contributions
+// k in [cptr[b], cptr[b+1]) of E_pr^T S E_pc (pr, pc < 0: no extension), formed in LDS
+// in the order E^T S first, then the product with E; written column-major.
+__global__ __launch_bounds__(64) void qaq_gather_k(int64_t nblk, int Np, const int64_t* __restrict__ cptr,
+                                                   const int64_t* __restrict__ src, const int* __restrict__ rf,
+                                                   const int* __restrict__ pr, const int* __restrict__ pc,
+                                                   const double* __restrict__ fv, const double* __restrict__ ext,
+                                                   double* __restrict__ out) {
+    __shared__ double S[XDG_NPMAX * XDG_NPMAX], W[XDG_NPMAX * XDG_NPMAX], E[XDG_NPMAX * XDG_NPMAX];
+    const int64_t b = blockIdx.x;
+    if (b >= nblk) return;
+    const int lane = threadIdx.x, NN = Np * Np;
+    double acc[(XDG_NPMAX * XDG_NPMAX + 63) / 64];
+#pragma unroll
+    for (int e = 0; e < (XDG_NPMAX * XDG_NPMAX + 63) / 64; ++e) acc[e] = 0.0;
+    for (int64_t k = cptr[b]; k < cptr[b + 1]; ++k) {
+        const double* Sg = fv + src[k];
+        const int R = rf[k], er = pr[k], ec = pc[k];
+        for (int e = lane; e < NN; e += 64) {   // S[m][n], row-major in LDS
+            const int m = e / Np, n = e - m * Np;
+            S[e] = Sg[(size_t)n * R + m];
+        }
+        __syncthreads();
+        if (er >= 0) {   // W = E^T S, E row-major [src mode][tgt mode]
+            for (int e = lane; e < NN; e += 64) E[e] = ext[(size_t)er * NN + e];
+            __syncthreads();
+            for (int e = lane; e < NN; e += 64) {
+                const int m = e / Np, n = e - m * Np;
+                double s = 0;
+                for (int t = 0; t < Np; ++t) s += E[t * Np + m] * S[t * Np + n];
+                W[e] = s;
+            }
+            __syncthreads();
+            for (int e = lane; e < NN; e += 64) S[e] = W[e];
+            __syncthreads();
+        }
+        if (ec >= 0) {   // W = S E
+            for (int e = lane; e < NN; e += 64) E[e] = ext[(size_t)ec * NN + e];
+            __syncthreads();
+            for (int e = lane; e < NN; e += 64) {
+                const int m = e / Np, n = e - m * Np;
+                double s = 0;
+                for (int t = 0; t < Np; ++t) s += S[m * Np + t] * E[t * Np + n];
+                W[e] = s;
+            }
+            __syncthreads();
+            for (int e = lane; e < NN; e += 64) S[e] = W[e];
+            __syncthreads();
+        }
+        int j = 0;
+        for (int e = lane; e < NN; e += 64, ++j) acc[j] += S[e];
+        __syncthreads();
+    }
+    int j = 0;
+    for (int e = lane; e < NN; e += 64, ++j) {
+        const int m = e / Np, n = e - m * Np;
+        out[(size_t)b * NN + (size_t)n * Np + m] = acc[j];
+    }
+}
+
+// cell diagonal blocks (species coupled) from the unit blocks: cell r of size R = nu * Np
+// takes unit block dblk[4 r + a * 2 + c] (or zero, -1) at rows a Np.., columns c Np..
+__global__ __launch_bounds__(64) void qaq_dcell_k(int J, int Np, const int* __restrict__ rsz, const int64_t* __restrict__ doff,
+                                                  const int64_t* __restrict__ dblk, const double* __restrict__ uv,
+                                                  double* __restrict__ dv) {
+    const int r = blockIdx.x;
+    if (r >= J) return;
+    const int R = rsz[r];
+    for (int e = threadIdx.x; e < R * R; e += 64) {
+        const int j = e / R, i = e - j * R;   // column-major: entry (i, j)
+        const int a = i / Np, c = j / Np;
+        const int64_t bk = dblk[4 * r + a * 2 + c];
+        dv[doff[r] + e] = bk >= 0 ? uv[bk * Np * Np + (size_t)(j - c * Np) * Np + (i - a * Np)] : 0.0;
+    }
+}
+
+void qaq_flags(bh_ctx* ctx, int64_t n, int Np, const int64_t* src, const int* rf, const double* fv, unsigned char* flags) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(qaq_flags_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n, Np, src, rf, fv, flags);
+    BH_HIP(hipGetLastError());
+}
+void qaq_gather(bh_ctx* ctx, int64_t nblk, int Np, const int64_t* cptr, const int64_t* src, const int* rf, const int* pr,
+                const int* pc, const double* fv, const double* ext, double* out) {
+    BH_CHECK(Np <= XDG_NPMAX, "agglomeration: Np too large for the device Q^T A Q");
+    if (nblk == 0) return;
+    hipLaunchKernelGGL(qaq_gather_k, dim3((unsigned)nblk), dim3(64), 0, ctx->stream, nblk, Np, cptr, src, rf, pr, pc, fv, ext, out);
+    BH_HIP(hipGetLastError());
+}
+void qaq_dcell(bh_ctx* ctx, int J, int Np, const int* rsz, const int64_t* doff, const int64_t* dblk, const double* uv, double* dv) {
+    if (J == 0) return;
+    hipLaunchKernelGGL(qaq_dcell_k, dim3((unsigned)J), dim3(64), 0, ctx->stream, J, Np, rsz, doff, dblk, uv, dv);
+    BH_HIP(hipGetLastError());
+}
+
 template <typename T>
 void xdg_restrict(bh_ctx* ctx, const Vbsr& A, int Np, int Npc, const T* r, T* rc) {
     const int64_t n = (int64_t)A.nrows * Npc;

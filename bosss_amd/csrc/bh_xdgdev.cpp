@@ -208,127 +208,100 @@ void xdg_build(bh_xdg* X) {
             }
         }
     }
-    // reduced (agglomerated) matrix Q^T A Q on the host (FP64)
-    std::vector<double> fv((size_t)X->Afull.nvals);
-    ctx->download(fv.data(), X->Afull.vals.p, fv.size() * sizeof(double));
-    // map (cell, species) -> (reduced cell, position, extension block or -1)
-    std::vector<int> pair_of[2];
-    for (int s = 0; s < 2; ++s) pair_of[s].assign(J, -1);
-    for (int i = 0; i < npair; ++i) pair_of[p.agg_pairs[3 * i + 2]][p.agg_pairs[3 * i]] = i;
-    auto target = [&](int c, int s, int& rc, int& rpos, int& pe) {
-        const int t = p.agg_target[s][c];
-        if (t >= 0) { rc = t; pe = pair_of[s][c]; }
-        else { rc = c; pe = -1; }
-        rpos = spos_red(p, rc, s);
-    };
-    std::vector<std::map<int, std::vector<double>>> rows(J);   // row-major dense blocks
-    std::vector<double> tmp((size_t)Np * Np), tmp2((size_t)Np * Np);
-    for (int r = 0; r < J; ++r) {
-        const int Rf = p.bsz_full[r];
-        for (int64_t bp = X->Afull.rowptr[r]; bp < X->Afull.rowptr[r + 1]; ++bp) {
-            const int c = X->Afull.col[bp];
-            const double* B = &fv[X->Afull.voff[bp]];   // column-major Rf x Cf
-            for (int sr = 0; sr < 2; ++sr) {
-                if (!((p.spmask[r] >> sr) & 1)) continue;
-                for (int sc = 0; sc < 2; ++sc) {
-                    if (!((p.spmask[c] >> sc) & 1)) continue;
-                    const int r0 = spos_of(p, r, sr) * Np, c0 = spos_of(p, c, sc) * Np;
-                    // sub-block S[m][n] = B[r0+m, c0+n]
-                    for (int m = 0; m < Np; ++m)
-                        for (int n = 0; n < Np; ++n) tmp[m * Np + n] = B[(size_t)(c0 + n) * Rf + r0 + m];
-                    int rr, rpos, pr, cc, cpos, pc;
-                    target(r, sr, rr, rpos, pr);
-                    target(c, sc, cc, cpos, pc);
-                    bool zero = true;
-                    for (double v : tmp) if (v != 0.0) { zero = false; break; }
-                    if (zero) continue;
-                    if (pr >= 0) {   // E^T S
-                        const double* E = &X->ext[(size_t)pr * Np * Np];
-                        for (int m = 0; m < Np; ++m)
-                            for (int n = 0; n < Np; ++n) {
-                                double s = 0;
-                                for (int t = 0; t < Np; ++t) s += E[t * Np + m] * tmp[t * Np + n];
-                                tmp2[m * Np + n] = s;
-                            }
-                        tmp.swap(tmp2);
-                    }
-                    if (pc >= 0) {   // S E
-                        const double* E = &X->ext[(size_t)pc * Np * Np];
-                        for (int m = 0; m < Np; ++m)
-                            for (int n = 0; n < Np; ++n) {
-                                double s = 0;
-                                for (int t = 0; t < Np; ++t) s += tmp[m * Np + t] * E[t * Np + n];
-                                tmp2[m * Np + n] = s;
-                            }
-                        tmp.swap(tmp2);
-                    }
-                    const int Rr = p.bsz_red[rr], Cr = p.bsz_red[cc];
-                    auto& blk = rows[rr][cc];
-                    if (blk.empty()) blk.assign((size_t)Rr * Cr, 0.0);
-                    for (int m = 0; m < Np; ++m)
-                        for (int n = 0; n < Np; ++n) blk[(size_t)(rpos * Np + m) * Cr + cpos * Np + n] += tmp[m * Np + n];
-                }
-            }
-        }
-    }
-    fv.clear();
-    fv.shrink_to_fit();
+    // Reduced (agglomerated) matrix Q^T A Q, as a uniform BSR over species units (a
+    // unit = one kept species block of one cell, Np DOFs; cut cells hold two): the DOF
+    // layout is the reduced one, the zero species cross blocks between neighbours drop
+    // out.  The host lists the contributions (species sub-blocks of the full matrix
+    // with their target unit blocks and extension blocks) from the structure alone;
+    // the device flags the nonzero ones, the host builds the unit pattern from the
+    // flags, and the device sums the values per unit block.  The full matrix never
+    // leaves the device.  The cell diagonal blocks (species coupled) for the smoother
+    // are gathered from the unit blocks.
     {
-        // The agglomerated matrix as a uniform BSR over species units (a unit = one
-        // species block of one cell, Np DOFs; cut cells hold two): the DOF layout is
-        // unchanged, the (10/20)-sized cell blocks become Np x Np unit blocks and the
-        // zero species cross blocks between neighbours drop out.  The cell-level
-        // diagonal blocks (species coupled) are kept separately for the smoother.
+        std::vector<int> pair_of[2];
+        for (int s = 0; s < 2; ++s) pair_of[s].assign(J, -1);
+        for (int i = 0; i < npair; ++i) pair_of[p.agg_pairs[3 * i + 2]][p.agg_pairs[3 * i]] = i;
         std::vector<int> unit0(J + 1, 0);
         for (int r = 0; r < J; ++r) unit0[r + 1] = unit0[r] + p.bsz_red[r] / Np;
         const int U = unit0[J];
-        std::vector<int64_t> urp(U + 1, 0);
-        std::vector<int> ucol;
-        std::vector<double> uval;
-        std::vector<std::vector<int>> dcols(J);
-        std::vector<double> dval;
-        {
-            // capacity bounds first: the value arrays reach hundreds of MB, and
-            // growing them by doubling would copy them several times
-            size_t nb = 0, nd = 0;
-            for (int r = 0; r < J; ++r) {
-                const int R = p.bsz_red[r];
-                if (R == 0) continue;
-                nd += (size_t)R * R;
-                size_t cu = rows[r].count(r) ? 0 : R / Np;
-                for (auto& kv : rows[r]) cu += p.bsz_red[kv.first] / Np;
-                nb += (size_t)(R / Np) * cu;
-            }
-            ucol.reserve(nb);
-            uval.reserve(nb * Np * Np);
-            dval.reserve(nd);
-        }
+        auto target = [&](int c, int s, int& unit, int& pe) {
+            const int t = p.agg_target[s][c];
+            const int rc = t >= 0 ? t : c;
+            pe = t >= 0 ? pair_of[s][c] : -1;
+            unit = unit0[rc] + spos_red(p, rc, s);
+        };
+        // contributions in row order of the full matrix
+        std::vector<int64_t> csrc;
+        std::vector<int> crf, cu, cv, cpr, cpc;
         for (int r = 0; r < J; ++r) {
-            const int R = p.bsz_red[r];
-            if (R == 0) continue;
-            if (!rows[r].count(r)) rows[r][r].assign((size_t)R * R, 0.0);
-            dcols[r].push_back(r);
-            const auto& dg = rows[r][r];
-            for (int j = 0; j < R; ++j)                  // column-major cell diagonal block
-                for (int i = 0; i < R; ++i) dval.push_back(dg[(size_t)i * R + j]);
-            for (int a = 0; a < R / Np; ++a) {
-                const int u = unit0[r] + a;
-                for (auto& kv : rows[r]) {
-                    const int c = kv.first, C = p.bsz_red[c];
-                    const auto& blk = kv.second;
-                    for (int b = 0; b < C / Np; ++b) {
-                        const int v = unit0[c] + b;
-                        bool nz = (u == v);
-                        for (int m = 0; m < Np && !nz; ++m)
-                            for (int n = 0; n < Np; ++n)
-                                if (blk[(size_t)(a * Np + m) * C + b * Np + n] != 0.0) { nz = true; break; }
-                        if (!nz) continue;
-                        ucol.push_back(v);
-                        for (int n = 0; n < Np; ++n)          // column-major unit block
-                            for (int m = 0; m < Np; ++m) uval.push_back(blk[(size_t)(a * Np + m) * C + b * Np + n]);
+            const int Rf = p.bsz_full[r];
+            for (int64_t bp = X->Afull.rowptr[r]; bp < X->Afull.rowptr[r + 1]; ++bp) {
+                const int c = X->Afull.col[bp];
+                for (int sr = 0; sr < 2; ++sr) {
+                    if (!((p.spmask[r] >> sr) & 1)) continue;
+                    for (int sc = 0; sc < 2; ++sc) {
+                        if (!((p.spmask[c] >> sc) & 1)) continue;
+                        int u, v, pr, pc;
+                        target(r, sr, u, pr);
+                        target(c, sc, v, pc);
+                        csrc.push_back(X->Afull.voff[bp] + (int64_t)spos_of(p, c, sc) * Np * Rf + spos_of(p, r, sr) * Np);
+                        crf.push_back(Rf);
+                        cu.push_back(u);
+                        cv.push_back(v);
+                        cpr.push_back(pr);
+                        cpc.push_back(pc);
                     }
                 }
-                urp[u + 1] = (int64_t)ucol.size();
+            }
+        }
+        const int64_t nc = (int64_t)csrc.size();
+        std::vector<unsigned char> flg(std::max<int64_t>(nc, 1));
+        {
+            DevBuf dsrc(std::max<int64_t>(nc, 1) * sizeof(int64_t)), drf(std::max<int64_t>(nc, 1) * sizeof(int)), dfl(std::max<int64_t>(nc, 1));
+            ctx->upload(dsrc.p, csrc.data(), nc * sizeof(int64_t));
+            ctx->upload(drf.p, crf.data(), nc * sizeof(int));
+            qaq_flags(ctx, nc, Np, dsrc.as<int64_t>(), drf.as<int>(), X->Afull.vals.as<double>(), (unsigned char*)dfl.p);
+            ctx->download(flg.data(), dfl.p, nc);
+        }
+        // unit pattern: the blocks of nonzero contributions and every diagonal block,
+        // columns ascending per row
+        std::vector<std::vector<int>> ucols(U);
+        for (int u = 0; u < U; ++u) ucols[u].push_back(u);
+        for (int64_t i = 0; i < nc; ++i)
+            if (flg[i]) ucols[cu[i]].push_back(cv[i]);
+        std::vector<int64_t> urp(U + 1, 0);
+        std::vector<int> ucol;
+        for (int u = 0; u < U; ++u) {
+            auto& cl = ucols[u];
+            std::sort(cl.begin(), cl.end());
+            cl.erase(std::unique(cl.begin(), cl.end()), cl.end());
+            ucol.insert(ucol.end(), cl.begin(), cl.end());
+            urp[u + 1] = (int64_t)ucol.size();
+            std::vector<int>().swap(cl);
+        }
+        const int64_t nb = urp[U];
+        auto block_of = [&](int u, int v) -> int64_t {
+            const auto b0 = ucol.begin() + urp[u], b1 = ucol.begin() + urp[u + 1];
+            const auto it = std::lower_bound(b0, b1, v);
+            return (it != b1 && *it == v) ? (int64_t)(it - ucol.begin()) : -1;
+        };
+        // contributions grouped by target block, row order kept within a block
+        std::vector<int64_t> cblk(nc, -1), cptr(nb + 1, 0);
+        for (int64_t i = 0; i < nc; ++i)
+            if (flg[i]) { cblk[i] = block_of(cu[i], cv[i]); ++cptr[cblk[i] + 1]; }
+        for (int64_t b = 0; b < nb; ++b) cptr[b + 1] += cptr[b];
+        const int64_t nk = cptr[nb];
+        std::vector<int64_t> ssrc(std::max<int64_t>(nk, 1));
+        std::vector<int> srf(std::max<int64_t>(nk, 1)), spr(std::max<int64_t>(nk, 1)), spc(std::max<int64_t>(nk, 1));
+        {
+            std::vector<int64_t> fillp(cptr.begin(), cptr.end() - 1);
+            for (int64_t i = 0; i < nc; ++i) {
+                if (cblk[i] < 0) continue;
+                const int64_t k = fillp[cblk[i]]++;
+                ssrc[k] = csrc[i];
+                srf[k] = crf[i];
+                spr[k] = cpr[i];
+                spc[k] = cpc[i];
             }
         }
         bh_bsr* M = new bh_bsr();
@@ -338,15 +311,48 @@ void xdg_build(bh_xdg* X) {
         M->bs = Np;
         M->h_rowptr = urp;
         M->h_col = ucol;
-        M->nnzb = urp[U];
+        M->nnzb = nb;
         M->rowptr.alloc((U + 1) * sizeof(int64_t));
-        M->col.alloc(std::max<size_t>(ucol.size(), 1) * sizeof(int));
+        M->col.alloc(std::max<int64_t>(nb, 1) * sizeof(int));
         ctx->upload(M->rowptr.p, urp.data(), (U + 1) * sizeof(int64_t));
-        ctx->upload(M->col.p, ucol.data(), ucol.size() * sizeof(int));
-        M->vals.alloc(std::max<size_t>(uval.size(), 1) * sizeof(double));
-        ctx->upload(M->vals.p, uval.data(), uval.size() * sizeof(double));
+        ctx->upload(M->col.p, ucol.data(), nb * sizeof(int));
+        M->vals.alloc(std::max<int64_t>(nb, 1) * Np * Np * sizeof(double));
+        {
+            DevBuf dcp((nb + 1) * sizeof(int64_t)), dsrc(std::max<int64_t>(nk, 1) * sizeof(int64_t));
+            DevBuf drf(std::max<int64_t>(nk, 1) * sizeof(int)), dpr(std::max<int64_t>(nk, 1) * sizeof(int)),
+                dpc(std::max<int64_t>(nk, 1) * sizeof(int)), dext(std::max<size_t>(X->ext.size(), 1) * sizeof(double));
+            ctx->upload(dcp.p, cptr.data(), (nb + 1) * sizeof(int64_t));
+            ctx->upload(dsrc.p, ssrc.data(), nk * sizeof(int64_t));
+            ctx->upload(drf.p, srf.data(), nk * sizeof(int));
+            ctx->upload(dpr.p, spr.data(), nk * sizeof(int));
+            ctx->upload(dpc.p, spc.data(), nk * sizeof(int));
+            ctx->upload(dext.p, X->ext.data(), X->ext.size() * sizeof(double));
+            qaq_gather(ctx, nb, Np, dcp.as<int64_t>(), dsrc.as<int64_t>(), drf.as<int>(), dpr.as<int>(), dpc.as<int>(),
+                       X->Afull.vals.as<double>(), dext.as<double>(), M->vals.as<double>());
+            ctx->sync();
+        }
+        // cell diagonal blocks
+        std::vector<std::vector<int>> dcols(J);
+        std::vector<int64_t> dblk((size_t)4 * J, -1);
+        for (int r = 0; r < J; ++r) {
+            if (p.bsz_red[r] == 0) continue;
+            dcols[r].push_back(r);
+            const int nu = p.bsz_red[r] / Np;
+            for (int a = 0; a < nu; ++a)
+                for (int c = 0; c < nu; ++c) dblk[(size_t)4 * r + a * 2 + c] = block_of(unit0[r] + a, unit0[r] + c);
+        }
         X->Dcell.build(ctx, J, p.bsz_red, dcols);
-        ctx->upload(X->Dcell.vals.p, dval.data(), dval.size() * sizeof(double));
+        {
+            std::vector<int64_t> doff(J, 0);
+            for (int r = 0; r < J; ++r)
+                if (X->Dcell.rowptr[r + 1] > X->Dcell.rowptr[r]) doff[r] = X->Dcell.voff[X->Dcell.rowptr[r]];
+            DevBuf ddo(std::max(J, 1) * sizeof(int64_t)), ddb(std::max<size_t>(dblk.size(), 1) * sizeof(int64_t));
+            ctx->upload(ddo.p, doff.data(), J * sizeof(int64_t));
+            ctx->upload(ddb.p, dblk.data(), dblk.size() * sizeof(int64_t));
+            qaq_dcell(ctx, J, Np, X->Dcell.rsz_d.as<int>(), ddo.as<int64_t>(), ddb.as<int64_t>(), M->vals.as<double>(),
+                      X->Dcell.vals.as<double>());
+            ctx->sync();
+        }
     }
     // boundary nodes (Dirichlet data)
     X->bnd_x.clear(); X->bnd_w.clear(); X->bnd_nu.clear(); X->bnd_mu.clear();

@@ -56,6 +56,13 @@ struct XdgDevice {
 };
 
 void xdg_assemble(XdgDevice& X, Vbsr& M);
+// agglomeration Q^T A Q on the device (see bh_xdg.hip): nonzero flags of the contributions,
+// the unit-block sums, and the cell diagonal blocks gathered from the unit blocks
+constexpr int XDG_NPMAX = 10;
+void qaq_flags(bh_ctx*, int64_t n, int Np, const int64_t* src, const int* rf, const double* fv, unsigned char* flags);
+void qaq_gather(bh_ctx*, int64_t nblk, int Np, const int64_t* cptr, const int64_t* src, const int* rf, const int* pr,
+                const int* pc, const double* fv, const double* ext, double* out);
+void qaq_dcell(bh_ctx*, int J, int Np, const int* rsz, const int64_t* doff, const int64_t* dblk, const double* uv, double* dv);
 // frame <-> compressed layouts of a variable-block matrix: dst_c[i] = src_f[fwd[i]];
 // dst_f[k] = inv[k] >= 0 ? src_c[inv[k]] : 0 (unused frame slots become zero)
 template <typename T> void frame_gather(bh_ctx*, int64_t nc, const int64_t* fwd, const T* src_f, T* dst_c);
