@@ -120,6 +120,8 @@ SIGNATURES = {
     "bh_xdg_spmv": (_c_i, [_c_p, _c_i, _c_d, _c_p, _c_d, _c_p, _c_i, _c_i]),
     "bh_xdg_extend": (_c_i, [_c_p, _c_p, _c_p]),
     "bh_xdg_evaluate": (_c_i, [_c_p, _c_d, _c_p, _c_d, _c_p, _c_i]),
+    "bh_xdg_create_poly": (_c_i, [_c_p, _c_i, _c_i, _c_p, _c_d, _c_d, _c_d, _c_d, _c_i, _PP]),
+    "bh_xdg_cut_rule_fallbacks": (_c_i, [_c_p]),
     "bh_xdg_precond_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
     "bh_xdg_precond_create_v": (_c_i, [_c_p, _c_p, _c_i, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
     "bh_xdg_dof_frame": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p]),
@@ -770,20 +772,36 @@ class XdgPoisson(_Owned):
 
     _destroy_fn = "bh_xdg_destroy"
 
-    def __init__(self, grid, k, center, radius, nu_in, nu_out, penalty_base=1.3, agg_threshold=0.1, quad_order=0):
+    def __init__(self, grid, k, center, radius, nu_in, nu_out, penalty_base=1.3, agg_threshold=0.1, quad_order=0,
+                 _handle=None):
         self.grid = grid
         self.ctx = grid.ctx
         self.k = k
-        c = np.zeros(3)
-        c[:grid.D] = center
-        h = ctypes.c_void_p()
-        _check(lib().bh_xdg_create(grid.h, k, c.ctypes.data, radius, nu_in, nu_out, penalty_base, agg_threshold,
-                                   quad_order, ctypes.byref(h)))
-        self._register(self.ctx, h.value)
+        if _handle is None:
+            c = np.zeros(3)
+            c[:grid.D] = center
+            h = ctypes.c_void_p()
+            _check(lib().bh_xdg_create(grid.h, k, c.ctypes.data, radius, nu_in, nu_out, penalty_base, agg_threshold,
+                                       quad_order, ctypes.byref(h)))
+            _handle = h.value
+        self._register(self.ctx, _handle)
         info = np.zeros(13, np.int64)
         _check(lib().bh_xdg_info(self.h, info.ctypes.data))
         (self.N_full, self.N, self.ncut, self.npairs, self.nfailed, self.nelem, self.nnodes, self.nvals_full,
          self.nvals, self.nbnd, self.Np, self.J, self.nunits) = [int(v) for v in info]
+
+    @classmethod
+    def poly(cls, grid, k, coeffs, nu_in, nu_out, penalty_base=1.3, agg_threshold=0.1, quad_order=0):
+        """Level set phi(x) = sum coeffs[i0, i1(, i2)] x^i0 y^i1 z^i2 (an array of shape
+        (deg+1,)*D indexed [i0, i1, i2]); species A where phi < 0 (bh_xdg_create_poly)."""
+        c = np.asarray(coeffs, np.float64)
+        assert c.ndim == grid.D and len(set(c.shape)) == 1
+        deg = c.shape[0] - 1
+        flat = np.ascontiguousarray(c.transpose(tuple(range(grid.D))[::-1]).ravel())   # i0 fastest
+        h = ctypes.c_void_p()
+        _check(lib().bh_xdg_create_poly(grid.h, k, deg, flat.ctypes.data, nu_in, nu_out, penalty_base, agg_threshold,
+                                        quad_order, ctypes.byref(h)))
+        return cls(grid, k, None, None, nu_in, nu_out, _handle=h.value)
 
     def cells(self):
         J = self.J
@@ -911,6 +929,13 @@ class XdgPrecondV(_Solver):
         _check(lib().bh_xdg_precond_create_v(grid.h, M.h, Np, coarse_degree, min_coarse, omega, pre, post,
                                              kcycle_inner, prec, ctypes.byref(h)))
         super().__init__(M.ctx, h.value, M.N, (grid, M))
+
+
+def xdg_cut_rule_fallbacks():
+    """boxes (process-wide) where the polynomial cut rules used the low-order fallback"""
+    v = ctypes.c_long()
+    _check(lib().bh_xdg_cut_rule_fallbacks(ctypes.byref(v)))
+    return v.value
 
 
 def xdg_precond_set_fp64_blocks(precond, on=True):

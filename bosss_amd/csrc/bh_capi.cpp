@@ -1,5 +1,6 @@
 // extern "C" entry points of libbosss_hip.so (declared in include/bosss_hip.h).
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 
@@ -1184,13 +1185,10 @@ void xdg_set_rhs(bh_xdg* X, const double* f, const double* g);
 void xdg_extend(const bh_xdg* X, const double* ured, double* ufull);
 }
 
-int bh_xdg_create(bh_grid* g, int k, const double* center, double radius, double nu_in, double nu_out,
-                  double penalty_base, double agg_threshold, int quad_order, bh_xdg** out) {
-    BH_API_BEGIN
-    BH_CHECK(g && center && out, "bh_xdg_create: null argument");
+static bh_xdg* xdg_make(bh_grid* g, int k, double nu_in, double nu_out, double penalty_base, double agg_threshold,
+                        int quad_order, const std::function<void(XdgProblem&)>& set_ls) {
     BH_CHECK(k >= 1 && k <= 3, "bh_xdg_create: degree must be 1..3");
     BH_CHECK(!(g->D == 3 && k == 3), "bh_xdg_create: D = 3 supports k = 1, 2 (the element kernels hold 4 Np^2 <= 448 entries per wave)");
-    BH_CHECK(radius > 0, "bh_xdg_create: radius must be positive");
     BH_CHECK(agg_threshold >= 0 && agg_threshold < 0.5, "bh_xdg_create: agglomeration threshold must be in [0, 0.5)");
     auto X = std::make_unique<bh_xdg>();
     X->ctx = g->ctx;
@@ -1201,8 +1199,7 @@ int bh_xdg_create(bh_grid* g, int k, const double* center, double radius, double
     p.g = g;
     p.k = k;
     p.q = quad_order > 0 ? quad_order : k + 1;
-    for (int d = 0; d < 3; ++d) p.S.c[d] = d < g->D ? center[d] : 0.0;
-    p.S.r = radius;
+    set_ls(p);
     p.nu[0] = nu_in;
     p.nu[1] = nu_out;
     p.penalty_base = penalty_base;
@@ -1210,7 +1207,44 @@ int bh_xdg_create(bh_grid* g, int k, const double* center, double radius, double
     p.setup();
     BH_CHECK(p.n_failed == 0, "bh_xdg_create: agglomeration failed for " + std::to_string(p.n_failed) + " cells");
     if (X->ctx) xdg_build(X.get());
-    *out = X.release();
+    return X.release();
+}
+
+int bh_xdg_create(bh_grid* g, int k, const double* center, double radius, double nu_in, double nu_out,
+                  double penalty_base, double agg_threshold, int quad_order, bh_xdg** out) {
+    BH_API_BEGIN
+    BH_CHECK(g && center && out, "bh_xdg_create: null argument");
+    BH_CHECK(radius > 0, "bh_xdg_create: radius must be positive");
+    *out = xdg_make(g, k, nu_in, nu_out, penalty_base, agg_threshold, quad_order, [&](XdgProblem& p) {
+        for (int d = 0; d < 3; ++d) p.S.c[d] = d < g->D ? center[d] : 0.0;
+        p.S.r = radius;
+        p.ls_kind = 0;
+    });
+    BH_API_END
+}
+
+int bh_xdg_create_poly(bh_grid* g, int k, int deg, const double* coeffs, double nu_in, double nu_out, double penalty_base,
+                       double agg_threshold, int quad_order, bh_xdg** out) {
+    BH_API_BEGIN
+    BH_CHECK(g && coeffs && out, "bh_xdg_create_poly: null argument");
+    BH_CHECK(deg >= 1 && deg <= 8, "bh_xdg_create_poly: degree per variable must be 1..8");
+    int ncoef = 1;
+    for (int d = 0; d < g->D; ++d) ncoef *= deg + 1;
+    *out = xdg_make(g, k, nu_in, nu_out, penalty_base, agg_threshold, quad_order, [&](XdgProblem& p) {
+        p.ls_kind = 1;
+        p.L.D = g->D;
+        p.L.n = deg;
+        p.L.c.assign(coeffs, coeffs + ncoef);
+        for (int d = 0; d < 3; ++d) p.S.c[d] = 0.0;
+        p.S.r = 0.0;
+    });
+    BH_API_END
+}
+
+int bh_xdg_cut_rule_fallbacks(long* count) {
+    BH_API_BEGIN
+    BH_CHECK(count, "bh_xdg_cut_rule_fallbacks: null argument");
+    *count = poly_fallbacks();
     BH_API_END
 }
 

@@ -24,6 +24,24 @@ struct Sphere {
     }
 };
 
+// Polynomial level set phi(x) = sum over i0, i1, i2 <= n of
+// c[i0 + (n + 1) (i1 + (n + 1) i2)] x^i0 y^i1 z^i2 (2D: i2 = 0); species A is phi < 0
+struct PolyLS {
+    int D = 3, n = 0;
+    std::vector<double> c;
+    double eval(const double* x) const;
+};
+// Cut-cell rules of a polynomial level set on axis-aligned boxes (bh_saye.cpp):
+// poly_classify: -1 phi < 0 on the box, +1 phi > 0, 0 the box may be cut (Bernstein
+// bounds); a face has lo[d] == hi[d] in its normal direction.  species -1 = A, +1 = B;
+// q Gauss points per 1D piece.  poly_fallbacks: boxes where no height direction was
+// found at the subdivision limit (a low-order rule was used there).
+int poly_classify(const PolyLS& L, const double* lo, const double* hi);
+void poly_volume_rule(const PolyLS& L, const double* lo, const double* hi, int species, int q, QRule& out);
+void poly_interface_rule(const PolyLS& L, const double* lo, const double* hi, int q, QRule& out);
+void poly_face_rule(const PolyLS& L, const double* lo, const double* hi, int dir, int species, int q, QRule& out);
+long poly_fallbacks();
+
 // Cut-cell rules for a sphere and an axis-aligned box (bh_xdgsetup.cpp).
 // species: -1 = inside (phi < 0, "A"), +1 = outside ("B").
 void sphere_volume_rule(int D, const double* lo, const double* hi, const Sphere& S, int species, int q, QRule& out);
@@ -50,6 +68,8 @@ struct XdgProblem {
     bh_grid* g;
     int D, k, Np, q;
     Sphere S;
+    int ls_kind = 0;                 // 0: the sphere S (closed-form roots), 1: the polynomial L
+    PolyLS L;
     double nu[2];
     double penalty_base, penalty_deg, agg_threshold;
     // per cell

@@ -409,37 +409,54 @@ void XdgProblem::setup() {
         if (!R.nrm.empty()) nn.insert(nn.end(), R.nrm.begin(), R.nrm.end());
         else nn.insert(nn.end(), R.x.size(), 0.0);
     };
-    // classify cells: distance range of the box to the centre
-    std::vector<int> cut(J, 0);
-    std::vector<int> vol_rule0[2], vol_rulen[2], if_r0(J, 0), if_rn(J, 0);
-    for (int s = 0; s < 2; ++s) { vol_rule0[s].assign(J, 0); vol_rulen[s].assign(J, 0); }
-    ncut = 0;
-    // pass 1: classify by the box's distance range to the centre; candidates get
-    // their species and interface rules computed in parallel
-    std::vector<int> cand_of(J, -1), cand;
-    for (int c = 0; c < J; ++c) {
-        double lo[3], hi[3];
-        box(c, lo, hi);
+    // the level set: a sphere (closed-form roots) or a polynomial (Saye's algorithm);
+    // classify: -1 phi < 0 on the box, +1 phi > 0, 0 possibly cut
+    auto classify = [&](const double* lo, const double* hi) {
+        if (ls_kind == 1) return poly_classify(L, lo, hi);
         double dmin2 = 0, dmax2 = 0;
         for (int d = 0; d < D; ++d) {
             const double a = lo[d] - S.c[d], b = hi[d] - S.c[d];
             dmin2 += (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
             dmax2 += std::max(a * a, b * b);
         }
+        const double r2 = S.r * S.r;
+        return dmax2 <= r2 ? -1 : (dmin2 >= r2 ? +1 : 0);
+    };
+    auto vrule = [&](const double* lo, const double* hi, int sp, QRule& R) {
+        if (ls_kind == 1) poly_volume_rule(L, lo, hi, sp, q, R);
+        else sphere_volume_rule(D, lo, hi, S, sp, q, R);
+    };
+    auto irule = [&](const double* lo, const double* hi, QRule& R) {
+        if (ls_kind == 1) poly_interface_rule(L, lo, hi, q, R);
+        else sphere_interface_rule(D, lo, hi, S, q, R);
+    };
+    auto frule_of = [&](const double* lo, const double* hi, int dir, int sp, QRule& R) {
+        if (ls_kind == 1) poly_face_rule(L, lo, hi, dir, sp, q, R);
+        else sphere_face_rule(D, lo, hi, dir, S, sp, q, R);
+    };
+    std::vector<int> cut(J, 0);
+    std::vector<int> vol_rule0[2], vol_rulen[2], if_r0(J, 0), if_rn(J, 0);
+    for (int s = 0; s < 2; ++s) { vol_rule0[s].assign(J, 0); vol_rulen[s].assign(J, 0); }
+    ncut = 0;
+    // pass 1: classify; candidates get their species and interface rules computed in parallel
+    std::vector<int> cand_of(J, -1), cand;
+    for (int c = 0; c < J; ++c) {
+        double lo[3], hi[3];
+        box(c, lo, hi);
         double V = 1;
         for (int d = 0; d < D; ++d) V *= hi[d] - lo[d];
-        const double r2 = S.r * S.r;
-        if (dmax2 <= r2) { spmask[c] = 1; vol[0][c] = V; }
-        else if (dmin2 >= r2) { spmask[c] = 2; vol[1][c] = V; }
+        const int cl = classify(lo, hi);
+        if (cl < 0) { spmask[c] = 1; vol[0][c] = V; }
+        else if (cl > 0) { spmask[c] = 2; vol[1][c] = V; }
         else { cand_of[c] = (int)cand.size(); cand.push_back(c); }
     }
     std::vector<QRule> crule(3 * cand.size());
     parallel_for((int)cand.size(), [&](int i) {
         double lo[3], hi[3];
         box(cand[i], lo, hi);
-        sphere_volume_rule(D, lo, hi, S, -1, q, crule[3 * i]);
-        sphere_volume_rule(D, lo, hi, S, +1, q, crule[3 * i + 1]);
-        sphere_interface_rule(D, lo, hi, S, q, crule[3 * i + 2]);
+        vrule(lo, hi, -1, crule[3 * i]);
+        vrule(lo, hi, +1, crule[3 * i + 1]);
+        irule(lo, hi, crule[3 * i + 2]);
     });
     // pass 2, in cell order: species volumes, masks and rule slots
     for (int c = 0; c < J; ++c) {
@@ -499,16 +516,10 @@ void XdgProblem::setup() {
                 double A = 1;
                 for (int e = 0; e < D; ++e) if (e != d) A *= fh[e] - fl[e];
                 // face cut?
-                double dmin2 = 0, dmax2 = 0;
-                for (int e = 0; e < D; ++e) {
-                    const double a = fl[e] - S.c[e], b = fh[e] - S.c[e];
-                    dmin2 += (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
-                    dmax2 += std::max(a * a, b * b);
-                }
-                const double r2 = S.r * S.r;
+                const int cl = classify(fl, fh);
                 for (int sp = 0; sp < 2; ++sp) { F.area[sp] = 0; F.rn[sp] = 0; F.tensor[sp] = false; }
-                if (dmax2 <= r2) { F.area[0] = A; F.tensor[0] = true; }
-                else if (dmin2 >= r2) { F.area[1] = A; F.tensor[1] = true; }
+                if (cl < 0) { F.area[0] = A; F.tensor[0] = true; }
+                else if (cl > 0) { F.area[1] = A; F.tensor[1] = true; }
                 else {
                     CutFace cf;
                     cf.face = (int)faces.size();
@@ -523,7 +534,7 @@ void XdgProblem::setup() {
     parallel_for((int)cut_faces.size(), [&](int i) {
         const CutFace& cf = cut_faces[i];
         for (int sp = 0; sp < 2; ++sp)
-            sphere_face_rule(D, cf.lo, cf.hi, faces[cf.face].dir, S, sp == 0 ? -1 : +1, q, frule[2 * i + sp]);
+            frule_of(cf.lo, cf.hi, faces[cf.face].dir, sp == 0 ? -1 : +1, frule[2 * i + sp]);
     });
     for (size_t i = 0; i < cut_faces.size(); ++i) {
         FaceInfo& F = faces[cut_faces[i].face];

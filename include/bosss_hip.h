@@ -366,6 +366,21 @@ typedef struct bh_xdg bh_xdg;
  * no target fails the call. */
 int bh_xdg_create(bh_grid* grid, int k, const double* center, double radius, double nu_in, double nu_out,
                   double penalty_base, double agg_threshold, int quad_order, bh_xdg** x);
+/* The same XDG problem for a level set given as a polynomial on the whole domain,
+ * phi(x) = sum over i0, i1, i2 <= deg of coeffs[i0 + (deg + 1) (i1 + (deg + 1) i2)]
+ * x^i0 y^i1 z^i2 (2D: (deg + 1)^2 coefficients), species A (nu_in) where phi < 0.
+ * The cut-cell, cut-face and interface rules come from Saye's dimension reduction
+ * (X/Quadrature/SayeGaussRule_Algorithm.cs, SayeGaussRule_Cube.cs): Bernstein bounds
+ * decide which boxes are cut and in which directions phi is monotone, the roots are
+ * found by a Newton iteration safeguarded by bisection, boxes without a monotone
+ * direction are halved (up to 10 times, then a low-order rule is used and counted,
+ * bh_xdg_cut_rule_fallbacks); deg <= 8.  The level set is one polynomial on the
+ * whole grid, so both sides of every face see the same phi; BoSSS's per-cell DG
+ * level-set field is not taken in this form. */
+int bh_xdg_create_poly(bh_grid* g, int k, int deg, const double* coeffs, double nu_in, double nu_out,
+                       double penalty_base, double agg_threshold, int quad_order, bh_xdg** x);
+/* boxes (process-wide) where the polynomial cut rules used the low-order fallback */
+int bh_xdg_cut_rule_fallbacks(long* count);
 int bh_xdg_destroy(bh_xdg* x);
 /* info[13]: N_full, N_red, cut cells, agglomeration pairs, failed sources, elements,
  * cut-rule nodes, full matrix values, reduced matrix values, boundary nodes, Np, J,
