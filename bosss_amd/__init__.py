@@ -106,6 +106,7 @@ SIGNATURES = {
     "bh_euler_destroy": (_c_i, [_c_p]),
     "bh_euler_info": (_c_i, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i)]),
     "bh_euler_rhs": (_c_i, [_c_p, _c_p, _c_p, _c_i, _c_i]),
+    "bh_euler_implicit_step": (_c_i, [_c_p, _c_p, _c_p, _c_d, _c_d, _c_i, _c_i, _c_p, _c_p, _c_i]),
     "bh_euler_tvd3": (_c_i, [_c_p, _c_p, _c_d, _c_i, _c_i, _c_i]),
     "bh_xdg_create": (_c_i, [_c_p, _c_i, _c_p, _c_d, _c_d, _c_d, _c_d, _c_d, _c_i, _PP]),
     "bh_xdg_destroy": (_c_i, [_c_p]),
@@ -737,6 +738,18 @@ class EulerOperator(_Owned):
         up, fu = _vec_args(U)
         _check(lib().bh_euler_tvd3(self.h, up, dt, steps, prec, fu))
         return U
+
+    def implicit_step(self, Un, dt, U=None, ConvCrit=1e-10, MaxIter=50, restart=30):
+        """Backward-Euler step U - Un - dt R(U) = 0 by Newton-Krylov (bh_euler_implicit_step);
+        returns (U, dict(newton, gmres, cuts, res0, res))."""
+        Un = np.ascontiguousarray(Un, np.float64)
+        U = Un.copy() if U is None else np.ascontiguousarray(U, np.float64)
+        info = np.zeros(3, np.int32)
+        res = np.zeros(2)
+        _check(lib().bh_euler_implicit_step(self.h, Un.ctypes.data, U.ctypes.data, dt, ConvCrit, MaxIter, restart,
+                                            info.ctypes.data, res.ctypes.data, 0))
+        return U, dict(newton=int(info[0]), gmres=int(info[1]), cuts=int(info[2]), res0=float(res[0]),
+                       res=float(res[1]))
 
 
 class OrthonormalizationMultigrid(_Solver):

@@ -1135,6 +1135,21 @@ int bh_euler_rhs(bh_euler* w, const void* U, void* Rv, int prec, int flags) {
     if (!dev) e->ctx->sync();
     BH_API_END
 }
+int bh_euler_implicit_step(bh_euler* w, const double* Un, double* U, double dt, double conv, int maxit, int restart,
+                           int* info, double* res, int flags) {
+    BH_API_BEGIN
+    BH_CHECK(w && Un && U, "bh_euler_implicit_step: null argument");
+    BH_CHECK(dt > 0 && conv > 0 && maxit >= 1 && restart >= 1 && restart <= 200,
+             "bh_euler_implicit_step: need dt > 0, conv > 0, maxit >= 1, 1 <= restart <= 200");
+    EulerOp* e = w->e;
+    const bool dev = flags & BH_DEVICE_PTR;
+    Staged sn(e->ctx, Un, e->N * sizeof(double), dev, true);
+    Staged su(e->ctx, U, e->N * sizeof(double), dev, true);
+    newton_euler(e, (const double*)sn.dptr, (double*)su.dptr, dt, conv, maxit, restart, info, res);
+    su.download(U, e->N * sizeof(double), dev);
+    if (!dev) e->ctx->sync();
+    BH_API_END
+}
 int bh_euler_tvd3(bh_euler* w, void* U, double dt, int steps, int prec, int flags) {
     BH_API_BEGIN
     EulerOp* e = w->e;

@@ -22,6 +22,10 @@ struct EulerOp {
     template <typename R> void tvd3_t(R* U, double dt, int steps);
 };
 
+// Newton-Krylov solve of the backward-Euler step U - Un - dt R(U) = 0 (bh_newton.cpp)
+void newton_euler(EulerOp* E, const double* Un, double* U, double dt, double conv, int maxit, int restart, int* info,
+                  double* res);
+
 }  // namespace bh
 
 struct bh_euler {

@@ -20,19 +20,23 @@ void bh_solver_precondition_default(bh_solver* s, double* x, const double* b) {
 
 // ------------------------------------------------------------ LinOp
 void LinOp::apply(const double* x, double* y) const {
-    if (V) V->spmv(1.0, x, 0.0, y, BH_FP64);
+    if (fn) fn(x, y);
+    else if (V) V->spmv(1.0, x, 0.0, y, BH_FP64);
     else if (M) bsr_spmv(M, 1.0, x, 0.0, y, BH_FP64);
     else sip_apply(op, 1.0, x, 0.0, y, false, BH_FP64);
 }
 void LinOp::residual(const double* b, const double* x, double* r) const {
-    if (V) V->spmv(-1.0, x, 0.0, r, BH_FP64, b);
+    if (fn) {
+        fn(x, r);
+        axpby(ctx(), 1.0, b, -1.0, r, N);
+    } else if (V) V->spmv(-1.0, x, 0.0, r, BH_FP64, b);
     else if (M) bsr_residual(M, b, x, r, BH_FP64);
     else {
         copy(ctx(), r, b, N);
         sip_apply(op, -1.0, x, 1.0, r, false, BH_FP64);
     }
 }
-bh_ctx* LinOp::ctx() const { return V ? V->ctx : M ? M->ctx : op->ctx; }
+bh_ctx* LinOp::ctx() const { return fn ? fctx : V ? V->ctx : M ? M->ctx : op->ctx; }
 
 // ------------------------------------------------------------ BlockJacobi
 // AS/BlockJacobi.cs:100-255.  NoOfIterations + 1 sweeps (the reference's loop

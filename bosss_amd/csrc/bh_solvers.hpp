@@ -1,5 +1,6 @@
 // Solver classes of libbosss_hip.so (device resident).
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "bh_internal.hpp"
@@ -14,6 +15,9 @@ struct LinOp {
     bh_bsr* M = nullptr;
     bh_op* op = nullptr;
     Vbsr* V = nullptr;
+    // or any operator given as y = A x on device vectors (e.g. a finite-difference Jacobian)
+    std::function<void(const double*, double*)> fn;
+    bh_ctx* fctx = nullptr;
     int64_t N = 0;
     void apply(const double* x, double* y) const;                  // y = A x
     void residual(const double* b, const double* x, double* r) const;  // r = b - A x

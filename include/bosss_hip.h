@@ -344,6 +344,17 @@ int bh_euler_info(bh_euler* e, int64_t* N, int* Np);
 int bh_euler_rhs(bh_euler* e, const void* U, void* R, int prec, int flags);
 /* `steps` TVD3 steps of size dt, in place */
 int bh_euler_tvd3(bh_euler* e, void* U, double dt, int steps, int prec, int flags);
+/* Newton (AS/Newton.cs:195-370, 880-960, 1546-1600) on the backward-Euler step
+ * F(U) = U - Un - dt R(U) = 0 of this operator: matrix-free Jacobian by the
+ * reference's directional difference (dirder), each step's J s = -F solved by
+ * GMRES(restart) without preconditioner to 1e-5 |F| (the inexact-Newton
+ * threshold of its matrix-free option), Armijo line search with parabolic
+ * backtracking (at most 30 cuts); stops at |F| <= conv |F(U_start)| + conv or
+ * after maxit steps.  U holds the initial guess (e.g. Un) and returns the solution;
+ * info[3]: Newton steps, GMRES iterations, line-search cuts (may be NULL);
+ * res[2]: |F| at the start and at the end (may be NULL).  FP64. */
+int bh_euler_implicit_step(bh_euler* e, const double* Un, double* U, double dt, double conv, int maxit, int restart,
+                           int* info, double* res, int flags);
 
 /* ---------------------------------------------------------------- XDG two-phase Poisson
  * Cut-cell SIP Poisson with a spherical level set phi = |x - c| - r, species
