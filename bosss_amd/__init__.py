@@ -98,6 +98,7 @@ SIGNATURES = {
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_schwarz": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_set_schwarz_overlap": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_pmg_set_diag_equilib": (_c_i, [_c_p]),
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_learned_fallback": (_c_i, [_c_p, _c_i, _c_d]),
@@ -695,9 +696,11 @@ def pmg_learned_info(pmg):
     return bool(a.value), g.value
 
 
-def pmg_set_schwarz(pmg, enable=True):
-    """Additive Schwarz on the multigrid aggregates (no overlap) as the smoother of the aggregated levels."""
-    _check(lib().bh_pmg_set_schwarz(pmg.h, 1 if enable else 0))
+def pmg_set_schwarz(pmg, enable=True, overlap=0, overlap_scaling=False):
+    """Additive Schwarz on the multigrid aggregates, grown by `overlap` (0-2) layers of
+    face neighbours, as the smoother of the aggregated levels; with overlap_scaling a
+    DOF's sum of block solutions is divided by the number of blocks holding it."""
+    _check(lib().bh_pmg_set_schwarz_overlap(pmg.h, 1 if enable else 0, int(overlap), 1 if overlap_scaling else 0))
 
 
 def pmg_set_diag_equilib(pmg):

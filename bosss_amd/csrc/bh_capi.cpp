@@ -1067,14 +1067,17 @@ int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
     BH_API_END
 }
 
-int bh_pmg_set_schwarz(bh_solver* s, int enable) {
+int bh_pmg_set_schwarz(bh_solver* s, int enable) { return bh_pmg_set_schwarz_overlap(s, enable, 0, 0); }
+
+int bh_pmg_set_schwarz_overlap(bh_solver* s, int enable, int overlap, int overlap_scaling) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
     XdgPrecond* xp = dynamic_cast<XdgPrecond*>(compressed(s));
     BH_CHECK(mg || xp, "bh_pmg_set_schwarz: solver is not a multigrid preconditioner");
+    BH_CHECK(overlap >= 0 && overlap <= 2, "bh_pmg_set_schwarz: overlap must be 0, 1 or 2");
     if (xp) mg = xp->cmg.get();
     (void)hipStreamSynchronize(mg->ctx->stream);
-    mg->set_schwarz(enable != 0);
+    mg->set_schwarz(enable != 0, overlap, overlap_scaling != 0);
     BH_API_END
 }
 

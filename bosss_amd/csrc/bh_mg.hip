@@ -519,6 +519,42 @@ __global__ __launch_bounds__(256) void schwarz_apply_k(int nagg, int bs, int cma
     }
 }
 
+// Overlapping blocks (Schwarz with overlap): phase 1, per block I the local solution
+// Y_I = Ainv_I R_I r (thread = block row); phase 2, per DOF the sum over the blocks
+// that hold it, in a fixed order (no atomics): x (+)= omega sum_I (R_I^T Y_I).
+template <typename T>
+__global__ __launch_bounds__(256) void schwarz_local_k(int nblk, int bs, int bmax, const int* __restrict__ bptr,
+                                                       const int* __restrict__ bidx, const T* __restrict__ Ainv,
+                                                       const T* __restrict__ r, T* __restrict__ Y) {
+    const int BA = bmax * bs;
+    const int64_t tot = (int64_t)nblk * BA;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int I = (int)(t / BA), row = (int)(t - (int64_t)I * BA);
+        const int c0 = bptr[I], nc = bptr[I + 1] - c0;
+        if (row >= nc * bs) continue;
+        const T* __restrict__ Ai = Ainv + (size_t)I * BA * BA;
+        T acc = 0;
+        for (int u = 0; u < nc; ++u) {
+            const T* __restrict__ ru = r + (size_t)bidx[c0 + u] * bs;
+            for (int j = 0; j < bs; ++j) acc += Ai[(size_t)(u * bs + j) * BA + row] * ru[j];
+        }
+        Y[(size_t)c0 * bs + row] = acc;
+    }
+}
+
+template <typename T, bool ADD>
+__global__ __launch_bounds__(256) void schwarz_sum_k(int ncell, int bs, const int* __restrict__ iptr, const int* __restrict__ iidx,
+                                                     const T* __restrict__ Y, T* __restrict__ x, T omega, int scale) {
+    const int64_t tot = (int64_t)ncell * bs;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(t / bs), i = (int)(t - (int64_t)c * bs);
+        T s = 0;
+        for (int e = iptr[c]; e < iptr[c + 1]; ++e) s += Y[(size_t)iidx[e] * bs + i];
+        if (scale) s = s / (T)(iptr[c + 1] - iptr[c]);   // one over the multiplicity
+        x[t] = ADD ? x[t] + omega * s : omega * s;
+    }
+}
+
 // ---------------------------------------------------------------- K-cycle scalars
 // The inner flexible GMRES of the K-cycle keeps its Hessenberg matrix, Gram-
 // Schmidt coefficients and least-squares solution in device memory (layout
@@ -633,6 +669,21 @@ void schwarz_apply(bh_ctx* ctx, int nagg, int bs, int cmax, const int* cptr, con
     else hipLaunchKernelGGL((schwarz_apply_k<T, false>), grid1d(tot), dim3(256), 0, ctx->stream, nagg, bs, cmax, cptr, cidx, Ainv, r, x, (T)omega);
     BH_HIP(hipGetLastError());
 }
+
+template <typename T>
+void schwarz_apply_overlap(bh_ctx* ctx, int nblk, int bs, int bmax, const int* bptr, const int* bidx, const int* iptr,
+                           const int* iidx, int ncell, const T* Ainv, const T* r, T* Y, T* x, double omega, bool add,
+                           bool scale) {
+    hipLaunchKernelGGL((schwarz_local_k<T>), grid1d((int64_t)nblk * bmax * bs), dim3(256), 0, ctx->stream, nblk, bs, bmax, bptr,
+                       bidx, Ainv, r, Y);
+    if (add) hipLaunchKernelGGL((schwarz_sum_k<T, true>), grid1d((int64_t)ncell * bs), dim3(256), 0, ctx->stream, ncell, bs, iptr, iidx, Y, x, (T)omega, scale ? 1 : 0);
+    else hipLaunchKernelGGL((schwarz_sum_k<T, false>), grid1d((int64_t)ncell * bs), dim3(256), 0, ctx->stream, ncell, bs, iptr, iidx, Y, x, (T)omega, scale ? 1 : 0);
+    BH_HIP(hipGetLastError());
+}
+template void schwarz_apply_overlap<float>(bh_ctx*, int, int, int, const int*, const int*, const int*, const int*, int,
+                                           const float*, const float*, float*, float*, double, bool, bool);
+template void schwarz_apply_overlap<double>(bh_ctx*, int, int, int, const int*, const int*, const int*, const int*, int,
+                                            const double*, const double*, double*, double*, double, bool, bool);
 
 template <typename T>
 void scale_inv_norm(bh_ctx* ctx, const T* src, T* dst, int64_t n, const double* s2, double tiny) {

@@ -58,6 +58,11 @@ template <typename T> void scale_inv_norm(bh_ctx*, const T* src, T* dst, int64_t
 void gather_agg_blocks(bh_ctx*, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const bh_bsr* A, double* out);
 template <typename T> void schwarz_apply(bh_ctx*, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const T* Ainv,
                                          const T* r, T* x, double omega, bool add);
+// overlapping blocks (bptr/bidx: cells per block; iptr/iidx: per cell its flat (block, slot) positions);
+// scale: the block sum of a DOF divided by its multiplicity
+template <typename T> void schwarz_apply_overlap(bh_ctx*, int nblk, int bs, int bmax, const int* bptr, const int* bidx,
+                                                 const int* iptr, const int* iidx, int ncell, const T* Ainv, const T* r,
+                                                 T* Y, T* x, double omega, bool add, bool scale);
 void invert_dense_blocks(bh_ctx*, int nblocks, int n, const double* blocks, double* inv64, float* inv32);
 // Gauss-Jordan inverse (partial pivoting) of a dense row-major n x n device matrix; A is overwritten
 void dense_inverse_dev(bh_ctx*, double* A, double* Inv, int n);
@@ -89,6 +94,10 @@ struct MgLevel {
     // additive Schwarz smoother over this level's aggregates (sw_cmax > 0: in use)
     int sw_cmax = 0, sw_nagg = 0;
     DevBuf sw64, sw32;
+    // with overlap (sw_overlap > 0): block cell lists, per-cell positions, local solutions
+    int sw_overlap = 0;
+    bool sw_scale = false;
+    DevBuf sw_bptr, sw_bidx, sw_iptr, sw_iidx, sw_y64, sw_y32;
     // block equilibration (level in the transformed basis x~ = L^-T x): the inverse
     // factors L (and L^T, level 0 only, for the output) and the transfer blocks to the next level
     DevBuf eqL64, eqL32, eqLt64, eqLt32, eqP64, eqP32, eqcptr, eqcidx, eqparent;
@@ -154,7 +163,9 @@ struct PMultigrid : bh_solver {
     template <typename T> void smooth(int l, const T* r, T* x, bool add);   // x (+)= omega M_l^-1 r
     template <typename T> void restrict_to(int l, const T* r);     // lv[l+1].b = R_l r
     template <typename T> void prolong_add(int l, T* x);           // x += P_l lv[l+1].x
-    void set_schwarz(bool on);     // aggregate-block Schwarz on every level that has aggregates
+    // aggregate-block Schwarz on every level that has aggregates, the blocks grown by
+    // `overlap` layers of face neighbours; scale: overlap scaling
+    void set_schwarz(bool on, int overlap = 0, bool scale = false);
     void set_equilib();            // DiagBlockEquilib (assembled hierarchies; irreversible)
     int op_prec(int l, bool f32) const {
         return f32 ? ((l == 0 && mfma && l0_mfma_apply) ? BH_FP16_MFMA : BH_FP32) : BH_FP64;

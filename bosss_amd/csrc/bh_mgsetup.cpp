@@ -493,7 +493,11 @@ void PMultigrid::smooth(int l, const T* r, T* x, bool add) {
     if (equilib && L.eqL64.p) {
         // block-equilibrated level: the diagonal blocks are diag(S) (signs), block-Jacobi is x (+)= omega S r
         sign_smooth<T>(ctx, L.N, omega, (const T*)(f32 ? L.eqS32.p : L.eqS64.p), r, x, add);
-    } else if (L.sw_cmax > 0)
+    } else if (L.sw_cmax > 0 && L.sw_overlap > 0)
+        schwarz_apply_overlap<T>(ctx, L.sw_nagg, L.bs, L.sw_cmax, L.sw_bptr.as<int>(), L.sw_bidx.as<int>(), L.sw_iptr.as<int>(),
+                                 L.sw_iidx.as<int>(), L.J, (const T*)(f32 ? L.sw32.p : L.sw64.p), r,
+                                 (T*)(f32 ? L.sw_y32.p : L.sw_y64.p), x, omega, add, L.sw_scale);
+    else if (L.sw_cmax > 0)
         schwarz_apply<T>(ctx, L.sw_nagg, L.bs, L.sw_cmax, L.cptr.as<int>(), L.cidx.as<int>(),
                          (const T*)(f32 ? L.sw32.p : L.sw64.p), r, x, omega, add);
     else if (f32 && L.dinv16.p)
@@ -599,27 +603,100 @@ void PMultigrid::set_equilib() {
     equilib = true;
 }
 
-void PMultigrid::set_schwarz(bool on) {
+// Schwarz blocks: the cells of every aggregate of an aggregation level (AS/Schwarz.cs,
+// multigrid-aggregate blocking), grown by `overlap` layers of face neighbours on the
+// level's grid and sorted (Schwarz.cs:683-730); each block's dense matrix is inverted
+// on the device.  Without overlap every DOF is in one block and the correction is
+// written directly; with overlap the local solutions are summed per DOF (additive,
+// Schwarz.cs:1026-1085) and, with `scale` (EnableOverlapScaling, :518-524, :791-815),
+// divided by the number of blocks that hold the DOF.
+void PMultigrid::set_schwarz(bool on, int overlap, bool scale) {
+    BH_CHECK(overlap >= 0 && overlap <= 2, "multigrid: Schwarz overlap must be 0, 1 or 2");
     drop_graphs();
     const int D = dim;
     for (size_t l = 0; l + 1 < lv.size(); ++l) {
         MgLevel& L = lv[l];
         L.sw_cmax = 0;
-        L.sw64.release();
-        L.sw32.release();
+        L.sw_overlap = 0;
+        L.sw_scale = false;
+        for (DevBuf* b : {&L.sw64, &L.sw32, &L.sw_bptr, &L.sw_bidx, &L.sw_iptr, &L.sw_iidx, &L.sw_y64, &L.sw_y32}) b->release();
         if (!on || L.kind != 1) continue;
-        const int cmax = 1 << D;
-        const int BA = cmax * L.bs;
-        if (BA > 64) continue;   // block inversion kernel limit: such levels keep block-Jacobi
         const int nagg = lv[l + 1].J;
+        if (overlap == 0) {
+            const int cmax = 1 << D;
+            const int BA = cmax * L.bs;
+            if (BA > 64) continue;   // block inversion kernel limit: such levels keep block-Jacobi
+            DevBuf dense((size_t)nagg * BA * BA * sizeof(double));
+            gather_agg_blocks(ctx, nagg, L.bs, cmax, L.cptr.as<int>(), L.cidx.as<int>(), L.A, dense.as<double>());
+            L.sw64.alloc((size_t)nagg * BA * BA * sizeof(double));
+            if (prec == BH_FP32) L.sw32.alloc((size_t)nagg * BA * BA * sizeof(float));
+            invert_dense_blocks(ctx, nagg, BA, dense.as<double>(), L.sw64.as<double>(), prec == BH_FP32 ? L.sw32.as<float>() : nullptr);
+            if (prec == BH_FP32) L.sw64.release();
+            L.sw_cmax = cmax;
+            L.sw_nagg = nagg;
+            continue;
+        }
+        // the aggregates' children, then the face-neighbour layers
+        std::vector<int> cptr(nagg + 1), cidx(L.J);
+        ctx->download(cptr.data(), L.cptr.p, (nagg + 1) * sizeof(int));
+        ctx->download(cidx.data(), L.cidx.p, L.J * sizeof(int));
+        std::vector<int> bptr(nagg + 1, 0), bidx;
+        std::vector<char> mark(L.J, 0);
+        int bmax = 0;
+        for (int I = 0; I < nagg; ++I) {
+            std::vector<int> blk(cidx.begin() + cptr[I], cidx.begin() + cptr[I + 1]);
+            for (int c : blk) mark[c] = 1;
+            for (int ov = 0; ov < overlap; ++ov) {
+                const size_t nb0 = blk.size();
+                for (size_t q = 0; q < nb0; ++q) {
+                    int idx[3];
+                    ijk_of(D, L.n, blk[q], idx);
+                    for (int d = 0; d < D; ++d)
+                        for (int sd = -1; sd <= 1; sd += 2) {
+                            int nb[3] = {idx[0], idx[1], idx[2]};
+                            nb[d] += sd;
+                            if (nb[d] < 0 || nb[d] >= L.n[d]) continue;
+                            const int cn = cell_of(D, L.n, nb[0], nb[1], nb[2]);
+                            if (!mark[cn]) { mark[cn] = 1; blk.push_back(cn); }
+                        }
+                }
+            }
+            std::sort(blk.begin(), blk.end());
+            for (int c : blk) mark[c] = 0;
+            bidx.insert(bidx.end(), blk.begin(), blk.end());
+            bptr[I + 1] = (int)bidx.size();
+            bmax = std::max(bmax, (int)blk.size());
+        }
+        const int BA = bmax * L.bs;
+        if (BA > 128) continue;   // the in-LDS inversion's limit: such levels keep block-Jacobi
+        // per cell its flat positions in bidx, in block order
+        std::vector<int> iptr(L.J + 1, 0), iidx(bidx.size());
+        for (int c : bidx) ++iptr[c + 1];
+        for (int c = 0; c < L.J; ++c) iptr[c + 1] += iptr[c];
+        {
+            std::vector<int> fillp(iptr.begin(), iptr.end() - 1);
+            for (size_t pq = 0; pq < bidx.size(); ++pq) iidx[fillp[bidx[pq]]++] = (int)pq;
+        }
+        L.sw_bptr.alloc(bptr.size() * sizeof(int));
+        L.sw_bidx.alloc(std::max<size_t>(bidx.size(), 1) * sizeof(int));
+        L.sw_iptr.alloc(iptr.size() * sizeof(int));
+        L.sw_iidx.alloc(std::max<size_t>(iidx.size(), 1) * sizeof(int));
+        ctx->upload(L.sw_bptr.p, bptr.data(), bptr.size() * sizeof(int));
+        ctx->upload(L.sw_bidx.p, bidx.data(), bidx.size() * sizeof(int));
+        ctx->upload(L.sw_iptr.p, iptr.data(), iptr.size() * sizeof(int));
+        ctx->upload(L.sw_iidx.p, iidx.data(), iidx.size() * sizeof(int));
         DevBuf dense((size_t)nagg * BA * BA * sizeof(double));
-        gather_agg_blocks(ctx, nagg, L.bs, cmax, L.cptr.as<int>(), L.cidx.as<int>(), L.A, dense.as<double>());
+        gather_agg_blocks(ctx, nagg, L.bs, bmax, L.sw_bptr.as<int>(), L.sw_bidx.as<int>(), L.A, dense.as<double>());
         L.sw64.alloc((size_t)nagg * BA * BA * sizeof(double));
         if (prec == BH_FP32) L.sw32.alloc((size_t)nagg * BA * BA * sizeof(float));
         invert_dense_blocks(ctx, nagg, BA, dense.as<double>(), L.sw64.as<double>(), prec == BH_FP32 ? L.sw32.as<float>() : nullptr);
         if (prec == BH_FP32) L.sw64.release();
-        L.sw_cmax = cmax;
+        if (prec == BH_FP32) L.sw_y32.alloc(std::max<size_t>(bidx.size(), 1) * L.bs * sizeof(float));
+        else L.sw_y64.alloc(std::max<size_t>(bidx.size(), 1) * L.bs * sizeof(double));
+        L.sw_cmax = bmax;
         L.sw_nagg = nagg;
+        L.sw_overlap = overlap;
+        L.sw_scale = scale;
     }
 }
 

@@ -216,7 +216,6 @@ BPoly from_poly(const PolyLS& L, const double* lo, const double* hi, int* act) {
         a.swap(nb);
     }
     P.b.resize(P.size());
-    int cnt = 0;
     for (int i = 0; i < N; ++i) {
         bool keep = true;
         int idx = 0, mul = 1;
@@ -225,7 +224,7 @@ BPoly from_poly(const PolyLS& L, const double* lo, const double* hi, int* act) {
             if (hi[d] > lo[d]) { idx += id * mul; mul *= m; }
             else if (id != 0) keep = false;
         }
-        if (keep) { P.b[idx] = a[i]; ++cnt; }
+        if (keep) P.b[idx] = a[i];
     }
     return P;
 }

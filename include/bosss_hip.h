@@ -316,6 +316,15 @@ int bh_pmg_set_learned_fallback(bh_solver* pmg, int probe, double min_gain);
  * block-Jacobi.  Applies to bh_pmg_create and to the coarse hierarchy of
  * bh_xdg_precond_create. */
 int bh_pmg_set_schwarz(bh_solver* pmg, int enable);
+/* Schwarz with overlap (Schwarz.Config.Overlap, AS/Schwarz.cs:491-512; block growth
+ * :683-730; the additive sum of the block solutions :1026-1085): each aggregate block
+ * is grown by `overlap` (0, 1 or 2) sweeps over the face neighbours of its cells and
+ * sorted; a DOF in several blocks receives the sum of their local solutions, divided
+ * by their number when overlap_scaling is set (EnableOverlapScaling, :518-524,
+ * :791-815; the reference scales the whole iterate, which is the same for a smoother
+ * started from zero).  Levels whose largest block exceeds 128 DOF keep block-Jacobi.
+ * overlap = 0 is bh_pmg_set_schwarz. */
+int bh_pmg_set_schwarz_overlap(bh_solver* pmg, int enable, int overlap, int overlap_scaling);
 /* DiagBlockEquilib (AS/MultigridOperator_ChangeOfBasis.cs:819-828; the factor of
  * SymmInv, :624): every multigrid level but the coarsest is taken into the basis
  * in which its diagonal blocks are diagonals of signs (D_j = C_j S_j C_j^T, level

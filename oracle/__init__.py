@@ -72,7 +72,7 @@ def lib():
             "orc_pmg_vcycle_from": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_level_spmv": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_set": (None, [_p, _d, _i, _i]),
-            "orc_pmg_set_schwarz": (_i, [_p, _i]),
+            "orc_pmg_set_schwarz": (_i, [_p, _i, _i, _i]),
             "orc_pmg_smooth": (None, [_p, _i, _p, _p]),
             "orc_pmg_level_info": (None, [_p, _i, _ip]),
             "orc_pmg_level_transfer": (_i, [_p, _i, _ip, _dp]),
@@ -325,9 +325,11 @@ class Pmg:
         lib().orc_pmg_smooth(self.h, level, np.ascontiguousarray(r, np.float64).ctypes.data, x.ctypes.data)
         return x
 
-    def set_schwarz(self, on=True):
-        """Additive Schwarz over the aggregates (no overlap) instead of block-Jacobi on aggregated levels."""
-        if lib().orc_pmg_set_schwarz(self.h, 1 if on else 0) != 0:
+    def set_schwarz(self, on=True, overlap=0, scaling=False):
+        """Additive Schwarz over the aggregates, grown by `overlap` (0-2) face-neighbour
+        layers (with `scaling`, a DOF's block sum divided by its multiplicity), instead
+        of block-Jacobi on aggregated levels."""
+        if lib().orc_pmg_set_schwarz(self.h, 1 if on else 0, int(overlap), 1 if scaling else 0) != 0:
             raise ArithmeticException_("Schwarz block inversion failed")
 
     def level_info(self, level):

@@ -808,11 +808,13 @@ struct MgLev {
     std::vector<int> parent;
     std::vector<double> P;      // J x bs x bs, [fine mode][coarse mode]
     // additive Schwarz over the aggregates (AS/Schwarz.cs with multigrid-aggregate
-    // blocks, no overlap): children per aggregate (ascending cell index) and the
-    // inverse of A restricted to their DOFs, padded to cmax children with identity
+    // blocks): cells per block (the aggregate's children and, with overlap, their
+    // neighbour layers; ascending cell index) and the inverse of A restricted to
+    // their DOFs, padded to cmax cells with identity
     int sw_cmax = 0;
     std::vector<std::vector<int>> sw_children;
     std::vector<double> sw_inv;  // nagg x (cmax*bs)^2, row-major
+    std::vector<int> sw_mult;    // with overlap scaling: blocks holding each cell
 };
 
 struct Pmg {
@@ -958,18 +960,59 @@ Pmg* pmg_build(const Bsr& Af, const Grid& g, const int* deg, int ndeg, int min_c
     return P;
 }
 
-void pmg_set_schwarz(Pmg& P, bool on) {
+// Schwarz blocks (AS/Schwarz.cs:683-730): the children of every aggregate, grown by
+// `overlap` sweeps over the face neighbours of the cells present at the start of the
+// sweep, then sorted by cell index; `scale`: the block sum of a DOF divided by the
+// number of blocks that hold it (EnableOverlapScaling, :791-815).
+void pmg_set_schwarz(Pmg& P, bool on, int overlap, bool scale) {
+    if (overlap < 0 || overlap > 2) throw std::runtime_error("Schwarz overlap must be 0, 1 or 2");
+    const int D = P.D;
     for (size_t l = 0; l + 1 < P.lv.size(); ++l) {
         MgLev& L = P.lv[l];
         L.sw_cmax = 0;
         L.sw_children.clear();
         L.sw_inv.clear();
+        L.sw_mult.clear();
         if (!on || L.kind != 1) continue;
-        const int bs = L.A.bs, cmax = 1 << P.D, BA = cmax * bs;
-        if (BA > 64) continue;   // as the product: such levels keep block-Jacobi
+        const int bs = L.A.bs, J = L.A.nrows;
         const int Jc = P.lv[l + 1].A.nrows;
-        L.sw_children.assign(Jc, {});
-        for (int c = 0; c < L.A.nrows; ++c) L.sw_children[L.parent[c]].push_back(c);
+        std::vector<std::vector<int>> blocks(Jc);
+        for (int c = 0; c < J; ++c) blocks[L.parent[c]].push_back(c);
+        if (overlap > 0) {
+            std::vector<char> in(J, 0);
+            for (auto& blk : blocks) {
+                for (int c : blk) in[c] = 1;
+                for (int sweep = 0; sweep < overlap; ++sweep) {
+                    const size_t n0 = blk.size();
+                    for (size_t q = 0; q < n0; ++q) {
+                        const int c = blk[q];
+                        int ijk[3];
+                        if (D == 2) { ijk[0] = c / L.n[1]; ijk[1] = c % L.n[1]; ijk[2] = 0; }
+                        else { ijk[0] = c % L.n[0]; ijk[1] = (c / L.n[0]) % L.n[1]; ijk[2] = c / (L.n[0] * L.n[1]); }
+                        for (int d = 0; d < D; ++d)
+                            for (int sg : {-1, 1}) {
+                                int nb[3] = {ijk[0], ijk[1], ijk[2]};
+                                nb[d] += sg;
+                                if (nb[d] < 0 || nb[d] >= L.n[d]) continue;
+                                const int cn = D == 2 ? nb[0] * L.n[1] + nb[1] : nb[0] + nb[1] * L.n[0] + nb[2] * L.n[0] * L.n[1];
+                                if (!in[cn]) { in[cn] = 1; blk.push_back(cn); }
+                            }
+                    }
+                }
+                std::sort(blk.begin(), blk.end());
+                for (int c : blk) in[c] = 0;
+            }
+        }
+        int cmax = overlap > 0 ? 0 : 1 << D;
+        for (auto& blk : blocks) cmax = std::max(cmax, (int)blk.size());
+        const int BA = cmax * bs;
+        if (BA > (overlap > 0 ? 128 : 64)) continue;   // as the product: such levels keep block-Jacobi
+        L.sw_children = std::move(blocks);
+        if (scale) {
+            L.sw_mult.assign(J, 0);
+            for (auto& blk : L.sw_children)
+                for (int c : blk) ++L.sw_mult[c];
+        }
         L.sw_inv.assign((size_t)Jc * BA * BA, 0.0);
         for (int I = 0; I < Jc; ++I) {
             const std::vector<int>& ch = L.sw_children[I];
@@ -992,11 +1035,14 @@ void pmg_set_schwarz(Pmg& P, bool on) {
     }
 }
 
+// x (+)= omega y with y = sum_I R_I^T A_I^-1 R_I r (additive: a DOF in several blocks
+// sums their local solutions, AS/Schwarz.cs:1056-1085, no overlap scaling) or, without
+// Schwarz blocks, y = D^-1 r
 static void bgemv(const MgLev& L, const double* r, double* x, double omega, bool add) {
     int bs = L.A.bs;
     if (L.sw_cmax > 0) {
         const int BA = L.sw_cmax * bs;
-        std::vector<double> rr(BA);
+        std::vector<double> rr(BA), y((size_t)L.A.nrows * bs, 0.0);
         for (size_t I = 0; I < L.sw_children.size(); ++I) {
             const std::vector<int>& ch = L.sw_children[I];
             std::fill(rr.begin(), rr.end(), 0.0);
@@ -1007,10 +1053,12 @@ static void bgemv(const MgLev& L, const double* r, double* x, double omega, bool
                 for (int i = 0; i < bs; ++i) {
                     double v = 0;
                     for (int c = 0; c < BA; ++c) v += B[(s * bs + i) * BA + c] * rr[c];
-                    double& xo = x[(size_t)ch[s] * bs + i];
-                    xo = (add ? xo : 0.0) + omega * v;
+                    y[(size_t)ch[s] * bs + i] += v;
                 }
         }
+        if (!L.sw_mult.empty())
+            for (size_t t = 0; t < y.size(); ++t) y[t] = y[t] / L.sw_mult[t / bs];
+        for (size_t t = 0; t < y.size(); ++t) x[t] = (add ? x[t] : 0.0) + omega * y[t];
         return;
     }
     for (int row = 0; row < L.A.nrows; ++row) {
@@ -1430,8 +1478,8 @@ void orc_pmg_smooth(void* p, int l, const double* r, double* x) {
     Pmg* P = (Pmg*)p;
     bgemv(P->lv[l], r, x, P->omega, false);
 }
-int orc_pmg_set_schwarz(void* p, int on) {
-    try { pmg_set_schwarz(*(Pmg*)p, on != 0); return 0; } catch (...) { return -1; }
+int orc_pmg_set_schwarz(void* p, int on, int overlap, int scale) {
+    try { pmg_set_schwarz(*(Pmg*)p, on != 0, overlap, scale != 0); return 0; } catch (...) { return -1; }
 }
 // level l: info = {block rows, block size, transfer kind to l+1, degree, n0, n1, n2}
 void orc_pmg_level_info(void* p, int l, int* info) {

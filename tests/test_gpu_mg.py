@@ -71,6 +71,53 @@ def test_vcycle_schwarz_matches_oracle(ctx, matrix_free, prec, tol):
     assert rel(mg.Solve(np.zeros(oo.N), b), P.vcycle(b)) < tol
 
 
+@pytest.mark.parametrize("overlap,scaling", [(1, False), (1, True), (2, True)])
+@pytest.mark.parametrize("prec,tol", [(B.BH_FP64, 1e-11), (B.BH_FP32, 1e-4)])
+def test_vcycle_schwarz_overlap_matches_oracle(ctx, overlap, scaling, prec, tol):
+    """Schwarz with overlap (AS/Schwarz.cs:683-730: aggregate blocks grown by `overlap`
+    face-neighbour sweeps; the block solutions summed): device V-cycle vs the oracle
+    cycle.  On 6x5x4 cells at k = 1 one layer gives blocks of up to 32 cells (128 DOF,
+    the limit); two layers exceed it on the fine level, which keeps block-Jacobi on
+    both sides, while the coarser h-levels take the grown blocks.  With scaling the
+    block sums are divided by the DOF's multiplicity (:791-815)."""
+    _say(f"case vcycle schwarz overlap={overlap} scaling={scaling} prec={prec}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(6).standard_normal(oo.N)
+    P = O.Pmg(Ao, og, [3, 1], min_coarse=8, omega=0.5, pre=2, post=1)
+    P.set_schwarz(True, overlap=overlap, scaling=scaling)
+    ref = P.vcycle(b)
+    mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, omega=0.5, pre=2, post=1, kcycle_inner=0, prec=prec)
+    B.pmg_set_schwarz(mg, True, overlap=overlap, overlap_scaling=scaling)
+    out = mg.Solve(np.zeros(oo.N), b)
+    assert rel(out, ref) < tol
+    # a second set-up on the same solver replaces the blocks
+    B.pmg_set_schwarz(mg, True, overlap=0)
+    P.set_schwarz(True, overlap=0)
+    assert rel(mg.Solve(np.zeros(oo.N), b), P.vcycle(b)) < tol
+
+
+def test_schwarz_overlap_fgmres_converges(ctx):
+    """FGMRES + V-cycle with one layer of overlap and overlap scaling reaches 1e-8 and
+    gives the solution of the smoother without overlap."""
+    _say("case fgmres schwarz overlap 1, scaled")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(7).standard_normal(oo.N)
+    its, sols = [], []
+    for ov in (0, 1):
+        mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, omega=0.5, pre=2, post=2, kcycle_inner=0)
+        B.pmg_set_schwarz(mg, True, overlap=ov, overlap_scaling=True)
+        S = B.FlexGMRES(op=go, Precond=mg, MaxKrylovDim=40, ConvergenceCriterion=1e-8)
+        u = S.Solve(np.zeros(oo.N), b)
+        st = S.stats()
+        _say(f"  overlap {ov}: {st['iterations']} iterations")
+        assert st["converged"]
+        its.append(st["iterations"])
+        sols.append(u)
+    assert rel(sols[1], sols[0]) < 1e-6
+    with pytest.raises(B.BoSSSHipError):
+        B.pmg_set_schwarz(mg, True, overlap=3)
+
+
 def _kcycle_numpy(P, level, b, kin):
     """kin flexible-GMRES steps on level `level` (CGS2), V-cycle preconditioned."""
     n = b.size

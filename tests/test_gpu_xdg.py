@@ -287,3 +287,26 @@ def test_polynomial_level_set_torus_solve(ctx):
     _say(f"  {X.ncut} cut cells, {X.npairs} agglomeration pairs, stats {st}")
     assert st["converged"]
     assert rel(A @ u, b) <= 1.5e-8
+
+
+def test_schwarz_overlap_in_coarse_hierarchy(ctx):
+    """Schwarz with one layer of overlap and overlap scaling (AS/Schwarz.cs:683-730,
+    :791-815) in the coarse p/h hierarchy of the two-level XDG preconditioner: FGMRES
+    converges to 1e-8 and gives the block-Jacobi preconditioner's solution."""
+    _say("case xdg solve 16^3, schwarz overlap 1 in the coarse hierarchy")
+    x = np.linspace(0, 1, 17)
+    g = B.Grid(ctx, x, x, x)
+    X = B.XdgPoisson(g, 2, [0.5, 0.47, 0.52], 0.27, NU[0], NU[1])
+    X.set_rhs([1.0, 1.0], None)
+    u0, b, st0 = _solve(X, 1e-8)
+    P = B.XdgPrecond(X, coarse_degree=1, prec=B.BH_FP32)
+    B.pmg_set_schwarz(P, True, overlap=1, overlap_scaling=True)
+    S = B.XdgFlexGMRES(X, P, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=2000)
+    u1 = np.zeros(X.N)
+    S.Solve(u1, b)
+    st1 = S.stats()
+    _say("  block-Jacobi", st0, " schwarz overlap 1", st1)
+    assert st0["converged"] and st1["converged"]
+    A = X.scipy_matrix(1)
+    assert rel(A @ u1, b) <= 1.5e-8
+    assert rel(u1, u0) <= 1e-6
