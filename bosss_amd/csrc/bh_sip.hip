@@ -754,12 +754,15 @@ __device__ __forceinline__ void face_side_traces(R* uP, const SideCoef<R>& c, NB
 }
 
 template <int D, int K, typename R, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void sip_apply_tile_kernel(SipArgs<R> a, int nseg, int nrb, int ntiles, int per_xcd) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3))) void sip_apply_tile_kernel(SipArgs<R> a, int nseg, int nrb, int ntiles, int per_xcd) {
     constexpr int Np = binom(K + D, D);
     constexpr int NFM = binom(K + D - 1, D - 1);   // face modes
     constexpr int DF = D == 2 ? 1 : 0;             // consecutive cells: lanes
     constexpr int DT = D == 2 ? 0 : 1;             // rows of the workgroup: LDS
-    __shared__ R tr[W][2][2 * NFM][64];            // [wave][face '-' / '+'][A, B per face mode][lane]
+    // one face's traces per wave at a time (A, B per face mode), then the staged results:
+    // at 64^3 k = 3 FP64 43 KB per workgroup of 4 waves, 3 workgroups per CU
+    constexpr int TRW = 2 * NFM > Np + 1 ? 2 * NFM : Np + 1;
+    __shared__ R tr[W][TRW][64];
     const int t = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (t >= ntiles) return;                       // whole workgroup, before any barrier
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -811,20 +814,21 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
         cjP += (R(1) + R(0.5) * R((idx[d] == 0) + (idx[d] == nn[d] - 1))) * ih[d];
     }
     const R pen = R(a.pen);
-    // 1. own traces on both DT faces -> LDS
-    static_for<Np>([&](auto FF) {
-        constexpr int f = FF;
-        if constexpr (mode_exponent(D, K, f, DT) == 0) {
-            constexpr int fs = face_slot(D, K, f, DT);
-            R Am, Bm, Ap, Bp;
-            modal_trace<D, K, R, DT, 0, f>(uP, Am, Bm);
-            modal_trace<D, K, R, DT, 1, f>(uP, Ap, Bp);
-            tr[w][0][2 * fs][lane] = Am;
-            tr[w][0][2 * fs + 1][lane] = Bm;
-            tr[w][1][2 * fs][lane] = Ap;
-            tr[w][1][2 * fs + 1][lane] = Bp;
-        }
-    });
+    // 1. own traces on the '+' DT face -> LDS (the '-' face's follow after the first DT side)
+    auto put_traces = [&](auto SS) {
+        constexpr int s = SS;
+        static_for<Np>([&](auto FF) {
+            constexpr int f = FF;
+            if constexpr (mode_exponent(D, K, f, DT) == 0) {
+                constexpr int fs = face_slot(D, K, f, DT);
+                R A, B;
+                modal_trace<D, K, R, DT, s, f>(uP, A, B);
+                tr[w][2 * fs][lane] = A;
+                tr[w][2 * fs + 1][lane] = B;
+            }
+        });
+    };
+    put_traces(std::integral_constant<int, 1>{});
     // 2. volume
     volume_term_cf<D, K, R>(uP, s2, nuP, y);
     // 3. DF faces from the adjacent lanes
@@ -907,12 +911,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
             }, y);
         });
     }
-    // 5. DT faces: the adjacent waves' traces from LDS, the workgroup's edge rows from memory
-    __syncthreads();
+    // 5. DT faces: the adjacent waves' traces from LDS, the workgroup's edge rows from memory;
+    // the '-' side reads wave w - 1's '+' traces, then every wave replaces its '+' traces by
+    // its '-' traces, which the '+' side of wave w + 1 reads
     {
         constexpr int d = DT;
         const R ihd = ih[d];
-        static_for<2>([&](auto SS) {
+        auto dt_side = [&](auto SS) {
             constexpr int s = SS;
             const auto g = side_geom<D, R, d, s>(a, idx, nn, cjP, ihd, pen);
             const R nuQ = a.nu[g.cq];
@@ -922,8 +927,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
                 const int wr = g.bnd ? w : wn;
                 face_side_traces<D, K, R, d, s>(uP, cf, [&](auto, auto FS, R& An, R& Bn) {
                     constexpr int fs = decltype(FS)::value;
-                    An = tr[wr][1 - s][2 * fs][lane];   // on a boundary side: finite, multiplied by zero
-                    Bn = tr[wr][1 - s][2 * fs + 1][lane];
+                    An = tr[wr][2 * fs][lane];   // on a boundary side: finite, multiplied by zero
+                    Bn = tr[wr][2 * fs + 1][lane];
                 }, y);
             } else {
                 R un[Np];
@@ -939,16 +944,21 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) voi
                     modal_trace<D, K, R, d, 1 - s, decltype(F)::value>(un, An, Bn);
                 }, y);
             }
-        });
+        };
+        __syncthreads();   // every wave's '+' traces are in LDS
+        dt_side(std::integral_constant<int, 0>{});
+        __syncthreads();   // ... and have been read
+        put_traces(std::integral_constant<int, 0>{});
+        __syncthreads();
+        dt_side(std::integral_constant<int, 1>{});
     }
     // 6. results: the wave's row of cells is contiguous in memory; after the barrier no wave
     // reads the trace buffers any more, so each wave stages its results in its own
-    static_assert(4 * NFM >= Np + 1, "trace buffer too small to stage the results");
     __syncthreads();
     const int nvalid = r < nn[DT] ? (nn[DF] - x0 < 64 ? nn[DF] - x0 : 64) : 0;
     int idx0[3] = {idx[0], idx[1], idx[2]};
     idx0[DF] = x0;
-    store_cells_coalesced<Np, R>(&tr[w][0][0][0], y, lane, (size_t)encode<D>(idx0, a.n0, a.n1), nvalid, a);
+    store_cells_coalesced<Np, R>(&tr[w][0][0], y, lane, (size_t)encode<D>(idx0, a.n0, a.n1), nvalid, a);
 }
 
 // Assembly: thread = (cell r, trial mode n).  Column n of the diagonal block
