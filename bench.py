@@ -227,7 +227,9 @@ def solve_leg(B, ctx, grid, op, n, k, out):
             res[name].update({"learned_steps_taken": acc, "learned_steps_declined": rej,
                               "learned_still_active": active,
                               "learned_mean_removed_residual_sq_fraction": gain,
-                              "learned_fallback": "probe 8 calls, switch off below 1e-2 mean removed |r|^2 fraction"})
+                              "learned_probe_costs": B.pmg_learned_costs(mg),
+                              "learned_fallback": "probe 8 calls; switch off below 1e-2 mean removed |r|^2 fraction "
+                                                  "or when the step's log-reduction per ms is below the cycle's"})
         del fg, mg, x, r
     base = res["plain_fp64"]["solver_run_s"]
     base_all = base + res["plain_fp64"]["solver_init_s"]
@@ -305,7 +307,8 @@ def learned_leg(B, ctx, out, n=64, k=2, steps=2000, batch=32, nu_seed=5):
             acc, rej = B.pmg_learned_stats(mg)
             active, gain = B.pmg_learned_info(mg)
             r.update({"learned_accepted": acc, "learned_rejected": rej, "learned_still_active": active,
-                      "learned_mean_removed_residual_sq_fraction": gain})
+                      "learned_mean_removed_residual_sq_fraction": gain,
+                      "learned_probe_costs": B.pmg_learned_costs(mg)})
         res[name] = r
         del fg
     B.pmg_set_learned(mg, None)

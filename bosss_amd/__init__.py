@@ -103,6 +103,8 @@ SIGNATURES = {
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_learned_fallback": (_c_i, [_c_p, _c_i, _c_d]),
     "bh_pmg_learned_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_d)]),
+    "bh_pmg_set_learned_cost_rule": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_learned_costs": (_c_i, [_c_p, _c_p]),
     "bh_euler_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_d, _PP]),
     "bh_euler_destroy": (_c_i, [_c_p]),
     "bh_euler_info": (_c_i, [_c_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_i)]),
@@ -687,6 +689,21 @@ def pmg_learned_stats(pmg):
 def pmg_set_learned_fallback(pmg, probe=8, min_gain=1e-2):
     """After `probe` calls, a mean removed |r'|^2 fraction below min_gain switches the network off."""
     _check(lib().bh_pmg_set_learned_fallback(pmg.h, int(probe), float(min_gain)))
+
+
+def pmg_set_learned_cost_rule(pmg, on=True):
+    """At the end of the probe window keep the network only if flexible GMRES with it
+    reduces the residual at least as fast per millisecond as without it (on by default)."""
+    _check(lib().bh_pmg_set_learned_cost_rule(pmg.h, 1 if on else 0))
+
+
+def pmg_learned_costs(pmg):
+    """dict(cycle_ms, step_ms, cycle_log_reduction, step_log_reduction): per iteration of
+    flexible GMRES preconditioned without (cycle_) and with (step_) the learned step."""
+    out = np.zeros(4)
+    _check(lib().bh_pmg_learned_costs(pmg.h, out.ctypes.data))
+    return {"cycle_ms": float(out[0]), "step_ms": float(out[1]), "cycle_log_reduction": float(out[2]),
+            "step_log_reduction": float(out[3])}
 
 
 def pmg_learned_info(pmg):

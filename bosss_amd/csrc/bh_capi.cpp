@@ -1039,6 +1039,24 @@ int bh_pmg_set_learned_fallback(bh_solver* s, int probe, double min_gain) {
     mg->learned_min_gain = min_gain;
     BH_API_END
 }
+int bh_pmg_set_learned_cost_rule(bh_solver* s, int on) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_set_learned_cost_rule: solver is not a multigrid preconditioner");
+    mg->learned_cost_rule = on != 0;
+    BH_API_END
+}
+int bh_pmg_learned_costs(bh_solver* s, double* out) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_learned_costs: solver is not a multigrid preconditioner");
+    BH_CHECK(out, "bh_pmg_learned_costs: out is NULL");
+    out[0] = mg->learned_t_cycle;
+    out[1] = mg->learned_t_net;
+    out[2] = mg->learned_l_cycle;
+    out[3] = mg->learned_l_net;
+    BH_API_END
+}
 int bh_pmg_learned_info(bh_solver* s, int* active, double* mean_gain) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);

@@ -136,6 +136,13 @@ struct PMultigrid : bh_solver {
     int learned_calls = 0;
     bool learned_off = false;
     double learned_mean_gain = 0.0;
+    // cost rule at the end of the probe window: flexible GMRES preconditioned with the
+    // learned step must reduce the residual at least as fast per millisecond as with
+    // the plain cycle (per-iteration times and log-reductions of both)
+    bool learned_cost_rule = true;
+    bool in_cost_rule = false;
+    double learned_t_cycle = 0.0, learned_t_net = 0.0, learned_l_cycle = 0.0, learned_l_net = 0.0;
+    void krylov_rate(bool net, int steps, double& lred, double& ms);
     template <typename T> void learned_correction();
     void attach_learned(struct Mlp* m);
     ~PMultigrid() override;

@@ -14,6 +14,7 @@
 // Cycle: V(pre, post) with block-Jacobi smoothing x <- x + omega Dinv (b - A x)
 // on every level (AS/BlockJacobi.cs:196-255), in FP64 or FP32.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <map>
 
@@ -761,6 +762,59 @@ void PMultigrid::attach_learned(Mlp* m) {
     learned_calls = 0;
     learned_off = false;
     learned_mean_gain = 0.0;
+    learned_t_cycle = learned_t_net = learned_l_cycle = learned_l_net = 0.0;
+}
+
+// `steps` iterations of flexible GMRES (FP64, no restart) on the finest level from
+// x = 0, preconditioned by this multigrid with (net) or without the learned step, on
+// the last call's right-hand side.  lred: log-reduction of |b - A x| per iteration;
+// ms: wall time per iteration (the solver synchronises with the host in every
+// iteration, with or without the network alike).  The calls run eagerly, outside the
+// probe count, and leave the learned step's statistics as they were.
+void PMultigrid::krylov_rate(bool net, int steps, double& lred, double& ms) {
+    MgLevel& L = lv[0];
+    double keep[8];
+    ctx->download(keep, lstat.p, sizeof(keep));
+    DevBuf b64(N * sizeof(double)), x64(N * sizeof(double));
+    if (prec == BH_FP32) convert_f32_f64(ctx, vec<float>(L.b), b64.as<double>(), N);
+    else copy(ctx, b64.as<double>(), vec<double>(L.b), N);
+    fill(ctx, x64.as<double>(), 0.0, N);
+    LinOp A;
+    if (L.op) A.op = L.op;
+    else A.M = L.A;
+    A.N = N;
+    const bool graphs = use_graphs, off = learned_off;
+    const int its = iterations;
+    use_graphs = false;   // the captured cycle has the learned step or not: run eagerly
+    learned_off = !net;
+    in_cost_rule = true;
+    double t = 0.0;
+    double r0 = 0.0, r1 = 0.0;
+    int done = 0;
+    try {
+        GmresSolver gm(A, this, steps + 1, 1e-300, steps, true);
+        ctx->sync();
+        const auto t0 = std::chrono::steady_clock::now();
+        gm.solve_dev(x64.as<double>(), b64.as<double>());
+        ctx->sync();
+        t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        r0 = gm.res0;
+        r1 = gm.res;
+        done = gm.iterations;
+    } catch (...) {
+        use_graphs = graphs;
+        learned_off = off;
+        in_cost_rule = false;
+        iterations = its;
+        throw;
+    }
+    use_graphs = graphs;
+    learned_off = off;
+    in_cost_rule = false;
+    iterations = its;
+    ctx->upload(lstat.p, keep, sizeof(keep));
+    lred = (done > 0 && r0 > 0 && r1 > 0) ? std::log(r0 / r1) / done : 0.0;
+    ms = done > 0 ? t / done : 0.0;
 }
 
 // After the cycle (x = z1, the cycle's correction from zero, in lv[0].x):
@@ -815,11 +869,24 @@ void PMultigrid::precondition(double* x, const double* b) {
         else copy(ctx, x, vec<double>(L.x), N);
     }
     iterations++;
-    if (learned && !learned_off && learned_probe > 0 && ++learned_calls == learned_probe) {
+    if (learned && !learned_off && !in_cost_rule && learned_probe > 0 && ++learned_calls == learned_probe) {
         double st[8];
         ctx->download(st, lstat.p, sizeof(st));
         learned_mean_gain = st[6] / learned_probe;
-        if (!(learned_mean_gain >= learned_min_gain)) {
+        bool keep = learned_mean_gain >= learned_min_gain;
+        if (keep && learned_cost_rule) {
+            // what the network removes may be residual the Krylov iteration removes
+            // anyway: compare flexible GMRES's residual reduction per millisecond
+            // (each run twice, the first as a warm-up of both paths)
+            constexpr int steps = 12;
+            for (int rep = 0; rep < 2; ++rep) {
+                krylov_rate(false, steps, learned_l_cycle, learned_t_cycle);
+                krylov_rate(true, steps, learned_l_net, learned_t_net);
+            }
+            keep = learned_t_cycle > 0 && learned_t_net > 0 &&
+                   learned_l_net / learned_t_net >= learned_l_cycle / learned_t_cycle;
+        }
+        if (!keep) {
             learned_off = true;   // the plain multigrid takes over
             drop_graphs();
         }

@@ -308,6 +308,17 @@ int bh_pmg_learned_stats(bh_solver* pmg, int* accepted, int* rejected);
  * the cycle's correction that the network's direction removes (over the probe
  * window once it is closed, else over the calls so far). */
 int bh_pmg_set_learned_fallback(bh_solver* pmg, int probe, double min_gain);
+/* No counterpart in the reference.  Cost rule of the fallback (on by default): when
+ * the probe window closes with the mean gain above min_gain, 12 iterations of
+ * flexible GMRES (FP64) on the last call's right-hand side are run twice on the
+ * finest level, preconditioned by the cycle alone and by the cycle with the learned
+ * step, and the network stays only if its run reduces |b - A x| at least as fast per
+ * millisecond: what the network removes after a cycle may be residual the Krylov
+ * iteration would remove anyway.  bh_pmg_learned_costs writes [ms per iteration
+ * without, ms per iteration with the network, log-reduction per iteration without,
+ * with] (zeros until the rule has run). */
+int bh_pmg_set_learned_cost_rule(bh_solver* pmg, int on);
+int bh_pmg_learned_costs(bh_solver* pmg, double* out4);
 /* Schwarz (AS/Schwarz.cs:56-1454) with the multigrid-aggregate blocking and no
  * overlap: on every level that is aggregated (the k = 1 level and the h-levels
  * below it), the smoother becomes x (+)= omega sum_I R_I^T A_I^-1 R_I r over

@@ -183,3 +183,19 @@ def test_learned_correction_gain(ctx):
     _say(f"  FGMRES iterations over 4 fresh right-hand sides: plain {its[False]}, learned {its[True]}")
     assert gain >= 0.2
     assert its[True] <= 1.1 * its[False]
+    # the cost rule decided at the end of the probe window: the network stays exactly
+    # when flexible GMRES with it reduces the residual at least as fast per millisecond
+    # as without it; the step costs time
+    c = B.pmg_learned_costs(mg)
+    _say(f"  probe costs: {c}")
+    assert 0 < c["cycle_ms"] < c["step_ms"] and c["cycle_log_reduction"] > 0 and c["step_log_reduction"] > 0
+    keep = c["step_log_reduction"] / c["step_ms"] >= c["cycle_log_reduction"] / c["cycle_ms"]
+    assert active == keep
+    # without the cost rule the gain alone decides: the network stays on
+    B.pmg_set_learned(mg, net)
+    B.pmg_set_learned_cost_rule(mg, False)
+    fg = B.FlexGMRES(op=op, Precond=mg, MaxKrylovDim=30, ConvergenceCriterion=1e-8, MaxSolverIterations=300)
+    fg.Solve(np.zeros(op.N), bench.sine_rhs_coeffs(n, 2, 6000))
+    assert fg.stats()["converged"]
+    assert B.pmg_learned_info(mg)[0]
+    assert B.pmg_learned_costs(mg)["step_ms"] == 0.0
