@@ -98,6 +98,7 @@ SIGNATURES = {
     "bh_pmg_learned_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)]),
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_schwarz": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_set_level_pmg": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_schwarz_overlap": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_pmg_set_diag_equilib": (_c_i, [_c_p]),
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
@@ -711,6 +712,12 @@ def pmg_learned_info(pmg):
     a, g = _c_i(), _c_d()
     _check(lib().bh_pmg_learned_info(pmg.h, ctypes.byref(a), ctypes.byref(g)))
     return bool(a.value), g.value
+
+
+def pmg_set_level_pmg(pmg, enable=True):
+    """LevelPmg on the finest level: low-order coarse correction, then each cell's
+    high-order modes with the inverse of their diagonal sub-block."""
+    _check(lib().bh_pmg_set_level_pmg(pmg.h, 1 if enable else 0))
 
 
 def pmg_set_schwarz(pmg, enable=True, overlap=0, overlap_scaling=False):

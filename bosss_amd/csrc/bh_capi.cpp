@@ -1085,6 +1085,15 @@ int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
     BH_API_END
 }
 
+int bh_pmg_set_level_pmg(bh_solver* s, int enable) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_set_level_pmg: solver is not a multigrid preconditioner");
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    mg->set_level_pmg(enable != 0);
+    BH_API_END
+}
+
 int bh_pmg_set_schwarz(bh_solver* s, int enable) { return bh_pmg_set_schwarz_overlap(s, enable, 0, 0); }
 
 int bh_pmg_set_schwarz_overlap(bh_solver* s, int enable, int overlap, int overlap_scaling) {

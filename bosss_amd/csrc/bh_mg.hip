@@ -555,6 +555,37 @@ __global__ __launch_bounds__(256) void schwarz_sum_k(int ncell, int bs, const in
     }
 }
 
+// ---------------------------------------------------------------- LevelPmg
+// high-order part of every cell's diagonal block, H_c(i, j) = D_c(lo + i, lo + j)
+// (blocks column-major in and out)
+__global__ __launch_bounds__(256) void hi_blocks_k(int J, int bs, int lo, const int64_t* __restrict__ diag_idx,
+                                                   const double* __restrict__ vals, double* __restrict__ out) {
+    const int nh = bs - lo;
+    const int64_t tot = (int64_t)J * nh * nh;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(t / (nh * nh));
+        const int e = (int)(t - (int64_t)c * nh * nh);
+        const int j = e / nh, i = e - j * nh;
+        out[t] = vals[(size_t)diag_idx[c] * bs * bs + (size_t)(lo + j) * bs + lo + i];
+    }
+}
+
+// x_c[lo + i] += sum_j Hinv_c(i, j) r_c[lo + j]: thread (cell, high-order row)
+template <typename T>
+__global__ __launch_bounds__(256) void hi_smooth_k(int J, int bs, int lo, const T* __restrict__ Hinv, const T* __restrict__ r,
+                                                   T* __restrict__ x) {
+    const int nh = bs - lo;
+    const int64_t tot = (int64_t)J * nh;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(t / nh), i = (int)(t - (int64_t)c * nh);
+        const T* __restrict__ H = Hinv + (size_t)c * nh * nh;
+        const T* __restrict__ rc = r + (size_t)c * bs + lo;
+        T acc = 0;
+        for (int j = 0; j < nh; ++j) acc += H[(size_t)j * nh + i] * rc[j];
+        x[(size_t)c * bs + lo + i] += acc;
+    }
+}
+
 // ---------------------------------------------------------------- K-cycle scalars
 // The inner flexible GMRES of the K-cycle keeps its Hessenberg matrix, Gram-
 // Schmidt coefficients and least-squares solution in device memory (layout
@@ -655,6 +686,17 @@ __global__ __launch_bounds__(256) void sub_k(const T* __restrict__ b, const T* _
 }
 
 // ------------------------------------------------------------------ launchers
+void hi_blocks(bh_ctx* ctx, int J, int bs, int lo, const bh_bsr* A, double* out) {
+    hipLaunchKernelGGL(hi_blocks_k, grid1d((int64_t)J * (bs - lo) * (bs - lo)), dim3(256), 0, ctx->stream, J, bs, lo,
+                       A->diag_idx.as<int64_t>(), A->vals.as<double>(), out);
+    BH_HIP(hipGetLastError());
+}
+template <typename T>
+void hi_smooth(bh_ctx* ctx, int J, int bs, int lo, const T* Hinv, const T* r, T* x) {
+    hipLaunchKernelGGL((hi_smooth_k<T>), grid1d((int64_t)J * (bs - lo)), dim3(256), 0, ctx->stream, J, bs, lo, Hinv, r, x);
+    BH_HIP(hipGetLastError());
+}
+
 void gather_agg_blocks(bh_ctx* ctx, int nagg, int bs, int cmax, const int* cptr, const int* cidx, const bh_bsr* A,
                        double* out) {
     hipLaunchKernelGGL(gather_agg_blocks_k, grid1d((int64_t)nagg * cmax * cmax), dim3(256), 0, ctx->stream, nagg, bs, cmax,
@@ -878,7 +920,8 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void scale_inv_norm<T>(bh_ctx*, const T*, T*, int64_t, const double*, double);           \
     template void schwarz_apply<T>(bh_ctx*, int, int, int, const int*, const int*, const T*, const T*, T*, double, bool); \
     template void orth2_step<T>(bh_ctx*, const T*, T*, int64_t, double*, int*); \
-    template void sub_vec<T>(bh_ctx*, const T*, const T*, T*, int64_t);
+    template void sub_vec<T>(bh_ctx*, const T*, const T*, T*, int64_t);            \
+    template void hi_smooth<T>(bh_ctx*, int, int, int, const T*, const T*, T*);
 BH_INST(float)
 BH_INST(double)
 #undef BH_INST

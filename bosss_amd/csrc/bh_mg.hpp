@@ -21,6 +21,9 @@ template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr
 template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc,
                                           T* xf);
 template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
+// LevelPmg: high-order parts (modes lo..bs-1) of the diagonal blocks; x_hi += Hinv r_hi per cell
+void hi_blocks(bh_ctx*, int J, int bs, int lo, const bh_bsr* A, double* out);
+template <typename T> void hi_smooth(bh_ctx*, int J, int bs, int lo, const T* Hinv, const T* r, T* x);
 // block equilibration (DiagBlockEquilib): factors of the diagonal blocks D = C S C^T (S signs) and
 // L = C^-1 (L D L^T = S), the transform A_ij <- L_i A_ij L_j^T, transfer blocks Pt_i = C_i^T P_i L_I^T,
 // and the smoother x (+)= omega S r of an equilibrated level
@@ -113,6 +116,12 @@ struct PMultigrid : bh_solver {
     int pre = 1, post = 1;
     bool matrix_free = true;
     bool equilib = false;          // DiagBlockEquilib: every level but the coarsest in its block-equilibrated basis
+    // LevelPmg on the finest level: coarse (low-order) correction, then the high-order modes
+    // of each cell with the inverse of their diagonal sub-block
+    bool level_pmg = false;
+    int lp_lo = 0;
+    DevBuf lp64, lp32;
+    void set_level_pmg(bool on);
     bh_bsr* eqA0 = nullptr;        // the transformed finest matrix (the caller's matrix stays untouched)
     int kcycle_level = -1;         // level solved by `kin` inner FGMRES steps (K-cycle), -1: none
     int kin = 0;

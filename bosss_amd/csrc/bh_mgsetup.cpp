@@ -444,6 +444,18 @@ void PMultigrid::vcycle(int l, const T* b, T* x) {
         else dense_gemv<T, double>(ctx, ncoarse, Ainv64.as<double>(), b, x);
         return;
     }
+    if (l == 0 && level_pmg) {
+        // LevelPmg (AS/LevelPmg.cs:561-600): the low-order correction from x = 0 (the
+        // residual is b), then the high-order modes against the new residual
+        restrict_to<T>(0, b);
+        solve_level<T>(1);
+        if constexpr (sizeof(T) == 4) fill32(ctx, (float*)x, 0.0f, L.N);
+        else fill(ctx, (double*)x, 0.0, L.N);
+        prolong_add<T>(0, x);
+        residual<T>(0, b, x, r);
+        hi_smooth<T>(ctx, L.J, L.bs, lp_lo, (const T*)(f32 ? lp32.p : lp64.p), r, x);
+        return;
+    }
     // pre-smoothing from a zero guess: the first sweep is x = omega M^-1 b
     smooth<T>(l, b, x, false);
     for (int s = 1; s < pre; ++s) {
@@ -482,6 +494,33 @@ void PMultigrid::prolong_add(int l, T* x) {
     if (equilib) rect_prolong_add<T>(ctx, L.J, L.bs, C.bs, L.eqparent.as<int>(), Pt, vec<T>(C.x), x);
     else if (L.kind == 0) p_prolong_add<T>(ctx, L.J, L.bs, C.bs, vec<T>(C.x), x);
     else agg_prolong_add<T>(ctx, L.J, L.bs, L.parent.as<int>(), L.pidx.as<int>(), P, vec<T>(C.x), x);
+}
+
+// LevelPmg with UseDiagonalPmg (AS/LevelPmg.cs:232-330): the coarse system is the next
+// level (the modes of degree <= its degree: the first lv[1].bs modes of a cell); the
+// high-order modes of each cell are solved with the inverse of their part of the
+// cell's diagonal block (the reference factorises it by LU; here it is inverted once,
+// like the block-Jacobi blocks).  Replaces the finest level's smoothing.
+void PMultigrid::set_level_pmg(bool on) {
+    drop_graphs();
+    level_pmg = false;
+    lp64.release();
+    lp32.release();
+    if (!on) return;
+    BH_CHECK(!equilib, "multigrid: LevelPmg and DiagBlockEquilib are exclusive");
+    BH_CHECK(lv.size() >= 2 && lv[0].kind == 0, "multigrid: LevelPmg needs a p-level (degree truncation) below the finest");
+    MgLevel& L = lv[0];
+    const int lo = lv[1].bs, nh = L.bs - lo;
+    BH_CHECK(nh > 0 && nh <= 128, "multigrid: LevelPmg needs 1 to 128 high-order modes per cell");
+    L.A->ensure_diag();
+    DevBuf dense((size_t)L.J * nh * nh * sizeof(double));
+    hi_blocks(ctx, L.J, L.bs, lo, L.A, dense.as<double>());
+    lp64.alloc((size_t)L.J * nh * nh * sizeof(double));
+    if (prec == BH_FP32) lp32.alloc((size_t)L.J * nh * nh * sizeof(float));
+    invert_dense_blocks(ctx, L.J, nh, dense.as<double>(), lp64.as<double>(), prec == BH_FP32 ? lp32.as<float>() : nullptr);
+    if (prec == BH_FP32) lp64.release();
+    lp_lo = lo;
+    level_pmg = true;
 }
 
 // smoother of level l: block-Jacobi on the cell blocks (AS/BlockJacobi.cs:196-255), or

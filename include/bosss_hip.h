@@ -327,6 +327,16 @@ int bh_pmg_learned_costs(bh_solver* pmg, double* out4);
  * block-Jacobi.  Applies to bh_pmg_create and to the coarse hierarchy of
  * bh_xdg_precond_create. */
 int bh_pmg_set_schwarz(bh_solver* pmg, int enable);
+/* LevelPmg with UseDiagonalPmg (AS/LevelPmg.cs:22-60 config, :232-330 set-up, :397-470
+ * CoarseSolve / FineSolve, :561-600 Solve) as the finest level of a bh_pmg_create
+ * hierarchy whose second level is a degree truncation: x = P x_c with x_c the
+ * next level's solve of the low-order part of the residual (the modes of degree <=
+ * that level's), then the high-order modes of every cell corrected with the inverse
+ * of their part of the cell's diagonal block against the new residual.  Replaces
+ * the finest level's pre- and post-smoothing (enable = 1), or restores them (0).
+ * The low-order system is solved by the next levels (V- or K-cycle), not by a
+ * sparse direct solver. */
+int bh_pmg_set_level_pmg(bh_solver* pmg, int enable);
 /* Schwarz with overlap (Schwarz.Config.Overlap, AS/Schwarz.cs:491-512; block growth
  * :683-730; the additive sum of the block solutions :1026-1085): each aggregate block
  * is grown by `overlap` (0, 1 or 2) sweeps over the face neighbours of its cells and

@@ -73,6 +73,7 @@ def lib():
             "orc_pmg_level_spmv": (None, [_p, _i, _dp, _dp]),
             "orc_pmg_set": (None, [_p, _d, _i, _i]),
             "orc_pmg_set_schwarz": (_i, [_p, _i, _i, _i]),
+            "orc_pmg_set_level_pmg": (_i, [_p, _i]),
             "orc_pmg_smooth": (None, [_p, _i, _p, _p]),
             "orc_pmg_level_info": (None, [_p, _i, _ip]),
             "orc_pmg_level_transfer": (_i, [_p, _i, _ip, _dp]),
@@ -324,6 +325,12 @@ class Pmg:
         x = np.zeros(self.sizes[level])
         lib().orc_pmg_smooth(self.h, level, np.ascontiguousarray(r, np.float64).ctypes.data, x.ctypes.data)
         return x
+
+    def set_level_pmg(self, on=True):
+        """LevelPmg on the finest level: low-order coarse correction, then the high-order
+        modes of every cell with their diagonal sub-block (AS/LevelPmg.cs, UseDiagonalPmg)."""
+        if lib().orc_pmg_set_level_pmg(self.h, 1 if on else 0) != 0:
+            raise ArithmeticException_("LevelPmg set-up failed (needs a p-level above the next level)")
 
     def set_schwarz(self, on=True, overlap=0, scaling=False):
         """Additive Schwarz over the aggregates, grown by `overlap` (0-2) face-neighbour

@@ -824,6 +824,11 @@ struct Pmg {
     int nc = 0;
     double omega = 1.0;
     int pre = 1, post = 1;
+    // LevelPmg on the finest level: per cell the inverse of the diagonal block's
+    // high-order part (modes lp_lo.. of the cell), row-major
+    bool level_pmg = false;
+    int lp_lo = 0;
+    std::vector<double> lp_inv;
 };
 
 // P[m][n] = int_f phi^f_m phi^c_n dx, evaluated with the fine cell's Gauss rule
@@ -1071,6 +1076,34 @@ static void bgemv(const MgLev& L, const double* r, double* x, double omega, bool
     }
 }
 
+// LevelPmg (AS/LevelPmg.cs:232-330 set-up with UseDiagonalPmg, :397-470 CoarseSolve and
+// FineSolve, :561-600 Solve): the low-order modes (degree <= the next level's) form the
+// coarse system; the high-order modes of each cell are solved with the inverse of
+// their part of the cell's diagonal block.
+int pmg_set_level_pmg(Pmg& P, bool on) {
+    P.level_pmg = false;
+    P.lp_inv.clear();
+    if (!on) return 0;
+    if (P.lv.size() < 2 || P.lv[0].kind != 0) return -1;
+    const Bsr& A = P.lv[0].A;
+    const int bs = A.bs, lo = P.lv[1].A.bs, nh = bs - lo;
+    if (nh <= 0) return -1;
+    P.lp_lo = lo;
+    P.lp_inv.assign((size_t)A.nrows * nh * nh, 0.0);
+    for (int c = 0; c < A.nrows; ++c) {
+        const double* Dc = nullptr;
+        for (int64_t p = A.rowptr[c]; p < A.rowptr[c + 1]; ++p)
+            if (A.col[p] == c) Dc = &A.val[(size_t)p * bs * bs];
+        if (!Dc) return -1;
+        double* H = &P.lp_inv[(size_t)c * nh * nh];
+        for (int i = 0; i < nh; ++i)
+            for (int j = 0; j < nh; ++j) H[i * nh + j] = Dc[(lo + i) * bs + lo + j];
+        if (!invert_block(H, nh)) return -1;
+    }
+    P.level_pmg = true;
+    return 0;
+}
+
 void pmg_vcycle(const Pmg& P, int l, const double* b, double* x) {
     const MgLev& L = P.lv[l];
     size_t N = (size_t)L.A.nrows * L.A.bs;
@@ -1084,6 +1117,28 @@ void pmg_vcycle(const Pmg& P, int l, const double* b, double* x) {
     }
     std::vector<double> r(N);
     auto resid = [&]() { std::copy(b, b + N, r.begin()); bsr_spmv(L.A, -1.0, x, 1.0, r.data()); };
+    if (l == 0 && P.level_pmg) {
+        // x = coarse correction from zero (low-order modes), then the high-order modes
+        const int bs = L.A.bs, lo = P.lp_lo, nh = bs - lo;
+        const size_t Nc = (size_t)L.A.nrows * lo;
+        std::vector<double> bc(Nc), xc(Nc, 0.0);
+        for (int j = 0; j < L.A.nrows; ++j)
+            for (int m = 0; m < lo; ++m) bc[(size_t)j * lo + m] = b[(size_t)j * bs + m];
+        pmg_vcycle(P, 1, bc.data(), xc.data());
+        std::fill(x, x + N, 0.0);
+        for (int j = 0; j < L.A.nrows; ++j)
+            for (int m = 0; m < lo; ++m) x[(size_t)j * bs + m] = xc[(size_t)j * lo + m];
+        resid();
+        for (int c = 0; c < L.A.nrows; ++c) {
+            const double* H = &P.lp_inv[(size_t)c * nh * nh];
+            for (int i = 0; i < nh; ++i) {
+                double s = 0;
+                for (int j = 0; j < nh; ++j) s += H[i * nh + j] * r[(size_t)c * bs + lo + j];
+                x[(size_t)c * bs + lo + i] += s;
+            }
+        }
+        return;
+    }
     bgemv(L, b, x, P.omega, false);
     for (int s = 1; s < P.pre; ++s) { resid(); bgemv(L, r.data(), x, P.omega, true); }
     resid();
@@ -1477,6 +1532,9 @@ void orc_pmg_set(void* p, double omega, int pre, int post) {
 void orc_pmg_smooth(void* p, int l, const double* r, double* x) {
     Pmg* P = (Pmg*)p;
     bgemv(P->lv[l], r, x, P->omega, false);
+}
+int orc_pmg_set_level_pmg(void* p, int on) {
+    try { return pmg_set_level_pmg(*(Pmg*)p, on != 0); } catch (...) { return -1; }
 }
 int orc_pmg_set_schwarz(void* p, int on, int overlap, int scale) {
     try { pmg_set_schwarz(*(Pmg*)p, on != 0, overlap, scale != 0); return 0; } catch (...) { return -1; }

@@ -118,6 +118,49 @@ def test_schwarz_overlap_fgmres_converges(ctx):
         B.pmg_set_schwarz(mg, True, overlap=3)
 
 
+@pytest.mark.parametrize("matrix_free", [False, True])
+@pytest.mark.parametrize("prec,tol", [(B.BH_FP64, 1e-11), (B.BH_FP32, 1e-4)])
+def test_level_pmg_matches_oracle(ctx, matrix_free, prec, tol):
+    """LevelPmg (AS/LevelPmg.cs, UseDiagonalPmg): the low-order correction through the
+    p/h hierarchy below, then each cell's high-order modes with the inverse of their
+    diagonal sub-block; device cycle vs the oracle cycle."""
+    _say(f"case level pmg matrix_free={matrix_free} prec={prec}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(8).standard_normal(oo.N)
+    P = O.Pmg(Ao, og, [3, 1], min_coarse=8, omega=0.7, pre=2, post=1)
+    P.set_level_pmg(True)
+    ref = P.vcycle(b)
+    mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, omega=0.7, pre=2, post=1, kcycle_inner=0,
+                      matrix_free=matrix_free, prec=prec)
+    B.pmg_set_level_pmg(mg, True)
+    out = mg.Solve(np.zeros(oo.N), b)
+    assert rel(out, ref) < tol
+    B.pmg_set_level_pmg(mg, False)
+    P.set_level_pmg(False)
+    assert rel(mg.Solve(np.zeros(oo.N), b), P.vcycle(b)) < tol
+
+
+def test_level_pmg_fgmres_converges(ctx):
+    """FGMRES preconditioned by LevelPmg over the K-cycle reaches 1e-8 in the true
+    residual and gives the plain multigrid's solution."""
+    _say("case fgmres level pmg")
+    og, gg, oo, go, Ao = _problem(ctx)
+    b = np.random.default_rng(9).standard_normal(oo.N)
+    sols = []
+    for on in (False, True):
+        mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, kcycle_inner=4, prec=B.BH_FP32)
+        B.pmg_set_level_pmg(mg, on)
+        S = B.FlexGMRES(op=go, Precond=mg, MaxKrylovDim=40, ConvergenceCriterion=1e-8)
+        u = S.Solve(np.zeros(oo.N), b)
+        st = S.stats()
+        _say(f"  level pmg {on}: {st['iterations']} iterations")
+        assert st["converged"]
+        r = b - go.Evaluate(1.0, 0.0, np.zeros(oo.N), u, affine=False)
+        assert np.linalg.norm(r) <= 1.0001e-8 * np.linalg.norm(b)
+        sols.append(u)
+    assert rel(sols[1], sols[0]) < 1e-6
+
+
 def _kcycle_numpy(P, level, b, kin):
     """kin flexible-GMRES steps on level `level` (CGS2), V-cycle preconditioned."""
     n = b.size

@@ -404,3 +404,36 @@ def test_schwarz_single_aggregate_is_exact_solve():
     b = np.random.default_rng(1).standard_normal(op.N)
     xe = np.linalg.solve(A.dense(), b)
     assert np.abs(P.vcycle(b) - xe).max() < 1e-10 * np.abs(xe).max()
+
+
+@pytest.mark.parametrize("D,k", [(2, 2), (3, 2), (3, 3)])
+def test_level_pmg_against_dense(D, k):
+    """LevelPmg with diagonal high-order blocks (AS/LevelPmg.cs:232-330, :397-470,
+    :561-600): the low-order modes (degree <= 1) solved exactly, then every cell's
+    high-order modes with the inverse of their part of its diagonal block, against
+    numpy on the dense matrix; a two-level hierarchy, so the low-order solve is the
+    oracle's dense coarsest solve."""
+    if D == 2:
+        g = O.Grid(*nodes_2d(5, 4))
+    else:
+        g = O.Grid(*nodes_3d(4, 3, 3, jitter=0.3, seed=2))
+    op = O.Sip(g, k, log_uniform_nu(g.J, 3))
+    A, _ = op.assemble()
+    P = O.Pmg(A, g, [k, 1], min_coarse=10 ** 6, omega=0.7, pre=1, post=1)
+    assert len(P.sizes) == 2
+    P.set_level_pmg(True)
+    M = A.dense()
+    Np = op.Np
+    Npl = D + 1
+    lo = np.concatenate([np.arange(c * Np, c * Np + Npl) for c in range(g.J)])
+    b = np.random.default_rng(11).standard_normal(op.N)
+    x = np.zeros(op.N)
+    x[lo] = np.linalg.solve(M[np.ix_(lo, lo)], b[lo])
+    r = b - M @ x
+    for c in range(g.J):
+        hi = np.arange(c * Np + Npl, (c + 1) * Np)
+        x[hi] += np.linalg.solve(M[np.ix_(hi, hi)], r[hi])
+    xo = P.vcycle(b)
+    assert np.abs(xo - x).max() < 1e-11 * np.abs(x).max()
+    P.set_level_pmg(False)
+    assert np.abs(P.vcycle(b) - xo).max() > 1e-3 * np.abs(x).max()   # the plain cycle differs
