@@ -102,6 +102,7 @@ SIGNATURES = {
     "bh_pmg_set_schwarz_overlap": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_pmg_set_diag_equilib": (_c_i, [_c_p]),
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
+    "bh_xdg_precond_coarse_export": (_c_i, [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "bh_pmg_set_learned_fallback": (_c_i, [_c_p, _c_i, _c_d]),
     "bh_pmg_learned_info": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_d)]),
     "bh_pmg_set_learned_cost_rule": (_c_i, [_c_p, _c_i]),
@@ -976,6 +977,17 @@ def xdg_cut_rule_fallbacks():
     v = ctypes.c_long()
     _check(lib().bh_xdg_cut_rule_fallbacks(ctypes.byref(v)))
     return v.value
+
+
+def xdg_precond_coarse_matrix(precond):
+    """(rowptr, col, blocks[nnzb, bs, bs]) of an XDG preconditioner's coarse matrix."""
+    n, bs, nz = _c_i(), _c_i(), _c_i64()
+    _check(lib().bh_xdg_precond_coarse_export(precond.h, ctypes.byref(n), ctypes.byref(bs), ctypes.byref(nz), None, None, None))
+    rp = np.zeros(n.value + 1, np.int64)
+    col = np.zeros(nz.value, np.int32)
+    val = np.zeros(nz.value * bs.value * bs.value)
+    _check(lib().bh_xdg_precond_coarse_export(precond.h, None, None, None, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+    return rp, col, val.reshape(nz.value, bs.value, bs.value)
 
 
 def xdg_precond_set_fp64_blocks(precond, on=True):

@@ -1077,6 +1077,21 @@ int bh_pmg_learned_info(bh_solver* s, int* active, double* mean_gain) {
     BH_API_END
 }
 
+int bh_xdg_precond_coarse_export(bh_solver* s, int* nrows, int* bs, int64_t* nnzb, int64_t* rowptr, int* col, double* vals) {
+    BH_API_BEGIN
+    XdgPrecond* xp = dynamic_cast<XdgPrecond*>(compressed(s));
+    BH_CHECK(xp && xp->Ac, "bh_xdg_precond_coarse_export: solver is not an XDG preconditioner");
+    bh_bsr* M = xp->Ac;
+    if (nrows) *nrows = M->nrows;
+    if (bs) *bs = M->bs;
+    if (nnzb) *nnzb = M->nnzb;
+    if (rowptr || col || vals) {
+        (void)hipStreamSynchronize(M->ctx->stream);
+        BH_CHECK(bh_bsr_export(M, rowptr, col, vals) == 0, "bh_xdg_precond_coarse_export: export failed");
+    }
+    BH_API_END
+}
+
 int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
     BH_API_BEGIN
     XdgPrecond* xp = dynamic_cast<XdgPrecond*>(compressed(s));

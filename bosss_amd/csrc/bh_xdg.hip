@@ -675,6 +675,65 @@ template void xdg_prolong_add<float>(bh_ctx*, const Vbsr&, int, int, const float
 template void xdg_prolong_add<double>(bh_ctx*, const Vbsr&, int, int, const double*, double*);
 
 // ------------------------------------------------------------------ host
+// Coarse matrix of the XDG preconditioner on the device.  Species units: block q of
+// Ac is the sum, in list order, of the leading Npc x Npc parts of the unit blocks
+// p listed for q (column-major Np x Np blocks in, Npc x Npc out).
+__global__ __launch_bounds__(256) void xdg_coarse_gather_k(int64_t nq, int Np, int Npc, const int64_t* __restrict__ qptr,
+                                                           const int64_t* __restrict__ plist, const double* __restrict__ vals,
+                                                           double* __restrict__ out) {
+    const int E = Npc * Npc;
+    const int64_t tot = nq * E;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = t / E;
+        const int e = (int)(t - q * E);
+        const int n = e / Npc, m = e - n * Npc;
+        double s = 0.0;
+        for (int64_t k = qptr[q]; k < qptr[q + 1]; ++k) s += vals[(size_t)plist[k] * Np * Np + (size_t)n * Np + m];
+        out[t] = s;
+    }
+}
+
+// A caller's cell blocks (R x C, column-major, R and C multiples of Np): block p of Ac
+// is Ac_p(m, n) = sum over the species blocks a < R / Np, b < C / Np of A_p(a Np + m, b Np + n)
+__global__ __launch_bounds__(256) void xdg_coarse_vblocks_k(int64_t nnzb, int Np, int Npc, const int* __restrict__ prow,
+                                                            const int* __restrict__ col, const int* __restrict__ rsz,
+                                                            const int64_t* __restrict__ voff, const double* __restrict__ vals,
+                                                            double* __restrict__ out) {
+    const int E = Npc * Npc;
+    const int64_t tot = nnzb * E;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = t / E;
+        const int e = (int)(t - p * E);
+        const int n = e / Npc, m = e - n * Npc;
+        const int R = rsz[prow[p]], C = rsz[col[p]];
+        const double* __restrict__ B = vals + voff[p];
+        double s = 0.0;
+        for (int a = 0; a < R / Np; ++a)
+            for (int b = 0; b < C / Np; ++b) s += B[(size_t)(b * Np + n) * R + a * Np + m];
+        out[t] = s;
+    }
+}
+
+static dim3 xdg_grid1d(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+void xdg_coarse_gather(bh_ctx* ctx, int64_t nq, int Np, int Npc, const int64_t* qptr, const int64_t* plist, const double* vals,
+                       double* out) {
+    hipLaunchKernelGGL(xdg_coarse_gather_k, xdg_grid1d(nq * Npc * Npc), dim3(256), 0, ctx->stream, nq, Np, Npc, qptr, plist, vals, out);
+    BH_HIP(hipGetLastError());
+}
+
+void xdg_coarse_vblocks(bh_ctx* ctx, const Vbsr& A, int Np, int Npc, const int* prow, double* out) {
+    const int64_t nnzb = (int64_t)A.col.size();
+    hipLaunchKernelGGL(xdg_coarse_vblocks_k, xdg_grid1d(nnzb * Npc * Npc), dim3(256), 0, ctx->stream, nnzb, Np, Npc, prow,
+                       A.col_d.as<int>(), A.rsz_d.as<int>(), A.voff_d.as<int64_t>(), A.vals.as<double>(), out);
+    BH_HIP(hipGetLastError());
+}
+
 void Vbsr::upload_structure() {
     rowptr_d.alloc((nrows + 1) * sizeof(int64_t));
     col_d.alloc(std::max<size_t>(col.size(), 1) * sizeof(int));

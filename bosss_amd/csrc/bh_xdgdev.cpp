@@ -482,47 +482,56 @@ void XdgPrecond::build(bh_grid* g, int degc, int min_coarse, int kcycle_inner) {
         if (Av) Av->ensure_fp32();
         else A->ensure_fp32();
     }
-    // coarse matrix Ac = R^T A R (species merged, first Npc modes of every species block)
-    std::vector<std::map<int, std::vector<double>>> crow(J);
+    // coarse matrix Ac = R^T A R (species merged, first Npc modes of every species block):
+    // the cell pattern on the host from the block structure, the values summed on the
+    // device (the matrix values never leave it)
+    std::vector<int64_t> rp(J + 1, 0);
+    std::vector<int> col;
+    DevBuf cvals;
     if (Av) {
-        std::vector<double> v(std::max<int64_t>(Av->nvals, 1));
-        ctx->download(v.data(), Av->vals.p, Av->nvals * sizeof(double));
+        rp = Av->rowptr;
+        col = Av->col;
+        std::vector<int> prow(std::max<size_t>(col.size(), 1));
         for (int r = 0; r < J; ++r)
-            for (int64_t p = Av->rowptr[r]; p < Av->rowptr[r + 1]; ++p) {
-                const int c = Av->col[p], R = Av->rsz[r], C = Av->rsz[c];
-                const double* B = &v[Av->voff[p]];   // column-major R x C
-                auto& blk = crow[r][c];
-                if (blk.empty()) blk.assign((size_t)Npc * Npc, 0.0);
-                for (int a = 0; a < R / Np; ++a)
-                    for (int b = 0; b < C / Np; ++b)
-                        for (int n = 0; n < Npc; ++n)
-                            for (int m = 0; m < Npc; ++m)
-                                blk[(size_t)n * Npc + m] += B[(size_t)(b * Np + n) * R + (a * Np + m)];
-            }
+            for (int64_t p = Av->rowptr[r]; p < Av->rowptr[r + 1]; ++p) prow[p] = r;
+        DevBuf dprow(prow.size() * sizeof(int));
+        ctx->upload(dprow.p, prow.data(), prow.size() * sizeof(int));
+        cvals.alloc(std::max<size_t>(col.size(), 1) * Npc * Npc * sizeof(double));
+        if (!col.empty()) xdg_coarse_vblocks(ctx, *Av, Np, Npc, dprow.as<int>(), cvals.as<double>());
+        ctx->sync();
     } else {
         std::vector<int> cell_of(A->nrows);
         for (int c = 0, u = 0; c < J; ++c)
             for (int a = 0; a < Dc->rsz[c] / Np; ++a) cell_of[u++] = c;
-        std::vector<double> v((size_t)A->nnzb * Np * Np);
-        ctx->download(v.data(), A->vals.p, v.size() * sizeof(double));
+        // per cell row: (cell column, unit block) in unit-row then unit-column order; a
+        // stable sort by cell column keeps that order within each coarse block
+        std::vector<std::vector<std::pair<int, int64_t>>> ent(J);
         for (int u = 0; u < A->nrows; ++u)
-            for (int64_t bp = A->h_rowptr[u]; bp < A->h_rowptr[u + 1]; ++bp) {
-                const double* B = &v[(size_t)bp * Np * Np];   // column-major Np x Np
-                auto& blk = crow[cell_of[u]][cell_of[A->h_col[bp]]];
-                if (blk.empty()) blk.assign((size_t)Npc * Npc, 0.0);
-                for (int n = 0; n < Npc; ++n)
-                    for (int m = 0; m < Npc; ++m) blk[(size_t)n * Npc + m] += B[(size_t)n * Np + m];
+            for (int64_t bp = A->h_rowptr[u]; bp < A->h_rowptr[u + 1]; ++bp) ent[cell_of[u]].push_back({cell_of[A->h_col[bp]], bp});
+        std::vector<int64_t> qptr{0}, plist;
+        plist.reserve((size_t)A->nnzb);
+        for (int r = 0; r < J; ++r) {
+            auto& e = ent[r];
+            std::stable_sort(e.begin(), e.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (size_t i = 0; i < e.size(); ++i) {
+                if (i == 0 || e[i].first != e[i - 1].first) {
+                    if (i > 0) qptr.push_back((int64_t)plist.size());
+                    col.push_back(e[i].first);
+                }
+                plist.push_back(e[i].second);
             }
-    }
-    std::vector<int64_t> rp(J + 1, 0);
-    std::vector<int> col;
-    std::vector<double> cv;
-    for (int r = 0; r < J; ++r) {
-        for (auto& kv : crow[r]) {
-            col.push_back(kv.first);
-            cv.insert(cv.end(), kv.second.begin(), kv.second.end());
+            if (!e.empty()) qptr.push_back((int64_t)plist.size());
+            rp[r + 1] = (int64_t)col.size();
+            std::vector<std::pair<int, int64_t>>().swap(e);
         }
-        rp[r + 1] = (int64_t)col.size();
+        DevBuf dq(qptr.size() * sizeof(int64_t)), dl(std::max<size_t>(plist.size(), 1) * sizeof(int64_t));
+        ctx->upload(dq.p, qptr.data(), qptr.size() * sizeof(int64_t));
+        ctx->upload(dl.p, plist.data(), plist.size() * sizeof(int64_t));
+        cvals.alloc(std::max<size_t>(col.size(), 1) * Npc * Npc * sizeof(double));
+        if (!col.empty())
+            xdg_coarse_gather(ctx, (int64_t)col.size(), Np, Npc, dq.as<int64_t>(), dl.as<int64_t>(), A->vals.as<double>(),
+                              cvals.as<double>());
+        ctx->sync();
     }
     Ac = new bh_bsr();
     Ac->ctx = ctx;
@@ -535,8 +544,7 @@ void XdgPrecond::build(bh_grid* g, int degc, int min_coarse, int kcycle_inner) {
     Ac->col.alloc(std::max<int64_t>(Ac->nnzb, 1) * sizeof(int));
     ctx->upload(Ac->rowptr.p, rp.data(), (J + 1) * sizeof(int64_t));
     ctx->upload(Ac->col.p, col.data(), col.size() * sizeof(int));
-    Ac->vals.alloc(std::max<size_t>(cv.size(), 1) * sizeof(double));
-    ctx->upload(Ac->vals.p, cv.data(), cv.size() * sizeof(double));
+    Ac->vals = std::move(cvals);
     cmg.reset(new PMultigrid());
     cmg->prec = prec;
     cmg->omega = omega;

@@ -74,6 +74,11 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
 template <typename T> void xdg_restrict(bh_ctx*, const Vbsr& A, int Np, int Npc, const T* r, T* rc);
 template <typename T> void xdg_prolong_add(bh_ctx*, const Vbsr& A, int Np, int Npc, const T* xc, T* x);
 
+// coarse matrix of the XDG preconditioner (leading Npc x Npc parts summed per cell pair)
+void xdg_coarse_gather(bh_ctx*, int64_t nq, int Np, int Npc, const int64_t* qptr, const int64_t* plist, const double* vals,
+                       double* out);
+void xdg_coarse_vblocks(bh_ctx*, const Vbsr& A, int Np, int Npc, const int* prow, double* out);
+
 // Two-level preconditioner of the agglomerated XDG system: FP32 (or FP64)
 // block-Jacobi smoothing on the variable-block matrix and a coarse
 // correction in the species-merged degree-1 space, solved by the p/h

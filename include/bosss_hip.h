@@ -477,6 +477,12 @@ int bh_xdg_dof_frame(bh_xdg* x, int which, int64_t* frame_i0, int* len, int64_t*
  * FP32); cut cells that couple two species at high contrast have blocks too
  * ill-conditioned for FP32-rounded inverses.  on = 1 / 0. */
 int bh_xdg_precond_set_fp64_blocks(bh_solver* solver, int on);
+/* No counterpart in the reference: the coarse matrix of an XDG preconditioner (cell
+ * rows, blocks of Np(coarse degree), species merged: the sum over the species blocks
+ * of a cell pair of their leading Npc x Npc parts), for checks.  Sizes first (arrays
+ * NULL), then rowptr (nrows + 1), col (nnzb) and row-major vals (nnzb bs^2). */
+int bh_xdg_precond_coarse_export(bh_solver* solver, int* nrows, int* bs, int64_t* nnzb, int64_t* rowptr, int* col,
+                                 double* vals);
 /* FGMRES on the agglomerated system (AS/FlexGMRES.cs), FP64 */
 int bh_xdg_fgmres_create(bh_xdg* x, bh_solver* precond, int restart, double tol, int maxit, bh_solver** solver);
 

@@ -171,3 +171,43 @@ def test_frame_solve_matches_library(ctx):
     uc = _from_frame(uf, i0, ln, c0, X.N)
     assert rel(A @ uc, b) <= 1.5e-10
     assert rel(uc, u) <= 1e-7
+
+
+def test_coarse_matrix_of_the_preconditioner(ctx):
+    """The preconditioner's coarse matrix, summed on the device from the species
+    units (library matrix) or from the caller's cell blocks (frame matrix), equals
+    R A R^T in numpy: R takes, per cell, the leading Np(1) modes of each present
+    species block and adds them (species merged)."""
+    _say("case xdg coarse matrix 12^3, unit blocks and caller's cell blocks")
+    g, X = _problem(ctx)
+    A = X.scipy_matrix(1).tocsr()
+    i0, ln, c0 = X.dof_frame(1)
+    Np, J = X.Np, X.J
+    Npc = 4
+    rows, cols = [], []
+    for c in range(J):
+        for a in range(ln[c] // Np):
+            for m in range(Npc):
+                rows.append(c * Npc + m)
+                cols.append(c0[c] + a * Np + m)
+    import scipy.sparse as sp
+    R = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(J * Npc, X.N))
+    ref = (R @ A @ R.T).toarray()
+    scale = np.abs(ref).max()
+
+    def dense(rp, col, blk):
+        out = np.zeros((J * Npc, J * Npc))
+        for r in range(J):
+            for p in range(rp[r], rp[r + 1]):
+                out[r * Npc:(r + 1) * Npc, col[p] * Npc:(col[p] + 1) * Npc] = blk[p]
+        return out
+
+    P = B.XdgPrecond(X, coarse_degree=1, prec=B.BH_FP32)
+    rp, col, blk = B.xdg_precond_coarse_matrix(P)
+    assert blk.shape[1:] == (Npc, Npc) and len(rp) == J + 1
+    assert np.all(np.diff(col[rp[0]:rp[1]]) > 0)
+    assert np.abs(dense(rp, col, blk) - ref).max() <= 1e-13 * scale
+    M, _, _, _ = _frame_matrix(ctx, X, 1)
+    Pv = B.XdgPrecondV(g, M, Np, coarse_degree=1, prec=B.BH_FP32)
+    rpv, colv, blkv = B.xdg_precond_coarse_matrix(Pv)
+    assert np.abs(dense(rpv, colv, blkv) - ref).max() <= 1e-13 * scale
