@@ -459,7 +459,7 @@ def xdg_leg(B, ctx, out, n=48, seed=3):
     res["evaluate_fp64"] = {"ms": ms, "dof_per_s": X.N_full / (ms * 1e-3), "elements": nel, "colour_groups": 14,
                             "algorithmic_bytes": nbytes, "GBps": nbytes / (ms * 1e-3) / 1e9,
                             "hbm_frac": nbytes / (ms * 1e-3) / PEAK_HBM,
-                            "note": "one wave per element; the basis is evaluated at every node (compute-bound)"}
+                            "note": "node-parallel: one quadrature node per lane, uncut elements packed several per wave; the basis is evaluated at every node (compute-bound)"}
     del uf, yf
     b = X.rhs(1)
     # Schwarz on the aggregates of the coarse hierarchy (both precisions); in FP32 the

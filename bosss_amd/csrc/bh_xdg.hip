@@ -8,15 +8,16 @@
 // XEvaluatorLinear does (X/XSpatialOperatorMk2_XEvaluators.cs:163) into a
 // block matrix over (cell, species) blocks.
 //
-// One wave per element.  Uncut cells and uncut faces use the tensor Gauss
-// rule (27 / 9 nodes for k = 2), cut cells and cut faces their own rules
-// (a few hundred nodes) -- both kinds are items of the same launch.  At each
-// node the lanes evaluate the orthonormal basis (and gradients) of both sides
-// into LDS, then every lane accumulates its (test, trial) entries of the
-// element matrix; the element matrix is added to the global blocks with plain
-// adds: the elements run in colour groups (XDG_COLOURS) within which no two touch
-// the same entry, one launch per group in a fixed order, so the result is
-// bitwise reproducible.
+// Assembly: one wave per element.  Uncut cells and uncut faces use the tensor
+// Gauss rule (27 / 9 nodes for k = 2), cut cells and cut faces their own rules
+// (a few hundred nodes).  At each node the lanes evaluate the orthonormal basis
+// (and gradients) of both sides into LDS, then every lane accumulates its (test,
+// trial) entries of the element matrix; the element matrix is added to the global
+// blocks with plain adds: the elements run in colour groups (XDG_COLOURS) within
+// which no two touch the same entry, one launch per group in a fixed order, so the
+// result is bitwise reproducible.  The matrix-free evaluate is node-parallel
+// (one quadrature node per lane, several uncut elements per wave; see
+// xdg_evaluate_kernel) over the same colour groups.
 #include <algorithm>
 #include <cmath>
 
@@ -283,91 +284,182 @@ struct XEvalArgs {
 };
 
 // y += alpha * A u element by element (XEvaluatorNonlin, X/XSpatialOperatorMk2_XEvaluators.cs:467-646, for
-// the linear XLaplace forms): one wave per element, the same element list and rules as the assembly.
-// Per node, the lanes evaluate both sides' basis into LDS, every lane forms the traces of u (sums over the
-// modes in LDS), and lane m (P side) / Np + m (Q side) accumulates the test-function m contribution; one
-// plain add per (element, test function) at the end (colour groups, as in the assembly: deterministic).
-template <int D>
-__global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
-    const XArgs& a = e.g;
-    __shared__ double sv[2][XMAXNP], sd[2][XMAXNP], sg[XMAXNP][3], su[2][XMAXNP];
-    __shared__ double node[4];
-    const int lane = threadIdx.x;
-    const int Np = a.Np;
-    for (int ei = blockIdx.x; ei < a.count; ei += gridDim.x) {
-        const XElem E = a.elems[a.perm[a.start + ei]];
-        double loP[3], hP[3], loQ[3], hQ[3];
-        cell_box<D>(a, E.cellP, loP, hP);
-        cell_box<D>(a, E.cellQ, loQ, hQ);
-        const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
-        const int64_t oQ = e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np;
-        const bool twoSided = E.type == 1 || E.type == 3;
-        if (lane < Np) su[0][lane] = e.u[oP + lane];
-        else if (lane < 2 * Np) su[1][lane - Np] = twoSided ? e.u[oQ + lane - Np] : 0.0;
-        int nnodes;
-        if (E.rn > 0) nnodes = E.rn;
-        else if (E.type == 0) { nnodes = 1; for (int d = 0; d < D; ++d) nnodes *= a.nq1; }
-        else { nnodes = 1; for (int d = 0; d < D - 1; ++d) nnodes *= a.nq1; }
-        double acc = 0;
-        for (int qn = 0; qn < nnodes; ++qn) {
-            if (lane == 0) {
-                double x[3], w;
-                element_node<D>(a, E, qn, loP, hP, x, &w);
-                node[0] = x[0]; node[1] = x[1]; node[2] = x[2]; node[3] = w;
-            }
-            __syncthreads();
-            const double x[3] = {node[0], node[1], node[2]};
-            const double w = node[3];
-            double nrm[3];
-            const double fsgn = E.type == 1 ? 1.0 : (E.side ? 1.0 : -1.0);
+// the linear XLaplace forms), node-parallel: a wave is split into segments of S lanes, one element per
+// segment, one quadrature node per lane (the segment loops when the element has more nodes).  At its node a
+// lane evaluates the orthonormal basis of both sides (1D Legendre tables, products over the exponents),
+// forms the traces of u (coefficients in LDS), and accumulates w * (form of test function m) for every m in
+// registers; a butterfly over the segment's lanes sums the nodes (every lane ends with the same bits), and
+// one plain add per (element, test function) writes the result.  Colour groups as in the assembly: no two
+// elements of a launch touch the same entries, so the result is deterministic.  TYPE: 0 volume, 1 interior
+// face, 2 boundary face, 3 interface; all elements of a colour group have one type.
+template <int D, int K>
+struct Leg1 {
+    double P[D][K + 1], dP[D][K + 1];
+    double sc;   // 1 / sqrt(det J)
+    double g2[D];   // 2 / h_d
+    __device__ __forceinline__ void at(const double* lo, const double* h, const double* x) {
+        double detJ = 1;
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-                nrm[d] = E.type == 3 ? a.nn[(size_t)(E.r0 + qn) * 3 + d] : (E.type != 0 && d == E.dir ? fsgn : 0.0);
-            if (lane < 2 * Np) {
-                const int side = lane / Np, m = lane - side * Np;
-                double v, g[3];
-                basis_at<D>(a, side ? loQ : loP, side ? hQ : hP, x, m, &v, g);
-                double dn = 0;
-#pragma unroll
-                for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
-                sv[side][m] = v;
-                sd[side][m] = dn;
-                if (side == 0) for (int d = 0; d < 3; ++d) sg[m][d] = d < D ? g[d] : 0.0;
+        for (int d = 0; d < D; ++d) {
+            const double xi = 2.0 * (x[d] - lo[d]) / h[d] - 1.0;
+            detJ *= 0.5 * h[d];
+            g2[d] = 2.0 / h[d];
+            // orthonormal Legendre sqrt(n + 1/2) P_n and derivatives, n <= 3
+            P[d][0] = 0.7071067811865476;
+            dP[d][0] = 0.0;
+            if constexpr (K >= 1) { P[d][1] = 1.224744871391589 * xi; dP[d][1] = 1.224744871391589; }
+            if constexpr (K >= 2) { P[d][2] = 1.5811388300841898 * (1.5 * xi * xi - 0.5); dP[d][2] = 1.5811388300841898 * 3.0 * xi; }
+            if constexpr (K >= 3) {
+                P[d][3] = 1.8708286933869707 * (2.5 * xi * xi - 1.5) * xi;
+                dP[d][3] = 1.8708286933869707 * (7.5 * xi * xi - 1.5);
             }
-            __syncthreads();
-            // traces of u (every lane, LDS broadcasts)
-            double uP = 0, dP = 0, uQ = 0, dQ = 0, gu[3] = {0, 0, 0};
-            for (int n = 0; n < Np; ++n) {
-                const double cP = su[0][n], cQ = su[1][n];
-                uP += sv[0][n] * cP;
-                dP += sd[0][n] * cP;
-                uQ += sv[1][n] * cQ;
-                dQ += sd[1][n] * cQ;
-                if (E.type == 0) for (int d = 0; d < 3; ++d) gu[d] += sg[n][d] * cP;
-            }
-            if (lane < 2 * Np) {
-                const int side = lane / Np, m = lane - side * Np;
-                double val = 0;
-                if (E.type == 0) {
-                    if (side == 0) {
-                        double s = 0;
-                        for (int d = 0; d < D; ++d) s += sg[m][d] * gu[d];
-                        val = -E.nuP * s;
-                    }
-                } else if (E.type == 2) {
-                    if (side == 0) val = E.nuP * dP * sv[0][m] + E.nuP * sd[0][m] * uP - E.nuP * E.mu * uP * sv[0][m];
-                } else {
-                    const double jump = uP - uQ, flux = 0.5 * (E.nuP * dP + E.nuQ * dQ);
-                    if (side == 0) val = flux * sv[0][m] + 0.5 * E.nuP * sd[0][m] * jump - E.mu * jump * sv[0][m];
-                    else val = -flux * sv[1][m] + 0.5 * E.nuQ * sd[1][m] * jump + E.mu * jump * sv[1][m];
-                }
-                acc += w * val;
-            }
-            __syncthreads();
         }
-        if (lane < Np) e.y[oP + lane] += e.alpha * acc;
-        else if (lane < 2 * Np && twoSided) e.y[oQ + lane - Np] += e.alpha * acc;
+        sc = 1.0 / sqrt(detJ);
+    }
+    // value and gradient of mode n (exponents known at compile time: the tables stay in registers)
+    template <int n>
+    __device__ __forceinline__ void mode(double& v, double* g) const {
+        double pv[D], dv[D];
+        static_for<D>([&](auto DD) {
+            constexpr int d = DD;
+            constexpr int ed = mode_exponent(D, K, n, d);
+            pv[d] = P[d][ed];
+            dv[d] = dP[d][ed];
+        });
+        double prod = sc;
+#pragma unroll
+        for (int d = 0; d < D; ++d) prod *= pv[d];
+        v = prod;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double gg = sc * g2[d];
+#pragma unroll
+            for (int dd = 0; dd < D; ++dd) gg *= dd == d ? dv[dd] : pv[dd];
+            g[d] = gg;
+        }
+    }
+};
+
+template <int D, int K, int TYPE, int S>
+__global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
+    constexpr int Np = D == 2 ? (K + 1) * (K + 2) / 2 : (K + 1) * (K + 2) * (K + 3) / 6;
+    constexpr int NSEG = 64 / S;
+    constexpr bool TWO = TYPE == 1 || TYPE == 3;
+    constexpr int NACC = TWO ? 2 * Np : Np;
+    const XArgs& a = e.g;
+    __shared__ double su[NSEG][2][Np];
+    const int lane = threadIdx.x, sg = lane / S, j = lane - sg * S;
+    for (int base = blockIdx.x * NSEG; base < a.count; base += gridDim.x * NSEG) {
+        const int ei = base + sg;
+        const bool valid = ei < a.count;
+        XElem E{};
+        double loP[3] = {0, 0, 0}, hP[3] = {1, 1, 1}, loQ[3] = {0, 0, 0}, hQ[3] = {1, 1, 1};
+        int64_t oP = 0, oQ = 0;
+        int nnodes = 0;
+        if (valid) {
+            E = a.elems[a.perm[a.start + ei]];
+            cell_box<D>(a, E.cellP, loP, hP);
+            cell_box<D>(a, E.cellQ, loQ, hQ);
+            oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
+            oQ = e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np;
+            if (E.rn > 0) nnodes = E.rn;
+            else {
+                nnodes = 1;
+                for (int d = 0; d < (TYPE == 0 ? D : D - 1); ++d) nnodes *= a.nq1;
+            }
+            for (int t = j; t < 2 * Np; t += S)
+                su[sg][t / Np][t % Np] = t < Np ? e.u[oP + t] : (TWO ? e.u[oQ + t - Np] : 0.0);
+        }
         __syncthreads();
+        double acc[NACC];
+#pragma unroll
+        for (int m = 0; m < NACC; ++m) acc[m] = 0.0;
+        const double fsgn = TYPE == 1 ? 1.0 : (E.side ? 1.0 : -1.0);
+        for (int q = j; q < nnodes; q += S) {
+            double x[3], w;
+            element_node<D>(a, E, q, loP, hP, x, &w);
+            double nrm[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                nrm[d] = TYPE == 3 ? a.nn[(size_t)(E.r0 + q) * 3 + d] : (TYPE != 0 && d == E.dir ? fsgn : 0.0);
+            Leg1<D, K> LP, LQ;
+            LP.at(loP, hP, x);
+            if constexpr (TWO) LQ.at(loQ, hQ, x);
+            // traces of u
+            double uP = 0, dPn = 0, uQ = 0, dQn = 0, gu[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) gu[d] = 0;
+            static_for<Np>([&](auto NN) {
+                constexpr int n = NN;
+                double v, g[D];
+                LP.template mode<n>(v, g);
+                const double c = su[sg][0][n];
+                if constexpr (TYPE == 0) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) gu[d] += g[d] * c;
+                } else {
+                    double dn = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
+                    uP += v * c;
+                    dPn += dn * c;
+                }
+                if constexpr (TWO) {
+                    double vq, gq[D];
+                    LQ.template mode<n>(vq, gq);
+                    double dnq = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) dnq += gq[d] * nrm[d];
+                    const double cq = su[sg][1][n];
+                    uQ += vq * cq;
+                    dQn += dnq * cq;
+                }
+            });
+            // test functions
+            const double jump = uP - uQ, flux = 0.5 * (E.nuP * dPn + E.nuQ * dQn);
+            static_for<Np>([&](auto MM) {
+                constexpr int m = MM;
+                double v, g[D];
+                LP.template mode<m>(v, g);
+                if constexpr (TYPE == 0) {
+                    double sgu = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) sgu += g[d] * gu[d];
+                    acc[m] += w * (-E.nuP * sgu);
+                } else {
+                    double dn = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
+                    if constexpr (TYPE == 2)
+                        acc[m] += w * (E.nuP * dPn * v + E.nuP * dn * uP - E.nuP * E.mu * uP * v);
+                    else
+                        acc[m] += w * (flux * v + 0.5 * E.nuP * dn * jump - E.mu * jump * v);
+                }
+                if constexpr (TWO) {
+                    double vq, gq[D];
+                    LQ.template mode<m>(vq, gq);
+                    double dnq = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) dnq += gq[d] * nrm[d];
+                    acc[Np + m] += w * (-flux * vq + 0.5 * E.nuQ * dnq * jump + E.mu * jump * vq);
+                }
+            });
+        }
+        // sum over the segment's nodes (butterfly: every lane of the segment ends with the same sums)
+#pragma unroll
+        for (int m = 0; m < NACC; ++m) {
+#pragma unroll
+            for (int off = S / 2; off > 0; off >>= 1) acc[m] += __shfl_xor(acc[m], off, 64);
+        }
+        if (valid) {
+#pragma unroll
+            for (int m = 0; m < NACC; ++m)
+                if (j == m % S) {
+                    if (m < Np) e.y[oP + m] += e.alpha * acc[m];
+                    else e.y[oQ + m - Np] += e.alpha * acc[m];
+                }
+        }
+        __syncthreads();   // su is reused by the next element of the segment
     }
 }
 
@@ -867,6 +959,46 @@ void xdg_assemble(XdgDevice& X, Vbsr& M) {
     }
 }
 
+constexpr int pow_i(int b, int n) { return n == 0 ? 1 : b * pow_i(b, n - 1); }
+constexpr int seg_lanes(int nodes) { return nodes <= 8 ? 8 : nodes <= 16 ? 16 : nodes <= 32 ? 32 : 64; }
+
+template <int D, int K, int TYPE>
+static void xdg_eval_type(hipStream_t st, bool cut, const XEvalArgs& e) {
+    constexpr int S_T = seg_lanes(pow_i(K + 1, TYPE == 0 ? D : D - 1));
+    if (cut || TYPE == 3) {
+        const int blocks = e.g.count < 65536 ? e.g.count : 65536;
+        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, 64>), dim3(blocks), dim3(64), 0, st, e);
+    } else {
+        constexpr int NSEG = 64 / S_T;
+        const int nb = (e.g.count + NSEG - 1) / NSEG;
+        const int blocks = nb < 65536 ? nb : 65536;
+        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, S_T>), dim3(blocks), dim3(64), 0, st, e);
+    }
+    BH_HIP(hipGetLastError());
+}
+
+template <int D, int K>
+static void xdg_eval_dk(hipStream_t st, int type, bool cut, const XEvalArgs& e) {
+    switch (type) {
+        case 0: xdg_eval_type<D, K, 0>(st, cut, e); break;
+        case 1: xdg_eval_type<D, K, 1>(st, cut, e); break;
+        case 2: xdg_eval_type<D, K, 2>(st, cut, e); break;
+        default: xdg_eval_type<D, K, 3>(st, cut, e); break;
+    }
+}
+
+static void xdg_evaluate_launch(hipStream_t st, int D, int k, int type, bool cut, const XEvalArgs& e) {
+    if (D == 2) {
+        if (k == 1) xdg_eval_dk<2, 1>(st, type, cut, e);
+        else if (k == 2) xdg_eval_dk<2, 2>(st, type, cut, e);
+        else xdg_eval_dk<2, 3>(st, type, cut, e);
+    } else {
+        if (k == 1) xdg_eval_dk<3, 1>(st, type, cut, e);
+        else if (k == 2) xdg_eval_dk<3, 2>(st, type, cut, e);
+        else xdg_eval_dk<3, 3>(st, type, cut, e);
+    }
+}
+
 void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y) {
     XdgProblem& P = *X.P;
     const int64_t N = P.N_full;
@@ -887,16 +1019,18 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
     e.u = u;
     e.y = y;
     e.alpha = alpha;
-    BH_CHECK(2 * P.Np <= 64 && P.Np <= XMAXNP, "XDG evaluate: degree too high");
+    BH_CHECK(P.k >= 1 && P.k <= 3 && (P.D == 2 || P.D == 3), "XDG evaluate: degrees 1 to 3 in 2D or 3D");
     a.perm = X.perm.as<int>();
     for (int c = 0; c < XDG_COLOURS; ++c) {
-        a.start = X.colour_off[c];
-        a.count = X.colour_off[c + 1] - a.start;
-        if (a.count == 0) continue;
-        const int blocks = a.count < 65536 ? a.count : 65536;
-        if (P.D == 2) hipLaunchKernelGGL(xdg_evaluate_kernel<2>, dim3(blocks), dim3(64), 0, X.ctx->stream, e);
-        else hipLaunchKernelGGL(xdg_evaluate_kernel<3>, dim3(blocks), dim3(64), 0, X.ctx->stream, e);
-        BH_HIP(hipGetLastError());
+        // the colour's cut items (own rules, many nodes) with whole waves, then its uncut items
+        // (tensor rules) with segments of the next power of two above the tensor node count
+        for (int part = 0; part < 2; ++part) {
+            a.start = part == 0 ? X.colour_off[c] : X.cut_end[c];
+            a.count = (part == 0 ? X.cut_end[c] : X.colour_off[c + 1]) - a.start;
+            if (a.count == 0) continue;
+            const int type = c == 0 ? 0 : c == 1 ? 3 : c < 8 ? 2 : 1;
+            xdg_evaluate_launch(X.ctx->stream, P.D, P.k, type, part == 0, e);
+        }
     }
 }
 

@@ -56,6 +56,8 @@ void XdgDevice::upload() {
     }
     colour_off.assign(XDG_COLOURS + 1, 0);
     for (int c = 0; c <= XDG_COLOURS; ++c) colour_off[c] = cnt[2 * c];
+    cut_end.assign(XDG_COLOURS, 0);
+    for (int c = 0; c < XDG_COLOURS; ++c) cut_end[c] = cnt[2 * c + 1];
     perm.alloc(pm.size() * sizeof(int));
     ctx->upload(perm.p, pm.data(), pm.size() * sizeof(int));
     auto up = [&](DevBuf& b, const std::vector<double>& v) {

@@ -52,6 +52,7 @@ struct XdgDevice {
     DevBuf nodes1, elems, nx, nw, nn, ex, gx, gw, spmask, offf;
     DevBuf perm;                            // element indices grouped by colour (cut items first in each)
     std::vector<int> colour_off;            // [XDG_COLOURS + 1] offsets into perm
+    std::vector<int> cut_end;               // [XDG_COLOURS] end of each colour's cut items
     void upload();
 };
 
