@@ -54,21 +54,23 @@ __global__ __launch_bounds__(256) void bsr_spmv_kernel(int nrows, const int64_t*
     }
 }
 
-// The same stream with 16-byte value loads, for block sizes whose columns are whole
-// 16-byte units (bs * sizeof(T) a multiple of 16): a lane reads VW = 16 / sizeof(T)
-// consecutive rows of one column, L = bs / VW lanes cover a column, and the G = 64 / L
-// groups read G consecutive columns per step (bs = 20 FP64: 10 lanes per column, 6
-// columns = 960 contiguous bytes per load instruction).  Each lane keeps VW partial
-// sums; the groups' partials are combined in LDS.
-template <typename T, int BS>
+// The same stream with VB-byte value loads (16, or 8 where a column is not whole
+// 16-byte units, e.g. bs = 10 in FP32), for block sizes whose columns are whole VB-byte
+// units: a lane reads VW = VB / sizeof(T) consecutive rows of one column, L = bs / VW
+// lanes cover a column, and the G = 64 / L groups read G consecutive columns per step
+// (bs = 20 FP64: 10 lanes per column, 6 columns = 960 contiguous bytes per load
+// instruction).  Each lane keeps VW partial sums; the groups' partials are combined in LDS.
+template <typename T, int BS, int VB = 16>
 __global__ __launch_bounds__(256) void bsr_spmv_vec_kernel(int nrows, const int64_t* __restrict__ rowptr,
                                                            const int* __restrict__ col, const T* __restrict__ vals,
                                                            const T* __restrict__ x, T* __restrict__ y, T alpha, T beta,
                                                            const T* __restrict__ b_sub) {
-    constexpr int VW = 16 / sizeof(T);
-    static_assert(BS % VW == 0, "columns must be whole 16-byte units");
+    constexpr int VW = VB / sizeof(T);
+    static_assert(BS % VW == 0, "columns must be whole VB-byte units");
     constexpr int L = BS / VW, G = 64 / L;
-    using V = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    using V16 = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+    using V8 = typename std::conditional<sizeof(T) == 8, double, float2>::type;
+    using V = typename std::conditional<VB == 16, V16, V8>::type;
     __shared__ T red[4][VW][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + wave;
@@ -173,7 +175,9 @@ static void launch_spmv_bs(dim3 grd, hipStream_t st, int nrows, const int64_t* r
         const dim3 g2((unsigned)((nrows + 4 * RPW - 1) / (4 * RPW)));
         hipLaunchKernelGGL((bsr_spmv_small_kernel<T, BS>), g2, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
     } else if constexpr ((BS * sizeof(T)) % 16 == 0 && BS / (16 / sizeof(T)) >= 5)
-        hipLaunchKernelGGL((bsr_spmv_vec_kernel<T, BS>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
+        hipLaunchKernelGGL((bsr_spmv_vec_kernel<T, BS, 16>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
+    else if constexpr (sizeof(T) == 4 && (BS * sizeof(T)) % 8 == 0 && BS / 2 >= 5)
+        hipLaunchKernelGGL((bsr_spmv_vec_kernel<T, BS, 8>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
     else
         hipLaunchKernelGGL((bsr_spmv_kernel<T, BS>), grd, dim3(256), 0, st, nrows, rp, cl, vals, x, y, alpha, beta, bsub);
 }
