@@ -51,6 +51,15 @@ void parallel_for(int n, const std::function<void(int)>& f);
 void sphere_interface_rule(int D, const double* lo, const double* hi, const Sphere& S, int q, QRule& out);
 // face in the plane x_dir = lo[dir] (lo[dir] == hi[dir])
 void sphere_face_rule(int D, const double* lo, const double* hi, int dir, const Sphere& S, int species, int q, QRule& out);
+// The same rules on the device, one thread per item (bh_xdgrule.hip); out[i] is the
+// rule of items[i].  kind: 0 / 1 volume of species A / B, 2 interface, 3 / 4 face
+// of species A / B (face in the plane x_dir = lo[dir])
+struct SphereRuleItem {
+    double lo[3], hi[3];
+    int kind, dir;
+};
+void sphere_rules_device(bh_ctx* ctx, int D, int q, const Sphere& S, const std::vector<SphereRuleItem>& items,
+                         std::vector<QRule>& out);
 
 // element of the XDG bilinear form (one wave of the assembly kernel each)
 struct XElem {

@@ -23,280 +23,29 @@
 #include <stdexcept>
 #include <thread>
 
+#include "bh_sphrule.hpp"
 #include "bh_xdg.hpp"
 
 namespace bh {
 
 namespace {
 
+// rules through the shared implementation (bh_sphrule.hpp): the emitter appends to a QRule
+struct QRuleEmit {
+    QRule& R;
+    void operator()(const double* x, double w, const double* nrm) {
+        R.x.insert(R.x.end(), x, x + 3);
+        R.w.push_back(w);
+        if (nrm) R.nrm.insert(R.nrm.end(), nrm, nrm + 3);
+    }
+};
+
 struct Gauss1 {
     std::vector<double> x, w;
     explicit Gauss1(int q) : x(q), w(q) { gauss_rule(q, x.data(), w.data()); }
 };
 
-// split [a,b] at the breakpoints in (a,b)
-std::vector<double> pieces(double a, double b, std::vector<double> bp) {
-    std::vector<double> p{a};
-    std::sort(bp.begin(), bp.end());
-    const double eps = 1e-14 * std::max(1.0, std::fabs(b - a));
-    for (double v : bp)
-        if (v > a + eps && v < b - eps && v > p.back() + eps) p.push_back(v);
-    p.push_back(b);
-    return p;
-}
-
-// roots c0 +- sqrt(R2) of (t - c0)^2 = R2
-void add_roots(std::vector<double>& v, double c0, double R2) {
-    if (R2 > 0) { const double s = std::sqrt(R2); v.push_back(c0 - s); v.push_back(c0 + s); }
-}
-
-// Outer-direction nodes of a piece [a, b] whose integrand may carry a
-// square-root singularity at an anchor point e outside the piece or at its end
-// (the extreme x_o = co +- R of a circle, where two breakpoints of the inner
-// direction meet).  If the nearest anchor lies within one piece length of the
-// piece, the piece is integrated in t with x_o = e + (f - e) t^2 (f the far end),
-// which turns sqrt(x_o - e) into a linear function of t; otherwise plain Gauss.
-void outer_nodes(double a, double b, const std::vector<double>& anchors, const Gauss1& G, std::vector<double>& X,
-                 std::vector<double>& W) {
-    const int q = (int)G.x.size();
-    const double L = b - a;
-    const double tol = 1e-12 * std::max(std::fabs(L), 1e-300);
-    double el = -1e300, er = 1e300;
-    for (double e : anchors) {
-        if (e <= a + tol) el = std::max(el, e);
-        if (e >= b - tol) er = std::min(er, e);
-    }
-    const bool use_l = a - el <= L, use_r = er - b <= L;
-    auto plain = [&](double x0, double x1) {
-        for (int i = 0; i < q; ++i) { X.push_back(x0 + 0.5 * (G.x[i] + 1) * (x1 - x0)); W.push_back(0.5 * (x1 - x0) * G.w[i]); }
-    };
-    // x = e + (f - e) t^2 for t in [t0, 1], covering [x_near, f]
-    auto graded = [&](double e, double xn, double f) {
-        const double t0 = std::sqrt(std::max(0.0, (xn - e) / (f - e)));
-        for (int i = 0; i < q; ++i) {
-            const double t = t0 + 0.5 * (G.x[i] + 1) * (1.0 - t0);
-            X.push_back(e + (f - e) * t * t);
-            W.push_back(std::fabs(f - e) * 2.0 * t * 0.5 * (1.0 - t0) * G.w[i]);
-        }
-    };
-    if (use_l && use_r) {
-        const double m = 0.5 * (a + b);
-        graded(el, a, m);
-        graded(er, b, m);
-    } else if (use_l) {
-        graded(el, a, b);
-    } else if (use_r) {
-        graded(er, b, a);
-    } else {
-        plain(a, b);
-    }
-}
-
-// Generic 2D integration over [lo_o,hi_o] x [lo_k2,hi_k2] of a function whose
-// smoothness breaks along circles |x' - c'|^2 = R2_b (b over `radii`): outer
-// breakpoints where a circle crosses x_k2 = lo/hi and at the circle's extremes
-// x_o = co +- R (also the anchors of outer_nodes), inner breakpoints at the
-// circle crossings.
-void circle_2d(double lo_o, double hi_o, double lo_k, double hi_k, double co, double ck, const std::vector<double>& radii,
-               const Gauss1& G, const std::function<void(double, double, double)>& visit) {
-    std::vector<double> bo, anchors;
-    for (double R2 : radii) {
-        if (R2 <= 0) continue;
-        add_roots(bo, co, R2 - (lo_k - ck) * (lo_k - ck));
-        add_roots(bo, co, R2 - (hi_k - ck) * (hi_k - ck));
-        add_roots(bo, co, R2);
-        add_roots(anchors, co, R2);
-    }
-    const auto po = pieces(lo_o, hi_o, bo);
-    const int q = (int)G.x.size();
-    std::vector<double> X, W;
-    for (size_t a = 0; a + 1 < po.size(); ++a) {
-        X.clear();
-        W.clear();
-        outer_nodes(po[a], po[a + 1], anchors, G, X, W);
-        for (size_t i = 0; i < X.size(); ++i) {
-            const double xo = X[i], wo = W[i];
-            std::vector<double> bk;
-            for (double R2 : radii) add_roots(bk, ck, R2 - (xo - co) * (xo - co));
-            const auto pk = pieces(lo_k, hi_k, bk);
-            for (size_t b = 0; b + 1 < pk.size(); ++b) {
-                const double ka = pk[b], kb = pk[b + 1];
-                for (int j = 0; j < q; ++j) visit(xo, ka + 0.5 * (G.x[j] + 1) * (kb - ka), wo * 0.5 * (kb - ka) * G.w[j]);
-            }
-        }
-    }
-}
-
-int argmax_abs(const double* v, const int* dims, int n) {
-    int best = dims[0];
-    for (int i = 1; i < n; ++i)
-        if (std::fabs(v[dims[i]]) > std::fabs(v[best])) best = dims[i];
-    return best;
-}
-
-// Volume (mode 0) or interface (mode 1) rule in a 3D/2D box, recursive.
-void sphere_rule_rec(int D, const double* lo, const double* hi, const Sphere& S, int species, int mode, const Gauss1& G,
-                     int depth, QRule& out) {
-    const double r2 = S.r * S.r;
-    {
-        // a box (or sub-box) the sphere does not cut: tensor Gauss rule or nothing
-        double dmin2 = 0, dmax2 = 0;
-        for (int d = 0; d < D; ++d) {
-            const double a = lo[d] - S.c[d], b = hi[d] - S.c[d];
-            dmin2 += (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
-            dmax2 += std::max(a * a, b * b);
-        }
-        const bool inside = dmax2 <= r2, outside = dmin2 >= r2;
-        if (inside || outside) {
-            if (mode == 1 || inside != (species < 0)) return;
-            const int q = (int)G.x.size();
-            int n = 1;
-            for (int d = 0; d < D; ++d) n *= q;
-            for (int t = 0; t < n; ++t) {
-                double x[3] = {0, 0, 0}, w = 1;
-                int r = t;
-                for (int d = 0; d < D; ++d) {
-                    const int id = r % q;
-                    r /= q;
-                    x[d] = lo[d] + 0.5 * (G.x[id] + 1) * (hi[d] - lo[d]);
-                    w *= 0.5 * (hi[d] - lo[d]) * G.w[id];
-                }
-                out.x.insert(out.x.end(), x, x + 3);
-                out.w.push_back(w);
-            }
-            return;
-        }
-    }
-    double xc[3], rad[3];
-    for (int d = 0; d < D; ++d) { xc[d] = 0.5 * (lo[d] + hi[d]); rad[d] = xc[d] - S.c[d]; }
-    int all[3] = {0, 1, 2};
-    const int k = argmax_abs(rad, all, D);
-    int od[2], no = 0;
-    for (int d = 0; d < D; ++d) if (d != k) od[no++] = d;
-    // silhouette check: r^2 - |x' - c'|^2 must not change sign over the face box
-    double mn = 0, mx = 0;
-    for (int i = 0; i < no; ++i) {
-        const int d = od[i];
-        const double a = lo[d] - S.c[d], b = hi[d] - S.c[d];
-        const double m1 = (a <= 0 && b >= 0) ? 0.0 : std::min(a * a, b * b);
-        mn += m1;
-        mx += std::max(a * a, b * b);
-    }
-    const bool ok_sil = (r2 - mx > 0) || (r2 - mn <= 0);
-    auto subdivide = [&]() {
-        const int nc = 1 << D;
-        for (int cidx = 0; cidx < nc; ++cidx) {
-            double l2[3], h2[3];
-            for (int d = 0; d < D; ++d) {
-                const bool up = (cidx >> d) & 1;
-                l2[d] = up ? xc[d] : lo[d];
-                h2[d] = up ? hi[d] : xc[d];
-            }
-            sphere_rule_rec(D, l2, h2, S, species, mode, G, depth + 1, out);
-        }
-    };
-    if (!ok_sil) {
-        if (depth < 6) { subdivide(); return; }
-    }
-    // emit one node at x (full coordinates)
-    auto emit_line = [&](const double* xp) {
-        // along k: roots t = c_k +- sqrt(r^2 - |x'-c'|^2)
-        double s2 = r2;
-        for (int i = 0; i < no; ++i) s2 -= (xp[od[i]] - S.c[od[i]]) * (xp[od[i]] - S.c[od[i]]);
-        return s2;
-    };
-    if (D == 3) {
-        const int d0 = od[0], d1 = od[1];
-        // inner 2D direction: larger radial component
-        const int k2 = std::fabs(rad[d0]) >= std::fabs(rad[d1]) ? d0 : d1;
-        const int o = k2 == d0 ? d1 : d0;
-        std::vector<double> radii;
-        radii.push_back(r2 - (lo[k] - S.c[k]) * (lo[k] - S.c[k]));
-        radii.push_back(r2 - (hi[k] - S.c[k]) * (hi[k] - S.c[k]));
-        circle_2d(lo[o], hi[o], lo[k2], hi[k2], S.c[o], S.c[k2], radii, G, [&](double xo, double xk2, double w2) {
-            double xp[3];
-            xp[o] = xo;
-            xp[k2] = xk2;
-            xp[k] = 0;
-            const double s2 = emit_line(xp);
-            if (mode == 0) {
-                std::vector<double> bt;
-                add_roots(bt, S.c[k], s2);
-                const auto pt = pieces(lo[k], hi[k], bt);
-                for (size_t b = 0; b + 1 < pt.size(); ++b) {
-                    const double ta = pt[b], tb = pt[b + 1];
-                    double xm[3] = {xp[0], xp[1], xp[2]};
-                    xm[k] = 0.5 * (ta + tb);
-                    if ((S.phi(xm, 3) < 0) != (species < 0)) continue;
-                    const int q = (int)G.x.size();
-                    for (int j = 0; j < q; ++j) {
-                        double x[3] = {xp[0], xp[1], xp[2]};
-                        x[k] = ta + 0.5 * (G.x[j] + 1) * (tb - ta);
-                        out.x.insert(out.x.end(), x, x + 3);
-                        out.w.push_back(w2 * 0.5 * (tb - ta) * G.w[j]);
-                    }
-                }
-            } else if (s2 > 0) {
-                const double s = std::sqrt(s2);
-                for (double t : {S.c[k] - s, S.c[k] + s}) {
-                    if (t <= lo[k] || t >= hi[k]) continue;
-                    double x[3] = {xp[0], xp[1], xp[2]};
-                    x[k] = t;
-                    out.x.insert(out.x.end(), x, x + 3);
-                    out.w.push_back(w2 * S.r / std::fabs(t - S.c[k]));   // dS = |grad phi| / |d phi/dx_k| dx'
-                    for (int d = 0; d < 3; ++d) out.nrm.push_back((x[d] - S.c[d]) / S.r);
-                }
-            }
-        });
-    } else {
-        // D == 2: one outer direction o, height direction k
-        const int o = od[0];
-        const int q = (int)G.x.size();
-        std::vector<double> bo;
-        add_roots(bo, S.c[o], r2 - (lo[k] - S.c[k]) * (lo[k] - S.c[k]));
-        add_roots(bo, S.c[o], r2 - (hi[k] - S.c[k]) * (hi[k] - S.c[k]));
-        const auto po = pieces(lo[o], hi[o], bo);
-        std::vector<double> anchors, XO, WO;
-        add_roots(anchors, S.c[o], r2);   // silhouette: the height roots meet there
-        for (size_t a = 0; a + 1 < po.size(); ++a) {
-            XO.clear();
-            WO.clear();
-            outer_nodes(po[a], po[a + 1], anchors, G, XO, WO);
-            for (size_t i = 0; i < XO.size(); ++i) {
-                const double xo = XO[i], wo = WO[i];
-                double xp[3] = {0, 0, 0};
-                xp[o] = xo;
-                const double s2 = r2 - (xo - S.c[o]) * (xo - S.c[o]);
-                if (mode == 0) {
-                    std::vector<double> bt;
-                    add_roots(bt, S.c[k], s2);
-                    const auto pt = pieces(lo[k], hi[k], bt);
-                    for (size_t b = 0; b + 1 < pt.size(); ++b) {
-                        double xm[3] = {xp[0], xp[1], 0};
-                        xm[k] = 0.5 * (pt[b] + pt[b + 1]);
-                        if ((S.phi(xm, 2) < 0) != (species < 0)) continue;
-                        for (int j = 0; j < q; ++j) {
-                            double x[3] = {xp[0], xp[1], 0};
-                            x[k] = pt[b] + 0.5 * (G.x[j] + 1) * (pt[b + 1] - pt[b]);
-                            out.x.insert(out.x.end(), x, x + 3);
-                            out.w.push_back(wo * 0.5 * (pt[b + 1] - pt[b]) * G.w[j]);
-                        }
-                    }
-                } else if (s2 > 0) {
-                    const double s = std::sqrt(s2);
-                    for (double t : {S.c[k] - s, S.c[k] + s}) {
-                        if (t <= lo[k] || t >= hi[k]) continue;
-                        double x[3] = {xp[0], xp[1], 0};
-                        x[k] = t;
-                        out.x.insert(out.x.end(), x, x + 3);
-                        out.w.push_back(wo * S.r / std::fabs(t - S.c[k]));
-                        for (int d = 0; d < 3; ++d) out.nrm.push_back(d < 2 ? (x[d] - S.c[d]) / S.r : 0.0);
-                    }
-                }
-            }
-        }
-    }
-}
+sph::Ball ball_of(const Sphere& S) { return sph::Ball{{S.c[0], S.c[1], S.c[2]}, S.r}; }
 
 }  // namespace
 
@@ -329,58 +78,29 @@ void parallel_for(int n, const std::function<void(int)>& f) {
 }
 
 void sphere_volume_rule(int D, const double* lo, const double* hi, const Sphere& S, int species, int q, QRule& out) {
+    BH_CHECK(q <= sph::QMAX, "XDG cut rules: at most 32 Gauss points per direction");
     Gauss1 G(q);
-    sphere_rule_rec(D, lo, hi, S, species, 0, G, 0, out);
+    QRuleEmit e{out};
+    BH_CHECK(sph::volume_or_interface(D, lo, hi, ball_of(S), species, 0, G.x.data(), G.w.data(), q, e),
+             "XDG cut rules: breakpoint or subdivision capacity exceeded");
 }
 
 void sphere_interface_rule(int D, const double* lo, const double* hi, const Sphere& S, int q, QRule& out) {
+    BH_CHECK(q <= sph::QMAX, "XDG cut rules: at most 32 Gauss points per direction");
     Gauss1 G(q);
-    sphere_rule_rec(D, lo, hi, S, 0, 1, G, 0, out);
+    QRuleEmit e{out};
+    BH_CHECK(sph::volume_or_interface(D, lo, hi, ball_of(S), 0, 1, G.x.data(), G.w.data(), q, e),
+             "XDG cut rules: breakpoint or subdivision capacity exceeded");
 }
 
 // Face rule: the face is a (D-1)-box in the plane x_dir = lo[dir]; the level
 // set restricted to it is a circle (3D) / two points (2D).
 void sphere_face_rule(int D, const double* lo, const double* hi, int dir, const Sphere& S, int species, int q, QRule& out) {
+    BH_CHECK(q <= sph::QMAX, "XDG cut rules: at most 32 Gauss points per direction");
     Gauss1 G(q);
-    const double xd = lo[dir];
-    const double R2 = S.r * S.r - (xd - S.c[dir]) * (xd - S.c[dir]);
-    if (D == 2) {
-        const int o = 1 - dir;
-        std::vector<double> bp;
-        add_roots(bp, S.c[o], R2);
-        const auto p = pieces(lo[o], hi[o], bp);
-        for (size_t a = 0; a + 1 < p.size(); ++a) {
-            double xm[3] = {0, 0, 0};
-            xm[dir] = xd;
-            xm[o] = 0.5 * (p[a] + p[a + 1]);
-            if ((S.phi(xm, 2) < 0) != (species < 0)) continue;
-            for (int i = 0; i < q; ++i) {
-                double x[3] = {0, 0, 0};
-                x[dir] = xd;
-                x[o] = p[a] + 0.5 * (G.x[i] + 1) * (p[a + 1] - p[a]);
-                out.x.insert(out.x.end(), x, x + 3);
-                out.w.push_back(0.5 * (p[a + 1] - p[a]) * G.w[i]);
-            }
-        }
-        return;
-    }
-    int od[2], no = 0;
-    for (int d = 0; d < 3; ++d) if (d != dir) od[no++] = d;
-    const double rad0 = 0.5 * (lo[od[0]] + hi[od[0]]) - S.c[od[0]];
-    const double rad1 = 0.5 * (lo[od[1]] + hi[od[1]]) - S.c[od[1]];
-    const int k2 = std::fabs(rad0) >= std::fabs(rad1) ? od[0] : od[1];
-    const int o = k2 == od[0] ? od[1] : od[0];
-    const std::vector<double> radii{R2};
-    // pieces never straddle the circle: keep the nodes of the requested species
-    circle_2d(lo[o], hi[o], lo[k2], hi[k2], S.c[o], S.c[k2], radii, G, [&](double xo, double xk, double w) {
-        double x[3];
-        x[dir] = xd;
-        x[o] = xo;
-        x[k2] = xk;
-        if ((S.phi(x, 3) < 0) != (species < 0)) return;
-        out.x.insert(out.x.end(), x, x + 3);
-        out.w.push_back(w);
-    });
+    QRuleEmit e{out};
+    BH_CHECK(sph::face(D, lo, hi, dir, ball_of(S), species, G.x.data(), G.w.data(), q, e),
+             "XDG cut rules: breakpoint capacity exceeded");
 }
 
 // ------------------------------------------------------------------ setup
@@ -451,13 +171,26 @@ void XdgProblem::setup() {
         else { cand_of[c] = (int)cand.size(); cand.push_back(c); }
     }
     std::vector<QRule> crule(3 * cand.size());
-    parallel_for((int)cand.size(), [&](int i) {
-        double lo[3], hi[3];
-        box(cand[i], lo, hi);
-        vrule(lo, hi, -1, crule[3 * i]);
-        vrule(lo, hi, +1, crule[3 * i + 1]);
-        irule(lo, hi, crule[3 * i + 2]);
-    });
+    const bool dev_rules = ctx != nullptr && ls_kind == 0;   // sphere rules on the device
+    if (dev_rules) {
+        std::vector<SphereRuleItem> items(3 * cand.size());
+        for (size_t i = 0; i < cand.size(); ++i)
+            for (int t = 0; t < 3; ++t) {
+                SphereRuleItem& it = items[3 * i + t];
+                box(cand[i], it.lo, it.hi);
+                it.kind = t;
+                it.dir = 0;
+            }
+        sphere_rules_device(ctx, D, q, S, items, crule);
+    } else {
+        parallel_for((int)cand.size(), [&](int i) {
+            double lo[3], hi[3];
+            box(cand[i], lo, hi);
+            vrule(lo, hi, -1, crule[3 * i]);
+            vrule(lo, hi, +1, crule[3 * i + 1]);
+            irule(lo, hi, crule[3 * i + 2]);
+        });
+    }
     // pass 2, in cell order: species volumes, masks and rule slots
     for (int c = 0; c < J; ++c) {
         double lo[3], hi[3];
@@ -531,11 +264,23 @@ void XdgProblem::setup() {
     }
     // species rules of the cut faces, in parallel, then areas and rule slots in face order
     std::vector<QRule> frule(2 * cut_faces.size());
-    parallel_for((int)cut_faces.size(), [&](int i) {
-        const CutFace& cf = cut_faces[i];
-        for (int sp = 0; sp < 2; ++sp)
-            frule_of(cf.lo, cf.hi, faces[cf.face].dir, sp == 0 ? -1 : +1, frule[2 * i + sp]);
-    });
+    if (dev_rules) {
+        std::vector<SphereRuleItem> items(2 * cut_faces.size());
+        for (size_t i = 0; i < cut_faces.size(); ++i)
+            for (int sp = 0; sp < 2; ++sp) {
+                SphereRuleItem& it = items[2 * i + sp];
+                for (int e = 0; e < 3; ++e) { it.lo[e] = cut_faces[i].lo[e]; it.hi[e] = cut_faces[i].hi[e]; }
+                it.kind = 3 + sp;
+                it.dir = faces[cut_faces[i].face].dir;
+            }
+        sphere_rules_device(ctx, D, q, S, items, frule);
+    } else {
+        parallel_for((int)cut_faces.size(), [&](int i) {
+            const CutFace& cf = cut_faces[i];
+            for (int sp = 0; sp < 2; ++sp)
+                frule_of(cf.lo, cf.hi, faces[cf.face].dir, sp == 0 ? -1 : +1, frule[2 * i + sp]);
+        });
+    }
     for (size_t i = 0; i < cut_faces.size(); ++i) {
         FaceInfo& F = faces[cut_faces[i].face];
         for (int sp = 0; sp < 2; ++sp) {

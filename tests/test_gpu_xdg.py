@@ -310,3 +310,28 @@ def test_schwarz_overlap_in_coarse_hierarchy(ctx):
     A = X.scipy_matrix(1)
     assert rel(A @ u1, b) <= 1.5e-8
     assert rel(u1, u0) <= 1e-6
+
+
+@pytest.mark.parametrize("D,n,q", [(3, 12, 6), (3, 10, 10), (2, 16, 8)])
+def test_device_cut_rules_equal_host(ctx, D, n, q):
+    """Sphere cut-cell, interface and cut-face rules built on the device (one thread
+    per rule) equal the host set-up's bit for bit (the same construction,
+    bh_sphrule.hpp, compiled without multiply-add contraction; IEEE add, multiply,
+    divide and square root round alike): the same element lists, nodes, weights,
+    normals, species masks and agglomeration.  n = 10 puts the sphere's centre
+    coordinate 0.5 on a grid plane, where roots meet piece ends."""
+    _say(f"case device cut rules D={D} n={n} q={q}")
+    c = [0.5, 0.47, 0.52][:D]
+    r = 0.27 if D == 3 else 0.31
+    x = np.linspace(0, 1, n + 1)
+    Xd = B.XdgPoisson(B.Grid(ctx, *([x] * D)), 2, c, r, NU[0], NU[1], agg_threshold=0.1, quad_order=q)
+    Xh = B.XdgPoisson(B.Grid(None, *([x] * D)), 2, c, r, NU[0], NU[1], agg_threshold=0.1, quad_order=q)
+    ed, eh = Xd.elements(), Xh.elements()
+    assert np.array_equal(ed["ints"], eh["ints"])
+    assert len(ed["nw"]) == len(eh["nw"]) > 0
+    for key in ("nx", "nw", "nn", "dbls"):
+        assert np.array_equal(ed[key], eh[key]), key
+    cd, ch = Xd.cells(), Xh.cells()
+    assert np.array_equal(cd["spmask"], ch["spmask"])
+    assert np.array_equal(cd["vol"], ch["vol"])
+    assert np.array_equal(Xd.agglomeration(), Xh.agglomeration())
