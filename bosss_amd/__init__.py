@@ -99,6 +99,8 @@ SIGNATURES = {
     "bh_pmg_set_graphs": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_schwarz": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_level_pmg": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_set_cell_ilu": (_c_i, [_c_p, _c_i]),
+    "bh_pmg_smooth": (_c_i, [_c_p, _c_i, _c_p, _c_p]),
     "bh_pmg_set_schwarz_overlap": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_pmg_set_diag_equilib": (_c_i, [_c_p]),
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
@@ -713,6 +715,20 @@ def pmg_learned_info(pmg):
     a, g = _c_i(), _c_d()
     _check(lib().bh_pmg_learned_info(pmg.h, ctypes.byref(a), ctypes.byref(g)))
     return bool(a.value), g.value
+
+
+def pmg_set_cell_ilu(pmg, ilu_level=1):
+    """CellILU (block ILU on the pattern of A^(ilu_level+1)) as the finest level's smoother;
+    ilu_level < 0: block-Jacobi again."""
+    _check(lib().bh_pmg_set_cell_ilu(pmg.h, int(ilu_level)))
+
+
+def pmg_smooth(pmg, level, r):
+    """x = S(r): the smoother of `level` from a zero guess (host FP64 vectors)."""
+    r = np.ascontiguousarray(r, np.float64)
+    x = np.zeros_like(r)
+    _check(lib().bh_pmg_smooth(pmg.h, int(level), r.ctypes.data, x.ctypes.data))
+    return x
 
 
 def pmg_set_level_pmg(pmg, enable=True):

@@ -1100,6 +1100,37 @@ int bh_xdg_precond_set_fp64_blocks(bh_solver* s, int on) {
     BH_API_END
 }
 
+int bh_pmg_set_cell_ilu(bh_solver* s, int ilu_level) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_set_cell_ilu: solver is not a multigrid preconditioner");
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    mg->set_cell_ilu(ilu_level);
+    BH_API_END
+}
+
+int bh_pmg_smooth(bh_solver* s, int level, const double* r, double* x) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_smooth: solver is not a multigrid preconditioner");
+    BH_CHECK(level >= 0 && level + 1 < (int)mg->lv.size(), "bh_pmg_smooth: level out of range");
+    MgLevel& L = mg->lv[level];
+    bh_ctx* ctx = mg->ctx;
+    DevBuf d(L.N * sizeof(double));
+    ctx->upload(d.p, r, L.N * sizeof(double));
+    if (mg->prec == BH_FP32) {
+        convert_f64_f32(ctx, d.as<double>(), (float*)L.b.p, L.N);
+        mg->smooth<float>(level, (const float*)L.b.p, (float*)L.x.p, false);
+        convert_f32_f64(ctx, (const float*)L.x.p, d.as<double>(), L.N);
+    } else {
+        copy(ctx, (double*)L.b.p, d.as<double>(), L.N);
+        mg->smooth<double>(level, (const double*)L.b.p, (double*)L.x.p, false);
+        copy(ctx, d.as<double>(), (const double*)L.x.p, L.N);
+    }
+    ctx->download(x, d.p, L.N * sizeof(double));
+    BH_API_END
+}
+
 int bh_pmg_set_level_pmg(bh_solver* s, int enable) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);

@@ -134,6 +134,79 @@ __host__ __device__ constexpr int mode_exponent(int D, int k, int n, int d) {
 }
 }  // namespace bh
 
+// ------------------------------------------------------------------ device helpers
+#if defined(__HIPCC__)
+// In-place Gauss-Jordan inverse with partial pivoting of an n x n matrix M in LDS
+// (n <= 128), by one workgroup of up to 256 threads: per column a pivot search (ties
+// to the lowest row), a row swap, the pivot row scaled and every other row
+// eliminated; the recorded row swaps are undone as column swaps at the end.  F: n
+// doubles of LDS.  Returns whether a pivot was zero (the matrix is then garbage).
+// The storage order does not matter: the inverse of the transpose is the transpose
+// of the inverse.
+struct GjScratch {
+    int piv[128];
+    double bv[256];
+    int bi[256];
+};
+__device__ inline bool gj_invert_inplace(double* M, double* F, int n, GjScratch& s) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    bool singular = false;
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bix = k;
+        for (int r = k + t; r < n; r += nt) {
+            const double v = fabs(M[(size_t)r * n + k]);
+            if (v > best) { best = v; bix = r; }
+        }
+        s.bv[t] = best;
+        s.bi[t] = bix;
+        __syncthreads();
+        for (int s2 = nt / 2; s2 > 0; s2 >>= 1) {
+            if (t < s2 && (s.bv[t + s2] > s.bv[t] || (s.bv[t + s2] == s.bv[t] && s.bi[t + s2] < s.bi[t]))) {
+                s.bv[t] = s.bv[t + s2];
+                s.bi[t] = s.bi[t + s2];
+            }
+            __syncthreads();
+        }
+        const int p = s.bi[0];
+        if (!(s.bv[0] > 0)) singular = true;
+        if (t == 0) s.piv[k] = p;
+        if (p != k)
+            for (int c = t; c < n; c += nt) {
+                const double tmp = M[(size_t)k * n + c];
+                M[(size_t)k * n + c] = M[(size_t)p * n + c];
+                M[(size_t)p * n + c] = tmp;
+            }
+        __syncthreads();
+        const double d = singular ? 0.0 : 1.0 / M[(size_t)k * n + k];
+        for (int r = t; r < n; r += nt) F[r] = M[(size_t)r * n + k];
+        __syncthreads();
+        if (t == 0) M[(size_t)k * n + k] = 1.0;
+        __syncthreads();
+        for (int c = t; c < n; c += nt) M[(size_t)k * n + c] *= d;
+        for (int r = t; r < n; r += nt)
+            if (r != k) M[(size_t)r * n + k] = 0.0;
+        __syncthreads();
+        for (int e = t; e < n * n; e += nt) {
+            const int r = e / n, c = e - r * n;
+            if (r != k) M[e] -= F[r] * M[(size_t)k * n + c];
+        }
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int p = s.piv[k];
+        if (p != k)
+            for (int r = t; r < n; r += nt) {
+                const double tmp = M[(size_t)r * n + k];
+                M[(size_t)r * n + k] = M[(size_t)r * n + p];
+                M[(size_t)r * n + p] = tmp;
+            }
+        __syncthreads();
+    }
+    return singular;
+}
+#endif
+
 // ------------------------------------------------------------------ tables
 // Reference-element tables of the orthonormal basis at the Gauss nodes.
 namespace bh {

@@ -294,74 +294,20 @@ void bsr_invert_diag(bh_bsr* A, DevBuf& dinv64, DevBuf* dinv32) {
     BH_CHECK(h == 0, "block inversion: singular diagonal block in row " + std::to_string(h - 1));
 }
 
-// In-place Gauss-Jordan inverse with partial pivoting of one dense n x n block per
-// workgroup (n <= 128, the block in LDS): per column a pivot search, a row swap, the
-// pivot row scaled and every other row eliminated; the recorded row swaps are undone as
-// column swaps at the end.  Blocks larger than invert_blocks_kernel's augmented [A | I]
-// in LDS allows.
+// Inverse of one dense n x n block per workgroup (n <= 128), in place in LDS
+// (gj_invert_inplace): blocks larger than invert_blocks_kernel's augmented [A | I] in
+// LDS allows.
 __global__ __launch_bounds__(256) void invert_inplace_k(int nblocks, int n, const double* __restrict__ blocks,
                                                         double* __restrict__ inv64, float* __restrict__ inv32,
                                                         int* __restrict__ status) {
     extern __shared__ double M[];   // n * n, then n factors
-    double* F = M + (size_t)n * n;
-    __shared__ int piv[128];
-    __shared__ double bv[256];
-    __shared__ int bi[256];
+    __shared__ GjScratch gs;
     const int b = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
     if (b >= nblocks) return;
     const double* src = blocks + (size_t)b * n * n;
     for (int e = t; e < n * n; e += nt) M[e] = src[e];
     __syncthreads();
-    bool singular = false;
-    for (int k = 0; k < n; ++k) {
-        double best = -1.0;
-        int bix = k;
-        for (int r = k + t; r < n; r += nt) {
-            const double v = fabs(M[(size_t)r * n + k]);
-            if (v > best) { best = v; bix = r; }
-        }
-        bv[t] = best;
-        bi[t] = bix;
-        __syncthreads();
-        for (int s2 = nt / 2; s2 > 0; s2 >>= 1) {
-            if (t < s2 && (bv[t + s2] > bv[t] || (bv[t + s2] == bv[t] && bi[t + s2] < bi[t]))) { bv[t] = bv[t + s2]; bi[t] = bi[t + s2]; }
-            __syncthreads();
-        }
-        const int p = bi[0];
-        if (!(bv[0] > 0)) singular = true;
-        if (t == 0) piv[k] = p;
-        if (p != k)
-            for (int c = t; c < n; c += nt) {
-                const double tmp = M[(size_t)k * n + c];
-                M[(size_t)k * n + c] = M[(size_t)p * n + c];
-                M[(size_t)p * n + c] = tmp;
-            }
-        __syncthreads();
-        const double d = singular ? 0.0 : 1.0 / M[(size_t)k * n + k];
-        for (int r = t; r < n; r += nt) F[r] = M[(size_t)r * n + k];
-        __syncthreads();
-        if (t == 0) M[(size_t)k * n + k] = 1.0;
-        __syncthreads();
-        for (int c = t; c < n; c += nt) M[(size_t)k * n + c] *= d;
-        for (int r = t; r < n; r += nt)
-            if (r != k) M[(size_t)r * n + k] = 0.0;
-        __syncthreads();
-        for (int e = t; e < n * n; e += nt) {
-            const int r = e / n, c = e - r * n;
-            if (r != k) M[e] -= F[r] * M[(size_t)k * n + c];
-        }
-        __syncthreads();
-    }
-    for (int k = n - 1; k >= 0; --k) {
-        const int p = piv[k];
-        if (p != k)
-            for (int r = t; r < n; r += nt) {
-                const double tmp = M[(size_t)r * n + k];
-                M[(size_t)r * n + k] = M[(size_t)r * n + p];
-                M[(size_t)r * n + p] = tmp;
-            }
-        __syncthreads();
-    }
+    const bool singular = gj_invert_inplace(M, M + (size_t)n * n, n, gs);
     if (singular && t == 0) status[0] = b + 1;
     for (int e = t; e < n * n; e += nt) {
         if (inv64) inv64[(size_t)b * n * n + e] = M[e];

@@ -586,7 +586,134 @@ __global__ __launch_bounds__(256) void hi_smooth_k(int J, int bs, int lo, const 
     }
 }
 
-// ---------------------------------------------------------------- K-cycle scalars
+// ---------------------------------------------------------------- cell ILU
+// Block ILU on a pattern (CellILU, AS/CellILU.cs): W holds the blocks of the pattern
+// (column-major, E = bs^2 values each), rows CSR (prp, pcol sorted, pdiag the position
+// of the diagonal).  Rows are processed level by level; a row depends only on rows of
+// earlier levels, so the rows of a level run in parallel, one workgroup per row.
+
+// W = 0 on the pattern, then A's blocks at their pattern positions
+__global__ __launch_bounds__(256) void ilu_scatter_k(int64_t nnzb, int E, const int64_t* __restrict__ wpos,
+                                                     const double* __restrict__ A, double* __restrict__ W) {
+    const int64_t tot = nnzb * E;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = t / E;
+        W[(size_t)wpos[p] * E + (t - p * E)] = A[t];
+    }
+}
+
+__device__ inline int64_t ilu_find(const int64_t* prp, const int* pcol, int row, int c) {
+    int64_t lo = prp[row], hi = prp[row + 1] - 1;
+    while (lo <= hi) {
+        const int64_t m = (lo + hi) >> 1;
+        const int v = pcol[m];
+        if (v == c) return m;
+        if (v < c) lo = m + 1;
+        else hi = m - 1;
+    }
+    return -1;
+}
+
+// Row i (IKJ order, the same operations per block as the reference's k-i-j loops): for
+// every k < i of the row, ascending, W_ik <- W_ik Uinv_k and W_ij <- W_ij - W_ik W_kj for
+// the row's j > k that row k holds; then Uinv_i = W_ii^-1.  LDS: the scaled W_ik, and the
+// diagonal block for its inversion.
+__global__ __launch_bounds__(256) void ilu_factor_k(const int* __restrict__ rows, int n, int bs, const int64_t* __restrict__ prp,
+                                                    const int* __restrict__ pcol, const int64_t* __restrict__ pdiag,
+                                                    double* __restrict__ W, double* __restrict__ Uinv, int* __restrict__ status) {
+    extern __shared__ double sh[];   // L (E), M (E), F (bs)
+    __shared__ GjScratch gs;
+    if ((int)blockIdx.x >= n) return;
+    const int i = rows[blockIdx.x];
+    const int t = threadIdx.x, nt = blockDim.x, E = bs * bs;
+    double* Ls = sh;
+    double* M = sh + E;
+    const int64_t p0 = prp[i], p1 = prp[i + 1], pd = pdiag[i];
+    for (int64_t p = p0; p < pd; ++p) {
+        const int k = pcol[p];
+        const double* __restrict__ Uk = Uinv + (size_t)k * E;
+        double* __restrict__ Wp = W + (size_t)p * E;
+        for (int e = t; e < E; e += nt) {
+            const int c = e / bs, r = e - c * bs;
+            double s = 0;
+            for (int m = 0; m < bs; ++m) s += Wp[(size_t)m * bs + r] * Uk[(size_t)c * bs + m];
+            Ls[e] = s;
+        }
+        __syncthreads();
+        for (int e = t; e < E; e += nt) Wp[e] = Ls[e];
+        for (int64_t q = p + 1; q < p1; ++q) {
+            const int64_t pk = ilu_find(prp, pcol, k, pcol[q]);
+            if (pk < 0) continue;
+            const double* __restrict__ Wk = W + (size_t)pk * E;
+            double* __restrict__ Wq = W + (size_t)q * E;
+            for (int e = t; e < E; e += nt) {
+                const int c = e / bs, r = e - c * bs;
+                double s = 0;
+                for (int m = 0; m < bs; ++m) s += Ls[(size_t)m * bs + r] * Wk[(size_t)c * bs + m];
+                Wq[e] -= s;
+            }
+        }
+        __syncthreads();
+    }
+    const double* __restrict__ Wd = W + (size_t)pd * E;
+    for (int e = t; e < E; e += nt) M[e] = Wd[e];
+    __syncthreads();
+    const bool singular = gj_invert_inplace(M, M + E, bs, gs);
+    if (singular && t == 0) status[0] = i + 1;
+    for (int e = t; e < E; e += nt) Uinv[(size_t)i * E + e] = M[e];
+}
+
+// forward sweep of one level: y_i = r_i - sum_{k < i} L_ik y_k, one wave per row
+template <typename T>
+__global__ __launch_bounds__(256) void ilu_fwd_k(const int* __restrict__ rows, int n, int bs, const int64_t* __restrict__ prp,
+                                                 const int* __restrict__ pcol, const int64_t* __restrict__ pdiag,
+                                                 const T* __restrict__ W, const T* __restrict__ r, T* __restrict__ y) {
+    const int wv = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (wv >= n || lane >= bs) return;
+    const int i = rows[wv];
+    const int E = bs * bs;
+    T s = r[(size_t)i * bs + lane];
+    for (int64_t p = prp[i]; p < pdiag[i]; ++p) {
+        const T* __restrict__ Lb = W + (size_t)p * E;
+        const T* __restrict__ yk = y + (size_t)pcol[p] * bs;
+        for (int m = 0; m < bs; ++m) s -= Lb[(size_t)m * bs + lane] * yk[m];
+    }
+    y[(size_t)i * bs + lane] = s;
+}
+
+// backward sweep of one level: z_i = Uinv_i (y_i - sum_{j > i} U_ij z_j) and
+// x_i (+)= omega z_i, one wave per row
+template <typename T, bool ADD>
+__global__ __launch_bounds__(256) void ilu_bwd_k(const int* __restrict__ rows, int n, int bs, const int64_t* __restrict__ prp,
+                                                 const int* __restrict__ pcol, const int64_t* __restrict__ pdiag,
+                                                 const T* __restrict__ W, const T* __restrict__ Uinv, const T* __restrict__ y,
+                                                 T* __restrict__ z, T* __restrict__ x, T omega) {
+    __shared__ T sv[4][64];
+    const int w = threadIdx.x >> 6, wv = blockIdx.x * 4 + w, lane = threadIdx.x & 63;
+    const bool act = wv < n && lane < bs;
+    const int i = wv < n ? rows[wv] : 0;
+    const int E = bs * bs;
+    if (act) {
+        T s = y[(size_t)i * bs + lane];
+        for (int64_t p = pdiag[i] + 1; p < prp[i + 1]; ++p) {
+            const T* __restrict__ Ub = W + (size_t)p * E;
+            const T* __restrict__ zj = z + (size_t)pcol[p] * bs;
+            for (int m = 0; m < bs; ++m) s -= Ub[(size_t)m * bs + lane] * zj[m];
+        }
+        sv[w][lane] = s;
+    }
+    __syncthreads();
+    if (act) {
+        const T* __restrict__ Ui = Uinv + (size_t)i * E;
+        T v = 0;
+        for (int m = 0; m < bs; ++m) v += Ui[(size_t)m * bs + lane] * sv[w][m];
+        z[(size_t)i * bs + lane] = v;
+        const size_t o = (size_t)i * bs + lane;
+        x[o] = ADD ? x[o] + omega * v : omega * v;
+    }
+}
+
+// ---------------------------------------------------------------- K-cycle scalars// ---------------------------------------------------------------- K-cycle scalars
 // The inner flexible GMRES of the K-cycle keeps its Hessenberg matrix, Gram-
 // Schmidt coefficients and least-squares solution in device memory (layout
 // KcycleScalars), so the whole preconditioner needs no host round trip.
@@ -686,6 +813,37 @@ __global__ __launch_bounds__(256) void sub_k(const T* __restrict__ b, const T* _
 }
 
 // ------------------------------------------------------------------ launchers
+void ilu_scatter(bh_ctx* ctx, int64_t nnzb, int bs, const int64_t* wpos, const double* A, double* W) {
+    hipLaunchKernelGGL(ilu_scatter_k, grid1d(nnzb * bs * bs), dim3(256), 0, ctx->stream, nnzb, bs * bs, wpos, A, W);
+    BH_HIP(hipGetLastError());
+}
+void ilu_factor_level(bh_ctx* ctx, const int* rows, int n, int bs, const int64_t* prp, const int* pcol, const int64_t* pdiag,
+                      double* W, double* Uinv, int* status) {
+    const size_t shm = (2 * (size_t)bs * bs + bs) * sizeof(double);
+    hipLaunchKernelGGL(ilu_factor_k, dim3((unsigned)n), dim3(256), shm, ctx->stream, rows, n, bs, prp, pcol, pdiag, W, Uinv, status);
+    BH_HIP(hipGetLastError());
+}
+template <typename T>
+void ilu_sweeps(bh_ctx* ctx, int nlev, const std::vector<int>& lvptr, const int* rows, int bs, const int64_t* prp,
+                const int* pcol, const int64_t* pdiag, const T* W, const T* Uinv, const T* r, T* y, T* z, T* x, double omega,
+                bool add) {
+    for (int l = 0; l < nlev; ++l) {
+        const int n = lvptr[l + 1] - lvptr[l];
+        hipLaunchKernelGGL((ilu_fwd_k<T>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, rows + lvptr[l], n, bs, prp,
+                           pcol, pdiag, W, r, y);
+    }
+    for (int l = nlev - 1; l >= 0; --l) {
+        const int n = lvptr[l + 1] - lvptr[l];
+        if (add)
+            hipLaunchKernelGGL((ilu_bwd_k<T, true>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, rows + lvptr[l], n, bs,
+                               prp, pcol, pdiag, W, Uinv, y, z, x, (T)omega);
+        else
+            hipLaunchKernelGGL((ilu_bwd_k<T, false>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, ctx->stream, rows + lvptr[l], n,
+                               bs, prp, pcol, pdiag, W, Uinv, y, z, x, (T)omega);
+    }
+    BH_HIP(hipGetLastError());
+}
+
 void hi_blocks(bh_ctx* ctx, int J, int bs, int lo, const bh_bsr* A, double* out) {
     hipLaunchKernelGGL(hi_blocks_k, grid1d((int64_t)J * (bs - lo) * (bs - lo)), dim3(256), 0, ctx->stream, J, bs, lo,
                        A->diag_idx.as<int64_t>(), A->vals.as<double>(), out);
@@ -921,7 +1079,9 @@ void vcopy(bh_ctx* ctx, const T* s, T* d, int64_t n) {
     template void schwarz_apply<T>(bh_ctx*, int, int, int, const int*, const int*, const T*, const T*, T*, double, bool); \
     template void orth2_step<T>(bh_ctx*, const T*, T*, int64_t, double*, int*); \
     template void sub_vec<T>(bh_ctx*, const T*, const T*, T*, int64_t);            \
-    template void hi_smooth<T>(bh_ctx*, int, int, int, const T*, const T*, T*);
+    template void hi_smooth<T>(bh_ctx*, int, int, int, const T*, const T*, T*);     \
+    template void ilu_sweeps<T>(bh_ctx*, int, const std::vector<int>&, const int*, int, const int64_t*, const int*, \
+                                const int64_t*, const T*, const T*, const T*, T*, T*, T*, double, bool);
 BH_INST(float)
 BH_INST(double)
 #undef BH_INST

@@ -21,6 +21,14 @@ template <typename T> void agg_restrict(bh_ctx*, int Jc, int bs, const int* cptr
 template <typename T> void agg_prolong_add(bh_ctx*, int Jf, int bs, const int* parent, const int* pidx, const T* P, const T* xc,
                                           T* xf);
 template <typename T> void block_gemv(bh_ctx*, int nrows, int bs, const T* Dinv, const T* r, T* x, double omega, bool add);
+// cell ILU (block ILU on a pattern, level-scheduled): A's blocks into the pattern, one level
+// of the factorisation, and the two sweeps x (+)= omega U^-1 L^-1 r
+void ilu_scatter(bh_ctx*, int64_t nnzb, int bs, const int64_t* wpos, const double* A, double* W);
+void ilu_factor_level(bh_ctx*, const int* rows, int n, int bs, const int64_t* prp, const int* pcol, const int64_t* pdiag,
+                      double* W, double* Uinv, int* status);
+template <typename T>
+void ilu_sweeps(bh_ctx*, int nlev, const std::vector<int>& lvptr, const int* rows, int bs, const int64_t* prp, const int* pcol,
+                const int64_t* pdiag, const T* W, const T* Uinv, const T* r, T* y, T* z, T* x, double omega, bool add);
 // LevelPmg: high-order parts (modes lo..bs-1) of the diagonal blocks; x_hi += Hinv r_hi per cell
 void hi_blocks(bh_ctx*, int J, int bs, int lo, const bh_bsr* A, double* out);
 template <typename T> void hi_smooth(bh_ctx*, int J, int bs, int lo, const T* Hinv, const T* r, T* x);
@@ -122,6 +130,12 @@ struct PMultigrid : bh_solver {
     int lp_lo = 0;
     DevBuf lp64, lp32;
     void set_level_pmg(bool on);
+    // cell ILU as the finest level's smoother (ILU(p) pattern = block pattern of A^(p+1)):
+    // rows grouped by dependency level, the factors on the pattern and the diagonal inverses
+    int ilu_p = -1, ilu_nlev = 0;
+    std::vector<int> ilu_lvptr;
+    DevBuf ilu_rows, ilu_prp, ilu_pcol, ilu_pdiag, ilu_W64, ilu_W32, ilu_U64, ilu_U32, ilu_y, ilu_z;
+    void set_cell_ilu(int p);
     bh_bsr* eqA0 = nullptr;        // the transformed finest matrix (the caller's matrix stays untouched)
     int kcycle_level = -1;         // level solved by `kin` inner FGMRES steps (K-cycle), -1: none
     int kin = 0;

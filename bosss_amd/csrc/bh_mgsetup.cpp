@@ -523,6 +523,120 @@ void PMultigrid::set_level_pmg(bool on) {
     level_pmg = true;
 }
 
+// CellILU (AS/CellILU.cs:1229-1310 ComputeILU, :1490-1560 GetPattern, :182-250 Solve) as
+// the finest level's smoother: the ILU(p) pattern is the block pattern of A^(p+1); the
+// factorisation restricted to it runs row by row in IKJ order (for every block the same
+// operations in the same order as the reference's k-i-j loops), rows grouped by
+// dependency level (a row depends on the rows k < i of its pattern row), one launch per
+// level; the diagonal blocks are inverted (the reference keeps their LU factors).  The
+// smoother is x (+)= omega U^-1 L^-1 r: a forward sweep over the levels, a backward one
+// in reverse order; in the FP32 cycle the factors are rounded to FP32 (the reference
+// back-substitutes in single precision too).  p < 0 switches it off.
+void PMultigrid::set_cell_ilu(int p) {
+    drop_graphs();
+    ilu_p = -1;
+    ilu_nlev = 0;
+    ilu_lvptr.clear();
+    for (DevBuf* b : {&ilu_rows, &ilu_prp, &ilu_pcol, &ilu_pdiag, &ilu_W64, &ilu_W32, &ilu_U64, &ilu_U32, &ilu_y, &ilu_z})
+        b->release();
+    if (p < 0) return;
+    BH_CHECK(p <= 2, "multigrid: cell ILU level must be 0, 1 or 2");
+    BH_CHECK(!equilib, "multigrid: cell ILU and DiagBlockEquilib are exclusive");
+    BH_CHECK(lv.size() >= 2, "multigrid: cell ILU needs a hierarchy");
+    MgLevel& L = lv[0];
+    bh_bsr* A = L.A;
+    const int J = L.J, bs = L.bs, E = bs * bs;
+    BH_CHECK(bs <= 56, "multigrid: cell ILU supports blocks up to 56 (k <= 5 in 3D)");
+    // pattern rows
+    std::vector<std::vector<int>> base(J), pat;
+    for (int i = 0; i < J; ++i) base[i].assign(A->h_col.begin() + A->h_rowptr[i], A->h_col.begin() + A->h_rowptr[i + 1]);
+    for (auto& r : base) std::sort(r.begin(), r.end());
+    pat = base;
+    std::vector<char> mark(J, 0);
+    for (int it = 0; it < p; ++it) {
+        std::vector<std::vector<int>> nxt(J);
+        for (int i = 0; i < J; ++i) {
+            for (int k : pat[i])
+                for (int j : base[k])
+                    if (!mark[j]) { mark[j] = 1; nxt[i].push_back(j); }
+            for (int j : nxt[i]) mark[j] = 0;
+            std::sort(nxt[i].begin(), nxt[i].end());
+        }
+        pat.swap(nxt);
+    }
+    std::vector<int64_t> prp(J + 1, 0), pdiag(J, -1);
+    std::vector<int> pcol;
+    for (int i = 0; i < J; ++i) {
+        for (int j : pat[i]) {
+            if (j == i) pdiag[i] = (int64_t)pcol.size();
+            pcol.push_back(j);
+        }
+        prp[i + 1] = (int64_t)pcol.size();
+        BH_CHECK(pdiag[i] >= 0, "multigrid: cell ILU needs every diagonal block");
+    }
+    // dependency levels and the rows grouped by level (ascending row index within a level)
+    std::vector<int> lev(J, 0);
+    int nlev = 0;
+    for (int i = 0; i < J; ++i) {
+        int m = -1;
+        for (int64_t q = prp[i]; q < pdiag[i]; ++q) m = std::max(m, lev[pcol[q]]);
+        lev[i] = m + 1;
+        nlev = std::max(nlev, lev[i] + 1);
+    }
+    std::vector<int> lvptr(nlev + 1, 0), rows(J);
+    for (int i = 0; i < J; ++i) ++lvptr[lev[i] + 1];
+    for (int l = 0; l < nlev; ++l) lvptr[l + 1] += lvptr[l];
+    {
+        std::vector<int> pos(lvptr.begin(), lvptr.end() - 1);
+        for (int i = 0; i < J; ++i) rows[pos[lev[i]]++] = i;
+    }
+    // A's blocks at their pattern positions
+    std::vector<int64_t> wpos(std::max<int64_t>(A->nnzb, 1));
+    for (int i = 0; i < J; ++i)
+        for (int64_t q = A->h_rowptr[i]; q < A->h_rowptr[i + 1]; ++q) {
+            const auto it = std::lower_bound(pcol.begin() + prp[i], pcol.begin() + prp[i + 1], A->h_col[q]);
+            wpos[q] = it - pcol.begin();
+        }
+    ilu_rows.alloc(J * sizeof(int));
+    ilu_prp.alloc((J + 1) * sizeof(int64_t));
+    ilu_pcol.alloc(pcol.size() * sizeof(int));
+    ilu_pdiag.alloc(J * sizeof(int64_t));
+    ctx->upload(ilu_rows.p, rows.data(), J * sizeof(int));
+    ctx->upload(ilu_prp.p, prp.data(), (J + 1) * sizeof(int64_t));
+    ctx->upload(ilu_pcol.p, pcol.data(), pcol.size() * sizeof(int));
+    ctx->upload(ilu_pdiag.p, pdiag.data(), J * sizeof(int64_t));
+    DevBuf dwpos(wpos.size() * sizeof(int64_t)), st(sizeof(int));
+    ctx->upload(dwpos.p, wpos.data(), wpos.size() * sizeof(int64_t));
+    const int64_t nnzp = (int64_t)pcol.size();
+    ilu_W64.alloc((size_t)nnzp * E * sizeof(double));
+    ilu_U64.alloc((size_t)J * E * sizeof(double));
+    fill(ctx, ilu_W64.as<double>(), 0.0, nnzp * E);
+    ilu_scatter(ctx, A->nnzb, bs, dwpos.as<int64_t>(), A->vals.as<double>(), ilu_W64.as<double>());
+    const int zero = 0;
+    ctx->upload(st.p, &zero, sizeof(int));
+    for (int l = 0; l < nlev; ++l)
+        ilu_factor_level(ctx, ilu_rows.as<int>() + lvptr[l], lvptr[l + 1] - lvptr[l], bs, ilu_prp.as<int64_t>(),
+                         ilu_pcol.as<int>(), ilu_pdiag.as<int64_t>(), ilu_W64.as<double>(), ilu_U64.as<double>(), st.as<int>());
+    int h = 0;
+    ctx->download(&h, st.p, sizeof(int));
+    BH_CHECK(h == 0, "multigrid: cell ILU: singular diagonal block in row " + std::to_string(h - 1));
+    const size_t es = prec == BH_FP32 ? sizeof(float) : sizeof(double);
+    if (prec == BH_FP32) {
+        ilu_W32.alloc((size_t)nnzp * E * sizeof(float));
+        ilu_U32.alloc((size_t)J * E * sizeof(float));
+        convert_f64_f32(ctx, ilu_W64.as<double>(), ilu_W32.as<float>(), nnzp * E);
+        convert_f64_f32(ctx, ilu_U64.as<double>(), ilu_U32.as<float>(), (int64_t)J * E);
+        ctx->sync();
+        ilu_W64.release();
+        ilu_U64.release();
+    }
+    ilu_y.alloc(L.N * es);
+    ilu_z.alloc(L.N * es);
+    ilu_lvptr = lvptr;
+    ilu_nlev = nlev;
+    ilu_p = p;
+}
+
 // smoother of level l: block-Jacobi on the cell blocks (AS/BlockJacobi.cs:196-255), or
 // additive Schwarz on the aggregates of the level when enabled (AS/Schwarz.cs, MG-aggregate
 // blocking, no overlap)
@@ -530,7 +644,11 @@ template <typename T>
 void PMultigrid::smooth(int l, const T* r, T* x, bool add) {
     MgLevel& L = lv[l];
     const bool f32 = sizeof(T) == 4;
-    if (equilib && L.eqL64.p) {
+    if (l == 0 && ilu_nlev > 0) {
+        ilu_sweeps<T>(ctx, ilu_nlev, ilu_lvptr, ilu_rows.as<int>(), L.bs, ilu_prp.as<int64_t>(), ilu_pcol.as<int>(),
+                      ilu_pdiag.as<int64_t>(), (const T*)(f32 ? ilu_W32.p : ilu_W64.p), (const T*)(f32 ? ilu_U32.p : ilu_U64.p), r,
+                      (T*)ilu_y.p, (T*)ilu_z.p, x, omega, add);
+    } else if (equilib && L.eqL64.p) {
         // block-equilibrated level: the diagonal blocks are diag(S) (signs), block-Jacobi is x (+)= omega S r
         sign_smooth<T>(ctx, L.N, omega, (const T*)(f32 ? L.eqS32.p : L.eqS64.p), r, x, add);
     } else if (L.sw_cmax > 0 && L.sw_overlap > 0)

@@ -337,6 +337,16 @@ int bh_pmg_set_schwarz(bh_solver* pmg, int enable);
  * The low-order system is solved by the next levels (V- or K-cycle), not by a
  * sparse direct solver. */
 int bh_pmg_set_level_pmg(bh_solver* pmg, int enable);
+/* CellILU (AS/CellILU.cs:1229-1310 ComputeILU, :1490-1560 GetPattern, :182-250 Solve)
+ * as the smoother of the finest level of a bh_pmg_create hierarchy: block ILU on the
+ * pattern of A^(ilu_level + 1) (0, 1 or 2; the reference's default is 1), factorised
+ * on the device level by level, applied as x (+)= omega U^-1 L^-1 r (FP32 factors in
+ * an FP32 cycle); ilu_level < 0 restores block-Jacobi.  The sweeps are sequential
+ * over the dependency levels of the pattern (one launch per level and sweep). */
+int bh_pmg_set_cell_ilu(bh_solver* pmg, int ilu_level);
+/* No counterpart in the reference (for checks): x = S(r) with the smoother S of level
+ * `level` from a zero guess, in the hierarchy's precision (host FP64 vectors). */
+int bh_pmg_smooth(bh_solver* pmg, int level, const double* r, double* x);
 /* Schwarz with overlap (Schwarz.Config.Overlap, AS/Schwarz.cs:491-512; block growth
  * :683-730; the additive sum of the block solutions :1026-1085): each aggregate block
  * is grown by `overlap` (0, 1 or 2) sweeps over the face neighbours of its cells and
