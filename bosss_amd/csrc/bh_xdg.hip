@@ -340,7 +340,9 @@ struct Leg1 {
     }
 };
 
-template <int D, int K, int TYPE, int S>
+// DIR: the face direction of a colour group of axis-aligned faces (TYPE 1, 2), known at
+// compile time so that only the normal derivative's one direction is formed; -1 otherwise.
+template <int D, int K, int TYPE, int S, int DIR>
 __global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
     constexpr int Np = D == 2 ? (K + 1) * (K + 2) / 2 : (K + 1) * (K + 2) * (K + 3) / 6;
     constexpr int NSEG = 64 / S;
@@ -381,7 +383,8 @@ __global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
             double nrm[D];
 #pragma unroll
             for (int d = 0; d < D; ++d)
-                nrm[d] = TYPE == 3 ? a.nn[(size_t)(E.r0 + q) * 3 + d] : (TYPE != 0 && d == E.dir ? fsgn : 0.0);
+                nrm[d] = TYPE == 3 ? a.nn[(size_t)(E.r0 + q) * 3 + d]
+                                   : (TYPE != 0 && d == (DIR >= 0 ? DIR : E.dir) ? fsgn : 0.0);
             Leg1<D, K> LP, LQ;
             LP.at(loP, hP, x);
             if constexpr (TWO) LQ.at(loQ, hQ, x);
@@ -962,40 +965,47 @@ void xdg_assemble(XdgDevice& X, Vbsr& M) {
 constexpr int pow_i(int b, int n) { return n == 0 ? 1 : b * pow_i(b, n - 1); }
 constexpr int seg_lanes(int nodes) { return nodes <= 8 ? 8 : nodes <= 16 ? 16 : nodes <= 32 ? 32 : 64; }
 
-template <int D, int K, int TYPE>
+template <int D, int K, int TYPE, int DIR>
 static void xdg_eval_type(hipStream_t st, bool cut, const XEvalArgs& e) {
     constexpr int S_T = seg_lanes(pow_i(K + 1, TYPE == 0 ? D : D - 1));
     if (cut || TYPE == 3) {
         const int blocks = e.g.count < 65536 ? e.g.count : 65536;
-        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, 64>), dim3(blocks), dim3(64), 0, st, e);
+        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, 64, DIR>), dim3(blocks), dim3(64), 0, st, e);
     } else {
         constexpr int NSEG = 64 / S_T;
         const int nb = (e.g.count + NSEG - 1) / NSEG;
         const int blocks = nb < 65536 ? nb : 65536;
-        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, S_T>), dim3(blocks), dim3(64), 0, st, e);
+        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, S_T, DIR>), dim3(blocks), dim3(64), 0, st, e);
     }
     BH_HIP(hipGetLastError());
 }
 
+template <int D, int K, int TYPE>
+static void xdg_eval_face(hipStream_t st, int dir, bool cut, const XEvalArgs& e) {
+    if (dir == 0) xdg_eval_type<D, K, TYPE, 0>(st, cut, e);
+    else if (dir == 1) xdg_eval_type<D, K, TYPE, 1>(st, cut, e);
+    else if constexpr (D == 3) xdg_eval_type<D, K, TYPE, 2>(st, cut, e);
+}
+
 template <int D, int K>
-static void xdg_eval_dk(hipStream_t st, int type, bool cut, const XEvalArgs& e) {
+static void xdg_eval_dk(hipStream_t st, int type, int dir, bool cut, const XEvalArgs& e) {
     switch (type) {
-        case 0: xdg_eval_type<D, K, 0>(st, cut, e); break;
-        case 1: xdg_eval_type<D, K, 1>(st, cut, e); break;
-        case 2: xdg_eval_type<D, K, 2>(st, cut, e); break;
-        default: xdg_eval_type<D, K, 3>(st, cut, e); break;
+        case 0: xdg_eval_type<D, K, 0, -1>(st, cut, e); break;
+        case 1: xdg_eval_face<D, K, 1>(st, dir, cut, e); break;
+        case 2: xdg_eval_face<D, K, 2>(st, dir, cut, e); break;
+        default: xdg_eval_type<D, K, 3, -1>(st, cut, e); break;
     }
 }
 
-static void xdg_evaluate_launch(hipStream_t st, int D, int k, int type, bool cut, const XEvalArgs& e) {
+static void xdg_evaluate_launch(hipStream_t st, int D, int k, int type, int dir, bool cut, const XEvalArgs& e) {
     if (D == 2) {
-        if (k == 1) xdg_eval_dk<2, 1>(st, type, cut, e);
-        else if (k == 2) xdg_eval_dk<2, 2>(st, type, cut, e);
-        else xdg_eval_dk<2, 3>(st, type, cut, e);
+        if (k == 1) xdg_eval_dk<2, 1>(st, type, dir, cut, e);
+        else if (k == 2) xdg_eval_dk<2, 2>(st, type, dir, cut, e);
+        else xdg_eval_dk<2, 3>(st, type, dir, cut, e);
     } else {
-        if (k == 1) xdg_eval_dk<3, 1>(st, type, cut, e);
-        else if (k == 2) xdg_eval_dk<3, 2>(st, type, cut, e);
-        else xdg_eval_dk<3, 3>(st, type, cut, e);
+        if (k == 1) xdg_eval_dk<3, 1>(st, type, dir, cut, e);
+        else if (k == 2) xdg_eval_dk<3, 2>(st, type, dir, cut, e);
+        else xdg_eval_dk<3, 3>(st, type, dir, cut, e);
     }
 }
 
@@ -1029,7 +1039,8 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
             a.count = (part == 0 ? X.cut_end[c] : X.colour_off[c + 1]) - a.start;
             if (a.count == 0) continue;
             const int type = c == 0 ? 0 : c == 1 ? 3 : c < 8 ? 2 : 1;
-            xdg_evaluate_launch(X.ctx->stream, P.D, P.k, type, part == 0, e);
+            const int dir = c < 2 ? -1 : c < 8 ? (c - 2) / 2 : (c - 8) / 2;   // colours of the faces: per direction
+            xdg_evaluate_launch(X.ctx->stream, P.D, P.k, type, dir, part == 0, e);
         }
     }
 }
