@@ -101,6 +101,9 @@ SIGNATURES = {
     "bh_pmg_set_level_pmg": (_c_i, [_c_p, _c_i]),
     "bh_pmg_set_cell_ilu": (_c_i, [_c_p, _c_i]),
     "bh_pmg_smooth": (_c_i, [_c_p, _c_i, _c_p, _c_p]),
+    "bh_pmg_level_info": (_c_i, [_c_p, _c_i, _c_p, _c_p]),
+    "bh_pmg_level_transfer": (_c_i, [_c_p, _c_i, _c_p, _c_p]),
+    "bh_pmg_level_matrix": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p]),
     "bh_pmg_set_schwarz_overlap": (_c_i, [_c_p, _c_i, _c_i, _c_i]),
     "bh_pmg_set_diag_equilib": (_c_i, [_c_p]),
     "bh_xdg_precond_set_fp64_blocks": (_c_i, [_c_p, _c_i]),
@@ -729,6 +732,39 @@ def pmg_smooth(pmg, level, r):
     x = np.zeros_like(r)
     _check(lib().bh_pmg_smooth(pmg.h, int(level), r.ctypes.data, x.ctypes.data))
     return x
+
+
+def pmg_levels(pmg):
+    """The multigrid mapping: per level dict(nrows, bs, kind (0 truncation, 1 aggregation,
+    -1 coarsest), deg, n, nnzb)."""
+    info = np.zeros(7, np.int32)
+    _check(lib().bh_pmg_level_info(pmg.h, -1, info.ctypes.data, None))
+    out = []
+    for lvl in range(int(info[0])):
+        nz = _c_i64()
+        _check(lib().bh_pmg_level_info(pmg.h, lvl, info.ctypes.data, ctypes.byref(nz)))
+        out.append(dict(nrows=int(info[0]), bs=int(info[1]), kind=int(info[2]), deg=int(info[3]),
+                        n=[int(v) for v in info[4:7]], nnzb=int(nz.value)))
+    return out
+
+
+def pmg_level_transfer(pmg, level):
+    """(parent aggregate per cell, P[cell][fine mode][coarse mode]) of an aggregation level."""
+    inf = pmg_levels(pmg)[level]
+    par = np.zeros(inf["nrows"], np.int32)
+    P = np.zeros((inf["nrows"], inf["bs"], inf["bs"]))
+    _check(lib().bh_pmg_level_transfer(pmg.h, int(level), par.ctypes.data, P.ctypes.data))
+    return par, P
+
+
+def pmg_level_matrix(pmg, level):
+    """(rowptr, col, blocks[nnzb, bs, bs]) of a level's Galerkin matrix."""
+    inf = pmg_levels(pmg)[level]
+    rp = np.zeros(inf["nrows"] + 1, np.int64)
+    col = np.zeros(inf["nnzb"], np.int32)
+    val = np.zeros(inf["nnzb"] * inf["bs"] * inf["bs"])
+    _check(lib().bh_pmg_level_matrix(pmg.h, int(level), rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+    return rp, col, val.reshape(inf["nnzb"], inf["bs"], inf["bs"])
 
 
 def pmg_set_level_pmg(pmg, enable=True):

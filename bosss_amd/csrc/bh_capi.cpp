@@ -1109,6 +1109,59 @@ int bh_pmg_set_cell_ilu(bh_solver* s, int ilu_level) {
     BH_API_END
 }
 
+int bh_pmg_level_info(bh_solver* s, int level, int* info, int64_t* nnzb) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_level_info: solver is not a multigrid preconditioner");
+    const int nl = (int)mg->lv.size();
+    if (level < 0) {   // the number of levels
+        if (info) info[0] = nl;
+        return 0;
+    }
+    BH_CHECK(level < nl, "bh_pmg_level_info: level out of range");
+    const MgLevel& L = mg->lv[level];
+    if (info) {
+        info[0] = L.J;
+        info[1] = L.bs;
+        info[2] = level + 1 < nl ? L.kind : -1;
+        info[3] = L.deg;
+        for (int d = 0; d < 3; ++d) info[4 + d] = L.n[d];
+    }
+    if (nnzb) *nnzb = L.A ? L.A->nnzb : 0;
+    BH_API_END
+}
+
+int bh_pmg_level_transfer(bh_solver* s, int level, int* parent, double* P) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_level_transfer: solver is not a multigrid preconditioner");
+    BH_CHECK(level >= 0 && level + 1 < (int)mg->lv.size() && mg->lv[level].kind == 1,
+             "bh_pmg_level_transfer: not an aggregation level");
+    const MgLevel& L = mg->lv[level];
+    const int bs = L.bs;
+    std::vector<int> par(L.J), pidx(L.J);
+    mg->ctx->download(par.data(), L.parent.p, L.J * sizeof(int));
+    mg->ctx->download(pidx.data(), L.pidx.p, L.J * sizeof(int));
+    const size_t nP = L.P64.bytes / sizeof(double);
+    std::vector<double> Pd(nP);
+    mg->ctx->download(Pd.data(), L.P64.p, nP * sizeof(double));
+    if (parent) std::copy(par.begin(), par.end(), parent);
+    if (P)
+        for (int i = 0; i < L.J; ++i)
+            std::copy(Pd.begin() + (size_t)pidx[i] * bs * bs, Pd.begin() + (size_t)(pidx[i] + 1) * bs * bs, P + (size_t)i * bs * bs);
+    BH_API_END
+}
+
+int bh_pmg_level_matrix(bh_solver* s, int level, int64_t* rowptr, int* col, double* vals) {
+    BH_API_BEGIN
+    PMultigrid* mg = dynamic_cast<PMultigrid*>(s);
+    BH_CHECK(mg, "bh_pmg_level_matrix: solver is not a multigrid preconditioner");
+    BH_CHECK(level >= 0 && level < (int)mg->lv.size() && mg->lv[level].A, "bh_pmg_level_matrix: no assembled matrix on this level");
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    BH_CHECK(bh_bsr_export(mg->lv[level].A, rowptr, col, vals) == 0, "bh_pmg_level_matrix: export failed");
+    BH_API_END
+}
+
 int bh_pmg_smooth(bh_solver* s, int level, const double* r, double* x) {
     BH_API_BEGIN
     PMultigrid* mg = dynamic_cast<PMultigrid*>(s);

@@ -347,6 +347,17 @@ int bh_pmg_set_cell_ilu(bh_solver* pmg, int ilu_level);
 /* No counterpart in the reference (for checks): x = S(r) with the smoother S of level
  * `level` from a zero guess, in the hierarchy's precision (host FP64 vectors). */
 int bh_pmg_smooth(bh_solver* pmg, int level, const double* r, double* x);
+/* The multigrid mapping of a bh_pmg_create hierarchy (MultigridMapping /
+ * AggregationGridBasis, AS/MultigridOperator.cs:573-640, AS/AggregationGridBasis.cs),
+ * for checks.  level < 0: info[0] = number of levels.  Else info[7] = {cells, DOF per
+ * cell, transfer to the next level (0 degree truncation, 1 aggregation, -1 coarsest),
+ * degree, logical grid n0, n1, n2} and *nnzb the blocks of the level matrix.
+ * bh_pmg_level_transfer: per fine cell its aggregate and its prolongation block
+ * P[fine mode][coarse mode] (row-major); bh_pmg_level_matrix: the Galerkin level
+ * matrix (row-major blocks, as bh_bsr_export). */
+int bh_pmg_level_info(bh_solver* pmg, int level, int* info7, int64_t* nnzb);
+int bh_pmg_level_transfer(bh_solver* pmg, int level, int* parent, double* P);
+int bh_pmg_level_matrix(bh_solver* pmg, int level, int64_t* rowptr, int* col, double* vals);
 /* Schwarz with overlap (Schwarz.Config.Overlap, AS/Schwarz.cs:491-512; block growth
  * :683-730; the additive sum of the block solutions :1026-1085): each aggregate block
  * is grown by `overlap` (0, 1 or 2) sweeps over the face neighbours of its cells and

@@ -221,6 +221,35 @@ def test_cell_ilu_fgmres_converges(ctx):
     assert rel(sols[1], sols[0]) < 1e-6
 
 
+@pytest.mark.parametrize("matrix_free", [False, True])
+def test_multigrid_mapping_matches_oracle(ctx, matrix_free):
+    """The device hierarchy's multigrid mapping (levels, DOF per cell, degrees, logical
+    grids, aggregation parents and prolongation blocks) equals the oracle's, and every
+    Galerkin level matrix equals the oracle's to 1e-12 (AS/MultigridOperator.cs:573-640,
+    AS/AggregationGridBasis.cs)."""
+    _say(f"case multigrid mapping matrix_free={matrix_free}")
+    og, gg, oo, go, Ao = _problem(ctx)
+    P = O.Pmg(Ao, og, [3, 1], min_coarse=8, omega=0.7, pre=1, post=1)
+    mg = B.PMultigrid(go, degrees=[3, 1], min_coarse=8, omega=0.7, pre=1, post=1, kcycle_inner=0, matrix_free=matrix_free)
+    lv = B.pmg_levels(mg)
+    assert len(lv) == len(P.sizes) >= 4
+    for l, L in enumerate(lv):
+        ref = P.level_info(l)
+        assert (L["nrows"], L["bs"], L["deg"]) == (ref["nrows"], ref["bs"], ref["deg"])
+        if l + 1 < len(lv):
+            assert L["kind"] == ref["kind"]
+            assert L["n"] == ref["n"]
+        if L["kind"] == 1:
+            par, Pb = B.pmg_level_transfer(mg, l)
+            rpar, rPb = P.level_transfer(l)
+            assert np.array_equal(par, rpar)
+            assert np.abs(Pb - rPb).max() <= 1e-14 * np.abs(rPb).max()
+        rp, col, blk = B.pmg_level_matrix(mg, l)
+        orp, ocol, oblk = P.level_matrix(l).export()
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+        assert np.abs(blk - oblk).max() <= 1e-12 * np.abs(oblk).max()
+
+
 def _kcycle_numpy(P, level, b, kin):
     """kin flexible-GMRES steps on level `level` (CGS2), V-cycle preconditioned."""
     n = b.size
