@@ -136,6 +136,7 @@ SIGNATURES = {
     "bh_xdg_precond_create": (_c_i, [_c_p, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
     "bh_xdg_precond_create_v": (_c_i, [_c_p, _c_p, _c_i, _c_i, _c_i, _c_d, _c_i, _c_i, _c_i, _c_i, _PP]),
     "bh_xdg_dof_frame": (_c_i, [_c_p, _c_i, _c_p, _c_p, _c_p]),
+    "bh_xdg_mass_matrix": (_c_i, [_c_p, _c_p]),
     "bh_xdg_fgmres_create": (_c_i, [_c_p, _c_p, _c_i, _c_d, _c_i, _PP]),
     "bh_solver_solve": (_c_i, [_c_p, _c_p, _c_p, _c_i]),
     "bh_solver_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_d),
@@ -994,6 +995,13 @@ class XdgPoisson(_Owned):
         c0 = np.zeros(self.J, np.int64)
         _check(lib().bh_xdg_dof_frame(self.h, which, i0.ctypes.data, ln.ctypes.data, c0.ctypes.data))
         return i0, ln, c0
+
+    def mass_matrix(self):
+        """Per (cell, species) mass blocks over the full layout: array (N_full, Np),
+        row r holds the block row of DOF r (identity blocks in uncut cells)."""
+        out = np.zeros((self.N_full, self.Np))
+        _check(lib().bh_xdg_mass_matrix(self.h, out.ctypes.data))
+        return out
 
     def extend(self, u):
         u = np.ascontiguousarray(u, np.float64)

@@ -1431,6 +1431,16 @@ int bh_xdg_cells(bh_xdg* X, int* spmask, double* vol, double* iface_area, double
     BH_API_END
 }
 
+int bh_xdg_mass_matrix(bh_xdg* X, double* out) {
+    BH_API_BEGIN
+    BH_CHECK(X->ctx, "bh_xdg_mass_matrix: needs a device context");
+    const int64_t n = X->P->N_full * X->P->Np;
+    DevBuf d(std::max<int64_t>(n, 1) * sizeof(double));
+    xdg_mass(X->X, d.as<double>());
+    X->ctx->download(out, d.p, n * sizeof(double));
+    BH_API_END
+}
+
 int bh_xdg_agglomeration(bh_xdg* X, int* pairs) {
     BH_API_BEGIN
     std::copy(X->P->agg_pairs.begin(), X->P->agg_pairs.end(), pairs);

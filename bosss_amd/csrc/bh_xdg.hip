@@ -466,6 +466,68 @@ __global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
     }
 }
 
+// XDG mass matrix (X/MassMatrixFactory.cs:35; the block of cell j and species s is
+// M_mn = int_{K_j cap species s} phi_m phi_n): one workgroup per species volume item;
+// uncut items (the whole cell, orthonormal basis) give the identity, cut items sum
+// w phi_m phi_n over their rule.  out: row r of the full (cell, species) layout holds
+// its block row, out[r * Np + n].
+template <int D>
+__global__ __launch_bounds__(64) void xdg_mass_k(XEvalArgs e, const int* __restrict__ vol, int nvol, double* __restrict__ out) {
+    const XArgs& a = e.g;
+    const int Np = a.Np, NN = Np * Np;
+    for (int v = blockIdx.x; v < nvol; v += gridDim.x) {
+        const XElem E = a.elems[vol[v]];
+        double lo[3], h[3];
+        cell_box<D>(a, E.cellP, lo, h);
+        const int64_t o = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
+        for (int t = threadIdx.x; t < NN; t += blockDim.x) {
+            const int m = t / Np, n = t - m * Np;
+            double s = 0;
+            if (E.rn == 0) {
+                s = m == n ? 1.0 : 0.0;
+            } else {
+                for (int q = 0; q < E.rn; ++q) {
+                    double x[3], w, vm, vn, g[3];
+                    element_node<D>(a, E, q, lo, h, x, &w);
+                    basis_at<D>(a, lo, h, x, m, &vm, g);
+                    basis_at<D>(a, lo, h, x, n, &vn, g);
+                    s += w * vm * vn;
+                }
+            }
+            out[(o + m) * Np + n] = s;
+        }
+    }
+}
+
+void xdg_mass(XdgDevice& X, double* out_dev) {
+    XdgProblem& P = *X.P;
+    std::vector<int> vol;
+    for (size_t i = 0; i < P.elems.size(); ++i)
+        if (P.elems[i].type == 0) vol.push_back((int)i);
+    if (vol.empty()) return;
+    DevBuf dv(vol.size() * sizeof(int));
+    X.ctx->upload(dv.p, vol.data(), vol.size() * sizeof(int));
+    XEvalArgs e{};
+    XArgs& a = e.g;
+    a.D = P.D; a.k = P.k; a.Np = P.Np; a.nq1 = P.k + 1;
+    a.n0 = P.g->n[0]; a.n1 = P.g->n[1]; a.n2 = P.g->n[2];
+    a.nodes1 = X.nodes1.as<double>();
+    a.nstride = X.nstride;
+    a.elems = X.elems.as<XElem>();
+    a.nx = X.nx.as<double>(); a.nw = X.nw.as<double>(); a.nn = X.nn.as<double>();
+    a.ex = X.ex.as<int>();
+    a.gx = X.gx.as<double>(); a.gw = X.gw.as<double>();
+    a.spmask = X.spmask.as<int>();
+    e.off = X.offf.as<int64_t>();
+    BH_HIP(hipMemsetAsync(out_dev, 0, (size_t)P.N_full * P.Np * sizeof(double), X.ctx->stream));
+    const int n = (int)vol.size();
+    const int blocks = n < 65536 ? n : 65536;
+    if (P.D == 2) hipLaunchKernelGGL(xdg_mass_k<2>, dim3(blocks), dim3(64), 0, X.ctx->stream, e, dv.as<int>(), n, out_dev);
+    else hipLaunchKernelGGL(xdg_mass_k<3>, dim3(blocks), dim3(64), 0, X.ctx->stream, e, dv.as<int>(), n, out_dev);
+    BH_HIP(hipGetLastError());
+    X.ctx->sync();
+}
+
 __global__ void xdg_scale_k(double* __restrict__ y, double beta, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = beta == 0.0 ? 0.0 : beta * y[i];

@@ -68,6 +68,8 @@ void qaq_dcell(bh_ctx*, int J, int Np, const int* rsz, const int64_t* doff, cons
 // dst_f[k] = inv[k] >= 0 ? src_c[inv[k]] : 0 (unused frame slots become zero)
 template <typename T> void frame_gather(bh_ctx*, int64_t nc, const int64_t* fwd, const T* src_f, T* dst_c);
 template <typename T> void frame_scatter(bh_ctx*, int64_t nf, const int64_t* inv, const T* src_c, T* dst_f);
+// XDG mass matrix: per (cell, species) block of the full layout, out[row * Np + n] (device)
+void xdg_mass(XdgDevice& X, double* out_dev);
 // y <- beta y + alpha A u on the full (cell, species) DOF layout, matrix-free
 void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y);
 // coarse-space transfer: per cell, the first Npc modes of every species block are

@@ -493,6 +493,11 @@ int bh_xdg_precond_create_v(bh_grid* g, bh_bsr* mat, int Np, int coarse_degree, 
  * (present, non-agglomerated species times Np) and its offset in this library's
  * compressed layout (bh_xdg_matrix's dof_off); any pointer may be NULL. */
 int bh_xdg_dof_frame(bh_xdg* x, int which, int64_t* frame_i0, int* len, int64_t* compressed_i0);
+/* The XDG mass matrix (X/MassMatrixFactory.cs:35), computed on the device: the block of
+ * cell j and species s is M_mn = int over K_j and the species' domain of phi_m phi_n
+ * (the identity for an uncut cell), over the full (cell, species) layout: row r of
+ * that layout holds its block row in out[r * Np .. r * Np + Np) (N_full * Np values). */
+int bh_xdg_mass_matrix(bh_xdg* x, double* out);
 /* No counterpart in the reference: in an FP32 preconditioner, apply the cell
  * diagonal blocks with their FP64 inverse and FP64 accumulation (vectors stay
  * FP32); cut cells that couple two species at high contrast have blocks too

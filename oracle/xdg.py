@@ -560,6 +560,27 @@ class XdgAssembly:
         cols.append(jj.ravel())
         vals.append(M.ravel())
 
+    # ------------------------------------------------------------ mass matrix
+    def mass_blocks(self, elems, nx, nw):
+        """XDG mass matrix (X/MassMatrixFactory.cs:35-120: per cell and species
+        int_{K_j cap A_s} phi_m phi_n on the species' volume rule), as an (N, Np) array:
+        row r of the full layout holds its block row.  Uncut cells use the Gauss k+1
+        tensor rule (the identity up to rounding, the basis being orthonormal)."""
+        D, k, Np = self.g.D, self.k, self.Np
+        out = np.zeros((self.N, Np))
+        for (t, P, Q, sP, sQ, d, side, r0, rn) in elems:
+            if t != 0:
+                continue
+            lo, hi = self.g.box(P)
+            h = hi - lo
+            if rn > 0:
+                X, W = nx[r0:r0 + rn, :D], nw[r0:r0 + rn]
+            else:
+                X, W = _tensor_nodes(D, k, lo, h, 0)
+            v, _ = _phys_basis(D, k, lo, h, X)
+            out[self.dofs(P, sP)] = np.einsum("q,qm,qn->mn", W, v, v)
+        return out
+
     # ------------------------------------------------------------ agglomeration
     def extension(self, src, tgt):
         """E[m, n] = int_src phi^src_m phi^tgt_n (Gauss k+1, exact for the polynomial product)."""

@@ -1,5 +1,9 @@
-"""Shared problem builders for the tests (seeded, reference-style inputs)."""
+"""Shared problem builders and checks for the tests (seeded, reference-style inputs)."""
+import math
+
 import numpy as np
+
+from oracle import xdg as _OX
 
 
 def nodes_2d(nx, ny, stretch=False, x0=0.0, x1=1.0, y0=0.0, y1=1.0):
@@ -40,3 +44,43 @@ def sine_rhs(X, seed, nmodes=8, kmax=6):
             t = t * np.sin(np.pi * w[d] * X[..., d])
         f += a * t
     return f
+
+
+# ---------------------------------------------------------------- XDG mass matrix
+def _moment_coeffs(G, k, j, d, c):
+    """Coefficients of x_d - c_d in the orthonormal basis of cell j (least squares on
+    the Gauss k+1 tensor nodes, exact since the function lies in the space)."""
+    lo, hi = G.box(j)
+    X_, _ = _OX._tensor_nodes(G.D, k, lo, hi - lo, 0)
+    v, _ = _OX._phys_basis(G.D, k, lo, hi - lo, X_)
+    return np.linalg.lstsq(v, X_[:, d] - c[d], rcond=None)[0]
+
+
+def check_mass_blocks(M, G, asm, k, c, r, tol):
+    """Properties the XDG mass matrix must have (M: (N, Np), rows of the full layout):
+    the identity on uncut cells; M_A + M_B = I on cut cells; M_s[0, 0] = species volume /
+    cell volume (closed forms); symmetric positive semi-definite blocks; and the second
+    moments of the ball and of its complement in the unit cube,
+    sum_j U_j^T M_s,j U_j with U_j the coefficients of x_d - c_d:
+    4 pi r^5 / 15 for the ball, ((1 - c_d)^3 + c_d^3) / 3 minus that for the rest."""
+    Np = asm.Np
+    mom = np.zeros((2, G.D))
+    for j in range(G.J):
+        blocks = {}
+        for s in (0, 1):
+            if (G.spmask[j] >> s) & 1:
+                B_ = M[asm.dofs(j, s)]
+                blocks[s] = B_
+                assert np.abs(B_ - B_.T).max() <= 1e-14
+                assert np.linalg.eigvalsh(B_).min() >= -1e-13
+                assert abs(B_[0, 0] - G.vol[s, j] / G.cellvol[j]) <= tol
+                for d in range(G.D):
+                    U = _moment_coeffs(G, k, j, d, c)
+                    mom[s, d] += U @ B_ @ U
+        if len(blocks) == 1:
+            assert np.abs(next(iter(blocks.values())) - np.eye(Np)).max() <= 1e-13
+        else:
+            assert np.abs(blocks[0] + blocks[1] - np.eye(Np)).max() <= tol
+    ball = 4 * math.pi * r ** 5 / 15
+    cube = np.array([((1 - c[d]) ** 3 + c[d] ** 3) / 3 for d in range(G.D)])
+    return mom, ball, cube

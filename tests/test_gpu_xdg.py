@@ -19,6 +19,7 @@ import scipy.sparse.linalg as spla
 
 import bosss_amd as B
 from oracle import xdg as OX
+from tests.helpers import check_mass_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -335,3 +336,18 @@ def test_device_cut_rules_equal_host(ctx, D, n, q):
     assert np.array_equal(cd["spmask"], ch["spmask"])
     assert np.array_equal(cd["vol"], ch["vol"])
     assert np.array_equal(Xd.agglomeration(), Xh.agglomeration())
+
+
+def test_mass_matrix_device(ctx):
+    """XDG mass matrix on the device (X/MassMatrixFactory.cs:35) vs the oracle's on the
+    same cut rules, and the closed-form volumes and ball moments (q = 10)."""
+    _say("case xdg mass matrix D=3 n=10 q=10")
+    c, r = [0.5, 0.47, 0.52], 0.27
+    X, G, asm, tgt, e, L = _setup(ctx, n=10, c=c, r=r, q=10)
+    M = X.mass_matrix()
+    Mo = asm.mass_blocks(e["ints"], e["nx"], e["nw"])
+    assert M.shape == Mo.shape
+    assert np.abs(M - Mo).max() <= 1e-13
+    mom, ball, cube = check_mass_blocks(M, G, asm, 2, np.array(c), r, 1e-11)
+    assert np.allclose(mom[0], ball, rtol=1e-10)
+    assert np.allclose(mom[1], cube - ball, rtol=1e-10)

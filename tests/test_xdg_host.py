@@ -10,6 +10,7 @@ import pytest
 
 import bosss_amd as B
 from oracle import xdg as OX
+from tests.helpers import check_mass_blocks
 
 
 # ---------------------------------------------------------------- oracle pins
@@ -229,3 +230,18 @@ def test_dof_maps():
     assert X.N_full == nsp.sum() * X.Np
     assert X.N == X.N_full - X.npairs * X.Np
     assert X.ncut == int((cp["spmask"] == 3).sum())
+
+
+# ---------------------------------------------------------------- mass matrix
+def test_oracle_mass_blocks_pinned():
+    """The oracle's XDG mass blocks on the host set-up's cut rules (q = 10) against the
+    closed-form volumes and second moments of the ball."""
+    r, c = 0.27, np.array([0.5, 0.47, 0.52])
+    x, g, X = _product(3, 10, c, r, 10)
+    G = OX.Geometry([x] * 3, c, r)
+    e = X.elements()
+    asm = OX.XdgAssembly(G, 2, (1.0, 1000.0), 1.3, X.cells()["len"], e["ints"], e["nx"], e["nw"], e["nn"])
+    M = asm.mass_blocks(e["ints"], e["nx"], e["nw"])
+    mom, ball, cube = check_mass_blocks(M, G, asm, 2, c, r, 1e-11)
+    assert np.allclose(mom[0], ball, rtol=1e-10)
+    assert np.allclose(mom[1], cube - ball, rtol=1e-10)
