@@ -60,6 +60,14 @@ struct SphereRuleItem {
 };
 void sphere_rules_device(bh_ctx* ctx, int D, int q, const Sphere& S, const std::vector<SphereRuleItem>& items,
                          std::vector<QRule>& out);
+// the same items for a polynomial level set (bh_sayerule.hpp on the device, degree <= 4)
+bool poly_rules_on_device(const PolyLS& L);
+void poly_rules_device(bh_ctx* ctx, const PolyLS& L, int q, const std::vector<SphereRuleItem>& items, std::vector<QRule>& out);
+// poly_classify of the boxes [lo, hi] of the items on the device (kind, dir unused)
+void poly_classify_device(bh_ctx* ctx, const PolyLS& L, const std::vector<SphereRuleItem>& items, std::vector<int>& out);
+// Gauss-Legendre points and weights on [0, 1]
+void gauss01(int q, double* t, double* w);
+void poly_fallbacks_add(long n);
 
 // element of the XDG bilinear form (one wave of the assembly kernel each)
 struct XElem {

@@ -158,6 +158,44 @@ void XdgProblem::setup() {
     std::vector<int> vol_rule0[2], vol_rulen[2], if_r0(J, 0), if_rn(J, 0);
     for (int s = 0; s < 2; ++s) { vol_rule0[s].assign(J, 0); vol_rulen[s].assign(J, 0); }
     ncut = 0;
+    // classification of many boxes at once: a polynomial's on the device (degree <= 4),
+    // otherwise on the host threads
+    auto classify_batch = [&](const std::vector<SphereRuleItem>& bx, std::vector<int>& out) {
+        if (ls_kind == 1 && ctx != nullptr && poly_rules_on_device(L)) {
+            poly_classify_device(ctx, L, bx, out);
+            return;
+        }
+        out.assign(bx.size(), 0);
+        parallel_for((int)bx.size(), [&](int i) { out[i] = classify(bx[i].lo, bx[i].hi); });
+    };
+    // the boxes of the faces, in the order the face pass below visits them: per cell,
+    // per direction, the '-' side on the boundary only, then the '+' side
+    auto face_box = [&](int c, int d, int s, double* fl, double* fh) {
+        double lo[3], hi[3];
+        box(c, lo, hi);
+        for (int e = 0; e < 3; ++e) { fl[e] = lo[e]; fh[e] = hi[e]; }
+        fl[d] = fh[d] = s ? hi[d] : lo[d];
+    };
+    std::vector<int> ccl, fcl;
+    {
+        std::vector<SphereRuleItem> bx(J);
+        for (int c = 0; c < J; ++c) box(c, bx[c].lo, bx[c].hi);
+        classify_batch(bx, ccl);
+        bx.clear();
+        for (int c = 0; c < J; ++c) {
+            int idx[3];
+            g->cell_ijk(c, idx);
+            for (int d = 0; d < D; ++d)
+                for (int s = 0; s < 2; ++s) {
+                    const bool bnd = s == 0 ? idx[d] == 0 : idx[d] == g->n[d] - 1;
+                    if (!bnd && s == 0) continue;
+                    SphereRuleItem it{};
+                    face_box(c, d, s, it.lo, it.hi);
+                    bx.push_back(it);
+                }
+        }
+        classify_batch(bx, fcl);
+    }
     // pass 1: classify; candidates get their species and interface rules computed in parallel
     std::vector<int> cand_of(J, -1), cand;
     for (int c = 0; c < J; ++c) {
@@ -165,30 +203,38 @@ void XdgProblem::setup() {
         box(c, lo, hi);
         double V = 1;
         for (int d = 0; d < D; ++d) V *= hi[d] - lo[d];
-        const int cl = classify(lo, hi);
+        const int cl = ccl[c];
         if (cl < 0) { spmask[c] = 1; vol[0][c] = V; }
         else if (cl > 0) { spmask[c] = 2; vol[1][c] = V; }
         else { cand_of[c] = (int)cand.size(); cand.push_back(c); }
     }
-    std::vector<QRule> crule(3 * cand.size());
-    const bool dev_rules = ctx != nullptr && ls_kind == 0;   // sphere rules on the device
+    // rules kind-major (species A volumes, species B volumes, interfaces): on the device
+    // neighbouring threads then run the same kind of construction
+    const size_t nc = cand.size();
+    std::vector<QRule> crule(3 * nc);
+    // cut rules on the device: the sphere's, and a polynomial's of degree <= 4
+    const bool dev_rules = ctx != nullptr && (ls_kind == 0 || poly_rules_on_device(L));
+    auto rules_device = [&](const std::vector<SphereRuleItem>& items, std::vector<QRule>& out) {
+        if (ls_kind == 1) poly_rules_device(ctx, L, q, items, out);
+        else sphere_rules_device(ctx, D, q, S, items, out);
+    };
     if (dev_rules) {
         std::vector<SphereRuleItem> items(3 * cand.size());
         for (size_t i = 0; i < cand.size(); ++i)
             for (int t = 0; t < 3; ++t) {
-                SphereRuleItem& it = items[3 * i + t];
+                SphereRuleItem& it = items[t * nc + i];
                 box(cand[i], it.lo, it.hi);
                 it.kind = t;
                 it.dir = 0;
             }
-        sphere_rules_device(ctx, D, q, S, items, crule);
+        rules_device(items, crule);
     } else {
         parallel_for((int)cand.size(), [&](int i) {
             double lo[3], hi[3];
             box(cand[i], lo, hi);
-            vrule(lo, hi, -1, crule[3 * i]);
-            vrule(lo, hi, +1, crule[3 * i + 1]);
-            irule(lo, hi, crule[3 * i + 2]);
+            vrule(lo, hi, -1, crule[i]);
+            vrule(lo, hi, +1, crule[nc + i]);
+            irule(lo, hi, crule[2 * nc + i]);
         });
     }
     // pass 2, in cell order: species volumes, masks and rule slots
@@ -198,9 +244,9 @@ void XdgProblem::setup() {
         double V = 1;
         for (int d = 0; d < D; ++d) V *= hi[d] - lo[d];
         if (cand_of[c] >= 0) {
-            const QRule& RA = crule[3 * cand_of[c]];
-            const QRule& RB = crule[3 * cand_of[c] + 1];
-            const QRule& RI = crule[3 * cand_of[c] + 2];
+            const QRule& RA = crule[cand_of[c]];
+            const QRule& RB = crule[nc + cand_of[c]];
+            const QRule& RI = crule[2 * nc + cand_of[c]];
             double va = 0, vb = 0, ai = 0;
             for (double w : RA.w) va += w;
             for (double w : RB.w) vb += w;
@@ -228,6 +274,7 @@ void XdgProblem::setup() {
     std::vector<CutFace> cut_faces;
     std::vector<double> edge_sum[2];
     for (int s = 0; s < 2; ++s) edge_sum[s].assign(J, 0.0);
+    size_t fpos = 0;
     for (int c = 0; c < J; ++c) {
         int idx[3];
         g->cell_ijk(c, idx);
@@ -244,12 +291,11 @@ void XdgProblem::setup() {
                 if (bnd) F.b = -1;
                 else { int nb[3] = {idx[0], idx[1], idx[2]}; nb[d]++; F.b = g->cell_index(nb[0], nb[1], nb[2]); }
                 double fl[3], fh[3];
-                for (int e = 0; e < 3; ++e) { fl[e] = lo[e]; fh[e] = hi[e]; }
-                fl[d] = fh[d] = s ? hi[d] : lo[d];
+                face_box(c, d, s, fl, fh);
                 double A = 1;
                 for (int e = 0; e < D; ++e) if (e != d) A *= fh[e] - fl[e];
                 // face cut?
-                const int cl = classify(fl, fh);
+                const int cl = fcl[fpos++];
                 for (int sp = 0; sp < 2; ++sp) { F.area[sp] = 0; F.rn[sp] = 0; F.tensor[sp] = false; }
                 if (cl < 0) { F.area[0] = A; F.tensor[0] = true; }
                 else if (cl > 0) { F.area[1] = A; F.tensor[1] = true; }
@@ -263,28 +309,29 @@ void XdgProblem::setup() {
             }
     }
     // species rules of the cut faces, in parallel, then areas and rule slots in face order
-    std::vector<QRule> frule(2 * cut_faces.size());
+    const size_t nfc = cut_faces.size();   // species-major, as the cell rules
+    std::vector<QRule> frule(2 * nfc);
     if (dev_rules) {
         std::vector<SphereRuleItem> items(2 * cut_faces.size());
         for (size_t i = 0; i < cut_faces.size(); ++i)
             for (int sp = 0; sp < 2; ++sp) {
-                SphereRuleItem& it = items[2 * i + sp];
+                SphereRuleItem& it = items[sp * nfc + i];
                 for (int e = 0; e < 3; ++e) { it.lo[e] = cut_faces[i].lo[e]; it.hi[e] = cut_faces[i].hi[e]; }
                 it.kind = 3 + sp;
                 it.dir = faces[cut_faces[i].face].dir;
             }
-        sphere_rules_device(ctx, D, q, S, items, frule);
+        rules_device(items, frule);
     } else {
         parallel_for((int)cut_faces.size(), [&](int i) {
             const CutFace& cf = cut_faces[i];
             for (int sp = 0; sp < 2; ++sp)
-                frule_of(cf.lo, cf.hi, faces[cf.face].dir, sp == 0 ? -1 : +1, frule[2 * i + sp]);
+                frule_of(cf.lo, cf.hi, faces[cf.face].dir, sp == 0 ? -1 : +1, frule[sp * nfc + i]);
         });
     }
     for (size_t i = 0; i < cut_faces.size(); ++i) {
         FaceInfo& F = faces[cut_faces[i].face];
         for (int sp = 0; sp < 2; ++sp) {
-            const QRule& R = frule[2 * i + sp];
+            const QRule& R = frule[sp * nfc + i];
             double a = 0;
             for (double w : R.w) a += w;
             F.area[sp] = a;

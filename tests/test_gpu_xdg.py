@@ -351,3 +351,61 @@ def test_mass_matrix_device(ctx):
     mom, ball, cube = check_mass_blocks(M, G, asm, 2, np.array(c), r, 1e-11)
     assert np.allclose(mom[0], ball, rtol=1e-10)
     assert np.allclose(mom[1], cube - ball, rtol=1e-10)
+
+
+def _torus_poly(R, rr):
+    S = _sphere_poly(3, [0.5, 0.5, 0.5], 0.0)
+    S[0, 0, 0] += R * R - rr * rr
+    P = _sphere_poly(3, [0.5, 0.5, 0.0], 0.0)
+    P[0, 0, 2] -= 1.0
+    C = np.zeros((5, 5, 5))
+    for ia in np.ndindex(S.shape):
+        for ib in np.ndindex(S.shape):
+            C[tuple(ia[d] + ib[d] for d in range(3))] += S[ia] * S[ib]
+    C[:3, :3, :3] -= 4 * R * R * P
+    return C
+
+
+def _superellipse_poly(s):
+    C = np.zeros((5, 5))      # (x - 1/2)^4 + (y - 1/2)^4 - s^4
+    for i, v in enumerate([1 / 16, -1 / 2, 3 / 2, -2, 1]):
+        C[i, 0] += v
+        C[0, i] += v
+    C[0, 0] -= s ** 4
+    return C
+
+
+@pytest.mark.parametrize("case", ["sphere3d", "torus", "plane", "superellipse2d"])
+def test_device_poly_rules_equal_host(ctx, case):
+    """Polynomial level sets: the cut-cell, interface and cut-face rules built on the
+    device (Saye's algorithm, one thread per rule, bh_sayerule.hpp compiled without
+    multiply-add contraction) equal the host set-up's bit for bit, including the boxes
+    that need halvings (the superellipse's flat sides, the torus)."""
+    if case == "sphere3d":
+        D, n, q, C = 3, 10, 10, _sphere_poly(3, [0.5, 0.47, 0.52], 0.27)
+    elif case == "torus":
+        D, n, q, C = 3, 16, 8, _torus_poly(0.3, 0.12)
+    elif case == "plane":
+        D, n, q = 3, 10, 4
+        C = np.zeros((2, 2, 2))
+        C[0, 0, 1], C[0, 0, 0], C[1, 0, 0], C[0, 1, 0] = 1.0, -0.3, -0.2, -0.1
+    else:
+        D, n, q, C = 2, 20, 12, _superellipse_poly(0.3)
+    _say(f"case device polynomial cut rules {case} D={D} n={n} q={q}")
+    x = np.linspace(0, 1, n + 1)
+    f0 = B.xdg_cut_rule_fallbacks()
+    Xd = B.XdgPoisson.poly(B.Grid(ctx, *([x] * D)), 2, C, NU[0], NU[1], agg_threshold=0.1, quad_order=q)
+    f1 = B.xdg_cut_rule_fallbacks()
+    Xh = B.XdgPoisson.poly(B.Grid(None, *([x] * D)), 2, C, NU[0], NU[1], agg_threshold=0.1, quad_order=q)
+    f2 = B.xdg_cut_rule_fallbacks()
+    ed, eh = Xd.elements(), Xh.elements()
+    assert Xd.ncut == Xh.ncut > 0
+    assert np.array_equal(ed["ints"], eh["ints"])
+    assert len(ed["nw"]) == len(eh["nw"]) > 0
+    for key in ("nx", "nw", "nn", "dbls"):
+        assert np.array_equal(ed[key], eh[key]), key
+    cd, ch = Xd.cells(), Xh.cells()
+    for key in ("spmask", "vol", "iface_area"):
+        assert np.array_equal(cd[key], ch[key]), key
+    assert np.array_equal(Xd.agglomeration(), Xh.agglomeration())
+    assert f1 - f0 == f2 - f1
