@@ -436,7 +436,9 @@ int bh_xdg_create(bh_grid* grid, int k, const double* center, double radius, dou
  * decide which boxes are cut and in which directions phi is monotone, the roots are
  * found by a Newton iteration safeguarded by bisection, boxes without a monotone
  * direction are halved (up to 10 times, then a low-order rule is used and counted,
- * bh_xdg_cut_rule_fallbacks); deg <= 8.  The level set is one polynomial on the
+ * bh_xdg_cut_rule_fallbacks); deg <= 8.  On a grid with a device, deg <= 4 runs
+ * the classification and the rules on the device (one thread per rule, the same rules
+ * bit for bit as the host's).  The level set is one polynomial on the
  * whole grid, so both sides of every face see the same phi; BoSSS's per-cell DG
  * level-set field is not taken in this form. */
 int bh_xdg_create_poly(bh_grid* g, int k, int deg, const double* coeffs, double nu_in, double nu_out,
