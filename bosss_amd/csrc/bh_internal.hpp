@@ -132,6 +132,33 @@ __host__ __device__ constexpr int mode_exponent(int D, int k, int n, int d) {
     }
     return 0;
 }
+__host__ __device__ constexpr int binom(int n, int k) {
+    int r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+// The modal form of an axis-aligned face between cells of a tensor grid: the trace of
+// mode n on a face of direction d is p_j(+-1) times the face's tangential mode, with
+// j = the exponent of n in d; the modes with the same tangential exponents (a face mode)
+// are addressed by their root, the mode with j = 0.
+__host__ __device__ constexpr int face_root(int D, int k, int n, int d) {
+    const int Np = binom(k + D, D);
+    for (int m = 0; m < Np; ++m) {
+        if (mode_exponent(D, k, m, d) != 0) continue;
+        bool same = true;
+        for (int dd = 0; dd < D; ++dd)
+            if (dd != d && mode_exponent(D, k, m, dd) != mode_exponent(D, k, n, dd)) same = false;
+        if (same) return m;
+    }
+    return -1;
+}
+// s_j = sqrt(j + 1/2), correctly rounded
+__host__ __device__ constexpr double sqrt_half(int j) {
+    return j == 0 ? 0.7071067811865476 : j == 1 ? 1.224744871391589 : j == 2 ? 1.5811388300841898
+         : j == 3 ? 1.8708286933869707 : j == 4 ? 2.1213203435596424 : j == 5 ? 2.345207879911715
+         : 2.5495097567963922;
+}
+
 }  // namespace bh
 
 // ------------------------------------------------------------------ device helpers

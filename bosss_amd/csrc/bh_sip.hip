@@ -48,11 +48,6 @@ struct SipArgs {
     const R* __restrict__ b0;   // if set: y = b0 + alpha * (A u)  (residual form; beta ignored)
 };
 
-constexpr int binom(int n, int k) {
-    int r = 1;
-    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
-    return r;
-}
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
 template <int D>
@@ -251,23 +246,6 @@ __device__ __forceinline__ void face_pair(const SipArgs<R>& a, const R* uP, cons
 // The result equals the quadrature up to rounding, with about 4 Np multiply-adds
 // per side instead of 4 Np per face node.  A face mode is addressed by its root:
 // the mode with the same e' and j = 0.
-__host__ __device__ constexpr int face_root(int D, int k, int n, int d) {
-    const int Np = binom(k + D, D);
-    for (int m = 0; m < Np; ++m) {
-        if (mode_exponent(D, k, m, d) != 0) continue;
-        bool same = true;
-        for (int dd = 0; dd < D; ++dd)
-            if (dd != d && mode_exponent(D, k, m, dd) != mode_exponent(D, k, n, dd)) same = false;
-        if (same) return m;
-    }
-    return -1;
-}
-// s_j = sqrt(j + 1/2), correctly rounded
-__host__ __device__ constexpr double sqrt_half(int j) {
-    return j == 0 ? 0.7071067811865476 : j == 1 ? 1.224744871391589 : j == 2 ? 1.5811388300841898
-         : j == 3 ? 1.8708286933869707 : j == 4 ? 2.1213203435596424 : j == 5 ? 2.345207879911715
-         : 2.5495097567963922;
-}
 
 // The modal face terms of one side of direction d: s = 0 is the '-' face (neighbour
 // M, its '+' face), s = 1 the '+' face (neighbour Q, its '-' face).  The coefficients

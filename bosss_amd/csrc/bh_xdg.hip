@@ -15,9 +15,9 @@
 // trial) entries of the element matrix; the element matrix is added to the global
 // blocks with plain adds: the elements run in colour groups (XDG_COLOURS) within
 // which no two touch the same entry, one launch per group in a fixed order, so the
-// result is bitwise reproducible.  The matrix-free evaluate is node-parallel
-// (one quadrature node per lane, several uncut elements per wave; see
-// xdg_evaluate_kernel) over the same colour groups.
+// result is bitwise reproducible.  The evaluate runs over the same colour groups:
+// uncut items in closed modal form (xdg_modal_kernel), cut items from the element
+// matrices the assembly kept (xdg_emat_kernel).
 #include <algorithm>
 #include <cmath>
 
@@ -48,6 +48,8 @@ struct XArgs {
     const int64_t* __restrict__ voff;
     const int* __restrict__ bsz;
     double* __restrict__ vals;
+    double* __restrict__ emat;          // element matrices of the cut items (see XdgDevice)
+    const int64_t* __restrict__ eoff;
 };
 
 template <int D>
@@ -221,6 +223,16 @@ __global__ __launch_bounds__(64) void xdg_assemble_kernel(XArgs a) {
             }
             __syncthreads();
         }
+        // cut items: keep the element matrix for the evaluate
+        if (E.rn > 0 && a.emat) {
+            double* em = a.emat + a.eoff[a.perm[a.start + ei]];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const int t = lane + 64 * j;
+                if (t >= nent) break;
+                em[t] = acc[j];
+            }
+        }
         // scatter into the global blocks
 #pragma unroll
         for (int j = 0; j < 7; ++j) {
@@ -283,187 +295,28 @@ struct XEvalArgs {
     double alpha;
 };
 
-// y += alpha * A u element by element (XEvaluatorNonlin, X/XSpatialOperatorMk2_XEvaluators.cs:467-646, for
-// the linear XLaplace forms), node-parallel: a wave is split into segments of S lanes, one element per
-// segment, one quadrature node per lane (the segment loops when the element has more nodes).  At its node a
-// lane evaluates the orthonormal basis of both sides (1D Legendre tables, products over the exponents),
-// forms the traces of u (coefficients in LDS), and accumulates w * (form of test function m) for every m in
-// registers; a butterfly over the segment's lanes sums the nodes (every lane ends with the same bits), and
-// one plain add per (element, test function) writes the result.  Colour groups as in the assembly: no two
-// elements of a launch touch the same entries, so the result is deterministic.  TYPE: 0 volume, 1 interior
-// face, 2 boundary face, 3 interface; all elements of a colour group have one type.
-template <int D, int K>
-struct Leg1 {
-    double P[D][K + 1], dP[D][K + 1];
-    double sc;   // 1 / sqrt(det J)
-    double g2[D];   // 2 / h_d
-    __device__ __forceinline__ void at(const double* lo, const double* h, const double* x) {
-        double detJ = 1;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const double xi = 2.0 * (x[d] - lo[d]) / h[d] - 1.0;
-            detJ *= 0.5 * h[d];
-            g2[d] = 2.0 / h[d];
-            // orthonormal Legendre sqrt(n + 1/2) P_n and derivatives, n <= 3
-            P[d][0] = 0.7071067811865476;
-            dP[d][0] = 0.0;
-            if constexpr (K >= 1) { P[d][1] = 1.224744871391589 * xi; dP[d][1] = 1.224744871391589; }
-            if constexpr (K >= 2) { P[d][2] = 1.5811388300841898 * (1.5 * xi * xi - 0.5); dP[d][2] = 1.5811388300841898 * 3.0 * xi; }
-            if constexpr (K >= 3) {
-                P[d][3] = 1.8708286933869707 * (2.5 * xi * xi - 1.5) * xi;
-                dP[d][3] = 1.8708286933869707 * (7.5 * xi * xi - 1.5);
-            }
-        }
-        sc = 1.0 / sqrt(detJ);
-    }
-    // value and gradient of mode n (exponents known at compile time: the tables stay in registers)
-    template <int n>
-    __device__ __forceinline__ void mode(double& v, double* g) const {
-        double pv[D], dv[D];
-        static_for<D>([&](auto DD) {
-            constexpr int d = DD;
-            constexpr int ed = mode_exponent(D, K, n, d);
-            pv[d] = P[d][ed];
-            dv[d] = dP[d][ed];
-        });
-        double prod = sc;
-#pragma unroll
-        for (int d = 0; d < D; ++d) prod *= pv[d];
-        v = prod;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            double gg = sc * g2[d];
-#pragma unroll
-            for (int dd = 0; dd < D; ++dd) gg *= dd == d ? dv[dd] : pv[dd];
-            g[d] = gg;
-        }
-    }
-};
-
-// DIR: the face direction of a colour group of axis-aligned faces (TYPE 1, 2), known at
-// compile time so that only the normal derivative's one direction is formed; -1 otherwise.
-template <int D, int K, int TYPE, int S, int DIR>
-__global__ __launch_bounds__(64) void xdg_evaluate_kernel(XEvalArgs e) {
-    constexpr int Np = D == 2 ? (K + 1) * (K + 2) / 2 : (K + 1) * (K + 2) * (K + 3) / 6;
-    constexpr int NSEG = 64 / S;
-    constexpr bool TWO = TYPE == 1 || TYPE == 3;
-    constexpr int NACC = TWO ? 2 * Np : Np;
+// Cut items of the evaluate: y += alpha M u with the element matrix M that the assembly
+// kept (the same values it adds into the global blocks), one lane per (item, test row):
+// a row of the test side's two blocks (trial P, trial Q) against both sides' coefficients.
+// Colour groups as in the assembly: no two items of a launch touch the same rows.
+template <bool TWO>
+__global__ __launch_bounds__(64) void xdg_emat_kernel(XEvalArgs e) {
     const XArgs& a = e.g;
-    __shared__ double su[NSEG][2][Np];
-    const int lane = threadIdx.x, sg = lane / S, j = lane - sg * S;
-    for (int base = blockIdx.x * NSEG; base < a.count; base += gridDim.x * NSEG) {
-        const int ei = base + sg;
-        const bool valid = ei < a.count;
-        XElem E{};
-        double loP[3] = {0, 0, 0}, hP[3] = {1, 1, 1}, loQ[3] = {0, 0, 0}, hQ[3] = {1, 1, 1};
-        int64_t oP = 0, oQ = 0;
-        int nnodes = 0;
-        if (valid) {
-            E = a.elems[a.perm[a.start + ei]];
-            cell_box<D>(a, E.cellP, loP, hP);
-            cell_box<D>(a, E.cellQ, loQ, hQ);
-            oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
-            oQ = e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np;
-            if (E.rn > 0) nnodes = E.rn;
-            else {
-                nnodes = 1;
-                for (int d = 0; d < (TYPE == 0 ? D : D - 1); ++d) nnodes *= a.nq1;
-            }
-            for (int t = j; t < 2 * Np; t += S)
-                su[sg][t / Np][t % Np] = t < Np ? e.u[oP + t] : (TWO ? e.u[oQ + t - Np] : 0.0);
-        }
-        __syncthreads();
-        double acc[NACC];
-#pragma unroll
-        for (int m = 0; m < NACC; ++m) acc[m] = 0.0;
-        const double fsgn = TYPE == 1 ? 1.0 : (E.side ? 1.0 : -1.0);
-        for (int q = j; q < nnodes; q += S) {
-            double x[3], w;
-            element_node<D>(a, E, q, loP, hP, x, &w);
-            double nrm[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                nrm[d] = TYPE == 3 ? a.nn[(size_t)(E.r0 + q) * 3 + d]
-                                   : (TYPE != 0 && d == (DIR >= 0 ? DIR : E.dir) ? fsgn : 0.0);
-            Leg1<D, K> LP, LQ;
-            LP.at(loP, hP, x);
-            if constexpr (TWO) LQ.at(loQ, hQ, x);
-            // traces of u
-            double uP = 0, dPn = 0, uQ = 0, dQn = 0, gu[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) gu[d] = 0;
-            static_for<Np>([&](auto NN) {
-                constexpr int n = NN;
-                double v, g[D];
-                LP.template mode<n>(v, g);
-                const double c = su[sg][0][n];
-                if constexpr (TYPE == 0) {
-#pragma unroll
-                    for (int d = 0; d < D; ++d) gu[d] += g[d] * c;
-                } else {
-                    double dn = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
-                    uP += v * c;
-                    dPn += dn * c;
-                }
-                if constexpr (TWO) {
-                    double vq, gq[D];
-                    LQ.template mode<n>(vq, gq);
-                    double dnq = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) dnq += gq[d] * nrm[d];
-                    const double cq = su[sg][1][n];
-                    uQ += vq * cq;
-                    dQn += dnq * cq;
-                }
-            });
-            // test functions
-            const double jump = uP - uQ, flux = 0.5 * (E.nuP * dPn + E.nuQ * dQn);
-            static_for<Np>([&](auto MM) {
-                constexpr int m = MM;
-                double v, g[D];
-                LP.template mode<m>(v, g);
-                if constexpr (TYPE == 0) {
-                    double sgu = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) sgu += g[d] * gu[d];
-                    acc[m] += w * (-E.nuP * sgu);
-                } else {
-                    double dn = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) dn += g[d] * nrm[d];
-                    if constexpr (TYPE == 2)
-                        acc[m] += w * (E.nuP * dPn * v + E.nuP * dn * uP - E.nuP * E.mu * uP * v);
-                    else
-                        acc[m] += w * (flux * v + 0.5 * E.nuP * dn * jump - E.mu * jump * v);
-                }
-                if constexpr (TWO) {
-                    double vq, gq[D];
-                    LQ.template mode<m>(vq, gq);
-                    double dnq = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) dnq += gq[d] * nrm[d];
-                    acc[Np + m] += w * (-flux * vq + 0.5 * E.nuQ * dnq * jump + E.mu * jump * vq);
-                }
-            });
-        }
-        // sum over the segment's nodes (butterfly: every lane of the segment ends with the same sums)
-#pragma unroll
-        for (int m = 0; m < NACC; ++m) {
-#pragma unroll
-            for (int off = S / 2; off > 0; off >>= 1) acc[m] += __shfl_xor(acc[m], off, 64);
-        }
-        if (valid) {
-#pragma unroll
-            for (int m = 0; m < NACC; ++m)
-                if (j == m % S) {
-                    if (m < Np) e.y[oP + m] += e.alpha * acc[m];
-                    else e.y[oQ + m - Np] += e.alpha * acc[m];
-                }
-        }
-        __syncthreads();   // su is reused by the next element of the segment
-    }
+    const int Np = a.Np, R = TWO ? 2 * Np : Np;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.count * R) return;
+    const int ei = (int)(t / R), r = (int)(t - (int64_t)ei * R);
+    const int eg = a.perm[a.start + ei];
+    const XElem E = a.elems[eg];
+    const int side = TWO ? r / Np : 0, m = r - side * Np;
+    const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
+    const int64_t oQ = TWO ? e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np : oP;
+    const double* M = a.emat + a.eoff[eg] + (int64_t)(2 * side) * Np * Np + (int64_t)m * Np;
+    double s = 0;
+    for (int n = 0; n < Np; ++n) s += M[n] * e.u[oP + n];
+    if (TWO)
+        for (int n = 0; n < Np; ++n) s += M[Np * Np + n] * e.u[oQ + n];
+    e.y[(side ? oQ : oP) + m] += e.alpha * s;
 }
 
 // XDG mass matrix (X/MassMatrixFactory.cs:35; the block of cell j and species s is
@@ -1012,6 +865,21 @@ void xdg_assemble(XdgDevice& X, Vbsr& M) {
     a.rowptr = M.rowptr_d.as<int64_t>(); a.col = M.col_d.as<int>(); a.voff = M.voff_d.as<int64_t>(); a.bsz = M.rsz_d.as<int>();
     a.vals = M.vals.as<double>();
     BH_CHECK(P.Np <= XMAXNP && 4 * P.Np * P.Np <= 7 * 64, "XDG assembly: degree too high");
+    {
+        std::vector<int64_t> off(P.elems.size(), -1);
+        int64_t tot = 0;
+        for (size_t i = 0; i < P.elems.size(); ++i) {
+            const XElem& E = P.elems[i];
+            if (E.rn == 0) continue;
+            off[i] = tot;
+            tot += (int64_t)(E.type == 1 || E.type == 3 ? 4 : 1) * P.Np * P.Np;
+        }
+        X.emat.alloc(std::max<int64_t>(tot, 1) * sizeof(double));
+        X.eoff.alloc(std::max<size_t>(off.size(), 1) * sizeof(int64_t));
+        X.ctx->upload(X.eoff.p, off.data(), off.size() * sizeof(int64_t));
+    }
+    a.emat = X.emat.as<double>();
+    a.eoff = X.eoff.as<int64_t>();
     a.perm = X.perm.as<int>();
     for (int c = 0; c < XDG_COLOURS; ++c) {
         a.start = X.colour_off[c];
@@ -1024,20 +892,148 @@ void xdg_assemble(XdgDevice& X, Vbsr& M) {
     }
 }
 
-constexpr int pow_i(int b, int n) { return n == 0 ? 1 : b * pow_i(b, n - 1); }
-constexpr int seg_lanes(int nodes) { return nodes <= 8 ? 8 : nodes <= 16 ? 16 : nodes <= 32 ? 32 : 64; }
+// Uncut items (tensor rules: the whole cell of one species, or a whole face of one species)
+// in modal form, one lane per item: on a Cartesian cell the orthonormal basis makes the
+// volume term -nu sum_d (2/h_d)^2 S_d u (S_d the 1D Legendre stiffness in direction d,
+// S_ij = sqrt((i + 1/2)(j + 1/2)) min(i,j)(min(i,j) + 1) for i + j even), and a face of
+// direction d couples only modes with the same tangential exponents, through the traces
+// A = sum_j p_j(+-1) u_j / sqrt(h_d / 2) and B = (2 / h_d) sum_j p_j'(+-1) u_j / sqrt(h_d / 2)
+// of each face mode (the face's tangential basis is orthonormal).  The forms are the
+// assembly kernel's forms (XLaplace_Bulk, SIPLaplace boundary), integrated exactly.
+template <int D, int K>
+__host__ __device__ constexpr bool same_tangential(int m, int n, int d) {
+    for (int dd = 0; dd < D; ++dd)
+        if (dd != d && mode_exponent(D, K, m, dd) != mode_exponent(D, K, n, dd)) return false;
+    return true;
+}
+__host__ __device__ constexpr double leg_stiff(int i, int j) {
+    const int m = i < j ? i : j;
+    return ((i + j) & 1) ? 0.0 : sqrt_half(i) * sqrt_half(j) * (double)(m * (m + 1));
+}
+// p_j(+1), p_j(-1), p_j'(+1), p_j'(-1) of the orthonormal Legendre polynomial p_j
+__host__ __device__ constexpr double leg_val(int j, int side) { return side ? sqrt_half(j) : ((j & 1) ? -sqrt_half(j) : sqrt_half(j)); }
+__host__ __device__ constexpr double leg_der(int j, int side) {
+    return (side ? 1.0 : ((j & 1) ? 1.0 : -1.0)) * sqrt_half(j) * (double)(j * (j + 1) / 2);
+}
 
 template <int D, int K, int TYPE, int DIR>
-static void xdg_eval_type(hipStream_t st, bool cut, const XEvalArgs& e) {
-    constexpr int S_T = seg_lanes(pow_i(K + 1, TYPE == 0 ? D : D - 1));
-    if (cut || TYPE == 3) {
-        const int blocks = e.g.count < 65536 ? e.g.count : 65536;
-        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, 64, DIR>), dim3(blocks), dim3(64), 0, st, e);
+__global__ __launch_bounds__(64) void xdg_modal_kernel(XEvalArgs e) {
+    constexpr int Np = binom(K + D, D);
+    const XArgs& a = e.g;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.count) return;
+    const XElem E = a.elems[a.perm[a.start + t]];
+    double loP[3], hP[3];
+    cell_box<D>(a, E.cellP, loP, hP);
+    const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
+    double uP[Np], yP[Np];
+#pragma unroll
+    for (int n = 0; n < Np; ++n) { uP[n] = e.u[oP + n]; yP[n] = 0.0; }
+    if constexpr (TYPE == 0) {
+        double s2[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) s2[d] = -E.nuP * 4.0 / (hP[d] * hP[d]);
+        static_for<Np>([&](auto MM) {
+            constexpr int m = MM;
+            static_for<Np>([&](auto NN) {
+                constexpr int n = NN;
+                static_for<D>([&](auto DD) {
+                    constexpr int d = DD;
+                    constexpr int i = mode_exponent(D, K, m, d), j = mode_exponent(D, K, n, d);
+                    if constexpr (same_tangential<D, K>(m, n, d) && i > 0 && j > 0 && leg_stiff(i, j) != 0.0)
+                        yP[m] += s2[d] * (leg_stiff(i, j) * uP[n]);
+                });
+            });
+        });
     } else {
-        constexpr int NSEG = 64 / S_T;
-        const int nb = (e.g.count + NSEG - 1) / NSEG;
-        const int blocks = nb < 65536 ? nb : 65536;
-        hipLaunchKernelGGL((xdg_evaluate_kernel<D, K, TYPE, S_T, DIR>), dim3(blocks), dim3(64), 0, st, e);
+        constexpr int d = DIR;
+        const double sP = 1.0 / sqrt(0.5 * hP[d]), gP = 2.0 / hP[d];
+        if constexpr (TYPE == 2) {
+            // boundary face: side 1 at x_d = hi (outward +e_d), side 0 at x_d = lo (outward -e_d)
+            const int side = E.side;
+            const double fsgn = side ? 1.0 : -1.0;
+            static_for<Np>([&](auto FF) {
+                constexpr int f = FF;
+                if constexpr (mode_exponent(D, K, f, d) == 0) {
+                    double A = 0, B = 0;
+                    static_for<Np>([&](auto NN) {
+                        constexpr int n = NN;
+                        if constexpr (face_root(D, K, n, d) == f) {
+                            constexpr int j = mode_exponent(D, K, n, d);
+                            A += (side ? leg_val(j, 1) : leg_val(j, 0)) * uP[n];
+                            B += (side ? leg_der(j, 1) : leg_der(j, 0)) * uP[n];
+                        }
+                    });
+                    A *= sP;
+                    const double dn = fsgn * gP * sP * B;
+                    const double CV = E.nuP * dn - E.nuP * E.mu * A, CD = E.nuP * A * fsgn * gP * sP;
+                    static_for<Np>([&](auto MM) {
+                        constexpr int m = MM;
+                        if constexpr (face_root(D, K, m, d) == f) {
+                            constexpr int j = mode_exponent(D, K, m, d);
+                            yP[m] += CV * sP * (side ? leg_val(j, 1) : leg_val(j, 0)) + CD * (side ? leg_der(j, 1) : leg_der(j, 0));
+                        }
+                    });
+                }
+            });
+        } else {
+            // interior face from P (x_d = hi of P) to Q (x_d = lo of Q), normal +e_d
+            double loQ[3], hQ[3];
+            cell_box<D>(a, E.cellQ, loQ, hQ);
+            const int64_t oQ = e.off[E.cellQ] + (int64_t)spos(a, E.cellQ, E.spQ) * Np;
+            const double sQ = 1.0 / sqrt(0.5 * hQ[d]), gQ = 2.0 / hQ[d];
+            double uQ[Np], yQ[Np];
+#pragma unroll
+            for (int n = 0; n < Np; ++n) { uQ[n] = e.u[oQ + n]; yQ[n] = 0.0; }
+            static_for<Np>([&](auto FF) {
+                constexpr int f = FF;
+                if constexpr (mode_exponent(D, K, f, d) == 0) {
+                    double AP = 0, BP = 0, AQ = 0, BQ = 0;
+                    static_for<Np>([&](auto NN) {
+                        constexpr int n = NN;
+                        if constexpr (face_root(D, K, n, d) == f) {
+                            constexpr int j = mode_exponent(D, K, n, d);
+                            AP += leg_val(j, 1) * uP[n];
+                            BP += leg_der(j, 1) * uP[n];
+                            AQ += leg_val(j, 0) * uQ[n];
+                            BQ += leg_der(j, 0) * uQ[n];
+                        }
+                    });
+                    AP *= sP;
+                    BP *= gP * sP;
+                    AQ *= sQ;
+                    BQ *= gQ * sQ;
+                    const double jump = AP - AQ, flux = 0.5 * (E.nuP * BP + E.nuQ * BQ);
+                    const double CVP = (flux - E.mu * jump) * sP, CDP = 0.5 * E.nuP * jump * gP * sP;
+                    const double CVQ = (E.mu * jump - flux) * sQ, CDQ = 0.5 * E.nuQ * jump * gQ * sQ;
+                    static_for<Np>([&](auto MM) {
+                        constexpr int m = MM;
+                        if constexpr (face_root(D, K, m, d) == f) {
+                            constexpr int j = mode_exponent(D, K, m, d);
+                            yP[m] += CVP * leg_val(j, 1) + CDP * leg_der(j, 1);
+                            yQ[m] += CVQ * leg_val(j, 0) + CDQ * leg_der(j, 0);
+                        }
+                    });
+                }
+            });
+#pragma unroll
+            for (int m = 0; m < Np; ++m) e.y[oQ + m] += e.alpha * yQ[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < Np; ++m) e.y[oP + m] += e.alpha * yP[m];
+}
+
+
+// cut items (own rules) from their element matrices; uncut items modal, one lane each
+template <int D, int K, int TYPE, int DIR>
+static void xdg_eval_type(hipStream_t st, bool cut, const XEvalArgs& e) {
+    if (cut || TYPE == 3) {
+        constexpr bool TWO = TYPE == 1 || TYPE == 3;
+        const int64_t n = (int64_t)e.g.count * (TWO ? 2 : 1) * e.g.Np;
+        hipLaunchKernelGGL((xdg_emat_kernel<TWO>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, e);
+    } else {
+        hipLaunchKernelGGL((xdg_modal_kernel<D, K, TYPE, DIR>), dim3((unsigned)((e.g.count + 63) / 64)), dim3(64), 0, st, e);
     }
     BH_HIP(hipGetLastError());
 }
@@ -1087,6 +1083,8 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
     a.ex = X.ex.as<int>();
     a.gx = X.gx.as<double>(); a.gw = X.gw.as<double>();
     a.spmask = X.spmask.as<int>();
+    a.emat = X.emat.as<double>();
+    a.eoff = X.eoff.as<int64_t>();
     e.off = X.offf.as<int64_t>();
     e.u = u;
     e.y = y;
@@ -1094,8 +1092,7 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
     BH_CHECK(P.k >= 1 && P.k <= 3 && (P.D == 2 || P.D == 3), "XDG evaluate: degrees 1 to 3 in 2D or 3D");
     a.perm = X.perm.as<int>();
     for (int c = 0; c < XDG_COLOURS; ++c) {
-        // the colour's cut items (own rules, many nodes) with whole waves, then its uncut items
-        // (tensor rules) with segments of the next power of two above the tensor node count
+        // the colour's cut items (own rules, many nodes), then its uncut items (tensor rules)
         for (int part = 0; part < 2; ++part) {
             a.start = part == 0 ? X.colour_off[c] : X.cut_end[c];
             a.count = (part == 0 ? X.cut_end[c] : X.colour_off[c + 1]) - a.start;

@@ -53,6 +53,10 @@ struct XdgDevice {
     DevBuf perm;                            // element indices grouped by colour (cut items first in each)
     std::vector<int> colour_off;            // [XDG_COLOURS + 1] offsets into perm
     std::vector<int> cut_end;               // [XDG_COLOURS] end of each colour's cut items
+    // element matrices of the cut items (their own rules), kept by the assembly for the
+    // evaluate: eoff[element] = offset into emat (-1 for uncut items); a face or interface
+    // item holds 4 Np x Np blocks (test P / Q x trial P / Q), a volume item one
+    DevBuf emat, eoff;
     void upload();
 };
 
