@@ -300,13 +300,13 @@ struct XEvalArgs {
 // a row of the test side's two blocks (trial P, trial Q) against both sides' coefficients.
 // Colour groups as in the assembly: no two items of a launch touch the same rows.
 template <bool TWO>
-__global__ __launch_bounds__(64) void xdg_emat_kernel(XEvalArgs e) {
+__device__ __forceinline__ void xdg_emat_item(const XEvalArgs& e, int start, int count, int64_t t) {
     const XArgs& a = e.g;
     const int Np = a.Np, R = TWO ? 2 * Np : Np;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)a.count * R) return;
+    if (t >= (int64_t)count * R) return;
     const int ei = (int)(t / R), r = (int)(t - (int64_t)ei * R);
-    const int eg = a.perm[a.start + ei];
+    const int eg = a.perm[start + ei];
+    if (a.eoff[eg] < 0) return;   // every cut item has a kept matrix; nothing else reads one
     const XElem E = a.elems[eg];
     const int side = TWO ? r / Np : 0, m = r - side * Np;
     const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
@@ -917,12 +917,11 @@ __host__ __device__ constexpr double leg_der(int j, int side) {
 }
 
 template <int D, int K, int TYPE, int DIR>
-__global__ __launch_bounds__(64) void xdg_modal_kernel(XEvalArgs e) {
+__device__ __forceinline__ void xdg_modal_item(const XEvalArgs& e, int start, int count, int t) {
     constexpr int Np = binom(K + D, D);
     const XArgs& a = e.g;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.count) return;
-    const XElem E = a.elems[a.perm[a.start + t]];
+    if (t >= count) return;
+    const XElem E = a.elems[a.perm[start + t]];
     double loP[3], hP[3];
     cell_box<D>(a, E.cellP, loP, hP);
     const int64_t oP = e.off[E.cellP] + (int64_t)spos(a, E.cellP, E.spP) * Np;
@@ -1024,46 +1023,61 @@ __global__ __launch_bounds__(64) void xdg_modal_kernel(XEvalArgs e) {
     for (int m = 0; m < Np; ++m) e.y[oP + m] += e.alpha * yP[m];
 }
 
-
-// cut items (own rules) from their element matrices; uncut items modal, one lane each
+// One colour group per launch: its cut items (first nbc workgroups, a lane per item row,
+// from the kept element matrices) and its uncut items (the other workgroups, a lane per
+// item, modal); the group's items touch disjoint rows, so both parts run together.
 template <int D, int K, int TYPE, int DIR>
-static void xdg_eval_type(hipStream_t st, bool cut, const XEvalArgs& e) {
-    if (cut || TYPE == 3) {
-        constexpr bool TWO = TYPE == 1 || TYPE == 3;
-        const int64_t n = (int64_t)e.g.count * (TWO ? 2 : 1) * e.g.Np;
-        hipLaunchKernelGGL((xdg_emat_kernel<TWO>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, e);
-    } else {
-        hipLaunchKernelGGL((xdg_modal_kernel<D, K, TYPE, DIR>), dim3((unsigned)((e.g.count + 63) / 64)), dim3(64), 0, st, e);
+__global__ __launch_bounds__(64) void xdg_colour_kernel(XEvalArgs e, int cstart, int ccount, int ustart, int ucount, int nbc) {
+    if ((int)blockIdx.x < nbc) {
+        xdg_emat_item<TYPE == 1 || TYPE == 3>(e, cstart, ccount, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    } else if constexpr (TYPE != 3) {
+        xdg_modal_item<D, K, TYPE, DIR>(e, ustart, ucount, ((int)blockIdx.x - nbc) * (int)blockDim.x + (int)threadIdx.x);
     }
+}
+
+
+// one launch per colour group: cut items from their element matrices, uncut items modal
+struct ColourRange {
+    int cstart, ccount, ustart, ucount;
+};
+template <int D, int K, int TYPE, int DIR>
+static void xdg_eval_type(hipStream_t st, const ColourRange& cr, const XEvalArgs& e) {
+    constexpr bool TWO = TYPE == 1 || TYPE == 3;
+    const int64_t nc = (int64_t)cr.ccount * (TWO ? 2 : 1) * e.g.Np;
+    const int64_t nbc = (nc + 63) / 64, nbu = TYPE == 3 ? 0 : ((int64_t)cr.ucount + 63) / 64;
+    BH_CHECK(nbc + nbu < ((int64_t)1 << 31), "XDG evaluate: colour group too large");
+    if (nbc + nbu == 0) return;
+    hipLaunchKernelGGL((xdg_colour_kernel<D, K, TYPE, DIR>), dim3((unsigned)(nbc + nbu)), dim3(64), 0, st, e, cr.cstart,
+                       cr.ccount, cr.ustart, cr.ucount, (int)nbc);
     BH_HIP(hipGetLastError());
 }
 
 template <int D, int K, int TYPE>
-static void xdg_eval_face(hipStream_t st, int dir, bool cut, const XEvalArgs& e) {
-    if (dir == 0) xdg_eval_type<D, K, TYPE, 0>(st, cut, e);
-    else if (dir == 1) xdg_eval_type<D, K, TYPE, 1>(st, cut, e);
-    else if constexpr (D == 3) xdg_eval_type<D, K, TYPE, 2>(st, cut, e);
+static void xdg_eval_face(hipStream_t st, int dir, const ColourRange& cr, const XEvalArgs& e) {
+    if (dir == 0) xdg_eval_type<D, K, TYPE, 0>(st, cr, e);
+    else if (dir == 1) xdg_eval_type<D, K, TYPE, 1>(st, cr, e);
+    else if constexpr (D == 3) xdg_eval_type<D, K, TYPE, 2>(st, cr, e);
 }
 
 template <int D, int K>
-static void xdg_eval_dk(hipStream_t st, int type, int dir, bool cut, const XEvalArgs& e) {
+static void xdg_eval_dk(hipStream_t st, int type, int dir, const ColourRange& cr, const XEvalArgs& e) {
     switch (type) {
-        case 0: xdg_eval_type<D, K, 0, -1>(st, cut, e); break;
-        case 1: xdg_eval_face<D, K, 1>(st, dir, cut, e); break;
-        case 2: xdg_eval_face<D, K, 2>(st, dir, cut, e); break;
-        default: xdg_eval_type<D, K, 3, -1>(st, cut, e); break;
+        case 0: xdg_eval_type<D, K, 0, -1>(st, cr, e); break;
+        case 1: xdg_eval_face<D, K, 1>(st, dir, cr, e); break;
+        case 2: xdg_eval_face<D, K, 2>(st, dir, cr, e); break;
+        default: xdg_eval_type<D, K, 3, -1>(st, cr, e); break;
     }
 }
 
-static void xdg_evaluate_launch(hipStream_t st, int D, int k, int type, int dir, bool cut, const XEvalArgs& e) {
+static void xdg_evaluate_launch(hipStream_t st, int D, int k, int type, int dir, const ColourRange& cr, const XEvalArgs& e) {
     if (D == 2) {
-        if (k == 1) xdg_eval_dk<2, 1>(st, type, dir, cut, e);
-        else if (k == 2) xdg_eval_dk<2, 2>(st, type, dir, cut, e);
-        else xdg_eval_dk<2, 3>(st, type, dir, cut, e);
+        if (k == 1) xdg_eval_dk<2, 1>(st, type, dir, cr, e);
+        else if (k == 2) xdg_eval_dk<2, 2>(st, type, dir, cr, e);
+        else xdg_eval_dk<2, 3>(st, type, dir, cr, e);
     } else {
-        if (k == 1) xdg_eval_dk<3, 1>(st, type, dir, cut, e);
-        else if (k == 2) xdg_eval_dk<3, 2>(st, type, dir, cut, e);
-        else xdg_eval_dk<3, 3>(st, type, dir, cut, e);
+        if (k == 1) xdg_eval_dk<3, 1>(st, type, dir, cr, e);
+        else if (k == 2) xdg_eval_dk<3, 2>(st, type, dir, cr, e);
+        else xdg_eval_dk<3, 3>(st, type, dir, cr, e);
     }
 }
 
@@ -1092,15 +1106,12 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
     BH_CHECK(P.k >= 1 && P.k <= 3 && (P.D == 2 || P.D == 3), "XDG evaluate: degrees 1 to 3 in 2D or 3D");
     a.perm = X.perm.as<int>();
     for (int c = 0; c < XDG_COLOURS; ++c) {
-        // the colour's cut items (own rules, many nodes), then its uncut items (tensor rules)
-        for (int part = 0; part < 2; ++part) {
-            a.start = part == 0 ? X.colour_off[c] : X.cut_end[c];
-            a.count = (part == 0 ? X.cut_end[c] : X.colour_off[c + 1]) - a.start;
-            if (a.count == 0) continue;
-            const int type = c == 0 ? 0 : c == 1 ? 3 : c < 8 ? 2 : 1;
-            const int dir = c < 2 ? -1 : c < 8 ? (c - 2) / 2 : (c - 8) / 2;   // colours of the faces: per direction
-            xdg_evaluate_launch(X.ctx->stream, P.D, P.k, type, dir, part == 0, e);
-        }
+        // the colour's cut items (own rules) and its uncut items (tensor rules)
+        const ColourRange cr{X.colour_off[c], X.cut_end[c] - X.colour_off[c], X.cut_end[c], X.colour_off[c + 1] - X.cut_end[c]};
+        const int type = c == 0 ? 0 : c == 1 ? 3 : c < 8 ? 2 : 1;
+        const int dir = c < 2 ? -1 : c < 8 ? (c - 2) / 2 : (c - 8) / 2;   // colours of the faces: per direction
+        BH_CHECK(type != 3 || cr.ucount == 0, "XDG evaluate: interface items without a rule");
+        xdg_evaluate_launch(X.ctx->stream, P.D, P.k, type, dir, cr, e);
     }
 }
 
