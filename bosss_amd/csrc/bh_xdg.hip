@@ -16,8 +16,9 @@
 // blocks with plain adds: the elements run in colour groups (XDG_COLOURS) within
 // which no two touch the same entry, one launch per group in a fixed order, so the
 // result is bitwise reproducible.  The evaluate runs over the same colour groups:
-// uncut items in closed modal form (xdg_modal_kernel), cut items from the element
-// matrices the assembly kept (xdg_emat_kernel).
+// uncut items in closed modal form (xdg_modal_item), cut items from the element
+// matrices the assembly kept (xdg_emat_item), both in one launch per group
+// (xdg_colour_kernel).
 #include <algorithm>
 #include <cmath>
 
