@@ -474,8 +474,10 @@ int bh_xdg_rhs(bh_xdg* x, int which, double* b, int flags);
 /* y <- alpha A x + beta y with the full (0) or agglomerated (1) matrix (BlockMsrMatrix.SpMV) */
 int bh_xdg_spmv(bh_xdg* x, int which, double alpha, const void* xv, double beta, void* y, int prec, int flags);
 /* XEvaluatorNonlin.Evaluate for the linear XLaplace forms (X/XSpatialOperatorMk2_XEvaluators.cs:467-646):
- * y <- beta y + alpha A u on the full (cell, species) DOF layout, matrix-free by quadrature over the
- * element list (uncut items on tensor Gauss rules, cut items on their own rules, one launch) */
+ * y <- beta y + alpha A u on the full (cell, species) DOF layout over the element list: uncut items
+ * (whole cells or faces of one species) in closed modal form, equal to their tensor Gauss quadrature
+ * up to rounding; cut items through the element matrices their own rules gave at set-up; one launch
+ * per colour group, no atomics, bitwise reproducible */
 int bh_xdg_evaluate(bh_xdg* x, double alpha, const double* u, double beta, double* y, int flags);
 /* full-space DG coefficients from agglomerated ones (host vectors) */
 int bh_xdg_extend(bh_xdg* x, const double* ured, double* ufull);
