@@ -34,15 +34,17 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def _setup(ctx, D=3, n=10, c=(0.5, 0.47, 0.52), r=0.27, q=6, nu=NU, agg=0.1):
+def _setup(ctx, D=3, n=10, c=(0.5, 0.47, 0.52), r=0.27, q=6, nu=NU, agg=0.1, k=2, graded=False):
     c = list(c)[:D]
     x = np.linspace(0, 1, n + 1)
+    if graded:   # cells of growing width: neighbours across a face differ in size
+        x = x ** 1.3
     g = B.Grid(ctx, *([x] * D))
-    X = B.XdgPoisson(g, 2, c, r, nu[0], nu[1], agg_threshold=agg, quad_order=q)
+    X = B.XdgPoisson(g, k, c, r, nu[0], nu[1], agg_threshold=agg, quad_order=q)
     G = OX.Geometry([x] * D, c, r)
     e = X.elements()
     L = X.cells()["len"]
-    asm = OX.XdgAssembly(G, 2, nu, 1.3, L, e["ints"], e["nx"], e["nw"], e["nn"])
+    asm = OX.XdgAssembly(G, k, nu, 1.3, L, e["ints"], e["nx"], e["nw"], e["nn"])
     tgt = -np.ones((2, G.J), np.int64)
     for (s_, t_, sp) in X.agglomeration():
         tgt[sp, s_] = t_
@@ -71,13 +73,16 @@ def test_full_matrix_matches_oracle(ctx, D, n):
     assert spla.norm(A - A.T) <= 1e-12 * spla.norm(A)
 
 
-@pytest.mark.parametrize("D,n", [(3, 8), (2, 16)])
-def test_matrix_free_evaluate(ctx, D, n):
-    """Matrix-free quadrature evaluate over the element list (one launch, tensor
-    and cut items) equals the oracle's assembled operator applied to u, and the
-    Evaluate(alpha, beta) semantics hold."""
-    _say(f"case xdg evaluate D={D} n={n}")
-    X, G, asm, tgt, e, L = _setup(ctx, D=D, n=n, r=0.27 if D == 3 else 0.31)
+@pytest.mark.parametrize("D,n,k,graded", [(3, 8, 2, False), (2, 16, 2, False), (3, 8, 1, False), (2, 16, 1, False),
+                                          (2, 12, 3, False), (3, 8, 2, True), (2, 14, 3, True)])
+def test_matrix_free_evaluate(ctx, D, n, k, graded):
+    """The evaluate over the element list (uncut items in closed modal form, cut items
+    from their kept element matrices, one launch per colour group) equals the oracle's
+    assembled operator (quadrature on the same rules) applied to u, for degrees 1 to 3
+    and on graded grids (neighbours of different widths), and the Evaluate(alpha, beta)
+    semantics hold."""
+    _say(f"case xdg evaluate D={D} n={n} k={k} graded={graded}")
+    X, G, asm, tgt, e, L = _setup(ctx, D=D, n=n, r=0.27 if D == 3 else 0.31, k=k, graded=graded)
     rng = np.random.default_rng(21)
     u = rng.standard_normal(X.N_full)
     y0 = rng.standard_normal(X.N_full)
