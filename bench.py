@@ -468,13 +468,14 @@ def xdg_leg(B, ctx, out, n=48, seed=3):
                                     "element matrices kept by the assembly (one lane per item row)"}
     del uf, yf
     b = X.rhs(1)
-    # Schwarz on the aggregates of the coarse hierarchy (both precisions); in FP32 the
-    # cut-cell blocks are applied with their FP64 inverse (ill-conditioned at 1:1000)
+    # one configuration for both precisions, the fastest of a sweep over pre/post sweeps
+    # (1, 2), K-cycle steps (1, 2, 4, 6, 8), coarsest size (64 to 512 cells) and Schwarz
+    # on the coarse aggregates (on, off); in FP32 the cut-cell blocks are applied with
+    # their FP64 inverse (ill-conditioned at 1:1000)
     for name, prec in (("solve_fp32_precond", B.BH_FP32), ("solve_fp64_precond", B.BH_FP64)):
         t0 = time.perf_counter()
-        P = B.XdgPrecond(X, coarse_degree=1, prec=prec)
+        P = B.XdgPrecond(X, coarse_degree=1, min_coarse=512, pre=1, post=1, kcycle_inner=4, prec=prec)
         B.pmg_set_graphs(P, USE_GRAPHS)
-        B.pmg_set_schwarz(P, True)
         if prec == B.BH_FP32:
             B.xdg_precond_set_fp64_blocks(P, True)
         ctx.sync()
@@ -489,7 +490,9 @@ def xdg_leg(B, ctx, out, n=48, seed=3):
         st = S.stats()
         yv = X.SpMV(1.0, u, 0.0, np.zeros(X.N), which=1)
         res[name] = {"precond_init_s": t_init, "solver_run_s": t_run, "iterations": st["iterations"],
-                     "converged": st["converged"], "smoother": "block-Jacobi (cells) + Schwarz (coarse aggregates)",
+                     "converged": st["converged"],
+                     "preconditioner": "two-level: cell block-Jacobi (1 pre / 1 post sweep), p = 1 and aggregation "
+                                       "levels down to 512 cells with a 4-step K-cycle, dense coarsest solve",
                      "true_rel_residual": float(np.linalg.norm(b - yv) / np.linalg.norm(b))}
         del S, P
     out["xdg"] = res
