@@ -398,6 +398,9 @@ __device__ __forceinline__ void cell_geom(const SipArgs<R>& a, const int* idx, S
 // reads them back in chunk order.  One lane per cell writing its own Np values would
 // issue Np stores per wave that each touch 64 scattered rows.
 // out = alpha * (y + affine) + (b0 ? b0 : beta * out), element by element.
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+typedef float nt_float4 __attribute__((ext_vector_type(4)));
+
 template <int Np, typename R>
 __device__ __forceinline__ void store_cells_coalesced(R* __restrict__ buf, const R* y, int lane, size_t c0, int nvalid,
                                                       const SipArgs<R>& a) {
@@ -448,7 +451,12 @@ __device__ __forceinline__ void store_cells_coalesced(R* __restrict__ buf, const
 #pragma unroll
                 for (int t = 0; t < CH; ++t) ov[t] = be * bv[t] + al * v[t];
             }
-            *reinterpret_cast<V*>(yp + e0) = o;
+            // streaming store: the results are not read again by this kernel, so they
+            // should not displace the neighbour rows other waves still read from L2
+            using NV = std::conditional_t<sizeof(R) == 8, nt_double2, nt_float4>;
+            NV nv;
+            __builtin_memcpy(&nv, &o, sizeof(NV));
+            __builtin_nontemporal_store(nv, reinterpret_cast<NV*>(yp + e0));
         } else {
 #pragma unroll
             for (int t = 0; t < CH; ++t) {
