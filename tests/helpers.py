@@ -62,7 +62,8 @@ def check_mass_blocks(M, G, asm, k, c, r, tol):
     cell volume (closed forms); symmetric positive semi-definite blocks; and the second
     moments of the ball and of its complement in the unit cube,
     sum_j U_j^T M_s,j U_j with U_j the coefficients of x_d - c_d:
-    4 pi r^5 / 15 for the ball, ((1 - c_d)^3 + c_d^3) / 3 minus that for the rest."""
+    4 pi r^5 / 15 for the ball (pi r^4 / 4 for the disk in 2D), ((1 - c_d)^3 + c_d^3) / 3
+    minus that for the rest."""
     Np = asm.Np
     mom = np.zeros((2, G.D))
     for j in range(G.J):
@@ -81,6 +82,6 @@ def check_mass_blocks(M, G, asm, k, c, r, tol):
             assert np.abs(next(iter(blocks.values())) - np.eye(Np)).max() <= 1e-13
         else:
             assert np.abs(blocks[0] + blocks[1] - np.eye(Np)).max() <= tol
-    ball = 4 * math.pi * r ** 5 / 15
+    ball = 4 * math.pi * r ** 5 / 15 if G.D == 3 else math.pi * r ** 4 / 4
     cube = np.array([((1 - c[d]) ** 3 + c[d] ** 3) / 3 for d in range(G.D)])
     return mom, ball, cube

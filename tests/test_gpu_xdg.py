@@ -343,12 +343,13 @@ def test_device_cut_rules_equal_host(ctx, D, n, q):
     assert np.array_equal(Xd.agglomeration(), Xh.agglomeration())
 
 
-def test_mass_matrix_device(ctx):
+@pytest.mark.parametrize("D,n", [(3, 10), (2, 16)])
+def test_mass_matrix_device(ctx, D, n):
     """XDG mass matrix on the device (X/MassMatrixFactory.cs:35) vs the oracle's on the
-    same cut rules, and the closed-form volumes and ball moments (q = 10)."""
-    _say("case xdg mass matrix D=3 n=10 q=10")
-    c, r = [0.5, 0.47, 0.52], 0.27
-    X, G, asm, tgt, e, L = _setup(ctx, n=10, c=c, r=r, q=10)
+    same cut rules, and the closed-form volumes and ball (disk) moments (q = 10)."""
+    _say(f"case xdg mass matrix D={D} n={n} q=10")
+    c, r = [0.5, 0.47, 0.52][:D], 0.27 if D == 3 else 0.31
+    X, G, asm, tgt, e, L = _setup(ctx, D=D, n=n, c=c, r=r, q=10)
     M = X.mass_matrix()
     Mo = asm.mass_blocks(e["ints"], e["nx"], e["nw"])
     assert M.shape == Mo.shape

@@ -233,12 +233,13 @@ def test_dof_maps():
 
 
 # ---------------------------------------------------------------- mass matrix
-def test_oracle_mass_blocks_pinned():
+@pytest.mark.parametrize("D,n", [(3, 10), (2, 16)])
+def test_oracle_mass_blocks_pinned(D, n):
     """The oracle's XDG mass blocks on the host set-up's cut rules (q = 10) against the
-    closed-form volumes and second moments of the ball."""
-    r, c = 0.27, np.array([0.5, 0.47, 0.52])
-    x, g, X = _product(3, 10, c, r, 10)
-    G = OX.Geometry([x] * 3, c, r)
+    closed-form volumes and second moments of the ball (disk in 2D)."""
+    r, c = (0.27, np.array([0.5, 0.47, 0.52])) if D == 3 else (0.31, np.array([0.5, 0.47]))
+    x, g, X = _product(D, n, c, r, 10)
+    G = OX.Geometry([x] * D, c, r)
     e = X.elements()
     asm = OX.XdgAssembly(G, 2, (1.0, 1000.0), 1.3, X.cells()["len"], e["ints"], e["nx"], e["nw"], e["nn"])
     M = asm.mass_blocks(e["ints"], e["nx"], e["nw"])
