@@ -483,7 +483,9 @@ int bh_xdg_evaluate(bh_xdg* x, double alpha, const double* u, double beta, doubl
 int bh_xdg_extend(bh_xdg* x, const double* ured, double* ufull);
 /* two-level preconditioner of the agglomerated system: block-Jacobi (FP32/FP64) on
  * the cell blocks and a species-merged degree-`coarse_degree` coarse space solved
- * by the aggregation multigrid (K-cycle with kcycle_inner inner steps) */
+ * by the aggregation multigrid (K-cycle with kcycle_inner inner steps).  Measured
+ * fastest for a sphere in 48^3 at k = 2: coarse_degree 1, min_coarse 512, omega 0.8,
+ * pre = post = 1, kcycle_inner 4, FP32 with FP64 cut-cell blocks */
 int bh_xdg_precond_create(bh_xdg* x, int coarse_degree, int min_coarse, double omega, int pre, int post,
                           int kcycle_inner, int prec, bh_solver** solver);
 /* The same two-level preconditioner for a caller's variable-block matrix of cell
