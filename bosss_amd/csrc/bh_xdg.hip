@@ -293,14 +293,22 @@ struct XEvalArgs {
     const int64_t* __restrict__ off;   // full DOF offset per cell
     const double* __restrict__ u;
     double* __restrict__ y;
-    double alpha;
+    double alpha, beta;
 };
+
+// y_i += alpha v; the volume group, which holds exactly one item per (cell, species)
+// row, applies beta first: y_i = beta y_i + alpha v (beta = 0: y is not read)
+template <bool FIRST>
+__device__ __forceinline__ void xdg_put(const XEvalArgs& e, int64_t i, double v) {
+    if constexpr (FIRST) e.y[i] = (e.beta == 0.0 ? 0.0 : e.beta * e.y[i]) + e.alpha * v;
+    else e.y[i] += e.alpha * v;
+}
 
 // Cut items of the evaluate: y += alpha M u with the element matrix M that the assembly
 // kept (the same values it adds into the global blocks), one lane per (item, test row):
 // a row of the test side's two blocks (trial P, trial Q) against both sides' coefficients.
 // Colour groups as in the assembly: no two items of a launch touch the same rows.
-template <bool TWO>
+template <bool TWO, bool FIRST>
 __device__ __forceinline__ void xdg_emat_item(const XEvalArgs& e, int start, int count, int64_t t) {
     const XArgs& a = e.g;
     const int Np = a.Np, R = TWO ? 2 * Np : Np;
@@ -317,7 +325,7 @@ __device__ __forceinline__ void xdg_emat_item(const XEvalArgs& e, int start, int
     for (int n = 0; n < Np; ++n) s += M[n] * e.u[oP + n];
     if (TWO)
         for (int n = 0; n < Np; ++n) s += M[Np * Np + n] * e.u[oQ + n];
-    e.y[(side ? oQ : oP) + m] += e.alpha * s;
+    xdg_put<FIRST>(e, (side ? oQ : oP) + m, s);
 }
 
 // XDG mass matrix (X/MassMatrixFactory.cs:35; the block of cell j and species s is
@@ -380,11 +388,6 @@ void xdg_mass(XdgDevice& X, double* out_dev) {
     else hipLaunchKernelGGL(xdg_mass_k<3>, dim3(blocks), dim3(64), 0, X.ctx->stream, e, dv.as<int>(), n, out_dev);
     BH_HIP(hipGetLastError());
     X.ctx->sync();
-}
-
-__global__ void xdg_scale_k(double* __restrict__ y, double beta, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = beta == 0.0 ? 0.0 : beta * y[i];
 }
 
 // ------------------------------------------------------------------ variable-block SpMV
@@ -1021,7 +1024,7 @@ __device__ __forceinline__ void xdg_modal_item(const XEvalArgs& e, int start, in
         }
     }
 #pragma unroll
-    for (int m = 0; m < Np; ++m) e.y[oP + m] += e.alpha * yP[m];
+    for (int m = 0; m < Np; ++m) xdg_put<TYPE == 0>(e, oP + m, yP[m]);
 }
 
 // One colour group per launch: its cut items (first nbc workgroups, a lane per item row,
@@ -1030,7 +1033,7 @@ __device__ __forceinline__ void xdg_modal_item(const XEvalArgs& e, int start, in
 template <int D, int K, int TYPE, int DIR>
 __global__ __launch_bounds__(64) void xdg_colour_kernel(XEvalArgs e, int cstart, int ccount, int ustart, int ucount, int nbc) {
     if ((int)blockIdx.x < nbc) {
-        xdg_emat_item<TYPE == 1 || TYPE == 3>(e, cstart, ccount, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+        xdg_emat_item<TYPE == 1 || TYPE == 3, TYPE == 0>(e, cstart, ccount, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     } else if constexpr (TYPE != 3) {
         xdg_modal_item<D, K, TYPE, DIR>(e, ustart, ucount, ((int)blockIdx.x - nbc) * (int)blockDim.x + (int)threadIdx.x);
     }
@@ -1085,7 +1088,6 @@ static void xdg_evaluate_launch(hipStream_t st, int D, int k, int type, int dir,
 void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, double* y) {
     XdgProblem& P = *X.P;
     const int64_t N = P.N_full;
-    hipLaunchKernelGGL(xdg_scale_k, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, X.ctx->stream, y, beta, N);
     XEvalArgs e{};
     XArgs& a = e.g;
     a.D = P.D; a.k = P.k; a.Np = P.Np; a.nq1 = P.k + 1;
@@ -1104,6 +1106,8 @@ void xdg_evaluate(XdgDevice& X, double alpha, const double* u, double beta, doub
     e.u = u;
     e.y = y;
     e.alpha = alpha;
+    e.beta = beta;
+    BH_CHECK(X.colour_off[1] > X.colour_off[0] || N == 0, "XDG evaluate: no volume items");
     BH_CHECK(P.k >= 1 && P.k <= 3 && (P.D == 2 || P.D == 3), "XDG evaluate: degrees 1 to 3 in 2D or 3D");
     a.perm = X.perm.as<int>();
     for (int c = 0; c < XDG_COLOURS; ++c) {
